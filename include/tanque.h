@@ -1,0 +1,169 @@
+/* tanque.h -- C ABI of the MI355X FLIP/APIC substep library (libtanque.so).
+ *
+ * This is the drop-in boundary for the reference's CPU engine: the step/state interface that
+ * js/motor.js (MotorCPU, motor.js:79-120) drives on SimulacaoCPU (js/fisica-cpu.js:28-671).
+ * Every entry point returns 0 on success and a non-zero TQ_E* code on failure; the message of
+ * the last failure on the calling thread is available from tq_last_error().  No C++
+ * exception crosses this boundary.  Handles are opaque; every host buffer is caller-owned
+ * and is copied in or out (the library owns all device memory).  One handle must not be used
+ * from two threads at once.  Units follow the reference: metres, seconds, SDF in cells.
+ */
+#ifndef TANQUE_H
+#define TANQUE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TQ_OK 0
+#define TQ_E_ARG 1      /* bad argument (null pointer, size out of range, unknown id) */
+#define TQ_E_HIP 2      /* a HIP runtime call failed */
+#define TQ_E_CAPACITY 3 /* particle capacity or a fixed table size would be exceeded */
+#define TQ_E_STATE 4    /* call not valid in the current state */
+
+#define TQ_CAMPOS 12     /* floats per particle: x y vx vy cux cuy cvx cvy espuma absR absG absB (fisica-cpu.js:26) */
+#define TQ_MAX_BOLHAS 4096 /* bubble labels (constantes.js:20) */
+
+/* Precision of the multigrid preconditioner inside the FP32 PCG. */
+#define TQ_PREC_FP32 0   /* MG vectors stored and computed in FP32 */
+#define TQ_PREC_MIXED 1  /* MG vectors stored in FP16 (per-cycle scaled), computed in FP32 */
+
+typedef struct tq_sim tq_sim;
+
+/* Construction options: SimulacaoCPU(cena, {gravidade, iteracoes, tauDensidade, tensao, ar})
+ * (fisica-cpu.js:29-77) plus the grid of the Cena (cena.js:15-24) and device sizing. */
+typedef struct tq_desc {
+  int nx, ny;             /* grid cells */
+  double h;               /* cell size (m) */
+  double gx, gy;          /* gravidade, default (0, -9.81) */
+  int iteracoes;          /* PCG iterations (fixed count) or the iteration cap when tol > 0 */
+  double tau_densidade;   /* TAU_DENSIDADE (s) */
+  double tensao;          /* surface tension (N/m); 0 disables */
+  int ar;                 /* 1: track enclosed air pockets as pressure unknowns */
+  int max_particles;      /* device capacity; 0 = ceil(4.6 * nx * ny) as MotorCPU.maxParticulas */
+  int precision;          /* TQ_PREC_FP32 or TQ_PREC_MIXED */
+  double tol;             /* 0: fixed `iteracoes` (reference); >0: stop at ||r|| <= tol*||b|| */
+} tq_desc;
+
+/* Fills `d` with the reference defaults (constantes.js) for an nx x ny grid of cell size h. */
+void tq_desc_default(tq_desc* d, int nx, int ny, double h);
+
+/* new SimulacaoCPU(cena, opcoes) (fisica-cpu.js:29-77).  The SDF starts as the empty tank
+ * (the frame only, cena.js:139-144); upload walls with tq_set_sdf. */
+int tq_create(const tq_desc* desc, tq_sim** out);
+int tq_destroy(tq_sim* s);
+
+/* Upload rows [row0, row1] of the (2nx) x (2ny) SDF (cena.js:20-24, rows as in
+ * Cena.pegarLinhasAlteradas, cena.js:95-99) and recompute the solid cells
+ * (atualizarParedes, fisica-cpu.js:80-85).  `sdf` points at the FULL host array. */
+int tq_set_sdf(tq_sim* s, const float* sdf, int row0, int row1);
+
+/* adicionar(lista, checarCheia) (fisica-cpu.js:92-111): `count` particles of TQ_CAMPOS floats.
+ * Particles inside walls (distance <= 0.35 cells) and, when `check`, in cells already holding
+ * >= 3 particles are skipped; the rest are appended in order.  `accepted` may be NULL. */
+int tq_add_particles(tq_sim* s, const float* list, int count, int check, int* accepted);
+
+/* aplicarFerramenta (fisica-cpu.js:129-152). kind: 0 none, 1 esponja, 2 mao, 3 corante */
+typedef struct tq_tool {
+  int kind;
+  double x, y, raio, vx, vy;
+  double absorcao[3];
+} tq_tool;
+int tq_apply_tool(tq_sim* s, const tq_tool* t);
+
+/* avancar(dt, subpassos) (fisica-cpu.js:155-159): air labelling once, then `substeps`
+ * substeps of dt/substeps.  Blocks until the device is done. */
+int tq_step(tq_sim* s, double dt, int substeps);
+/* subpasso(dt) (fisica-cpu.js:176-189) */
+int tq_substep(tq_sim* s, double dt);
+/* atualizarAr (fisica-cpu.js:162-167) */
+int tq_update_air(tq_sim* s);
+/* reiniciarAr (fisica-cpu.js:170-174) */
+int tq_reset_air(tq_sim* s);
+/* MotorCPU.limparAgua (motor.js:95): n = 0, cell counts cleared, reiniciarAr */
+int tq_clear_water(tq_sim* s);
+/* sim.n = m for m <= n (keeps the first m particles in reference order) */
+int tq_truncate(tq_sim* s, int m);
+
+int tq_count(tq_sim* s, int* n);
+/* Copies min(n, max) particles in the reference's order (insertion order, removals keep
+ * order; fisica-cpu.js:106, 114-126) into dst (TQ_CAMPOS floats each). */
+int tq_read_particles(tq_sim* s, float* dst, int max, int* n);
+/* vMax of the last G2P (fisica-cpu.js:637); *has = 0 before the first substep */
+int tq_vmax(tq_sim* s, double* v, int* has);
+/* area() (fisica-cpu.js:192-194) */
+int tq_area(tq_sim* s, double* a);
+
+/* Runtime options (gravity, tension, tau, air, PCG iterations, precision, tolerance). */
+int tq_set_options(tq_sim* s, const tq_desc* d);
+
+/* Field access (state the reference exposes as typed arrays; ids as in the oracle):
+ *  0 u, 1 v, 2 uPeso, 3 vPeso (f32), 4 uValida, 5 vValida (u8), 6 densidade (f32),
+ *  7 contagemCelula (u32), 8 corCelula (f32, 3 per cell), 9 tipo (u8), 10 solidoCelula (u8),
+ *  11 div, 12 q, 13 fase, 14 normalX, 15 normalY, 16 curvatura, 17 qAr (f32),
+ *  18 rotulo (u32), 19 pressaoBolha, 20 volumeBolha, 21 rhsBolha, 22 yBolha,
+ *  23 volumeAlvo (f32 x 4096), 24 vivo (u8 x 4096).
+ * `bytes` must equal the field size. Writing is a test hook for phase-by-phase parity. */
+int tq_read_field(tq_sim* s, int field, void* dst, size_t bytes);
+int tq_write_field(tq_sim* s, int field, const void* src, size_t bytes);
+/* Replaces all particles (reference order = array order); test hook. */
+int tq_write_particles(tq_sim* s, const float* src, int n);
+/* Set / read the tracker's first-snapshot flag (RastreadorDeAr.primeira, ar.js:33). */
+int tq_set_tracker_first(tq_sim* s, int primeira);
+
+/* Runs one phase of the substep on the current state (test hook).  Phase ids:
+ *  0 P2G, 1 gravity+boundary, 2 types, 3 label propagation, 4 surface, 5 air pressure,
+ *  6 divergence, 7 bubble rhs, 8 pressure solve, 9 gradient, 10 extrapolation, 11 G2P,
+ *  12 air tracker update.  On this device, phases 0-2 always run fused (ids 1 and 2 are
+ *  accepted and do nothing after a phase 0 call). */
+int tq_run_phase(tq_sim* s, int phase, double dt);
+
+/* Statistics of the last pressure solve. */
+typedef struct tq_solver_stats {
+  int iterations;          /* PCG iterations run */
+  double norm0;            /* ||[r;rb]|| at the warm start */
+  double norm;             /* recursive residual norm after the last iteration */
+  double norm_b;           /* ||[-div; rhs]|| */
+  double true_resid;       /* ||b - A[q;y]|| recomputed (only when requested) */
+  int levels;              /* multigrid levels */
+  int active_bubbles;
+} tq_solver_stats;
+int tq_solver_stats_get(tq_sim* s, tq_solver_stats* st, int compute_true_residual);
+
+/* Per-kernel timing with HIP events on the library's stream.  enable=1 starts collecting;
+ * tq_timing_read copies (name, total ms, launches) records. */
+typedef struct tq_kernel_time {
+  char name[48];
+  double total_ms;
+  long launches;
+  double bytes_per_launch; /* algorithmic bytes per launch (0 if not modelled) */
+} tq_kernel_time;
+/* `only`: NULL times every kernel; otherwise only launches whose name equals it. */
+int tq_timing_enable(tq_sim* s, int enable, const char* only);
+int tq_timing_read(tq_sim* s, tq_kernel_time* out, int max, int* n);
+
+/* Waits for all queued device work of this handle. */
+int tq_sync(tq_sim* s);
+
+const char* tq_last_error(void);
+
+/* ---------------------------------------------------------------- host helpers
+ * Scene construction (js/cena.js) and seeding (fisica-cpu.js:705-727, testes/apoio.js:4-9),
+ * on caller-owned host arrays of (2nx)*(2ny) floats. */
+void tq_scene_frame(float* sdf, int nx, int ny);                      /* Cena #zerar (cena.js:139-144) */
+void tq_scene_stroke(float* sdf, int nx, int ny, double h, double x0, double y0, double x1, double y1,
+                     double raio, int apagar);                        /* Cena.tracar (cena.js:43-72) */
+/* semearRetangulo with the seeded LCG; returns the particle count; writes when out != NULL
+ * and max_out suffices (returns -1 otherwise). `seed` is the LCG state, updated in place. */
+int tq_seed_rect(double x0, double y0, double x1, double y1, double h, const double* absorcao,
+                 uint32_t* seed, float* out, int max_out);
+/* passoMaximo (constantes.js:167-175); has_vmax = 0 means vMax == null */
+double tq_passo_maximo(double h, double altura, double vmax, int has_vmax, double dt_quadro);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,94 @@
+"""Builds libtanque.so (HIP kernels for gfx950 + the C ABI) and the Node-API addon in-tree.
+
+Usage: python tanque-de-agua_amd/build.py [--force]
+"""
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB_DIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIB_DIR, "libtanque.so")
+NODE_ADDON = os.path.join(LIB_DIR, "tanque_node.node")
+
+HIP_SOURCES = ["k_particles.hip", "k_grid.hip", "k_air.hip", "k_solver.hip", "tq_api.hip"]
+CPP_SOURCES = ["scene.cpp"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# Packed-FP32 instructions are disabled for all device code (known-bad on these boxes).
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+             "-packed-fp32-ops", "-Wno-unused-result"]
+
+
+def _newer(target, deps):
+    if not os.path.exists(target):
+        return True
+    t = os.path.getmtime(target)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _headers():
+    return [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))] + \
+        [os.path.join(ROOT, "include", "tanque.h")]
+
+
+def _compile(src, force):
+    out = os.path.join(OBJ, os.path.basename(src) + ".o")
+    if not force and not _newer(out, [src] + _headers()):
+        return out
+    if src.endswith(".hip"):
+        cmd = [HIPCC] + HIP_FLAGS + ["-c", src, "-o", out]
+    else:
+        cmd = ["g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-c", src, "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    return out
+
+
+def build_lib(force=False):
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(LIB_DIR, exist_ok=True)
+    srcs = [os.path.join(CSRC, f) for f in HIP_SOURCES + CPP_SOURCES]
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = list(ex.map(lambda s: _compile(s, force), srcs))
+    if force or _newer(LIB, objs):
+        tmp = LIB + ".tmp"
+        cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
+        os.replace(tmp, LIB)
+    return LIB
+
+
+def build_node_addon(force=False):
+    """The thin Node-API addon over the C ABI (needs node's headers; skipped if absent)."""
+    src = os.path.join(CSRC, "napi_addon.cpp")
+    hdr = "/usr/include/node/node_api.h"
+    if not os.path.exists(src) or not os.path.exists(hdr):
+        return None
+    if not force and not _newer(NODE_ADDON, [src, LIB, os.path.join(ROOT, "include", "tanque.h")]):
+        return NODE_ADDON
+    tmp = NODE_ADDON + ".tmp"
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-I/usr/include/node", src, "-o", tmp,
+           "-L" + LIB_DIR, "-ltanque", "-Wl,-rpath,$ORIGIN", "-Wl,--unresolved-symbols=ignore-all"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("addon build failed:\n%s\n%s" % (r.stdout, r.stderr))
+    os.replace(tmp, NODE_ADDON)
+    return NODE_ADDON
+
+
+def build(force=False):
+    lib = build_lib(force)
+    addon = build_node_addon(force)
+    return lib, addon
+
+
+if __name__ == "__main__":
+    print(build(force="--force" in sys.argv))

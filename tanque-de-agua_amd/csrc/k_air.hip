@@ -1,0 +1,323 @@
+// k_air.hip -- the air tracker (RastreadorDeAr.atualizar, js/ar.js:56-153) on the GPU.
+//
+//   1. union-find over air cells (4-neighbourhood) with lock-free CAS linking of the larger
+//      root under the smaller one, so every component's root is its smallest cell index --
+//      the root the reference's sequential union-find ends with (ar.js:155-158);
+//   2. components are numbered by their root's raster position (an exclusive scan of root
+//      flags): the reference numbers them by first raster appearance, which is the root;
+//   3. per-component counts (cells, atmosphere cells, top-row contact) and the
+//      (component, old label) overlap table (count, first cell) with wavefront-aggregated
+//      atomics into an open-addressing hash table;
+//   4. one workgroup does the label bookkeeping in the reference's order: overlap totals,
+//      per-component volumes summed over old labels in first-appearance order, the stable
+//      volume-descending ordering of candidate bubbles, stable label reuse, free-label
+//      allocation and the born bubbles' pressures;
+//   5. every cell takes its component's label.
+#include "tq_device.hpp"
+
+namespace tq {
+
+constexpr unsigned long long HEMPTY = ~0ull;
+
+__device__ inline int ld_rel(const int32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+__device__ inline int uf_find(int32_t* pai, int x) {
+  while (true) {
+    const int p = ld_rel(&pai[x]);
+    if (p == x) return x;
+    const int gp = ld_rel(&pai[p]);
+    if (gp != p) __hip_atomic_store(&pai[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = p;
+  }
+}
+
+__device__ inline void uf_unite(int32_t* pai, int a, int b) {
+  while (true) {
+    a = uf_find(pai, a);
+    b = uf_find(pai, b);
+    if (a == b) return;
+    if (a < b) { int t = a; a = b; b = t; }
+    const int old = atomicCAS(&pai[a], a, b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_uf_init(const uint8_t* __restrict__ tipo, int C, int32_t* __restrict__ pai,
+                                                 uint32_t* __restrict__ cells, uint32_t* __restrict__ atm,
+                                                 uint32_t* __restrict__ topo, int32_t* __restrict__ dono,
+                                                 uint32_t* __restrict__ label, float* __restrict__ alvo) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  pai[c] = tipo[c] == AR ? c : -1;
+  cells[c] = 0; atm[c] = 0; topo[c] = 0; dono[c] = -1; label[c] = 0; alvo[c] = 0.f;
+}
+
+__global__ __launch_bounds__(256) void k_uf_union(const uint8_t* __restrict__ tipo, int nx, int ny, int32_t* __restrict__ pai) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny || tipo[c] != AR) return;
+  const int i = c % nx, j = c / nx;
+  if (i > 0 && tipo[c - 1] == AR) uf_unite(pai, c, c - 1);
+  if (j > 0 && tipo[c - nx] == AR) uf_unite(pai, c, c - nx);
+}
+
+__global__ __launch_bounds__(256) void k_uf_flatten(const uint8_t* __restrict__ tipo, int C, int32_t* __restrict__ pai,
+                                                    uint32_t* __restrict__ rootFlag) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  uint32_t f = 0;
+  if (tipo[c] == AR) {
+    const int r = uf_find(pai, c);
+    f = (r == c) ? 1u : 0u;
+  }
+  rootFlag[c] = f;
+}
+
+__device__ inline uint64_t hmix(uint64_t k) {
+  k ^= k >> 33; k *= 0xff51afd7ed558ccdull; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ull; k ^= k >> 33;
+  return k;
+}
+
+// per-cell: component index, per-component statistics, overlap table
+__global__ __launch_bounds__(256) void k_comp_stats(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
+                                                    int nx, int ny, const int32_t* __restrict__ pai,
+                                                    const uint32_t* __restrict__ compIdx, int32_t* __restrict__ comp,
+                                                    uint32_t* __restrict__ cells, uint32_t* __restrict__ atm,
+                                                    uint32_t* __restrict__ topo, unsigned long long* __restrict__ hkeys,
+                                                    uint32_t* __restrict__ hcount, uint32_t* __restrict__ hfirst,
+                                                    uint32_t* __restrict__ hlist, int hmask,
+                                                    DevScalars* __restrict__ sc) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int k = -1;
+  uint32_t old = 0;
+  bool istop = false;
+  if (c < nx * ny) {
+    if (tipo[c] == AR) {
+      int r = pai[c];
+      while (pai[r] != r) r = pai[r];
+      k = (int)compIdx[r];
+      old = rot[c];
+      istop = (c / nx) >= ny - MOLDURA - 1;
+    }
+    comp[c] = k;
+  }
+  // statistics grouped by component within the wavefront
+  bool pending = k >= 0;
+  while (true) {
+    unsigned long long b = __ballot(pending);
+    if (!b) break;
+    const int leader = __ffsll((long long)b) - 1;
+    const int lk = __shfl(k, leader, 64);
+    const bool m = pending && k == lk;
+    const unsigned long long bm = __ballot(m), ba = __ballot(m && old == 1), bt = __ballot(m && istop);
+    if (lane == leader) {
+      atomicAdd(&cells[lk], (uint32_t)__popcll(bm));
+      if (ba) atomicAdd(&atm[lk], (uint32_t)__popcll(ba));
+      if (bt) atomicOr(&topo[lk], 1u);
+    }
+    if (m) pending = false;
+  }
+  // overlap table entries for old bubble labels
+  pending = k >= 0 && old >= 2;
+  while (true) {
+    unsigned long long b = __ballot(pending);
+    if (!b) break;
+    const int leader = __ffsll((long long)b) - 1;
+    const int lk = __shfl(k, leader, 64);
+    const uint32_t lo = __shfl(old, leader, 64);
+    const bool m = pending && k == lk && old == lo;
+    const unsigned long long bm = __ballot(m);
+    if (lane == leader) {
+      const unsigned long long key = ((unsigned long long)(uint32_t)lk << 12) | lo;
+      uint32_t h = (uint32_t)hmix(key) & (uint32_t)hmask;
+      while (true) {
+        const unsigned long long prev = atomicCAS(&hkeys[h], HEMPTY, key);
+        if (prev == HEMPTY) {
+          const uint32_t idx = atomicAdd(&sc->npairs, 1u);
+          hlist[idx] = h;
+          break;
+        }
+        if (prev == key) break;
+        h = (h + 1) & (uint32_t)hmask;
+      }
+      atomicAdd(&hcount[h], (uint32_t)__popcll(bm));
+      atomicMin(&hfirst[h], (uint32_t)c);  // the leader is the group's first cell in raster order
+    }
+    if (m) pending = false;
+  }
+}
+
+// single-workgroup bitonic sort of n (key, value) pairs in global memory (padded with max keys)
+__device__ void block_bitonic(unsigned long long* keys, uint32_t* vals, int n) {
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int i = n + threadIdx.x; i < P; i += blockDim.x) { keys[i] = ~0ull; vals[i] = 0xFFFFFFFFu; }
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const bool up = (i & k) == 0;
+          const unsigned long long a = keys[i], b = keys[ixj];
+          if ((a > b) == up) {
+            keys[i] = b; keys[ixj] = a;
+            const uint32_t t = vals[i]; vals[i] = vals[ixj]; vals[ixj] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_bookkeep(const uint32_t* __restrict__ compIdxTotal, const uint32_t* __restrict__ cells,
+                                                   const uint32_t* __restrict__ atm, const uint32_t* __restrict__ topo,
+                                                   float* __restrict__ alvo, int32_t* __restrict__ dono,
+                                                   uint32_t* __restrict__ label, unsigned long long* __restrict__ hkeys,
+                                                   uint32_t* __restrict__ hcount, uint32_t* __restrict__ hfirst,
+                                                   const uint32_t* __restrict__ hlist, unsigned long long* __restrict__ skeys,
+                                                   uint32_t* __restrict__ svals, float* __restrict__ volAlvo,
+                                                   uint8_t* __restrict__ vivo, float* __restrict__ pBolha,
+                                                   DevScalars* __restrict__ sc) {
+  __shared__ uint32_t total[MAX_BOLHAS];
+  __shared__ uint8_t vivoNovo[MAX_BOLHAS];
+  __shared__ uint32_t ncand_s;
+  const int tid = threadIdx.x;
+  const uint32_t ncomp = *compIdxTotal;
+  const uint32_t np = sc->npairs;
+  const bool primeira = sc->primeira != 0;
+  for (int r = tid; r < MAX_BOLHAS; r += 1024) { total[r] = 0; vivoNovo[r] = 0; }
+  if (tid == 0) { sc->ncomp = ncomp; ncand_s = 0; }
+  __syncthreads();
+  // ---- overlap totals of the components that stay (ar.js:101-106)
+  for (uint32_t i = tid; i < np; i += 1024) {
+    const uint32_t h = hlist[i];
+    const unsigned long long key = hkeys[h];
+    const uint32_t k = (uint32_t)(key >> 12), r = (uint32_t)(key & 4095);
+    if (topo[k] || cells[k] >= MIN_BOLHA_CELULAS) atomicAdd(&total[r], hcount[h]);
+    skeys[i] = ((unsigned long long)k << 32) | hfirst[h];
+    svals[i] = h;
+  }
+  __syncthreads();
+  // ---- per-component volume over old labels in first-appearance order (ar.js:108-125)
+  if (np > 0 && !primeira) {
+    block_bitonic(skeys, svals, (int)np);
+    for (uint32_t i = tid; i < np; i += 1024) {
+      const uint32_t k = (uint32_t)(skeys[i] >> 32);
+      if (i > 0 && (uint32_t)(skeys[i - 1] >> 32) == k) continue;  // one thread per component segment
+      if (topo[k] || cells[k] < MIN_BOLHA_CELULAS) continue;
+      double v = atm[k];
+      uint32_t melhor = 0;
+      int dn = -1;
+      for (uint32_t e = i; e < np && (uint32_t)(skeys[e] >> 32) == k; e++) {
+        const uint32_t h = svals[e];
+        const uint32_t r = (uint32_t)(hkeys[h] & 4095);
+        const uint32_t qv = hcount[h];
+        v += (double)volAlvo[r] * ((double)qv / (double)total[r]);
+        if (qv > melhor) { melhor = qv; dn = (int)r; }
+      }
+      alvo[k] = (float)v;
+      dono[k] = dn;
+      label[k] = 0xFFFFFFFFu;  // marks "volume computed from overlaps"
+    }
+    __syncthreads();
+  }
+  // ---- volumes of the other components, candidates (ar.js:110-130)
+  for (uint32_t k = tid; k < ncomp; k += 1024) {
+    float a = 0.f;
+    if (topo[k]) { label[k] = 1u; alvo[k] = 0.f; continue; }
+    if (cells[k] < MIN_BOLHA_CELULAS) { label[k] = 0; alvo[k] = 0.f; continue; }
+    if (primeira) a = (float)cells[k];
+    else if (label[k] == 0xFFFFFFFFu) a = alvo[k];
+    else a = (float)atm[k];
+    alvo[k] = a;
+    label[k] = 0;
+    if ((double)a >= MIN_BOLHA_CELULAS) {
+      const uint32_t slot = atomicAdd(&ncand_s, 1u);
+      // descending volume, ascending component index (a stable sort of the raster order)
+      skeys[slot] = ((unsigned long long)(~__float_as_uint(a)) << 32) | k;
+      svals[slot] = k;
+    }
+  }
+  __syncthreads();
+  const int ncand = (int)ncand_s;
+  if (ncand > 0) block_bitonic(skeys, svals, ncand);
+  __syncthreads();
+  // ---- stable labels, free labels, born bubbles (ar.js:131-145); sequential like the reference
+  if (tid == 0) {
+    for (int e = 0; e < ncand; e++) {
+      const uint32_t k = svals[e];
+      const int r = dono[k];
+      if (r >= 2 && !vivoNovo[r]) { label[k] = (uint32_t)r; vivoNovo[r] = 1; }
+    }
+    int livre = 2;
+    for (int e = 0; e < ncand; e++) {
+      const uint32_t k = svals[e];
+      if (label[k]) continue;
+      while (livre < MAX_BOLHAS && (vivoNovo[livre] || vivo[livre])) livre++;
+      if (livre >= MAX_BOLHAS) break;
+      label[k] = (uint32_t)livre;
+      vivoNovo[livre] = 1;
+      const int dn = dono[k];
+      pBolha[livre] = dn >= 2 ? pBolha[dn] : 0.f;
+    }
+  }
+  __syncthreads();
+  // ---- volumeAlvo, vivo (ar.js:147-150)
+  for (int r = tid; r < MAX_BOLHAS; r += 1024) { volAlvo[r] = 0.f; vivo[r] = vivoNovo[r]; }
+  __syncthreads();
+  for (int e = tid; e < ncand; e += 1024) {
+    const uint32_t k = svals[e];
+    if (label[k] >= 2) volAlvo[label[k]] = alvo[k];
+  }
+  // ---- clear the overlap table for the next update
+  for (uint32_t i = tid; i < np; i += 1024) {
+    const uint32_t h = hlist[i];
+    hkeys[h] = HEMPTY; hcount[h] = 0; hfirst[h] = 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  if (tid == 0) { sc->npairs = 0; sc->primeira = 0; }
+}
+
+__global__ __launch_bounds__(256) void k_label_write(const int32_t* __restrict__ comp, const uint32_t* __restrict__ label,
+                                                     int C, uint32_t* __restrict__ rot) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  const int k = comp[c];
+  rot[c] = k >= 0 ? label[k] : 0u;
+}
+
+int tracker_update(tq_sim* s) {
+  const int C = s->C;
+  const dim3 g((C + 255) / 256), b(256);
+  int t = time_begin(s, "air_tracker", 16.0 * C);
+  hipLaunchKernelGGL(k_uf_init, g, b, 0, s->st, s->tipo, C, s->pai, s->compCells, s->compAtm, s->compTopo, s->compDono,
+                     s->compLabel, s->compAlvo);
+  hipLaunchKernelGGL(k_uf_union, g, b, 0, s->st, s->tipo, s->nx, s->ny, s->pai);
+  hipLaunchKernelGGL(k_uf_flatten, g, b, 0, s->st, s->tipo, C, s->pai, s->rootFlag);
+  TQ_LAUNCH_CHECK();
+  if (int e = scan_u32(s, s->rootFlag, s->compIdx, C)) return e;
+  hipLaunchKernelGGL(k_comp_stats, g, b, 0, s->st, s->tipo, s->rot, s->nx, s->ny, s->pai, s->compIdx, s->comp,
+                     s->compCells, s->compAtm, s->compTopo, s->hkeys, s->hcount, s->hfirst, s->hlist, s->hsize - 1, s->sc);
+  hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(1024), 0, s->st, s->compIdx + C, s->compCells, s->compAtm, s->compTopo,
+                     s->compAlvo, s->compDono, s->compLabel, s->hkeys, s->hcount, s->hfirst, s->hlist,
+                     (unsigned long long*)s->ordBuf, s->ordBuf + 2 * (size_t)s->hsize, s->volAlvo, s->vivo, s->pBolha,
+                     s->sc);
+  hipLaunchKernelGGL(k_label_write, g, b, 0, s->st, s->comp, s->compLabel, C, s->rot);
+  time_end(s, t);
+  TQ_LAUNCH_CHECK();
+  return 0;
+}
+
+int tracker_reset(tq_sim* s) {
+  TQ_HIP(hipMemsetAsync(s->rot, 0, sizeof(uint32_t) * s->C, s->st));
+  TQ_HIP(hipMemsetAsync(s->volAlvo, 0, sizeof(float) * MAX_BOLHAS, s->st));
+  TQ_HIP(hipMemsetAsync(s->vivo, 0, MAX_BOLHAS, s->st));
+  TQ_HIP(hipMemsetAsync(s->pBolha, 0, sizeof(float) * MAX_BOLHAS, s->st));
+  const int one = 1;
+  TQ_HIP(hipMemcpyAsync(&s->sc->primeira, &one, sizeof(int), hipMemcpyHostToDevice, s->st));
+  return 0;
+}
+
+}  // namespace tq

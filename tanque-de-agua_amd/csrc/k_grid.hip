@@ -1,0 +1,460 @@
+// k_grid.hip -- cell/face kernels of the substep: surface curvature (one LDS-tiled pass),
+// air-cell pressure + divergence, label propagation, per-bubble sums, the pressure gradient
+// and the velocity extrapolation.
+#include "tq_device.hpp"
+
+namespace tq {
+
+// ================================================================== #superficie (fisica-cpu.js:306-374)
+// One workgroup computes a 32x32 tile from a 58x58 LDS region (halo 13 = the dependency radius
+// of ext3, blur, ext3, blur, ext3, normal, curvature), so the 15 full-grid passes of the
+// reference become one launch.  Cells of the halo are computed too (garbage near the region
+// edge) and discarded.
+constexpr int SF_T = 32, SF_H = 13, SF_R = SF_T + 2 * SF_H;  // 58
+constexpr int SF_N = SF_R * SF_R;                            // 3364
+
+__global__ __launch_bounds__(256) void k_surface(const uint8_t* __restrict__ tipo, const float* __restrict__ dens, int nx,
+                                                 int ny, double h, float* __restrict__ fase, float* __restrict__ nXo,
+                                                 float* __restrict__ nYo, float* __restrict__ kap) {
+#pragma clang fp contract(off)
+  __shared__ float f[SF_N], t[SF_N], ga[SF_N], gb[SF_N];
+  __shared__ uint8_t ty[SF_N], fl[SF_N];
+  const int tid = threadIdx.x;
+  const int I0 = blockIdx.x * SF_T - SF_H, J0 = blockIdx.y * SF_T - SF_H;
+  for (int e = tid; e < SF_N; e += 256) {
+    int gi = I0 + e % SF_R, gj = J0 + e / SF_R;
+    if (gi >= 0 && gj >= 0 && gi < nx && gj < ny) {
+      size_t c = (size_t)gi + (size_t)gj * nx;
+      uint8_t tt = tipo[c];
+      ty[e] = tt;
+      f[e] = tt == SOLIDO ? 0.f : (float)fmin(1.0, (double)dens[c] / PPC);
+    } else {
+      ty[e] = 255;
+      f[e] = 0.f;
+    }
+  }
+  __syncthreads();
+  for (int passada = 0; passada <= 2; passada++) {
+    // #estenderNaParede: 3 layers of "unfilled cell := mean of filled 4-neighbours"
+    for (int e = tid; e < SF_N; e += 256) fl[e] = (ty[e] != SOLIDO && ty[e] != 255) ? 1 : 0;
+    __syncthreads();
+    for (int camada = 0; camada < 3; camada++) {
+      for (int e = tid; e < SF_N; e += 256) {
+        if (fl[e] || ty[e] == 255) continue;
+        int lx = e % SF_R, ly = e / SF_R, gi = I0 + lx, gj = J0 + ly;
+        double s = 0;
+        int k = 0;
+        if (gi > 0 && lx > 0 && fl[e - 1] == 1) { s += (double)f[e - 1]; k++; }
+        if (gi < nx - 1 && lx < SF_R - 1 && fl[e + 1] == 1) { s += (double)f[e + 1]; k++; }
+        if (gj > 0 && ly > 0 && fl[e - SF_R] == 1) { s += (double)f[e - SF_R]; k++; }
+        if (gj < ny - 1 && ly < SF_R - 1 && fl[e + SF_R] == 1) { s += (double)f[e + SF_R]; k++; }
+        if (k) { t[e] = (float)(s / k); fl[e] = 2; }
+      }
+      __syncthreads();
+      for (int e = tid; e < SF_N; e += 256)
+        if (fl[e] == 2) { f[e] = t[e]; fl[e] = 1; }
+      __syncthreads();
+    }
+    if (passada == 2) break;
+    // separable (1 2 1) blur
+    for (int e = tid; e < SF_N; e += 256) {
+      int lx = e % SF_R, gi = I0 + lx;
+      int em = (gi > 0 && lx > 0) ? e - 1 : e, ep = (gi < nx - 1 && lx < SF_R - 1) ? e + 1 : e;
+      t[e] = (float)(((double)f[em] + 2 * (double)f[e] + (double)f[ep]) * 0.25);
+    }
+    __syncthreads();
+    for (int e = tid; e < SF_N; e += 256) {
+      int ly = e / SF_R, gj = J0 + ly;
+      int em = (gj > 0 && ly > 0) ? e - SF_R : e, ep = (gj < ny - 1 && ly < SF_R - 1) ? e + SF_R : e;
+      f[e] = (float)(((double)t[em] + 2 * (double)t[e] + (double)t[ep]) * 0.25);
+    }
+    __syncthreads();
+  }
+  // normals (interior cells of the domain only; the reference never writes the border)
+  for (int e = tid; e < SF_N; e += 256) {
+    int lx = e % SF_R, ly = e / SF_R, gi = I0 + lx, gj = J0 + ly;
+    float a = 0.f, b = 0.f;
+    if (gi >= 1 && gj >= 1 && gi < nx - 1 && gj < ny - 1 && lx > 0 && ly > 0 && lx < SF_R - 1 && ly < SF_R - 1) {
+      double gx = ((double)f[e + 1] - (double)f[e - 1]) * 0.5, gy = ((double)f[e + SF_R] - (double)f[e - SF_R]) * 0.5;
+      double g = js_hypot(gx, gy);
+      a = g > 1e-3 ? (float)(gx / g) : 0.f;
+      b = g > 1e-3 ? (float)(gy / g) : 0.f;
+    }
+    ga[e] = a;
+    gb[e] = b;
+  }
+  __syncthreads();
+  for (int e = tid; e < SF_T * SF_T; e += 256) {
+    int lx = SF_H + e % SF_T, ly = SF_H + e / SF_T;
+    int gi = I0 + lx, gj = J0 + ly;
+    if (gi >= nx || gj >= ny) continue;
+    int q = lx + ly * SF_R;
+    size_t c = (size_t)gi + (size_t)gj * nx;
+    fase[c] = f[q];
+    nXo[c] = ga[q];
+    nYo[c] = gb[q];
+    float kk = 0.f;
+    if (gi >= 1 && gj >= 1 && gi < nx - 1 && gj < ny - 1)
+      kk = (float)(-(((double)ga[q + 1] - (double)ga[q - 1]) + ((double)gb[q + SF_R] - (double)gb[q - SF_R])) / (2 * h));
+    kap[c] = kk;
+  }
+}
+
+int surface(tq_sim* s) {
+  dim3 g((s->nx + SF_T - 1) / SF_T, (s->ny + SF_T - 1) / SF_T);
+  int t = time_begin(s, "surface", (1 + 4 + 4 * 4) * (double)s->C);
+  hipLaunchKernelGGL(k_surface, g, dim3(256), 0, s->st, s->tipo, s->dens, s->nx, s->ny, s->h, s->fase, s->nX, s->nY,
+                     s->kappa);
+  time_end(s, t);
+  TQ_LAUNCH_CHECK();
+  return 0;
+}
+
+// ================================================================== #pressaoDoAr + #classificar
+// qAr (fisica-cpu.js:381-403) of an air cell, in units of q
+__device__ inline float qar_at(const uint8_t* __restrict__ tipo, const float* __restrict__ kap, int nx, int ny, int i, int j,
+                               double tensao, double kMax, double escala) {
+#pragma clang fp contract(off)
+  if (i < 1 || j < 1 || i >= nx - 1 || j >= ny - 1) return 0.f;
+  const size_t c = (size_t)i + (size_t)j * nx;
+  if (tipo[c] != AR) return 0.f;
+  double p = 0;
+  if (tensao > 0) {
+    double s = kap[c];
+    int k = 1;
+    if (tipo[c - 1] == AGUA) { s += (double)kap[c - 1]; k++; }
+    if (tipo[c + 1] == AGUA) { s += (double)kap[c + 1]; k++; }
+    if (tipo[c - nx] == AGUA) { s += (double)kap[c - nx]; k++; }
+    if (tipo[c + nx] == AGUA) { s += (double)kap[c + nx]; k++; }
+    if (k > 1) p += tensao * fmax(-kMax, fmin(kMax, s / k));
+  }
+  return (float)(p * escala);
+}
+
+__global__ __launch_bounds__(256) void k_air_div(const uint8_t* __restrict__ tipo, const float* __restrict__ kap,
+                                                 const float* __restrict__ u, const float* __restrict__ v,
+                                                 const float* __restrict__ dens, const uint32_t* __restrict__ count,
+                                                 const uint8_t* __restrict__ solid, int nx, int ny, double tensao,
+                                                 double kMax, double escala, double alvo, int which,
+                                                 float* __restrict__ qAr, float* __restrict__ div,
+                                                 float* __restrict__ q) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  if (which & 1) qAr[c] = qar_at(tipo, kap, nx, ny, i, j, tensao, kMax, escala);
+  if (!(which & 2)) return;
+  if (tipo[c] != AGUA) { div[c] = 0.f; q[c] = 0.f; return; }
+  const double d = (double)u[(size_t)i + 1 + (size_t)j * (nx + 1)] - (double)u[(size_t)i + (size_t)j * (nx + 1)] +
+                   (double)v[(size_t)i + (size_t)(j + 1) * nx] - (double)v[(size_t)i + (size_t)j * nx];
+  double excesso = (double)dens[c] / PPC - 1;
+  if (excesso < 0) {
+    // #miolo (fisica-cpu.js:484-494)
+    bool miolo = i >= 1 && j >= 1 && i < nx - 1 && j < ny - 1;
+    for (int b = -1; b <= 1 && miolo; b++)
+      for (int a = -1; a <= 1; a++) {
+        size_t cc = (size_t)(i + a) + (size_t)(j + b) * nx;
+        if (solid[cc] || count[cc] == 0) { miolo = false; break; }
+      }
+    if (!miolo) excesso = 0;
+  }
+  excesso = fmin(EXCESSO_MAX, fmax(EXCESSO_MIN, excesso));
+  double ar = 0;
+  if (i > 0 && tipo[c - 1] == AR) ar += (double)qar_at(tipo, kap, nx, ny, i - 1, j, tensao, kMax, escala);
+  if (i < nx - 1 && tipo[c + 1] == AR) ar += (double)qar_at(tipo, kap, nx, ny, i + 1, j, tensao, kMax, escala);
+  if (j > 0 && tipo[c - nx] == AR) ar += (double)qar_at(tipo, kap, nx, ny, i, j - 1, tensao, kMax, escala);
+  if (j < ny - 1 && tipo[c + nx] == AR) ar += (double)qar_at(tipo, kap, nx, ny, i, j + 1, tensao, kMax, escala);
+  div[c] = (float)(d - alvo * excesso - ar);
+}
+
+int air_pressure_and_div(tq_sim* s, double dt, int which) {
+  const double escala = dt / (DENSIDADE_AGUA * s->h), kMax = 1 / s->h;
+  const double alvo = s->h / std::max(s->d.tau_densidade, dt);
+  int t = time_begin(s, "air_div", (1 + 4 + 4 + 4 + 4 + 1 + 4 + 4 + 4) * (double)s->C);
+  hipLaunchKernelGGL(k_air_div, dim3((s->C + 255) / 256), dim3(256), 0, s->st, s->tipo, s->kappa, s->u, s->v, s->dens,
+                     s->countCell, s->solid, s->nx, s->ny, s->d.tensao, kMax, escala, alvo, which, s->qAr, s->div, s->q);
+  time_end(s, t);
+  TQ_LAUNCH_CHECK();
+  return 0;
+}
+
+// ================================================================== #propagarRotulos (fisica-cpu.js:409-422)
+// The reference runs two raster-order in-place passes, so a new air cell can take its label
+// through a chain of other new cells.  Candidates (unlabelled interior air) are compacted; one
+// workgroup then iterates each pass to its fixpoint: rot[c] = L || R || D || U with L, D the
+// live values (already processed in raster order) and R, U the values at the start of the
+// pass (0 for candidates).  The dependency graph is acyclic (raster order), so the fixpoint is
+// unique and equals the sequential result.
+__global__ __launch_bounds__(256) void k_label_prep(const uint8_t* __restrict__ tipo, uint32_t* __restrict__ rot,
+                                                    uint8_t* __restrict__ cflag, uint32_t* __restrict__ cand,
+                                                    uint32_t* __restrict__ candCount, int nx, int ny) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  bool is_c = false;
+  if (c < nx * ny) {
+    const int i = c % nx, j = c / nx;
+    if (tipo[c] != AR) rot[c] = 0;
+    else if (i >= 1 && j >= 1 && i < nx - 1 && j < ny - 1 && rot[c] == 0) is_c = true;
+  }
+  unsigned long long b = __ballot(is_c);
+  if (!b) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)b) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(candCount, (uint32_t)__popcll(b));
+  base = __shfl(base, leader, 64);
+  if (is_c) {
+    uint32_t off = (uint32_t)__popcll(b & ((1ull << lane) - 1));
+    cand[base + off] = (uint32_t)c;
+    cflag[c] = 1;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_label_prop(uint32_t* __restrict__ rot, uint8_t* __restrict__ cflag,
+                                                     const uint32_t* __restrict__ cand,
+                                                     const uint32_t* __restrict__ candCount, int nx) {
+  __shared__ int changed;
+  const int tid = threadIdx.x;
+  const uint32_t m = *candCount;
+  for (int pass = 0; pass < 2; pass++) {
+    const uint8_t bit = (uint8_t)(1 << pass);
+    while (true) {
+      if (tid == 0) changed = 0;
+      __syncthreads();
+      for (uint32_t k = tid; k < m; k += 1024) {
+        const uint32_t c = cand[k];
+        if (!(cflag[c] & bit)) continue;
+        uint32_t val = rot[c - 1];
+        if (!val) val = (cflag[c + 1] & bit) ? 0u : rot[c + 1];
+        if (!val) val = rot[c - nx];
+        if (!val) val = (cflag[c + nx] & bit) ? 0u : rot[c + nx];
+        if (val != rot[c]) { rot[c] = val; changed = 1; }
+      }
+      __syncthreads();
+      const int ch = changed;
+      __syncthreads();
+      if (!ch) break;
+    }
+    if (pass == 0)
+      for (uint32_t k = tid; k < m; k += 1024) {
+        const uint32_t c = cand[k];
+        if (rot[c] == 0) cflag[c] |= 2;
+      }
+    __syncthreads();
+  }
+  for (uint32_t k = tid; k < m; k += 1024) cflag[cand[k]] = 0;
+}
+
+int propagate_labels(tq_sim* s) {
+  TQ_HIP(hipMemsetAsync(&s->sc->cand_count, 0, sizeof(uint32_t), s->st));
+  int t = time_begin(s, "label_prep", 6.0 * s->C);
+  hipLaunchKernelGGL(k_label_prep, dim3((s->C + 255) / 256), dim3(256), 0, s->st, s->tipo, s->rot, s->cflag, s->cand,
+                     &s->sc->cand_count, s->nx, s->ny);
+  time_end(s, t);
+  TQ_LAUNCH_CHECK();
+  t = time_begin(s, "label_prop", 0);
+  hipLaunchKernelGGL(k_label_prop, dim3(1), dim3(1024), 0, s->st, s->rot, s->cflag, s->cand, &s->sc->cand_count, s->nx);
+  time_end(s, t);
+  TQ_LAUNCH_CHECK();
+  return 0;
+}
+
+// ================================================================== #ladoDireitoDasBolhas (fisica-cpu.js:429-454)
+// Per-bubble sums: interior air cells of a bubble add their count, their water faces and the
+// inflow (u - qAr) of those faces.  Lanes of a wavefront that share a label are reduced first,
+// then one atomic per label (FP64 for the inflow sum).
+__global__ __launch_bounds__(256) void k_bubble_rhs(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
+                                                    const float* __restrict__ u, const float* __restrict__ v,
+                                                    const float* __restrict__ qAr, int nx, int ny,
+                                                    uint32_t* __restrict__ volB, uint32_t* __restrict__ facesB,
+                                                    uint32_t* __restrict__ cellsB, double* __restrict__ rhsB) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t r = 0;
+  int interior = 0, faces = 0;
+  double rhs = 0;
+  if (c < nx * ny) {
+    const int i = c % nx, j = c / nx;
+    if (tipo[c] == AR && rot[c] >= 2) {
+      r = rot[c];
+      if (i >= 1 && j >= 1 && i < nx - 1 && j < ny - 1) {
+        interior = 1;
+        const double tq = qAr[c];
+        if (tipo[c - 1] == AGUA) { rhs += (double)u[(size_t)i + (size_t)j * (nx + 1)] - tq; faces++; }
+        if (tipo[c + 1] == AGUA) { rhs += -(double)u[(size_t)i + 1 + (size_t)j * (nx + 1)] - tq; faces++; }
+        if (tipo[c - nx] == AGUA) { rhs += (double)v[(size_t)i + (size_t)j * nx] - tq; faces++; }
+        if (tipo[c + nx] == AGUA) { rhs += -(double)v[(size_t)i + (size_t)(j + 1) * nx] - tq; faces++; }
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63;
+  bool pending = r >= 2;
+  while (true) {
+    unsigned long long b = __ballot(pending);
+    if (!b) break;
+    const int leader = __ffsll((long long)b) - 1;
+    const uint32_t lr = __shfl(r, leader, 64);
+    const bool m = pending && r == lr;
+    double sr = wave_sum(m ? rhs : 0.0);
+    unsigned long long bm = __ballot(m);
+    unsigned long long bi = __ballot(m && interior);
+    int sf = (int)wave_sum(m ? (double)faces : 0.0);
+    if (lane == leader) {
+      atomicAdd(&cellsB[lr], (uint32_t)__popcll(bm));
+      if (bi) {
+        atomicAdd(&volB[lr], (uint32_t)__popcll(bi));
+        atomicAdd(&facesB[lr], (uint32_t)sf);
+        atomicAdd(&rhsB[lr], sr);
+      }
+    }
+    if (m) pending = false;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_bubble_finish(const uint32_t* __restrict__ volB, double* __restrict__ rhsB,
+                                                        float* __restrict__ rhsB32, float* __restrict__ yB32,
+                                                        const float* __restrict__ pBolha,
+                                                        const float* __restrict__ volAlvo, double h, double dt) {
+#pragma clang fp contract(off)
+  const double paraQ = dt / (DENSIDADE_AGUA * h);
+  for (int r = threadIdx.x; r < MAX_BOLHAS; r += blockDim.x) {
+    if (r < 2 || !volB[r]) { rhsB32[r] = (float)rhsB[r]; continue; }
+    double x = rhsB[r] + VOLTA_VOLUME_BOLHA * ((double)volAlvo[r] - (double)volB[r]) * h / dt;
+    rhsB[r] = x;
+    rhsB32[r] = (float)x;
+    yB32[r] = (float)((double)pBolha[r] * paraQ);
+  }
+}
+
+int bubble_rhs(tq_sim* s, double dt) {
+  TQ_HIP(hipMemsetAsync(s->volB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
+  TQ_HIP(hipMemsetAsync(s->facesB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
+  TQ_HIP(hipMemsetAsync(s->cellsB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
+  TQ_HIP(hipMemsetAsync(s->rhsB, 0, sizeof(double) * MAX_BOLHAS, s->st));
+  int t = time_begin(s, "bubble_rhs", (1 + 4 + 4 + 4 + 4) * (double)s->C);
+  hipLaunchKernelGGL(k_bubble_rhs, dim3((s->C + 255) / 256), dim3(256), 0, s->st, s->tipo, s->rot, s->u, s->v, s->qAr,
+                     s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB);
+  time_end(s, t);
+  TQ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bubble_finish, dim3(1), dim3(1024), 0, s->st, s->volB, s->rhsB, s->rhsB32, s->yB32, s->pBolha,
+                     s->volAlvo, s->h, dt);
+  TQ_LAUNCH_CHECK();
+  return 0;
+}
+
+// ================================================================== #pressao post (fisica-cpu.js:503-514)
+__global__ __launch_bounds__(256) void k_pressure_post(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
+                                                       const float* __restrict__ qAr, const uint32_t* __restrict__ volB,
+                                                       const float* __restrict__ yB32, int C, int ar,
+                                                       float* __restrict__ q) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C || tipo[c] != AR) return;
+  const uint32_t r = rot[c];
+  q[c] = (float)((double)qAr[c] + ((ar && r >= 2 && volB[r]) ? (double)yB32[r] : 0.0));
+}
+
+__global__ __launch_bounds__(1024) void k_bubble_pressure(const uint32_t* __restrict__ volB, const float* __restrict__ yB32,
+                                                          float* __restrict__ pBolha, double paraPa) {
+#pragma clang fp contract(off)
+  for (int r = threadIdx.x; r < MAX_BOLHAS; r += blockDim.x)
+    if (r >= 2 && volB[r])
+      pBolha[r] = (float)fmax(-PRESSAO_BOLHA_MAX, fmin(PRESSAO_BOLHA_MAX, (double)yB32[r] * paraPa));
+}
+
+int pressure_post(tq_sim* s, double dt) {
+  hipLaunchKernelGGL(k_pressure_post, dim3((s->C + 255) / 256), dim3(256), 0, s->st, s->tipo, s->rot, s->qAr, s->volB,
+                     s->yB32, s->C, s->d.ar, s->q);
+  TQ_LAUNCH_CHECK();
+  if (s->d.ar) {
+    hipLaunchKernelGGL(k_bubble_pressure, dim3(1), dim3(1024), 0, s->st, s->volB, s->yB32, s->pBolha,
+                       (DENSIDADE_AGUA * s->h) / dt);
+    TQ_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+// ================================================================== #gradiente (fisica-cpu.js:519-545)
+__global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ tipo, const float* __restrict__ q, int nx,
+                                                  int ny, float* __restrict__ u, uint8_t* __restrict__ uVal,
+                                                  float* __restrict__ v, uint8_t* __restrict__ vVal) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nu = (nx + 1) * ny, nv = nx * (ny + 1);
+  if (k < nu) {
+    const int i = k % (nx + 1), j = k / (nx + 1);
+    const uint8_t tl = i > 0 ? tipo[(size_t)(i - 1) + (size_t)j * nx] : SOLIDO;
+    const uint8_t tr = i < nx ? tipo[(size_t)i + (size_t)j * nx] : SOLIDO;
+    if (tl == SOLIDO || tr == SOLIDO) { u[k] = 0.f; uVal[k] = 1; }
+    else if (tl == AGUA || tr == AGUA) {
+      u[k] = (float)((double)u[k] - ((double)q[(size_t)i + (size_t)j * nx] - (double)q[(size_t)(i - 1) + (size_t)j * nx]));
+      uVal[k] = 1;
+    } else uVal[k] = 0;
+  }
+  if (k < nv) {
+    const int i = k % nx, j = k / nx;
+    const uint8_t tb = j > 0 ? tipo[(size_t)i + (size_t)(j - 1) * nx] : SOLIDO;
+    const uint8_t tt = j < ny ? tipo[(size_t)i + (size_t)j * nx] : SOLIDO;
+    if (tb == SOLIDO || tt == SOLIDO) { v[k] = 0.f; vVal[k] = 1; }
+    else if (tb == AGUA || tt == AGUA) {
+      v[k] = (float)((double)v[k] - ((double)q[(size_t)i + (size_t)j * nx] - (double)q[(size_t)i + (size_t)(j - 1) * nx]));
+      vVal[k] = 1;
+    } else vVal[k] = 0;
+  }
+}
+
+int gradient(tq_sim* s) {
+  const int n = std::max(s->nu, s->nv);
+  int t = time_begin(s, "gradient", 9.0 * (s->nu + s->nv));
+  hipLaunchKernelGGL(k_gradient, dim3((n + 255) / 256), dim3(256), 0, s->st, s->tipo, s->q, s->nx, s->ny, s->u, s->uVal,
+                     s->v, s->vVal);
+  time_end(s, t);
+  TQ_LAUNCH_CHECK();
+  return 0;
+}
+
+// ================================================================== #extrapolar (fisica-cpu.js:547-552, 673-690)
+// One layer for both components: an invalid face with valid 4-neighbours takes their mean.
+__global__ __launch_bounds__(256) void k_extrap(const float* __restrict__ ua, const uint8_t* __restrict__ uva,
+                                                float* __restrict__ ub, uint8_t* __restrict__ uvb,
+                                                const float* __restrict__ va, const uint8_t* __restrict__ vva,
+                                                float* __restrict__ vb, uint8_t* __restrict__ vvb, int nx, int ny) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nu = (nx + 1) * ny, nv = nx * (ny + 1);
+  for (int comp = 0; comp < 2; comp++) {
+    const float* a = comp ? va : ua;
+    const uint8_t* val = comp ? vva : uva;
+    float* o = comp ? vb : ub;
+    uint8_t* ov = comp ? vvb : uvb;
+    const int w = comp ? nx : nx + 1, hh = comp ? ny + 1 : ny, n = comp ? nv : nu;
+    if (k >= n) continue;
+    float res = a[k];
+    uint8_t vres = val[k];
+    if (!vres) {
+      const int i = k % w, j = k / w;
+      double soma = 0;
+      int m = 0;
+      if (i > 0 && val[k - 1]) { soma += (double)a[k - 1]; m++; }
+      if (i < w - 1 && val[k + 1]) { soma += (double)a[k + 1]; m++; }
+      if (j > 0 && val[k - w]) { soma += (double)a[k - w]; m++; }
+      if (j < hh - 1 && val[k + w]) { soma += (double)a[k + w]; m++; }
+      if (m > 0) { res = (float)(soma / m); vres = 1; }
+    }
+    o[k] = res;
+    ov[k] = vres;
+  }
+}
+
+int extrapolate(tq_sim* s) {
+  const int n = std::max(s->nu, s->nv);
+  int t = time_begin(s, "extrapolate", 10.0 * (s->nu + s->nv));
+  hipLaunchKernelGGL(k_extrap, dim3((n + 255) / 256), dim3(256), 0, s->st, s->u, s->uVal, s->uTmp, s->uValTmp, s->v,
+                     s->vVal, s->vTmp, s->vValTmp, s->nx, s->ny);
+  hipLaunchKernelGGL(k_extrap, dim3((n + 255) / 256), dim3(256), 0, s->st, s->uTmp, s->uValTmp, s->u, s->uVal, s->vTmp,
+                     s->vValTmp, s->v, s->vVal, s->nx, s->ny);
+  time_end(s, t);
+  TQ_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace tq

@@ -1,0 +1,171 @@
+// tq_device.hpp -- device helpers: wavefront reductions, the deterministic last-block
+// reduction, and the binary64 scene/number helpers shared by P2G, G2P and particle adds.
+#pragma once
+#include "tq_internal.hpp"
+
+namespace tq {
+
+// ------------------------------------------------------------------ wavefront (64 lanes) reductions
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline float wave_sumf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline float wave_maxf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline double wave_maxd(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Block sum of NV doubles per thread; result valid in thread 0.  `sh` holds >= NV*16 doubles.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) sh[k * 16 + wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double t = 0;
+      for (int w = 0; w < nw; w++) t += sh[k * 16 + w];
+      v[k] = t;
+    }
+  }
+  __syncthreads();
+}
+
+// Deterministic grid reduction of NV doubles: each block stores its partials, the last block
+// to finish (ticket) sums all partials in block order.  Returns true in every thread of the
+// last block; then `tot` (valid in all its threads) holds the grid totals.
+template <int NV>
+__device__ inline bool grid_sum_last(double (&v)[NV], double* partials, uint32_t* ticket, double (&tot)[NV],
+                                     double* sh) {
+  block_sum<NV>(v, sh);
+  __shared__ bool last;
+  const int nb = gridDim.x * gridDim.y;
+  const int bid = blockIdx.x + blockIdx.y * gridDim.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) partials[(size_t)k * MAX_RED_BLOCKS + bid] = v[k];
+    __threadfence();
+    unsigned t = atomicAdd(ticket, 1u);
+    last = (t == (unsigned)(nb - 1));
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) acc[k] = 0;
+  // fixed assignment of partials to threads -> deterministic totals
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < NV; k++) acc[k] += __builtin_nontemporal_load(&partials[(size_t)k * MAX_RED_BLOCKS + b]);
+  block_sum<NV>(acc, sh);
+  __shared__ double res[NV];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) res[k] = acc[k];
+    *ticket = 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) tot[k] = res[k];
+  return true;
+}
+
+// ------------------------------------------------------------------ binary64 number helpers
+// Math.hypot as V8 computes it (scaled, Kahan-compensated sum of squares).
+__device__ inline double js_hypot(double a, double b) {
+#pragma clang fp contract(off)
+  double xa = fabs(a), xb = fabs(b);
+  if (isinf(xa) || isinf(xb)) return INFINITY;
+  if (isnan(xa) || isnan(xb)) return NAN;
+  double mx = fmax(xa, xb);
+  if (mx == 0) return 0;
+  double sum = 0, comp = 0;
+  {
+    double n = xa / mx;
+    double summand = n * n - comp;
+    double prelim = sum + summand;
+    comp = (prelim - sum) - summand;
+    sum = prelim;
+  }
+  {
+    double n = xb / mx;
+    double summand = n * n - comp;
+    double prelim = sum + summand;
+    comp = (prelim - sum) - summand;
+    sum = prelim;
+  }
+  return sqrt(sum) * mx;
+}
+
+// clampi (fisica-cpu.js:697-699) on a double
+__device__ inline double clampd(double v, double a, double b) { return v < a ? a : v > b ? b : v; }
+
+// Cena #bilinear (cena.js:156-165) on the (2nx) x (2ny) SDF
+__device__ inline double sdf_bilinear(const float* __restrict__ sdf, int w, int hh, double fx, double fy) {
+#pragma clang fp contract(off)
+  if (fx != fx || fy != fy) return NAN;  // JS: Math.min(NaN, ..) is NaN and the lookup is NaN
+  if (fx < 0) fx = 0; else if (fx > w - 1) fx = w - 1;
+  if (fy < 0) fy = 0; else if (fy > hh - 1) fy = hh - 1;
+  double x0 = fmin(floor(fx), (double)(w - 2)), y0 = fmin(floor(fy), (double)(hh - 2));
+  double tx = fx - x0, ty = fy - y0;
+  size_t k = (size_t)y0 * w + (size_t)x0;
+  return ((double)sdf[k] * (1 - tx) + (double)sdf[k + 1] * tx) * (1 - ty) +
+         ((double)sdf[k + w] * (1 - tx) + (double)sdf[k + w + 1] * tx) * ty;
+}
+// Cena.distancia (cena.js:102-106)
+__device__ inline double sdf_distancia(const float* __restrict__ sdf, int w, int hh, double h, double x, double y) {
+#pragma clang fp contract(off)
+  double fx = (x / h) * 2 - 0.5, fy = (y / h) * 2 - 0.5;
+  return sdf_bilinear(sdf, w, hh, fx, fy);
+}
+// Cena.normal (cena.js:109-117)
+__device__ inline void sdf_normal(const float* __restrict__ sdf, int w, int hh, double h, double x, double y,
+                                  double* nxo, double* nyo) {
+#pragma clang fp contract(off)
+  double fx = (x / h) * 2 - 0.5, fy = (y / h) * 2 - 0.5;
+  double gx = sdf_bilinear(sdf, w, hh, fx + 1, fy) - sdf_bilinear(sdf, w, hh, fx - 1, fy);
+  double gy = sdf_bilinear(sdf, w, hh, fx, fy + 1) - sdf_bilinear(sdf, w, hh, fx, fy - 1);
+  double n = js_hypot(gx, gy);
+  if (n > 1e-9) { *nxo = gx / n; *nyo = gy / n; } else { *nxo = 0; *nyo = 1; }
+}
+
+// tile-major binning key of a cell (16x16 tiles)
+__device__ __host__ inline uint32_t cell_key(int ci, int cj, int ntx) {
+  return (uint32_t)((((cj >> 4) * ntx + (ci >> 4)) << 8) | ((cj & 15) << 4) | (ci & 15));
+}
+
+// count cell of a particle: clampi(floor(x/h*...)) in binary64 (fisica-cpu.js:258)
+__device__ inline void count_cell(float x, float y, double inv, int nx, int ny, int* ci, int* cj) {
+#pragma clang fp contract(off)
+  double a = clampd(floor((double)x * inv), 0, nx - 1), b = clampd(floor((double)y * inv), 0, ny - 1);
+  *ci = (int)a;
+  *cj = (int)b;
+}
+
+__device__ inline long long to_fx(double v) {
+  // fixed-point with 2^-32 resolution; non-finite or absurd contributions saturate
+  double s = v * FX_SCALE;
+  if (!(s > -9.0e18)) s = s != s ? 0.0 : -9.0e18;
+  if (s > 9.0e18) s = 9.0e18;
+  return (long long)__builtin_rint(s);
+}
+
+}  // namespace tq

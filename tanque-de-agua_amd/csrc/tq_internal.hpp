@@ -1,0 +1,241 @@
+// tq_internal.hpp -- device state of one tank and the launchers shared by the HIP sources.
+//
+// Layout in HBM (one tank, C = nx*ny cells):
+//   particles  SoA, two buffers (sorted by cell each substep): pos float2, vel float2,
+//              cm float4 (cux cuy cvx cvy), ext float4 (espuma absR absG absB), seq u32
+//              (insertion number: the reference's particle order is ascending seq).
+//   MAC grid   u (nx+1)*ny at (i h, (j+1/2) h), v nx*(ny+1) at ((i+1/2) h, j h),
+//              cell arrays i + j*nx, y up -- the reference's indexing (fisica-cpu.js:38-70).
+//   binning    keys are tile-major (16x16 cell tiles) so one tile's particles are contiguous.
+//   solver     per MG level: u8 mask (neighbour bits | type), x/b/tmp in f32 or f16.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tanque.h"
+
+namespace tq {
+
+// ------------------------------------------------------------------ constants (js/constantes.js)
+constexpr double GRAVIDADE = 9.81;
+constexpr double DENSIDADE_AGUA = 1000;
+constexpr double TENSAO_SUPERFICIAL = 0.0728;
+constexpr int MAX_BOLHAS = TQ_MAX_BOLHAS;
+constexpr double VOLTA_VOLUME_BOLHA = 0.05;
+constexpr double PRESSAO_BOLHA_MAX = 20000;
+constexpr int PPC = 4;
+constexpr double SDF_MAX = 10;
+constexpr int MOLDURA = 2;
+constexpr double RAIO_PARTICULA = 0.3;
+constexpr double MICROPASSO = 0.8;
+constexpr int MAX_MICROPASSOS = 4;
+constexpr double TAU_DENSIDADE = 0.1;
+constexpr double EXCESSO_MIN = -0.5, EXCESSO_MAX = 1.0;
+constexpr int ITERACOES_PCG = 3;
+constexpr double ESPUMA_V0 = 1.1, ESPUMA_V1 = 2.8, ESPUMA_TAXA = 5, ESPUMA_VIDA = 1.2;
+constexpr double DIFUSAO_CORANTE = 3;
+constexpr double PARTICULAS_MAX_POR_CELULA = 4.6;
+constexpr uint8_t SOLIDO = 0, AGUA = 1, AR = 2;
+constexpr int MIN_BOLHA_CELULAS = 4;
+
+constexpr int TILE = 16;                 // P2G binning tile and the MG smoother block (pressao-cpu.js:17)
+constexpr uint32_t DEAD_KEY = 0xFFFFFFFFu;
+constexpr double FX_SCALE = 4294967296.0;  // 2^32: fixed-point scale of the P2G sums
+constexpr double FX_INV = 1.0 / 4294967296.0;
+constexpr int RED_BLOCK = 256;           // block size of reduction kernels
+constexpr int MAX_RED_BLOCKS = 8192;     // partial slots per reduction
+
+// mask byte of a multigrid cell / operator cell
+//   bits 0-3: free neighbours (1 left, 2 right, 4 down, 8 up) as prepararVizinhos
+//   bits 4-5: cell type (0 solid, 1 water, 2 air)
+__host__ __device__ inline int mask_viz(uint8_t m) { return m & 15; }
+__host__ __device__ inline int mask_type(uint8_t m) { return (m >> 4) & 3; }
+__host__ __device__ inline int mask_cnt(uint8_t m) { return __builtin_popcount(m & 15); }
+
+// Device-side scalars of one tank (one allocation).
+struct DevScalars {
+  uint32_t n_slots;     // particle slots in the current buffer (live + dead-marked)
+  uint32_t n_live;      // live particles after the last binning
+  unsigned long long vmax_bits;  // max |v| of the last G2P (binary64 bits, non-negative)
+  uint32_t add_count;   // particles accepted by the last add
+  uint32_t cand_count;  // label-propagation candidates
+  uint32_t ncomp;       // air components of the last tracker update
+  uint32_t npairs;      // (component, old label) pairs of the last tracker update
+  uint32_t ticket[16];  // last-block counters of the reductions
+  int done;             // PCG converged / stopped
+  int iters;            // PCG iterations run
+  int primeira;         // tracker first snapshot flag
+  int active_bubbles;
+  double rz, pap, alpha, beta, norm2, norm0, normb, norm_last;
+  double tol;
+  double proj_mean;     // null-space component removed from b (residual-terminated mode)
+  int maxit;
+  float mg_scale;       // mixed precision: scale applied to the V-cycle input
+  float pad_;
+};
+
+struct Level {
+  int nx = 0, ny = 0, n = 0;
+  uint8_t* mask = nullptr;
+  void* b = nullptr;
+  void* x = nullptr;
+  void* tmp = nullptr;
+};
+
+struct KTime {
+  std::string name;
+  double total_ms = 0;
+  long launches = 0;
+  double bytes = 0;
+};
+
+}  // namespace tq
+
+struct tq_sim {
+  tq_desc d;
+  int nx, ny, C, nu, nv;
+  int ntx, nty, K;        // binning tiles and key space (ntx*nty*256)
+  double h;
+  hipStream_t st = nullptr;
+
+  // particles
+  int cap = 0;
+  int n_upper = 0;        // host-side upper bound of the slot count
+  int cur = 0;
+  float2* pos[2] = {};
+  float2* vel[2] = {};
+  float4* cm[2] = {};
+  float4* ext[2] = {};
+  uint32_t* seq[2] = {};
+  uint32_t* pkey = nullptr;
+  uint32_t* prank = nullptr;
+  uint32_t* addflag = nullptr;  // add/compaction scratch (cap+1)
+  uint32_t seq_next = 0;
+  float* stage = nullptr;       // staging for uploads of particle lists (cap*12)
+
+  // scene / grid
+  float* sdf = nullptr;
+  uint8_t* solid = nullptr;
+  uint32_t* countKey = nullptr;  // K
+  uint32_t* startKey = nullptr;  // K+1
+  uint32_t* countCell = nullptr; // C
+  float *u = nullptr, *v = nullptr, *uw = nullptr, *vw = nullptr;
+  float *uTmp = nullptr, *vTmp = nullptr;
+  uint8_t *uVal = nullptr, *vVal = nullptr, *uValTmp = nullptr, *vValTmp = nullptr;
+  float *dens = nullptr, *dye = nullptr;
+  uint8_t* tipo = nullptr;
+  float *fase = nullptr, *nX = nullptr, *nY = nullptr, *kappa = nullptr;
+  float *qAr = nullptr, *div = nullptr, *q = nullptr;
+
+  // air
+  uint32_t* rot = nullptr;
+  uint8_t* cflag = nullptr;
+  uint32_t* cand = nullptr;
+  int32_t* pai = nullptr;
+  int32_t* comp = nullptr;
+  uint32_t* rootFlag = nullptr;  // C+1 (scan input / component numbering)
+  uint32_t* compIdx = nullptr;   // C+1
+  uint32_t* compCells = nullptr; // per component: cells, atm, topo, (C each)
+  uint32_t* compAtm = nullptr;
+  uint32_t* compTopo = nullptr;
+  float* compAlvo = nullptr;     // novoAlvo per component
+  int32_t* compDono = nullptr;
+  uint32_t* compLabel = nullptr;
+  unsigned long long* hkeys = nullptr;  // (component, old label) hash table
+  uint32_t* hcount = nullptr;
+  uint32_t* hfirst = nullptr;
+  uint32_t* hlist = nullptr;
+  int hsize = 0;
+  uint32_t* ordBuf = nullptr;   // bookkeeping scratch (sort buffers), 4*C
+  // bubbles (MAX_BOLHAS each)
+  float* pBolha = nullptr;     // pressaoBolha (Pa)
+  uint32_t* volB = nullptr;    // volumeBolha (cell counts)
+  uint32_t* facesB = nullptr;  // water faces per bubble
+  uint32_t* cellsB = nullptr;  // cells per bubble (celulasDe)
+  double* rhsB = nullptr;      // bubble rhs (FP64 sums)
+  float* rhsB32 = nullptr;     // rhsBolha as the reference stores it
+  float* yB32 = nullptr;       // yBolha (f32 as stored by the reference)
+  float* volAlvo = nullptr;    // volumeAlvo
+  uint8_t* vivo = nullptr;
+  uint8_t* vivoNovo = nullptr;
+  double* bvec = nullptr;      // bubble CG vectors: yb rb zb zvb pb apb acc0 acc1 (8 x MAX_BOLHAS)
+  int* activeList = nullptr;   // active bubble labels (count in sc->active_bubbles)
+
+  // solver
+  std::vector<tq::Level> lv;
+  uint8_t* opMask = nullptr;
+  float *r = nullptr, *ap = nullptr;
+  float* zbuf[2] = {};
+  float* pbuf[2] = {};
+  int zc = 0, pc = 0;
+  double* partials = nullptr;   // MAX_RED_BLOCKS * 4
+  tq::DevScalars* sc = nullptr;
+  tq::DevScalars* sc_host = nullptr;  // pinned mirror
+
+  // scan scratch
+  uint32_t* scanBlocks = nullptr;
+
+  // host-side state
+  bool tipos_prontos = false;
+  bool has_vmax = false;
+  bool air_reset_pending = true;
+
+  // timing
+  bool timing = false;
+  std::vector<tq::KTime> ktimes;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+  std::vector<std::pair<int, int>> ev_pending;  // (ktime index, event pair index)
+  int ev_used = 0;
+};
+
+namespace tq {
+
+// error helpers (tq_api.cpp)
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+#define TQ_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) return tq::hip_fail(e_, #call, __FILE__, __LINE__); \
+  } while (0)
+#define TQ_LAUNCH_CHECK()                                                   \
+  do {                                                                      \
+    hipError_t e_ = hipGetLastError();                                      \
+    if (e_ != hipSuccess) return tq::hip_fail(e_, "kernel launch", __FILE__, __LINE__); \
+  } while (0)
+
+// timing brackets around launches (tq_api.cpp)
+int time_begin(tq_sim* s, const char* name, double bytes);
+void time_end(tq_sim* s, int slot);
+
+// ---- launchers (k_particles.hip)
+int scan_u32(tq_sim* s, const uint32_t* in, uint32_t* out, int n);  // exclusive, out[n] = total
+int bin_and_sort(tq_sim* s);                  // keys, counting sort into the other buffer
+int p2g(tq_sim* s, double dt, int gravity);   // fused P2G + gravity/boundary + types
+int g2p(tq_sim* s, double dt);
+int add_particles_dev(tq_sim* s, int count, int check);  // from s->stage
+int apply_tool_dev(tq_sim* s, const tq_tool* t);
+int count_live(tq_sim* s, int* n);
+int solid_cells(tq_sim* s);
+// ---- launchers (k_grid.hip)
+int surface(tq_sim* s);
+int air_pressure_and_div(tq_sim* s, double dt, int which);  // which: 1 qAr, 2 div, 3 both
+int propagate_labels(tq_sim* s);
+int bubble_rhs(tq_sim* s, double dt);
+int gradient(tq_sim* s);
+int extrapolate(tq_sim* s);
+int pressure_post(tq_sim* s, double dt);
+// ---- launchers (k_air.hip)
+int tracker_update(tq_sim* s);
+int tracker_reset(tq_sim* s);
+// ---- launchers (k_solver.hip)
+int solver_alloc(tq_sim* s);
+void solver_free(tq_sim* s);
+int pressure_solve(tq_sim* s);
+int true_residual(tq_sim* s, double* out);
+
+}  // namespace tq

@@ -1,0 +1,351 @@
+"""Python host binding of libtanque.so (the C ABI in include/tanque.h).
+
+`SimulacaoHIP` mirrors the reference's SimulacaoCPU (js/fisica-cpu.js:28-671): same method names,
+argument meaning and state accessors, so code written against the reference's CPU engine (its
+tests, MotorCPU) drives the MI355X path unchanged.  There is no CPU fallback: importing this
+module fails if the HIP library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libtanque.so")
+
+CAMPOS = 12
+MAX_BOLHAS = 4096
+PREC_FP32, PREC_MIXED = 0, 1
+
+F_U, F_V, F_UPESO, F_VPESO, F_UVAL, F_VVAL, F_DENS, F_CONT, F_COR, F_TIPO, F_SOLIDO = range(11)
+F_DIV, F_Q, F_FASE, F_NX, F_NY, F_KAPPA, F_QAR, F_ROT = range(11, 19)
+F_PBOLHA, F_VOLBOLHA, F_RHSBOLHA, F_YBOLHA, F_VALVO, F_VIVO = range(19, 25)
+FIELD_DTYPE = {F_UVAL: np.uint8, F_VVAL: np.uint8, F_TIPO: np.uint8, F_SOLIDO: np.uint8, F_VIVO: np.uint8,
+               F_CONT: np.uint32, F_ROT: np.uint32}
+
+
+class TqError(RuntimeError):
+    pass
+
+
+class Desc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("h", C.c_double), ("gx", C.c_double), ("gy", C.c_double),
+                ("iteracoes", C.c_int), ("tau_densidade", C.c_double), ("tensao", C.c_double), ("ar", C.c_int),
+                ("max_particles", C.c_int), ("precision", C.c_int), ("tol", C.c_double)]
+
+
+class Tool(C.Structure):
+    _fields_ = [("kind", C.c_int), ("x", C.c_double), ("y", C.c_double), ("raio", C.c_double),
+                ("vx", C.c_double), ("vy", C.c_double), ("absorcao", C.c_double * 3)]
+
+
+class SolverStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("norm0", C.c_double), ("norm", C.c_double), ("norm_b", C.c_double),
+                ("true_resid", C.c_double), ("levels", C.c_int), ("active_bubbles", C.c_int)]
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("total_ms", C.c_double), ("launches", C.c_long),
+                ("bytes_per_launch", C.c_double)]
+
+
+# every symbol include/tanque.h declares, with its ctypes signature
+SIGNATURES = {
+    "tq_desc_default": (None, [C.POINTER(Desc), C.c_int, C.c_int, C.c_double]),
+    "tq_create": (C.c_int, [C.POINTER(Desc), C.POINTER(C.c_void_p)]),
+    "tq_destroy": (C.c_int, [C.c_void_p]),
+    "tq_set_sdf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
+    "tq_add_particles": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "tq_apply_tool": (C.c_int, [C.c_void_p, C.POINTER(Tool)]),
+    "tq_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "tq_substep": (C.c_int, [C.c_void_p, C.c_double]),
+    "tq_update_air": (C.c_int, [C.c_void_p]),
+    "tq_reset_air": (C.c_int, [C.c_void_p]),
+    "tq_clear_water": (C.c_int, [C.c_void_p]),
+    "tq_truncate": (C.c_int, [C.c_void_p, C.c_int]),
+    "tq_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "tq_read_particles": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "tq_vmax": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "tq_area": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "tq_set_options": (C.c_int, [C.c_void_p, C.POINTER(Desc)]),
+    "tq_read_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
+    "tq_write_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
+    "tq_write_particles": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "tq_set_tracker_first": (C.c_int, [C.c_void_p, C.c_int]),
+    "tq_run_phase": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "tq_solver_stats_get": (C.c_int, [C.c_void_p, C.POINTER(SolverStats), C.c_int]),
+    "tq_timing_enable": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p]),
+    "tq_timing_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int, C.POINTER(C.c_int)]),
+    "tq_sync": (C.c_int, [C.c_void_p]),
+    "tq_last_error": (C.c_char_p, []),
+    "tq_scene_frame": (None, [C.c_void_p, C.c_int, C.c_int]),
+    "tq_scene_stroke": (None, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                               C.c_double, C.c_double, C.c_int]),
+    "tq_seed_rect": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                               C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_void_p, C.c_int]),
+    "tq_passo_maximo": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libtanque.so; raises if it is not built (there is no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TqError("libtanque.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _chk(rc):
+    if rc != 0:
+        raise TqError("libtanque error %d: %s" % (rc, load().tq_last_error().decode(errors="replace")))
+
+
+# ------------------------------------------------------------------ host helpers (scene, seeding)
+class Cena:
+    """Walls as an SDF on a 2x2-per-cell lattice (the reference's Cena, js/cena.js), built by the
+    library's host helpers."""
+
+    def __init__(self, nx, ny, h):
+        self.nx, self.ny, self.h = nx, ny, h
+        self.largura, self.altura = nx * h, ny * h
+        self.sdf = np.zeros(4 * nx * ny, np.float32)
+        self.tracos = []
+        load().tq_scene_frame(self.sdf.ctypes.data, nx, ny)
+
+    def tracar(self, x0, y0, x1, y1, raio, apagar=False):
+        self.tracos.append((x0, y0, x1, y1, raio, apagar))
+        load().tq_scene_stroke(self.sdf.ctypes.data, self.nx, self.ny, self.h, x0, y0, x1, y1, raio, 1 if apagar else 0)
+
+
+class Semente:
+    """Seeded LCG (testes/apoio.js:4-9) shared with seed_rect."""
+
+    def __init__(self, s=1):
+        self.state = C.c_uint32(s & 0xFFFFFFFF)
+
+
+def semear_retangulo(x0, y0, x1, y1, h, absorcao=(0, 0, 0), semente=None):
+    """semearRetangulo (fisica-cpu.js:705-727)."""
+    semente = semente or Semente(1)
+    L = load()
+    a3 = (C.c_double * 3)(*absorcao)
+    n = L.tq_seed_rect(x0, y0, x1, y1, h, a3, C.byref(semente.state), None, 0)
+    out = np.zeros(max(n, 0) * CAMPOS, np.float32)
+    if n > 0:
+        got = L.tq_seed_rect(x0, y0, x1, y1, h, a3, C.byref(semente.state), out.ctypes.data, n)
+        assert got == n
+    return out
+
+
+def passo_maximo(h, altura, vmax=None, dt_quadro=1 / 60):
+    return load().tq_passo_maximo(h, altura, 0.0 if vmax is None else vmax, 0 if vmax is None else 1, dt_quadro)
+
+
+# ------------------------------------------------------------------ the simulation
+class SimulacaoHIP:
+    """SimulacaoCPU's interface on the MI355X path (js/fisica-cpu.js:28-671)."""
+
+    def __init__(self, cena, gravidade=(0.0, -9.81), iteracoes=3, tauDensidade=0.1, tensao=0.0728, ar=True,
+                 precision=PREC_FP32, tol=0.0, max_particles=0):
+        L = load()
+        d = Desc()
+        L.tq_desc_default(C.byref(d), cena.nx, cena.ny, cena.h)
+        d.gx, d.gy = gravidade
+        d.iteracoes, d.tau_densidade, d.tensao, d.ar = iteracoes, tauDensidade, tensao, 1 if ar else 0
+        d.precision, d.tol, d.max_particles = precision, tol, max_particles
+        self._desc = d
+        h = C.c_void_p()
+        _chk(L.tq_create(C.byref(d), C.byref(h)))
+        self._h = h
+        self.cena, self.nx, self.ny, self.h = cena, cena.nx, cena.ny, cena.h
+        self.arAtivo = bool(ar)
+        self.atualizarParedes()
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.tq_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # ---- options
+    def set_options(self, **kw):
+        d = self._desc
+        for k, v in kw.items():
+            if k == "gravidade":
+                d.gx, d.gy = v
+            elif k == "ar":
+                d.ar = 1 if v else 0
+                self.arAtivo = bool(v)
+            else:
+                setattr(d, {"tauDensidade": "tau_densidade"}.get(k, k), v)
+        _chk(load().tq_set_options(self._h, C.byref(d)))
+
+    @property
+    def gravidade(self):
+        return (self._desc.gx, self._desc.gy)
+
+    @gravidade.setter
+    def gravidade(self, g):
+        self.set_options(gravidade=g)
+
+    # ---- the reference's methods
+    def atualizarParedes(self, row0=0, row1=None):
+        sdf = np.ascontiguousarray(self.cena.sdf, np.float32)
+        row1 = 2 * self.ny - 1 if row1 is None else row1
+        _chk(load().tq_set_sdf(self._h, sdf.ctypes.data, row0, row1))
+
+    def adicionar(self, lista, checarCheia=True):
+        lista = np.ascontiguousarray(lista, np.float32)
+        acc = C.c_int(0)
+        _chk(load().tq_add_particles(self._h, lista.ctypes.data, lista.size // CAMPOS, 1 if checarCheia else 0,
+                                     C.byref(acc)))
+        return acc.value
+
+    def aplicarFerramenta(self, f):
+        if not f or f.get("tipo", "nenhuma") == "nenhuma":
+            return
+        t = Tool()
+        t.kind = {"esponja": 1, "mao": 2, "corante": 3}[f["tipo"]]
+        t.x, t.y, t.raio = f["x"], f["y"], f["raio"]
+        t.vx, t.vy = f.get("vx", 0.0), f.get("vy", 0.0)
+        for k, a in enumerate(f.get("absorcao", (0, 0, 0))):
+            t.absorcao[k] = a
+        _chk(load().tq_apply_tool(self._h, C.byref(t)))
+
+    def avancar(self, dt, subpassos=1):
+        _chk(load().tq_step(self._h, dt, subpassos))
+
+    def subpasso(self, dt):
+        _chk(load().tq_substep(self._h, dt))
+
+    def atualizarAr(self):
+        _chk(load().tq_update_air(self._h))
+
+    def reiniciarAr(self):
+        _chk(load().tq_reset_air(self._h))
+
+    def limparAgua(self):
+        _chk(load().tq_clear_water(self._h))
+
+    def area(self):
+        a = C.c_double()
+        _chk(load().tq_area(self._h, C.byref(a)))
+        return a.value
+
+    @property
+    def n(self):
+        n = C.c_int()
+        _chk(load().tq_count(self._h, C.byref(n)))
+        return n.value
+
+    @n.setter
+    def n(self, m):
+        _chk(load().tq_truncate(self._h, m))
+
+    @property
+    def vMax(self):
+        v, has = C.c_double(), C.c_int()
+        _chk(load().tq_vmax(self._h, C.byref(v), C.byref(has)))
+        return v.value if has.value else None
+
+    def particulas(self):
+        n = C.c_int()
+        _chk(load().tq_read_particles(self._h, None, 0, C.byref(n)))
+        out = np.zeros(n.value * CAMPOS, np.float32)
+        _chk(load().tq_read_particles(self._h, out.ctypes.data, n.value, C.byref(n)))
+        return out.reshape(-1, CAMPOS)
+
+    @property
+    def p(self):
+        return self.particulas().reshape(-1)
+
+    # ---- state (typed arrays of the reference)
+    def field_len(self, f):
+        nx, ny = self.nx, self.ny
+        if f in (F_U, F_UPESO, F_UVAL):
+            return (nx + 1) * ny
+        if f in (F_V, F_VPESO, F_VVAL):
+            return nx * (ny + 1)
+        if f == F_COR:
+            return 3 * nx * ny
+        if f >= F_PBOLHA:
+            return MAX_BOLHAS
+        return nx * ny
+
+    def read(self, f):
+        out = np.zeros(self.field_len(f), FIELD_DTYPE.get(f, np.float32))
+        _chk(load().tq_read_field(self._h, f, out.ctypes.data, out.nbytes))
+        return out
+
+    def write(self, f, arr):
+        arr = np.ascontiguousarray(arr, FIELD_DTYPE.get(f, np.float32))
+        assert arr.size == self.field_len(f)
+        _chk(load().tq_write_field(self._h, f, arr.ctypes.data, arr.nbytes))
+
+    def write_particles(self, P):
+        P = np.ascontiguousarray(P, np.float32).reshape(-1)
+        _chk(load().tq_write_particles(self._h, P.ctypes.data, P.size // CAMPOS))
+
+    def set_primeira(self, v):
+        _chk(load().tq_set_tracker_first(self._h, 1 if v else 0))
+
+    def fase(self, ph, dt=1 / 240):
+        _chk(load().tq_run_phase(self._h, ph, dt))
+
+    @property
+    def tipo(self):
+        return self.read(F_TIPO)
+
+    @property
+    def rotulo(self):
+        return self.read(F_ROT)
+
+    @property
+    def contagemCelula(self):
+        return self.read(F_CONT)
+
+    @property
+    def solidoCelula(self):
+        return self.read(F_SOLIDO)
+
+    def solver_stats(self, true_residual=False):
+        st = SolverStats()
+        _chk(load().tq_solver_stats_get(self._h, C.byref(st), 1 if true_residual else 0))
+        return {k: getattr(st, k) for k, _ in SolverStats._fields_}
+
+    def timing_enable(self, on=True, only=None):
+        _chk(load().tq_timing_enable(self._h, 1 if on else 0, only.encode() if only else None))
+
+    def timing_read(self):
+        n = C.c_int()
+        _chk(load().tq_timing_read(self._h, None, 0, C.byref(n)))
+        arr = (KernelTime * max(n.value, 1))()
+        _chk(load().tq_timing_read(self._h, arr, n.value, C.byref(n)))
+        return {arr[k].name.decode(): {"ms": arr[k].total_ms, "launches": arr[k].launches,
+                                       "bytes_per_launch": arr[k].bytes_per_launch} for k in range(n.value)}
+
+    def sync(self):
+        _chk(load().tq_sync(self._h))
+
+
+def rodar(sim, segundos, a_cada=None, quadro=1 / 60):
+    """The reference tests' adaptive-substep driver (testes/fisica.test.js:12-19)."""
+    import math
+    quadros = int(math.floor(segundos / quadro + 0.5))
+    for f in range(quadros):
+        sub = math.ceil(quadro / passo_maximo(sim.h, sim.cena.altura, sim.vMax, quadro))
+        sim.avancar(quadro, sub)
+        if a_cada:
+            a_cada(f * quadro)
