@@ -130,6 +130,8 @@ typedef struct tq_solver_stats {
   double true_resid;       /* ||b - A[q;y]|| recomputed (only when requested) */
   int levels;              /* multigrid levels */
   int active_bubbles;
+  long long total_iterations; /* PCG iterations since creation */
+  long long solves;           /* pressure solves since creation */
 } tq_solver_stats;
 int tq_solver_stats_get(tq_sim* s, tq_solver_stats* st, int compute_true_residual);
 

@@ -582,7 +582,7 @@ struct Solver {
   std::vector<T> r, z, zVelho, p, ap;
   std::vector<double> normas;
   int iters_done = 0;
-  double norm_b = 0, true_resid = 0;
+  double norm_b = 0, true_resid = 0, proj_mean = 0;
   void init(int nx, int ny) {
     niveis = montar_niveis<T>(nx, ny);
     op = Nivel<T>();
@@ -671,13 +671,46 @@ static void resolver_pcg(Solver<T>& S, const uint8_t* tipo, const T* div, T* q, 
   aplicar_bolhas(q, yb, aq);
   for (int b : ativos) rb.set(b, (double)rhsB[b] - aq.get(b));
   auto norma = [&]() { return std::sqrt(produto(r, r, tipo) + produto_b(rb, rb)); };
+  // Residual-terminated mode only (an addition): when no water cell touches Dirichlet air
+  // (atmosphere, vacuum or an untracked pocket), the operator is singular with the constant
+  // over all unknowns as null vector; its component is removed from b so the tolerance is
+  // reachable.  S.proj_mean records it.
+  S.proj_mean = 0;
+  if (tol > 0) {
+    bool anchored = false;
+    double sb = 0, nunk = 0;
+    for (int j = 0; j < ny && !anchored; j++)
+      for (int i = 0; i < nx; i++) {
+        int c = i + j * nx;
+        if (tipo[c] != AGUA) continue;
+        int m = op.viz[c];
+        const int nb[4] = {c - 1, c + 1, c - nx, c + nx};
+        for (int q2 = 0; q2 < 4; q2++) {
+          if (!(m & (1 << q2))) continue;
+          int d = nb[q2];
+          if (tipo[d] != AR) continue;
+          bool bub = rot && rot[d] >= 2 && faces[rot[d]] > 0;
+          if (!bub) anchored = true;
+        }
+      }
+    if (!anchored) {
+      for (int c = 0; c < k; c++) if (tipo[c] == AGUA) { sb += -(double)div[c]; nunk += 1; }
+      for (int b : ativos) { sb += (double)rhsB[b]; nunk += 1; }
+      if (nunk > 0) {
+        double mean = sb / nunk;
+        S.proj_mean = mean;
+        for (int c = 0; c < k; c++) if (tipo[c] == AGUA) r[c] = (T)((double)r[c] - mean);
+        for (int b : ativos) rb.set(b, rb.get(b) - mean);
+      }
+    }
+  }
   S.normas.clear();
   S.normas.push_back(norma());
-  // ||b|| with b = [-div ; rhs] (reported and used by the tolerance mode)
+  // ||b|| with b = [-div ; rhs] (projected as above; reported and used by the tolerance mode)
   {
     double s = 0;
-    for (int c = 0; c < k; c++) if (tipo[c] == AGUA) s += (double)div[c] * (double)div[c];
-    for (int b : ativos) s += (double)rhsB[b] * (double)rhsB[b];
+    for (int c = 0; c < k; c++) if (tipo[c] == AGUA) { double e = -(double)div[c] - S.proj_mean; s += e * e; }
+    for (int b : ativos) { double e = (double)rhsB[b] - S.proj_mean; s += e * e; }
     S.norm_b = std::sqrt(s);
   }
   std::vector<int> celulasDe(MAX_BOLHAS, 0);
@@ -758,12 +791,12 @@ static void resolver_pcg(Solver<T>& S, const uint8_t* tipo, const T* div, T* q, 
       for (int i = 0; i < nx; i++) {
         int c = i + j * nx;
         if (tipo[c] != AGUA) continue;
-        double e = -(double)div[c] - aplicar(op, qq.data(), i, j);
+        double e = (-(double)div[c] - S.proj_mean) - aplicar(op, qq.data(), i, j);
         s += e * e;
       }
     BMap aq2;
     aplicar_bolhas(qq.data(), yb, aq2);
-    for (int b : ativos) { double e = (double)rhsB[b] - aq2.get(b); s += e * e; }
+    for (int b : ativos) { double e = ((double)rhsB[b] - S.proj_mean) - aq2.get(b); s += e * e; }
     S.true_resid = std::sqrt(s);
   }
   for (int c = 0; c < k; c++) if (tipo[c] != AGUA) q[c] = 0;

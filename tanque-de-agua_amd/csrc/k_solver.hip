@@ -230,6 +230,7 @@ __global__ __launch_bounds__(256) void k_pcg_init(const uint8_t* __restrict__ op
     sc->norm_last = sc->norm0;
     sc->normb = sqrt(bb);
     sc->iters = 0;
+    sc->solves += 1;
     sc->rz = 0;
     sc->beta = 0;
     sc->alpha = 0;
@@ -579,6 +580,7 @@ __global__ __launch_bounds__(256) void k_update(const uint8_t* __restrict__ opMa
     sc->norm2 = n2;
     sc->norm_last = nr;
     sc->iters += 1;
+    sc->total_iters += 1;
     bool stop = sc->iters >= sc->maxit;
     if (sc->tol > 0 && (nr == 0 || nr <= sc->tol * sc->normb)) stop = true;
     sc->done = stop ? 1 : 0;

@@ -594,6 +594,8 @@ int tq_solver_stats_get(tq_sim* s, tq_solver_stats* st, int compute_true) {
   st->norm_b = h.normb;
   st->levels = (int)s->lv.size();
   st->active_bubbles = h.active_bubbles;
+  st->total_iterations = (long long)h.total_iters;
+  st->solves = (long long)h.solves;
   st->true_resid = -1;
   if (compute_true) {
     double tr = 0;

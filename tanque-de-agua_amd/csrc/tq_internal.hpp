@@ -68,6 +68,8 @@ struct DevScalars {
   uint32_t ticket[16];  // last-block counters of the reductions
   int done;             // PCG converged / stopped
   int iters;            // PCG iterations run
+  unsigned long long total_iters;  // PCG iterations since creation
+  unsigned long long solves;       // pressure solves since creation
   int primeira;         // tracker first snapshot flag
   int active_bubbles;
   double rz, pap, alpha, beta, norm2, norm0, normb, norm_last;

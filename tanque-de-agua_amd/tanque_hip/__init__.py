@@ -42,7 +42,8 @@ class Tool(C.Structure):
 
 class SolverStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("norm0", C.c_double), ("norm", C.c_double), ("norm_b", C.c_double),
-                ("true_resid", C.c_double), ("levels", C.c_int), ("active_bubbles", C.c_int)]
+                ("true_resid", C.c_double), ("levels", C.c_int), ("active_bubbles", C.c_int),
+                ("total_iterations", C.c_longlong), ("solves", C.c_longlong)]
 
 
 class KernelTime(C.Structure):

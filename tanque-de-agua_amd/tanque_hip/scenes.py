@@ -1,0 +1,91 @@
+"""Synthetic workloads of the benchmark configurations, built as capsule strokes (the
+reference's Cena semantics) and seeded particle rectangles.
+
+Each builder returns a `Workload` with the grid, the strokes (metres), the particle list (the
+reference's 12-float records, in insertion order), the frame dt and a description; the same
+strokes and particles drive the HIP path and the CPU baseline, so both see identical inputs.
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import Cena, Semente, semear_retangulo
+
+TANK_W, TANK_H = 1.42, 0.80  # metres
+FRAME = 2                    # frame thickness in cells (MOLDURA)
+
+
+@dataclass
+class Workload:
+    name: str
+    nx: int
+    ny: int
+    h: float
+    strokes: list
+    particles: np.ndarray
+    dt_frame: float
+    frames: int
+    description: str
+    extra: dict = field(default_factory=dict)
+
+    def cena(self):
+        c = Cena(self.nx, self.ny, self.h)
+        for s in self.strokes:
+            c.tracar(*s)
+        return c
+
+
+def dam(seed=0):
+    """Plain dam break: 160x90 grid, 8.9 mm cells, empty tank (2-cell frame, no lid); a water
+    block x < 40 %, y < 50 % of the tank at 4 particles/cell (seeded jitter); 240 frames at
+    1/240 s."""
+    nx, ny = 160, 90
+    h = TANK_H / ny
+    W, H = nx * h, ny * h
+    m = FRAME * h
+    P = semear_retangulo(m, m, 0.4 * W, 0.5 * H, h, (0, 0, 0), Semente(seed))
+    return Workload("dam", nx, ny, h, [], P, 1 / 240, 240,
+                    "dam break 160x90 h=%.4g m, block 40%%x50%% at 4 ppc" % h)
+
+
+def hourglass_strokes(nx, ny, h, neck_cells=10, half_height=0.12, radius_cells=3):
+    """Two solid wedges from the side walls to the middle, leaving `neck_cells` between their
+    tips at mid-height: each wedge is a fan of capsules from the wall to its tip."""
+    W, H = nx * h, ny * h
+    r = radius_cells * h
+    tip_l = W / 2 - (neck_cells / 2) * h - r
+    tip_r = W / 2 + (neck_cells / 2) * h + r
+    n = int(math.ceil(2 * half_height / r))
+    strokes = []
+    for k in range(n + 1):
+        y = H / 2 - half_height + k * (2 * half_height / n)
+        strokes.append((0.0, y, tip_l, H / 2, r, False))
+        strokes.append((W, y, tip_r, H / 2, r, False))
+    return strokes
+
+
+def ultra_hourglass(seed=0, nx=896, ny=504):
+    """Ultra ~450k cells: 896x504 (1.59 mm cells), sealed hourglass -- two solid wedges leaving a
+    10-cell neck, closed by the tank's top frame; the upper chamber full of water at 4 ppc, the
+    lower chamber one enclosed air pocket (a bubble unknown after the first air update);
+    frames of 1/480 s."""
+    h = TANK_H / ny
+    W, H = nx * h, ny * h
+    strokes = hourglass_strokes(nx, ny, h)
+    m = FRAME * h
+    P = semear_retangulo(m, H / 2, W - m, H - m, h, (0, 0, 0), Semente(seed))
+    return Workload("ultra", nx, ny, h, strokes, P, 1 / 480, 480,
+                    "sealed hourglass %dx%d h=%.4g m, 10-cell neck, upper chamber full at 4 ppc" % (nx, ny, h))
+
+
+def scaled_hourglass(nx, ny, seed=0):
+    """The Ultra scene at another resolution (size sweeps past the 256 MB last-level cache)."""
+    w = ultra_hourglass(seed, nx, ny)
+    w.name = "hourglass%dx%d" % (nx, ny)
+    return w
+
+
+def passo_frame(h, altura, vmax, dt_frame):
+    from . import passo_maximo
+    return int(math.ceil(dt_frame / passo_maximo(h, altura, vmax, dt_frame)))

@@ -26,6 +26,23 @@ def clone_state(osim, gsim):
     gsim.set_primeira(bool(osim_primeira(osim)))
 
 
+def osim_snapshot(osim):
+    """The oracle's complete state as arrays (particles + every field + tracker flag)."""
+    return {"P": osim.particulas().copy(), "fields": {f: osim.read(f) for f in STATE_FIELDS},
+            "primeira": osim_primeira(osim), "cena": osim.cena,
+            "opts": getattr(osim, "_opts", None)}
+
+
+def oracle_fp64_clone(orc, snap, **opts):
+    """An FP64-storage oracle holding the snapshot's state (binary32 values are exact in FP64)."""
+    o = orc.Simulacao(snap["cena"], mode=1, **opts)
+    o.write(orc.F_PART, snap["P"])
+    for f, arr in snap["fields"].items():
+        o.write(f, arr)
+    o.set_primeira(bool(snap["primeira"]))
+    return o
+
+
 def osim_primeira(osim):
     from oracle import oracle as orc
     return orc.lib().or_sim_tracker_first(osim._p)

@@ -12,13 +12,15 @@ phase runs on both, and the outputs are compared:
 import numpy as np
 import pytest
 
-from gpu_common import (clone_state, cmp_close, cmp_exact, copo, dam, hip_sim_like, log, substep_dt, tanque_com_bolha,
-                        ampulheta)
+from gpu_common import (ampulheta, clone_state, cmp_close, cmp_exact, copo, dam, hip_sim_like, log,
+                        oracle_fp64_clone, osim_snapshot, rel_err, substep_dt, tanque_com_bolha)
 
 pytestmark = pytest.mark.gpu
 
 # tolerances (relative to the field's max magnitude)
-RTOL_P2G = 2e-6      # u, v, weights, density, dye: exact sums vs binary32 running sums
+RTOL_P2G = 1e-4      # u, v, weights, density, dye vs JS-exact: the reference's binary32 running
+                     # sums cancel in some faces (seen: 1.3e-5 in the hourglass)
+RTOL_P2G_FP64 = 3e-7  # the same fields vs the FP64 oracle: only the final binary32 rounding
 RTOL_RHSB = 1e-5     # bubble rhs: FP64 vs binary32 accumulation
 RTOL_SOLVE = 2e-3    # q after 3 warm-started PCG iterations (FP32 V-cycle rounding)
 
@@ -69,6 +71,7 @@ def test_phase_by_phase(oracle, tq, name):
             bad.append(nm)
     # P2G + gravity/boundary + types
     log(" phase 0-2 P2G")
+    osim_before = osim_snapshot(osim)
     run([0, 1, 2], 0)
     for f, nm in ((7, "contagem"), (9, "tipo")):
         if cmp_exact(nm, gsim.read(f), osim.read(f)):
@@ -76,6 +79,16 @@ def test_phase_by_phase(oracle, tq, name):
     for f, nm in ((0, "u"), (1, "v"), (2, "uPeso"), (3, "vPeso"), (6, "densidade"), (8, "corCelula")):
         if not cmp_close(nm, gsim.read(f), osim.read(f), RTOL_P2G):
             bad.append(nm)
+    # the same phase on the FP64 oracle from the same state: the GPU's error vs FP64
+    o64 = oracle_fp64_clone(oracle, osim_before)
+    for ph in (0, 1, 2):
+        o64.fase(ph, dt)
+    for f, nm in ((0, "u"), (1, "v"), (2, "uPeso"), (3, "vPeso"), (6, "densidade")):
+        e_g = rel_err(gsim.read(f), o64.read(f, True))
+        e_o = rel_err(osim.read(f), o64.read(f, True))
+        log("  vs FP64 %-10s gpu %.2e  js-exact %.2e" % (nm, e_g, e_o))
+        if e_g > RTOL_P2G_FP64:
+            bad.append(nm + "(fp64)")
     log(" phase 3 labels")
     run([3], 3)
     if cmp_exact("rotulo", gsim.read(18), osim.read(18)):
