@@ -21,7 +21,7 @@ sys.path.insert(0, os.path.join(ROOT, "tanque-de-agua_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-DOMINANT_KERNEL = "mg_smooth_l0"
+DOMINANT_KERNEL = "mg_up_l0"
 
 
 def log(*a):

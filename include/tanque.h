@@ -46,6 +46,9 @@ typedef struct tq_desc {
   int max_particles;      /* device capacity; 0 = ceil(4.6 * nx * ny) as MotorCPU.maxParticulas */
   int precision;          /* TQ_PREC_FP32 or TQ_PREC_MIXED */
   double tol;             /* 0: fixed `iteracoes` (reference); >0: stop at ||r|| <= tol*||b|| */
+  int mg_min_lds_level;   /* multigrid levels >= this (and small enough) run in one workgroup
+                             from LDS; 0 = as many as fit (set higher to keep levels on the
+                             multi-workgroup kernels) */
 } tq_desc;
 
 /* Fills `d` with the reference defaults (constantes.js) for an nx x ny grid of cell size h. */

@@ -266,7 +266,8 @@ __global__ __launch_bounds__(256) void k_bubble_rhs(const uint8_t* __restrict__ 
                                                     const float* __restrict__ u, const float* __restrict__ v,
                                                     const float* __restrict__ qAr, int nx, int ny,
                                                     uint32_t* __restrict__ volB, uint32_t* __restrict__ facesB,
-                                                    uint32_t* __restrict__ cellsB, double* __restrict__ rhsB) {
+                                                    uint32_t* __restrict__ cellsB, double* __restrict__ rhsB,
+                                                    int counts_only) {
 #pragma clang fp contract(off)
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t r = 0;
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(256) void k_bubble_rhs(const uint8_t* __restrict__ 
       if (bi) {
         atomicAdd(&volB[lr], (uint32_t)__popcll(bi));
         atomicAdd(&facesB[lr], (uint32_t)sf);
-        atomicAdd(&rhsB[lr], sr);
+        if (!counts_only) atomicAdd(&rhsB[lr], sr);
       }
     }
     if (m) pending = false;
@@ -332,11 +333,23 @@ int bubble_rhs(tq_sim* s, double dt) {
   TQ_HIP(hipMemsetAsync(s->rhsB, 0, sizeof(double) * MAX_BOLHAS, s->st));
   int t = time_begin(s, "bubble_rhs", (1 + 4 + 4 + 4 + 4) * (double)s->C);
   hipLaunchKernelGGL(k_bubble_rhs, dim3((s->C + 255) / 256), dim3(256), 0, s->st, s->tipo, s->rot, s->u, s->v, s->qAr,
-                     s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB);
+                     s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB, 0);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_bubble_finish, dim3(1), dim3(1024), 0, s->st, s->volB, s->rhsB, s->rhsB32, s->yB32, s->pBolha,
                      s->volAlvo, s->h, dt);
+  TQ_LAUNCH_CHECK();
+  return 0;
+}
+
+// Per-bubble cell and face counts only (the rhs is kept): lets a solve start from a state whose
+// bubble sums were computed elsewhere.
+int bubble_counts(tq_sim* s) {
+  TQ_HIP(hipMemsetAsync(s->volB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
+  TQ_HIP(hipMemsetAsync(s->facesB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
+  TQ_HIP(hipMemsetAsync(s->cellsB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
+  hipLaunchKernelGGL(k_bubble_rhs, dim3((s->C + 255) / 256), dim3(256), 0, s->st, s->tipo, s->rot, s->u, s->v, s->qAr,
+                     s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB, 1);
   TQ_LAUNCH_CHECK();
   return 0;
 }

@@ -572,7 +572,11 @@ int tq_run_phase(tq_sim* s, int ph, double dt) {
     case 5: e = air_pressure_and_div(s, dt, 1); break;
     case 6: e = air_pressure_and_div(s, dt, 2); break;
     case 7: e = bubble_rhs(s, dt); break;
-    case 8: e = pressure_solve(s); if (!e) e = pressure_post(s, dt); break;
+    case 8:
+      if (s->d.ar) e = bubble_counts(s);
+      if (!e) e = pressure_solve(s);
+      if (!e) e = pressure_post(s, dt);
+      break;
     case 9: e = gradient(s); break;
     case 10: e = extrapolate(s); break;
     case 11: e = g2p(s, dt); break;

@@ -161,7 +161,7 @@ __device__ inline void count_cell(float x, float y, double inv, int nx, int ny, 
 }
 
 __device__ inline long long to_fx(double v) {
-  // fixed-point with 2^-32 resolution; non-finite or absurd contributions saturate
+  // fixed-point with 2^-46 resolution; non-finite or absurd contributions saturate
   double s = v * FX_SCALE;
   if (!(s > -9.0e18)) s = s != s ? 0.0 : -9.0e18;
   if (s > 9.0e18) s = 9.0e18;

@@ -44,10 +44,14 @@ constexpr int MIN_BOLHA_CELULAS = 4;
 
 constexpr int TILE = 16;                 // P2G binning tile and the MG smoother block (pressao-cpu.js:17)
 constexpr uint32_t DEAD_KEY = 0xFFFFFFFFu;
-constexpr double FX_SCALE = 4294967296.0;  // 2^32: fixed-point scale of the P2G sums
-constexpr double FX_INV = 1.0 / 4294967296.0;
+// Fixed-point scale of the P2G sums: 2^46.  A particle far from a node can give it a weight of
+// ~1e-11 (binary32 positions quantise the fractional offset to ~1e-5 cells), which must stay
+// non-zero and keep its relative precision; |sum w v| stays below 2^63/2^46 ~ 1.3e5 m/s for
+// the speeds G2P allows (3.2 h/dt per particle, tens of particles per face).
+constexpr double FX_SCALE = 70368744177664.0;  // 2^46
+constexpr double FX_INV = 1.0 / 70368744177664.0;
 constexpr int RED_BLOCK = 256;           // block size of reduction kernels
-constexpr int MAX_RED_BLOCKS = 8192;     // partial slots per reduction
+constexpr int MAX_RED_BLOCKS = 65536;    // partial slots per reduction (one per block)
 
 // mask byte of a multigrid cell / operator cell
 //   bits 0-3: free neighbours (1 left, 2 right, 4 down, 8 up) as prepararVizinhos
@@ -86,6 +90,7 @@ struct Level {
   void* b = nullptr;
   void* x = nullptr;
   void* tmp = nullptr;
+  bool hx = false, hb = false;  // FP16 storage of x/tmp and of b on this level (this solve)
 };
 
 struct KTime {
@@ -174,7 +179,10 @@ struct tq_sim {
   float* zbuf[2] = {};
   float* pbuf[2] = {};
   int zc = 0, pc = 0;
-  double* partials = nullptr;   // MAX_RED_BLOCKS * 4
+  double* partials = nullptr;   // per-block partial sums: 8 x MAX_RED_BLOCKS
+  double* bpart = nullptr;      // per-block partial sums of the slotted bubbles: 8 x MAX_RED_BLOCKS
+  int* labelSlot = nullptr;     // bubble label -> reduction slot (-1: FP64 atomics)
+  int lc = 0;                   // first multigrid level solved inside the one-workgroup kernel
   tq::DevScalars* sc = nullptr;
   tq::DevScalars* sc_host = nullptr;  // pinned mirror
 
@@ -228,6 +236,7 @@ int surface(tq_sim* s);
 int air_pressure_and_div(tq_sim* s, double dt, int which);  // which: 1 qAr, 2 div, 3 both
 int propagate_labels(tq_sim* s);
 int bubble_rhs(tq_sim* s, double dt);
+int bubble_counts(tq_sim* s);
 int gradient(tq_sim* s);
 int extrapolate(tq_sim* s);
 int pressure_post(tq_sim* s, double dt);

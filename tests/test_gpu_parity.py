@@ -37,6 +37,7 @@ SCENARIOS = {
     "copo": (lambda orc: copo(orc), 30),
     "bolha": (lambda orc: tanque_com_bolha(orc), 6),
     "ampulheta": (lambda orc: ampulheta(orc), 20),
+    "dam160": (lambda orc: dam(orc, nx=160, ny=90, h=0.8 / 90, seed=0), 12),
 }
 
 
@@ -47,12 +48,15 @@ def advance(orc, osim, frames):
         osim.avancar(1 / 60, sub)
 
 
+@pytest.mark.parametrize("lds", [0, 99], ids=["mg-lds", "mg-multiblock"])
 @pytest.mark.parametrize("name", list(SCENARIOS))
-def test_phase_by_phase(oracle, tq, name):
+def test_phase_by_phase(oracle, tq, name, lds):
+    """lds=0: the small grids' V-cycle runs in one workgroup; lds=99 keeps every level but the
+    coarsest on the multi-workgroup kernels."""
     build, frames = SCENARIOS[name]
     osim = build(oracle)
     advance(oracle, osim, frames)
-    gsim = hip_sim_like(tq, osim)
+    gsim = hip_sim_like(tq, osim, mg_min_lds_level=lds)
     dt = substep_dt(oracle, osim)
     log("\n[%s] n=%d dt=%.6f" % (name, osim.n, dt))
     bad = []

@@ -40,6 +40,7 @@ def prepared(orc, build, frames):
     ("dam", lambda o: dam(o), 15),
     ("bolha", lambda o: tanque_com_bolha(o), 6),
     ("ampulheta", lambda o: ampulheta(o), 20),
+    ("dam160", lambda o: dam(o, nx=160, ny=90, h=0.8 / 90, seed=0), 12),
 ])
 def test_solve_to_tolerance(oracle, tq, name, build, frames):
     osim = prepared(oracle, build, frames)
