@@ -46,9 +46,13 @@ typedef struct tq_desc {
   int max_particles;      /* device capacity; 0 = ceil(4.6 * nx * ny) as MotorCPU.maxParticulas */
   int precision;          /* TQ_PREC_FP32 or TQ_PREC_MIXED */
   double tol;             /* 0: fixed `iteracoes` (reference); >0: stop at ||r|| <= tol*||b|| */
-  int mg_min_lds_level;   /* multigrid levels >= this (and small enough) run in one workgroup
-                             from LDS; 0 = as many as fit (set higher to keep levels on the
-                             multi-workgroup kernels) */
+  int mg_min_lds_level;   /* multigrid levels >= this run in one workgroup from LDS; 0 = the
+                             first level of at most 8192 cells whose suffix fits (set higher
+                             to keep levels on the multi-workgroup kernels) */
+  int mg_half_level;      /* TQ_PREC_MIXED: first level whose multigrid vectors are stored in
+                             FP16 (level 0 keeps x in FP32 and stores only b in FP16); levels
+                             under 4096 cells and those in LDS stay FP32; -1 = default (1) */
+  int use_graph;          /* 1: PCG iterations are replayed from a captured HIP graph */
 } tq_desc;
 
 /* Fills `d` with the reference defaults (constantes.js) for an nx x ny grid of cell size h. */

@@ -261,7 +261,15 @@ int propagate_labels(tq_sim* s) {
 // ================================================================== #ladoDireitoDasBolhas (fisica-cpu.js:429-454)
 // Per-bubble sums: interior air cells of a bubble add their count, their water faces and the
 // inflow (u - qAr) of those faces.  Lanes of a wavefront that share a label are reduced first,
-// then one atomic per label (FP64 for the inflow sum).
+// then one atomic per label.  The inflow is summed in 64-bit fixed point (2^-40 m/s steps), so
+// the total does not depend on the order of the atomics.
+constexpr double RHS_FX = 1099511627776.0;  // 2^40
+
+__device__ inline long long wave_sum_i64(long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 __global__ __launch_bounds__(256) void k_bubble_rhs(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
                                                     const float* __restrict__ u, const float* __restrict__ v,
                                                     const float* __restrict__ qAr, int nx, int ny,
@@ -295,7 +303,7 @@ __global__ __launch_bounds__(256) void k_bubble_rhs(const uint8_t* __restrict__ 
     const int leader = __ffsll((long long)b) - 1;
     const uint32_t lr = __shfl(r, leader, 64);
     const bool m = pending && r == lr;
-    double sr = wave_sum(m ? rhs : 0.0);
+    const long long sr = wave_sum_i64(m ? (long long)__builtin_rint(rhs * RHS_FX) : 0LL);
     unsigned long long bm = __ballot(m);
     unsigned long long bi = __ballot(m && interior);
     int sf = (int)wave_sum(m ? (double)faces : 0.0);
@@ -304,7 +312,7 @@ __global__ __launch_bounds__(256) void k_bubble_rhs(const uint8_t* __restrict__ 
       if (bi) {
         atomicAdd(&volB[lr], (uint32_t)__popcll(bi));
         atomicAdd(&facesB[lr], (uint32_t)sf);
-        if (!counts_only) atomicAdd(&rhsB[lr], sr);
+        if (!counts_only) atomicAdd((unsigned long long*)&rhsB[lr], (unsigned long long)sr);
       }
     }
     if (m) pending = false;
@@ -318,8 +326,11 @@ __global__ __launch_bounds__(1024) void k_bubble_finish(const uint32_t* __restri
 #pragma clang fp contract(off)
   const double paraQ = dt / (DENSIDADE_AGUA * h);
   for (int r = threadIdx.x; r < MAX_BOLHAS; r += blockDim.x) {
-    if (r < 2 || !volB[r]) { rhsB32[r] = (float)rhsB[r]; continue; }
-    double x = rhsB[r] + VOLTA_VOLUME_BOLHA * ((double)volAlvo[r] - (double)volB[r]) * h / dt;
+    // the fixed-point sum of k_bubble_rhs becomes the binary64 rhs
+    const double sum = (double)__double_as_longlong(rhsB[r]) / RHS_FX;
+    rhsB[r] = sum;
+    if (r < 2 || !volB[r]) { rhsB32[r] = (float)sum; continue; }
+    double x = sum + VOLTA_VOLUME_BOLHA * ((double)volAlvo[r] - (double)volB[r]) * h / dt;
     rhsB[r] = x;
     rhsB32[r] = (float)x;
     yB32[r] = (float)((double)pBolha[r] * paraQ);

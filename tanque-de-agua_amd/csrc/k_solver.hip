@@ -3,22 +3,27 @@
 //
 // CG vectors r, z, p, Ap are FP32 per cell; every dot product, the residual norm and the
 // bubble unknowns (y, r, z, p, Ap per label) are FP64.  Reductions are deterministic and free
-// of same-address atomics: grid-stride kernels store one partial per block (and one per block
-// for each of up to BSLOTS active bubbles), and a one-block "fin" kernel sums them in block
-// order and does the scalar/bubble work.  Bubbles beyond BSLOTS fall back to FP64 atomics.
+// of same-address atomics: each block stores one partial per quantity (and one per slotted
+// bubble), and a one-block "fin" kernel sums them in block order and does the scalar and
+// bubble work.  Bubbles beyond BSLOTS fall back to FP64 atomics.
 //
-// The V-cycle is launch-lean: per large level one kernel for both down-smoothing passes (the
-// half-block-shifted second pass recomputes the four first-pass tiles it overlaps), one
-// LDS-tiled restriction, and one up kernel (prolongation folded into the snapshot load, both
-// passes); the coarse levels that fit in LDS are solved by ONE workgroup; level 0's up kernel
-// also produces z and the r.z dots.  Mixed mode stores b in FP16 on level 0 and x, b in FP16
-// on the other large levels (per-cycle scaled by 1/||r||), with FP32 arithmetic.
+// Multigrid.  The reference smoother (suavizar, pressao-cpu.js:99-150) runs 16x16 tiles that
+// each read a snapshot of x (their halo stays at the snapshot) through 4 red-black sweeps.  One
+// wavefront owns one tile: each lane holds four cells in registers (columns 2p, 2p+1 of rows
+// r, r+8), so every lane updates one cell per row in every half-sweep; neighbours come from
+// the wave's private 18x18 LDS tile, synchronised by an LDS wait (no barrier).  Launches per
+// large level: down = [restriction from the finer level + first pass from x = 0] and the
+// second pass; up = [prolongation folded into the snapshot + first pass] and the second pass.
+// Level 0's first down pass also applies the PCG update (q += alpha p, r -= alpha Ap) and
+// builds the V-cycle input; its last up pass writes z and the r.z dots.  The small levels run
+// in one workgroup from LDS.  Buffer roles per level: pass A writes x, pass B x -> tmp, the
+// restriction reads tmp, prolongation+C tmp -> x, pass D x -> tmp (the level's final x).
 //
 // Kernel sequence of one PCG iteration:
 //   spmv (p = z + beta p fused, Ap, p.Ap) -> fin_spmv (alpha, bubble y, r)
-//   update (q, r, |r|^2) -> fin_update (norm, stop test)
-//   mg_input, [down_l, restrict_l] for l < lc, coarse_wg (levels >= lc), up_l for l = lc-1..0
-//   (up_0 writes z and the dots) -> fin_zdots (beta)
+//   in_a0 (update, |r|^2, V-cycle input, pass A on level 0) -> fin_update (norm, stop test)
+//   pass B_0, [restrict+A_l, B_l] for 0 < l < lc, restrict+A_lc, coarse_wg (levels >= lc),
+//   [prolong+C_l, D_l] for l = lc-1..0 (D_0 writes z and the dots) -> fin_zdots (beta)
 // Every kernel returns at once when the solve is done.
 #include "tq_device.hpp"
 
@@ -27,10 +32,14 @@ namespace tq {
 enum { BV_Y = 0, BV_R, BV_Z, BV_ZV, BV_P, BV_AP, BV_ACC0, BV_ACC1, BV_N };
 __device__ __host__ inline double* bv(double* base, int k) { return base + (size_t)k * MAX_BOLHAS; }
 
-constexpr int NBR = 512;        // blocks of the grid-stride CG kernels
-constexpr int BSLOTS = 8;       // bubbles reduced through per-block slots
-constexpr int WG_LDS = 147456;  // LDS pool of the one-workgroup coarse kernel (144 KiB)
-constexpr int WG_BYTES_PER_CELL = 13;
+constexpr int BSLOTS = 8;          // bubbles reduced through per-block slots
+constexpr int TS = 18;             // smoother tile with its 1-cell halo
+constexpr int TCELLS = TS * TS;
+constexpr int WG_WAVES = 16;       // one-workgroup kernel: 1024 threads
+constexpr int WG_POOL = 139264;    // its LDS pool for the level arrays (x, tmp, b, mask)
+constexpr int WG_AUTO_CELLS = 2048;  // default: levels from the first one this small run in one workgroup
+constexpr int DENSE_MAX = 192;       // coarsest levels up to this many cells use the dense operator
+constexpr int FIN_THREADS = 512;
 
 template <class S> __device__ inline float ld(const S* p, size_t i);
 template <> __device__ inline float ld<float>(const float* p, size_t i) { return p[i]; }
@@ -38,56 +47,20 @@ template <> __device__ inline float ld<__half>(const __half* p, size_t i) { retu
 template <class S> __device__ inline void st(S* p, size_t i, float v);
 template <> __device__ inline void st<float>(float* p, size_t i, float v) { p[i] = v; }
 template <> __device__ inline void st<__half>(__half* p, size_t i, float v) { p[i] = __float2half(v); }
+// the value a store in S keeps
+template <class S> __device__ inline float rnd(float v);
+template <> __device__ inline float rnd<float>(float v) { return v; }
+template <> __device__ inline float rnd<__half>(float v) { return __half2float(__float2half(v)); }
 
 // all LDS operations of this wavefront are complete (and not reordered across by the compiler)
 __device__ inline void wave_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
-#define GRID_STRIDE(c, n) for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < (n); c += gridDim.x * blockDim.x)
+__device__ inline int nslots(int na) { return na < BSLOTS ? na : BSLOTS; }
 
 __device__ inline void slot_add(double (&acc)[BSLOTS], int slot, double val) {
 #pragma unroll
   for (int k = 0; k < BSLOTS; k++) acc[k] += (k == slot) ? val : 0.0;
 }
-
-template <int NV>
-__device__ inline void write_partials(double (&v)[NV], double* part, double* sh) {
-  block_sum<NV>(v, sh);
-  if (threadIdx.x == 0) {
-    const int bid = blockIdx.x + blockIdx.y * gridDim.x;
-#pragma unroll
-    for (int k = 0; k < NV; k++) part[(size_t)k * MAX_RED_BLOCKS + bid] = v[k];
-  }
-}
-
-// Sum of nb partials per quantity in a fixed assignment (deterministic); result in all threads.
-template <int NV>
-__device__ inline void read_partials(const double* part, int nb, double (&tot)[NV], double* sh, double* res) {
-  double acc[NV];
-#pragma unroll
-  for (int k = 0; k < NV; k++) acc[k] = 0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x)
-#pragma unroll
-    for (int k = 0; k < NV; k++) acc[k] += part[(size_t)k * MAX_RED_BLOCKS + b];
-  block_sum<NV>(acc, sh);
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int k = 0; k < NV; k++) res[k] = acc[k];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; k++) tot[k] = res[k];
-  __syncthreads();
-}
-
-__device__ inline bool is_bub_cell(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
-                                   const uint32_t* __restrict__ facesB, int ar, size_t c, uint32_t* lab) {
-  if (!ar || tipo[c] != AR) return false;
-  const uint32_t r = rot[c];
-  if (r < 2 || facesB[r] == 0) return false;
-  *lab = r;
-  return true;
-}
-
-__device__ inline bool interior(int i, int j, int nx, int ny) { return i >= 1 && j >= 1 && i < nx - 1 && j < ny - 1; }
 
 // a bubble contribution: through the block's slot accumulators, or an FP64 atomic when the
 // label has no slot
@@ -98,22 +71,79 @@ __device__ inline void bub_add(double (&sl)[BSLOTS], const int* __restrict__ lab
   else atomicAdd(&accG[lab], val);
 }
 
+// Block sum of the first nv of v[]; thread 0 stores partial `bid` of each quantity.  nv must
+// be uniform over the block; `sh` holds NMAX*16 doubles.
+template <int NMAX>
+__device__ inline void store_partials(double (&v)[NMAX], int nv, double* __restrict__ part, double* sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NMAX; k++)
+    if (k < nv) {
+      const double t = wave_sum(v[k]);
+      if (lane == 0) sh[k * 16 + w] = t;
+    }
+  __syncthreads();
+  if ((int)threadIdx.x < nv) {
+    double t = 0;
+    for (int q = 0; q < nw; q++) t += sh[threadIdx.x * 16 + q];
+    part[(size_t)threadIdx.x * MAX_RED_BLOCKS + blockIdx.x + blockIdx.y * gridDim.x] = t;
+  }
+  __syncthreads();  // `sh` is free again for the caller's next reduction
+}
+
+// Totals of nb partials of the first nv quantities, in a fixed order (deterministic); the
+// result is valid in every thread.
+template <int NMAX>
+__device__ inline void sum_partials(const double* __restrict__ part, int nb, int nv, double (&tot)[NMAX], double* sh,
+                                    double* res) {
+  double acc[NMAX];
+#pragma unroll
+  for (int k = 0; k < NMAX; k++) acc[k] = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < NMAX; k++)
+      if (k < nv) acc[k] += part[(size_t)k * MAX_RED_BLOCKS + b];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NMAX; k++)
+    if (k < nv) {
+      const double t = wave_sum(acc[k]);
+      if (lane == 0) sh[k * 16 + w] = t;
+    }
+  __syncthreads();
+  if ((int)threadIdx.x < nv) {
+    double t = 0;
+    for (int q = 0; q < nw; q++) t += sh[threadIdx.x * 16 + q];
+    res[threadIdx.x] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NMAX; k++) tot[k] = k < nv ? res[k] : 0.0;
+  __syncthreads();
+}
+
 // ================================================================== setup (types and neighbour masks)
-// level 0: operator mask (true types) and MG mask (bubble air as water, pressao-cpu.js:305-308)
-__global__ __launch_bounds__(256) void k_setup0(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot, int nx,
-                                                int ny, int ar, uint8_t* __restrict__ opMask, uint8_t* __restrict__ mg0) {
+// level 0: operator mask (true types, bubble bits) and MG mask (air with a bubble label as
+// water, pressao-cpu.js:305-308; colour bit)
+__global__ __launch_bounds__(256) void k_setup0(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
+                                                const uint32_t* __restrict__ facesB, int nx, int ny, int ar,
+                                                uint8_t* __restrict__ opMask, uint8_t* __restrict__ mg0) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nx * ny) return;
-  const int i = c % nx, j = c / nx;
+  const int j = c / nx, i = c - j * nx;
   int m = 0;
   if (i > 0 && tipo[c - 1] != SOLIDO) m |= 1;
   if (i < nx - 1 && tipo[c + 1] != SOLIDO) m |= 2;
   if (j > 0 && tipo[c - nx] != SOLIDO) m |= 4;
   if (j < ny - 1 && tipo[c + nx] != SOLIDO) m |= 8;
   const uint8_t t = tipo[c];
-  const uint8_t tm = (ar && t == AR && rot[c] >= 2) ? AGUA : t;
-  opMask[c] = (uint8_t)(m | (t << 4));
-  mg0[c] = (uint8_t)(m | (tm << 4));
+  const uint32_t lb = (ar && t == AR) ? rot[c] : 0;
+  const bool labelled = lb >= 2 && lb < (uint32_t)MAX_BOLHAS;
+  const bool bub = labelled && facesB[lb] > 0;
+  const bool inner = i >= 1 && j >= 1 && i < nx - 1 && j < ny - 1;
+  const uint8_t tm = labelled ? AGUA : t;
+  opMask[c] = (uint8_t)(m | (t << 4) | (bub ? MK_BUB : 0) | (bub && inner ? MK_BUBIN : 0));
+  mg0[c] = (uint8_t)(m | (tm << 4) | (((i + j + 1) & 1) ? MK_COLOUR : 0));
 }
 
 // tiposGrossos (pressao-cpu.js:63-81): air if any child is air, else water if any is water
@@ -121,7 +151,7 @@ __global__ __launch_bounds__(256) void k_coarse_type(const uint8_t* __restrict__
                                                      uint8_t* __restrict__ cm, int cnx, int cny) {
   const int C = blockIdx.x * blockDim.x + threadIdx.x;
   if (C >= cnx * cny) return;
-  const int I = C % cnx, J = C / cnx;
+  const int J = C / cnx, I = C - J * cnx;
   bool temAr = false, temAgua = false;
   for (int b = 0; b < 2; b++)
     for (int a = 0; a < 2; a++) {
@@ -133,17 +163,17 @@ __global__ __launch_bounds__(256) void k_coarse_type(const uint8_t* __restrict__
   cm[C] = (uint8_t)((temAr ? AR : temAgua ? AGUA : SOLIDO) << 4);
 }
 
-// prepararVizinhos (pressao-cpu.js:44-60) on the type bits of a level
+// prepararVizinhos (pressao-cpu.js:44-60) on the type bits of a level, plus the colour bit
 __global__ __launch_bounds__(256) void k_viz(uint8_t* __restrict__ m, int nx, int ny) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nx * ny) return;
-  const int i = c % nx, j = c / nx;
+  const int j = c / nx, i = c - j * nx;
   int v = 0;
   if (i > 0 && mask_type(m[c - 1]) != SOLIDO) v |= 1;
   if (i < nx - 1 && mask_type(m[c + 1]) != SOLIDO) v |= 2;
   if (j > 0 && mask_type(m[c - nx]) != SOLIDO) v |= 4;
   if (j < ny - 1 && mask_type(m[c + nx]) != SOLIDO) v |= 8;
-  m[c] = (uint8_t)((m[c] & 0xF0) | v);
+  m[c] = (uint8_t)((m[c] & 0x30) | v | (((i + j + 1) & 1) ? MK_COLOUR : 0));
 }
 
 // bubble vectors, the active list (labels with water faces, in label order) and their slots
@@ -182,71 +212,71 @@ __global__ __launch_bounds__(1024) void k_bub_init(const uint32_t* __restrict__ 
 
 // ================================================================== init: warm-start residual
 // q := 0 off water, bubble cells := y (difundir); r = -div - A q; bubble rows A_b [q;y].
-__global__ __launch_bounds__(256) void k_pcg_init(const uint8_t* __restrict__ opMask, const uint8_t* __restrict__ tipo,
-                                                  const uint32_t* __restrict__ rot, const uint32_t* __restrict__ facesB,
+__global__ __launch_bounds__(256) void k_pcg_init(const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
                                                   const float* __restrict__ div, float* __restrict__ q,
-                                                  float* __restrict__ r, float* __restrict__ pbuf0, int nx, int ny,
-                                                  int ar, double* __restrict__ bvec, const int* __restrict__ labelSlot,
+                                                  float* __restrict__ r, float* __restrict__ pbuf0, int nx, int C,
+                                                  double* __restrict__ bvec, const int* __restrict__ labelSlot,
                                                   double* __restrict__ partials, double* __restrict__ bpart,
                                                   DevScalars* __restrict__ sc) {
   __shared__ double sh[BSLOTS * 16];
-  const int C = nx * ny;
-  const int na = sc->active_bubbles;
+  const int ns = nslots(sc->active_bubbles);
   double v[5] = {0, 0, 0, 0, 0};  // r.r, div.div, sum(b), n_water, anchored
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
   const double* yb = bv(bvec, BV_Y);
   double* acc0 = bv(bvec, BV_ACC0);
-  auto qe = [&](size_t d) -> float {
-    const uint8_t t = tipo[d];
-    if (t == AGUA) return q[d];
-    uint32_t lb;
-    if (is_bub_cell(tipo, rot, facesB, ar, d, &lb)) return (float)yb[lb];
+  auto qe = [&](int d, uint8_t md) -> float {
+    if (mask_type(md) == AGUA) return q[d];
+    if (md & MK_BUB) return (float)yb[rot[d]];
     return 0.f;
   };
-  GRID_STRIDE(c, C) {
-    pbuf0[c] = 0.f;
-    const float qc = qe(c);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < C) {
     const uint8_t m = opMask[c];
+    const float qc = qe(c, m);
+    pbuf0[c] = 0.f;
     if (mask_type(m) == AGUA) {
       double s = 0;
       bool anchored = false;
-      const size_t nb[4] = {(size_t)c - 1, (size_t)c + 1, (size_t)c - nx, (size_t)c + nx};
+      const int nb[4] = {c - 1, c + 1, c - nx, c + nx};
       for (int k = 0; k < 4; k++) {
         if (!(m & (1 << k))) continue;
-        s += (double)qe(nb[k]);
-        if (tipo[nb[k]] == AR) {
-          uint32_t lb;
-          if (is_bub_cell(tipo, rot, facesB, ar, nb[k], &lb)) {
-            if (interior((int)(nb[k] % nx), (int)(nb[k] / nx), nx, ny)) bub_add(sl, labelSlot, acc0, lb, -(double)qc);
+        const uint8_t md = opMask[nb[k]];
+        s += (double)qe(nb[k], md);
+        if (mask_type(md) == AR) {
+          if (md & MK_BUB) {
+            if (md & MK_BUBIN) bub_add(sl, labelSlot, acc0, rot[nb[k]], -(double)qc);
           } else anchored = true;
         }
       }
       const double aq = mask_cnt(m) * (double)qc - s;
       const float rr = (float)(-(double)div[c] - aq);
       r[c] = rr;
-      v[0] += (double)rr * rr;
-      v[1] += (double)div[c] * div[c];
-      v[2] += -(double)div[c];
-      v[3] += 1;
-      v[4] += anchored ? 1 : 0;
+      v[0] = (double)rr * rr;
+      v[1] = (double)div[c] * div[c];
+      v[2] = -(double)div[c];
+      v[3] = 1;
+      v[4] = anchored ? 1 : 0;
     } else r[c] = 0.f;
     q[c] = qc;
   }
-  write_partials<5>(v, partials, sh);
-  if (na > 0) write_partials<BSLOTS>(sl, bpart, sh);
+  store_partials<5>(v, 5, partials, sh);
+  if (ns > 0) store_partials<BSLOTS>(sl, ns, bpart, sh);
 }
 
-__global__ __launch_bounds__(256) void k_fin_init(const uint32_t* __restrict__ facesB, double* __restrict__ bvec,
-                                                  const double* __restrict__ rhsB, const int* __restrict__ activeList,
-                                                  const int* __restrict__ labelSlot, const double* __restrict__ partials,
-                                                  const double* __restrict__ bpart, int nb, DevScalars* __restrict__ sc) {
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_init(const uint32_t* __restrict__ facesB, double* __restrict__ bvec,
+                                                          const double* __restrict__ rhsB,
+                                                          const int* __restrict__ activeList,
+                                                          const int* __restrict__ labelSlot,
+                                                          const double* __restrict__ partials,
+                                                          const double* __restrict__ bpart, int nb,
+                                                          DevScalars* __restrict__ sc) {
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], slotTot[BSLOTS];
   __shared__ double mean_s;
   const int na = sc->active_bubbles;
   double tot[5];
-  read_partials<5>(partials, nb, tot, sh, res);
-  double stt[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (na > 0) read_partials<BSLOTS>(bpart, nb, stt, sh, res);
+  sum_partials<5>(partials, nb, 5, tot, sh, res);
+  double stt[BSLOTS];
+  sum_partials<BSLOTS>(bpart, nb, nslots(na), stt, sh, res);
   if (threadIdx.x == 0)
     for (int k = 0; k < BSLOTS; k++) slotTot[k] = stt[k];
   __syncthreads();
@@ -297,181 +327,350 @@ __global__ __launch_bounds__(256) void k_fin_init(const uint32_t* __restrict__ f
     const bool stop = sc->maxit <= 0 || (sc->tol > 0 && (sc->norm0 == 0 || sc->norm0 <= sc->tol * sc->normb));
     sc->done = stop ? 1 : 0;
     sc->mg_scale = sc->norm0 > 0 ? (float)(1.0 / sc->norm0) : 1.f;
+    sc->mg_scale_next = sc->mg_scale;
   }
   __syncthreads();
   if (mean_s != 0)
     for (int k = threadIdx.x; k < na; k += blockDim.x) rb[activeList[k]] -= mean_s;
 }
 
-// ================================================================== MG right-hand side
-template <class S>
-__global__ __launch_bounds__(256) void k_mg_input(const uint8_t* __restrict__ opMask, const uint8_t* __restrict__ tipo,
-                                                  const uint32_t* __restrict__ rot, const uint32_t* __restrict__ facesB,
-                                                  const uint32_t* __restrict__ cellsB, float* __restrict__ r, int C,
-                                                  int ar, const double* __restrict__ bvec, int first, int mixed,
-                                                  S* __restrict__ b0, const DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C) return;
-  const float scale = mixed ? sc->mg_scale : 1.f;
-  float val = 0.f;
-  if (mask_type(opMask[c]) == AGUA) {
-    val = r[c];
-    if (first && sc->proj_mean != 0) { val = (float)((double)val - sc->proj_mean); r[c] = val; }
-  } else {
-    uint32_t lb;
-    if (is_bub_cell(tipo, rot, facesB, ar, c, &lb)) val = (float)(bv((double*)bvec, BV_R)[lb] / cellsB[lb]);
-  }
-  st<S>(b0, c, val * scale);
+// ================================================================== smoother tiles
+// The lane's four cells: in rows ly0 and ly0 + 8 (ly0 = lane >> 3) the pair of columns 2p,
+// 2p + 1 (p = lane & 7); one cell of each pair is red and one black.  Cells4 indexes them
+// [colour][row], so a half-sweep touches registers at compile-time indices only.
+// m: bits 0-3 free neighbours, bit 4 updated by the sweeps (water with a free neighbour),
+// bit 5 water (kept in the output; other cells are written as 0).
+struct Cells4 {
+  float v[2][2], b[2][2], ik[2][2];
+  int m[2][2];
+};
+
+// column and row in the tile of the lane's cell of colour q in row r (colour (i + j + 1) & 1,
+// tile origins even)
+__device__ inline int cell_lx(int lane, int q) { return 2 * (lane & 7) + ((q + (lane >> 3) + 1) & 1); }
+__device__ inline int cell_ly(int lane, int r) { return (lane >> 3) + 8 * r; }
+__device__ inline int cell_e(int lane, int q, int r) { return cell_lx(lane, q) + 1 + (cell_ly(lane, r) + 1) * TS; }
+
+__device__ inline void cell_setup(Cells4& c, int q, int r, uint8_t mk, float b) {
+  const int cnt = mask_cnt(mk);
+  const bool water = mask_type(mk) == AGUA;
+  c.m[q][r] = mask_viz(mk) | (water && cnt > 0 ? 16 : 0) | (water ? 32 : 0);
+  c.b[q][r] = b;
+  c.ik[q][r] = cnt > 0 ? 1.f / (float)cnt : 0.f;
 }
 
-// ================================================================== fused block smoothers (suavizar, pressao-cpu.js:99-150)
-// A tile of the reference's smoother reads a fixed snapshot: its interior starts from the
-// snapshot and its 1-cell halo stays at the snapshot during its 4 red-black sweeps; the output
-// keeps water cells and zeroes the rest.  Two passes (tile offsets 0 and 8, or 8 and 0) are
-// fused: the block owns one second-pass tile T2, recomputes the four first-pass tiles that
-// cover T2's halo (a 32x32 "centre" region), then runs the second pass on T2.
-//
-// Centre coordinates: X = cx0 + lx, lx in [0, 32); first-pass tile a = lx / 16 (and b = ly / 16).
-// LDS: S snapshot of pass 1 on 34x34 (centre + 1 border), W pass-1 work values, L pass-2 tile.
-__device__ inline bool pass1_same_tile(int lx, int ly, int nlx, int nly) {
-  return nlx >= 0 && nly >= 0 && nlx < 32 && nly < 32 && (nlx >> 4) == (lx >> 4) && (nly >> 4) == (ly >> 4);
+__device__ inline void cell_clear(Cells4& c, int q, int r) {
+  c.m[q][r] = 0;
+  c.b[q][r] = 0.f;
+  c.ik[q][r] = 0.f;
 }
 
-// The four pass-1 tiles on the centre: W starts as S in every tile interior; sweeps use W inside
-// the cell's own tile and S elsewhere.  Thread t owns cells (t%16 + 16a, t/16 + 16b).
-__device__ inline void pass1_sweeps(const float* S, float* W, const float* B, const uint8_t* M, int cx0, int cy0, int nx,
-                                    int ny, int c0, int c1) {
-  const int tid = threadIdx.x;
-  for (int it = 0; it < 4; it++)
-    for (int o = 0; o < 2; o++) {
-      const int cor = o ? c1 : c0;
+// the cells of colour COR: x = (sum of free neighbours + b) / count
+template <int COR>
+__device__ inline void half_sweep(float* T, Cells4& c, int lane) {
+  // the lane's other-colour cell is the right neighbour when this one sits at column 2p
+  const bool left_is_mine = ((COR + (lane >> 3) + 1) & 1) != 0;
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const int lx = (tid & 15) + 16 * (k & 1), ly = (tid >> 4) + 16 * (k >> 1);
-        const int i = cx0 + lx, j = cy0 + ly;
-        if (i < 0 || j < 0 || i >= nx || j >= ny || ((i + j + 1) & 1) != cor) continue;
-        const int e = lx + ly * 32;
-        const uint8_t m = M[e];
-        const int kk = mask_cnt(m);
-        if (mask_type(m) != AGUA || kk == 0) continue;
-        const int es = (lx + 1) + (ly + 1) * 34;
-        float s = 0.f;
-        if (m & 1) s += pass1_same_tile(lx, ly, lx - 1, ly) ? W[es - 1] : S[es - 1];
-        if (m & 2) s += pass1_same_tile(lx, ly, lx + 1, ly) ? W[es + 1] : S[es + 1];
-        if (m & 4) s += pass1_same_tile(lx, ly, lx, ly - 1) ? W[es - 34] : S[es - 34];
-        if (m & 8) s += pass1_same_tile(lx, ly, lx, ly + 1) ? W[es + 34] : S[es + 34];
-        const float qv = W[es];
-        W[es] = qv + ((s + B[e]) / (float)kk - qv);
-      }
-      __syncthreads();
-    }
-}
-
-// pass 2 on T2 (centre offsets 8..23) from the masked pass-1 values; returns the cell's value
-__device__ inline float pass2_sweeps(const float* W, float* L, const float* B, const uint8_t* M, int cx0, int cy0,
-                                     int nx, int ny, int c0, int c1, bool* in_out) {
-  const int tid = threadIdx.x;
-  for (int e = tid; e < 18 * 18; e += 256) {
-    const int lx = 7 + e % 18, ly = 7 + e / 18;  // centre coords
-    const int i = cx0 + lx, j = cy0 + ly;
-    float val = 0.f;
-    if (i >= 0 && j >= 0 && i < nx && j < ny && mask_type(M[lx + ly * 32]) == AGUA) val = W[(lx + 1) + (ly + 1) * 34];
-    L[e] = val;
+  for (int r = 0; r < 2; r++) {
+    const int e = cell_e(lane, COR, r);
+    const float tl = T[e - 1], tr = T[e + 1], td = T[e - TS], tu = T[e + TS];
+    const float oth = c.v[1 - COR][r];
+    const int m = c.m[COR][r];
+    const float L = left_is_mine ? oth : tl;
+    const float R = left_is_mine ? tr : oth;
+    float sum = (m & 1) ? L : 0.f;
+    sum += (m & 2) ? R : 0.f;
+    sum += (m & 4) ? td : 0.f;
+    sum += (m & 8) ? tu : 0.f;
+    const float nv = (m & 16) ? (sum + c.b[COR][r]) * c.ik[COR][r] : c.v[COR][r];
+    c.v[COR][r] = nv;
+    T[e] = nv;
   }
-  const int tx = tid & 15, ty = tid >> 4;
-  const int lx = 8 + tx, ly = 8 + ty;
-  const int i = cx0 + lx, j = cy0 + ly;
-  const bool in = i >= 0 && j >= 0 && i < nx && j < ny;
-  const uint8_t m = in ? M[lx + ly * 32] : 0;
-  const int kk = mask_cnt(m);
-  const bool water = in && mask_type(m) == AGUA;
-  const bool act = water && kk > 0;
-  const float bb = act ? B[lx + ly * 32] : 0.f;
-  const int e = (tx + 1) + (ty + 1) * 18;
-  const int myc = (i + j + 1) & 1;
-  __syncthreads();
-  for (int it = 0; it < 4; it++)
-    for (int o = 0; o < 2; o++) {
-      const int cor = o ? c1 : c0;
-      if (act && myc == cor) {
-        float s = 0.f;
-        if (m & 1) s += L[e - 1];
-        if (m & 2) s += L[e + 1];
-        if (m & 4) s += L[e - 18];
-        if (m & 8) s += L[e + 18];
-        const float qv = L[e];
-        L[e] = qv + ((s + bb) / (float)kk - qv);
-      }
-      __syncthreads();
-    }
-  *in_out = in;
-  return water ? L[e] : 0.f;
 }
 
-// load b and the mask of the 32x32 centre
+// 4 red-black sweeps (colour order C0, 1 - C0) of the tile in T (18x18, halo fixed); the
+// lane's own cells live in registers and are mirrored into T after every update.
+template <int C0>
+__device__ inline void tile_sweeps(float* T, Cells4& c, int lane) {
+#pragma unroll
+  for (int it = 0; it < 4; it++) {
+    half_sweep<C0>(T, c, lane);
+    wave_sync();
+    half_sweep<1 - C0>(T, c, lane);
+    wave_sync();
+  }
+}
+
+// T := snapshot of src on the tile + halo (0 outside the grid)
+template <class SX>
+__device__ inline void fill_tile(float* T, const SX* __restrict__ src, int ox, int oy, int nx, int ny, int lane) {
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const int e = lane + 64 * q;
+    if (e < TCELLS) {
+      const int ly = e / TS, lx = e - ly * TS;
+      const int i = ox - 1 + lx, j = oy - 1 + ly;
+      T[e] = (i >= 0 && j >= 0 && i < nx && j < ny) ? ld<SX>(src, (size_t)i + (size_t)j * nx) : 0.f;
+    }
+  }
+}
+
+__device__ inline void zero_tile(float* T, int lane) {
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const int e = lane + 64 * q;
+    if (e < TCELLS) T[e] = 0.f;
+  }
+}
+
+// the lane's cells: mask and b from global memory (issued before the tile loads so both
+// overlap), then the cell state with values from T
+struct Own4 {
+  uint8_t m[2][2];
+  float b[2][2];
+};
+
 template <class SB>
-__device__ inline void load_centre(const SB* __restrict__ b, const uint8_t* __restrict__ mask, int cx0, int cy0, int nx,
-                                   int ny, float* B, uint8_t* M) {
-  for (int e = threadIdx.x; e < 32 * 32; e += 256) {
-    const int i = cx0 + (e & 31), j = cy0 + (e >> 5);
-    const bool in = i >= 0 && j >= 0 && i < nx && j < ny;
-    const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
-    M[e] = in ? mask[c] : 0;
-    B[e] = in ? ld<SB>(b, c) : 0.f;
-  }
+__device__ inline void prefetch_cells(Own4& o, const SB* __restrict__ b, const uint8_t* __restrict__ mask, int ox,
+                                      int oy, int nx, int ny, int lane) {
+#pragma unroll
+  for (int q = 0; q < 2; q++)
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      const int i = ox + cell_lx(lane, q), j = oy + cell_ly(lane, r);
+      const bool in = i < nx && j < ny && i >= 0 && j >= 0;
+      const size_t cc = in ? (size_t)i + (size_t)j * nx : 0;
+      o.m[q][r] = in ? mask[cc] : 0;
+      o.b[q][r] = in ? ld<SB>(b, cc) : 0.f;
+    }
 }
 
-// Down: passes (0, [red, black]) then (8, [red, black]) from x = 0 (cicloV starts every level
-// at zero, pressao-cpu.js:243 and 168).  Grid = offset-8 tiles.
-template <class SX, class SB>
-__global__ __launch_bounds__(256) void k_down(SX* __restrict__ dst, const SB* __restrict__ b,
-                                              const uint8_t* __restrict__ mask, int nx, int ny,
+__device__ inline void init_cells(Cells4& c, const Own4& o, const float* T, int lane) {
+#pragma unroll
+  for (int q = 0; q < 2; q++)
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      cell_setup(c, q, r, o.m[q][r], o.b[q][r]);
+      c.v[q][r] = T[cell_e(lane, q, r)];
+    }
+}
+
+template <class SX>
+__device__ inline void store_cells(const Cells4& c, SX* __restrict__ dst, int ox, int oy, int nx, int ny, int lane) {
+#pragma unroll
+  for (int q = 0; q < 2; q++)
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      const int i = ox + cell_lx(lane, q), j = oy + cell_ly(lane, r);
+      if (i < nx && j < ny && i >= 0 && j >= 0)
+        st<SX>(dst, (size_t)i + (size_t)j * nx, (c.m[q][r] & 32) ? c.v[q][r] : 0.f);
+    }
+}
+
+// tile of wave w of block (bx, by): 2x2 tiles per 256-thread block
+__device__ inline void block_tile(int* tx, int* ty) {
+  const int w = threadIdx.x >> 6;
+  *tx = 2 * blockIdx.x + (w & 1);
+  *ty = 2 * blockIdx.y + (w >> 1);
+}
+
+// ================================================================== level 0 input + first pass
+// UPD: q += alpha p, r -= alpha Ap and |r|^2 (PCG update, pressao-cpu.js:373-381); else the
+// first cycle (r minus the projected mean).  b0 = scale * r on water, rb/cells on bubble cells
+// (pressao-cpu.js:326-332), then pass A (offset 0, colours 0,1) from x = 0 (cicloV,
+// pressao-cpu.js:243) when level 0 is not the coarsest.
+template <class SB, bool UPD, bool PASSA>
+__global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMask, const uint8_t* __restrict__ mg0,
+                                               const uint32_t* __restrict__ rot, const uint32_t* __restrict__ cellsB,
+                                               const float* __restrict__ p, const float* __restrict__ ap,
+                                               float* __restrict__ q, float* __restrict__ r, int nx, int ny, int ntx,
+                                               int nty, const double* __restrict__ bvec, SB* __restrict__ b0,
+                                               float* __restrict__ x0, double* __restrict__ partials, int scaled,
+                                               const DevScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ float Tt[4][TCELLS];
+  __shared__ double sh[16];
+  const int lane = threadIdx.x & 63;
+  int tx, ty;
+  block_tile(&tx, &ty);
+  const bool valid = tx < ntx && ty < nty;
+  const int ox = tx * 16, oy = ty * 16;
+  const double alpha = UPD ? sc->alpha : 0.0;
+  const double mean = UPD ? 0.0 : sc->proj_mean;
+  const float scale = scaled ? sc->mg_scale : 1.f;
+  const double* rb = bv((double*)bvec, BV_R);
+  double rr[1] = {0};
+  // every load of the four cells first (one memory latency), then the arithmetic
+  uint8_t om[2][2], gm[2][2];
+  float qv[2][2], pv[2][2], rv[2][2], av[2][2];
+  size_t ci[2][2];
+  bool in[2][2];
+#pragma unroll
+  for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+    for (int rw = 0; rw < 2; rw++) {
+      const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
+      in[cq][rw] = valid && i < nx && j < ny;
+      const size_t c = in[cq][rw] ? (size_t)i + (size_t)j * nx : 0;
+      ci[cq][rw] = c;
+      om[cq][rw] = in[cq][rw] ? opMask[c] : 0;
+      gm[cq][rw] = in[cq][rw] ? mg0[c] : 0;
+      rv[cq][rw] = r[c];
+      if (UPD) {
+        qv[cq][rw] = q[c];
+        pv[cq][rw] = p[c];
+        av[cq][rw] = ap[c];
+      }
+    }
+  Cells4 cc;
+#pragma unroll
+  for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+    for (int rw = 0; rw < 2; rw++) {
+      cc.v[cq][rw] = 0.f;
+      if (in[cq][rw]) {
+        const size_t c = ci[cq][rw];
+        const uint8_t m = om[cq][rw];
+        float bval = 0.f;
+        if (mask_type(m) == AGUA) {
+          float rn;
+          if (UPD) {
+            q[c] = (float)((double)qv[cq][rw] + alpha * (double)pv[cq][rw]);
+            rn = (float)((double)rv[cq][rw] - alpha * (double)av[cq][rw]);
+            r[c] = rn;
+            rr[0] += (double)rn * rn;
+          } else {
+            rn = rv[cq][rw];
+            if (mean != 0) { rn = (float)((double)rn - mean); r[c] = rn; }
+          }
+          bval = rn;
+        } else if (m & MK_BUB) {
+          const uint32_t lb = rot[c];
+          bval = (float)(rb[lb] / cellsB[lb]);
+        }
+        bval = rnd<SB>(bval * scale);
+        st<SB>(b0, c, bval);
+        cell_setup(cc, cq, rw, gm[cq][rw], bval);
+      } else cell_clear(cc, cq, rw);
+    }
+  if (PASSA && valid) {
+    float* T = Tt[threadIdx.x >> 6];
+    zero_tile(T, lane);
+    wave_sync();
+    tile_sweeps<0>(T, cc, lane);
+    store_cells<float>(cc, x0, ox, oy, nx, ny, lane);
+  }
+  if (UPD) store_partials<1>(rr, 1, partials, sh);
+}
+
+// ================================================================== one smoothing pass
+template <class SX, class SB, int C0>
+__global__ __launch_bounds__(256) void k_pass(const SX* __restrict__ src, SX* __restrict__ dst,
+                                              const SB* __restrict__ b, const uint8_t* __restrict__ mask, int nx,
+                                              int ny, int desl, int ntx, int nty,
                                               const DevScalars* __restrict__ sc) {
   if (sc->done) return;
-  __shared__ float S[34 * 34], W[34 * 34], B[32 * 32], L[18 * 18];
-  __shared__ uint8_t M[32 * 32];
-  const int ox2 = blockIdx.x * 16 - 8, oy2 = blockIdx.y * 16 - 8;
-  const int cx0 = ox2 - 8, cy0 = oy2 - 8;
-  load_centre<SB>(b, mask, cx0, cy0, nx, ny, B, M);
-  for (int e = threadIdx.x; e < 34 * 34; e += 256) { S[e] = 0.f; W[e] = 0.f; }
-  __syncthreads();
-  pass1_sweeps(S, W, B, M, cx0, cy0, nx, ny, 0, 1);
-  bool in;
-  const float v = pass2_sweeps(W, L, B, M, cx0, cy0, nx, ny, 0, 1, &in);
-  if (in) {
-    const int i = ox2 + (threadIdx.x & 15), j = oy2 + (threadIdx.x >> 4);
-    st<SX>(dst, (size_t)i + (size_t)j * nx, v);
-  }
+  __shared__ float Tt[4][TCELLS];
+  const int lane = threadIdx.x & 63;
+  int tx, ty;
+  block_tile(&tx, &ty);
+  if (tx >= ntx || ty >= nty) return;
+  const int ox = tx * 16 - desl, oy = ty * 16 - desl;
+  float* T = Tt[threadIdx.x >> 6];
+  Own4 own;
+  prefetch_cells<SB>(own, b, mask, ox, oy, nx, ny, lane);
+  fill_tile<SX>(T, src, ox, oy, nx, ny, lane);
+  wave_sync();
+  Cells4 cc;
+  init_cells(cc, own, T, lane);
+  tile_sweeps<C0>(T, cc, lane);
+  store_cells<SX>(cc, dst, ox, oy, nx, ny, lane);
 }
 
-// prolongation term of a fine cell (prolongar, pressao-cpu.js:191-214)
-template <class SC>
-__device__ inline float prolong_at(const SC* __restrict__ cx, const uint8_t* __restrict__ cmk, int cnx, int cny, int i,
-                                   int j) {
-  const int I0 = (i & 1) == 0 ? (i >> 1) - 1 : (i - 1) >> 1;
-  const int J0 = (j & 1) == 0 ? (j >> 1) - 1 : (j - 1) >> 1;
-  const float wx0 = (i & 1) == 0 ? 0.25f : 0.75f, wy0 = (j & 1) == 0 ? 0.25f : 0.75f;
-  float e = 0.f;
-  for (int b = 0; b < 2; b++) {
-    const int J = J0 + b;
-    if (J < 0 || J >= cny) continue;
-    const float wy = b ? 1.f - wy0 : wy0;
-    for (int a = 0; a < 2; a++) {
-      const int I = I0 + a;
-      if (I < 0 || I >= cnx) continue;
-      const size_t C = (size_t)I + (size_t)J * cnx;
-      if (mask_type(cmk[C]) == AGUA) e += ld<SC>(cx, C) * (a ? 1.f - wx0 : wx0) * wy;
+// prolongar (pressao-cpu.js:191-214) folded into the snapshot of the first up pass (offset 8,
+// colours 1,0): water cells read x + P e, with the coarse values of the tile's 10x10 patch
+template <class SX, class SB, class SC>
+__global__ __launch_bounds__(256) void k_prolong_pass(const SX* __restrict__ src, SX* __restrict__ dst,
+                                                      const SB* __restrict__ b, const uint8_t* __restrict__ mask,
+                                                      int nx, int ny, int ntx, int nty, const SC* __restrict__ cx,
+                                                      const uint8_t* __restrict__ cmk, int cnx, int cny,
+                                                      const DevScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ float Tt[4][TCELLS];
+  __shared__ float CPt[4][100];
+  const int lane = threadIdx.x & 63;
+  int tx, ty;
+  block_tile(&tx, &ty);
+  if (tx >= ntx || ty >= nty) return;
+  const int ox = tx * 16 - 8, oy = ty * 16 - 8;
+  float* T = Tt[threadIdx.x >> 6];
+  float* CP = CPt[threadIdx.x >> 6];
+  const int PX = (ox >> 1) - 1, PY = (oy >> 1) - 1;
+  // all global loads first: own cells, the coarse patch, the snapshot with its masks
+  Own4 own;
+  prefetch_cells<SB>(own, b, mask, ox, oy, nx, ny, lane);
+  float cv[2];
+  uint8_t cm[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int e = lane + 64 * q;
+    const int bj = e / 10, ai = e - bj * 10;
+    const int I = PX + ai, J = PY + bj;
+    const bool in = e < 100 && I >= 0 && J >= 0 && I < cnx && J < cny;
+    const size_t C = in ? (size_t)I + (size_t)J * cnx : 0;
+    cm[q] = in ? cmk[C] : 0;
+    cv[q] = in ? ld<SC>(cx, C) : 0.f;
+  }
+  float sv[6];
+  uint8_t sm[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const int e = lane + 64 * q;
+    const int ly = e / TS, lx = e - ly * TS;
+    const int i = ox - 1 + lx, j = oy - 1 + ly;
+    const bool in = e < TCELLS && i >= 0 && j >= 0 && i < nx && j < ny;
+    const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
+    sv[q] = in ? ld<SX>(src, c) : 0.f;
+    sm[q] = in ? mask[c] : 0;
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int e = lane + 64 * q;
+    if (e < 100) CP[e] = mask_type(cm[q]) == AGUA ? cv[q] : 0.f;
+  }
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const int e = lane + 64 * q;
+    if (e < TCELLS) {
+      const int ly = e / TS, lx = e - ly * TS;
+      const int i = ox - 1 + lx, j = oy - 1 + ly;
+      float val = sv[q];
+      if (mask_type(sm[q]) == AGUA) {  // in the grid (sm is 0 outside)
+        const int I0 = ((i & 1) == 0 ? (i >> 1) - 1 : (i - 1) >> 1) - PX;
+        const int J0 = ((j & 1) == 0 ? (j >> 1) - 1 : (j - 1) >> 1) - PY;
+        const float wx0 = (i & 1) == 0 ? 0.25f : 0.75f, wy0 = (j & 1) == 0 ? 0.25f : 0.75f;
+        const float wx1 = 1.f - wx0, wy1 = 1.f - wy0;
+        float e2 = 0.f;
+        e2 += CP[I0 + J0 * 10] * wx0 * wy0;
+        e2 += CP[I0 + 1 + J0 * 10] * wx1 * wy0;
+        e2 += CP[I0 + (J0 + 1) * 10] * wx0 * wy1;
+        e2 += CP[I0 + 1 + (J0 + 1) * 10] * wx1 * wy1;
+        val += e2;
+      }
+      T[e] = val;
     }
   }
-  return e;
+  wave_sync();
+  Cells4 cc;
+  init_cells(cc, own, T, lane);
+  tile_sweeps<1>(T, cc, lane);
+  store_cells<SX>(cc, dst, ox, oy, nx, ny, lane);
 }
 
-struct ZArgs {  // level-0 up kernel: z = MG result and the dots r.z, r.z_old
+struct ZArgs {  // level-0 outputs: z = MG result and the dots r.z, r.z_old
   const uint8_t* opMask;
-  const uint8_t* tipo;
   const uint32_t* rot;
-  const uint32_t* facesB;
   const uint32_t* cellsB;
   const int* labelSlot;
   const float* r;
@@ -480,184 +679,319 @@ struct ZArgs {  // level-0 up kernel: z = MG result and the dots r.z, r.z_old
   double* bvec;
   double* partials;
   double* bpart;
-  int ar;
-  int mixed;
+  int scaled;
 };
 
-// Up: prolongation folded into the snapshot, then passes (8, [black, red]) and (0, [black, red]).
-// Grid = offset-0 tiles.  With ZD the block writes z and its dot/bubble partials instead of x.
-template <class SX, class SB, class SC, bool ZD>
-__global__ __launch_bounds__(256) void k_up(const SX* __restrict__ src, SX* __restrict__ dst, const SB* __restrict__ b,
-                                            const uint8_t* __restrict__ mask, int nx, int ny, const SC* __restrict__ cx,
-                                            const uint8_t* __restrict__ cmk, int cnx, int cny, ZArgs za,
-                                            const DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ float S[34 * 34], W[34 * 34], B[32 * 32], L[18 * 18];
-  __shared__ uint8_t M[32 * 32];
-  __shared__ double sh[BSLOTS * 16];
-  const int ox2 = blockIdx.x * 16, oy2 = blockIdx.y * 16;
-  const int cx0 = ox2 - 8, cy0 = oy2 - 8;
-  load_centre<SB>(b, mask, cx0, cy0, nx, ny, B, M);
-  for (int e = threadIdx.x; e < 34 * 34; e += 256) {
-    const int i = cx0 - 1 + e % 34, j = cy0 - 1 + e / 34;
-    float val = 0.f;
-    if (i >= 0 && j >= 0 && i < nx && j < ny) {
-      const size_t c = (size_t)i + (size_t)j * nx;
-      val = ld<SX>(src, c);
-      if (mask_type(mask[c]) == AGUA) val += prolong_at<SC>(cx, cmk, cnx, cny, i, j);
-    }
-    S[e] = val;
-    W[e] = val;
+// z and its dot/bubble contributions for one cell (x = MG output of level 0)
+__device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, float zoldc, float xv, float inv,
+                              double (&dv)[2], double (&sl)[BSLOTS]) {
+  const float x = xv * inv;
+  float z = 0.f;
+  if (mask_type(om) == AGUA) {
+    z = x;
+    dv[0] += (double)rc * z;
+    dv[1] += (double)rc * zoldc;
+  } else if (om & MK_BUB) {
+    const uint32_t lb = za.rot[c];
+    bub_add(sl, za.labelSlot, bv(za.bvec, BV_ACC1), lb, (double)x / za.cellsB[lb]);
   }
-  __syncthreads();
-  pass1_sweeps(S, W, B, M, cx0, cy0, nx, ny, 1, 0);
-  bool in;
-  const float v = pass2_sweeps(W, L, B, M, cx0, cy0, nx, ny, 1, 0, &in);
-  const int i = ox2 + (threadIdx.x & 15), j = oy2 + (threadIdx.x >> 4);
-  if (!ZD) {
-    if (in) st<SX>(dst, (size_t)i + (size_t)j * nx, v);
-    return;
-  }
-  // ---- z and the dots (level 0)
-  double dv[2] = {0, 0};
-  double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int na = sc->active_bubbles;
-  if (in) {
-    const size_t c = (size_t)i + (size_t)j * nx;
-    const float xv = v * (za.mixed ? 1.f / sc->mg_scale : 1.f);
-    float z = 0.f;
-    if (mask_type(za.opMask[c]) == AGUA) {
-      z = xv;
-      dv[0] = (double)za.r[c] * z;
-      dv[1] = (double)za.r[c] * za.zold[c];
-    } else {
-      uint32_t lb;
-      if (is_bub_cell(za.tipo, za.rot, za.facesB, za.ar, c, &lb))
-        bub_add(sl, za.labelSlot, bv(za.bvec, BV_ACC1), lb, (double)xv / za.cellsB[lb]);
-    }
-    za.znew[c] = z;
-  }
-  write_partials<2>(dv, za.partials, sh);
-  if (na > 0) write_partials<BSLOTS>(sl, za.bpart, sh);
+  za.znew[c] = z;
 }
 
-// ================================================================== restriction (restringir, pressao-cpu.js:159-188)
-// Coarse tile 16x16: the fine residual r = b - A x on the 34x34 fine cells it gathers from
-// (2I-1 .. 2I+2 per axis, weights 1/4 3/4 3/4 1/4), from x on 36x36, all in LDS.
-template <class SXF, class SBF, class SBC>
-__global__ __launch_bounds__(256) void k_restrict(const SXF* __restrict__ fx, const SBF* __restrict__ fb,
-                                                  const uint8_t* __restrict__ fm, int fnx, int fny, SBC* __restrict__ cb,
-                                                  const uint8_t* __restrict__ cmk, int cnx, int cny,
-                                                  const DevScalars* __restrict__ sc) {
+// last up pass of level 0 (offset 0, colours 1,0) writing z and the dot partials
+template <class SB>
+__global__ __launch_bounds__(256) void k_pass_z(const float* __restrict__ src, const SB* __restrict__ b,
+                                                const uint8_t* __restrict__ mask, int nx, int ny, int ntx, int nty,
+                                                ZArgs za, const DevScalars* __restrict__ sc) {
   if (sc->done) return;
-  __shared__ float X[36 * 36], R[34 * 34];
+  __shared__ float Tt[4][TCELLS];
+  __shared__ double sh[BSLOTS * 16];
+  const int lane = threadIdx.x & 63;
+  const int ns = nslots(sc->active_bubbles);
+  int tx, ty;
+  block_tile(&tx, &ty);
+  double dv[2] = {0, 0};
+  double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (tx < ntx && ty < nty) {
+    const int ox = tx * 16, oy = ty * 16;
+    float* T = Tt[threadIdx.x >> 6];
+    Own4 own;
+    prefetch_cells<SB>(own, b, mask, ox, oy, nx, ny, lane);
+    uint8_t om[2][2];
+    float rv[2][2], zo[2][2];
+#pragma unroll
+    for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+      for (int rw = 0; rw < 2; rw++) {
+        const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
+        const bool in = i < nx && j < ny;
+        const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
+        om[cq][rw] = in ? za.opMask[c] : 0;
+        rv[cq][rw] = za.r[c];
+        zo[cq][rw] = za.zold[c];
+      }
+    fill_tile<float>(T, src, ox, oy, nx, ny, lane);
+    wave_sync();
+    Cells4 cc;
+    init_cells(cc, own, T, lane);
+    tile_sweeps<1>(T, cc, lane);
+    const float inv = za.scaled ? 1.f / sc->mg_scale : 1.f;
+#pragma unroll
+    for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+      for (int rw = 0; rw < 2; rw++) {
+        const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
+        if (i < nx && j < ny)
+          z_cell(za, (size_t)i + (size_t)j * nx, om[cq][rw], rv[cq][rw], zo[cq][rw],
+                 (cc.m[cq][rw] & 32) ? cc.v[cq][rw] : 0.f, inv, dv, sl);
+      }
+  }
+  store_partials<2>(dv, 2, za.partials, sh);
+  if (ns > 0) store_partials<BSLOTS>(sl, ns, za.bpart, sh);
+}
+
+// z and the dots when level 0 itself is solved inside the workgroup (small grids)
+__global__ __launch_bounds__(256) void k_zdots0(const float* __restrict__ x0, ZArgs za, int C,
+                                                const DevScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double sh[BSLOTS * 16];
+  const int ns = nslots(sc->active_bubbles);
+  double dv[2] = {0, 0};
+  double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const float inv = za.scaled ? 1.f / sc->mg_scale : 1.f;
+  if (c < C) z_cell(za, c, za.opMask[c], za.r[c], za.zold[c], x0[c], inv, dv, sl);
+  store_partials<2>(dv, 2, za.partials, sh);
+  if (ns > 0) store_partials<BSLOTS>(sl, ns, za.bpart, sh);
+}
+
+// ================================================================== restriction + first pass of the coarse level
+// restringir (pressao-cpu.js:159-188): the block owns a 16x16 coarse tile; fine residuals
+// r = b - A x on the 34x34 fine cells it gathers from (2I-1 .. 2I+2 per axis, weights
+// 1/4 3/4 3/4 1/4), from x on 36x36, in LDS.  Then wave 0 runs the coarse level's pass A
+// (offset 0 = this tile) from x = 0 unless the coarse level is the coarsest.
+template <class SXF, class SBF, class SC, bool PASSA>
+__global__ __launch_bounds__(256) void k_restrict_a(const SXF* __restrict__ fx, const SBF* __restrict__ fb,
+                                                    const uint8_t* __restrict__ fm, int fnx, int fny,
+                                                    SC* __restrict__ cb, SC* __restrict__ cxo,
+                                                    const uint8_t* __restrict__ cmk, int cnx, int cny,
+                                                    const DevScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ float X[36 * 36], R[34 * 34], Bt[256], T[TCELLS];
+  const int tid = threadIdx.x;
   const int CI0 = blockIdx.x * 16, CJ0 = blockIdx.y * 16;
   const int fx0 = 2 * CI0 - 2, fy0 = 2 * CJ0 - 2;  // X origin
-  for (int e = threadIdx.x; e < 36 * 36; e += 256) {
-    const int i = fx0 + e % 36, j = fy0 + e / 36;
-    X[e] = (i >= 0 && j >= 0 && i < fnx && j < fny) ? ld<SXF>(fx, (size_t)i + (size_t)j * fnx) : 0.f;
+  // every global load first: the fine masks and b of the residual cells, the coarse masks
+  // (this thread's coarse cell and, for pass A, wave 0's four cells), then x
+  uint8_t mr[5];
+  float br[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const int e = tid + 256 * q;
+    const int ly = e / 34, lx = e - ly * 34;
+    const int i = fx0 + 1 + lx, j = fy0 + 1 + ly;
+    const bool in = e < 34 * 34 && i >= 0 && j >= 0 && i < fnx && j < fny;
+    const size_t c = in ? (size_t)i + (size_t)j * fnx : 0;
+    mr[q] = in ? fm[c] : 0;
+    br[q] = in ? ld<SBF>(fb, c) : 0.f;
+  }
+  const int Im = CI0 + (tid & 15), Jm = CJ0 + (tid >> 4);
+  const bool cin = Im < cnx && Jm < cny;
+  const size_t Cm = cin ? (size_t)Im + (size_t)Jm * cnx : 0;
+  const uint8_t cmine = cin ? cmk[Cm] : 0;
+  uint8_t cma[2][2];
+#pragma unroll
+  for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+    for (int rw = 0; rw < 2; rw++) {
+      const int I = CI0 + cell_lx(tid & 63, cq), J = CJ0 + cell_ly(tid & 63, rw);
+      cma[cq][rw] = (PASSA && I < cnx && J < cny) ? cmk[(size_t)I + (size_t)J * cnx] : 0;
+    }
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const int e = tid + 256 * q;
+    if (e < 36 * 36) {
+      const int ly = e / 36, lx = e - ly * 36;
+      const int i = fx0 + lx, j = fy0 + ly;
+      X[e] = (i >= 0 && j >= 0 && i < fnx && j < fny) ? ld<SXF>(fx, (size_t)i + (size_t)j * fnx) : 0.f;
+    }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 34 * 34; e += 256) {
-    const int lx = e % 34, ly = e / 34;
-    const int i = fx0 + 1 + lx, j = fy0 + 1 + ly;
-    float rr = 0.f;
-    if (i >= 0 && j >= 0 && i < fnx && j < fny) {
-      const size_t c = (size_t)i + (size_t)j * fnx;
-      const uint8_t m = fm[c];
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const int e = tid + 256 * q;
+    if (e < 34 * 34) {
+      const int ly = e / 34, lx = e - ly * 34;
+      const uint8_t m = mr[q];
+      float rr = 0.f;
       if (mask_type(m) == AGUA) {
         const int ex = (lx + 1) + (ly + 1) * 36;
-        float s = 0.f;
-        if (m & 1) s += X[ex - 1];
-        if (m & 2) s += X[ex + 1];
-        if (m & 4) s += X[ex - 36];
-        if (m & 8) s += X[ex + 36];
-        rr = ld<SBF>(fb, c) - ((float)mask_cnt(m) * X[ex] - s);
+        float s = (m & 1) ? X[ex - 1] : 0.f;
+        s += (m & 2) ? X[ex + 1] : 0.f;
+        s += (m & 4) ? X[ex - 36] : 0.f;
+        s += (m & 8) ? X[ex + 36] : 0.f;
+        rr = br[q] - ((float)mask_cnt(m) * X[ex] - s);
       }
+      R[e] = rr;
     }
-    R[e] = rr;
   }
   __syncthreads();
-  const int I = CI0 + (threadIdx.x & 15), J = CJ0 + (threadIdx.x >> 4);
-  if (I >= cnx || J >= cny) return;
-  const size_t Cc = (size_t)I + (size_t)J * cnx;
-  if (mask_type(cmk[Cc]) != AGUA) { st<SBC>(cb, Cc, 0.f); return; }
-  const float w[4] = {0.25f, 0.75f, 0.75f, 0.25f};
-  // fine cell 2I-1+bi sits at R index (2I-1+bi) - (fx0+1) = 2(I-CI0) + bi
-  const int rx = 2 * (I - CI0), ry = 2 * (J - CJ0);
-  float acc = 0.f;
-  for (int bj = 0; bj < 4; bj++) {
-    float racc = 0.f;
-    for (int bi = 0; bi < 4; bi++) racc += R[(rx + bi) + (ry + bj) * 34] * w[bi];
-    acc += racc * w[bj];
+  {
+    float bc = 0.f;
+    if (cin) {
+      if (mask_type(cmine) == AGUA) {
+        const float w[4] = {0.25f, 0.75f, 0.75f, 0.25f};
+        // fine cell 2I-1+bi sits at R index (2I-1+bi) - (fx0+1) = 2(I-CI0) + bi
+        const int rx = 2 * (tid & 15), ry = 2 * (tid >> 4);
+        float acc = 0.f;
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++) {
+          float racc = 0.f;
+#pragma unroll
+          for (int bi = 0; bi < 4; bi++) racc += R[(rx + bi) + (ry + bj) * 34] * w[bi];
+          acc += racc * w[bj];
+        }
+        bc = rnd<SC>(acc);
+      }
+      st<SC>(cb, Cm, bc);
+    }
+    Bt[tid] = bc;
   }
-  st<SBC>(cb, Cc, acc);
+  if (!PASSA) return;
+  __syncthreads();
+  if (tid >= 64) return;
+  const int lane = tid;
+  zero_tile(T, lane);
+  Cells4 cc;
+#pragma unroll
+  for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+    for (int rw = 0; rw < 2; rw++) {
+      const int li = cell_lx(lane, cq), lj = cell_ly(lane, rw);
+      const int I = CI0 + li, J = CJ0 + lj;
+      cc.v[cq][rw] = 0.f;
+      if (I < cnx && J < cny) cell_setup(cc, cq, rw, cma[cq][rw], Bt[li + 16 * lj]);
+      else cell_clear(cc, cq, rw);
+    }
+  wave_sync();
+  tile_sweeps<0>(T, cc, lane);
+  store_cells<SC>(cc, cxo, CI0, CJ0, cnx, cny, lane);
+}
+
+// ================================================================== coarsest-level operator
+// resolverGrosso (pressao-cpu.js:217-238) runs 64 red-black sweeps from x = 0, so its result is
+// x = G b for a matrix G fixed by the level's masks.  Column j is the result for b = e_j: one
+// wavefront per column runs the sweeps on its own copy of the level (LDS), cells in registers.
+// Columns of cells that are not water are zero (their b is never read).
+__global__ __launch_bounds__(64) void k_coarsest_op(const uint8_t* __restrict__ mask, int nx, int n,
+                                                    float* __restrict__ G) {
+  __shared__ float x[DENSE_MAX];
+  const int j = blockIdx.x, lane = threadIdx.x;
+  for (int c = lane; c < n; c += 64) x[c] = 0.f;
+  if (mask_type(mask[j]) != AGUA) {
+    for (int c = lane; c < n; c += 64) G[(size_t)c * n + j] = 0.f;
+    return;
+  }
+  int mk[3];
+  float ikk[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int c = lane + 64 * k;
+    const uint8_t m = c < n ? mask[c] : 0;
+    const int q = mask_cnt(m);
+    const bool act = mask_type(m) == AGUA && q > 0;
+    mk[k] = act ? (mask_viz(m) | 16 | ((m & MK_COLOUR) ? 32 : 0)) : 0;
+    ikk[k] = act ? 1.f / (float)q : 0.f;
+  }
+  wave_sync();
+  for (int k2 = 0; k2 < 64; k2++)
+#pragma unroll
+    for (int o = 0; o < 2; o++) {
+      const int cor = (k2 & 1) ? 1 - o : o;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const int c = lane + 64 * k;
+        const int m = mk[k];
+        if ((m & 16) && ((m >> 5) & 1) == cor) {
+          float sum = (m & 1) ? x[c - 1] : 0.f;
+          sum += (m & 2) ? x[c + 1] : 0.f;
+          sum += (m & 4) ? x[c - nx] : 0.f;
+          sum += (m & 8) ? x[c + nx] : 0.f;
+          x[c] = (sum + (c == j ? 1.f : 0.f)) * ikk[k];
+        }
+      }
+      wave_sync();
+    }
+  for (int c = lane; c < n; c += 64) G[(size_t)c * n + j] = x[c];
 }
 
 // ================================================================== coarse levels in one workgroup
 struct WGPlan {
   int lc, L;
+  const float* dense;  // coarsest-level operator G (n x n, row-major) or null
   int nx[12], ny[12];
   int offx[12], offt[12], offb[12], offm[12];  // byte offsets into the LDS pool
   const uint8_t* mask[12];
 };
 
-// one smoothing pass of an LDS-resident level: S -> W (each wave owns whole tiles and syncs
-// only itself inside a tile); the caller swaps S and W afterwards
-__device__ inline void wg_smooth(const float* S, float* W, const float* B, const uint8_t* M, int nx, int ny, int desl,
-                                 int c0, int c1) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-  const int tnx = (nx + desl + 15) / 16, tny = (ny + desl + 15) / 16;
-  for (int t = wave; t < tnx * tny; t += nw) {
-    const int ox = (t % tnx) * 16 - desl, oy = (t / tnx) * 16 - desl;
+// one smoothing pass of an LDS-resident level: S -> W, every wave owning whole tiles
+template <int C0>
+__device__ __forceinline__ void wg_pass(const float* S, float* W, const float* B, const uint8_t* M, int nx, int ny, int desl,
+                               float* T) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ntx = (nx + desl + 15) >> 4, nty = (ny + desl + 15) >> 4;
+  for (int t = wave; t < ntx * nty; t += WG_WAVES) {
+    const int ty = t / ntx, tx = t - ty * ntx;
+    const int ox = tx * 16 - desl, oy = ty * 16 - desl;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int e = lane + 64 * k, i = ox + (e & 15), j = oy + (e >> 4);
-      if (i >= 0 && j >= 0 && i < nx && j < ny) W[i + j * nx] = S[i + j * nx];
+    for (int q = 0; q < 6; q++) {
+      const int e = lane + 64 * q;
+      if (e < TCELLS) {
+        const int ly = e / TS, lx = e - ly * TS;
+        const int i = ox - 1 + lx, j = oy - 1 + ly;
+        T[e] = (i >= 0 && j >= 0 && i < nx && j < ny) ? S[i + j * nx] : 0.f;
+      }
     }
     wave_sync();
-    for (int it = 0; it < 4; it++)
-      for (int o = 0; o < 2; o++) {
-        const int cor = o ? c1 : c0;
+    Cells4 cc;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const int e = lane + 64 * k, i = ox + (e & 15), j = oy + (e >> 4);
-          if (i < 0 || j < 0 || i >= nx || j >= ny || ((i + j + 1) & 1) != cor) continue;
-          const int c = i + j * nx;
-          const uint8_t m = M[c];
-          const int kk = mask_cnt(m);
-          if (mask_type(m) != AGUA || kk == 0) continue;
-          const bool li = i - 1 >= ox, ri = i + 1 < ox + 16, di = j - 1 >= oy, ui = j + 1 < oy + 16;
-          float s = 0.f;
-          if (m & 1) s += li ? W[c - 1] : S[c - 1];
-          if (m & 2) s += ri ? W[c + 1] : S[c + 1];
-          if (m & 4) s += di ? W[c - nx] : S[c - nx];
-          if (m & 8) s += ui ? W[c + nx] : S[c + nx];
-          const float qv = W[c];
-          W[c] = qv + ((s + B[c]) / (float)kk - qv);
-        }
-        wave_sync();
+    for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+      for (int rw = 0; rw < 2; rw++) {
+        const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
+        if (i >= 0 && j >= 0 && i < nx && j < ny) cell_setup(cc, cq, rw, M[i + j * nx], B[i + j * nx]);
+        else cell_clear(cc, cq, rw);
+        cc.v[cq][rw] = T[cell_e(lane, cq, rw)];
       }
+    tile_sweeps<C0>(T, cc, lane);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int e = lane + 64 * k, i = ox + (e & 15), j = oy + (e >> 4);
-      if (i >= 0 && j >= 0 && i < nx && j < ny && mask_type(M[i + j * nx]) != AGUA) W[i + j * nx] = 0.f;
-    }
+    for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+      for (int rw = 0; rw < 2; rw++) {
+        const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
+        if (i >= 0 && j >= 0 && i < nx && j < ny) W[i + j * nx] = (cc.m[cq][rw] & 32) ? cc.v[cq][rw] : 0.f;
+      }
     wave_sync();
   }
   __syncthreads();
 }
 
-__global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __restrict__ b_lc, float* __restrict__ x_lc,
+// Levels lc..L-1 from LDS.  Level lc arrives with b and (unless it is the coarsest) x after
+// its pass A; the result of level lc's V-cycle is written to out_lc.
+__global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __restrict__ b_lc,
+                                                    const float* __restrict__ x_lc, float* __restrict__ out_lc,
                                                     const DevScalars* __restrict__ sc) {
   if (sc->done) return;
-  __shared__ __attribute__((aligned(16))) char pool[WG_LDS];
+  __shared__ __attribute__((aligned(16))) char pool[WG_POOL];
+  __shared__ float tiles[WG_WAVES][TCELLS];
+  float* T = tiles[threadIdx.x >> 6];
   // level arrays live at plan offsets; bit l of `sw` says x and tmp of level l are swapped
   unsigned sw = 0;
   auto Xp = [&](int l) { return (float*)(pool + (((sw >> l) & 1) ? P.offt[l] : P.offx[l])); };
   auto Tp = [&](int l) { return (float*)(pool + (((sw >> l) & 1) ? P.offx[l] : P.offt[l])); };
   auto Bp = [&](int l) { return (float*)(pool + P.offb[l]); };
   auto Mp = [&](int l) { return (uint8_t*)(pool + P.offm[l]); };
+  const bool xin = P.lc < P.L - 1;
   for (int l = P.lc; l < P.L; l++) {
     const int n = P.nx[l] * P.ny[l];
     float* X = Xp(l);
@@ -666,28 +1000,44 @@ __global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __res
     const uint8_t* gm = P.mask[l];
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
       M[c] = gm[c];
-      X[c] = 0.f;
-      if (l == P.lc) B[c] = b_lc[c];
+      X[c] = (l == P.lc && xin) ? x_lc[c] : 0.f;
+      B[c] = l == P.lc ? b_lc[c] : 0.f;
     }
   }
   __syncthreads();
   // ---- down
   for (int l = P.lc; l < P.L - 1; l++) {
     const int nx = P.nx[l], ny = P.ny[l];
-    wg_smooth(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 0, 0, 1);
+    if (l > P.lc) {  // pass A (level lc's ran before this kernel)
+      wg_pass<0>(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 0, T);
+      sw ^= 1u << l;
+    }
+    wg_pass<0>(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 8, T);
     sw ^= 1u << l;
-    wg_smooth(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 8, 0, 1);
-    sw ^= 1u << l;
-    // restriction to l+1 (b = P^T r, x = 0)
+    // restriction to l+1: fine residual into the free array, then b = P^T r, x = 0
     const float* X = Xp(l);
     const float* B = Bp(l);
     const uint8_t* M = Mp(l);
+    float* R = Tp(l);
+    for (int c = threadIdx.x; c < nx * ny; c += blockDim.x) {
+      const uint8_t m = M[c];
+      float rr = 0.f;
+      if (mask_type(m) == AGUA) {
+        float s = (m & 1) ? X[c - 1] : 0.f;
+        s += (m & 2) ? X[c + 1] : 0.f;
+        s += (m & 4) ? X[c - nx] : 0.f;
+        s += (m & 8) ? X[c + nx] : 0.f;
+        rr = B[c] - ((float)mask_cnt(m) * X[c] - s);
+      }
+      R[c] = rr;
+    }
+    __syncthreads();
     float* Bc = Bp(l + 1);
     float* Xc = Xp(l + 1);
     const uint8_t* Mc = Mp(l + 1);
     const int cnx = P.nx[l + 1], cny = P.ny[l + 1];
     for (int C = threadIdx.x; C < cnx * cny; C += blockDim.x) {
-      const int I = C % cnx, J = C / cnx;
+      const int J = C / cnx, I = C - J * cnx;
       float acc = 0.f;
       if (mask_type(Mc[C]) == AGUA) {
         const float w[4] = {0.25f, 0.75f, 0.75f, 0.25f};
@@ -698,15 +1048,7 @@ __global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __res
           for (int bi = 0; bi < 4; bi++) {
             const int i = 2 * I - 1 + bi;
             if (i < 0 || i >= nx) continue;
-            const int c = i + j * nx;
-            const uint8_t m = M[c];
-            if (mask_type(m) != AGUA) continue;
-            float s = 0.f;
-            if (m & 1) s += X[c - 1];
-            if (m & 2) s += X[c + 1];
-            if (m & 4) s += X[c - nx];
-            if (m & 8) s += X[c + nx];
-            racc += (B[c] - ((float)mask_cnt(m) * X[c] - s)) * w[bi];
+            racc += R[i + j * nx] * w[bi];
           }
           acc += racc * w[bj];
         }
@@ -716,32 +1058,72 @@ __global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __res
     }
     __syncthreads();
   }
-  // ---- coarsest: 64 red-black sweeps in place, colour order alternating (resolverGrosso)
+  // ---- coarsest: 64 red-black sweeps in place, colour order alternating (resolverGrosso,
+  // pressao-cpu.js:217-238)
   {
-    const int l = P.L - 1, nx = P.nx[l], ny = P.ny[l], n = nx * ny;
+    const int l = P.L - 1, nx = P.nx[l], n = nx * P.ny[l];
     float* x = Xp(l);
     const float* b = Bp(l);
     const uint8_t* M = Mp(l);
-    const bool single_wave = n <= 64 * 32;
-    if (!single_wave || threadIdx.x < 64) {
-      const int stride = single_wave ? 64 : (int)blockDim.x;
-      for (int k = 0; k < 64; k++)
-        for (int o = 0; o < 2; o++) {
-          const int cor = (k & 1) ? 1 - o : o;
-          for (int c = threadIdx.x; c < n; c += stride) {
-            const int i = c % nx, j = c / nx;
-            const uint8_t m = M[c];
-            if (mask_type(m) != AGUA || ((i + j + 1) & 1) != cor) continue;
-            const int q = mask_cnt(m);
-            if (!q) continue;
-            float s = 0.f;
-            if (m & 1) s += x[c - 1];
-            if (m & 2) s += x[c + 1];
-            if (m & 4) s += x[c - nx];
-            if (m & 8) s += x[c + nx];
-            x[c] = (s + b[c]) / (float)q;
+    if (P.dense) {
+      // x = G b with G the 64-sweep operator of this solve (k_coarsest_op)
+      const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+      for (int c = wave; c < n; c += WG_WAVES) {
+        float acc = 0.f;
+        for (int j = lane; j < n; j += 64) acc += P.dense[(size_t)c * n + j] * b[j];
+        acc = wave_sumf(acc);
+        if (lane == 0) x[c] = acc;
+      }
+    } else if (n <= 4 * 64) {
+      // one wavefront, each lane's (up to) four cells in registers; neighbours from LDS
+      if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        int mk[4];
+        float bk[4], ikk[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int c = lane + 64 * k;
+          const uint8_t m = c < n ? M[c] : 0;
+          const int q = mask_cnt(m);
+          const bool act = mask_type(m) == AGUA && q > 0;
+          mk[k] = act ? (mask_viz(m) | 16 | ((m & MK_COLOUR) ? 32 : 0)) : 0;
+          bk[k] = act ? b[c] : 0.f;
+          ikk[k] = act ? 1.f / (float)q : 0.f;
+        }
+        for (int k2 = 0; k2 < 64; k2++)
+#pragma unroll
+          for (int o = 0; o < 2; o++) {
+            const int cor = (k2 & 1) ? 1 - o : o;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+              const int c = lane + 64 * k;
+              const int m = mk[k];
+              if ((m & 16) && ((m >> 5) & 1) == cor) {
+                float sum = (m & 1) ? x[c - 1] : 0.f;
+                sum += (m & 2) ? x[c + 1] : 0.f;
+                sum += (m & 4) ? x[c - nx] : 0.f;
+                sum += (m & 8) ? x[c + nx] : 0.f;
+                x[c] = (sum + bk[k]) * ikk[k];
+              }
+            }
+            wave_sync();
           }
-          if (single_wave) wave_sync(); else __syncthreads();
+      }
+    } else {
+      for (int k2 = 0; k2 < 64; k2++)
+        for (int o = 0; o < 2; o++) {
+          const int cor = (k2 & 1) ? 1 - o : o;
+          for (int c = threadIdx.x; c < n; c += blockDim.x) {
+            const uint8_t m = M[c];
+            const int q = mask_cnt(m);
+            if (mask_type(m) != AGUA || ((m & MK_COLOUR) ? 1 : 0) != cor || !q) continue;
+            float sum = (m & 1) ? x[c - 1] : 0.f;
+            sum += (m & 2) ? x[c + 1] : 0.f;
+            sum += (m & 4) ? x[c - nx] : 0.f;
+            sum += (m & 8) ? x[c + nx] : 0.f;
+            x[c] = (sum + b[c]) / (float)q;
+          }
+          __syncthreads();
         }
     }
     __syncthreads();
@@ -755,7 +1137,7 @@ __global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __res
     const uint8_t* Mc = Mp(l + 1);
     for (int c = threadIdx.x; c < nx * ny; c += blockDim.x) {
       if (mask_type(M[c]) != AGUA) continue;
-      const int i = c % nx, j = c / nx;
+      const int j = c / nx, i = c - j * nx;
       const int I0 = (i & 1) == 0 ? (i >> 1) - 1 : (i - 1) >> 1;
       const int J0 = (j & 1) == 0 ? (j >> 1) - 1 : (j - 1) >> 1;
       const float wx0 = (i & 1) == 0 ? 0.25f : 0.75f, wy0 = (j & 1) == 0 ? 0.25f : 0.75f;
@@ -774,53 +1156,29 @@ __global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __res
       X[c] += e;
     }
     __syncthreads();
-    wg_smooth(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 8, 1, 0);
+    wg_pass<1>(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 8, T);
     sw ^= 1u << l;
-    wg_smooth(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 0, 1, 0);
+    wg_pass<1>(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 0, T);
     sw ^= 1u << l;
   }
   const int n = P.nx[P.lc] * P.ny[P.lc];
   const float* X = Xp(P.lc);
-  for (int c = threadIdx.x; c < n; c += blockDim.x) x_lc[c] = X[c];
+  for (int c = threadIdx.x; c < n; c += blockDim.x) out_lc[c] = X[c];
 }
 
-// z and the dots when level 0 itself is solved inside the workgroup (small grids)
-__global__ __launch_bounds__(256) void k_zdots0(const float* __restrict__ x0, ZArgs za, int C,
-                                                const DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ double sh[BSLOTS * 16];
-  const int na = sc->active_bubbles;
-  double dv[2] = {0, 0};
-  double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-  GRID_STRIDE(c, C) {
-    const float xv = x0[c];
-    float z = 0.f;
-    if (mask_type(za.opMask[c]) == AGUA) {
-      z = xv;
-      dv[0] += (double)za.r[c] * z;
-      dv[1] += (double)za.r[c] * za.zold[c];
-    } else {
-      uint32_t lb;
-      if (is_bub_cell(za.tipo, za.rot, za.facesB, za.ar, c, &lb))
-        bub_add(sl, za.labelSlot, bv(za.bvec, BV_ACC1), lb, (double)xv / za.cellsB[lb]);
-    }
-    za.znew[c] = z;
-  }
-  write_partials<2>(dv, za.partials, sh);
-  if (na > 0) write_partials<BSLOTS>(sl, za.bpart, sh);
-}
-
-__global__ __launch_bounds__(256) void k_fin_zdots(double* __restrict__ bvec, const int* __restrict__ activeList,
-                                                   const int* __restrict__ labelSlot, const double* __restrict__ partials,
-                                                   const double* __restrict__ bpart, int nb, int first,
-                                                   DevScalars* __restrict__ sc) {
+// ================================================================== z dots -> beta
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_zdots(double* __restrict__ bvec, const int* __restrict__ activeList,
+                                                           const int* __restrict__ labelSlot,
+                                                           const double* __restrict__ partials,
+                                                           const double* __restrict__ bpart, int nb, int first,
+                                                           DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], slotTot[BSLOTS];
   const int na = sc->active_bubbles;
   double tot[2];
-  read_partials<2>(partials, nb, tot, sh, res);
-  double stt[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (na > 0) read_partials<BSLOTS>(bpart, nb, stt, sh, res);
+  sum_partials<2>(partials, nb, 2, tot, sh, res);
+  double stt[BSLOTS];
+  sum_partials<BSLOTS>(bpart, nb, nslots(na), stt, sh, res);
   if (threadIdx.x == 0)
     for (int k = 0; k < BSLOTS; k++) slotTot[k] = stt[k];
   __syncthreads();
@@ -844,75 +1202,90 @@ __global__ __launch_bounds__(256) void k_fin_zdots(double* __restrict__ bvec, co
   if (threadIdx.x == 0) {
     const double rzNovo = tot[0] + bacc[0];
     const double cruzado = tot[1] + bacc[1];
+    // Polak-Ribiere beta (pressao-cpu.js:390-394)
     if (first) sc->beta = 0;
     else {
       const double rz = sc->rz;
       sc->beta = fabs(rz) < 1e-30 ? 0 : fmax(0.0, (rzNovo - cruzado) / rz);
     }
     sc->rz = rzNovo;
+    sc->mg_scale = sc->mg_scale_next;
   }
 }
 
 // ================================================================== p = z + beta p, Ap, p.Ap
-__global__ __launch_bounds__(256) void k_spmv(const uint8_t* __restrict__ opMask, const uint8_t* __restrict__ tipo,
-                                              const uint32_t* __restrict__ rot, const uint32_t* __restrict__ facesB,
+__global__ __launch_bounds__(256) void k_spmv(const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
                                               const float* __restrict__ z, const float* __restrict__ pold,
-                                              float* __restrict__ pnew, float* __restrict__ ap, int nx, int ny, int ar,
-                                              double* __restrict__ bvec, const int* __restrict__ labelSlot,
+                                              float* __restrict__ pnew, float* __restrict__ ap, int nx, int C,
+                                              const double* __restrict__ bvec, const int* __restrict__ labelSlot,
                                               double* __restrict__ partials, double* __restrict__ bpart,
                                               const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[BSLOTS * 16];
-  const int C = nx * ny;
   const double beta = sc->beta;
-  const int na = sc->active_bubbles;
-  const double* zb = bv(bvec, BV_Z);
-  const double* pb = bv(bvec, BV_P);
-  double* acc0 = bv(bvec, BV_ACC0);
+  const int ns = nslots(sc->active_bubbles);
+  const double* zb = bv((double*)bvec, BV_Z);
+  const double* pb = bv((double*)bvec, BV_P);
+  double* acc0 = bv((double*)bvec, BV_ACC0);
   double v[1] = {0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-  auto peff = [&](size_t d) -> float {
-    if (mask_type(opMask[d]) == AGUA) return (float)((double)z[d] + beta * (double)pold[d]);
-    uint32_t lb;
-    if (is_bub_cell(tipo, rot, facesB, ar, d, &lb)) return (float)(zb[lb] + beta * pb[lb]);
+  // p of a cell from its loaded z, p_old and mask (bubble cells from the bubble vectors)
+  auto peff = [&](int d, uint8_t md, float zd, float pd) -> float {
+    if (mask_type(md) == AGUA) return (float)((double)zd + beta * (double)pd);
+    if (md & MK_BUB) {
+      const uint32_t lb = rot[d];
+      return (float)(zb[lb] + beta * pb[lb]);
+    }
     return 0.f;
   };
-  GRID_STRIDE(c, C) {
-    const float pc = peff(c);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < C) {
+    // every load first (clamped indices; masks decide what is used)
+    int nb[5] = {c, c - 1, c + 1, c - nx, c + nx};
+    uint8_t mk[5];
+    float zv[5], pv[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      nb[k] = nb[k] < 0 ? 0 : (nb[k] >= C ? C - 1 : nb[k]);
+      mk[k] = opMask[nb[k]];
+      zv[k] = z[nb[k]];
+      pv[k] = pold[nb[k]];
+    }
+    const uint8_t m = mk[0];
+    const float pc = peff(c, m, zv[0], pv[0]);
     pnew[c] = pc;
-    const uint8_t m = opMask[c];
     if (mask_type(m) == AGUA) {
       double s = 0;
-      const size_t nb[4] = {(size_t)c - 1, (size_t)c + 1, (size_t)c - nx, (size_t)c + nx};
+#pragma unroll
       for (int k = 0; k < 4; k++) {
         if (!(m & (1 << k))) continue;
-        s += (double)peff(nb[k]);
-        uint32_t lb;
-        if (na > 0 && is_bub_cell(tipo, rot, facesB, ar, nb[k], &lb) &&
-            interior((int)(nb[k] % nx), (int)(nb[k] / nx), nx, ny))
-          bub_add(sl, labelSlot, acc0, lb, -(double)pc);
+        const uint8_t md = mk[k + 1];
+        s += (double)peff(nb[k + 1], md, zv[k + 1], pv[k + 1]);
+        if (md & MK_BUBIN) bub_add(sl, labelSlot, acc0, rot[nb[k + 1]], -(double)pc);
       }
       const float a = (float)(mask_cnt(m) * (double)pc - s);
       ap[c] = a;
-      v[0] += (double)pc * a;
+      v[0] = (double)pc * a;
     } else ap[c] = 0.f;
   }
-  write_partials<1>(v, partials, sh);
-  if (na > 0) write_partials<BSLOTS>(sl, bpart, sh);
+  store_partials<1>(v, 1, partials, sh);
+  if (ns > 0) store_partials<BSLOTS>(sl, ns, bpart, sh);
 }
 
-__global__ __launch_bounds__(256) void k_fin_spmv(const uint32_t* __restrict__ facesB, double* __restrict__ bvec,
-                                                  const int* __restrict__ activeList, const int* __restrict__ labelSlot,
-                                                  const double* __restrict__ partials, const double* __restrict__ bpart,
-                                                  int nb, DevScalars* __restrict__ sc) {
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_spmv(const uint32_t* __restrict__ facesB, double* __restrict__ bvec,
+                                                          const int* __restrict__ activeList,
+                                                          const int* __restrict__ labelSlot,
+                                                          const double* __restrict__ partials,
+                                                          const double* __restrict__ bpart, int nb,
+                                                          DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], slotTot[BSLOTS];
   __shared__ double alpha_s;
   const int na = sc->active_bubbles;
   double tot[1];
-  read_partials<1>(partials, nb, tot, sh, res);
-  double stt[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (na > 0) read_partials<BSLOTS>(bpart, nb, stt, sh, res);
+  sum_partials<1>(partials, nb, 1, tot, sh, res);
+  double stt[BSLOTS];
+  sum_partials<BSLOTS>(bpart, nb, nslots(na), stt, sh, res);
   if (threadIdx.x == 0)
     for (int k = 0; k < BSLOTS; k++) slotTot[k] = stt[k];
   __syncthreads();
@@ -954,31 +1327,12 @@ __global__ __launch_bounds__(256) void k_fin_spmv(const uint32_t* __restrict__ f
   if (threadIdx.x == 0) sc->pap = rbs[0];  // bubble part of |r|^2 for fin_update
 }
 
-// ================================================================== q += alpha p, r -= alpha Ap
-__global__ __launch_bounds__(256) void k_update(const uint8_t* __restrict__ opMask, const float* __restrict__ p,
-                                                const float* __restrict__ ap, float* __restrict__ q,
-                                                float* __restrict__ r, int C, double* __restrict__ partials,
-                                                const DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ double sh[16];
-  const double alpha = sc->alpha;
-  double v[1] = {0};
-  GRID_STRIDE(c, C) {
-    if (mask_type(opMask[c]) != AGUA) continue;
-    q[c] = (float)((double)q[c] + alpha * (double)p[c]);
-    const float rn = (float)((double)r[c] - alpha * (double)ap[c]);
-    r[c] = rn;
-    v[0] += (double)rn * rn;
-  }
-  write_partials<1>(v, partials, sh);
-}
-
-__global__ __launch_bounds__(256) void k_fin_update(const double* __restrict__ partials, int nb,
-                                                    DevScalars* __restrict__ sc) {
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_update(const double* __restrict__ partials, int nb,
+                                                            DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[16], res[1];
   double tot[1];
-  read_partials<1>(partials, nb, tot, sh, res);
+  sum_partials<1>(partials, nb, 1, tot, sh, res);
   if (threadIdx.x == 0) {
     const double n2 = tot[0] + sc->pap;
     const double nr = sqrt(n2);
@@ -989,7 +1343,7 @@ __global__ __launch_bounds__(256) void k_fin_update(const double* __restrict__ p
     bool stop = sc->iters >= sc->maxit;
     if (sc->tol > 0 && (nr == 0 || nr <= sc->tol * sc->normb)) stop = true;
     sc->done = stop ? 1 : 0;
-    sc->mg_scale = nr > 0 ? (float)(1.0 / nr) : 1.f;
+    sc->mg_scale_next = nr > 0 ? (float)(1.0 / nr) : 1.f;
   }
 }
 
@@ -1007,58 +1361,58 @@ __global__ __launch_bounds__(1024) void k_pcg_final_b(const double* __restrict__
 }
 
 // true residual ||b' - A[q;y]|| (diagnostics; b' is the projected rhs that was solved)
-__global__ __launch_bounds__(256) void k_true_res(const uint8_t* __restrict__ opMask, const uint8_t* __restrict__ tipo,
-                                                  const uint32_t* __restrict__ rot, const uint32_t* __restrict__ facesB,
+__global__ __launch_bounds__(256) void k_true_res(const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
                                                   const float* __restrict__ div, const float* __restrict__ q, int nx,
-                                                  int ny, int ar, double* __restrict__ bvec,
-                                                  const int* __restrict__ labelSlot, double* __restrict__ partials,
-                                                  double* __restrict__ bpart, const DevScalars* __restrict__ sc) {
+                                                  int C, double* __restrict__ bvec, const int* __restrict__ labelSlot,
+                                                  double* __restrict__ partials, double* __restrict__ bpart,
+                                                  const DevScalars* __restrict__ sc) {
   __shared__ double sh[BSLOTS * 16];
-  const int C = nx * ny;
-  const int na = sc->active_bubbles;
+  const int ns = nslots(sc->active_bubbles);
   const double* yb = bv(bvec, BV_Y);
   double* acc0 = bv(bvec, BV_ACC0);
   const double mean = sc->proj_mean;
   double v[1] = {0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-  auto qe = [&](size_t d) -> float {
-    if (tipo[d] == AGUA) return q[d];
-    uint32_t lb;
-    if (is_bub_cell(tipo, rot, facesB, ar, d, &lb)) return (float)yb[lb];
+  auto qe = [&](int d, uint8_t md) -> float {
+    if (mask_type(md) == AGUA) return q[d];
+    if (md & MK_BUB) return (float)yb[rot[d]];
     return 0.f;
   };
-  GRID_STRIDE(c, C) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < C) {
     const uint8_t m = opMask[c];
-    if (mask_type(m) != AGUA) continue;
-    const float qc = qe(c);
-    double s = 0;
-    const size_t nb[4] = {(size_t)c - 1, (size_t)c + 1, (size_t)c - nx, (size_t)c + nx};
-    for (int k = 0; k < 4; k++) {
-      if (!(m & (1 << k))) continue;
-      s += (double)qe(nb[k]);
-      uint32_t lb;
-      if (is_bub_cell(tipo, rot, facesB, ar, nb[k], &lb) && interior((int)(nb[k] % nx), (int)(nb[k] / nx), nx, ny))
-        bub_add(sl, labelSlot, acc0, lb, -(double)qc);
+    if (mask_type(m) == AGUA) {
+      const float qc = qe(c, m);
+      double s = 0;
+      const int nb[4] = {c - 1, c + 1, c - nx, c + nx};
+      for (int k = 0; k < 4; k++) {
+        if (!(m & (1 << k))) continue;
+        const uint8_t md = opMask[nb[k]];
+        s += (double)qe(nb[k], md);
+        if (md & MK_BUBIN) bub_add(sl, labelSlot, acc0, rot[nb[k]], -(double)qc);
+      }
+      const double e = (-(double)div[c] - mean) - (mask_cnt(m) * (double)qc - s);
+      v[0] = e * e;
     }
-    const double e = (-(double)div[c] - mean) - (mask_cnt(m) * (double)qc - s);
-    v[0] += e * e;
   }
-  write_partials<1>(v, partials, sh);
-  if (na > 0) write_partials<BSLOTS>(sl, bpart, sh);
+  store_partials<1>(v, 1, partials, sh);
+  if (ns > 0) store_partials<BSLOTS>(sl, ns, bpart, sh);
 }
 
-__global__ __launch_bounds__(256) void k_fin_true_res(const uint32_t* __restrict__ facesB, double* __restrict__ bvec,
-                                                      const double* __restrict__ rhsB, const int* __restrict__ activeList,
-                                                      const int* __restrict__ labelSlot,
-                                                      const double* __restrict__ partials,
-                                                      const double* __restrict__ bpart, int nb,
-                                                      const DevScalars* __restrict__ sc, double* __restrict__ out) {
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_true_res(const uint32_t* __restrict__ facesB,
+                                                              double* __restrict__ bvec, const double* __restrict__ rhsB,
+                                                              const int* __restrict__ activeList,
+                                                              const int* __restrict__ labelSlot,
+                                                              const double* __restrict__ partials,
+                                                              const double* __restrict__ bpart, int nb,
+                                                              const DevScalars* __restrict__ sc,
+                                                              double* __restrict__ out) {
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], slotTot[BSLOTS];
   const int na = sc->active_bubbles;
   double tot[1];
-  read_partials<1>(partials, nb, tot, sh, res);
-  double stt[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (na > 0) read_partials<BSLOTS>(bpart, nb, stt, sh, res);
+  sum_partials<1>(partials, nb, 1, tot, sh, res);
+  double stt[BSLOTS];
+  sum_partials<BSLOTS>(bpart, nb, nslots(na), stt, sh, res);
   if (threadIdx.x == 0)
     for (int k = 0; k < BSLOTS; k++) slotTot[k] = stt[k];
   __syncthreads();
@@ -1079,6 +1433,8 @@ __global__ __launch_bounds__(256) void k_fin_true_res(const uint32_t* __restrict
 }
 
 // ================================================================== host side
+static long wg_bytes(int n) { return 12L * n + ((n + 15) & ~15); }
+
 int solver_alloc(tq_sim* s) {
   s->lv.clear();
   {
@@ -1101,126 +1457,210 @@ int solver_alloc(tq_sim* s) {
     TQ_HIP(hipMalloc(&L.x, sizeof(float) * L.n));
     TQ_HIP(hipMalloc(&L.tmp, sizeof(float) * L.n));
   }
-  // first level kept in LDS: the suffix of levels whose arrays fit the workgroup pool
+  // the one-workgroup suffix: every level from lc on must fit the LDS pool
   const int L = (int)s->lv.size();
-  long cells = 0, bytes = 0;
-  int lc = L;
+  long bytes = 0;
+  int fit = L;
   for (int l = L - 1; l >= 0; l--) {
-    const long add = 12L * s->lv[l].n + ((s->lv[l].n + 15) & ~15);
-    if (bytes + add > WG_LDS) break;
-    bytes += add;
-    cells += s->lv[l].n;
-    lc = l;
+    if (bytes + wg_bytes(s->lv[l].n) > WG_POOL) break;
+    bytes += wg_bytes(s->lv[l].n);
+    fit = l;
   }
-  if (lc == L) {
+  if (fit == L) {
     set_error("grid too elongated: the coarsest multigrid level does not fit the one-workgroup solver");
     return TQ_E_CAPACITY;
   }
-  s->lc = std::max(lc, std::min(s->d.mg_min_lds_level, L - 1));
+  int lc = fit;
+  if (s->d.mg_min_lds_level <= 0) {
+    while (lc < L - 1 && s->lv[lc].n > WG_AUTO_CELLS) lc++;
+  } else {
+    lc = std::max(fit, std::min(s->d.mg_min_lds_level, L - 1));
+  }
+  s->lc = lc;
+  s->dense = nullptr;
+  if (s->lv.back().n <= DENSE_MAX) TQ_HIP(hipMalloc(&s->dense, sizeof(float) * DENSE_MAX * DENSE_MAX));
   TQ_HIP(hipMalloc(&s->bpart, sizeof(double) * BSLOTS * MAX_RED_BLOCKS));
   TQ_HIP(hipMalloc(&s->labelSlot, sizeof(int) * MAX_BOLHAS));
   return 0;
 }
 
+static void graph_drop(tq_sim* s) {
+  if (s->graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)s->graph_exec);
+  s->graph_exec = nullptr;
+  s->graph_key = -1;
+}
+
 void solver_free(tq_sim* s) {
+  graph_drop(s);
   for (auto& L : s->lv) {
     (void)hipFree(L.mask); (void)hipFree(L.b); (void)hipFree(L.x); (void)hipFree(L.tmp);
   }
   s->lv.clear();
   if (s->bpart) (void)hipFree(s->bpart);
   if (s->labelSlot) (void)hipFree(s->labelSlot);
+  if (s->dense) (void)hipFree(s->dense);
+  s->dense = nullptr;
   s->bpart = nullptr;
   s->labelSlot = nullptr;
 }
 
-// per-level storage for this solve (mixed: level 0 keeps x in FP32, large levels use FP16)
-static void plan_precision(tq_sim* s) {
+// per-level storage for this solve: mixed stores x and b of the multi-workgroup levels from
+// mg_half_level on in FP16 (level 0: b only); small levels and the LDS levels stay FP32
+static bool plan_precision(tq_sim* s) {
   const bool mixed = s->d.precision == TQ_PREC_MIXED;
+  const int h0 = s->d.mg_half_level < 0 ? 1 : s->d.mg_half_level;
+  bool any = false;
   for (int l = 0; l < (int)s->lv.size(); l++) {
     Level& L = s->lv[l];
-    const bool big = mixed && l < s->lc && L.n >= 4096;
-    L.hb = big;
-    L.hx = big && l > 0;
+    const bool half = mixed && l < s->lc && l >= h0 && L.n >= 4096;
+    L.hb = half;
+    L.hx = half && l > 0;
+    any = any || half;
   }
+  return any;
 }
 
-static int nbr_blocks(int C) { return std::max(1, std::min(NBR, (C + 255) / 256)); }
+static int tiles_of(int n, int desl) { return (n + desl + 15) / 16; }
+static dim3 tile_grid(int nx, int ny, int desl) {
+  return dim3((tiles_of(nx, desl) + 1) / 2, (tiles_of(ny, desl) + 1) / 2);
+}
+static int blocks_of(int C) { return (C + 255) / 256; }
 
-static void launch_down(tq_sim* s, int l) {
-  Level& F = s->lv[l];
-  dim3 g((F.nx + 8 + 15) / 16, (F.ny + 8 + 15) / 16);
-  int t = time_begin(s, l == 0 ? "mg_down_l0" : "mg_down", ((F.hx ? 2 : 4) + (F.hb ? 2 : 4) + 1) * (double)F.n);
-  if (!F.hx && !F.hb)
-    hipLaunchKernelGGL((k_down<float, float>), g, dim3(256), 0, s->st, (float*)F.x, (const float*)F.b, F.mask, F.nx,
-                       F.ny, s->sc);
-  else if (!F.hx)
-    hipLaunchKernelGGL((k_down<float, __half>), g, dim3(256), 0, s->st, (float*)F.x, (const __half*)F.b, F.mask, F.nx,
-                       F.ny, s->sc);
-  else
-    hipLaunchKernelGGL((k_down<__half, __half>), g, dim3(256), 0, s->st, (__half*)F.x, (const __half*)F.b, F.mask, F.nx,
-                       F.ny, s->sc);
+static double vbytes(bool h) { return h ? 2.0 : 4.0; }
+
+static void launch_in_a0(tq_sim* s, bool upd) {
+  Level& F = s->lv[0];
+  const int ntx = tiles_of(F.nx, 0), nty = tiles_of(F.ny, 0);
+  const dim3 g = tile_grid(F.nx, F.ny, 0);
+  const bool pa = s->lv.size() > 1;
+  const float* pp = s->pbuf[s->pc];
+  // bytes: opMask, mg0, b0 write, x0 write; update: q, p, r, Ap read + q, r write
+  const double by = (2 + vbytes(F.hb) + (pa ? 4 : 0) + (upd ? 24 : 4)) * (double)F.n;
+  int t = time_begin(s, upd ? "pcg_update_mg_in_l0" : "pcg_mg_in_l0", by);
+#define TQ_IN_A0(SB, U, P)                                                                                        \
+  hipLaunchKernelGGL((k_in_a0<SB, U, P>), g, dim3(256), 0, s->st, s->opMask, F.mask, s->rot, s->cellsB, pp, s->ap, \
+                     s->q, s->r, F.nx, F.ny, ntx, nty, s->bvec, (SB*)F.b, (float*)F.x, s->partials, s->mg_scaled, s->sc)
+  if (F.hb) {
+    if (upd) { if (pa) TQ_IN_A0(__half, true, true); else TQ_IN_A0(__half, true, false); }
+    else { if (pa) TQ_IN_A0(__half, false, true); else TQ_IN_A0(__half, false, false); }
+  } else {
+    if (upd) { if (pa) TQ_IN_A0(float, true, true); else TQ_IN_A0(float, true, false); }
+    else { if (pa) TQ_IN_A0(float, false, true); else TQ_IN_A0(float, false, false); }
+  }
+#undef TQ_IN_A0
   time_end(s, t);
 }
 
-template <class SXF, class SBF>
-static void restrict_c(tq_sim* s, Level& F, Level& G, dim3 g) {
-  if (G.hb)
-    hipLaunchKernelGGL((k_restrict<SXF, SBF, __half>), g, dim3(256), 0, s->st, (const SXF*)F.x, (const SBF*)F.b, F.mask,
-                       F.nx, F.ny, (__half*)G.b, G.mask, G.nx, G.ny, s->sc);
+// pass B (offset 8, colours 0,1): x -> tmp
+static void launch_pass_b(tq_sim* s, int l) {
+  Level& F = s->lv[l];
+  const int ntx = tiles_of(F.nx, 8), nty = tiles_of(F.ny, 8);
+  const dim3 g = tile_grid(F.nx, F.ny, 8);
+  int t = time_begin(s, l == 0 ? "mg_pass_b_l0" : "mg_pass_b", (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n);
+  if (F.hx)
+    hipLaunchKernelGGL((k_pass<__half, __half, 0>), g, dim3(256), 0, s->st, (const __half*)F.x, (__half*)F.tmp,
+                       (const __half*)F.b, F.mask, F.nx, F.ny, 8, ntx, nty, s->sc);
+  else if (F.hb)
+    hipLaunchKernelGGL((k_pass<float, __half, 0>), g, dim3(256), 0, s->st, (const float*)F.x, (float*)F.tmp,
+                       (const __half*)F.b, F.mask, F.nx, F.ny, 8, ntx, nty, s->sc);
   else
-    hipLaunchKernelGGL((k_restrict<SXF, SBF, float>), g, dim3(256), 0, s->st, (const SXF*)F.x, (const SBF*)F.b, F.mask,
-                       F.nx, F.ny, (float*)G.b, G.mask, G.nx, G.ny, s->sc);
+    hipLaunchKernelGGL((k_pass<float, float, 0>), g, dim3(256), 0, s->st, (const float*)F.x, (float*)F.tmp,
+                       (const float*)F.b, F.mask, F.nx, F.ny, 8, ntx, nty, s->sc);
+  time_end(s, t);
 }
 
-static void launch_restrict(tq_sim* s, int l) {
+// pass D (offset 0, colours 1,0): x -> tmp (levels > 0)
+static void launch_pass_d(tq_sim* s, int l) {
   Level& F = s->lv[l];
-  Level& G = s->lv[l + 1];
-  dim3 g((G.nx + 15) / 16, (G.ny + 15) / 16);
-  int t = time_begin(s, l == 0 ? "mg_restrict_l0" : "mg_restrict",
-                     ((F.hx ? 2 : 4) + (F.hb ? 2 : 4) + 1) * (double)F.n + (G.hb ? 2 : 4) * (double)G.n);
-  if (!F.hx && !F.hb) restrict_c<float, float>(s, F, G, g);
-  else if (!F.hx) restrict_c<float, __half>(s, F, G, g);
-  else restrict_c<__half, __half>(s, F, G, g);
+  const int ntx = tiles_of(F.nx, 0), nty = tiles_of(F.ny, 0);
+  const dim3 g = tile_grid(F.nx, F.ny, 0);
+  int t = time_begin(s, "mg_pass_d", (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n);
+  if (F.hx)
+    hipLaunchKernelGGL((k_pass<__half, __half, 1>), g, dim3(256), 0, s->st, (const __half*)F.x, (__half*)F.tmp,
+                       (const __half*)F.b, F.mask, F.nx, F.ny, 0, ntx, nty, s->sc);
+  else
+    hipLaunchKernelGGL((k_pass<float, float, 1>), g, dim3(256), 0, s->st, (const float*)F.x, (float*)F.tmp,
+                       (const float*)F.b, F.mask, F.nx, F.ny, 0, ntx, nty, s->sc);
+  time_end(s, t);
+}
+
+template <class SXF, class SBF, class SC>
+static void restrict_t(tq_sim* s, Level& F, Level& G, bool pa) {
+  const dim3 g((G.nx + 15) / 16, (G.ny + 15) / 16);
+  if (pa)
+    hipLaunchKernelGGL((k_restrict_a<SXF, SBF, SC, true>), g, dim3(256), 0, s->st, (const SXF*)F.tmp, (const SBF*)F.b,
+                       F.mask, F.nx, F.ny, (SC*)G.b, (SC*)G.x, G.mask, G.nx, G.ny, s->sc);
+  else
+    hipLaunchKernelGGL((k_restrict_a<SXF, SBF, SC, false>), g, dim3(256), 0, s->st, (const SXF*)F.tmp,
+                       (const SBF*)F.b, F.mask, F.nx, F.ny, (SC*)G.b, (SC*)G.x, G.mask, G.nx, G.ny, s->sc);
+}
+
+// restriction l-1 -> l and pass A of level l
+static void launch_restrict_a(tq_sim* s, int l) {
+  Level& F = s->lv[l - 1];
+  Level& G = s->lv[l];
+  const bool pa = l < (int)s->lv.size() - 1;
+  int t = time_begin(s, l == 1 ? "mg_restrict_a_l1" : "mg_restrict_a",
+                     (vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n + (1 + vbytes(G.hb) + (pa ? vbytes(G.hx) : 0)) * (double)G.n);
+  if (F.hx) {
+    if (G.hx) restrict_t<__half, __half, __half>(s, F, G, pa); else restrict_t<__half, __half, float>(s, F, G, pa);
+  } else if (F.hb) {
+    if (G.hx) restrict_t<float, __half, __half>(s, F, G, pa); else restrict_t<float, __half, float>(s, F, G, pa);
+  } else {
+    if (G.hx) restrict_t<float, float, __half>(s, F, G, pa); else restrict_t<float, float, float>(s, F, G, pa);
+  }
   time_end(s, t);
 }
 
 template <class SX, class SB, class SC>
-static void up_c(tq_sim* s, Level& F, Level& G, dim3 g, bool zd, const ZArgs& za) {
-  if (zd)
-    hipLaunchKernelGGL((k_up<SX, SB, SC, true>), g, dim3(256), 0, s->st, (const SX*)F.x, (SX*)F.tmp, (const SB*)F.b,
-                       F.mask, F.nx, F.ny, (const SC*)G.x, G.mask, G.nx, G.ny, za, s->sc);
-  else
-    hipLaunchKernelGGL((k_up<SX, SB, SC, false>), g, dim3(256), 0, s->st, (const SX*)F.x, (SX*)F.tmp, (const SB*)F.b,
-                       F.mask, F.nx, F.ny, (const SC*)G.x, G.mask, G.nx, G.ny, za, s->sc);
+static void prolong_t(tq_sim* s, Level& F, Level& G) {
+  const int ntx = tiles_of(F.nx, 8), nty = tiles_of(F.ny, 8);
+  const dim3 g = tile_grid(F.nx, F.ny, 8);
+  hipLaunchKernelGGL((k_prolong_pass<SX, SB, SC>), g, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x, (const SB*)F.b,
+                     F.mask, F.nx, F.ny, ntx, nty, (const SC*)G.tmp, G.mask, G.nx, G.ny, s->sc);
+}
+
+// prolongation from level l+1 (its final x in tmp) and pass C of level l: tmp -> x
+static void launch_prolong_c(tq_sim* s, int l) {
+  Level& F = s->lv[l];
+  Level& G = s->lv[l + 1];
+  int t = time_begin(s, l == 0 ? "mg_prolong_c_l0" : "mg_prolong_c",
+                     (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n + (vbytes(G.hx) + 1) * (double)G.n);
+  if (F.hx) {
+    if (G.hx) prolong_t<__half, __half, __half>(s, F, G); else prolong_t<__half, __half, float>(s, F, G);
+  } else if (F.hb) {
+    if (G.hx) prolong_t<float, __half, __half>(s, F, G); else prolong_t<float, __half, float>(s, F, G);
+  } else {
+    if (G.hx) prolong_t<float, float, __half>(s, F, G); else prolong_t<float, float, float>(s, F, G);
+  }
+  time_end(s, t);
 }
 
 static ZArgs zargs(tq_sim* s) {
   ZArgs za;
-  za.opMask = s->opMask; za.tipo = s->tipo; za.rot = s->rot; za.facesB = s->facesB; za.cellsB = s->cellsB;
+  za.opMask = s->opMask; za.rot = s->rot; za.cellsB = s->cellsB;
   za.labelSlot = s->labelSlot; za.r = s->r; za.zold = s->zbuf[s->zc]; za.znew = s->zbuf[1 - s->zc];
-  za.bvec = s->bvec; za.partials = s->partials; za.bpart = s->bpart; za.ar = s->d.ar;
-  za.mixed = s->lv[0].hb ? 1 : 0;
+  za.bvec = s->bvec; za.partials = s->partials; za.bpart = s->bpart;
+  za.scaled = s->mg_scaled;
   return za;
 }
 
-static void launch_up(tq_sim* s, int l) {
-  Level& F = s->lv[l];
-  Level& G = s->lv[l + 1];
-  dim3 g((F.nx + 15) / 16, (F.ny + 15) / 16);
-  const bool zd = l == 0;
+// pass D of level 0 -> z and the dots; returns the partial count
+static int launch_pass_z(tq_sim* s) {
+  Level& F = s->lv[0];
+  const int ntx = tiles_of(F.nx, 0), nty = tiles_of(F.ny, 0);
+  const dim3 g = tile_grid(F.nx, F.ny, 0);
   const ZArgs za = zargs(s);
-  int t = time_begin(s, l == 0 ? "mg_up_l0" : "mg_up",
-                     (2 * (F.hx ? 2 : 4) + (F.hb ? 2 : 4) + 1) * (double)F.n + (G.hx ? 2 : 4) * (double)G.n);
-  const bool gx = G.hx;  // FP16 coarse x (the LDS-resident level always hands over FP32)
-  if (!F.hx && !F.hb) {
-    if (gx) up_c<float, float, __half>(s, F, G, g, zd, za); else up_c<float, float, float>(s, F, G, g, zd, za);
-  } else if (!F.hx) {
-    if (gx) up_c<float, __half, __half>(s, F, G, g, zd, za); else up_c<float, __half, float>(s, F, G, g, zd, za);
-  } else {
-    if (gx) up_c<__half, __half, __half>(s, F, G, g, zd, za); else up_c<__half, __half, float>(s, F, G, g, zd, za);
-  }
+  // x, b, mask, opMask, r, zold read; z written
+  int t = time_begin(s, "mg_pass_z_l0", (4 + vbytes(F.hb) + 1 + 1 + 4 + 4 + 4) * (double)F.n);
+  if (F.hb)
+    hipLaunchKernelGGL((k_pass_z<__half>), g, dim3(256), 0, s->st, (const float*)F.x, (const __half*)F.b, F.mask, F.nx,
+                       F.ny, ntx, nty, za, s->sc);
+  else
+    hipLaunchKernelGGL((k_pass_z<float>), g, dim3(256), 0, s->st, (const float*)F.x, (const float*)F.b, F.mask, F.nx,
+                       F.ny, ntx, nty, za, s->sc);
   time_end(s, t);
-  if (!zd) std::swap(F.x, F.tmp);
+  return (int)(g.x * g.y);
 }
 
 static void launch_coarse_wg(tq_sim* s) {
@@ -1238,120 +1678,161 @@ static void launch_coarse_wg(tq_sim* s) {
     P.mask[l] = s->lv[l].mask;
   }
   for (int l = s->lc; l < L; l++) { P.offm[l] = off; off += (s->lv[l].n + 15) & ~15; }
+  P.dense = s->dense;
   Level& G = s->lv[s->lc];
-  int t = time_begin(s, "mg_coarse_wg", 9.0 * G.n);
-  hipLaunchKernelGGL(k_coarse_wg, dim3(1), dim3(1024), 0, s->st, P, (const float*)G.b, (float*)G.x, s->sc);
+  int t = time_begin(s, "mg_coarse_wg", 13.0 * G.n);
+  hipLaunchKernelGGL(k_coarse_wg, dim3(1), dim3(1024), 0, s->st, P, (const float*)G.b, (const float*)G.x,
+                     (float*)G.tmp, s->sc);
   time_end(s, t);
 }
 
-static void precondition(tq_sim* s, int first) {
-  const int C = s->C;
-  Level& L0 = s->lv[0];
-  int t = time_begin(s, "pcg_mg_input", (1 + 4 + (L0.hb ? 2 : 4)) * (double)C);
-  if (L0.hb)
-    hipLaunchKernelGGL(k_mg_input<__half>, dim3((C + 255) / 256), dim3(256), 0, s->st, s->opMask, s->tipo, s->rot,
-                       s->facesB, s->cellsB, s->r, C, s->d.ar, s->bvec, first, 1, (__half*)L0.b, s->sc);
-  else
-    hipLaunchKernelGGL(k_mg_input<float>, dim3((C + 255) / 256), dim3(256), 0, s->st, s->opMask, s->tipo, s->rot,
-                       s->facesB, s->cellsB, s->r, C, s->d.ar, s->bvec, first, 0, (float*)L0.b, s->sc);
-  time_end(s, t);
-  for (int l = 0; l < s->lc; l++) {
-    launch_down(s, l);
-    launch_restrict(s, l);
+// V-cycle on r (level 0 input already built by in_a0, whose pass A is done) -> z, beta
+static void precondition_rest(tq_sim* s, int first) {
+  const int lc = s->lc;
+  if (lc > 0) launch_pass_b(s, 0);
+  for (int l = 1; l <= lc; l++) {
+    launch_restrict_a(s, l);
+    if (l < lc) launch_pass_b(s, l);
   }
   launch_coarse_wg(s);
   int nb;
-  if (s->lc > 0) {
-    for (int l = s->lc - 1; l >= 0; l--) launch_up(s, l);
-    nb = ((L0.nx + 15) / 16) * ((L0.ny + 15) / 16);
+  if (lc > 0) {
+    for (int l = lc - 1; l >= 1; l--) {
+      launch_prolong_c(s, l);
+      launch_pass_d(s, l);
+    }
+    launch_prolong_c(s, 0);
+    nb = launch_pass_z(s);
   } else {
-    nb = nbr_blocks(C);
-    t = time_begin(s, "pcg_zdots", 16.0 * C);
-    hipLaunchKernelGGL(k_zdots0, dim3(nb), dim3(256), 0, s->st, (const float*)L0.x, zargs(s), C, s->sc);
+    nb = blocks_of(s->C);
+    int t = time_begin(s, "pcg_zdots", 16.0 * s->C);
+    hipLaunchKernelGGL(k_zdots0, dim3(nb), dim3(256), 0, s->st, (const float*)s->lv[0].tmp, zargs(s), s->C, s->sc);
     time_end(s, t);
   }
   s->zc = 1 - s->zc;
-  t = time_begin(s, "pcg_fin_zdots", 0);
-  hipLaunchKernelGGL(k_fin_zdots, dim3(1), dim3(256), 0, s->st, s->bvec, s->activeList, s->labelSlot, s->partials,
-                     s->bpart, nb, first, s->sc);
+  int t = time_begin(s, "pcg_fin_zdots", 0);
+  hipLaunchKernelGGL(k_fin_zdots, dim3(1), dim3(FIN_THREADS), 0, s->st, s->bvec, s->activeList, s->labelSlot,
+                     s->partials, s->bpart, nb, first, s->sc);
   time_end(s, t);
 }
 
 static void pcg_iteration(tq_sim* s) {
   const int C = s->C;
-  const int nb = nbr_blocks(C);
+  const int nb = blocks_of(C);
   const int pn = 1 - s->pc;
   int t = time_begin(s, "pcg_spmv", (1 + 4 + 4 + 4 + 4) * (double)C);
-  hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(256), 0, s->st, s->opMask, s->tipo, s->rot, s->facesB, s->zbuf[s->zc],
-                     s->pbuf[s->pc], s->pbuf[pn], s->ap, s->nx, s->ny, s->d.ar, s->bvec, s->labelSlot, s->partials,
-                     s->bpart, s->sc);
+  hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->zbuf[s->zc], s->pbuf[s->pc],
+                     s->pbuf[pn], s->ap, s->nx, C, s->bvec, s->labelSlot, s->partials, s->bpart, s->sc);
   time_end(s, t);
   t = time_begin(s, "pcg_fin_spmv", 0);
-  hipLaunchKernelGGL(k_fin_spmv, dim3(1), dim3(256), 0, s->st, s->facesB, s->bvec, s->activeList, s->labelSlot,
-                     s->partials, s->bpart, nb, s->sc);
+  hipLaunchKernelGGL(k_fin_spmv, dim3(1), dim3(FIN_THREADS), 0, s->st, s->facesB, s->bvec, s->activeList,
+                     s->labelSlot, s->partials, s->bpart, nb, s->sc);
   time_end(s, t);
   s->pc = pn;
-  t = time_begin(s, "pcg_update", (1 + 8 + 8 + 4 + 4) * (double)C);
-  hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, s->st, s->opMask, s->pbuf[s->pc], s->ap, s->q, s->r, C,
-                     s->partials, s->sc);
-  time_end(s, t);
+  launch_in_a0(s, true);
+  const dim3 g0 = tile_grid(s->lv[0].nx, s->lv[0].ny, 0);
   t = time_begin(s, "pcg_fin_update", 0);
-  hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(256), 0, s->st, s->partials, nb, s->sc);
+  hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(FIN_THREADS), 0, s->st, s->partials, (int)(g0.x * g0.y), s->sc);
   time_end(s, t);
-  precondition(s, 0);
+  precondition_rest(s, 0);
+}
+
+// Two PCG iterations (the buffer ping-pong period) captured once into a HIP graph and
+// replayed; the capture depends on the level plan, so a plan change recaptures.
+static int graph_pair(tq_sim* s) {
+  const int key = (s->d.precision << 8) | ((s->d.mg_half_level + 1) << 1) | (s->mg_scaled ? 1 : 0);
+  if (!s->graph_exec || s->graph_key != key) {
+    graph_drop(s);
+    hipGraph_t g = nullptr;
+    s->capturing = true;
+    TQ_HIP(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    pcg_iteration(s);
+    pcg_iteration(s);
+    const hipError_t e = hipStreamEndCapture(s->st, &g);
+    s->capturing = false;
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) return hip_fail(e2, "hipGraphInstantiate", __FILE__, __LINE__);
+    s->graph_exec = ex;
+    s->graph_key = key;
+    // the capture flipped the ping-pong state twice: back where it started
+  }
+  TQ_HIP(hipGraphLaunch((hipGraphExec_t)s->graph_exec, s->st));
+  return 0;
+}
+
+static int run_iterations(tq_sim* s, int n) {
+  if (s->d.use_graph && !s->timing) {
+    for (; n >= 2; n -= 2) {
+      const int rc = graph_pair(s);
+      if (rc) return rc;
+    }
+  }
+  for (; n > 0; n--) pcg_iteration(s);
+  TQ_LAUNCH_CHECK();
+  return 0;
 }
 
 int pressure_solve(tq_sim* s) {
   const int C = s->C;
   const int L = (int)s->lv.size();
-  plan_precision(s);
+  s->mg_scaled = plan_precision(s) ? 1 : 0;
+  s->pc = 0;
+  s->zc = 0;
   // ---- setup: masks of every level, bubble vectors and slots
   int t = time_begin(s, "pcg_setup", (1 + 4 + 1 + 1) * (double)C);
-  hipLaunchKernelGGL(k_setup0, dim3((C + 255) / 256), dim3(256), 0, s->st, s->tipo, s->rot, s->nx, s->ny, s->d.ar,
-                     s->opMask, s->lv[0].mask);
+  hipLaunchKernelGGL(k_setup0, dim3(blocks_of(C)), dim3(256), 0, s->st, s->tipo, s->rot, s->facesB, s->nx, s->ny,
+                     s->d.ar, s->opMask, s->lv[0].mask);
   for (int l = 1; l < L; l++) {
     const Level& F = s->lv[l - 1];
     const Level& G = s->lv[l];
-    hipLaunchKernelGGL(k_coarse_type, dim3((G.n + 255) / 256), dim3(256), 0, s->st, F.mask, F.nx, F.ny, G.mask, G.nx, G.ny);
-    hipLaunchKernelGGL(k_viz, dim3((G.n + 255) / 256), dim3(256), 0, s->st, G.mask, G.nx, G.ny);
+    hipLaunchKernelGGL(k_coarse_type, dim3(blocks_of(G.n)), dim3(256), 0, s->st, F.mask, F.nx, F.ny, G.mask, G.nx, G.ny);
+    hipLaunchKernelGGL(k_viz, dim3(blocks_of(G.n)), dim3(256), 0, s->st, G.mask, G.nx, G.ny);
   }
   hipLaunchKernelGGL(k_bub_init, dim3(1), dim3(1024), 0, s->st, s->facesB, s->yB32, s->d.ar, s->bvec, s->activeList,
                      s->labelSlot, s->sc);
+  if (s->dense) {
+    const Level& G = s->lv[L - 1];
+    hipLaunchKernelGGL(k_coarsest_op, dim3(G.n), dim3(64), 0, s->st, G.mask, G.nx, G.n, s->dense);
+  }
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   const double tol = s->d.tol;
   const int maxit = s->d.iteracoes;
   TQ_HIP(hipMemcpyAsync(&s->sc->tol, &tol, sizeof(double), hipMemcpyHostToDevice, s->st));
   TQ_HIP(hipMemcpyAsync(&s->sc->maxit, &maxit, sizeof(int), hipMemcpyHostToDevice, s->st));
-  const int nb = nbr_blocks(C);
+  const int nb = blocks_of(C);
   t = time_begin(s, "pcg_init", (1 + 1 + 4 + 4 + 4 + 4 + 4) * (double)C);
-  hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(256), 0, s->st, s->opMask, s->tipo, s->rot, s->facesB, s->div, s->q, s->r,
-                     s->pbuf[s->pc], s->nx, s->ny, s->d.ar, s->bvec, s->labelSlot, s->partials, s->bpart, s->sc);
-  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(256), 0, s->st, s->facesB, s->bvec, s->rhsB, s->activeList, s->labelSlot,
-                     s->partials, s->bpart, nb, s->sc);
+  hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->div, s->q, s->r, s->pbuf[s->pc],
+                     s->nx, C, s->bvec, s->labelSlot, s->partials, s->bpart, s->sc);
+  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(FIN_THREADS), 0, s->st, s->facesB, s->bvec, s->rhsB, s->activeList,
+                     s->labelSlot, s->partials, s->bpart, nb, s->sc);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
-  precondition(s, 1);
+  launch_in_a0(s, false);
+  precondition_rest(s, 1);
   TQ_LAUNCH_CHECK();
   if (tol <= 0) {
-    for (int it = 0; it < maxit; it++) pcg_iteration(s);
-    TQ_LAUNCH_CHECK();
+    const int rc = run_iterations(s, maxit);
+    if (rc) return rc;
   } else {
     // launch in chunks and poll the device stop flag (kernels after the stop exit at once)
     int launched = 0;
-    int chunk = std::max(2, s->sc_host->iters > 0 ? s->sc_host->iters : 8);
+    int chunk = std::max(2, s->sc_host->iters > 0 ? (s->sc_host->iters + 1) & ~1 : 8);
     while (launched < maxit) {
       const int n = std::min(chunk, maxit - launched);
-      for (int k = 0; k < n; k++) pcg_iteration(s);
+      const int rc = run_iterations(s, n);
+      if (rc) return rc;
       launched += n;
-      TQ_LAUNCH_CHECK();
       TQ_HIP(hipMemcpyAsync(s->sc_host, s->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s->st));
       TQ_HIP(hipStreamSynchronize(s->st));
       if (s->sc_host->done) break;
       chunk = 4;
     }
   }
-  hipLaunchKernelGGL(k_pcg_final, dim3((C + 255) / 256), dim3(256), 0, s->st, s->tipo, s->q, C);
+  hipLaunchKernelGGL(k_pcg_final, dim3(blocks_of(C)), dim3(256), 0, s->st, s->tipo, s->q, C);
   hipLaunchKernelGGL(k_pcg_final_b, dim3(1), dim3(1024), 0, s->st, s->bvec, s->activeList, s->yB32, s->sc);
   TQ_LAUNCH_CHECK();
   return 0;
@@ -1359,11 +1840,11 @@ int pressure_solve(tq_sim* s) {
 
 int true_residual(tq_sim* s, double* out) {
   const int C = s->C;
-  const int nb = nbr_blocks(C);
+  const int nb = blocks_of(C);
   double* dres = s->partials + 8 * (size_t)MAX_RED_BLOCKS - 1;
-  hipLaunchKernelGGL(k_true_res, dim3(nb), dim3(256), 0, s->st, s->opMask, s->tipo, s->rot, s->facesB, s->div, s->q,
-                     s->nx, s->ny, s->d.ar, s->bvec, s->labelSlot, s->partials, s->bpart, s->sc);
-  hipLaunchKernelGGL(k_fin_true_res, dim3(1), dim3(256), 0, s->st, s->facesB, s->bvec, s->rhsB, s->activeList,
+  hipLaunchKernelGGL(k_true_res, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->div, s->q, s->nx, C, s->bvec,
+                     s->labelSlot, s->partials, s->bpart, s->sc);
+  hipLaunchKernelGGL(k_fin_true_res, dim3(1), dim3(FIN_THREADS), 0, s->st, s->facesB, s->bvec, s->rhsB, s->activeList,
                      s->labelSlot, s->partials, s->bpart, nb, s->sc, dres);
   TQ_LAUNCH_CHECK();
   TQ_HIP(hipMemcpyAsync(out, dres, sizeof(double), hipMemcpyDeviceToHost, s->st));

@@ -27,7 +27,7 @@ struct TimingFilter { std::string only; };
 static thread_local TimingFilter g_filter;
 
 int time_begin(tq_sim* s, const char* name, double bytes) {
-  if (!s->timing) return -1;
+  if (!s->timing || s->capturing) return -1;
   if (!g_filter.only.empty() && g_filter.only != name) return -1;
   int ki = -1;
   for (size_t i = 0; i < s->ktimes.size(); i++)
@@ -273,6 +273,9 @@ void tq_desc_default(tq_desc* d, int nx, int ny, double h) {
   d->max_particles = 0;
   d->precision = TQ_PREC_FP32;
   d->tol = 0;
+  d->mg_min_lds_level = 0;
+  d->mg_half_level = -1;
+  d->use_graph = 0;
 }
 
 int tq_create(const tq_desc* d, tq_sim** out) {
@@ -510,6 +513,7 @@ int tq_set_options(tq_sim* s, const tq_desc* d) {
   if (d->precision != TQ_PREC_FP32 && d->precision != TQ_PREC_MIXED) { set_error("tq_set_options: precision"); return TQ_E_ARG; }
   s->d.gx = d->gx; s->d.gy = d->gy; s->d.iteracoes = d->iteracoes; s->d.tau_densidade = d->tau_densidade;
   s->d.tensao = d->tensao; s->d.ar = d->ar; s->d.precision = d->precision; s->d.tol = d->tol;
+  s->d.mg_half_level = d->mg_half_level; s->d.use_graph = d->use_graph;
   return 0;
 }
 

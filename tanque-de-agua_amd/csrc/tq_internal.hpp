@@ -56,6 +56,10 @@ constexpr int MAX_RED_BLOCKS = 65536;    // partial slots per reduction (one per
 // mask byte of a multigrid cell / operator cell
 //   bits 0-3: free neighbours (1 left, 2 right, 4 down, 8 up) as prepararVizinhos
 //   bits 4-5: cell type (0 solid, 1 water, 2 air)
+//   multigrid masks, bit 6: red-black colour (i + j + 1) & 1
+//   operator mask, bit 6: tracked bubble cell (air, label >= 2 with water faces);
+//                  bit 7: that bubble cell is interior (not on the grid border)
+constexpr uint8_t MK_COLOUR = 0x40, MK_BUB = 0x40, MK_BUBIN = 0x80;
 __host__ __device__ inline int mask_viz(uint8_t m) { return m & 15; }
 __host__ __device__ inline int mask_type(uint8_t m) { return (m >> 4) & 3; }
 __host__ __device__ inline int mask_cnt(uint8_t m) { return __builtin_popcount(m & 15); }
@@ -80,8 +84,8 @@ struct DevScalars {
   double tol;
   double proj_mean;     // null-space component removed from b (residual-terminated mode)
   int maxit;
-  float mg_scale;       // mixed precision: scale applied to the V-cycle input
-  float pad_;
+  float mg_scale;       // mixed precision: scale applied to the V-cycle input of this iteration
+  float mg_scale_next;  // 1/||r|| of the newest residual, the scale of the next iteration
 };
 
 struct Level {
@@ -183,6 +187,11 @@ struct tq_sim {
   double* bpart = nullptr;      // per-block partial sums of the slotted bubbles: 8 x MAX_RED_BLOCKS
   int* labelSlot = nullptr;     // bubble label -> reduction slot (-1: FP64 atomics)
   int lc = 0;                   // first multigrid level solved inside the one-workgroup kernel
+  int mg_scaled = 0;            // this solve stores some multigrid level in FP16 (input scaled)
+  float* dense = nullptr;       // coarsest-level operator (when that level is small)
+  void* graph_exec = nullptr;   // hipGraphExec_t of two PCG iterations
+  int graph_key = -1;
+  bool capturing = false;       // a stream capture is recording (no timing events)
   tq::DevScalars* sc = nullptr;
   tq::DevScalars* sc_host = nullptr;  // pinned mirror
 

@@ -41,6 +41,8 @@ def prepared(orc, build, frames):
     ("bolha", lambda o: tanque_com_bolha(o), 6),
     ("ampulheta", lambda o: ampulheta(o), 20),
     ("dam160", lambda o: dam(o, nx=160, ny=90, h=0.8 / 90, seed=0), 12),
+    # 256x144: level 1 (128x72) is large enough for FP16 storage in the mixed preconditioner
+    ("dam256", lambda o: dam(o, nx=256, ny=144, h=1.0 / 144, seed=0), 8),
 ])
 def test_solve_to_tolerance(oracle, tq, name, build, frames):
     osim = prepared(oracle, build, frames)
@@ -102,3 +104,30 @@ def test_mixed_matches_fp32_over_steps(tq, oracle):
     d = float(np.max(np.abs(a[:, :2].astype(np.float64) - b[:, :2])))
     log("\n  mixed vs fp32 after 10 frames: max position difference %.3e m (n=%d)" % (d, len(a)))
     assert d < 5e-3  # chaotic FLIP divergence over 40 substeps stays far below a cell (1 cm)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+@pytest.mark.parametrize("name,build", [
+    ("bolha", lambda o: tanque_com_bolha(o)),
+    ("dam160", lambda o: dam(o, nx=160, ny=90, h=0.8 / 90, seed=0)),
+])
+def test_graph_replay_matches_eager(tq, oracle, prec, name, build):
+    """PCG iterations replayed from the captured HIP graph give bit-identical states to eager
+    launches, over many replays with the substep's other kernels and host syncs in between."""
+    p = tq.PREC_FP32 if prec == "fp32" else tq.PREC_MIXED
+    osim = build(oracle)
+    P0 = osim.particulas().copy()
+    out = {}
+    for graph in (False, True):
+        g = hip_sim_like(tq, osim, iteracoes=200, tol=TOL, precision=p, use_graph=graph)
+        g.write_particles(P0)
+        for _ in range(6):
+            g.avancar(1 / 60, 3)
+        out[graph] = (g.particulas(), g.read(12), g.read(22), g.solver_stats())
+    a, b = out[False], out[True]
+    log("\n  graph vs eager (%s %s): iterations %d / %d" % (name, prec, a[3]["total_iterations"],
+                                                          b[3]["total_iterations"]))
+    assert a[3]["total_iterations"] == b[3]["total_iterations"]
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2], b[2])
