@@ -30,6 +30,8 @@ extern "C" {
 /* Precision of the multigrid preconditioner inside the FP32 PCG. */
 #define TQ_PREC_FP32 0   /* MG vectors stored and computed in FP32 */
 #define TQ_PREC_MIXED 1  /* MG vectors stored in FP16 (per-cycle scaled), computed in FP32 */
+#define TQ_PREC_MIXED_B 2 /* only the MG right-hand sides stored in FP16; x stays FP32 */
+#define TQ_PREC_MIXED_X 3 /* only the MG iterates x stored in FP16; right-hand sides stay FP32 */
 
 typedef struct tq_sim tq_sim;
 
@@ -44,12 +46,12 @@ typedef struct tq_desc {
   double tensao;          /* surface tension (N/m); 0 disables */
   int ar;                 /* 1: track enclosed air pockets as pressure unknowns */
   int max_particles;      /* device capacity; 0 = ceil(4.6 * nx * ny) as MotorCPU.maxParticulas */
-  int precision;          /* TQ_PREC_FP32 or TQ_PREC_MIXED */
+  int precision;          /* TQ_PREC_FP32 or one of the mixed modes TQ_PREC_MIXED* */
   double tol;             /* 0: fixed `iteracoes` (reference); >0: stop at ||r|| <= tol*||b|| */
   int mg_min_lds_level;   /* multigrid levels >= this run in one workgroup from LDS; 0 = the
-                             first level of at most 8192 cells whose suffix fits (set higher
+                             first level of at most 2048 cells whose suffix fits (set higher
                              to keep levels on the multi-workgroup kernels) */
-  int mg_half_level;      /* TQ_PREC_MIXED: first level whose multigrid vectors are stored in
+  int mg_half_level;      /* mixed modes: first level whose multigrid vectors are stored in
                              FP16 (level 0 keeps x in FP32 and stores only b in FP16); levels
                              under 4096 cells and those in LDS stay FP32; -1 = default (1) */
   int use_graph;          /* 1: PCG iterations are replayed from a captured HIP graph */

@@ -82,17 +82,30 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(const uint32_t* __restric
   }
 }
 
+// Exclusive scan of n values (out[n] = total): per-block sums, a scan of the block sums
+// (itself two-level past 4096 blocks), then the per-block scans with their offsets.
 int scan_u32(tq_sim* s, const uint32_t* in, uint32_t* out, int n) {
   if (n <= 0) {
     TQ_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), s->st));
     return 0;
   }
-  int nb = (n + SCAN_B - 1) / SCAN_B;
-  if (nb > SCAN_B) { set_error("scan: input too large"); return TQ_E_CAPACITY; }
+  const int nb = (n + SCAN_B - 1) / SCAN_B;
+  const int nb2 = (nb + SCAN_B - 1) / SCAN_B;
+  if (nb2 > SCAN_B || 2 * nb + nb2 + 2 > s->scan_cap) { set_error("scan: input too large"); return TQ_E_CAPACITY; }
+  uint32_t* blocks = s->scanBlocks;            // nb block sums
+  uint32_t* blocks2 = blocks + nb;             // nb2 sums of those (two-level case)
+  uint32_t* offs = blocks2 + nb2 + 1;          // nb block offsets (two-level case)
   int t = time_begin(s, "scan", 8.0 * n);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, s->st, in, n, s->scanBlocks);
-  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_T), 0, s->st, s->scanBlocks, nb, out + n);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_T), 0, s->st, in, n, s->scanBlocks, out);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, s->st, in, n, blocks);
+  if (nb <= SCAN_B) {
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_T), 0, s->st, blocks, nb, out + n);
+  } else {
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb2), dim3(SCAN_T), 0, s->st, blocks, nb, blocks2);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_T), 0, s->st, blocks2, nb2, out + n);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb2), dim3(SCAN_T), 0, s->st, blocks, nb, blocks2, offs);
+    blocks = offs;
+  }
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_T), 0, s->st, in, n, blocks, out);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   return 0;

@@ -46,11 +46,14 @@ template <> __device__ inline float ld<float>(const float* p, size_t i) { return
 template <> __device__ inline float ld<__half>(const __half* p, size_t i) { return __half2float(p[i]); }
 template <class S> __device__ inline void st(S* p, size_t i, float v);
 template <> __device__ inline void st<float>(float* p, size_t i, float v) { p[i] = v; }
-template <> __device__ inline void st<__half>(__half* p, size_t i, float v) { p[i] = __float2half(v); }
+// FP16 stores saturate at the largest finite value: an extreme entry is clipped instead of
+// becoming inf (the preconditioner is perturbed, the PCG vectors stay finite)
+__device__ inline __half sat_half(float v) { return __float2half(fminf(fmaxf(v, -65504.f), 65504.f)); }
+template <> __device__ inline void st<__half>(__half* p, size_t i, float v) { p[i] = sat_half(v); }
 // the value a store in S keeps
 template <class S> __device__ inline float rnd(float v);
 template <> __device__ inline float rnd<float>(float v) { return v; }
-template <> __device__ inline float rnd<__half>(float v) { return __half2float(__float2half(v)); }
+template <> __device__ inline float rnd<__half>(float v) { return __half2float(sat_half(v)); }
 
 // all LDS operations of this wavefront are complete (and not reordered across by the compiler)
 __device__ inline void wave_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
@@ -326,7 +329,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_init(const uint32_t* __rest
     sc->alpha = 0;
     const bool stop = sc->maxit <= 0 || (sc->tol > 0 && (sc->norm0 == 0 || sc->norm0 <= sc->tol * sc->normb));
     sc->done = stop ? 1 : 0;
-    sc->mg_scale = sc->norm0 > 0 ? (float)(1.0 / sc->norm0) : 1.f;
+    sc->mg_scale = sc->norm0 > 0 ? (float)(sc->mg_mult / sc->norm0) : 1.f;
     sc->mg_scale_next = sc->mg_scale;
   }
   __syncthreads();
@@ -770,10 +773,10 @@ __global__ __launch_bounds__(256) void k_zdots0(const float* __restrict__ x0, ZA
 // r = b - A x on the 34x34 fine cells it gathers from (2I-1 .. 2I+2 per axis, weights
 // 1/4 3/4 3/4 1/4), from x on 36x36, in LDS.  Then wave 0 runs the coarse level's pass A
 // (offset 0 = this tile) from x = 0 unless the coarse level is the coarsest.
-template <class SXF, class SBF, class SC, bool PASSA>
+template <class SXF, class SBF, class SCB, class SCX, bool PASSA>
 __global__ __launch_bounds__(256) void k_restrict_a(const SXF* __restrict__ fx, const SBF* __restrict__ fb,
                                                     const uint8_t* __restrict__ fm, int fnx, int fny,
-                                                    SC* __restrict__ cb, SC* __restrict__ cxo,
+                                                    SCB* __restrict__ cb, SCX* __restrict__ cxo,
                                                     const uint8_t* __restrict__ cmk, int cnx, int cny,
                                                     const DevScalars* __restrict__ sc) {
   if (sc->done) return;
@@ -851,9 +854,9 @@ __global__ __launch_bounds__(256) void k_restrict_a(const SXF* __restrict__ fx, 
           for (int bi = 0; bi < 4; bi++) racc += R[(rx + bi) + (ry + bj) * 34] * w[bi];
           acc += racc * w[bj];
         }
-        bc = rnd<SC>(acc);
+        bc = rnd<SCB>(acc);
       }
-      st<SC>(cb, Cm, bc);
+      st<SCB>(cb, Cm, bc);
     }
     Bt[tid] = bc;
   }
@@ -875,7 +878,7 @@ __global__ __launch_bounds__(256) void k_restrict_a(const SXF* __restrict__ fx, 
     }
   wave_sync();
   tile_sweeps<0>(T, cc, lane);
-  store_cells<SC>(cc, cxo, CI0, CJ0, cnx, cny, lane);
+  store_cells<SCX>(cc, cxo, CI0, CJ0, cnx, cny, lane);
 }
 
 // ================================================================== coarsest-level operator
@@ -1343,7 +1346,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_update(const double* __rest
     bool stop = sc->iters >= sc->maxit;
     if (sc->tol > 0 && (nr == 0 || nr <= sc->tol * sc->normb)) stop = true;
     sc->done = stop ? 1 : 0;
-    sc->mg_scale_next = nr > 0 ? (float)(1.0 / nr) : 1.f;
+    sc->mg_scale_next = nr > 0 ? (float)(sc->mg_mult / nr) : 1.f;
   }
 }
 
@@ -1504,18 +1507,22 @@ void solver_free(tq_sim* s) {
   s->labelSlot = nullptr;
 }
 
-// per-level storage for this solve: mixed stores x and b of the multi-workgroup levels from
-// mg_half_level on in FP16 (level 0: b only); small levels and the LDS levels stay FP32
+// per-level storage for this solve: the mixed modes store b (TQ_PREC_MIXED: and x) of the
+// multi-workgroup levels from mg_half_level on in FP16 (level 0: b only); small levels and the
+// LDS levels stay FP32
 static bool plan_precision(tq_sim* s) {
-  const bool mixed = s->d.precision == TQ_PREC_MIXED;
+  const int pr = s->d.precision;
+  const bool mixed = pr == TQ_PREC_MIXED || pr == TQ_PREC_MIXED_B || pr == TQ_PREC_MIXED_X;
+  const bool half_x = pr == TQ_PREC_MIXED || pr == TQ_PREC_MIXED_X;
+  const bool half_b = pr == TQ_PREC_MIXED || pr == TQ_PREC_MIXED_B;
   const int h0 = s->d.mg_half_level < 0 ? 1 : s->d.mg_half_level;
   bool any = false;
   for (int l = 0; l < (int)s->lv.size(); l++) {
     Level& L = s->lv[l];
     const bool half = mixed && l < s->lc && l >= h0 && L.n >= 4096;
-    L.hb = half;
-    L.hx = half && l > 0;
-    any = any || half;
+    L.hb = half && half_b;
+    L.hx = half && half_x && l > 0;
+    any = any || L.hb || L.hx;
   }
   return any;
 }
@@ -1551,88 +1558,78 @@ static void launch_in_a0(tq_sim* s, bool upd) {
   time_end(s, t);
 }
 
-// pass B (offset 8, colours 0,1): x -> tmp
-static void launch_pass_b(tq_sim* s, int l) {
+// storage-type dispatch: fn(x tag, b tag) with the level's element types
+template <class T> struct Tag { using type = T; };
+template <class Fn>
+static void with_types(const Level& L, Fn fn) {
+  if (L.hx) {
+    if (L.hb) fn(Tag<__half>{}, Tag<__half>{}); else fn(Tag<__half>{}, Tag<float>{});
+  } else {
+    if (L.hb) fn(Tag<float>{}, Tag<__half>{}); else fn(Tag<float>{}, Tag<float>{});
+  }
+}
+#define TQ_TYPES(tx, tb) using SX = typename decltype(tx)::type; using SB = typename decltype(tb)::type
+
+// one smoothing pass of a multi-workgroup level: x -> tmp.  Pass B: offset 8, colours 0,1;
+// pass D: offset 0, colours 1,0 (levels > 0; level 0's pass D writes z).
+template <int C0>
+static void launch_pass(tq_sim* s, int l, int desl, const char* name) {
   Level& F = s->lv[l];
-  const int ntx = tiles_of(F.nx, 8), nty = tiles_of(F.ny, 8);
-  const dim3 g = tile_grid(F.nx, F.ny, 8);
-  int t = time_begin(s, l == 0 ? "mg_pass_b_l0" : "mg_pass_b", (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n);
-  if (F.hx)
-    hipLaunchKernelGGL((k_pass<__half, __half, 0>), g, dim3(256), 0, s->st, (const __half*)F.x, (__half*)F.tmp,
-                       (const __half*)F.b, F.mask, F.nx, F.ny, 8, ntx, nty, s->sc);
-  else if (F.hb)
-    hipLaunchKernelGGL((k_pass<float, __half, 0>), g, dim3(256), 0, s->st, (const float*)F.x, (float*)F.tmp,
-                       (const __half*)F.b, F.mask, F.nx, F.ny, 8, ntx, nty, s->sc);
-  else
-    hipLaunchKernelGGL((k_pass<float, float, 0>), g, dim3(256), 0, s->st, (const float*)F.x, (float*)F.tmp,
-                       (const float*)F.b, F.mask, F.nx, F.ny, 8, ntx, nty, s->sc);
+  const int ntx = tiles_of(F.nx, desl), nty = tiles_of(F.ny, desl);
+  const dim3 g = tile_grid(F.nx, F.ny, desl);
+  int t = time_begin(s, name, (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n);
+  with_types(F, [&](auto tx, auto tb) {
+    TQ_TYPES(tx, tb);
+    hipLaunchKernelGGL((k_pass<SX, SB, C0>), g, dim3(256), 0, s->st, (const SX*)F.x, (SX*)F.tmp, (const SB*)F.b,
+                       F.mask, F.nx, F.ny, desl, ntx, nty, s->sc);
+  });
   time_end(s, t);
 }
-
-// pass D (offset 0, colours 1,0): x -> tmp (levels > 0)
-static void launch_pass_d(tq_sim* s, int l) {
-  Level& F = s->lv[l];
-  const int ntx = tiles_of(F.nx, 0), nty = tiles_of(F.ny, 0);
-  const dim3 g = tile_grid(F.nx, F.ny, 0);
-  int t = time_begin(s, "mg_pass_d", (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n);
-  if (F.hx)
-    hipLaunchKernelGGL((k_pass<__half, __half, 1>), g, dim3(256), 0, s->st, (const __half*)F.x, (__half*)F.tmp,
-                       (const __half*)F.b, F.mask, F.nx, F.ny, 0, ntx, nty, s->sc);
-  else
-    hipLaunchKernelGGL((k_pass<float, float, 1>), g, dim3(256), 0, s->st, (const float*)F.x, (float*)F.tmp,
-                       (const float*)F.b, F.mask, F.nx, F.ny, 0, ntx, nty, s->sc);
-  time_end(s, t);
-}
-
-template <class SXF, class SBF, class SC>
-static void restrict_t(tq_sim* s, Level& F, Level& G, bool pa) {
-  const dim3 g((G.nx + 15) / 16, (G.ny + 15) / 16);
-  if (pa)
-    hipLaunchKernelGGL((k_restrict_a<SXF, SBF, SC, true>), g, dim3(256), 0, s->st, (const SXF*)F.tmp, (const SBF*)F.b,
-                       F.mask, F.nx, F.ny, (SC*)G.b, (SC*)G.x, G.mask, G.nx, G.ny, s->sc);
-  else
-    hipLaunchKernelGGL((k_restrict_a<SXF, SBF, SC, false>), g, dim3(256), 0, s->st, (const SXF*)F.tmp,
-                       (const SBF*)F.b, F.mask, F.nx, F.ny, (SC*)G.b, (SC*)G.x, G.mask, G.nx, G.ny, s->sc);
-}
+static void launch_pass_b(tq_sim* s, int l) { launch_pass<0>(s, l, 8, l == 0 ? "mg_pass_b_l0" : "mg_pass_b"); }
+static void launch_pass_d(tq_sim* s, int l) { launch_pass<1>(s, l, 0, "mg_pass_d"); }
 
 // restriction l-1 -> l and pass A of level l
 static void launch_restrict_a(tq_sim* s, int l) {
   Level& F = s->lv[l - 1];
   Level& G = s->lv[l];
   const bool pa = l < (int)s->lv.size() - 1;
+  const dim3 g((G.nx + 15) / 16, (G.ny + 15) / 16);
   int t = time_begin(s, l == 1 ? "mg_restrict_a_l1" : "mg_restrict_a",
                      (vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n + (1 + vbytes(G.hb) + (pa ? vbytes(G.hx) : 0)) * (double)G.n);
-  if (F.hx) {
-    if (G.hx) restrict_t<__half, __half, __half>(s, F, G, pa); else restrict_t<__half, __half, float>(s, F, G, pa);
-  } else if (F.hb) {
-    if (G.hx) restrict_t<float, __half, __half>(s, F, G, pa); else restrict_t<float, __half, float>(s, F, G, pa);
-  } else {
-    if (G.hx) restrict_t<float, float, __half>(s, F, G, pa); else restrict_t<float, float, float>(s, F, G, pa);
-  }
+  with_types(F, [&](auto fx, auto fb) {
+    using SXF = typename decltype(fx)::type;
+    using SBF = typename decltype(fb)::type;
+    with_types(G, [&](auto cx, auto cb) {
+      using SCX = typename decltype(cx)::type;
+      using SCB = typename decltype(cb)::type;
+      if (pa)
+        hipLaunchKernelGGL((k_restrict_a<SXF, SBF, SCB, SCX, true>), g, dim3(256), 0, s->st, (const SXF*)F.tmp,
+                           (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny, s->sc);
+      else
+        hipLaunchKernelGGL((k_restrict_a<SXF, SBF, SCB, SCX, false>), g, dim3(256), 0, s->st, (const SXF*)F.tmp,
+                           (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny, s->sc);
+    });
+  });
   time_end(s, t);
-}
-
-template <class SX, class SB, class SC>
-static void prolong_t(tq_sim* s, Level& F, Level& G) {
-  const int ntx = tiles_of(F.nx, 8), nty = tiles_of(F.ny, 8);
-  const dim3 g = tile_grid(F.nx, F.ny, 8);
-  hipLaunchKernelGGL((k_prolong_pass<SX, SB, SC>), g, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x, (const SB*)F.b,
-                     F.mask, F.nx, F.ny, ntx, nty, (const SC*)G.tmp, G.mask, G.nx, G.ny, s->sc);
 }
 
 // prolongation from level l+1 (its final x in tmp) and pass C of level l: tmp -> x
 static void launch_prolong_c(tq_sim* s, int l) {
   Level& F = s->lv[l];
   Level& G = s->lv[l + 1];
+  const int ntx = tiles_of(F.nx, 8), nty = tiles_of(F.ny, 8);
+  const dim3 g = tile_grid(F.nx, F.ny, 8);
   int t = time_begin(s, l == 0 ? "mg_prolong_c_l0" : "mg_prolong_c",
                      (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n + (vbytes(G.hx) + 1) * (double)G.n);
-  if (F.hx) {
-    if (G.hx) prolong_t<__half, __half, __half>(s, F, G); else prolong_t<__half, __half, float>(s, F, G);
-  } else if (F.hb) {
-    if (G.hx) prolong_t<float, __half, __half>(s, F, G); else prolong_t<float, __half, float>(s, F, G);
-  } else {
-    if (G.hx) prolong_t<float, float, __half>(s, F, G); else prolong_t<float, float, float>(s, F, G);
-  }
+  with_types(F, [&](auto tx, auto tb) {
+    TQ_TYPES(tx, tb);
+    if (G.hx)
+      hipLaunchKernelGGL((k_prolong_pass<SX, SB, __half>), g, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
+                         (const SB*)F.b, F.mask, F.nx, F.ny, ntx, nty, (const __half*)G.tmp, G.mask, G.nx, G.ny, s->sc);
+    else
+      hipLaunchKernelGGL((k_prolong_pass<SX, SB, float>), g, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
+                         (const SB*)F.b, F.mask, F.nx, F.ny, ntx, nty, (const float*)G.tmp, G.mask, G.nx, G.ny, s->sc);
+  });
   time_end(s, t);
 }
 
@@ -1803,6 +1800,10 @@ int pressure_solve(tq_sim* s) {
   const int maxit = s->d.iteracoes;
   TQ_HIP(hipMemcpyAsync(&s->sc->tol, &tol, sizeof(double), hipMemcpyHostToDevice, s->st));
   TQ_HIP(hipMemcpyAsync(&s->sc->maxit, &maxit, sizeof(int), hipMemcpyHostToDevice, s->st));
+  // FP16 right-hand sides with FP32 x can sit 2^12 above unit norm (more of the residual stays
+  // in the normal FP16 range); FP16 x keeps unit-norm input so the V-cycle output stays in range
+  const float mult = s->d.precision == TQ_PREC_MIXED_B ? 4096.f : 1.f;
+  TQ_HIP(hipMemcpyAsync(&s->sc->mg_mult, &mult, sizeof(float), hipMemcpyHostToDevice, s->st));
   const int nb = blocks_of(C);
   t = time_begin(s, "pcg_init", (1 + 1 + 4 + 4 + 4 + 4 + 4) * (double)C);
   hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->div, s->q, s->r, s->pbuf[s->pc],

@@ -127,7 +127,13 @@ static int alloc_all(tq_sim* s) {
   for (int b = 0; b < 2; b++) { DALLOC(s->zbuf[b], C); DALLOC(s->pbuf[b], C); }
   DALLOC(s->partials, 8 * (size_t)MAX_RED_BLOCKS);
   DALLOC(s->sc, 1);
-  DALLOC(s->scanBlocks, 4096 + 1);
+  {
+    // block sums of the largest scan (particle slots, binning keys or cells), two levels
+    const size_t nmax = std::max({(size_t)s->cap, (size_t)s->K, (size_t)C}) + 1;
+    const size_t nb = (nmax + 4095) / 4096;
+    s->scan_cap = (int)(2 * nb + (nb + 4095) / 4096 + 2);
+    DALLOC(s->scanBlocks, (size_t)s->scan_cap);
+  }
   TQ_HIP(hipHostMalloc((void**)&s->sc_host, sizeof(DevScalars), hipHostMallocDefault));
   if (int e = solver_alloc(s)) return e;
   // zero state (the reference allocates zero-filled typed arrays)
@@ -283,6 +289,7 @@ int tq_create(const tq_desc* d, tq_sim** out) {
   *out = nullptr;
   if (d->nx < 4 || d->ny < 4 || !(d->h > 0)) { set_error("tq_create: grid must be at least 4x4 with h > 0"); return TQ_E_ARG; }
   if ((long long)d->nx * d->ny > (1LL << 26)) { set_error("tq_create: grid too large"); return TQ_E_CAPACITY; }
+  if (d->precision < TQ_PREC_FP32 || d->precision > TQ_PREC_MIXED_X) { set_error("tq_create: precision"); return TQ_E_ARG; }
   tq_sim* s = new tq_sim();
   s->d = *d;
   s->nx = d->nx; s->ny = d->ny; s->C = d->nx * d->ny; s->h = d->h;
@@ -510,7 +517,10 @@ int tq_area(tq_sim* s, double* a) {
 int tq_set_options(tq_sim* s, const tq_desc* d) {
   TQ_CHECK_SIM(s);
   if (!d) { set_error("tq_set_options: null"); return TQ_E_ARG; }
-  if (d->precision != TQ_PREC_FP32 && d->precision != TQ_PREC_MIXED) { set_error("tq_set_options: precision"); return TQ_E_ARG; }
+  if (d->precision < TQ_PREC_FP32 || d->precision > TQ_PREC_MIXED_X) {
+    set_error("tq_set_options: precision");
+    return TQ_E_ARG;
+  }
   s->d.gx = d->gx; s->d.gy = d->gy; s->d.iteracoes = d->iteracoes; s->d.tau_densidade = d->tau_densidade;
   s->d.tensao = d->tensao; s->d.ar = d->ar; s->d.precision = d->precision; s->d.tol = d->tol;
   s->d.mg_half_level = d->mg_half_level; s->d.use_graph = d->use_graph;

@@ -85,7 +85,9 @@ struct DevScalars {
   double proj_mean;     // null-space component removed from b (residual-terminated mode)
   int maxit;
   float mg_scale;       // mixed precision: scale applied to the V-cycle input of this iteration
-  float mg_scale_next;  // 1/||r|| of the newest residual, the scale of the next iteration
+  float mg_scale_next;  // mg_mult/||r|| of the newest residual, the scale of the next iteration
+  float mg_mult;        // numerator of the scale: FP16 values of the input sit this far above 1/||r||
+  float pad_;
 };
 
 struct Level {
@@ -197,6 +199,7 @@ struct tq_sim {
 
   // scan scratch
   uint32_t* scanBlocks = nullptr;
+  int scan_cap = 0;             // entries of scanBlocks
 
   // host-side state
   bool tipos_prontos = false;

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libtanque.so")
 
 CAMPOS = 12
 MAX_BOLHAS = 4096
-PREC_FP32, PREC_MIXED = 0, 1
+PREC_FP32, PREC_MIXED, PREC_MIXED_B, PREC_MIXED_X = 0, 1, 2, 3
 
 F_U, F_V, F_UPESO, F_VPESO, F_UVAL, F_VVAL, F_DENS, F_CONT, F_COR, F_TIPO, F_SOLIDO = range(11)
 F_DIV, F_Q, F_FASE, F_NX, F_NY, F_KAPPA, F_QAR, F_ROT = range(11, 19)

@@ -57,7 +57,7 @@ def test_solve_to_tolerance(oracle, tq, name, build, frames):
     s64 = o64.solver_stats()
     q64 = o64.read(12, True)
     res = {}
-    for prec in (tq.PREC_FP32, tq.PREC_MIXED):
+    for prec in (tq.PREC_FP32, tq.PREC_MIXED, tq.PREC_MIXED_B, tq.PREC_MIXED_X):
         g = hip_sim_like(tq, osim, iteracoes=200, tol=TOL, precision=prec)
         clone_state(osim, g)
         g.fase(8, dt)
@@ -70,7 +70,8 @@ def test_solve_to_tolerance(oracle, tq, name, build, frames):
         name, so["iters"], so["normas"][-1] / so["norm_b"], so["true_resid"] / so["norm_b"], s64["iters"],
         s64["normas"][-1] / s64["norm_b"], s64["true_resid"] / s64["norm_b"]))
     bad = []
-    for prec, lab in ((tq.PREC_FP32, "fp32"), (tq.PREC_MIXED, "mixed")):
+    for prec, lab in ((tq.PREC_FP32, "fp32"), (tq.PREC_MIXED, "mixed"), (tq.PREC_MIXED_B, "mix-b"),
+                      (tq.PREC_MIXED_X, "mix-x")):
         q, y, st = res[prec]
         rec = st["norm"] / st["norm_b"] if st["norm_b"] > 0 else 0
         tru = st["true_resid"] / st["norm_b"] if st["norm_b"] > 0 else 0
@@ -93,7 +94,7 @@ def test_mixed_matches_fp32_over_steps(tq, oracle):
     osim = dam(oracle)
     P0 = osim.particulas().copy()
     out = {}
-    for prec in (tq.PREC_FP32, tq.PREC_MIXED):
+    for prec in (tq.PREC_FP32, tq.PREC_MIXED, tq.PREC_MIXED_B, tq.PREC_MIXED_X):
         g = hip_sim_like(tq, osim, iteracoes=200, tol=TOL, precision=prec)
         g.write_particles(P0)
         for _ in range(10):
@@ -106,7 +107,7 @@ def test_mixed_matches_fp32_over_steps(tq, oracle):
     assert d < 5e-3  # chaotic FLIP divergence over 40 substeps stays far below a cell (1 cm)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+@pytest.mark.parametrize("prec", ["fp32", "mixed", "mix-b", "mix-x"])
 @pytest.mark.parametrize("name,build", [
     ("bolha", lambda o: tanque_com_bolha(o)),
     ("dam160", lambda o: dam(o, nx=160, ny=90, h=0.8 / 90, seed=0)),
@@ -114,7 +115,7 @@ def test_mixed_matches_fp32_over_steps(tq, oracle):
 def test_graph_replay_matches_eager(tq, oracle, prec, name, build):
     """PCG iterations replayed from the captured HIP graph give bit-identical states to eager
     launches, over many replays with the substep's other kernels and host syncs in between."""
-    p = tq.PREC_FP32 if prec == "fp32" else tq.PREC_MIXED
+    p = {"fp32": tq.PREC_FP32, "mixed": tq.PREC_MIXED, "mix-b": tq.PREC_MIXED_B, "mix-x": tq.PREC_MIXED_X}[prec]
     osim = build(oracle)
     P0 = osim.particulas().copy()
     out = {}
