@@ -1,13 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark of the FLIP/APIC substep on one MI355X.
 
-Workload: the Ultra sealed hourglass (896x504 cells, 1.59 mm, ~0.9 M particles), frames of
-1/480 s split into substeps by the reference's adaptive bound, pressure solved by MGPCG to a
-relative residual of 1e-6.  One bench step = one frame (air update + its substeps).  The
-mixed-precision run (FP16 multigrid storage, FP32 PCG, FP64 dots) is the reported value; the
-FP32-multigrid run replays the same substep schedule for the comparison.
+Workload: the Ultra sealed hourglass (896x504 cells, 1.59 mm, ~0.75 M particles after the
+wall check), frames of 1/480 s split into substeps by the reference's adaptive bound, the
+pressure solved by MGPCG to a relative residual of 1e-6.  One bench step = one frame (air
+update + its substeps).  The reported value is the mixed-precision run (multigrid vectors stored
+in FP16, computed in FP32, FP32 PCG, FP64 dots); the FP32 run replays the same substep schedule.
 
-Prints ONE JSON line (see the keys below).  --gpus N > 1 is not supported (one GPU per box).
+Timed runs use eager launches with HIP-event timing of the dominant kernel on the library's
+stream (the kernel with the largest share of device time in a short untimed probe of the same
+workload); the same runs are repeated with the PCG iterations replayed from a captured HIP graph
+and reported beside them.  A bounded sample of the same workload on the oracle (single-threaded
+C++ restatement of the reference CPU engine) is the CPU baseline.
+
+Prints ONE JSON line.  --gpus N > 1 is not supported (one GPU per box).
 """
 import argparse
 import json
@@ -21,17 +27,18 @@ sys.path.insert(0, os.path.join(ROOT, "tanque-de-agua_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-DOMINANT_KERNEL = "mg_up_l0"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_ultra_hourglass.json")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(tq, work, precision, warmup, steps, schedule=None, time_kernel=None, tol=1e-6, maxit=200):
+def run_gpu(tq, work, precision, warmup, steps, schedule=None, time_kernel=None, use_graph=False, tol=1e-6,
+            maxit=200, **opts):
     from tanque_hip.scenes import passo_frame
     cena = work.cena()
-    sim = tq.SimulacaoHIP(cena, precision=precision, tol=tol, iteracoes=maxit)
+    sim = tq.SimulacaoHIP(cena, precision=precision, tol=tol, iteracoes=maxit, use_graph=use_graph, **opts)
     sim.adicionar(work.particles, False)
     n0 = sim.n
     sched = []
@@ -76,11 +83,28 @@ def run_gpu(tq, work, precision, warmup, steps, schedule=None, time_kernel=None,
     return res
 
 
+def dominant_kernel(tq, work, precision, frames=3):
+    """Name of the kernel family with the largest device time on a few frames of the workload."""
+    from tanque_hip.scenes import passo_frame
+    cena = work.cena()
+    sim = tq.SimulacaoHIP(cena, precision=precision, tol=1e-6, iteracoes=200)
+    sim.adicionar(work.particles, False)
+    sim.avancar(work.dt_frame, passo_frame(work.h, cena.altura, sim.vMax, work.dt_frame))
+    sim.sync()
+    sim.timing_enable(True)
+    for _ in range(frames):
+        sim.avancar(work.dt_frame, passo_frame(work.h, cena.altura, sim.vMax, work.dt_frame))
+    sim.sync()
+    t = sim.timing_read()
+    sim.close()
+    name = max(t, key=lambda k: t[k]["ms"])
+    return name, t
+
+
 def cpu_baseline(work, budget_s=15.0, tol=1e-6, maxit=200):
     """The oracle (single-threaded C++ restatement of the reference CPU engine) on a bounded
     sample of the same workload: the first frames of the same scene, same particles."""
     from oracle import oracle as orc
-    from tanque_hip.scenes import passo_frame
     orc.build()
     cena = orc.Cena(work.nx, work.ny, work.h)
     for s in work.strokes:
@@ -95,9 +119,39 @@ def cpu_baseline(work, budget_s=15.0, tol=1e-6, maxit=200):
         subs += sub
         frames += 1
     el = time.perf_counter() - t0
-    return {"value": subs / el, "unit": "substeps/s", "cores": 1, "kind": "port",
-            "sample": "first %d frames (%d substeps) of the same workload, oracle JS-exact mode, tol %g, "
-                      "%.1f s" % (frames, subs, tol, el)}
+    return {"value": round(subs / el, 4), "unit": "substeps/s", "cores": 1, "kind": "port",
+            "sample": "first %d frames (%d substeps) of the same workload on the oracle (JS-exact binary64 "
+                      "compute / binary32 storage, PCG to 1e-6), %.1f s on one host core" % (frames, subs, el)}
+
+
+def copy_calibration(nbytes=256 << 20):
+    """One device-to-device copy of known size (the runtime's copy kernel), so that a
+    rocprofv3 --pmc run of this command can scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 0, ctypes.c_size_t(nbytes))
+    if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("copy calibration: %d bytes device to device" % nbytes)
+
+
+def traffic_for(kernel_symbol_hint):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy of known size), when it covers this kernel."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = d.get("kernels", {}).get(kernel_symbol_hint)
+    return k.get("hbm_bytes_per_launch") if k else None
 
 
 def main():
@@ -108,6 +162,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay runs")
+    ap.add_argument("--no-variants", action="store_true", help="skip the partial-FP16 variants")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="start with a 256 MiB device copy (reference for rocprofv3 --pmc byte counters)")
     ap.add_argument("--nx", type=int, default=896)
     ap.add_argument("--ny", type=int, default=504)
     args = ap.parse_args()
@@ -118,28 +176,51 @@ def main():
     import tanque_hip as tq
     from tanque_hip import scenes
     tq.load()
+    if args.copy_calibration:
+        copy_calibration()
     work = scenes.ultra_hourglass() if (args.nx, args.ny) == (896, 504) else scenes.scaled_hourglass(args.nx, args.ny)
     log("workload %s: %dx%d cells, %d particles seeded" % (work.name, work.nx, work.ny, len(work.particles) // 12))
 
-    mixed = run_gpu(tq, work, tq.PREC_MIXED, args.warmup, args.steps, time_kernel=DOMINANT_KERNEL)
+    dom, probe = dominant_kernel(tq, work, tq.PREC_MIXED)
+    log("dominant kernel (mixed): %s (%.1f%% of device time in the probe)" % (
+        dom, 100 * probe[dom]["ms"] / sum(v["ms"] for v in probe.values())))
+
+    mixed = run_gpu(tq, work, tq.PREC_MIXED, args.warmup, args.steps, time_kernel=dom)
     log("mixed: %.1f substeps/s, %.2f PCG it/solve, last rel resid %.2e (true %.2e)" % (
         mixed["substeps_per_s"], mixed["pcg_iters_per_solve"], mixed["last_rel_residual"],
         mixed["last_true_rel_residual"]))
+    sched = mixed["schedule"]
     fp32 = None
     if not args.no_fp32:
-        fp32 = run_gpu(tq, work, tq.PREC_FP32, args.warmup, args.steps, schedule=mixed["schedule"])
+        fp32 = run_gpu(tq, work, tq.PREC_FP32, args.warmup, args.steps, schedule=sched, time_kernel=dom)
         log("fp32:  %.1f substeps/s, %.2f PCG it/solve, last rel resid %.2e" % (
             fp32["substeps_per_s"], fp32["pcg_iters_per_solve"], fp32["last_rel_residual"]))
+    graph = {}
+    if not args.no_graph:
+        for lab, prec in (("mixed", tq.PREC_MIXED), ("fp32", tq.PREC_FP32)):
+            if lab == "fp32" and args.no_fp32:
+                continue
+            r = run_gpu(tq, work, prec, args.warmup, args.steps, schedule=sched, use_graph=True)
+            graph[lab] = {"substeps_per_s": round(r["substeps_per_s"], 2),
+                          "pcg_iters_per_solve": round(r["pcg_iters_per_solve"], 3)}
+            log("graph %s: %.1f substeps/s" % (lab, r["substeps_per_s"]))
+    variants = {}
+    if not args.no_variants:
+        for lab, prec, hl in (("fp16_b_only", tq.PREC_MIXED_B, -1), ("fp16_x_only_from_level2", tq.PREC_MIXED_X, 2)):
+            r = run_gpu(tq, work, prec, args.warmup, args.steps, schedule=sched, use_graph=True, mg_half_level=hl)
+            variants[lab] = {"substeps_per_s": round(r["substeps_per_s"], 2),
+                             "pcg_iters_per_solve": round(r["pcg_iters_per_solve"], 3)}
+            log("variant %s: %.1f substeps/s, %.2f it/solve" % (lab, r["substeps_per_s"], r["pcg_iters_per_solve"]))
 
-    kt = mixed["timing"].get(DOMINANT_KERNEL)
+    kt = mixed["timing"].get(dom)
     roofline = None
     if kt and kt["launches"]:
         avg_s = kt["ms"] / kt["launches"] * 1e-3
         achieved = kt["bytes_per_launch"] / avg_s / 1e9
-        roofline = {"bound": "hbm", "kernel": DOMINANT_KERNEL, "achieved": round(achieved, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": None, "avg_launch_us": round(avg_s * 1e6, 3),
-                    "bytes_per_launch": kt["bytes_per_launch"], "launches": kt["launches"]}
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                    "traffic": traffic_for(dom), "avg_launch_us": round(avg_s * 1e6, 3),
+                    "algorithmic_bytes_per_launch": kt["bytes_per_launch"], "launches": kt["launches"]}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -154,18 +235,21 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f16 MG storage, f64 dots",
+        "dtype": "f32 (MG storage f16, dots f64)",
         "data": "synthetic",
         "config": {"workload": "%s: %s, frames of %.6f s, PCG to rel. residual 1e-6" % (
             work.name, work.description, work.dt_frame),
             "cells": work.nx * work.ny, "particles": mixed["particles_start"],
             "substeps_timed": mixed["substeps"], "pcg_iters_per_solve": round(mixed["pcg_iters_per_solve"], 3),
             "last_rel_residual": mixed["last_rel_residual"], "last_true_rel_residual": mixed["last_true_rel_residual"],
-            "active_bubbles_end": mixed["active_bubbles"], "mg_levels": mixed["levels"]},
+            "active_bubbles_end": mixed["active_bubbles"], "mg_levels": mixed["levels"],
+            "launches": "eager, dominant kernel timed with HIP events"},
         "particle_steps_per_s": round(mixed["particle_steps_per_s"], 1),
         "fp32_substeps_per_s": round(fp32["substeps_per_s"], 2) if fp32 else None,
         "fp32_pcg_iters_per_solve": round(fp32["pcg_iters_per_solve"], 3) if fp32 else None,
         "mixed_over_fp32": round(mixed["substeps_per_s"] / fp32["substeps_per_s"], 4) if fp32 else None,
+        "graph_replay": graph or None,
+        "mixed_variants": variants or None,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -1677,7 +1677,11 @@ static void launch_coarse_wg(tq_sim* s) {
   for (int l = s->lc; l < L; l++) { P.offm[l] = off; off += (s->lv[l].n + 15) & ~15; }
   P.dense = s->dense;
   Level& G = s->lv[s->lc];
-  int t = time_begin(s, "mg_coarse_wg", 13.0 * G.n);
+  // b and x of level lc in, its result out, every LDS level's mask, the dense operator
+  double bytes = 12.0 * G.n;
+  for (int l = s->lc; l < L; l++) bytes += s->lv[l].n;
+  if (s->dense) bytes += 4.0 * s->lv[L - 1].n * s->lv[L - 1].n;
+  int t = time_begin(s, "mg_coarse_wg", bytes);
   hipLaunchKernelGGL(k_coarse_wg, dim3(1), dim3(1024), 0, s->st, P, (const float*)G.b, (const float*)G.x,
                      (float*)G.tmp, s->sc);
   time_end(s, t);

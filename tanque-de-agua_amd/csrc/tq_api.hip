@@ -22,36 +22,40 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
 }
 
 // ------------------------------------------------------------------ timing
-static const char* g_only_dummy = nullptr;
-struct TimingFilter { std::string only; };
-static thread_local TimingFilter g_filter;
+
+// waits for the stream and adds the pending event pairs to their kernels' totals
+static void timing_collect(tq_sim* s) {
+  (void)hipStreamSynchronize(s->st);
+  for (size_t k = 0; k < s->ev_pending.size(); k++) {
+    const auto& pe = s->ev_pending[k];
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, s->ev_pool[pe.second].first, s->ev_pool[pe.second].second) == hipSuccess) {
+      s->ktimes[pe.first].total_ms += ms;
+      s->ktimes[pe.first].launches += 1;
+      s->ktimes[pe.first].total_bytes += s->ev_pending_bytes[k];
+    }
+  }
+  s->ev_pending.clear();
+  s->ev_pending_bytes.clear();
+  s->ev_used = 0;
+}
 
 int time_begin(tq_sim* s, const char* name, double bytes) {
   if (!s->timing || s->capturing) return -1;
-  if (!g_filter.only.empty() && g_filter.only != name) return -1;
+  if (!s->timing_only.empty() && s->timing_only != name) return -1;
   int ki = -1;
   for (size_t i = 0; i < s->ktimes.size(); i++)
     if (s->ktimes[i].name == name) { ki = (int)i; break; }
   if (ki < 0) {
     KTime k;
     k.name = name;
-    k.bytes = bytes;
     s->ktimes.push_back(k);
     ki = (int)s->ktimes.size() - 1;
   }
   if (s->ev_used >= (int)s->ev_pool.size()) {
     if (s->ev_pool.size() >= 8192) {
       // flush: wait and accumulate what is pending, then reuse the pool
-      (void)hipStreamSynchronize(s->st);
-      for (auto& pe : s->ev_pending) {
-        float ms = 0;
-        if (hipEventElapsedTime(&ms, s->ev_pool[pe.second].first, s->ev_pool[pe.second].second) == hipSuccess) {
-          s->ktimes[pe.first].total_ms += ms;
-          s->ktimes[pe.first].launches += 1;
-        }
-      }
-      s->ev_pending.clear();
-      s->ev_used = 0;
+      timing_collect(s);
     } else {
       hipEvent_t a, b;
       if (hipEventCreate(&a) != hipSuccess) return -1;
@@ -62,6 +66,7 @@ int time_begin(tq_sim* s, const char* name, double bytes) {
   const int slot = s->ev_used++;
   (void)hipEventRecord(s->ev_pool[slot].first, s->st);
   s->ev_pending.emplace_back(ki, slot);
+  s->ev_pending_bytes.push_back(bytes);
   return slot;
 }
 
@@ -70,18 +75,7 @@ void time_end(tq_sim* s, int slot) {
   (void)hipEventRecord(s->ev_pool[slot].second, s->st);
 }
 
-static void timing_flush(tq_sim* s) {
-  (void)hipStreamSynchronize(s->st);
-  for (auto& pe : s->ev_pending) {
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, s->ev_pool[pe.second].first, s->ev_pool[pe.second].second) == hipSuccess) {
-      s->ktimes[pe.first].total_ms += ms;
-      s->ktimes[pe.first].launches += 1;
-    }
-  }
-  s->ev_pending.clear();
-  s->ev_used = 0;
-}
+static void timing_flush(tq_sim* s) { timing_collect(s); }
 
 // ------------------------------------------------------------------ allocation
 template <class T>
@@ -627,9 +621,8 @@ int tq_timing_enable(tq_sim* s, int enable, const char* only) {
   TQ_CHECK_SIM(s);
   if (s->timing) timing_flush(s);
   s->timing = enable != 0;
-  g_filter.only = only ? only : "";
+  s->timing_only = only ? only : "";
   if (enable) s->ktimes.clear();
-  (void)g_only_dummy;
   return 0;
 }
 
@@ -642,7 +635,7 @@ int tq_timing_read(tq_sim* s, tq_kernel_time* out, int max, int* n) {
     std::snprintf(out[k].name, sizeof out[k].name, "%s", s->ktimes[k].name.c_str());
     out[k].total_ms = s->ktimes[k].total_ms;
     out[k].launches = s->ktimes[k].launches;
-    out[k].bytes_per_launch = s->ktimes[k].bytes;
+    out[k].bytes_per_launch = s->ktimes[k].launches ? s->ktimes[k].total_bytes / s->ktimes[k].launches : 0.0;
   }
   if (n) *n = (int)s->ktimes.size();
   return 0;

@@ -103,7 +103,7 @@ struct KTime {
   std::string name;
   double total_ms = 0;
   long launches = 0;
-  double bytes = 0;
+  double total_bytes = 0;  // algorithmic bytes of the timed launches
 };
 
 }  // namespace tq
@@ -208,9 +208,11 @@ struct tq_sim {
 
   // timing
   bool timing = false;
+  std::string timing_only;      // time only launches of this name (empty: all)
   std::vector<tq::KTime> ktimes;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
   std::vector<std::pair<int, int>> ev_pending;  // (ktime index, event pair index)
+  std::vector<double> ev_pending_bytes;         // algorithmic bytes of each pending launch
   int ev_used = 0;
 };
 
