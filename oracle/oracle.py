@@ -85,6 +85,9 @@ def lib():
             "or_sim_write": (C.c_long, [vp, i, vp, i, i]),
             "or_sim_set_tracker_first": (None, [vp, i]),
             "or_sim_tracker_first": (i, [vp]),
+            "or_sim_mg_vcycle": (None, [vp, vp, vp]),
+            "or_sim_op_apply": (None, [vp, vp, vp]),
+            "or_sim_set_types_ready": (None, [vp, i]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -296,6 +299,24 @@ class Simulacao:
     @property
     def solido(self):
         return self.read(F_SOLIDO)
+
+    def vcycle(self, b):
+        """One V-cycle of the preconditioner on level-0 input b (test hook; current types)."""
+        b = np.ascontiguousarray(b, np.float64)
+        out = np.zeros_like(b)
+        lib().or_sim_mg_vcycle(self._p, b.ctypes.data, out.ctypes.data)
+        return out
+
+    def aplicar(self, v):
+        """The pressure operator A on the water cells (test hook; current types)."""
+        v = np.ascontiguousarray(v, np.float64)
+        out = np.zeros_like(v)
+        lib().or_sim_op_apply(self._p, v.ctypes.data, out.ctypes.data)
+        return out
+
+    def set_tipos_prontos(self, v=True):
+        """Test hook: the stored types count as current for the next air update."""
+        lib().or_sim_set_types_ready(self._p, 1 if v else 0)
 
     def set_primeira(self, v):
         lib().or_sim_set_tracker_first(self._p, 1 if v else 0)

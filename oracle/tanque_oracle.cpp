@@ -803,6 +803,41 @@ static void resolver_pcg(Solver<T>& S, const uint8_t* tipo, const T* div, T* q, 
   if (rot) for (int b : ativos) yB[b] = (T)yb.get(b);
 }
 
+// Test hooks (no counterpart in the reference): the level setup resolverPCG performs
+// (pressao-cpu.js:303-312) on the current types and labels, then one V-cycle on a given
+// level-0 right-hand side, or the operator A (aplicar, pressao-cpu.js:84-93) on a vector.
+template <class T>
+static void mg_setup(Solver<T>& S, const uint8_t* tipo, const uint32_t* rot) {
+  Nivel<T>& n0 = S.niveis[0];
+  const int k = n0.nx * n0.ny;
+  std::copy(tipo, tipo + k, S.op.tipo.begin());
+  preparar_vizinhos(S.op);
+  std::copy(tipo, tipo + k, n0.tipo.begin());
+  if (rot) for (int c = 0; c < k; c++) if (tipo[c] == AR && rot[c] >= 2) n0.tipo[c] = AGUA;
+  tipos_grossos(S.niveis);
+  for (auto& n : S.niveis) preparar_vizinhos(n);
+}
+template <class T>
+static void mg_vcycle_hook(Solver<T>& S, const uint8_t* tipo, const uint32_t* rot, const double* in, double* out) {
+  mg_setup(S, tipo, rot);
+  Nivel<T>& n0 = S.niveis[0];
+  const int k = n0.nx * n0.ny;
+  for (int c = 0; c < k; c++) n0.b[c] = (T)in[c];
+  ciclo_v(S.niveis);
+  for (int c = 0; c < k; c++) out[c] = (double)n0.x[c];
+}
+template <class T>
+static void op_apply_hook(Solver<T>& S, const uint8_t* tipo, const double* in, double* out) {
+  mg_setup(S, tipo, nullptr);
+  const int nx = S.op.nx, ny = S.op.ny;
+  std::vector<T> v(in, in + (size_t)nx * ny);
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      const int c = i + j * nx;
+      out[c] = tipo[c] == AGUA ? aplicar(S.op, v.data(), i, j) : 0.0;
+    }
+}
+
 // ---------------------------------------------------------------- SimulacaoCPU (js/fisica-cpu.js)
 template <class T>
 struct Sim {
@@ -1538,5 +1573,14 @@ long or_sim_write(OrSim* s, int f, const void* src, int as_double, int count) {
 }
 void or_sim_set_tracker_first(OrSim* s, int primeira) { DISPATCH(s, (S->rastreador.primeira = primeira != 0, 0)); }
 int or_sim_tracker_first(OrSim* s) { return DISPATCH(s, S->rastreador.primeira ? 1 : 0); }
+// test hooks: one V-cycle / the operator on the sim's current types (labels for the V-cycle)
+void or_sim_mg_vcycle(OrSim* s, const double* in, double* out) {
+  DISPATCH(s, (mg_vcycle_hook(S->solver, S->tipo.data(), S->arAtivo ? S->rastreador.rotulo.data() : nullptr, in, out), 0));
+}
+// test hook: treat the stored types as current (atualizarAr then skips the P2G it would run)
+void or_sim_set_types_ready(OrSim* s, int ready) { DISPATCH(s, (S->tiposProntos = ready != 0, 0)); }
+void or_sim_op_apply(OrSim* s, const double* in, double* out) {
+  DISPATCH(s, (op_apply_hook(S->solver, S->tipo.data(), in, out), 0));
+}
 
 }  // extern "C"

@@ -1,0 +1,167 @@
+"""The reference's physics tests (testes/fisica.test.js) on the HIP path: the same scenarios,
+assertions and thresholds as test_oracle_reference_suite.py, run through the C-ABI library.
+
+The test bodies are shared with the oracle suite: they receive a backend object with the
+oracle module's surface (Cena, Simulacao, semear_retangulo, Semente, rodar).  HipBackend maps it
+onto tanque_hip; the only oracle piece it uses is the scene's distance query, which the tests'
+own measurements need (menor_distancia_a_parede) and which the library does not export.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import test_oracle_reference_suite as S
+
+pytestmark = pytest.mark.gpu
+
+
+class HipCena:
+    """The library's scene (uploaded to the device) plus an oracle mirror for distance probes."""
+
+    def __init__(self, be, nx, ny, h):
+        self._t = be.tq.Cena(nx, ny, h)
+        self._o = be.orc.Cena(nx, ny, h)
+        self.nx, self.ny, self.h = nx, ny, h
+        self.largura, self.altura = self._t.largura, self._t.altura
+
+    @property
+    def sdf(self):
+        return self._t.sdf
+
+    @property
+    def tracos(self):
+        return self._t.tracos
+
+    def tracar(self, x0, y0, x1, y1, raio, apagar=False):
+        self._t.tracar(x0, y0, x1, y1, raio, apagar)
+        self._o.tracar(x0, y0, x1, y1, raio, apagar)
+
+    def distancia(self, x, y):
+        return self._o.distancia(x, y)
+
+
+class HipSim:
+    """SimulacaoHIP with the oracle wrapper's attribute names."""
+
+    def __init__(self, be, cena, gravidade=(0.0, -9.81), iteracoes=3, tau=0.1, tensao=0.0728, ar=True):
+        self._g = be.tq.SimulacaoHIP(cena, gravidade=gravidade, iteracoes=iteracoes, tauDensidade=tau,
+                                     tensao=tensao, ar=ar)
+        self.cena, self.nx, self.ny, self.h = cena, cena.nx, cena.ny, cena.h
+
+    def adicionar(self, lista, checar=True):
+        return self._g.adicionar(lista, checar)
+
+    def avancar(self, dt, subpassos=1):
+        self._g.avancar(dt, subpassos)
+
+    def particulas(self):
+        return self._g.particulas()
+
+    def area(self):
+        return self._g.area()
+
+    def limpar_agua(self):
+        self._g.limparAgua()
+
+    @property
+    def n(self):
+        return self._g.n
+
+    @property
+    def vmax(self):
+        return self._g.vMax
+
+    @property
+    def p(self):
+        return self._g.p
+
+    @property
+    def tipo(self):
+        return self._g.read(9)
+
+    @property
+    def rotulo(self):
+        return self._g.read(18)
+
+    @property
+    def contagem(self):
+        return self._g.read(7)
+
+    @property
+    def solido(self):
+        return self._g.read(10)
+
+
+class HipBackend:
+    def __init__(self, tq, orc):
+        self.tq, self.orc = tq, orc
+        self.Semente = tq.Semente
+        self.semear_retangulo = tq.semear_retangulo
+
+    def Cena(self, nx, ny, h):
+        return HipCena(self, nx, ny, h)
+
+    def Simulacao(self, cena, **kw):
+        return HipSim(self, cena, **kw)
+
+    def rodar(self, sim, segundos, a_cada=None, quadro=1 / 60):
+        """testes/fisica.test.js:12-19 with the library's passoMaximo."""
+        quadros = int(math.floor(segundos / quadro + 0.5))
+        for f in range(quadros):
+            sub = math.ceil(quadro / self.tq.passo_maximo(sim.h, sim.cena.altura, sim.vmax, quadro))
+            sim.avancar(quadro, sub)
+            if a_cada:
+                a_cada(f * quadro)
+
+
+@pytest.fixture(scope="module")
+def hip(oracle):
+    import tanque_hip
+    tanque_hip.load()
+    return HipBackend(tanque_hip, oracle)
+
+
+@pytest.mark.parametrize("nome,tensao,limite", [("sem tensao", 0.0, 0.01), ("com tensao", 0.0728, 0.02)])
+def test_agua_parada_fica_parada(hip, nome, tensao, limite):  # fisica.test.js:31-43
+    S.test_agua_parada_fica_parada(hip, nome, tensao, limite)
+
+
+def test_tensao_arredonda_gota_quadrada(hip):  # fisica.test.js:45-66
+    S.test_tensao_arredonda_gota_quadrada(hip)
+
+
+def test_copo_nao_vaza_nem_perde_agua(hip):  # fisica.test.js:68-76
+    S.test_copo_nao_vaza_nem_perde_agua(hip)
+
+
+def test_parede_fina_segura_jato(hip):  # fisica.test.js:78-94
+    S.test_parede_fina_segura_jato(hip)
+
+
+def test_volume_conservado_no_rompimento(hip):  # fisica.test.js:96-116
+    S.test_volume_conservado_no_rompimento(hip)
+
+
+def test_vasos_comunicantes(hip):  # fisica.test.js:118-138
+    S.test_vasos_comunicantes(hip)
+
+
+def test_densidade_perto_do_repouso(hip):  # fisica.test.js:140-159
+    S.test_densidade_perto_do_repouso(hip)
+
+
+def test_bolha_sobe_mantendo_volume(hip):  # fisica.test.js:190-200
+    S.test_bolha_sobe_mantendo_volume(hip)
+
+
+def test_remontar_nao_perde_bolha(hip):  # fisica.test.js:202-222
+    S.test_remontar_nao_perde_bolha(hip)
+
+
+def test_sem_ar_buraco_fecha(hip):  # fisica.test.js:224-230
+    S.test_sem_ar_buraco_fecha(hip)
+
+
+def test_ampulheta_com_ar_esvazia_devagar(hip):  # fisica.test.js:232-250
+    S.test_ampulheta_com_ar_esvazia_devagar(hip)
