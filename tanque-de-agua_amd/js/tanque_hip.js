@@ -1,0 +1,167 @@
+'use strict';
+// tanque_hip.js -- the MI355X substep for the reference's JavaScript host (CommonJS, Node >= 12).
+//
+// SimulacaoHIP keeps SimulacaoCPU's constructor and members (js/fisica-cpu.js:28-194): the same
+// options, `n` (settable to truncate), `p` / `particulas()`, `vMax`, `gravidade`, the typed state
+// arrays as read-only copies, and the methods atualizarParedes, adicionar, aplicarFerramenta,
+// avancar, subpasso, atualizarAr, reiniciarAr, area.  It accepts the reference's own Cena
+// (js/cena.js: nx, ny, h, sdf) or the Cena below.  MotorHIP has MotorCPU's surface
+// (js/motor.js:79-120) without the 2D renderer.
+const path = require('path');
+
+const nativo = require(path.join(__dirname, '..', 'lib', 'tanque_node.node'));
+
+const CAMPOS = 12;                      // floats per particle (fisica-cpu.js:26)
+const PARTICULAS_MAX_POR_CELULA = 4.6;  // constantes.js
+const CAMPO = {
+  u: 0, v: 1, uPeso: 2, vPeso: 3, uValida: 4, vValida: 5, densidade: 6, contagemCelula: 7,
+  corCelula: 8, tipo: 9, solidoCelula: 10, div: 11, q: 12, fase: 13, normalX: 14, normalY: 15,
+  curvatura: 16, qAr: 17, rotulo: 18, pressaoBolha: 19, volumeBolha: 20, rhsBolha: 21, yBolha: 22,
+  volumeAlvo: 23, vivo: 24,
+};
+
+/** Walls as an SDF on the reference's 2x2-per-cell lattice (js/cena.js), built by the library. */
+class Cena {
+  constructor(nx, ny, h) {
+    this.nx = nx;
+    this.ny = ny;
+    this.h = h;
+    this.largura = nx * h;
+    this.altura = ny * h;
+    this.sdf = nativo.cenaMoldura(nx, ny);
+    this.tracos = [];
+    this.linhasAlteradas = [0, 2 * ny - 1];
+  }
+
+  /** tracar (cena.js:43-72): a capsule stroke, `apagar` erases. */
+  tracar(x0, y0, x1, y1, raio, apagar = false) {
+    nativo.tracar(this.sdf, this.nx, this.ny, this.h, x0, y0, x1, y1, raio, apagar);
+    this.tracos.push([x0, y0, x1, y1, raio, apagar]);
+    this.linhasAlteradas = [0, 2 * this.ny - 1];
+  }
+
+  /** pegarLinhasAlteradas (cena.js:95-99). */
+  pegarLinhasAlteradas() {
+    const l = this.linhasAlteradas;
+    this.linhasAlteradas = null;
+    return l;
+  }
+}
+
+/** semente (testes/apoio.js:4-9): the seeded LCG; the state is advanced by semearRetangulo. */
+class Semente {
+  constructor(s = 1) { this.estado = s >>> 0; }
+}
+
+/** semearRetangulo (fisica-cpu.js:705-727) with a Semente. */
+function semearRetangulo(x0, y0, x1, y1, h, absorcao = [0, 0, 0], semente = new Semente(1)) {
+  const r = nativo.semearRetangulo(x0, y0, x1, y1, h, absorcao, semente.estado);
+  semente.estado = r.semente >>> 0;
+  return r.particulas;
+}
+
+/** passoMaximo (constantes.js:167-175). */
+function passoMaximo(h, altura, vMax, dtQuadro) {
+  return nativo.passoMaximo(h, altura, vMax === undefined ? null : vMax, dtQuadro);
+}
+
+class SimulacaoHIP {
+  constructor(cena, opcoes = {}) {
+    this.cena = cena;
+    this.nx = cena.nx;
+    this.ny = cena.ny;
+    this.h = cena.h;
+    this._opcoes = Object.assign({}, opcoes);
+    this._g = opcoes.gravidade !== undefined && opcoes.gravidade !== null ? opcoes.gravidade.slice() : [0, -9.81];
+    this._n = new nativo.SimulacaoHIP(cena.nx, cena.ny, cena.h, opcoes);
+    this.atualizarParedes();
+  }
+
+  /** atualizarParedes (fisica-cpu.js:80-85): every solid cell from the scene's SDF. */
+  atualizarParedes(linha0 = 0, linha1 = 2 * this.ny - 1) { this._n.definirSdf(this.cena.sdf, linha0, linha1); }
+  /** adicionar (fisica-cpu.js:92-111) -> accepted count. */
+  adicionar(lista, checarCheia = true) { return this._n.adicionar(lista, checarCheia); }
+  /** aplicarFerramenta (fisica-cpu.js:129-152). */
+  aplicarFerramenta(f) { if (f && f.tipo && f.tipo !== 'nenhuma') this._n.aplicarFerramenta(f); }
+  /** avancar (fisica-cpu.js:155-159). */
+  avancar(dt, subpassos = 1) { this._n.avancar(dt, subpassos); }
+  /** subpasso (fisica-cpu.js:176-189). */
+  subpasso(dt) { this._n.subpasso(dt); }
+  /** atualizarAr (fisica-cpu.js:162-167). */
+  atualizarAr() { this._n.atualizarAr(); }
+  /** reiniciarAr (fisica-cpu.js:170-174). */
+  reiniciarAr() { this._n.reiniciarAr(); }
+  /** area (fisica-cpu.js:192-194). */
+  area() { return this._n.area(); }
+  /** MotorCPU.limparAgua's effect (motor.js:95): n = 0, counts cleared, air reset. */
+  limparAgua() { this._n.limparAgua(); }
+
+  get n() { return this._n.contagem(); }
+  set n(m) { if (m === 0) this._n.limparAgua(); else this._n.truncar(m); }
+  get vMax() { return this._n.lerVMax(); }
+  /** The first n particle records, in the reference's order (a copy). */
+  get p() { return this._n.particulas(); }
+  particulas() { return this._n.particulas(); }
+
+  get gravidade() { return this._g.slice(); }
+  set gravidade(g) {
+    this._g = g.slice();
+    this._opcoes.gravidade = this._g;
+    this._n.opcoes(this._opcoes);
+  }
+
+  lerCampo(nome) { return this._n.lerCampo(CAMPO[nome]); }
+  get tipo() { return this.lerCampo('tipo'); }
+  get rotulo() { return this.lerCampo('rotulo'); }
+  get contagemCelula() { return this.lerCampo('contagemCelula'); }
+  get solidoCelula() { return this.lerCampo('solidoCelula'); }
+  get q() { return this.lerCampo('q'); }
+
+  esperar() { this._n.esperar(); }
+  destruir() { this._n.destruir(); }
+}
+
+/** MotorCPU's surface (js/motor.js:79-120) on SimulacaoHIP; drawing is the caller's. */
+class MotorHIP {
+  constructor(canvas, cena, opcoes = {}) {
+    this.tipo = 'hip';
+    this.nome = 'MI355X';
+    this.canvas = canvas;
+    this.cena = cena;
+    this.sim = new SimulacaoHIP(cena, opcoes);
+    this.pendentes = [];
+    this.custo = null;
+  }
+
+  get h() { return this.cena.h; }
+  get maxParticulas() { return Math.ceil(this.sim.nx * this.sim.ny * PARTICULAS_MAX_POR_CELULA); }
+  get vMax() { const v = this.sim.vMax; return v === undefined ? null : v; }
+  paredesMudaram() { this.sim.atualizarParedes(); }
+  adicionar(lista, checar = true) { this.pendentes.push([lista, checar]); }
+  limparAgua() { this.sim.limparAgua(); this.pendentes = []; }
+  set gravidade(g) { this.sim.gravidade = g; }
+
+  quadro(dt, subpassos, ferramenta) {
+    this.sim.aplicarFerramenta(ferramenta);
+    const teto = this.maxParticulas;
+    for (const [l, c] of this.pendentes) {
+      const cabe = Math.max(0, teto - this.sim.n) * CAMPOS;
+      if (cabe > 0) this.sim.adicionar(l.length > cabe ? l.subarray(0, cabe) : l, c);
+    }
+    this.pendentes = [];
+    if (subpassos > 0) {
+      const t0 = process.hrtime.bigint();
+      this.sim.avancar(dt, subpassos);
+      const ms = Number(process.hrtime.bigint() - t0) / 1e6 / subpassos;
+      this.custo = this.custo === null ? ms : this.custo + (ms - this.custo) * 0.2;
+    }
+  }
+
+  get custoPorSubpasso() { return this.custo; }
+  async contagem() { return this.sim.n; }
+  async particulas() { return this.sim.particulas(); }
+  async esperar() { this.sim.esperar(); }
+  destruir() { this.sim.destruir(); }
+}
+
+module.exports = { SimulacaoHIP, MotorHIP, Cena, Semente, semearRetangulo, passoMaximo, CAMPOS, CAMPO };

@@ -1,0 +1,63 @@
+"""The Node-API addon (the JavaScript host's binding of the C ABI) and its JS wrappers
+(tanque-de-agua_amd/js/tanque_hip.js): host helpers on the CPU, and on the GPU a dam break
+driven through MotorHIP in node gives particles bit-identical to the same run through the
+ctypes binding (same library, deterministic kernels)."""
+import base64
+import json
+import math
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "tanque-de-agua_amd", "lib", "tanque_node.node")
+DRIVER = os.path.join(ROOT, "tests", "js", "run_dam.js")
+
+needs_node = pytest.mark.skipif(shutil.which("node") is None or not os.path.exists(ADDON),
+                                reason="node or the built addon is absent")
+
+
+def run_node(*args, timeout=300):
+    r = subprocess.run(["node", DRIVER] + [str(a) for a in args], capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    d["P"] = np.frombuffer(base64.b64decode(d.pop("particulas")), np.float32).reshape(-1, 12)
+    return d
+
+
+@needs_node
+def test_addon_host_helpers_match_the_library():
+    import tanque_hip as tq
+    tq.load()
+    h = 0.01
+    d = run_node(80, 60, h, 0.1, "host-only")
+    sem = tq.Semente(9)
+    P = tq.semear_retangulo(2 * h, 2 * h, 0.3, 0.42, h, (0, 0, 0), sem).reshape(-1, 12)
+    assert np.array_equal(d["P"], P)
+    assert d["semente"] == sem.state.value
+    assert d["passo"] == tq.passo_maximo(h, 60 * h, None, 1 / 60)
+
+
+@needs_node
+@pytest.mark.gpu
+def test_motor_hip_in_node_matches_ctypes_run():
+    import tanque_hip as tq
+    tq.load()
+    nx, ny, h, seconds = 80, 60, 0.01, 0.5
+    d = run_node(nx, ny, h, seconds)
+    cena = tq.Cena(nx, ny, h)
+    sim = tq.SimulacaoHIP(cena)
+    sim.adicionar(tq.semear_retangulo(2 * h, 2 * h, 0.3, 0.42, h, (0, 0, 0), tq.Semente(9)), False)
+    quadro = 1 / 60
+    for _ in range(int(math.floor(seconds / quadro + 0.5))):
+        sub = math.ceil(quadro / tq.passo_maximo(h, cena.altura, sim.vMax, quadro))
+        sim.avancar(quadro, sub)
+    P = sim.particulas()
+    print("\n  node MotorHIP: n %d, vMax %.4f, %.3f ms/substep; ctypes n %d" % (
+        d["n"], d["vMax"], d["custoPorSubpasso"], len(P)))
+    assert d["n"] == len(P)
+    assert np.array_equal(d["P"], P)
+    assert d["vMax"] == sim.vMax
