@@ -9,21 +9,21 @@
 //
 // Multigrid.  The reference smoother (suavizar, pressao-cpu.js:99-150) runs 16x16 tiles that
 // each read a snapshot of x (their halo stays at the snapshot) through 4 red-black sweeps.  One
-// wavefront owns one tile: each lane holds four cells in registers (columns 2p, 2p+1 of rows
-// r, r+8), so every lane updates one cell per row in every half-sweep; neighbours come from
-// the wave's private 18x18 LDS tile, synchronised by an LDS wait (no barrier).  Launches per
-// large level: down = [restriction from the finer level + first pass from x = 0] and the
-// second pass; up = [prolongation folded into the snapshot + first pass] and the second pass.
-// Level 0's first down pass also applies the PCG update (q += alpha p, r -= alpha Ap) and
-// builds the V-cycle input; its last up pass writes z and the r.z dots.  The small levels run
-// in one workgroup from LDS.  Buffer roles per level: pass A writes x, pass B x -> tmp, the
-// restriction reads tmp, prolongation+C tmp -> x, pass D x -> tmp (the level's final x).
+// wavefront owns one tile: each lane holds four cells in registers (one red and one black in
+// each of two rows), so every lane updates one cell per row in every half-sweep; neighbours come
+// from the wave's private 18x18 LDS tile, synchronised by an LDS wait (no barrier).  A dependent
+// kernel costs ~4-5 us here while one pass over a level costs well under 1 us of arithmetic, so
+// consecutive passes are fused and the overlapping tiles recomputed: one kernel per level down
+// (pass B + restriction + the coarse level's pass A) and one per level up (prolongation +
+// pass C + pass D).  Level 0's pass A also applies the PCG update and builds the V-cycle input;
+// its pass D writes z and the r.z dots.  The small levels run in one workgroup from LDS, the
+// coarsest as a dense operator.  Buffer roles per level: pass A writes x, pass B x -> tmp (x_B),
+// the up kernel tmp -> x (the level's result).
 //
 // Kernel sequence of one PCG iteration:
 //   spmv (p = z + beta p fused, Ap, p.Ap) -> fin_spmv (alpha, bubble y, r)
 //   in_a0 (update, |r|^2, V-cycle input, pass A on level 0) -> fin_update (norm, stop test)
-//   pass B_0, [restrict+A_l, B_l] for 0 < l < lc, restrict+A_lc, coarse_wg (levels >= lc),
-//   [prolong+C_l, D_l] for l = lc-1..0 (D_0 writes z and the dots) -> fin_zdots (beta)
+//   down_l for l < lc, coarse_wg (levels >= lc), up_l for l = lc-1..0 -> fin_zdots (beta)
 // Every kernel returns at once when the solve is done.
 #include "tq_device.hpp"
 
@@ -566,30 +566,54 @@ __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMas
   if (UPD) store_partials<1>(rr, 1, partials, sh);
 }
 
-// ================================================================== one smoothing pass
-template <class SX, class SB, int C0>
-__global__ __launch_bounds__(256) void k_pass(const SX* __restrict__ src, SX* __restrict__ dst,
-                                              const SB* __restrict__ b, const uint8_t* __restrict__ mask, int nx,
-                                              int ny, int desl, int ntx, int nty,
-                                              const DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ float Tt[4][TCELLS];
-  const int lane = threadIdx.x & 63;
-  int tx, ty;
-  block_tile(&tx, &ty);
-  if (tx >= ntx || ty >= nty) return;
-  const int ox = tx * 16 - desl, oy = ty * 16 - desl;
-  float* T = Tt[threadIdx.x >> 6];
-  Own4 own;
-  prefetch_cells<SB>(own, b, mask, ox, oy, nx, ny, lane);
-  fill_tile<SX>(T, src, ox, oy, nx, ny, lane);
-  wave_sync();
-  Cells4 cc;
-  init_cells(cc, own, T, lane);
-  tile_sweeps<C0>(T, cc, lane);
-  store_cells<SX>(cc, dst, ox, oy, nx, ny, lane);
+struct ZArgs {  // level-0 outputs: z = MG result and the dots r.z, r.z_old
+  const uint8_t* opMask;
+  const uint32_t* rot;
+  const uint32_t* cellsB;
+  const int* labelSlot;
+  const float* r;
+  const float* zold;
+  float* znew;
+  double* bvec;
+  double* partials;
+  double* bpart;
+  int scaled;
+};
+
+// z and its dot/bubble contributions for one cell (x = MG output of level 0)
+__device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, float zoldc, float xv, float inv,
+                              double (&dv)[2], double (&sl)[BSLOTS]) {
+  const float x = xv * inv;
+  float z = 0.f;
+  if (mask_type(om) == AGUA) {
+    z = x;
+    dv[0] += (double)rc * z;
+    dv[1] += (double)rc * zoldc;
+  } else if (om & MK_BUB) {
+    const uint32_t lb = za.rot[c];
+    bub_add(sl, za.labelSlot, bv(za.bvec, BV_ACC1), lb, (double)x / za.cellsB[lb]);
+  }
+  za.znew[c] = z;
 }
 
+// z and the dots when level 0 itself is solved inside the workgroup (small grids)
+__global__ __launch_bounds__(256) void k_zdots0(const float* __restrict__ x0, ZArgs za, int C,
+                                                const DevScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double sh[BSLOTS * 16];
+  const int ns = nslots(sc->active_bubbles);
+  double dv[2] = {0, 0};
+  double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const float inv = za.scaled ? 1.f / sc->mg_scale : 1.f;
+  if (c < C) z_cell(za, c, za.opMask[c], za.r[c], za.zold[c], x0[c], inv, dv, sl);
+  store_partials<2>(dv, 2, za.partials, sh);
+  if (ns > 0) store_partials<BSLOTS>(sl, ns, za.bpart, sh);
+}
+
+// ================================================================== up on level 0 (unfused)
+// Level 0 is large enough that recomputing pass C four times costs more than a kernel
+// boundary, so its up path stays two kernels: prolongation + pass C, then pass D writing z.
 // prolongar (pressao-cpu.js:191-214) folded into the snapshot of the first up pass (offset 8,
 // colours 1,0): water cells read x + P e, with the coarse values of the tile's 10x10 patch
 template <class SX, class SB, class SC>
@@ -671,36 +695,6 @@ __global__ __launch_bounds__(256) void k_prolong_pass(const SX* __restrict__ src
   store_cells<SX>(cc, dst, ox, oy, nx, ny, lane);
 }
 
-struct ZArgs {  // level-0 outputs: z = MG result and the dots r.z, r.z_old
-  const uint8_t* opMask;
-  const uint32_t* rot;
-  const uint32_t* cellsB;
-  const int* labelSlot;
-  const float* r;
-  const float* zold;
-  float* znew;
-  double* bvec;
-  double* partials;
-  double* bpart;
-  int scaled;
-};
-
-// z and its dot/bubble contributions for one cell (x = MG output of level 0)
-__device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, float zoldc, float xv, float inv,
-                              double (&dv)[2], double (&sl)[BSLOTS]) {
-  const float x = xv * inv;
-  float z = 0.f;
-  if (mask_type(om) == AGUA) {
-    z = x;
-    dv[0] += (double)rc * z;
-    dv[1] += (double)rc * zoldc;
-  } else if (om & MK_BUB) {
-    const uint32_t lb = za.rot[c];
-    bub_add(sl, za.labelSlot, bv(za.bvec, BV_ACC1), lb, (double)x / za.cellsB[lb]);
-  }
-  za.znew[c] = z;
-}
-
 // last up pass of level 0 (offset 0, colours 1,0) writing z and the dot partials
 template <class SB>
 __global__ __launch_bounds__(256) void k_pass_z(const float* __restrict__ src, const SB* __restrict__ b,
@@ -753,53 +747,51 @@ __global__ __launch_bounds__(256) void k_pass_z(const float* __restrict__ src, c
   if (ns > 0) store_partials<BSLOTS>(sl, ns, za.bpart, sh);
 }
 
-// z and the dots when level 0 itself is solved inside the workgroup (small grids)
-__global__ __launch_bounds__(256) void k_zdots0(const float* __restrict__ x0, ZArgs za, int C,
-                                                const DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ double sh[BSLOTS * 16];
-  const int ns = nslots(sc->active_bubbles);
-  double dv[2] = {0, 0};
-  double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const float inv = za.scaled ? 1.f / sc->mg_scale : 1.f;
-  if (c < C) z_cell(za, c, za.opMask[c], za.r[c], za.zold[c], x0[c], inv, dv, sl);
-  store_partials<2>(dv, 2, za.partials, sh);
-  if (ns > 0) store_partials<BSLOTS>(sl, ns, za.bpart, sh);
-}
+// ================================================================== down: pass B + restriction + pass A
+// One block owns a 16x16 tile of the coarse level l+1.  Its restriction (restringir,
+// pressao-cpu.js:159-188) reads the fine residual r = b - A x_B on the 34x34 fine cells
+// 2I-1 .. 2I+2 around it, i.e. x_B on a 36x36 window, which nine of level l's pass-B tiles
+// (offset 8, suavizar pressao-cpu.js:99-150) cover: nine wavefronts compute them (each B tile is
+// recomputed by the blocks whose windows overlap it, cheap next to a kernel boundary) into a
+// 48x48 LDS window.  The block writes x_B of its own 32x32 fine cells (the up path needs it),
+// then the coarse b, then wave 0 runs the coarse level's pass A (offset 0 = this tile) from x = 0.
+constexpr int BRA_WAVES = 9;
+constexpr int XBW = 48;  // x_B window: fine cells 2*CI0 - 8 .. 2*CI0 + 39
 
-// ================================================================== restriction + first pass of the coarse level
-// restringir (pressao-cpu.js:159-188): the block owns a 16x16 coarse tile; fine residuals
-// r = b - A x on the 34x34 fine cells it gathers from (2I-1 .. 2I+2 per axis, weights
-// 1/4 3/4 3/4 1/4), from x on 36x36, in LDS.  Then wave 0 runs the coarse level's pass A
-// (offset 0 = this tile) from x = 0 unless the coarse level is the coarsest.
 template <class SXF, class SBF, class SCB, class SCX, bool PASSA>
-__global__ __launch_bounds__(256) void k_restrict_a(const SXF* __restrict__ fx, const SBF* __restrict__ fb,
-                                                    const uint8_t* __restrict__ fm, int fnx, int fny,
-                                                    SCB* __restrict__ cb, SCX* __restrict__ cxo,
-                                                    const uint8_t* __restrict__ cmk, int cnx, int cny,
-                                                    const DevScalars* __restrict__ sc) {
+__global__ __launch_bounds__(64 * BRA_WAVES) void k_bra(const SXF* __restrict__ fx, SXF* __restrict__ fxb,
+                                                       const SBF* __restrict__ fb, const uint8_t* __restrict__ fm,
+                                                       int fnx, int fny, SCB* __restrict__ cb, SCX* __restrict__ cxo,
+                                                       const uint8_t* __restrict__ cmk, int cnx, int cny,
+                                                       const DevScalars* __restrict__ sc) {
   if (sc->done) return;
-  __shared__ float X[36 * 36], R[34 * 34], Bt[256], T[TCELLS];
-  const int tid = threadIdx.x;
+  __shared__ float Tt[BRA_WAVES][TCELLS];
+  __shared__ float XB[XBW * XBW];
+  __shared__ float R[34 * 34];
+  __shared__ float Bt[256];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int CI0 = blockIdx.x * 16, CJ0 = blockIdx.y * 16;
-  const int fx0 = 2 * CI0 - 2, fy0 = 2 * CJ0 - 2;  // X origin
-  // every global load first: the fine masks and b of the residual cells, the coarse masks
-  // (this thread's coarse cell and, for pass A, wave 0's four cells), then x
-  uint8_t mr[5];
-  float br[5];
+  const int wx0 = 2 * CI0 - 8, wy0 = 2 * CJ0 - 8;   // window origin (fine)
+  const int ox = wx0 + 16 * (wave % 3), oy = wy0 + 16 * (wave / 3);  // this wave's B tile
+  float* T = Tt[wave];
+  // ---- every global load first: the B tile's cells, the residual cells' masks and b, the
+  // coarse masks (this thread's coarse cell, wave 0's pass-A cells), the B tile's snapshot
+  Own4 own;
+  prefetch_cells<SBF>(own, fb, fm, ox, oy, fnx, fny, lane);
+  uint8_t mr[3];
+  float br[3];
 #pragma unroll
-  for (int q = 0; q < 5; q++) {
-    const int e = tid + 256 * q;
+  for (int q = 0; q < 3; q++) {
+    const int e = tid + 64 * BRA_WAVES * q;
     const int ly = e / 34, lx = e - ly * 34;
-    const int i = fx0 + 1 + lx, j = fy0 + 1 + ly;
+    const int i = 2 * CI0 - 1 + lx, j = 2 * CJ0 - 1 + ly;
     const bool in = e < 34 * 34 && i >= 0 && j >= 0 && i < fnx && j < fny;
     const size_t c = in ? (size_t)i + (size_t)j * fnx : 0;
     mr[q] = in ? fm[c] : 0;
     br[q] = in ? ld<SBF>(fb, c) : 0.f;
   }
-  const int Im = CI0 + (tid & 15), Jm = CJ0 + (tid >> 4);
-  const bool cin = Im < cnx && Jm < cny;
+  const int Im = CI0 + (tid & 15), Jm = CJ0 + ((tid >> 4) & 15);
+  const bool cin = tid < 256 && Im < cnx && Jm < cny;
   const size_t Cm = cin ? (size_t)Im + (size_t)Jm * cnx : 0;
   const uint8_t cmine = cin ? cmk[Cm] : 0;
   uint8_t cma[2][2];
@@ -807,44 +799,55 @@ __global__ __launch_bounds__(256) void k_restrict_a(const SXF* __restrict__ fx, 
   for (int cq = 0; cq < 2; cq++)
 #pragma unroll
     for (int rw = 0; rw < 2; rw++) {
-      const int I = CI0 + cell_lx(tid & 63, cq), J = CJ0 + cell_ly(tid & 63, rw);
-      cma[cq][rw] = (PASSA && I < cnx && J < cny) ? cmk[(size_t)I + (size_t)J * cnx] : 0;
+      const int I = CI0 + cell_lx(lane, cq), J = CJ0 + cell_ly(lane, rw);
+      cma[cq][rw] = (PASSA && wave == 0 && I < cnx && J < cny) ? cmk[(size_t)I + (size_t)J * cnx] : 0;
     }
+  fill_tile<SXF>(T, fx, ox, oy, fnx, fny, lane);
+  wave_sync();
+  // ---- pass B on this wave's tile (colours 0,1), into the window and (own cells) to memory
+  {
+    Cells4 cc;
+    init_cells(cc, own, T, lane);
+    tile_sweeps<0>(T, cc, lane);
 #pragma unroll
-  for (int q = 0; q < 6; q++) {
-    const int e = tid + 256 * q;
-    if (e < 36 * 36) {
-      const int ly = e / 36, lx = e - ly * 36;
-      const int i = fx0 + lx, j = fy0 + ly;
-      X[e] = (i >= 0 && j >= 0 && i < fnx && j < fny) ? ld<SXF>(fx, (size_t)i + (size_t)j * fnx) : 0.f;
-    }
+    for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+      for (int rw = 0; rw < 2; rw++) {
+        const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
+        const bool in = i >= 0 && j >= 0 && i < fnx && j < fny;
+        const float v = (in && (cc.m[cq][rw] & 32)) ? rnd<SXF>(cc.v[cq][rw]) : 0.f;
+        XB[(i - wx0) + (j - wy0) * XBW] = v;
+        if (in && i >= 2 * CI0 && i < 2 * CI0 + 32 && j >= 2 * CJ0 && j < 2 * CJ0 + 32)
+          st<SXF>(fxb, (size_t)i + (size_t)j * fnx, v);
+      }
   }
   __syncthreads();
+  // ---- fine residual on the 34x34 cells (R index (lx, ly) = fine (2CI0-1+lx, 2CJ0-1+ly))
 #pragma unroll
-  for (int q = 0; q < 5; q++) {
-    const int e = tid + 256 * q;
+  for (int q = 0; q < 3; q++) {
+    const int e = tid + 64 * BRA_WAVES * q;
     if (e < 34 * 34) {
       const int ly = e / 34, lx = e - ly * 34;
       const uint8_t m = mr[q];
       float rr = 0.f;
       if (mask_type(m) == AGUA) {
-        const int ex = (lx + 1) + (ly + 1) * 36;
-        float s = (m & 1) ? X[ex - 1] : 0.f;
-        s += (m & 2) ? X[ex + 1] : 0.f;
-        s += (m & 4) ? X[ex - 36] : 0.f;
-        s += (m & 8) ? X[ex + 36] : 0.f;
-        rr = br[q] - ((float)mask_cnt(m) * X[ex] - s);
+        const int ex = (lx + 7) + (ly + 7) * XBW;
+        float sn = (m & 1) ? XB[ex - 1] : 0.f;
+        sn += (m & 2) ? XB[ex + 1] : 0.f;
+        sn += (m & 4) ? XB[ex - XBW] : 0.f;
+        sn += (m & 8) ? XB[ex + XBW] : 0.f;
+        rr = br[q] - ((float)mask_cnt(m) * XB[ex] - sn);
       }
       R[e] = rr;
     }
   }
   __syncthreads();
-  {
+  // ---- coarse b = P^T r (weights 1/4 3/4 3/4 1/4 per axis)
+  if (tid < 256) {
     float bc = 0.f;
     if (cin) {
       if (mask_type(cmine) == AGUA) {
         const float w[4] = {0.25f, 0.75f, 0.75f, 0.25f};
-        // fine cell 2I-1+bi sits at R index (2I-1+bi) - (fx0+1) = 2(I-CI0) + bi
         const int rx = 2 * (tid & 15), ry = 2 * (tid >> 4);
         float acc = 0.f;
 #pragma unroll
@@ -862,8 +865,8 @@ __global__ __launch_bounds__(256) void k_restrict_a(const SXF* __restrict__ fx, 
   }
   if (!PASSA) return;
   __syncthreads();
-  if (tid >= 64) return;
-  const int lane = tid;
+  if (wave != 0) return;
+  // ---- pass A of the coarse level on this tile, from x = 0 (cicloV, pressao-cpu.js:243)
   zero_tile(T, lane);
   Cells4 cc;
 #pragma unroll
@@ -879,6 +882,158 @@ __global__ __launch_bounds__(256) void k_restrict_a(const SXF* __restrict__ fx, 
   wave_sync();
   tile_sweeps<0>(T, cc, lane);
   store_cells<SCX>(cc, cxo, CI0, CJ0, cnx, cny, lane);
+}
+
+// ================================================================== up: prolongation + pass C + pass D
+// One block owns a 16x16 tile T of level l at offset 0.  Pass D (offset 0, colours 1,0) on T
+// reads pass C's output on T's 18x18 window, which the four offset-8 tiles overlapping T cover:
+// four wavefronts run pass C (colours 1,0) on them from a snapshot with the prolongation folded
+// in (x_B + P e on water cells, prolongar pressao-cpu.js:191-214; coarse values from a 10x10 patch
+// per tile) into a 32x32 LDS window; then wave 0 runs pass D on T.  The result goes to `xout`
+// (a different array from the x_B the neighbouring blocks read), or with ZD (level 0) becomes z
+// and the r.z, r.z_old partials.
+constexpr int CW = 32;  // pass-C window: cells 16a - 8 .. 16a + 23
+
+template <class SX, class SB, class SC, bool ZD>
+__global__ __launch_bounds__(256) void k_pcd(const SX* __restrict__ xb, SX* __restrict__ xout,
+                                             const SB* __restrict__ b, const uint8_t* __restrict__ mask, int nx,
+                                             int ny, const SC* __restrict__ cx, const uint8_t* __restrict__ cmk,
+                                             int cnx, int cny, ZArgs za, const DevScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ float Tt[4][TCELLS];
+  __shared__ float CPt[4][100];
+  __shared__ float W[CW * CW];
+  __shared__ double sh[BSLOTS * 16];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int ax = blockIdx.x * 16, ay = blockIdx.y * 16;     // tile T
+  const int wx0 = ax - 8, wy0 = ay - 8;                      // window origin
+  const int ox = wx0 + 16 * (wave & 1), oy = wy0 + 16 * (wave >> 1);  // this wave's C tile
+  float* T = Tt[wave];
+  float* CP = CPt[wave];
+  const int PX = (ox >> 1) - 1, PY = (oy >> 1) - 1;
+  // ---- every global load first
+  Own4 own, ownD;
+  prefetch_cells<SB>(own, b, mask, ox, oy, nx, ny, lane);
+  if (wave == 0) prefetch_cells<SB>(ownD, b, mask, ax, ay, nx, ny, lane);
+  uint8_t om[2][2] = {{0, 0}, {0, 0}};
+  float rv[2][2] = {{0, 0}, {0, 0}}, zo[2][2] = {{0, 0}, {0, 0}};
+  if (ZD && wave == 0) {
+#pragma unroll
+    for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+      for (int rw = 0; rw < 2; rw++) {
+        const int i = ax + cell_lx(lane, cq), j = ay + cell_ly(lane, rw);
+        const bool in = i < nx && j < ny;
+        const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
+        om[cq][rw] = in ? za.opMask[c] : 0;
+        rv[cq][rw] = za.r[c];
+        zo[cq][rw] = za.zold[c];
+      }
+  }
+  float cv[2];
+  uint8_t cm[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int e = lane + 64 * q;
+    const int bj = e / 10, ai = e - bj * 10;
+    const int I = PX + ai, J = PY + bj;
+    const bool in = e < 100 && I >= 0 && J >= 0 && I < cnx && J < cny;
+    const size_t C = in ? (size_t)I + (size_t)J * cnx : 0;
+    cm[q] = in ? cmk[C] : 0;
+    cv[q] = in ? ld<SC>(cx, C) : 0.f;
+  }
+  float sv[6];
+  uint8_t sm[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const int e = lane + 64 * q;
+    const int ly = e / TS, lx = e - ly * TS;
+    const int i = ox - 1 + lx, j = oy - 1 + ly;
+    const bool in = e < TCELLS && i >= 0 && j >= 0 && i < nx && j < ny;
+    const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
+    sv[q] = in ? ld<SX>(xb, c) : 0.f;
+    sm[q] = in ? mask[c] : 0;
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int e = lane + 64 * q;
+    if (e < 100) CP[e] = mask_type(cm[q]) == AGUA ? cv[q] : 0.f;
+  }
+  wave_sync();
+  // ---- snapshot x_B + P e, pass C on this wave's tile, into the window
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const int e = lane + 64 * q;
+    if (e < TCELLS) {
+      const int ly = e / TS, lx = e - ly * TS;
+      const int i = ox - 1 + lx, j = oy - 1 + ly;
+      float val = sv[q];
+      if (mask_type(sm[q]) == AGUA) {  // in the grid (sm is 0 outside)
+        const int I0 = ((i & 1) == 0 ? (i >> 1) - 1 : (i - 1) >> 1) - PX;
+        const int J0 = ((j & 1) == 0 ? (j >> 1) - 1 : (j - 1) >> 1) - PY;
+        const float wx0 = (i & 1) == 0 ? 0.25f : 0.75f, wy0 = (j & 1) == 0 ? 0.25f : 0.75f;
+        const float wx1 = 1.f - wx0, wy1 = 1.f - wy0;
+        float e2 = 0.f;
+        e2 += CP[I0 + J0 * 10] * wx0 * wy0;
+        e2 += CP[I0 + 1 + J0 * 10] * wx1 * wy0;
+        e2 += CP[I0 + (J0 + 1) * 10] * wx0 * wy1;
+        e2 += CP[I0 + 1 + (J0 + 1) * 10] * wx1 * wy1;
+        val += e2;
+      }
+      T[e] = val;
+    }
+  }
+  wave_sync();
+  {
+    Cells4 cc;
+    init_cells(cc, own, T, lane);
+    tile_sweeps<1>(T, cc, lane);
+#pragma unroll
+    for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+      for (int rw = 0; rw < 2; rw++) {
+        const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
+        const bool in = i >= 0 && j >= 0 && i < nx && j < ny;
+        W[(i - wx0) + (j - wy0) * CW] = (in && (cc.m[cq][rw] & 32)) ? rnd<SX>(cc.v[cq][rw]) : 0.f;
+      }
+  }
+  __syncthreads();
+  double dv[2] = {0, 0};
+  double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (wave == 0) {
+    // ---- pass D on T from the window (T's 18x18 = window cells 7 .. 24)
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const int e = lane + 64 * q;
+      if (e < TCELLS) {
+        const int ly = e / TS, lx = e - ly * TS;
+        T[e] = W[(lx + 7) + (ly + 7) * CW];
+      }
+    }
+    wave_sync();
+    Cells4 cc;
+    init_cells(cc, ownD, T, lane);
+    tile_sweeps<1>(T, cc, lane);
+    if (!ZD) {
+      store_cells<SX>(cc, xout, ax, ay, nx, ny, lane);
+    } else {
+      const float inv = za.scaled ? 1.f / sc->mg_scale : 1.f;
+#pragma unroll
+      for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+        for (int rw = 0; rw < 2; rw++) {
+          const int i = ax + cell_lx(lane, cq), j = ay + cell_ly(lane, rw);
+          if (i < nx && j < ny)
+            z_cell(za, (size_t)i + (size_t)j * nx, om[cq][rw], rv[cq][rw], zo[cq][rw],
+                   (cc.m[cq][rw] & 32) ? cc.v[cq][rw] : 0.f, inv, dv, sl);
+        }
+    }
+  }
+  if (ZD) {
+    const int ns = nslots(sc->active_bubbles);
+    store_partials<2>(dv, 2, za.partials, sh);
+    if (ns > 0) store_partials<BSLOTS>(sl, ns, za.bpart, sh);
+  }
 }
 
 // ================================================================== coarsest-level operator
@@ -980,10 +1135,10 @@ __device__ __forceinline__ void wg_pass(const float* S, float* W, const float* B
 }
 
 // Levels lc..L-1 from LDS.  Level lc arrives with b and (unless it is the coarsest) x after
-// its pass A; the result of level lc's V-cycle is written to out_lc.
-__global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __restrict__ b_lc,
-                                                    const float* __restrict__ x_lc, float* __restrict__ out_lc,
-                                                    const DevScalars* __restrict__ sc) {
+// its pass A; the result of level lc's V-cycle is written to out_lc, which may be x_lc (every
+// input is in LDS before the first output is stored).
+__global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __restrict__ b_lc, const float* x_lc,
+                                                    float* out_lc, const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ __attribute__((aligned(16))) char pool[WG_POOL];
   __shared__ float tiles[WG_WAVES][TCELLS];
@@ -1570,32 +1725,17 @@ static void with_types(const Level& L, Fn fn) {
 }
 #define TQ_TYPES(tx, tb) using SX = typename decltype(tx)::type; using SB = typename decltype(tb)::type
 
-// one smoothing pass of a multi-workgroup level: x -> tmp.  Pass B: offset 8, colours 0,1;
-// pass D: offset 0, colours 1,0 (levels > 0; level 0's pass D writes z).
-template <int C0>
-static void launch_pass(tq_sim* s, int l, int desl, const char* name) {
+// pass B of level l, restriction to l+1 and pass A of l+1 (l+1 not the coarsest):
+// x(l) -> tmp(l) (x_B), b(l+1), x(l+1)
+static void launch_bra(tq_sim* s, int l) {
   Level& F = s->lv[l];
-  const int ntx = tiles_of(F.nx, desl), nty = tiles_of(F.ny, desl);
-  const dim3 g = tile_grid(F.nx, F.ny, desl);
-  int t = time_begin(s, name, (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n);
-  with_types(F, [&](auto tx, auto tb) {
-    TQ_TYPES(tx, tb);
-    hipLaunchKernelGGL((k_pass<SX, SB, C0>), g, dim3(256), 0, s->st, (const SX*)F.x, (SX*)F.tmp, (const SB*)F.b,
-                       F.mask, F.nx, F.ny, desl, ntx, nty, s->sc);
-  });
-  time_end(s, t);
-}
-static void launch_pass_b(tq_sim* s, int l) { launch_pass<0>(s, l, 8, l == 0 ? "mg_pass_b_l0" : "mg_pass_b"); }
-static void launch_pass_d(tq_sim* s, int l) { launch_pass<1>(s, l, 0, "mg_pass_d"); }
-
-// restriction l-1 -> l and pass A of level l
-static void launch_restrict_a(tq_sim* s, int l) {
-  Level& F = s->lv[l - 1];
-  Level& G = s->lv[l];
-  const bool pa = l < (int)s->lv.size() - 1;
+  Level& G = s->lv[l + 1];
+  const bool pa = l + 1 < (int)s->lv.size() - 1;
   const dim3 g((G.nx + 15) / 16, (G.ny + 15) / 16);
-  int t = time_begin(s, l == 1 ? "mg_restrict_a_l1" : "mg_restrict_a",
-                     (vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n + (1 + vbytes(G.hb) + (pa ? vbytes(G.hx) : 0)) * (double)G.n);
+  // x(l) and b(l) and mask(l) in, x_B(l) out; coarse mask in, b (and x) out
+  int t = time_begin(s, l == 0 ? "mg_down_l0" : "mg_down",
+                     (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n +
+                         (1 + vbytes(G.hb) + (pa ? vbytes(G.hx) : 0)) * (double)G.n);
   with_types(F, [&](auto fx, auto fb) {
     using SXF = typename decltype(fx)::type;
     using SBF = typename decltype(fb)::type;
@@ -1603,34 +1743,46 @@ static void launch_restrict_a(tq_sim* s, int l) {
       using SCX = typename decltype(cx)::type;
       using SCB = typename decltype(cb)::type;
       if (pa)
-        hipLaunchKernelGGL((k_restrict_a<SXF, SBF, SCB, SCX, true>), g, dim3(256), 0, s->st, (const SXF*)F.tmp,
-                           (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny, s->sc);
+        hipLaunchKernelGGL((k_bra<SXF, SBF, SCB, SCX, true>), g, dim3(64 * BRA_WAVES), 0, s->st, (const SXF*)F.x,
+                           (SXF*)F.tmp, (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny,
+                           s->sc);
       else
-        hipLaunchKernelGGL((k_restrict_a<SXF, SBF, SCB, SCX, false>), g, dim3(256), 0, s->st, (const SXF*)F.tmp,
-                           (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny, s->sc);
+        hipLaunchKernelGGL((k_bra<SXF, SBF, SCB, SCX, false>), g, dim3(64 * BRA_WAVES), 0, s->st, (const SXF*)F.x,
+                           (SXF*)F.tmp, (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny,
+                           s->sc);
     });
   });
   time_end(s, t);
 }
 
-// prolongation from level l+1 (its final x in tmp) and pass C of level l: tmp -> x
-static void launch_prolong_c(tq_sim* s, int l) {
+static ZArgs zargs(tq_sim* s);
+
+// prolongation from level l+1 (its final x in x(l+1)), passes C and D of level l:
+// tmp(l) (x_B) -> x(l); on level 0 the result becomes z and the dot partials (returns their count)
+static int launch_pcd(tq_sim* s, int l) {
   Level& F = s->lv[l];
   Level& G = s->lv[l + 1];
-  const int ntx = tiles_of(F.nx, 8), nty = tiles_of(F.ny, 8);
-  const dim3 g = tile_grid(F.nx, F.ny, 8);
-  int t = time_begin(s, l == 0 ? "mg_prolong_c_l0" : "mg_prolong_c",
-                     (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n + (vbytes(G.hx) + 1) * (double)G.n);
+  const bool zd = l == 0;
+  const dim3 g((F.nx + 15) / 16, (F.ny + 15) / 16);
+  const ZArgs za = zargs(s);
+  // x_B, b, mask in; x out (or z with r, z_old, opMask in); coarse x and mask in
+  const double fine = zd ? (vbytes(F.hx) + vbytes(F.hb) + 1 + 1 + 4 + 4 + 4) : (2 * vbytes(F.hx) + vbytes(F.hb) + 1);
+  int t = time_begin(s, zd ? "mg_up_z_l0" : "mg_up", fine * F.n + (vbytes(G.hx) + 1) * (double)G.n);
   with_types(F, [&](auto tx, auto tb) {
     TQ_TYPES(tx, tb);
-    if (G.hx)
-      hipLaunchKernelGGL((k_prolong_pass<SX, SB, __half>), g, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
-                         (const SB*)F.b, F.mask, F.nx, F.ny, ntx, nty, (const __half*)G.tmp, G.mask, G.nx, G.ny, s->sc);
-    else
-      hipLaunchKernelGGL((k_prolong_pass<SX, SB, float>), g, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
-                         (const SB*)F.b, F.mask, F.nx, F.ny, ntx, nty, (const float*)G.tmp, G.mask, G.nx, G.ny, s->sc);
+    auto go = [&](auto tc) {
+      using SC = typename decltype(tc)::type;
+      if (zd)
+        hipLaunchKernelGGL((k_pcd<SX, SB, SC, true>), g, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
+                           (const SB*)F.b, F.mask, F.nx, F.ny, (const SC*)G.x, G.mask, G.nx, G.ny, za, s->sc);
+      else
+        hipLaunchKernelGGL((k_pcd<SX, SB, SC, false>), g, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
+                           (const SB*)F.b, F.mask, F.nx, F.ny, (const SC*)G.x, G.mask, G.nx, G.ny, za, s->sc);
+    };
+    if (G.hx) go(Tag<__half>{}); else go(Tag<float>{});
   });
   time_end(s, t);
+  return (int)(g.x * g.y);
 }
 
 static ZArgs zargs(tq_sim* s) {
@@ -1640,24 +1792,6 @@ static ZArgs zargs(tq_sim* s) {
   za.bvec = s->bvec; za.partials = s->partials; za.bpart = s->bpart;
   za.scaled = s->mg_scaled;
   return za;
-}
-
-// pass D of level 0 -> z and the dots; returns the partial count
-static int launch_pass_z(tq_sim* s) {
-  Level& F = s->lv[0];
-  const int ntx = tiles_of(F.nx, 0), nty = tiles_of(F.ny, 0);
-  const dim3 g = tile_grid(F.nx, F.ny, 0);
-  const ZArgs za = zargs(s);
-  // x, b, mask, opMask, r, zold read; z written
-  int t = time_begin(s, "mg_pass_z_l0", (4 + vbytes(F.hb) + 1 + 1 + 4 + 4 + 4) * (double)F.n);
-  if (F.hb)
-    hipLaunchKernelGGL((k_pass_z<__half>), g, dim3(256), 0, s->st, (const float*)F.x, (const __half*)F.b, F.mask, F.nx,
-                       F.ny, ntx, nty, za, s->sc);
-  else
-    hipLaunchKernelGGL((k_pass_z<float>), g, dim3(256), 0, s->st, (const float*)F.x, (const float*)F.b, F.mask, F.nx,
-                       F.ny, ntx, nty, za, s->sc);
-  time_end(s, t);
-  return (int)(g.x * g.y);
 }
 
 static void launch_coarse_wg(tq_sim* s) {
@@ -1683,31 +1817,57 @@ static void launch_coarse_wg(tq_sim* s) {
   if (s->dense) bytes += 4.0 * s->lv[L - 1].n * s->lv[L - 1].n;
   int t = time_begin(s, "mg_coarse_wg", bytes);
   hipLaunchKernelGGL(k_coarse_wg, dim3(1), dim3(1024), 0, s->st, P, (const float*)G.b, (const float*)G.x,
-                     (float*)G.tmp, s->sc);
+                     (float*)G.x, s->sc);
   time_end(s, t);
+}
+
+// level 0 up: prolongation from level 1 (x(1)) + pass C, tmp(0) -> x(0); then pass D on x(0)
+// writing z and the dot partials (returns their count)
+static int launch_up0(tq_sim* s) {
+  Level& F = s->lv[0];
+  Level& G = s->lv[1];
+  const int ntx8 = tiles_of(F.nx, 8), nty8 = tiles_of(F.ny, 8);
+  const dim3 g8 = tile_grid(F.nx, F.ny, 8);
+  int t = time_begin(s, "mg_prolong_c_l0", (8 + vbytes(F.hb) + 1) * (double)F.n + (vbytes(G.hx) + 1) * (double)G.n);
+  with_types(F, [&](auto tx, auto tb) {
+    TQ_TYPES(tx, tb);
+    if (G.hx)
+      hipLaunchKernelGGL((k_prolong_pass<SX, SB, __half>), g8, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
+                         (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8, (const __half*)G.x, G.mask, G.nx, G.ny,
+                         s->sc);
+    else
+      hipLaunchKernelGGL((k_prolong_pass<SX, SB, float>), g8, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
+                         (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8, (const float*)G.x, G.mask, G.nx, G.ny,
+                         s->sc);
+  });
+  time_end(s, t);
+  const int ntx = tiles_of(F.nx, 0), nty = tiles_of(F.ny, 0);
+  const dim3 g = tile_grid(F.nx, F.ny, 0);
+  const ZArgs za = zargs(s);
+  t = time_begin(s, "mg_pass_z_l0", (4 + vbytes(F.hb) + 1 + 1 + 4 + 4 + 4) * (double)F.n);
+  if (F.hb)
+    hipLaunchKernelGGL((k_pass_z<__half>), g, dim3(256), 0, s->st, (const float*)F.x, (const __half*)F.b, F.mask, F.nx,
+                       F.ny, ntx, nty, za, s->sc);
+  else
+    hipLaunchKernelGGL((k_pass_z<float>), g, dim3(256), 0, s->st, (const float*)F.x, (const float*)F.b, F.mask, F.nx,
+                       F.ny, ntx, nty, za, s->sc);
+  time_end(s, t);
+  return (int)(g.x * g.y);
 }
 
 // V-cycle on r (level 0 input already built by in_a0, whose pass A is done) -> z, beta
 static void precondition_rest(tq_sim* s, int first) {
   const int lc = s->lc;
-  if (lc > 0) launch_pass_b(s, 0);
-  for (int l = 1; l <= lc; l++) {
-    launch_restrict_a(s, l);
-    if (l < lc) launch_pass_b(s, l);
-  }
+  for (int l = 0; l < lc; l++) launch_bra(s, l);
   launch_coarse_wg(s);
   int nb;
   if (lc > 0) {
-    for (int l = lc - 1; l >= 1; l--) {
-      launch_prolong_c(s, l);
-      launch_pass_d(s, l);
-    }
-    launch_prolong_c(s, 0);
-    nb = launch_pass_z(s);
+    for (int l = lc - 1; l >= 1; l--) launch_pcd(s, l);
+    nb = launch_up0(s);
   } else {
     nb = blocks_of(s->C);
     int t = time_begin(s, "pcg_zdots", 16.0 * s->C);
-    hipLaunchKernelGGL(k_zdots0, dim3(nb), dim3(256), 0, s->st, (const float*)s->lv[0].tmp, zargs(s), s->C, s->sc);
+    hipLaunchKernelGGL(k_zdots0, dim3(nb), dim3(256), 0, s->st, (const float*)s->lv[0].x, zargs(s), s->C, s->sc);
     time_end(s, t);
   }
   s->zc = 1 - s->zc;
