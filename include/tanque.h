@@ -55,6 +55,10 @@ typedef struct tq_desc {
                              FP16 (level 0 keeps x in FP32 and stores only b in FP16); levels
                              under 4096 cells and those in LDS stay FP32; -1 = default (1) */
   int use_graph;          /* 1: PCG iterations are replayed from a captured HIP graph */
+  int mg_dense;           /* 1 (default): when the first one-workgroup level has at most 2048
+                             cells and is not the coarsest, its V-cycle is built once per solve
+                             as a dense matrix and applied by a many-workgroup product;
+                             0: that V-cycle runs in one workgroup every iteration */
 } tq_desc;
 
 /* Fills `d` with the reference defaults (constantes.js) for an nx x ny grid of cell size h. */

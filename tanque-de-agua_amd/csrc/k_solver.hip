@@ -39,6 +39,7 @@ constexpr int WG_WAVES = 16;       // one-workgroup kernel: 1024 threads
 constexpr int WG_POOL = 139264;    // its LDS pool for the level arrays (x, tmp, b, mask)
 constexpr int WG_AUTO_CELLS = 2048;  // default: levels from the first one this small run in one workgroup
 constexpr int DENSE_MAX = 192;       // coarsest levels up to this many cells use the dense operator
+constexpr int VOP_MAX = 2048;        // largest level whose V-cycle may be applied as a dense matrix
 constexpr int FIN_THREADS = 512;
 
 template <class S> __device__ inline float ld(const S* p, size_t i);
@@ -1089,16 +1090,17 @@ struct WGPlan {
   const float* dense;  // coarsest-level operator G (n x n, row-major) or null
   int nx[12], ny[12];
   int offx[12], offt[12], offb[12], offm[12];  // byte offsets into the LDS pool
+  int pool;                                     // bytes of the pool in use
   const uint8_t* mask[12];
 };
 
 // one smoothing pass of an LDS-resident level: S -> W, every wave owning whole tiles
-template <int C0>
+template <int C0, int NW>
 __device__ __forceinline__ void wg_pass(const float* S, float* W, const float* B, const uint8_t* M, int nx, int ny, int desl,
                                float* T) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ntx = (nx + desl + 15) >> 4, nty = (ny + desl + 15) >> 4;
-  for (int t = wave; t < ntx * nty; t += WG_WAVES) {
+  for (int t = wave; t < ntx * nty; t += NW) {
     const int ty = t / ntx, tx = t - ty * ntx;
     const int ox = tx * 16 - desl, oy = ty * 16 - desl;
 #pragma unroll
@@ -1134,43 +1136,25 @@ __device__ __forceinline__ void wg_pass(const float* S, float* W, const float* B
   __syncthreads();
 }
 
-// Levels lc..L-1 from LDS.  Level lc arrives with b and (unless it is the coarsest) x after
-// its pass A; the result of level lc's V-cycle is written to out_lc, which may be x_lc (every
-// input is in LDS before the first output is stored).
-__global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __restrict__ b_lc, const float* x_lc,
-                                                    float* out_lc, const DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ __attribute__((aligned(16))) char pool[WG_POOL];
-  __shared__ float tiles[WG_WAVES][TCELLS];
-  float* T = tiles[threadIdx.x >> 6];
+// The V-cycle of levels lc..L-1 on LDS-resident arrays, NW waves per workgroup.  On entry every
+// level's mask is in LDS, level lc holds b and x (x after level lc's pass A when `passA_done`,
+// else zero) and the deeper levels hold b = x = 0.  Returns where level lc's result is.
+template <int NW>
+__device__ __forceinline__ const float* wg_vcycle(const WGPlan& P, char* pool, float* T, bool passA_done) {
   // level arrays live at plan offsets; bit l of `sw` says x and tmp of level l are swapped
   unsigned sw = 0;
   auto Xp = [&](int l) { return (float*)(pool + (((sw >> l) & 1) ? P.offt[l] : P.offx[l])); };
   auto Tp = [&](int l) { return (float*)(pool + (((sw >> l) & 1) ? P.offx[l] : P.offt[l])); };
   auto Bp = [&](int l) { return (float*)(pool + P.offb[l]); };
   auto Mp = [&](int l) { return (uint8_t*)(pool + P.offm[l]); };
-  const bool xin = P.lc < P.L - 1;
-  for (int l = P.lc; l < P.L; l++) {
-    const int n = P.nx[l] * P.ny[l];
-    float* X = Xp(l);
-    float* B = Bp(l);
-    uint8_t* M = Mp(l);
-    const uint8_t* gm = P.mask[l];
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-      M[c] = gm[c];
-      X[c] = (l == P.lc && xin) ? x_lc[c] : 0.f;
-      B[c] = l == P.lc ? b_lc[c] : 0.f;
-    }
-  }
-  __syncthreads();
   // ---- down
   for (int l = P.lc; l < P.L - 1; l++) {
     const int nx = P.nx[l], ny = P.ny[l];
-    if (l > P.lc) {  // pass A (level lc's ran before this kernel)
-      wg_pass<0>(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 0, T);
+    if (l > P.lc || !passA_done) {  // pass A
+      wg_pass<0, NW>(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 0, T);
       sw ^= 1u << l;
     }
-    wg_pass<0>(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 8, T);
+    wg_pass<0, NW>(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 8, T);
     sw ^= 1u << l;
     // restriction to l+1: fine residual into the free array, then b = P^T r, x = 0
     const float* X = Xp(l);
@@ -1226,7 +1210,7 @@ __global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __res
     if (P.dense) {
       // x = G b with G the 64-sweep operator of this solve (k_coarsest_op)
       const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-      for (int c = wave; c < n; c += WG_WAVES) {
+      for (int c = wave; c < n; c += NW) {
         float acc = 0.f;
         for (int j = lane; j < n; j += 64) acc += P.dense[(size_t)c * n + j] * b[j];
         acc = wave_sumf(acc);
@@ -1314,15 +1298,112 @@ __global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __res
       X[c] += e;
     }
     __syncthreads();
-    wg_pass<1>(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 8, T);
+    wg_pass<1, NW>(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 8, T);
     sw ^= 1u << l;
-    wg_pass<1>(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 0, T);
+    wg_pass<1, NW>(Xp(l), Tp(l), Bp(l), Mp(l), nx, ny, 0, T);
     sw ^= 1u << l;
   }
+  return Xp(P.lc);
+}
+
+// copies each LDS level's mask in and zeroes its vectors
+__device__ __forceinline__ void wg_load_masks(const WGPlan& P, char* pool) {
+  for (int l = P.lc; l < P.L; l++) {
+    const int n = P.nx[l] * P.ny[l];
+    float* X = (float*)(pool + P.offx[l]);
+    float* B = (float*)(pool + P.offb[l]);
+    uint8_t* M = (uint8_t*)(pool + P.offm[l]);
+    const uint8_t* gm = P.mask[l];
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+      M[c] = gm[c];
+      X[c] = 0.f;
+      B[c] = 0.f;
+    }
+  }
+}
+
+// Levels lc..L-1 from LDS in one workgroup.  Level lc arrives with b and (unless it is the
+// coarsest) x after its pass A; the result of level lc's V-cycle is written to out_lc, which may
+// be x_lc (every input is in LDS before the first output is stored).
+__global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __restrict__ b_lc, const float* x_lc,
+                                                    float* out_lc, const DevScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ __attribute__((aligned(16))) char pool[WG_POOL];
+  __shared__ float tiles[WG_WAVES][TCELLS];
+  wg_load_masks(P, pool);
+  const bool xin = P.lc < P.L - 1;
   const int n = P.nx[P.lc] * P.ny[P.lc];
-  const float* X = Xp(P.lc);
+  float* X0 = (float*)(pool + P.offx[P.lc]);
+  float* B0 = (float*)(pool + P.offb[P.lc]);
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    X0[c] = xin ? x_lc[c] : 0.f;
+    B0[c] = b_lc[c];
+  }
+  __syncthreads();
+  const float* X = wg_vcycle<WG_WAVES>(P, pool, tiles[threadIdx.x >> 6], true);
   for (int c = threadIdx.x; c < n; c += blockDim.x) out_lc[c] = X[c];
 }
+
+// ---------------------------------------------------------- level lc's V-cycle as a dense matrix
+// The V-cycle from x = 0 is a linear map of b, x = V b, with V symmetric (the up passes repeat
+// the down passes in reverse colour order and prolongation is the transpose of restriction).
+// Row j of C is V e_j, one workgroup per row; the product below then reads C by rows,
+// (C b)_i = sum_j (V e_i)_j b_j = (V b)_i.  Rows of cells that are not water are zero: V
+// neither reads b nor writes x there.
+constexpr int VOP_WAVES = 4;
+__global__ __launch_bounds__(64 * VOP_WAVES) void k_vcycle_op(WGPlan P, int ld, float* __restrict__ C) {
+  extern __shared__ __attribute__((aligned(16))) char vpool[];
+  const int lc = P.lc, n = P.nx[lc] * P.ny[lc], j = blockIdx.x;
+  float* row = C + (size_t)j * ld;
+  if (mask_type(P.mask[lc][j]) != AGUA) {
+    for (int c = threadIdx.x; c < ld; c += blockDim.x) row[c] = 0.f;
+    return;
+  }
+  wg_load_masks(P, vpool);
+  __syncthreads();
+  if (threadIdx.x == 0) ((float*)(vpool + P.offb[lc]))[j] = 1.f;
+  __syncthreads();
+  float* T = (float*)(vpool + P.pool) + (threadIdx.x >> 6) * TCELLS;
+  const float* X = wg_vcycle<VOP_WAVES>(P, vpool, T, false);
+  for (int c = threadIdx.x; c < ld; c += blockDim.x) row[c] = c < n ? X[c] : 0.f;
+}
+
+// x = C b on level lc: VOP_ROWS rows per wavefront, each lane reading float4 columns
+constexpr int VOP_ROWS = 2;
+__global__ __launch_bounds__(256) void k_vcycle_apply(const float* __restrict__ C, int ld, int n,
+                                                      const float* __restrict__ b, float* __restrict__ x,
+                                                      const DevScalars* __restrict__ sc) {
+  if (sc->done) return;
+  const int lane = threadIdx.x & 63;
+  const int r0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * VOP_ROWS;
+  if (r0 >= n) return;
+  const int nk = ld >> 8;  // float4 columns per lane (ld is a multiple of 256, at most 2048)
+  float4 cv[VOP_ROWS][8];
+  float4 bv[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    if (k < nk) {
+      const int c = 4 * lane + 256 * k;
+#pragma unroll
+      for (int r = 0; r < VOP_ROWS; r++)
+        cv[r][k] = r0 + r < n ? *(const float4*)(C + (size_t)(r0 + r) * ld + c) : make_float4(0.f, 0.f, 0.f, 0.f);
+      bv[k].x = c < n ? b[c] : 0.f;
+      bv[k].y = c + 1 < n ? b[c + 1] : 0.f;
+      bv[k].z = c + 2 < n ? b[c + 2] : 0.f;
+      bv[k].w = c + 3 < n ? b[c + 3] : 0.f;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < VOP_ROWS; r++) {
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (k < nk) acc += cv[r][k].x * bv[k].x + cv[r][k].y * bv[k].y + cv[r][k].z * bv[k].z + cv[r][k].w * bv[k].w;
+    acc = wave_sumf(acc);
+    if (lane == 0 && r0 + r < n) x[r0 + r] = acc;
+  }
+}
+
 
 // ================================================================== z dots -> beta
 __global__ __launch_bounds__(FIN_THREADS) void k_fin_zdots(double* __restrict__ bvec, const int* __restrict__ activeList,
@@ -1593,6 +1674,9 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_true_res(const uint32_t* __
 // ================================================================== host side
 static long wg_bytes(int n) { return 12L * n + ((n + 15) & ~15); }
 
+// level lc's V-cycle is applied as the dense matrix of this solve
+static bool vop_active(const tq_sim* s) { return s->vop_ld > 0 && s->d.mg_dense; }
+
 int solver_alloc(tq_sim* s) {
   s->lv.clear();
   {
@@ -1637,6 +1721,18 @@ int solver_alloc(tq_sim* s) {
   s->lc = lc;
   s->dense = nullptr;
   if (s->lv.back().n <= DENSE_MAX) TQ_HIP(hipMalloc(&s->dense, sizeof(float) * DENSE_MAX * DENSE_MAX));
+  // level lc's V-cycle as a matrix: lc above the coarsest, at most VOP_MAX cells, the
+  // builder's LDS (levels lc.. plus one tile per wave) under 64 KiB
+  s->vop = nullptr;
+  s->vop_ld = 0;
+  if (lc > 0 && lc < L - 1 && s->lv[lc].n <= VOP_MAX) {
+    long pool = 0;
+    for (int l = lc; l < L; l++) pool += wg_bytes(s->lv[l].n);
+    if (pool + VOP_WAVES * TCELLS * 4 <= 65536) {
+      s->vop_ld = (s->lv[lc].n + 255) & ~255;
+      TQ_HIP(hipMalloc(&s->vop, sizeof(float) * (size_t)s->vop_ld * s->lv[lc].n));
+    }
+  }
   TQ_HIP(hipMalloc(&s->bpart, sizeof(double) * BSLOTS * MAX_RED_BLOCKS));
   TQ_HIP(hipMalloc(&s->labelSlot, sizeof(int) * MAX_BOLHAS));
   return 0;
@@ -1658,6 +1754,9 @@ void solver_free(tq_sim* s) {
   if (s->labelSlot) (void)hipFree(s->labelSlot);
   if (s->dense) (void)hipFree(s->dense);
   s->dense = nullptr;
+  if (s->vop) (void)hipFree(s->vop);
+  s->vop = nullptr;
+  s->vop_ld = 0;
   s->bpart = nullptr;
   s->labelSlot = nullptr;
 }
@@ -1730,7 +1829,7 @@ static void with_types(const Level& L, Fn fn) {
 static void launch_bra(tq_sim* s, int l) {
   Level& F = s->lv[l];
   Level& G = s->lv[l + 1];
-  const bool pa = l + 1 < (int)s->lv.size() - 1;
+  const bool pa = l + 1 < (int)s->lv.size() - 1 && !(l + 1 == s->lc && vop_active(s));
   const dim3 g((G.nx + 15) / 16, (G.ny + 15) / 16);
   // x(l) and b(l) and mask(l) in, x_B(l) out; coarse mask in, b (and x) out
   int t = time_begin(s, l == 0 ? "mg_down_l0" : "mg_down",
@@ -1794,7 +1893,8 @@ static ZArgs zargs(tq_sim* s) {
   return za;
 }
 
-static void launch_coarse_wg(tq_sim* s) {
+// LDS layout of levels lc..L-1: x, tmp, b (FP32) per level, then the masks
+static WGPlan wg_plan(tq_sim* s) {
   WGPlan P{};
   const int L = (int)s->lv.size();
   P.lc = s->lc;
@@ -1809,8 +1909,24 @@ static void launch_coarse_wg(tq_sim* s) {
     P.mask[l] = s->lv[l].mask;
   }
   for (int l = s->lc; l < L; l++) { P.offm[l] = off; off += (s->lv[l].n + 15) & ~15; }
+  P.pool = off;
   P.dense = s->dense;
+  return P;
+}
+
+static void launch_coarse_wg(tq_sim* s) {
+  const int L = (int)s->lv.size();
   Level& G = s->lv[s->lc];
+  if (vop_active(s)) {
+    // x(lc) = C b(lc): the matrix and b in, x out
+    const int waves = (G.n + VOP_ROWS - 1) / VOP_ROWS;
+    int t = time_begin(s, "mg_coarse_mv", 4.0 * s->vop_ld * G.n + 8.0 * G.n);
+    hipLaunchKernelGGL(k_vcycle_apply, dim3((waves + 3) / 4), dim3(256), 0, s->st, (const float*)s->vop, s->vop_ld,
+                       G.n, (const float*)G.b, (float*)G.x, s->sc);
+    time_end(s, t);
+    return;
+  }
+  const WGPlan P = wg_plan(s);
   // b and x of level lc in, its result out, every LDS level's mask, the dense operator
   double bytes = 12.0 * G.n;
   for (int l = s->lc; l < L; l++) bytes += s->lv[l].n;
@@ -1818,6 +1934,16 @@ static void launch_coarse_wg(tq_sim* s) {
   int t = time_begin(s, "mg_coarse_wg", bytes);
   hipLaunchKernelGGL(k_coarse_wg, dim3(1), dim3(1024), 0, s->st, P, (const float*)G.b, (const float*)G.x,
                      (float*)G.x, s->sc);
+  time_end(s, t);
+}
+
+// builds this solve's matrix of level lc's V-cycle (after the level masks and the coarsest operator)
+static void launch_vcycle_op(tq_sim* s) {
+  const WGPlan P = wg_plan(s);
+  const Level& G = s->lv[s->lc];
+  int t = time_begin(s, "mg_vcycle_op", 4.0 * s->vop_ld * G.n);
+  hipLaunchKernelGGL(k_vcycle_op, dim3(G.n), dim3(64 * VOP_WAVES), P.pool + VOP_WAVES * TCELLS * sizeof(float), s->st,
+                     P, s->vop_ld, s->vop);
   time_end(s, t);
 }
 
@@ -1901,7 +2027,8 @@ static void pcg_iteration(tq_sim* s) {
 // Two PCG iterations (the buffer ping-pong period) captured once into a HIP graph and
 // replayed; the capture depends on the level plan, so a plan change recaptures.
 static int graph_pair(tq_sim* s) {
-  const int key = (s->d.precision << 8) | ((s->d.mg_half_level + 1) << 1) | (s->mg_scaled ? 1 : 0);
+  const int key = (s->d.precision << 8) | ((s->d.mg_half_level + 1) << 2) | (vop_active(s) ? 2 : 0) |
+                  (s->mg_scaled ? 1 : 0);
   if (!s->graph_exec || s->graph_key != key) {
     graph_drop(s);
     hipGraph_t g = nullptr;
@@ -1959,6 +2086,7 @@ int pressure_solve(tq_sim* s) {
     hipLaunchKernelGGL(k_coarsest_op, dim3(G.n), dim3(64), 0, s->st, G.mask, G.nx, G.n, s->dense);
   }
   time_end(s, t);
+  if (vop_active(s)) launch_vcycle_op(s);
   TQ_LAUNCH_CHECK();
   const double tol = s->d.tol;
   const int maxit = s->d.iteracoes;

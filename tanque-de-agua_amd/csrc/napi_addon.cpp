@@ -141,6 +141,7 @@ napi_value construct(napi_env env, napi_callback_info info) {
     d.precision = (int)num_or(env, o, "precision", d.precision);
     d.tol = num_or(env, o, "tol", d.tol);
     d.use_graph = num_or(env, o, "useGraph", d.use_graph) != 0;
+    d.mg_dense = num_or(env, o, "mgDense", d.mg_dense) != 0;
     d.max_particles = (int)num_or(env, o, "maxParticulas", d.max_particles);
   }
   Native* n = new Native();
@@ -373,6 +374,7 @@ napi_value m_options(napi_env env, napi_callback_info info) {
   d.precision = (int)num_or(env, o, "precision", d.precision);
   d.tol = num_or(env, o, "tol", d.tol);
   d.use_graph = num_or(env, o, "useGraph", d.use_graph) != 0;
+  d.mg_dense = num_or(env, o, "mgDense", d.mg_dense) != 0;
   throw_tq(env, tq_set_options(n->s, &d), "tq_set_options");
   return undefined(env);
 }

@@ -276,6 +276,7 @@ void tq_desc_default(tq_desc* d, int nx, int ny, double h) {
   d->mg_min_lds_level = 0;
   d->mg_half_level = -1;
   d->use_graph = 0;
+  d->mg_dense = 1;
 }
 
 int tq_create(const tq_desc* d, tq_sim** out) {
@@ -517,7 +518,7 @@ int tq_set_options(tq_sim* s, const tq_desc* d) {
   }
   s->d.gx = d->gx; s->d.gy = d->gy; s->d.iteracoes = d->iteracoes; s->d.tau_densidade = d->tau_densidade;
   s->d.tensao = d->tensao; s->d.ar = d->ar; s->d.precision = d->precision; s->d.tol = d->tol;
-  s->d.mg_half_level = d->mg_half_level; s->d.use_graph = d->use_graph;
+  s->d.mg_half_level = d->mg_half_level; s->d.use_graph = d->use_graph; s->d.mg_dense = d->mg_dense;
   return 0;
 }
 

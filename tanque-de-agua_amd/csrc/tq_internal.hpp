@@ -191,6 +191,8 @@ struct tq_sim {
   int lc = 0;                   // first multigrid level solved inside the one-workgroup kernel
   int mg_scaled = 0;            // this solve stores some multigrid level in FP16 (input scaled)
   float* dense = nullptr;       // coarsest-level operator (when that level is small)
+  float* vop = nullptr;         // level lc's V-cycle as a dense matrix (rows of vop_ld floats)
+  int vop_ld = 0;               // 0: level lc has no dense V-cycle operator
   void* graph_exec = nullptr;   // hipGraphExec_t of two PCG iterations
   int graph_key = -1;
   bool capturing = false;       // a stream capture is recording (no timing events)

@@ -33,7 +33,8 @@ class Desc(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("h", C.c_double), ("gx", C.c_double), ("gy", C.c_double),
                 ("iteracoes", C.c_int), ("tau_densidade", C.c_double), ("tensao", C.c_double), ("ar", C.c_int),
                 ("max_particles", C.c_int), ("precision", C.c_int), ("tol", C.c_double),
-                ("mg_min_lds_level", C.c_int), ("mg_half_level", C.c_int), ("use_graph", C.c_int)]
+                ("mg_min_lds_level", C.c_int), ("mg_half_level", C.c_int), ("use_graph", C.c_int),
+                ("mg_dense", C.c_int)]
 
 
 class Tool(C.Structure):
@@ -159,7 +160,7 @@ class SimulacaoHIP:
 
     def __init__(self, cena, gravidade=(0.0, -9.81), iteracoes=3, tauDensidade=0.1, tensao=0.0728, ar=True,
                  precision=PREC_FP32, tol=0.0, max_particles=0, mg_min_lds_level=0, mg_half_level=-1,
-                 use_graph=False):
+                 use_graph=False, mg_dense=True):
         L = load()
         d = Desc()
         L.tq_desc_default(C.byref(d), cena.nx, cena.ny, cena.h)
@@ -169,6 +170,7 @@ class SimulacaoHIP:
         d.mg_min_lds_level = mg_min_lds_level
         d.mg_half_level = mg_half_level
         d.use_graph = 1 if use_graph else 0
+        d.mg_dense = 1 if mg_dense else 0
         self._desc = d
         h = C.c_void_p()
         _chk(L.tq_create(C.byref(d), C.byref(h)))

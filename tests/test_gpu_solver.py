@@ -132,3 +132,43 @@ def test_graph_replay_matches_eager(tq, oracle, prec, name, build):
     assert np.array_equal(a[0], b[0])
     assert np.array_equal(a[1], b[1])
     assert np.array_equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("name,build,frames", [
+    ("bolha", lambda o: tanque_com_bolha(o), 6),        # 60x60: level 1 (30x30) above a 15x15 and an 8x8 level
+    ("dam160", lambda o: dam(o, nx=160, ny=90, h=0.8 / 90, seed=0), 12),
+    ("dam256", lambda o: dam(o, nx=256, ny=144, h=1.0 / 144, seed=0), 8),
+])
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+def test_dense_vcycle_matches_workgroup_vcycle(oracle, tq, name, build, frames, prec):
+    """The V-cycle of the first one-workgroup level applied as this solve's dense matrix gives the
+    PCG iterates of the same V-cycle run in one workgroup, to FP32 rounding: q after the
+    reference's fixed iteration counts (3 and 10) and the iteration count to 1e-6."""
+    p = {"fp32": tq.PREC_FP32, "mixed": tq.PREC_MIXED}[prec]
+    osim = prepared(oracle, build, frames)
+    dt = substep_dt(oracle, osim)
+    for ph in range(8):
+        osim.fase(ph, dt)
+    bad = []
+    for its, tol in ((3, 0.0), (10, 0.0), (200, TOL)):
+        res = {}
+        for dense in (True, False):
+            g = hip_sim_like(tq, osim, iteracoes=its, tol=tol, precision=p, mg_dense=dense)
+            clone_state(osim, g)
+            g.timing_enable(True)
+            g.fase(8, dt)
+            names = set(g.timing_read())
+            res[dense] = (g.read(12), g.solver_stats()["iterations"], names)
+        (qd, itd, nd), (qw, itw, nw) = res[True], res[False]
+        e = rel_err(qd, qw)
+        log("\n  [%s %s] %3d its tol %g: dense vs workgroup q %.2e, iterations %d / %d" % (
+            name, prec, its, tol, e, itd, itw))
+        if "mg_coarse_mv" not in nd or "mg_vcycle_op" not in nd or "mg_coarse_wg" in nd:
+            bad.append("dense path not taken: %s" % sorted(nd))
+        if "mg_coarse_wg" not in nw or "mg_coarse_mv" in nw:
+            bad.append("workgroup path not taken: %s" % sorted(nw))
+        if e > 1e-4:
+            bad.append("%d its: q differs by %.2e" % (its, e))
+        if abs(itd - itw) > 1:
+            bad.append("iterations %d vs %d" % (itd, itw))
+    assert not bad, bad
