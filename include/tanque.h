@@ -145,6 +145,7 @@ typedef struct tq_solver_stats {
   int active_bubbles;
   long long total_iterations; /* PCG iterations since creation */
   long long solves;           /* pressure solves since creation */
+  long long operator_builds;  /* solves that rebuilt the dense coarse operators (their masks changed) */
 } tq_solver_stats;
 int tq_solver_stats_get(tq_sim* s, tq_solver_stats* st, int compute_true_residual);
 

@@ -201,6 +201,7 @@ __global__ __launch_bounds__(1024) void k_bub_init(const uint32_t* __restrict__ 
     int acc = 0;
     for (int t = 0; t < 1024; t++) { int c = cnt[t]; cnt[t] = acc; acc += c; }
     sc->active_bubbles = acc;
+    sc->op_dirty = 0;
   }
   __syncthreads();
   int o = cnt[tid];
@@ -1038,12 +1039,34 @@ __global__ __launch_bounds__(256) void k_pcd(const SX* __restrict__ xb, SX* __re
 }
 
 // ================================================================== coarsest-level operator
+// The dense operators (coarsest G, level lc's V-cycle matrix) depend only on the masks of levels
+// lc..L-1.  This compares them with the copy taken at the last build (one thread per cell, the
+// levels concatenated), takes the new copy and flags the solve when any cell differs or `force`.
+struct OpMasks {
+  int lc, L;
+  int off[13];  // first index of each level in the concatenation (off[L] = total)
+  const uint8_t* mask[12];
+};
+__global__ __launch_bounds__(256) void k_op_check(OpMasks M, uint8_t* __restrict__ saved, int force,
+                                                  DevScalars* __restrict__ sc) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= M.off[M.L]) return;
+  int l = M.lc;
+  while (c >= M.off[l + 1]) l++;
+  const uint8_t m = M.mask[l][c - M.off[l]];
+  if (force || saved[c] != m) {
+    saved[c] = m;
+    sc->op_dirty = 1;
+  }
+}
+
 // resolverGrosso (pressao-cpu.js:217-238) runs 64 red-black sweeps from x = 0, so its result is
 // x = G b for a matrix G fixed by the level's masks.  Column j is the result for b = e_j: one
 // wavefront per column runs the sweeps on its own copy of the level (LDS), cells in registers.
 // Columns of cells that are not water are zero (their b is never read).
 __global__ __launch_bounds__(64) void k_coarsest_op(const uint8_t* __restrict__ mask, int nx, int n,
-                                                    float* __restrict__ G) {
+                                                    float* __restrict__ G, const DevScalars* __restrict__ sc) {
+  if (!sc->op_dirty) return;  // masks as at the last build: G is current
   __shared__ float x[DENSE_MAX];
   const int j = blockIdx.x, lane = threadIdx.x;
   for (int c = lane; c < n; c += 64) x[c] = 0.f;
@@ -1351,7 +1374,9 @@ __global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __res
 // (C b)_i = sum_j (V e_i)_j b_j = (V b)_i.  Rows of cells that are not water are zero: V
 // neither reads b nor writes x there.
 constexpr int VOP_WAVES = 4;
-__global__ __launch_bounds__(64 * VOP_WAVES) void k_vcycle_op(WGPlan P, int ld, float* __restrict__ C) {
+__global__ __launch_bounds__(64 * VOP_WAVES) void k_vcycle_op(WGPlan P, int ld, float* __restrict__ C,
+                                                             DevScalars* __restrict__ sc) {
+  if (!sc->op_dirty) return;  // masks as at the last build: C is current
   extern __shared__ __attribute__((aligned(16))) char vpool[];
   const int lc = P.lc, n = P.nx[lc] * P.ny[lc], j = blockIdx.x;
   float* row = C + (size_t)j * ld;
@@ -1366,6 +1391,10 @@ __global__ __launch_bounds__(64 * VOP_WAVES) void k_vcycle_op(WGPlan P, int ld, 
   float* T = (float*)(vpool + P.pool) + (threadIdx.x >> 6) * TCELLS;
   const float* X = wg_vcycle<VOP_WAVES>(P, vpool, T, false);
   for (int c = threadIdx.x; c < ld; c += blockDim.x) row[c] = c < n ? X[c] : 0.f;
+}
+
+__global__ __launch_bounds__(64) void k_op_count(DevScalars* __restrict__ sc) {
+  if (threadIdx.x == 0 && sc->op_dirty) sc->op_builds += 1;
 }
 
 // x = C b on level lc: VOP_ROWS rows per wavefront, each lane reading float4 columns
@@ -1721,6 +1750,11 @@ int solver_alloc(tq_sim* s) {
   s->lc = lc;
   s->dense = nullptr;
   if (s->lv.back().n <= DENSE_MAX) TQ_HIP(hipMalloc(&s->dense, sizeof(float) * DENSE_MAX * DENSE_MAX));
+  s->op_cells = 0;
+  for (int l = lc; l < L; l++) s->op_cells += s->lv[l].n;
+  TQ_HIP(hipMalloc(&s->op_masks, s->op_cells));
+  TQ_HIP(hipMemset(s->op_masks, 0xFF, s->op_cells));  // no mask byte is 0xFF: the first solve builds
+  s->vop_was_active = false;
   // level lc's V-cycle as a matrix: lc above the coarsest, at most VOP_MAX cells, the
   // builder's LDS (levels lc.. plus one tile per wave) under 64 KiB
   s->vop = nullptr;
@@ -1757,6 +1791,8 @@ void solver_free(tq_sim* s) {
   if (s->vop) (void)hipFree(s->vop);
   s->vop = nullptr;
   s->vop_ld = 0;
+  if (s->op_masks) (void)hipFree(s->op_masks);
+  s->op_masks = nullptr;
   s->bpart = nullptr;
   s->labelSlot = nullptr;
 }
@@ -1943,7 +1979,7 @@ static void launch_vcycle_op(tq_sim* s) {
   const Level& G = s->lv[s->lc];
   int t = time_begin(s, "mg_vcycle_op", 4.0 * s->vop_ld * G.n);
   hipLaunchKernelGGL(k_vcycle_op, dim3(G.n), dim3(64 * VOP_WAVES), P.pool + VOP_WAVES * TCELLS * sizeof(float), s->st,
-                     P, s->vop_ld, s->vop);
+                     P, s->vop_ld, s->vop, s->sc);
   time_end(s, t);
 }
 
@@ -2081,12 +2117,31 @@ int pressure_solve(tq_sim* s) {
   }
   hipLaunchKernelGGL(k_bub_init, dim3(1), dim3(1024), 0, s->st, s->facesB, s->yB32, s->d.ar, s->bvec, s->activeList,
                      s->labelSlot, s->sc);
+  {
+    // rebuild the dense operators only when their masks changed (or the level-lc matrix was
+    // not kept current while it was switched off)
+    OpMasks M{};
+    M.lc = s->lc;
+    M.L = L;
+    int off = 0;
+    for (int l = s->lc; l < L; l++) {
+      M.off[l] = off;
+      M.mask[l] = s->lv[l].mask;
+      off += s->lv[l].n;
+    }
+    M.off[L] = off;
+    const bool force = vop_active(s) && !s->vop_was_active;
+    hipLaunchKernelGGL(k_op_check, dim3(blocks_of(off)), dim3(256), 0, s->st, M, s->op_masks, force ? 1 : 0, s->sc);
+    s->vop_was_active = vop_active(s);
+  }
   if (s->dense) {
     const Level& G = s->lv[L - 1];
-    hipLaunchKernelGGL(k_coarsest_op, dim3(G.n), dim3(64), 0, s->st, G.mask, G.nx, G.n, s->dense);
+    hipLaunchKernelGGL(k_coarsest_op, dim3(G.n), dim3(64), 0, s->st, G.mask, G.nx, G.n, s->dense,
+                       (const DevScalars*)s->sc);
   }
   time_end(s, t);
   if (vop_active(s)) launch_vcycle_op(s);
+  hipLaunchKernelGGL(k_op_count, dim3(1), dim3(64), 0, s->st, s->sc);
   TQ_LAUNCH_CHECK();
   const double tol = s->d.tol;
   const int maxit = s->d.iteracoes;

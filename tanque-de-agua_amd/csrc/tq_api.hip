@@ -609,6 +609,7 @@ int tq_solver_stats_get(tq_sim* s, tq_solver_stats* st, int compute_true) {
   st->active_bubbles = h.active_bubbles;
   st->total_iterations = (long long)h.total_iters;
   st->solves = (long long)h.solves;
+  st->operator_builds = (long long)h.op_builds;
   st->true_resid = -1;
   if (compute_true) {
     double tr = 0;

@@ -87,7 +87,8 @@ struct DevScalars {
   float mg_scale;       // mixed precision: scale applied to the V-cycle input of this iteration
   float mg_scale_next;  // mg_mult/||r|| of the newest residual, the scale of the next iteration
   float mg_mult;        // numerator of the scale: FP16 values of the input sit this far above 1/||r||
-  float pad_;
+  int op_dirty;         // this solve's one-workgroup level masks differ from the last operator build
+  unsigned long long op_builds;  // operator builds (coarsest and level-lc matrices) since creation
 };
 
 struct Level {
@@ -193,6 +194,9 @@ struct tq_sim {
   float* dense = nullptr;       // coarsest-level operator (when that level is small)
   float* vop = nullptr;         // level lc's V-cycle as a dense matrix (rows of vop_ld floats)
   int vop_ld = 0;               // 0: level lc has no dense V-cycle operator
+  uint8_t* op_masks = nullptr;  // masks of levels lc..L-1 at the last operator build
+  int op_cells = 0;             // their total cell count
+  bool vop_was_active = false;  // the last solve applied the level-lc matrix
   void* graph_exec = nullptr;   // hipGraphExec_t of two PCG iterations
   int graph_key = -1;
   bool capturing = false;       // a stream capture is recording (no timing events)

@@ -172,3 +172,35 @@ def test_dense_vcycle_matches_workgroup_vcycle(oracle, tq, name, build, frames, 
         if abs(itd - itw) > 1:
             bad.append("iterations %d vs %d" % (itd, itw))
     assert not bad, bad
+
+
+def test_dense_operators_rebuilt_when_masks_change(tq, oracle):
+    """The dense coarse operators are rebuilt only when the one-workgroup levels' masks change.
+    Over a dam break (fixed 3 iterations, the reference's mode) a run that reuses them tracks a
+    run with the workgroup V-cycle to FP32 rounding, and so does a run that switches the matrix
+    off and on again (the switch forces a rebuild).  A stale operator would change the
+    preconditioner, which with 3 iterations moves q far beyond rounding."""
+    osim = dam(oracle, nx=160, ny=90, h=0.8 / 90, seed=0)
+    P0 = osim.particulas().copy()
+    sims = {}
+    for name, dense in (("reuse", True), ("workgroup", False), ("toggled", True)):
+        g = hip_sim_like(tq, osim, iteracoes=3, mg_dense=dense)
+        g.write_particles(P0)
+        sims[name] = g
+    worst = 0.0
+    for f in range(8):
+        if f == 3:
+            sims["toggled"].set_options(mg_dense=False)
+        if f == 5:
+            sims["toggled"].set_options(mg_dense=True)
+        for g in sims.values():
+            g.avancar(1 / 60, 3)
+        qw = sims["workgroup"].read(12)
+        for name in ("reuse", "toggled"):
+            e = rel_err(sims[name].read(12), qw)
+            worst = max(worst, e)
+            log("\n  frame %d %-7s vs workgroup: q %.2e" % (f, name, e))
+    st = sims["reuse"].solver_stats()
+    log("  operator builds %d over %d solves" % (st["operator_builds"], st["solves"]))
+    assert 1 <= st["operator_builds"] <= st["solves"]
+    assert worst < 1e-3, worst
