@@ -2,7 +2,10 @@
 //
 //   1. union-find over air cells (4-neighbourhood) with lock-free CAS linking of the larger
 //      root under the smaller one, so every component's root is its smallest cell index --
-//      the root the reference's sequential union-find ends with (ar.js:155-158);
+//      the root the reference's sequential union-find ends with (ar.js:155-158).  Each 32x32
+//      tile is first united in LDS (its components' roots are their smallest cells, which are
+//      also the smallest global indices); only the tile-border cells then unite through the
+//      global array;
 //   2. components are numbered by their root's raster position (an exclusive scan of root
 //      flags): the reference numbers them by first raster appearance, which is the root;
 //   3. per-component counts (cells, atmosphere cells, top-row contact) and the
@@ -43,22 +46,101 @@ __device__ inline void uf_unite(int32_t* pai, int a, int b) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_uf_init(const uint8_t* __restrict__ tipo, int C, int32_t* __restrict__ pai,
-                                                 uint32_t* __restrict__ cells, uint32_t* __restrict__ atm,
+__global__ __launch_bounds__(256) void k_uf_init(int C, uint32_t* __restrict__ cells, uint32_t* __restrict__ atm,
                                                  uint32_t* __restrict__ topo, int32_t* __restrict__ dono,
                                                  uint32_t* __restrict__ label, float* __restrict__ alvo) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
-  pai[c] = tipo[c] == AR ? c : -1;
   cells[c] = 0; atm[c] = 0; topo[c] = 0; dono[c] = -1; label[c] = 0; alvo[c] = 0.f;
 }
 
-__global__ __launch_bounds__(256) void k_uf_union(const uint8_t* __restrict__ tipo, int nx, int ny, int32_t* __restrict__ pai) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nx * ny || tipo[c] != AR) return;
-  const int i = c % nx, j = c / nx;
-  if (i > 0 && tipo[c - 1] == AR) uf_unite(pai, c, c - 1);
-  if (j > 0 && tipo[c - nx] == AR) uf_unite(pai, c, c - nx);
+// ---- 1a. union-find inside a 32x32 tile (LDS, workgroup-scope atomics)
+constexpr int UF_T = 32;
+__device__ inline int lds_find(int* lp, int x) {
+  while (true) {
+    const int p = __hip_atomic_load(&lp[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (p == x) return x;
+    const int gp = __hip_atomic_load(&lp[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (gp != p) __hip_atomic_store(&lp[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    x = p;
+  }
+}
+
+__device__ inline void lds_unite(int* lp, int a, int b) {
+  while (true) {
+    a = lds_find(lp, a);
+    b = lds_find(lp, b);
+    if (a == b) return;
+    if (a < b) { int t = a; a = b; b = t; }
+    int expected = a;
+    if (__hip_atomic_compare_exchange_strong(&lp[a], &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP))
+      return;
+    a = expected;
+  }
+}
+
+// Every cell of the tile gets pai = the global index of its tile-component's root (-1: not air).
+// Local indices are row-major in the tile, so the smallest local index is the smallest global one.
+__global__ __launch_bounds__(256) void k_uf_local(const uint8_t* __restrict__ tipo, int nx, int ny,
+                                                  int32_t* __restrict__ pai) {
+  __shared__ int lp[UF_T * UF_T];
+  const int ox = blockIdx.x * UF_T, oy = blockIdx.y * UF_T;
+  bool air[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int e = threadIdx.x + 256 * k;
+    const int i = ox + (e & (UF_T - 1)), j = oy + e / UF_T;
+    air[k] = i < nx && j < ny && tipo[(size_t)i + (size_t)j * nx] == AR;
+    lp[e] = air[k] ? e : -1;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int e = threadIdx.x + 256 * k;
+    if (!air[k]) continue;
+    const int lx = e & (UF_T - 1), ly = e / UF_T;
+    if (lx > 0 && __hip_atomic_load(&lp[e - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= 0)
+      lds_unite(lp, e, e - 1);
+    if (ly > 0 && __hip_atomic_load(&lp[e - UF_T], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= 0)
+      lds_unite(lp, e, e - UF_T);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int e = threadIdx.x + 256 * k;
+    const int i = ox + (e & (UF_T - 1)), j = oy + e / UF_T;
+    if (i >= nx || j >= ny) continue;
+    int g = -1;
+    if (air[k]) {
+      const int r = lds_find(lp, e);
+      g = (oy + r / UF_T) * nx + ox + (r & (UF_T - 1));
+    }
+    pai[(size_t)i + (size_t)j * nx] = g;
+  }
+}
+
+// ---- 1b. tile-border cells unite with their neighbour across the border (global array):
+// first the left borders (i = 32a, a > 0), then the top borders (j = 32b, b > 0)
+__global__ __launch_bounds__(256) void k_uf_border(const uint8_t* __restrict__ tipo, int nx, int ny,
+                                                   int32_t* __restrict__ pai) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ntx = (nx + UF_T - 1) / UF_T, nty = (ny + UF_T - 1) / UF_T;
+  const int nv = (ntx - 1) * ny;
+  int c, d;
+  if (t < nv) {
+    const int a = 1 + t / ny, j = t - (t / ny) * ny;
+    c = j * nx + a * UF_T;
+    d = c - 1;
+  } else if (t < nv + (nty - 1) * nx) {
+    const int u = t - nv;
+    const int b = 1 + u / nx, i = u - (u / nx) * nx;
+    c = b * UF_T * nx + i;
+    d = c - nx;
+  } else {
+    return;
+  }
+  if (tipo[c] == AR && tipo[d] == AR) uf_unite(pai, c, d);
 }
 
 __global__ __launch_bounds__(256) void k_uf_flatten(const uint8_t* __restrict__ tipo, int C, int32_t* __restrict__ pai,
@@ -292,9 +374,13 @@ int tracker_update(tq_sim* s) {
   const int C = s->C;
   const dim3 g((C + 255) / 256), b(256);
   int t = time_begin(s, "air_tracker", 16.0 * C);
-  hipLaunchKernelGGL(k_uf_init, g, b, 0, s->st, s->tipo, C, s->pai, s->compCells, s->compAtm, s->compTopo, s->compDono,
-                     s->compLabel, s->compAlvo);
-  hipLaunchKernelGGL(k_uf_union, g, b, 0, s->st, s->tipo, s->nx, s->ny, s->pai);
+  hipLaunchKernelGGL(k_uf_init, g, b, 0, s->st, C, s->compCells, s->compAtm, s->compTopo, s->compDono, s->compLabel,
+                     s->compAlvo);
+  const int ntx = (s->nx + UF_T - 1) / UF_T, nty = (s->ny + UF_T - 1) / UF_T;
+  hipLaunchKernelGGL(k_uf_local, dim3(ntx, nty), b, 0, s->st, s->tipo, s->nx, s->ny, s->pai);
+  const int nborder = (ntx - 1) * s->ny + (nty - 1) * s->nx;
+  if (nborder > 0)
+    hipLaunchKernelGGL(k_uf_border, dim3((nborder + 255) / 256), b, 0, s->st, s->tipo, s->nx, s->ny, s->pai);
   hipLaunchKernelGGL(k_uf_flatten, g, b, 0, s->st, s->tipo, C, s->pai, s->rootFlag);
   TQ_LAUNCH_CHECK();
   if (int e = scan_u32(s, s->rootFlag, s->compIdx, C)) return e;

@@ -12,20 +12,35 @@ namespace tq {
 // edge) and discarded.
 constexpr int SF_T = 32, SF_H = 13, SF_R = SF_T + 2 * SF_H;  // 58
 constexpr int SF_N = SF_R * SF_R;                            // 3364
+constexpr int SF_THREADS = 1024;
+constexpr int SF_K = (SF_N + SF_THREADS - 1) / SF_THREADS;   // region cells per thread (4)
+static_assert(SF_T * SF_T == SF_THREADS, "one owned cell per thread");
 
-__global__ __launch_bounds__(256) void k_surface(const uint8_t* __restrict__ tipo, const float* __restrict__ dens, int nx,
-                                                 int ny, double h, float* __restrict__ fase, float* __restrict__ nXo,
-                                                 float* __restrict__ nYo, float* __restrict__ kap) {
+__global__ __launch_bounds__(SF_THREADS) void k_surface(const uint8_t* __restrict__ tipo, const float* __restrict__ dens,
+                                                        int nx, int ny, double h, float* __restrict__ fase,
+                                                        float* __restrict__ nXo, float* __restrict__ nYo,
+                                                        float* __restrict__ kap) {
 #pragma clang fp contract(off)
   __shared__ float f[SF_N], t[SF_N], ga[SF_N], gb[SF_N];
   __shared__ uint8_t ty[SF_N], fl[SF_N];
   const int tid = threadIdx.x;
   const int I0 = blockIdx.x * SF_T - SF_H, J0 = blockIdx.y * SF_T - SF_H;
-  for (int e = tid; e < SF_N; e += 256) {
-    int gi = I0 + e % SF_R, gj = J0 + e / SF_R;
+  // this thread's region cells, fixed for every pass
+  int lxk[SF_K], lyk[SF_K];
+#pragma unroll
+  for (int k = 0; k < SF_K; k++) {
+    const int e = tid + SF_THREADS * k;
+    lyk[k] = e / SF_R;
+    lxk[k] = e - lyk[k] * SF_R;
+  }
+#pragma unroll
+  for (int k = 0; k < SF_K; k++) {
+    const int e = tid + SF_THREADS * k;
+    if (e >= SF_N) continue;
+    const int gi = I0 + lxk[k], gj = J0 + lyk[k];
     if (gi >= 0 && gj >= 0 && gi < nx && gj < ny) {
-      size_t c = (size_t)gi + (size_t)gj * nx;
-      uint8_t tt = tipo[c];
+      const size_t c = (size_t)gi + (size_t)gj * nx;
+      const uint8_t tt = tipo[c];
       ty[e] = tt;
       f[e] = tt == SOLIDO ? 0.f : (float)fmin(1.0, (double)dens[c] / PPC);
     } else {
@@ -36,47 +51,65 @@ __global__ __launch_bounds__(256) void k_surface(const uint8_t* __restrict__ tip
   __syncthreads();
   for (int passada = 0; passada <= 2; passada++) {
     // #estenderNaParede: 3 layers of "unfilled cell := mean of filled 4-neighbours"
-    for (int e = tid; e < SF_N; e += 256) fl[e] = (ty[e] != SOLIDO && ty[e] != 255) ? 1 : 0;
+#pragma unroll
+    for (int k = 0; k < SF_K; k++) {
+      const int e = tid + SF_THREADS * k;
+      if (e < SF_N) fl[e] = (ty[e] != SOLIDO && ty[e] != 255) ? 1 : 0;
+    }
     __syncthreads();
     for (int camada = 0; camada < 3; camada++) {
-      for (int e = tid; e < SF_N; e += 256) {
-        if (fl[e] || ty[e] == 255) continue;
-        int lx = e % SF_R, ly = e / SF_R, gi = I0 + lx, gj = J0 + ly;
-        double s = 0;
-        int k = 0;
-        if (gi > 0 && lx > 0 && fl[e - 1] == 1) { s += (double)f[e - 1]; k++; }
-        if (gi < nx - 1 && lx < SF_R - 1 && fl[e + 1] == 1) { s += (double)f[e + 1]; k++; }
-        if (gj > 0 && ly > 0 && fl[e - SF_R] == 1) { s += (double)f[e - SF_R]; k++; }
-        if (gj < ny - 1 && ly < SF_R - 1 && fl[e + SF_R] == 1) { s += (double)f[e + SF_R]; k++; }
-        if (k) { t[e] = (float)(s / k); fl[e] = 2; }
+#pragma unroll
+      for (int k = 0; k < SF_K; k++) {
+        const int e = tid + SF_THREADS * k;
+        if (e >= SF_N || fl[e] || ty[e] == 255) continue;
+        const int lx = lxk[k], ly = lyk[k], gi = I0 + lx, gj = J0 + ly;
+        double sum = 0;
+        int n = 0;
+        if (gi > 0 && lx > 0 && fl[e - 1] == 1) { sum += (double)f[e - 1]; n++; }
+        if (gi < nx - 1 && lx < SF_R - 1 && fl[e + 1] == 1) { sum += (double)f[e + 1]; n++; }
+        if (gj > 0 && ly > 0 && fl[e - SF_R] == 1) { sum += (double)f[e - SF_R]; n++; }
+        if (gj < ny - 1 && ly < SF_R - 1 && fl[e + SF_R] == 1) { sum += (double)f[e + SF_R]; n++; }
+        if (n) { t[e] = (float)(sum / n); fl[e] = 2; }
       }
       __syncthreads();
-      for (int e = tid; e < SF_N; e += 256)
-        if (fl[e] == 2) { f[e] = t[e]; fl[e] = 1; }
+#pragma unroll
+      for (int k = 0; k < SF_K; k++) {
+        const int e = tid + SF_THREADS * k;
+        if (e < SF_N && fl[e] == 2) { f[e] = t[e]; fl[e] = 1; }
+      }
       __syncthreads();
     }
     if (passada == 2) break;
     // separable (1 2 1) blur
-    for (int e = tid; e < SF_N; e += 256) {
-      int lx = e % SF_R, gi = I0 + lx;
-      int em = (gi > 0 && lx > 0) ? e - 1 : e, ep = (gi < nx - 1 && lx < SF_R - 1) ? e + 1 : e;
+#pragma unroll
+    for (int k = 0; k < SF_K; k++) {
+      const int e = tid + SF_THREADS * k;
+      if (e >= SF_N) continue;
+      const int lx = lxk[k], gi = I0 + lx;
+      const int em = (gi > 0 && lx > 0) ? e - 1 : e, ep = (gi < nx - 1 && lx < SF_R - 1) ? e + 1 : e;
       t[e] = (float)(((double)f[em] + 2 * (double)f[e] + (double)f[ep]) * 0.25);
     }
     __syncthreads();
-    for (int e = tid; e < SF_N; e += 256) {
-      int ly = e / SF_R, gj = J0 + ly;
-      int em = (gj > 0 && ly > 0) ? e - SF_R : e, ep = (gj < ny - 1 && ly < SF_R - 1) ? e + SF_R : e;
+#pragma unroll
+    for (int k = 0; k < SF_K; k++) {
+      const int e = tid + SF_THREADS * k;
+      if (e >= SF_N) continue;
+      const int ly = lyk[k], gj = J0 + ly;
+      const int em = (gj > 0 && ly > 0) ? e - SF_R : e, ep = (gj < ny - 1 && ly < SF_R - 1) ? e + SF_R : e;
       f[e] = (float)(((double)t[em] + 2 * (double)t[e] + (double)t[ep]) * 0.25);
     }
     __syncthreads();
   }
   // normals (interior cells of the domain only; the reference never writes the border)
-  for (int e = tid; e < SF_N; e += 256) {
-    int lx = e % SF_R, ly = e / SF_R, gi = I0 + lx, gj = J0 + ly;
+#pragma unroll
+  for (int k = 0; k < SF_K; k++) {
+    const int e = tid + SF_THREADS * k;
+    if (e >= SF_N) continue;
+    const int lx = lxk[k], ly = lyk[k], gi = I0 + lx, gj = J0 + ly;
     float a = 0.f, b = 0.f;
     if (gi >= 1 && gj >= 1 && gi < nx - 1 && gj < ny - 1 && lx > 0 && ly > 0 && lx < SF_R - 1 && ly < SF_R - 1) {
-      double gx = ((double)f[e + 1] - (double)f[e - 1]) * 0.5, gy = ((double)f[e + SF_R] - (double)f[e - SF_R]) * 0.5;
-      double g = js_hypot(gx, gy);
+      const double gx = ((double)f[e + 1] - (double)f[e - 1]) * 0.5, gy = ((double)f[e + SF_R] - (double)f[e - SF_R]) * 0.5;
+      const double g = js_hypot(gx, gy);
       a = g > 1e-3 ? (float)(gx / g) : 0.f;
       b = g > 1e-3 ? (float)(gy / g) : 0.f;
     }
@@ -84,27 +117,30 @@ __global__ __launch_bounds__(256) void k_surface(const uint8_t* __restrict__ tip
     gb[e] = b;
   }
   __syncthreads();
-  for (int e = tid; e < SF_T * SF_T; e += 256) {
-    int lx = SF_H + e % SF_T, ly = SF_H + e / SF_T;
-    int gi = I0 + lx, gj = J0 + ly;
-    if (gi >= nx || gj >= ny) continue;
-    int q = lx + ly * SF_R;
-    size_t c = (size_t)gi + (size_t)gj * nx;
-    fase[c] = f[q];
-    nXo[c] = ga[q];
-    nYo[c] = gb[q];
-    float kk = 0.f;
-    if (gi >= 1 && gj >= 1 && gi < nx - 1 && gj < ny - 1)
-      kk = (float)(-(((double)ga[q + 1] - (double)ga[q - 1]) + ((double)gb[q + SF_R] - (double)gb[q - SF_R])) / (2 * h));
-    kap[c] = kk;
+  // the owned 32x32 cells: one per thread
+  {
+    const int lx = SF_H + (tid & (SF_T - 1)), ly = SF_H + tid / SF_T;
+    const int gi = I0 + lx, gj = J0 + ly;
+    if (gi < nx && gj < ny) {
+      const int q = lx + ly * SF_R;
+      const size_t c = (size_t)gi + (size_t)gj * nx;
+      fase[c] = f[q];
+      nXo[c] = ga[q];
+      nYo[c] = gb[q];
+      float kk = 0.f;
+      if (gi >= 1 && gj >= 1 && gi < nx - 1 && gj < ny - 1)
+        kk = (float)(-(((double)ga[q + 1] - (double)ga[q - 1]) + ((double)gb[q + SF_R] - (double)gb[q - SF_R])) /
+                     (2 * h));
+      kap[c] = kk;
+    }
   }
 }
 
 int surface(tq_sim* s) {
   dim3 g((s->nx + SF_T - 1) / SF_T, (s->ny + SF_T - 1) / SF_T);
   int t = time_begin(s, "surface", (1 + 4 + 4 * 4) * (double)s->C);
-  hipLaunchKernelGGL(k_surface, g, dim3(256), 0, s->st, s->tipo, s->dens, s->nx, s->ny, s->h, s->fase, s->nX, s->nY,
-                     s->kappa);
+  hipLaunchKernelGGL(k_surface, g, dim3(SF_THREADS), 0, s->st, s->tipo, s->dens, s->nx, s->ny, s->h, s->fase, s->nX,
+                     s->nY, s->kappa);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   return 0;
