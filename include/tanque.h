@@ -134,6 +134,12 @@ int tq_set_tracker_first(tq_sim* s, int primeira);
  *  accepted and do nothing after a phase 0 call). */
 int tq_run_phase(tq_sim* s, int phase, double dt);
 
+/* One application of the multigrid preconditioner (cicloV, pressao-cpu.js:241-255) to a
+ * level-0 vector b of nx*ny floats on the current cell types, z = M b (test hook; no
+ * counterpart in the reference's API).  Runs the pressure solve's setup first; b is meant for
+ * water cells (bubble cells take the setup's zero bubble residual). */
+int tq_debug_vcycle(tq_sim* s, const float* b, float* z);
+
 /* Statistics of the last pressure solve. */
 typedef struct tq_solver_stats {
   int iterations;          /* PCG iterations run */

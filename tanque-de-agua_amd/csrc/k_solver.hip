@@ -127,27 +127,79 @@ __device__ inline void sum_partials(const double* __restrict__ part, int nb, int
 }
 
 // ================================================================== setup (types and neighbour masks)
-// level 0: operator mask (true types, bubble bits) and MG mask (air with a bubble label as
-// water, pressao-cpu.js:305-308; colour bit)
-__global__ __launch_bounds__(256) void k_setup0(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
-                                                const uint32_t* __restrict__ facesB, int nx, int ny, int ar,
-                                                uint8_t* __restrict__ opMask, uint8_t* __restrict__ mg0) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nx * ny) return;
-  const int j = c / nx, i = c - j * nx;
-  int m = 0;
-  if (i > 0 && tipo[c - 1] != SOLIDO) m |= 1;
-  if (i < nx - 1 && tipo[c + 1] != SOLIDO) m |= 2;
-  if (j > 0 && tipo[c - nx] != SOLIDO) m |= 4;
-  if (j < ny - 1 && tipo[c + nx] != SOLIDO) m |= 8;
-  const uint8_t t = tipo[c];
-  const uint32_t lb = (ar && t == AR) ? rot[c] : 0;
-  const bool labelled = lb >= 2 && lb < (uint32_t)MAX_BOLHAS;
-  const bool bub = labelled && facesB[lb] > 0;
-  const bool inner = i >= 1 && j >= 1 && i < nx - 1 && j < ny - 1;
-  const uint8_t tm = labelled ? AGUA : t;
-  opMask[c] = (uint8_t)(m | (t << 4) | (bub ? MK_BUB : 0) | (bub && inner ? MK_BUBIN : 0));
-  mg0[c] = (uint8_t)(m | (tm << 4) | (((i + j + 1) & 1) ? MK_COLOUR : 0));
+// Level 0: operator mask (true types, bubble bits) and MG mask (air with a bubble label as water,
+// pressao-cpu.js:305-308; colour bit).  Levels 1..PYR_LEVELS: their type bits (tiposGrossos,
+// pressao-cpu.js:63-81: air if any child is air, else water if any is water) as a pyramid in LDS.
+// A level-l cell covers level-0 cells [2^l I, 2^l (I+1)) on each axis (clipped to the grid), so
+// a 64x64 level-0 tile holds whole cells of levels 1..6.  Cells outside a level are 0xFF.
+constexpr int PYR_T = 64;
+constexpr int PYR_LEVELS = 6;
+struct PyrPlan {
+  int L;
+  int nx[12], ny[12];
+  uint8_t* mask[12];
+};
+
+__global__ __launch_bounds__(1024) void k_setup_pyr(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
+                                                    const uint32_t* __restrict__ facesB, int ar,
+                                                    uint8_t* __restrict__ opMask, PyrPlan P) {
+  __shared__ uint8_t ta[PYR_T * PYR_T], tb[PYR_T * PYR_T / 4];
+  const int tid = threadIdx.x;
+  const int nx = P.nx[0], ny = P.ny[0];
+  const int ox = blockIdx.x * PYR_T, oy = blockIdx.y * PYR_T;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int e = tid + 1024 * k;
+    const int i = ox + (e & (PYR_T - 1)), j = oy + e / PYR_T;
+    uint8_t tmo = 0xFF;
+    if (i < nx && j < ny) {
+      const size_t c = (size_t)i + (size_t)j * nx;
+      int m = 0;
+      if (i > 0 && tipo[c - 1] != SOLIDO) m |= 1;
+      if (i < nx - 1 && tipo[c + 1] != SOLIDO) m |= 2;
+      if (j > 0 && tipo[c - nx] != SOLIDO) m |= 4;
+      if (j < ny - 1 && tipo[c + nx] != SOLIDO) m |= 8;
+      const uint8_t t = tipo[c];
+      const uint32_t lb = (ar && t == AR) ? rot[c] : 0;
+      const bool labelled = lb >= 2 && lb < (uint32_t)MAX_BOLHAS;
+      const bool bub = labelled && facesB[lb] > 0;
+      const bool inner = i >= 1 && j >= 1 && i < nx - 1 && j < ny - 1;
+      const uint8_t tm = labelled ? AGUA : t;
+      opMask[c] = (uint8_t)(m | (t << 4) | (bub ? MK_BUB : 0) | (bub && inner ? MK_BUBIN : 0));
+      P.mask[0][c] = (uint8_t)(m | (tm << 4) | (((i + j + 1) & 1) ? MK_COLOUR : 0));
+      tmo = tm;
+    }
+    ta[e] = tmo;
+  }
+  __syncthreads();
+  uint8_t* src = ta;
+  uint8_t* dst = tb;
+  for (int l = 1; l <= PYR_LEVELS && l < P.L; l++) {
+    const int edge = PYR_T >> l;
+    const int gi0 = ox >> l, gj0 = oy >> l;
+    for (int e = tid; e < edge * edge; e += 1024) {
+      const int I = e % edge, J = e / edge;
+      uint8_t t = 0xFF;
+      if (gi0 + I < P.nx[l] && gj0 + J < P.ny[l]) {
+        bool temAr = false, temAgua = false;
+#pragma unroll
+        for (int bb = 0; bb < 2; bb++)
+#pragma unroll
+          for (int aa = 0; aa < 2; aa++) {
+            const uint8_t ct = src[(2 * I + aa) + (2 * J + bb) * (2 * edge)];
+            if (ct == AR) temAr = true;
+            else if (ct == AGUA) temAgua = true;
+          }
+        t = temAr ? AR : temAgua ? AGUA : SOLIDO;
+        P.mask[l][(size_t)(gi0 + I) + (size_t)(gj0 + J) * P.nx[l]] = (uint8_t)(t << 4);
+      }
+      dst[e] = t;
+    }
+    __syncthreads();
+    uint8_t* tmp = src;
+    src = dst;
+    dst = tmp;
+  }
 }
 
 // tiposGrossos (pressao-cpu.js:63-81): air if any child is air, else water if any is water
@@ -167,17 +219,43 @@ __global__ __launch_bounds__(256) void k_coarse_type(const uint8_t* __restrict__
   cm[C] = (uint8_t)((temAr ? AR : temAgua ? AGUA : SOLIDO) << 4);
 }
 
-// prepararVizinhos (pressao-cpu.js:44-60) on the type bits of a level, plus the colour bit
-__global__ __launch_bounds__(256) void k_viz(uint8_t* __restrict__ m, int nx, int ny) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nx * ny) return;
-  const int j = c / nx, i = c - j * nx;
-  int v = 0;
-  if (i > 0 && mask_type(m[c - 1]) != SOLIDO) v |= 1;
-  if (i < nx - 1 && mask_type(m[c + 1]) != SOLIDO) v |= 2;
-  if (j > 0 && mask_type(m[c - nx]) != SOLIDO) v |= 4;
-  if (j < ny - 1 && mask_type(m[c + nx]) != SOLIDO) v |= 8;
-  m[c] = (uint8_t)((m[c] & 0x30) | v | (((i + j + 1) & 1) ? MK_COLOUR : 0));
+// prepararVizinhos (pressao-cpu.js:44-60) on the type bits of levels v0..L-1 (level 0's are set by
+// k_setup_pyr), plus the colour bit; one thread per cell of the levels concatenated.  The dense
+// operators (coarsest G, level lc's V-cycle matrix) depend only on the masks of levels lc..L-1:
+// those are compared with the copy taken at the last build, which takes the new masks, and the
+// solve is flagged when any cell differs or `force`.
+struct VizPlan {
+  int v0, L, lc;
+  int off[13];  // first index of each level in the concatenation (off[L] = total)
+  int nx[12], ny[12];
+  uint8_t* mask[12];
+};
+__global__ __launch_bounds__(256) void k_viz_all(VizPlan P, uint8_t* __restrict__ saved, int force,
+                                                 DevScalars* __restrict__ sc) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= P.off[P.L]) return;
+  int l = P.v0;
+  while (g >= P.off[l + 1]) l++;
+  const int c = g - P.off[l], nx = P.nx[l], ny = P.ny[l];
+  uint8_t* m = P.mask[l];
+  uint8_t mk = m[c];
+  if (l > 0) {
+    const int j = c / nx, i = c - j * nx;
+    int v = 0;
+    if (i > 0 && mask_type(m[c - 1]) != SOLIDO) v |= 1;
+    if (i < nx - 1 && mask_type(m[c + 1]) != SOLIDO) v |= 2;
+    if (j > 0 && mask_type(m[c - nx]) != SOLIDO) v |= 4;
+    if (j < ny - 1 && mask_type(m[c + nx]) != SOLIDO) v |= 8;
+    mk = (uint8_t)((mk & 0x30) | v | (((i + j + 1) & 1) ? MK_COLOUR : 0));
+    m[c] = mk;
+  }
+  if (l >= P.lc) {
+    const int s = g - P.off[P.lc];
+    if (force || saved[s] != mk) {
+      saved[s] = mk;
+      sc->op_dirty = 1;
+    }
+  }
 }
 
 // bubble vectors, the active list (labels with water faces, in label order) and their slots
@@ -1039,27 +1117,6 @@ __global__ __launch_bounds__(256) void k_pcd(const SX* __restrict__ xb, SX* __re
 }
 
 // ================================================================== coarsest-level operator
-// The dense operators (coarsest G, level lc's V-cycle matrix) depend only on the masks of levels
-// lc..L-1.  This compares them with the copy taken at the last build (one thread per cell, the
-// levels concatenated), takes the new copy and flags the solve when any cell differs or `force`.
-struct OpMasks {
-  int lc, L;
-  int off[13];  // first index of each level in the concatenation (off[L] = total)
-  const uint8_t* mask[12];
-};
-__global__ __launch_bounds__(256) void k_op_check(OpMasks M, uint8_t* __restrict__ saved, int force,
-                                                  DevScalars* __restrict__ sc) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= M.off[M.L]) return;
-  int l = M.lc;
-  while (c >= M.off[l + 1]) l++;
-  const uint8_t m = M.mask[l][c - M.off[l]];
-  if (force || saved[c] != m) {
-    saved[c] = m;
-    sc->op_dirty = 1;
-  }
-}
-
 // resolverGrosso (pressao-cpu.js:217-238) runs 64 red-black sweeps from x = 0, so its result is
 // x = G b for a matrix G fixed by the level's masks.  Column j is the result for b = e_j: one
 // wavefront per column runs the sweeps on its own copy of the level (LDS), cells in registers.
@@ -2099,39 +2156,48 @@ static int run_iterations(tq_sim* s, int n) {
   return 0;
 }
 
-int pressure_solve(tq_sim* s) {
+// masks of every level, bubble vectors and slots, the dense operators when their masks changed
+static int solve_setup(tq_sim* s) {
   const int C = s->C;
   const int L = (int)s->lv.size();
-  s->mg_scaled = plan_precision(s) ? 1 : 0;
-  s->pc = 0;
-  s->zc = 0;
-  // ---- setup: masks of every level, bubble vectors and slots
   int t = time_begin(s, "pcg_setup", (1 + 4 + 1 + 1) * (double)C);
-  hipLaunchKernelGGL(k_setup0, dim3(blocks_of(C)), dim3(256), 0, s->st, s->tipo, s->rot, s->facesB, s->nx, s->ny,
-                     s->d.ar, s->opMask, s->lv[0].mask);
-  for (int l = 1; l < L; l++) {
+  {
+    PyrPlan Pp{};
+    Pp.L = L;
+    for (int l = 0; l < L; l++) {
+      Pp.nx[l] = s->lv[l].nx;
+      Pp.ny[l] = s->lv[l].ny;
+      Pp.mask[l] = s->lv[l].mask;
+    }
+    hipLaunchKernelGGL(k_setup_pyr, dim3((s->nx + PYR_T - 1) / PYR_T, (s->ny + PYR_T - 1) / PYR_T), dim3(1024), 0,
+                       s->st, s->tipo, s->rot, s->facesB, s->d.ar, s->opMask, Pp);
+  }
+  for (int l = PYR_LEVELS + 1; l < L; l++) {  // levels past the pyramid (grids over 768 cells a side)
     const Level& F = s->lv[l - 1];
     const Level& G = s->lv[l];
     hipLaunchKernelGGL(k_coarse_type, dim3(blocks_of(G.n)), dim3(256), 0, s->st, F.mask, F.nx, F.ny, G.mask, G.nx, G.ny);
-    hipLaunchKernelGGL(k_viz, dim3(blocks_of(G.n)), dim3(256), 0, s->st, G.mask, G.nx, G.ny);
   }
   hipLaunchKernelGGL(k_bub_init, dim3(1), dim3(1024), 0, s->st, s->facesB, s->yB32, s->d.ar, s->bvec, s->activeList,
                      s->labelSlot, s->sc);
   {
-    // rebuild the dense operators only when their masks changed (or the level-lc matrix was
-    // not kept current while it was switched off)
-    OpMasks M{};
-    M.lc = s->lc;
-    M.L = L;
+    // neighbour bits of levels 1.., and whether the dense operators must be rebuilt (their masks
+    // changed, or the level-lc matrix was not kept current while it was switched off)
+    VizPlan V{};
+    V.v0 = std::min(1, s->lc);
+    V.L = L;
+    V.lc = s->lc;
     int off = 0;
-    for (int l = s->lc; l < L; l++) {
-      M.off[l] = off;
-      M.mask[l] = s->lv[l].mask;
+    for (int l = V.v0; l < L; l++) {
+      V.off[l] = off;
+      V.nx[l] = s->lv[l].nx;
+      V.ny[l] = s->lv[l].ny;
+      V.mask[l] = s->lv[l].mask;
       off += s->lv[l].n;
     }
-    M.off[L] = off;
+    V.off[L] = off;
     const bool force = vop_active(s) && !s->vop_was_active;
-    hipLaunchKernelGGL(k_op_check, dim3(blocks_of(off)), dim3(256), 0, s->st, M, s->op_masks, force ? 1 : 0, s->sc);
+    if (off > 0)
+      hipLaunchKernelGGL(k_viz_all, dim3(blocks_of(off)), dim3(256), 0, s->st, V, s->op_masks, force ? 1 : 0, s->sc);
     s->vop_was_active = vop_active(s);
   }
   if (s->dense) {
@@ -2143,6 +2209,15 @@ int pressure_solve(tq_sim* s) {
   if (vop_active(s)) launch_vcycle_op(s);
   hipLaunchKernelGGL(k_op_count, dim3(1), dim3(64), 0, s->st, s->sc);
   TQ_LAUNCH_CHECK();
+  return 0;
+}
+
+int pressure_solve(tq_sim* s) {
+  const int C = s->C;
+  s->mg_scaled = plan_precision(s) ? 1 : 0;
+  s->pc = 0;
+  s->zc = 0;
+  if (int e = solve_setup(s)) return e;
   const double tol = s->d.tol;
   const int maxit = s->d.iteracoes;
   TQ_HIP(hipMemcpyAsync(&s->sc->tol, &tol, sizeof(double), hipMemcpyHostToDevice, s->st));
@@ -2152,7 +2227,7 @@ int pressure_solve(tq_sim* s) {
   const float mult = s->d.precision == TQ_PREC_MIXED_B ? 4096.f : 1.f;
   TQ_HIP(hipMemcpyAsync(&s->sc->mg_mult, &mult, sizeof(float), hipMemcpyHostToDevice, s->st));
   const int nb = blocks_of(C);
-  t = time_begin(s, "pcg_init", (1 + 1 + 4 + 4 + 4 + 4 + 4) * (double)C);
+  int t = time_begin(s, "pcg_init", (1 + 1 + 4 + 4 + 4 + 4 + 4) * (double)C);
   hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->div, s->q, s->r, s->pbuf[s->pc],
                      s->nx, C, s->bvec, s->labelSlot, s->partials, s->bpart, s->sc);
   hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(FIN_THREADS), 0, s->st, s->facesB, s->bvec, s->rhsB, s->activeList,
@@ -2196,6 +2271,34 @@ int true_residual(tq_sim* s, double* out) {
                      s->labelSlot, s->partials, s->bpart, nb, s->sc, dres);
   TQ_LAUNCH_CHECK();
   TQ_HIP(hipMemcpyAsync(out, dres, sizeof(double), hipMemcpyDeviceToHost, s->st));
+  TQ_HIP(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+// One application of the preconditioner to a level-0 vector (test hook): this solve's setup on
+// the current types, then the V-cycle of r = b from x = 0, z out.  Bubble cells take the
+// bubble residual of the setup (zero), so inputs are meant for water cells.
+int debug_vcycle(tq_sim* s, const float* hb, float* hz) {
+  const int C = s->C;
+  s->mg_scaled = plan_precision(s) ? 1 : 0;
+  s->pc = 0;
+  s->zc = 0;
+  if (int e = solve_setup(s)) return e;
+  double n2 = 0;
+  for (int c = 0; c < C; c++) n2 += (double)hb[c] * hb[c];
+  const float mult = s->d.precision == TQ_PREC_MIXED_B ? 4096.f : 1.f;
+  const float scale = n2 > 0 ? (float)(mult / std::sqrt(n2)) : 1.f;
+  const int zero = 0;
+  const double dzero = 0;
+  TQ_HIP(hipMemcpyAsync(s->r, hb, sizeof(float) * C, hipMemcpyHostToDevice, s->st));
+  TQ_HIP(hipMemcpyAsync(&s->sc->done, &zero, sizeof(int), hipMemcpyHostToDevice, s->st));
+  TQ_HIP(hipMemcpyAsync(&s->sc->proj_mean, &dzero, sizeof(double), hipMemcpyHostToDevice, s->st));
+  TQ_HIP(hipMemcpyAsync(&s->sc->mg_scale, &scale, sizeof(float), hipMemcpyHostToDevice, s->st));
+  TQ_HIP(hipMemcpyAsync(&s->sc->mg_mult, &mult, sizeof(float), hipMemcpyHostToDevice, s->st));
+  launch_in_a0(s, false);
+  precondition_rest(s, 1);
+  TQ_LAUNCH_CHECK();
+  TQ_HIP(hipMemcpyAsync(hz, s->zbuf[s->zc], sizeof(float) * C, hipMemcpyDeviceToHost, s->st));
   TQ_HIP(hipStreamSynchronize(s->st));
   return 0;
 }

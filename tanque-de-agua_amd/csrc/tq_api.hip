@@ -596,6 +596,14 @@ int tq_run_phase(tq_sim* s, int ph, double dt) {
   return sync(s);
 }
 
+int tq_debug_vcycle(tq_sim* s, const float* b, float* z) {
+  TQ_CHECK_SIM(s);
+  if (!b || !z) { set_error("tq_debug_vcycle: null argument"); return TQ_E_ARG; }
+  if (s->d.ar)
+    if (int e = bubble_counts(s)) return e;
+  return debug_vcycle(s, b, z);
+}
+
 int tq_solver_stats_get(tq_sim* s, tq_solver_stats* st, int compute_true) {
   TQ_CHECK_SIM(s);
   if (!st) { set_error("tq_solver_stats_get: null"); return TQ_E_ARG; }

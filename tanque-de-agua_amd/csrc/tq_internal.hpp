@@ -268,5 +268,6 @@ int solver_alloc(tq_sim* s);
 void solver_free(tq_sim* s);
 int pressure_solve(tq_sim* s);
 int true_residual(tq_sim* s, double* out);
+int debug_vcycle(tq_sim* s, const float* b, float* z);
 
 }  // namespace tq

@@ -77,6 +77,7 @@ SIGNATURES = {
     "tq_write_particles": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "tq_set_tracker_first": (C.c_int, [C.c_void_p, C.c_int]),
     "tq_run_phase": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "tq_debug_vcycle": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tq_solver_stats_get": (C.c_int, [C.c_void_p, C.POINTER(SolverStats), C.c_int]),
     "tq_timing_enable": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p]),
     "tq_timing_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int, C.POINTER(C.c_int)]),
@@ -311,6 +312,13 @@ class SimulacaoHIP:
 
     def fase(self, ph, dt=1 / 240):
         _chk(load().tq_run_phase(self._h, ph, dt))
+
+    def vcycle(self, b):
+        """z = M b: one application of the multigrid preconditioner on the current types (test hook)."""
+        b = np.ascontiguousarray(b, np.float32)
+        z = np.empty_like(b)
+        _chk(load().tq_debug_vcycle(self._h, b.ctypes.data, z.ctypes.data))
+        return z
 
     @property
     def tipo(self):
