@@ -4,8 +4,10 @@
 // CG vectors r, z, p, Ap are FP32 per cell; every dot product, the residual norm and the
 // bubble unknowns (y, r, z, p, Ap per label) are FP64.  Reductions are deterministic and free
 // of same-address atomics: each block stores one partial per quantity (and one per slotted
-// bubble), and a one-block "fin" kernel sums them in block order and does the scalar and
-// bubble work.  Bubbles beyond BSLOTS fall back to FP64 atomics.
+// bubble); the next kernel's blocks each re-sum them in block order -- so every block holds
+// bit-identical scalars -- and redo the per-bubble scalar work, block 0 alone storing the
+// bubble vectors.  Bubbles beyond BSLOTS fall back to FP64 atomics.  What one kernel reads and
+// rewrites alternates by iteration parity (bubble z, p, r, the atomic accumulators, rz).
 //
 // Multigrid.  The reference smoother (suavizar, pressao-cpu.js:99-150) runs 16x16 tiles that
 // each read a snapshot of x (their halo stays at the snapshot) through 4 red-black sweeps.  One
@@ -16,21 +18,36 @@
 // consecutive passes are fused and the overlapping tiles recomputed: one kernel per level down
 // (pass B + restriction + the coarse level's pass A) and one per level up (prolongation +
 // pass C + pass D).  Level 0's pass A also applies the PCG update and builds the V-cycle input;
-// its pass D writes z and the r.z dots.  The small levels run in one workgroup from LDS, the
-// coarsest as a dense operator.  Buffer roles per level: pass A writes x, pass B x -> tmp (x_B),
-// the up kernel tmp -> x (the level's result).
+// its pass D writes z and the r.z dots.  The small levels run in one workgroup from LDS, or as
+// a dense matrix built once per solve; the coarsest as a dense operator.  Buffer roles per
+// level: pass A writes x, pass B x -> tmp (x_B), the up kernel tmp -> x (the level's result).
 //
-// Kernel sequence of one PCG iteration:
-//   spmv (p = z + beta p fused, Ap, p.Ap) -> fin_spmv (alpha, bubble y, r)
-//   in_a0 (update, |r|^2, V-cycle input, pass A on level 0) -> fin_update (norm, stop test)
-//   down_l for l < lc, coarse_wg (levels >= lc), up_l for l = lc-1..0 -> fin_zdots (beta)
+// Kernel sequence of PCG iteration k (parity par = k mod 2):
+//   spmv      beta from the z dots, bubble z; p = z + beta p, Ap, p.Ap
+//   in_a0     alpha from p.Ap, bubble y, r; q, r update, |r|^2, V-cycle input, pass A on level 0
+//   down_0    norm and stop test from |r|^2; pass B on level 0 ...
+//   down_l for 0 < l < lc, the levels >= lc, up_l for l = lc-1..0 (level 0 writes z, r.z)
 // Every kernel returns at once when the solve is done.
 #include "tq_device.hpp"
 
 namespace tq {
 
-enum { BV_Y = 0, BV_R, BV_Z, BV_ZV, BV_P, BV_AP, BV_ACC0, BV_ACC1, BV_N };
+// bubble vectors (MAX_BOLHAS doubles each): y, ap, and by iteration parity r, z, p and the
+// atomic accumulators of spmv (A) and of the z dots (Z); I: the accumulator outside the iterations
+enum {
+  BV_Y = 0, BV_AP, BV_R0, BV_R1, BV_Z0, BV_Z1, BV_P0, BV_P1,
+  BV_ACCA0, BV_ACCA1, BV_ACCZ0, BV_ACCZ1, BV_ACCI, BV_N
+};
+static_assert(BV_N == BV_COUNT, "bubble vector count");
 __device__ __host__ inline double* bv(double* base, int k) { return base + (size_t)k * MAX_BOLHAS; }
+
+// partial-sum arrays (slots of MAX_RED_BLOCKS doubles): 0..4 the solve's set-up and the true
+// residual, then the products of one iteration, each read by the following kernel
+constexpr int PS_SPMV = 5;   // p.Ap
+constexpr int PS_RR = 6;     // |r|^2 after the update
+constexpr int PS_Z = 7;      // r.z, r.z_old (two slots)
+constexpr int PS_LAST = RED_SLOTS - 1;  // last element: the true residual's result
+__host__ __device__ inline size_t ps_off(int slot) { return (size_t)slot * MAX_RED_BLOCKS; }
 
 constexpr int BSLOTS = 8;          // bubbles reduced through per-block slots
 constexpr int TS = 18;             // smoother tile with its 1-cell halo
@@ -124,6 +141,173 @@ __device__ inline void sum_partials(const double* __restrict__ part, int nb, int
 #pragma unroll
   for (int k = 0; k < NMAX; k++) tot[k] = k < nv ? res[k] : 0.0;
   __syncthreads();
+}
+
+// block sum valid in every thread (`res` holds NV doubles of LDS)
+template <int NV>
+__device__ inline void block_sum_all(double (&v)[NV], double* sh, double* res) {
+  block_sum<NV>(v, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) res[k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = res[k];
+  __syncthreads();
+}
+
+__device__ inline bool first_block() { return blockIdx.x == 0 && blockIdx.y == 0; }
+
+// ================================================================== reductions folded into consumers
+struct BubArgs {
+  double* bvec;
+  const int* activeList;
+  const int* labelSlot;
+  const uint32_t* facesB;
+};
+
+// The z dots of iteration k's input (pressao-cpu.js:386-394), at the start of spmv: r.z and
+// r.z_old from the partials, the bubbles' new z (slot totals in zs[], else the atomic
+// accumulator) and Polak-Ribiere beta = max(0, (r.z - r.z_old) / rz_prev), 0 on the first
+// iteration.  Block 0 stores z of this parity, clears the accumulator the next z dots use
+// (the other parity) and records rz.
+__device__ inline double fold_zdots(const double* __restrict__ part, const double* __restrict__ bpartZ, int nb,
+                                    int par, int first, const BubArgs& B, double* sh, double* res, double* zs,
+                                    DevScalars* __restrict__ sc) {
+  const int na = sc->active_bubbles;
+  double tot[2];
+  sum_partials<2>(part, nb, 2, tot, sh, res);
+  double stt[BSLOTS];
+  sum_partials<BSLOTS>(bpartZ, nb, nslots(na), stt, sh, res);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < BSLOTS; k++) zs[k] = stt[k];
+  __syncthreads();
+  double* Zn = bv(B.bvec, BV_Z0 + par);
+  const double* Zo = bv(B.bvec, BV_Z0 + 1 - par);
+  const double* Rc = bv(B.bvec, BV_R0 + 1 - par);
+  const double* accZ = bv(B.bvec, BV_ACCZ0 + par);
+  double* accZnext = bv(B.bvec, BV_ACCZ0 + 1 - par);
+  const bool b0 = first_block();
+  double bacc[2] = {0, 0};
+  for (int k = threadIdx.x; k < na; k += blockDim.x) {
+    const int b = B.activeList[k];
+    const int sl = B.labelSlot[b];
+    const double zn = sl >= 0 ? zs[sl] : accZ[b];
+    bacc[0] += Rc[b] * zn;
+    bacc[1] += Rc[b] * Zo[b];
+    if (b0) {
+      Zn[b] = zn;
+      accZnext[b] = 0;
+    }
+  }
+  block_sum_all<2>(bacc, sh, res);
+  const double rzNovo = tot[0] + bacc[0], cruzado = tot[1] + bacc[1];
+  const double rzPrev = sc->rzp[1 - par];
+  const double beta = first ? 0.0 : (fabs(rzPrev) < 1e-30 ? 0.0 : fmax(0.0, (rzNovo - cruzado) / rzPrev));
+  if (b0 && threadIdx.x == 0) {
+    sc->rzp[par] = rzNovo;
+    sc->rz = rzNovo;
+    sc->beta = beta;
+  }
+  return beta;
+}
+
+// alpha = rz / p.Ap (pressao-cpu.js:370-380) at the start of in_a0, with the bubbles' p and Ap
+// (slot totals, else the atomic accumulator) and their new residuals (rbs[] for the slotted
+// ones).  Block 0 stores p, Ap, y += alpha p, r -= alpha Ap of the bubbles, the bubble part of
+// |r|^2 and clears the accumulator of the next spmv (the other parity).
+__device__ inline double fold_spmv(const double* __restrict__ part, const double* __restrict__ bpartA, int nb, int par,
+                                   const BubArgs& B, double* sh, double* res, double* aps, double* rbs,
+                                   DevScalars* __restrict__ sc) {
+  const int na = sc->active_bubbles;
+  double tot[1];
+  sum_partials<1>(part, nb, 1, tot, sh, res);
+  double stt[BSLOTS];
+  sum_partials<BSLOTS>(bpartA, nb, nslots(na), stt, sh, res);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < BSLOTS; k++) aps[k] = stt[k];
+  __syncthreads();
+  const double beta = sc->beta, rz = sc->rzp[par];
+  const double* Zn = bv(B.bvec, BV_Z0 + par);
+  const double* Po = bv(B.bvec, BV_P0 + 1 - par);
+  const double* Ro = bv(B.bvec, BV_R0 + 1 - par);
+  const double* accA = bv(B.bvec, BV_ACCA0 + par);
+  auto pap_of = [&](int b, double* pn_out) -> double {
+    const int sl = B.labelSlot[b];
+    const double pn = Zn[b] + beta * Po[b];
+    *pn_out = pn;
+    return (double)B.facesB[b] * pn + (sl >= 0 ? aps[sl] : accA[b]);
+  };
+  double bacc[1] = {0};
+  for (int k = threadIdx.x; k < na; k += blockDim.x) {
+    double pn;
+    const double apb = pap_of(B.activeList[k], &pn);
+    bacc[0] += pn * apb;
+  }
+  block_sum_all<1>(bacc, sh, res);
+  const double pap = tot[0] + bacc[0];
+  const double alpha = fabs(pap) < 1e-30 ? 0.0 : rz / pap;
+  const bool b0 = first_block();
+  double* Pn = bv(B.bvec, BV_P0 + par);
+  double* Rn = bv(B.bvec, BV_R0 + par);
+  double* Yb = bv(B.bvec, BV_Y);
+  double* APb = bv(B.bvec, BV_AP);
+  double* accAnext = bv(B.bvec, BV_ACCA0 + 1 - par);
+  double rr[1] = {0};
+  for (int k = threadIdx.x; k < na; k += blockDim.x) {
+    const int b = B.activeList[k];
+    double pn;
+    const double apb = pap_of(b, &pn);
+    const double rbn = Ro[b] - alpha * apb;
+    const int sl = B.labelSlot[b];
+    if (sl >= 0) rbs[sl] = rbn;
+    rr[0] += rbn * rbn;
+    if (b0) {
+      Pn[b] = pn;
+      APb[b] = apb;
+      Yb[b] += alpha * pn;
+      Rn[b] = rbn;
+      accAnext[b] = 0;
+    }
+  }
+  block_sum<1>(rr, sh);
+  if (b0 && threadIdx.x == 0) {
+    sc->alpha = alpha;
+    sc->pap = rr[0];  // bubble part of |r|^2
+  }
+  __syncthreads();
+  return alpha;
+}
+
+// the new residual of bubble b inside in_a0 (slotted: rbs[]; else recomputed like fold_spmv)
+__device__ inline double bubble_r(const BubArgs& B, int b, int par, double alpha, double beta, const double* rbs) {
+  const int sl = B.labelSlot[b];
+  if (sl >= 0) return rbs[sl];
+  const double pn = bv(B.bvec, BV_Z0 + par)[b] + beta * bv(B.bvec, BV_P0 + 1 - par)[b];
+  const double apb = (double)B.facesB[b] * pn + bv(B.bvec, BV_ACCA0 + par)[b];
+  return bv(B.bvec, BV_R0 + 1 - par)[b] - alpha * apb;
+}
+
+// the norm and stop test after iteration k's update (pressao-cpu.js:382-386), at the start of
+// level 0's down kernel; block 0 records them, the iteration count and the next V-cycle input
+// scale.  Returns true when the solve stops.
+__device__ inline bool fold_update(const double* __restrict__ part, int nb, int par, double* sh, double* res,
+                                   DevScalars* __restrict__ sc) {
+  double tot[1];
+  sum_partials<1>(part, nb, 1, tot, sh, res);
+  const double n2 = tot[0] + sc->pap;
+  const double nr = sqrt(n2);
+  const int itn = sc->itp[1 - par] + 1;
+  bool stop = itn >= sc->maxit;
+  if (sc->tol > 0 && (nr == 0 || nr <= sc->tol * sc->normb)) stop = true;
+  if (first_block() && threadIdx.x == 0) {
+    sc->norm2 = n2;
+    sc->norm_last = nr;
+    sc->itp[par] = itn;
+    sc->mgs[1 - par] = nr > 0 ? (float)(sc->mg_mult / nr) : 1.f;
+    if (stop) sc->done = 1;
+  }
+  return stop;
 }
 
 // ================================================================== setup (types and neighbour masks)
@@ -306,7 +490,7 @@ __global__ __launch_bounds__(256) void k_pcg_init(const uint8_t* __restrict__ op
   double v[5] = {0, 0, 0, 0, 0};  // r.r, div.div, sum(b), n_water, anchored
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
   const double* yb = bv(bvec, BV_Y);
-  double* acc0 = bv(bvec, BV_ACC0);
+  double* acc0 = bv(bvec, BV_ACCI);
   auto qe = [&](int d, uint8_t md) -> float {
     if (mask_type(md) == AGUA) return q[d];
     if (md & MK_BUB) return (float)yb[rot[d]];
@@ -364,8 +548,8 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_init(const uint32_t* __rest
     for (int k = 0; k < BSLOTS; k++) slotTot[k] = stt[k];
   __syncthreads();
   double bacc[3] = {0, 0, 0};  // rb.rb, rhs.rhs, sum(rhs)
-  double* rb = bv(bvec, BV_R);
-  double* acc0 = bv(bvec, BV_ACC0);
+  double* rb = bv(bvec, BV_R1);  // iteration 0 (parity 0) reads the residual of parity 1
+  double* acc0 = bv(bvec, BV_ACCI);
   const double* yb = bv(bvec, BV_Y);
   for (int k = threadIdx.x; k < na; k += blockDim.x) {
     const int b = activeList[k];
@@ -411,6 +595,10 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_init(const uint32_t* __rest
     sc->done = stop ? 1 : 0;
     sc->mg_scale = sc->norm0 > 0 ? (float)(sc->mg_mult / sc->norm0) : 1.f;
     sc->mg_scale_next = sc->mg_scale;
+    // the first cycle (parity 1) and iteration 0 both scale by 1/||r0||
+    sc->mgs[0] = sc->mgs[1] = sc->mg_scale;
+    sc->rzp[0] = sc->rzp[1] = 0;
+    sc->itp[0] = sc->itp[1] = 0;
   }
   __syncthreads();
   if (mean_s != 0)
@@ -567,21 +755,22 @@ __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMas
                                                const uint32_t* __restrict__ rot, const uint32_t* __restrict__ cellsB,
                                                const float* __restrict__ p, const float* __restrict__ ap,
                                                float* __restrict__ q, float* __restrict__ r, int nx, int ny, int ntx,
-                                               int nty, const double* __restrict__ bvec, SB* __restrict__ b0,
-                                               float* __restrict__ x0, double* __restrict__ partials, int scaled,
-                                               const DevScalars* __restrict__ sc) {
+                                               int nty, BubArgs B, const double* __restrict__ partA,
+                                               const double* __restrict__ bpartA, int nbs, int par,
+                                               SB* __restrict__ b0, float* __restrict__ x0,
+                                               double* __restrict__ partRR, int scaled, const float* __restrict__ mgs,
+                                               DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ float Tt[4][TCELLS];
-  __shared__ double sh[16];
+  __shared__ double sh[BSLOTS * 16], res[BSLOTS], aps[BSLOTS], rbs[BSLOTS];
   const int lane = threadIdx.x & 63;
   int tx, ty;
   block_tile(&tx, &ty);
   const bool valid = tx < ntx && ty < nty;
   const int ox = tx * 16, oy = ty * 16;
-  const double alpha = UPD ? sc->alpha : 0.0;
   const double mean = UPD ? 0.0 : sc->proj_mean;
-  const float scale = scaled ? sc->mg_scale : 1.f;
-  const double* rb = bv((double*)bvec, BV_R);
+  const float scale = scaled ? *mgs : 1.f;
+  const double* rb = bv(B.bvec, BV_R0 + 1 - par);
   double rr[1] = {0};
   // every load of the four cells first (one memory latency), then the arithmetic
   uint8_t om[2][2], gm[2][2];
@@ -605,6 +794,11 @@ __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMas
         av[cq][rw] = ap[c];
       }
     }
+  // UPD: alpha and the bubbles' new residuals from spmv's partials (fold_spmv, while the loads
+  // above are in flight); else (the first cycle) the current bubble residuals are those of
+  // parity 1 - par
+  const double alpha = UPD ? fold_spmv(partA, bpartA, nbs, par, B, sh, res, aps, rbs, sc) : 0.0;
+  const double beta = UPD ? sc->beta : 0.0;
   Cells4 cc;
 #pragma unroll
   for (int cq = 0; cq < 2; cq++)
@@ -629,7 +823,8 @@ __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMas
           bval = rn;
         } else if (m & MK_BUB) {
           const uint32_t lb = rot[c];
-          bval = (float)(rb[lb] / cellsB[lb]);
+          const double rbl = UPD ? bubble_r(B, lb, par, alpha, beta, rbs) : rb[lb];
+          bval = (float)(rbl / cellsB[lb]);
         }
         bval = rnd<SB>(bval * scale);
         st<SB>(b0, c, bval);
@@ -643,7 +838,7 @@ __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMas
     tile_sweeps<0>(T, cc, lane);
     store_cells<float>(cc, x0, ox, oy, nx, ny, lane);
   }
-  if (UPD) store_partials<1>(rr, 1, partials, sh);
+  if (UPD) store_partials<1>(rr, 1, partRR, sh);
 }
 
 struct ZArgs {  // level-0 outputs: z = MG result and the dots r.z, r.z_old
@@ -654,9 +849,10 @@ struct ZArgs {  // level-0 outputs: z = MG result and the dots r.z, r.z_old
   const float* r;
   const float* zold;
   float* znew;
-  double* bvec;
-  double* partials;
-  double* bpart;
+  double* accZ;       // bubble z sums of labels without a slot (FP64 atomics)
+  double* partials;   // r.z, r.z_old partials (two slots)
+  double* bpart;      // bubble z sums of the slotted labels
+  const float* mgs;   // the V-cycle input scale of this cycle
   int scaled;
 };
 
@@ -671,7 +867,7 @@ __device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, f
     dv[1] += (double)rc * zoldc;
   } else if (om & MK_BUB) {
     const uint32_t lb = za.rot[c];
-    bub_add(sl, za.labelSlot, bv(za.bvec, BV_ACC1), lb, (double)x / za.cellsB[lb]);
+    bub_add(sl, za.labelSlot, za.accZ, lb, (double)x / za.cellsB[lb]);
   }
   za.znew[c] = z;
 }
@@ -685,7 +881,7 @@ __global__ __launch_bounds__(256) void k_zdots0(const float* __restrict__ x0, ZA
   double dv[2] = {0, 0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const float inv = za.scaled ? 1.f / sc->mg_scale : 1.f;
+  const float inv = za.scaled ? 1.f / *za.mgs : 1.f;
   if (c < C) z_cell(za, c, za.opMask[c], za.r[c], za.zold[c], x0[c], inv, dv, sl);
   store_partials<2>(dv, 2, za.partials, sh);
   if (ns > 0) store_partials<BSLOTS>(sl, ns, za.bpart, sh);
@@ -812,7 +1008,7 @@ __global__ __launch_bounds__(256) void k_pass_z(const float* __restrict__ src, c
     Cells4 cc;
     init_cells(cc, own, T, lane);
     tile_sweeps<1>(T, cc, lane);
-    const float inv = za.scaled ? 1.f / sc->mg_scale : 1.f;
+    const float inv = za.scaled ? 1.f / *za.mgs : 1.f;
 #pragma unroll
     for (int cq = 0; cq < 2; cq++)
 #pragma unroll
@@ -843,7 +1039,8 @@ __global__ __launch_bounds__(64 * BRA_WAVES) void k_bra(const SXF* __restrict__ 
                                                        const SBF* __restrict__ fb, const uint8_t* __restrict__ fm,
                                                        int fnx, int fny, SCB* __restrict__ cb, SCX* __restrict__ cxo,
                                                        const uint8_t* __restrict__ cmk, int cnx, int cny,
-                                                       const DevScalars* __restrict__ sc) {
+                                                       const double* __restrict__ partRR, int nbu, int par, int fold,
+                                                       DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ float Tt[BRA_WAVES][TCELLS];
   __shared__ float XB[XBW * XBW];
@@ -883,6 +1080,10 @@ __global__ __launch_bounds__(64 * BRA_WAVES) void k_bra(const SXF* __restrict__ 
       cma[cq][rw] = (PASSA && wave == 0 && I < cnx && J < cny) ? cmk[(size_t)I + (size_t)J * cnx] : 0;
     }
   fill_tile<SXF>(T, fx, ox, oy, fnx, fny, lane);
+  if (fold) {  // level 0 of an iteration: the norm and stop test (fold_update) while the loads land
+    __shared__ double fsh[16], fres[1];
+    if (fold_update(partRR, nbu, par, fsh, fres, sc)) return;
+  }
   wave_sync();
   // ---- pass B on this wave's tile (colours 0,1), into the window and (own cells) to memory
   {
@@ -1097,7 +1298,7 @@ __global__ __launch_bounds__(256) void k_pcd(const SX* __restrict__ xb, SX* __re
     if (!ZD) {
       store_cells<SX>(cc, xout, ax, ay, nx, ny, lane);
     } else {
-      const float inv = za.scaled ? 1.f / sc->mg_scale : 1.f;
+      const float inv = za.scaled ? 1.f / *za.mgs : 1.f;
 #pragma unroll
       for (int cq = 0; cq < 2; cq++)
 #pragma unroll
@@ -1491,185 +1692,88 @@ __global__ __launch_bounds__(256) void k_vcycle_apply(const float* __restrict__ 
 }
 
 
-// ================================================================== z dots -> beta
-__global__ __launch_bounds__(FIN_THREADS) void k_fin_zdots(double* __restrict__ bvec, const int* __restrict__ activeList,
-                                                           const int* __restrict__ labelSlot,
-                                                           const double* __restrict__ partials,
-                                                           const double* __restrict__ bpart, int nb, int first,
-                                                           DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ double sh[BSLOTS * 16], res[BSLOTS], slotTot[BSLOTS];
-  const int na = sc->active_bubbles;
-  double tot[2];
-  sum_partials<2>(partials, nb, 2, tot, sh, res);
-  double stt[BSLOTS];
-  sum_partials<BSLOTS>(bpart, nb, nslots(na), stt, sh, res);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < BSLOTS; k++) slotTot[k] = stt[k];
-  __syncthreads();
-  double* rb = bv(bvec, BV_R);
-  double* zb = bv(bvec, BV_Z);
-  double* zvb = bv(bvec, BV_ZV);
-  double* acc1 = bv(bvec, BV_ACC1);
-  double bacc[2] = {0, 0};
-  for (int k = threadIdx.x; k < na; k += blockDim.x) {
-    const int b = activeList[k];
-    const int s = labelSlot[b];
-    double znew;
-    if (s >= 0) znew = slotTot[s];
-    else { znew = acc1[b]; acc1[b] = 0; }
-    zvb[b] = zb[b];
-    zb[b] = znew;
-    bacc[0] += rb[b] * zb[b];
-    bacc[1] += rb[b] * zvb[b];
-  }
-  block_sum<2>(bacc, sh);
-  if (threadIdx.x == 0) {
-    const double rzNovo = tot[0] + bacc[0];
-    const double cruzado = tot[1] + bacc[1];
-    // Polak-Ribiere beta (pressao-cpu.js:390-394)
-    if (first) sc->beta = 0;
-    else {
-      const double rz = sc->rz;
-      sc->beta = fabs(rz) < 1e-30 ? 0 : fmax(0.0, (rzNovo - cruzado) / rz);
-    }
-    sc->rz = rzNovo;
-    sc->mg_scale = sc->mg_scale_next;
-  }
-}
-
-// ================================================================== p = z + beta p, Ap, p.Ap
+// First kernel of iteration k: the z dots and beta (fold_zdots), then per cell p = z + beta p_old
+// (bubble cells: their new z and old p), Ap and the p.Ap partials (produto / aplicar,
+// pressao-cpu.js:84-93, 257-261); SPMV_CELLS cells per thread so the partials stay few.
+constexpr int SPMV_CELLS = 4;
 __global__ __launch_bounds__(256) void k_spmv(const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
                                               const float* __restrict__ z, const float* __restrict__ pold,
                                               float* __restrict__ pnew, float* __restrict__ ap, int nx, int C,
-                                              const double* __restrict__ bvec, const int* __restrict__ labelSlot,
-                                              double* __restrict__ partials, double* __restrict__ bpart,
-                                              const DevScalars* __restrict__ sc) {
+                                              BubArgs B, const double* __restrict__ partZ,
+                                              const double* __restrict__ bpartZ, int nbz, int par, int first,
+                                              double* __restrict__ partA, double* __restrict__ bpartA,
+                                              DevScalars* __restrict__ sc) {
   if (sc->done) return;
-  __shared__ double sh[BSLOTS * 16];
-  const double beta = sc->beta;
+  __shared__ double sh[BSLOTS * 16], res[BSLOTS], zs[BSLOTS];
+  // every load of the cells first (clamped indices; masks decide what is used): they do not
+  // depend on beta, so their latency overlaps the fold's reductions
+  int nbr[SPMV_CELLS][5];
+  uint8_t mk[SPMV_CELLS][5];
+  float zv[SPMV_CELLS][5], pv[SPMV_CELLS][5];
+#pragma unroll
+  for (int q = 0; q < SPMV_CELLS; q++) {
+    const int c = blockIdx.x * (256 * SPMV_CELLS) + q * 256 + threadIdx.x;
+    const int cc = c < C ? c : C - 1;
+    const int nb0[5] = {cc, cc - 1, cc + 1, cc - nx, cc + nx};
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      const int d = nb0[k] < 0 ? 0 : (nb0[k] >= C ? C - 1 : nb0[k]);
+      nbr[q][k] = d;
+      mk[q][k] = opMask[d];
+      zv[q][k] = z[d];
+      pv[q][k] = pold[d];
+    }
+  }
+  const double beta = fold_zdots(partZ, bpartZ, nbz, par, first, B, sh, res, zs, sc);
   const int ns = nslots(sc->active_bubbles);
-  const double* zb = bv((double*)bvec, BV_Z);
-  const double* pb = bv((double*)bvec, BV_P);
-  double* acc0 = bv((double*)bvec, BV_ACC0);
-  double v[1] = {0};
-  double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double* Po = bv(B.bvec, BV_P0 + 1 - par);
+  const double* accZ = bv(B.bvec, BV_ACCZ0 + par);
+  double* accA = bv(B.bvec, BV_ACCA0 + par);
   // p of a cell from its loaded z, p_old and mask (bubble cells from the bubble vectors)
   auto peff = [&](int d, uint8_t md, float zd, float pd) -> float {
     if (mask_type(md) == AGUA) return (float)((double)zd + beta * (double)pd);
     if (md & MK_BUB) {
       const uint32_t lb = rot[d];
-      return (float)(zb[lb] + beta * pb[lb]);
+      const int sl = B.labelSlot[lb];
+      const double zn = sl >= 0 ? zs[sl] : accZ[lb];
+      return (float)(zn + beta * Po[lb]);
     }
     return 0.f;
   };
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < C) {
-    // every load first (clamped indices; masks decide what is used)
-    int nb[5] = {c, c - 1, c + 1, c - nx, c + nx};
-    uint8_t mk[5];
-    float zv[5], pv[5];
+  double v[1] = {0};
+  double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-    for (int k = 0; k < 5; k++) {
-      nb[k] = nb[k] < 0 ? 0 : (nb[k] >= C ? C - 1 : nb[k]);
-      mk[k] = opMask[nb[k]];
-      zv[k] = z[nb[k]];
-      pv[k] = pold[nb[k]];
+  for (int q = 0; q < SPMV_CELLS; q++) {
+    const int c = blockIdx.x * (256 * SPMV_CELLS) + q * 256 + threadIdx.x;
+    if (c < C) {
+      const uint8_t m = mk[q][0];
+      const float pc = peff(c, m, zv[q][0], pv[q][0]);
+      pnew[c] = pc;
+      if (mask_type(m) == AGUA) {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          if (!(m & (1 << k))) continue;
+          const uint8_t md = mk[q][k + 1];
+          s += (double)peff(nbr[q][k + 1], md, zv[q][k + 1], pv[q][k + 1]);
+          if (md & MK_BUBIN) bub_add(sl, B.labelSlot, accA, rot[nbr[q][k + 1]], -(double)pc);
+        }
+        const float a = (float)(mask_cnt(m) * (double)pc - s);
+        ap[c] = a;
+        v[0] += (double)pc * a;
+      } else ap[c] = 0.f;
     }
-    const uint8_t m = mk[0];
-    const float pc = peff(c, m, zv[0], pv[0]);
-    pnew[c] = pc;
-    if (mask_type(m) == AGUA) {
-      double s = 0;
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        if (!(m & (1 << k))) continue;
-        const uint8_t md = mk[k + 1];
-        s += (double)peff(nb[k + 1], md, zv[k + 1], pv[k + 1]);
-        if (md & MK_BUBIN) bub_add(sl, labelSlot, acc0, rot[nb[k + 1]], -(double)pc);
-      }
-      const float a = (float)(mask_cnt(m) * (double)pc - s);
-      ap[c] = a;
-      v[0] = (double)pc * a;
-    } else ap[c] = 0.f;
   }
-  store_partials<1>(v, 1, partials, sh);
-  if (ns > 0) store_partials<BSLOTS>(sl, ns, bpart, sh);
+  store_partials<1>(v, 1, partA, sh);
+  if (ns > 0) store_partials<BSLOTS>(sl, ns, bpartA, sh);
 }
 
-__global__ __launch_bounds__(FIN_THREADS) void k_fin_spmv(const uint32_t* __restrict__ facesB, double* __restrict__ bvec,
-                                                          const int* __restrict__ activeList,
-                                                          const int* __restrict__ labelSlot,
-                                                          const double* __restrict__ partials,
-                                                          const double* __restrict__ bpart, int nb,
-                                                          DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ double sh[BSLOTS * 16], res[BSLOTS], slotTot[BSLOTS];
-  __shared__ double alpha_s;
-  const int na = sc->active_bubbles;
-  double tot[1];
-  sum_partials<1>(partials, nb, 1, tot, sh, res);
-  double stt[BSLOTS];
-  sum_partials<BSLOTS>(bpart, nb, nslots(na), stt, sh, res);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < BSLOTS; k++) slotTot[k] = stt[k];
-  __syncthreads();
-  const double beta = sc->beta;
-  double* zb = bv(bvec, BV_Z);
-  double* pb = bv(bvec, BV_P);
-  double* apb = bv(bvec, BV_AP);
-  double* acc0 = bv(bvec, BV_ACC0);
-  double bacc[1] = {0};
-  for (int k = threadIdx.x; k < na; k += blockDim.x) {
-    const int b = activeList[k];
-    const int s = labelSlot[b];
-    double accB;
-    if (s >= 0) accB = slotTot[s];
-    else { accB = acc0[b]; acc0[b] = 0; }
-    const double pn = zb[b] + beta * pb[b];
-    pb[b] = pn;
-    apb[b] = (double)facesB[b] * pn + accB;
-    bacc[0] += pn * apb[b];
-  }
-  block_sum<1>(bacc, sh);
-  if (threadIdx.x == 0) {
-    const double pap = tot[0] + bacc[0];
-    alpha_s = fabs(pap) < 1e-30 ? 0 : sc->rz / pap;
-    sc->alpha = alpha_s;
-  }
-  __syncthreads();
-  const double alpha = alpha_s;
-  double* yb = bv(bvec, BV_Y);
-  double* rb = bv(bvec, BV_R);
-  double rbs[1] = {0};
-  for (int k = threadIdx.x; k < na; k += blockDim.x) {
-    const int b = activeList[k];
-    yb[b] += alpha * pb[b];
-    rb[b] -= alpha * apb[b];
-    rbs[0] += rb[b] * rb[b];
-  }
-  block_sum<1>(rbs, sh);
-  if (threadIdx.x == 0) sc->pap = rbs[0];  // bubble part of |r|^2 for fin_update
-}
-
-__global__ __launch_bounds__(FIN_THREADS) void k_fin_update(const double* __restrict__ partials, int nb,
-                                                            DevScalars* __restrict__ sc) {
+// the update fold as a kernel of its own, when level 0 has no down kernel (one-workgroup grids)
+__global__ __launch_bounds__(FIN_THREADS) void k_fold_update(const double* __restrict__ part, int nb, int par,
+                                                             DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[16], res[1];
-  double tot[1];
-  sum_partials<1>(partials, nb, 1, tot, sh, res);
-  if (threadIdx.x == 0) {
-    const double n2 = tot[0] + sc->pap;
-    const double nr = sqrt(n2);
-    sc->norm2 = n2;
-    sc->norm_last = nr;
-    sc->iters += 1;
-    sc->total_iters += 1;
-    bool stop = sc->iters >= sc->maxit;
-    if (sc->tol > 0 && (nr == 0 || nr <= sc->tol * sc->normb)) stop = true;
-    sc->done = stop ? 1 : 0;
-    sc->mg_scale_next = nr > 0 ? (float)(sc->mg_mult / nr) : 1.f;
-  }
+  (void)fold_update(part, nb, par, sh, res, sc);
 }
 
 // ================================================================== end of the solve
@@ -1678,7 +1782,12 @@ __global__ __launch_bounds__(256) void k_pcg_final(const uint8_t* __restrict__ t
   if (c < C && tipo[c] != AGUA) q[c] = 0.f;
 }
 __global__ __launch_bounds__(1024) void k_pcg_final_b(const double* __restrict__ bvec, const int* __restrict__ activeList,
-                                                      float* __restrict__ yB32, const DevScalars* __restrict__ sc) {
+                                                      float* __restrict__ yB32, DevScalars* __restrict__ sc) {
+  if (threadIdx.x == 0) {
+    const int it = sc->itp[0] > sc->itp[1] ? sc->itp[0] : sc->itp[1];
+    sc->iters = it;
+    sc->total_iters += (unsigned long long)it;
+  }
   for (int k = threadIdx.x; k < sc->active_bubbles; k += blockDim.x) {
     const int b = activeList[k];
     yB32[b] = (float)bv((double*)bvec, BV_Y)[b];
@@ -1694,7 +1803,7 @@ __global__ __launch_bounds__(256) void k_true_res(const uint8_t* __restrict__ op
   __shared__ double sh[BSLOTS * 16];
   const int ns = nslots(sc->active_bubbles);
   const double* yb = bv(bvec, BV_Y);
-  double* acc0 = bv(bvec, BV_ACC0);
+  double* acc0 = bv(bvec, BV_ACCI);
   const double mean = sc->proj_mean;
   double v[1] = {0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1742,7 +1851,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_true_res(const uint32_t* __
     for (int k = 0; k < BSLOTS; k++) slotTot[k] = stt[k];
   __syncthreads();
   const double* yb = bv(bvec, BV_Y);
-  double* acc0 = bv(bvec, BV_ACC0);
+  double* acc0 = bv(bvec, BV_ACCI);
   double bacc[1] = {0};
   for (int k = threadIdx.x; k < na; k += blockDim.x) {
     const int b = activeList[k];
@@ -1824,7 +1933,7 @@ int solver_alloc(tq_sim* s) {
       TQ_HIP(hipMalloc(&s->vop, sizeof(float) * (size_t)s->vop_ld * s->lv[lc].n));
     }
   }
-  TQ_HIP(hipMalloc(&s->bpart, sizeof(double) * BSLOTS * MAX_RED_BLOCKS));
+  TQ_HIP(hipMalloc(&s->bpart, sizeof(double) * 2 * BSLOTS * MAX_RED_BLOCKS));  // spmv's, then the z dots'
   TQ_HIP(hipMalloc(&s->labelSlot, sizeof(int) * MAX_BOLHAS));
   return 0;
 }
@@ -1882,6 +1991,23 @@ static int blocks_of(int C) { return (C + 255) / 256; }
 
 static double vbytes(bool h) { return h ? 2.0 : 4.0; }
 
+// bubble-slot partials: spmv (and the solve's set-up) use the first half, the z dots the second
+static double* bpart_a(tq_sim* s) { return s->bpart; }
+static double* bpart_z(tq_sim* s) { return s->bpart + (size_t)BSLOTS * MAX_RED_BLOCKS; }
+static BubArgs bubargs(tq_sim* s) {
+  BubArgs B;
+  B.bvec = s->bvec;
+  B.activeList = s->activeList;
+  B.labelSlot = s->labelSlot;
+  B.facesB = s->facesB;
+  return B;
+}
+// Parities of the launch in progress (iteration k = s->it_k; k = -1: the first cycle).  The
+// cycle of iteration k scales its input by mgs[k & 1] and leaves its z sums for iteration k+1
+// (accumulator and partials of parity (k+1) & 1).
+static int it_par(const tq_sim* s) { return s->it_k & 1; }
+static int z_par(const tq_sim* s) { return (s->it_k + 1) & 1; }
+
 static void launch_in_a0(tq_sim* s, bool upd) {
   Level& F = s->lv[0];
   const int ntx = tiles_of(F.nx, 0), nty = tiles_of(F.ny, 0);
@@ -1891,9 +2017,15 @@ static void launch_in_a0(tq_sim* s, bool upd) {
   // bytes: opMask, mg0, b0 write, x0 write; update: q, p, r, Ap read + q, r write
   const double by = (2 + vbytes(F.hb) + (pa ? 4 : 0) + (upd ? 24 : 4)) * (double)F.n;
   int t = time_begin(s, upd ? "pcg_update_mg_in_l0" : "pcg_mg_in_l0", by);
+  // the first cycle reads the bubble residuals of parity 1 (par 0 selects them)
+  const int par = upd ? it_par(s) : 0;
+  const BubArgs B = bubargs(s);
+  const float* mgs = &s->sc->mgs[it_par(s)];
 #define TQ_IN_A0(SB, U, P)                                                                                        \
   hipLaunchKernelGGL((k_in_a0<SB, U, P>), g, dim3(256), 0, s->st, s->opMask, F.mask, s->rot, s->cellsB, pp, s->ap, \
-                     s->q, s->r, F.nx, F.ny, ntx, nty, s->bvec, (SB*)F.b, (float*)F.x, s->partials, s->mg_scaled, s->sc)
+                     s->q, s->r, F.nx, F.ny, ntx, nty, B, (const double*)(s->partials + ps_off(PS_SPMV)),        \
+                     (const double*)bpart_a(s), s->nb_spmv, par, (SB*)F.b, (float*)F.x,                          \
+                     s->partials + ps_off(PS_RR), s->mg_scaled, mgs, s->sc)
   if (F.hb) {
     if (upd) { if (pa) TQ_IN_A0(__half, true, true); else TQ_IN_A0(__half, true, false); }
     else { if (pa) TQ_IN_A0(__half, false, true); else TQ_IN_A0(__half, false, false); }
@@ -1903,6 +2035,7 @@ static void launch_in_a0(tq_sim* s, bool upd) {
   }
 #undef TQ_IN_A0
   time_end(s, t);
+  s->nb_upd = (int)(g.x * g.y);
 }
 
 // storage-type dispatch: fn(x tag, b tag) with the level's element types
@@ -1924,6 +2057,9 @@ static void launch_bra(tq_sim* s, int l) {
   Level& G = s->lv[l + 1];
   const bool pa = l + 1 < (int)s->lv.size() - 1 && !(l + 1 == s->lc && vop_active(s));
   const dim3 g((G.nx + 15) / 16, (G.ny + 15) / 16);
+  // level 0 of an iteration (not the first cycle) starts with the norm and stop test
+  const int fold = (l == 0 && s->it_k >= 0) ? 1 : 0;
+  const double* rr = s->partials + ps_off(PS_RR);
   // x(l) and b(l) and mask(l) in, x_B(l) out; coarse mask in, b (and x) out
   int t = time_begin(s, l == 0 ? "mg_down_l0" : "mg_down",
                      (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n +
@@ -1937,11 +2073,11 @@ static void launch_bra(tq_sim* s, int l) {
       if (pa)
         hipLaunchKernelGGL((k_bra<SXF, SBF, SCB, SCX, true>), g, dim3(64 * BRA_WAVES), 0, s->st, (const SXF*)F.x,
                            (SXF*)F.tmp, (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny,
-                           s->sc);
+                           rr, s->nb_upd, it_par(s), fold, s->sc);
       else
         hipLaunchKernelGGL((k_bra<SXF, SBF, SCB, SCX, false>), g, dim3(64 * BRA_WAVES), 0, s->st, (const SXF*)F.x,
                            (SXF*)F.tmp, (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny,
-                           s->sc);
+                           rr, s->nb_upd, it_par(s), fold, s->sc);
     });
   });
   time_end(s, t);
@@ -1981,7 +2117,10 @@ static ZArgs zargs(tq_sim* s) {
   ZArgs za;
   za.opMask = s->opMask; za.rot = s->rot; za.cellsB = s->cellsB;
   za.labelSlot = s->labelSlot; za.r = s->r; za.zold = s->zbuf[s->zc]; za.znew = s->zbuf[1 - s->zc];
-  za.bvec = s->bvec; za.partials = s->partials; za.bpart = s->bpart;
+  za.accZ = bv(s->bvec, BV_ACCZ0 + z_par(s));
+  za.partials = s->partials + ps_off(PS_Z);
+  za.bpart = bpart_z(s);
+  za.mgs = &s->sc->mgs[it_par(s)];
   za.scaled = s->mg_scaled;
   return za;
 }
@@ -2074,47 +2213,46 @@ static int launch_up0(tq_sim* s) {
   return (int)(g.x * g.y);
 }
 
-// V-cycle on r (level 0 input already built by in_a0, whose pass A is done) -> z, beta
-static void precondition_rest(tq_sim* s, int first) {
+// V-cycle on r (level 0 input already built by in_a0, whose pass A is done) -> z and the z dots
+static void precondition_rest(tq_sim* s) {
   const int lc = s->lc;
+  if (lc == 0 && s->it_k >= 0) {  // no level-0 down kernel to carry the norm and stop test
+    int t = time_begin(s, "pcg_update_fold", 0);
+    hipLaunchKernelGGL(k_fold_update, dim3(1), dim3(FIN_THREADS), 0, s->st,
+                       (const double*)(s->partials + ps_off(PS_RR)), s->nb_upd, it_par(s), s->sc);
+    time_end(s, t);
+  }
   for (int l = 0; l < lc; l++) launch_bra(s, l);
   launch_coarse_wg(s);
-  int nb;
   if (lc > 0) {
     for (int l = lc - 1; l >= 1; l--) launch_pcd(s, l);
-    nb = launch_up0(s);
+    s->nb_z = launch_up0(s);
   } else {
-    nb = blocks_of(s->C);
+    s->nb_z = blocks_of(s->C);
     int t = time_begin(s, "pcg_zdots", 16.0 * s->C);
-    hipLaunchKernelGGL(k_zdots0, dim3(nb), dim3(256), 0, s->st, (const float*)s->lv[0].x, zargs(s), s->C, s->sc);
+    hipLaunchKernelGGL(k_zdots0, dim3(s->nb_z), dim3(256), 0, s->st, (const float*)s->lv[0].x, zargs(s), s->C, s->sc);
     time_end(s, t);
   }
   s->zc = 1 - s->zc;
-  int t = time_begin(s, "pcg_fin_zdots", 0);
-  hipLaunchKernelGGL(k_fin_zdots, dim3(1), dim3(FIN_THREADS), 0, s->st, s->bvec, s->activeList, s->labelSlot,
-                     s->partials, s->bpart, nb, first, s->sc);
-  time_end(s, t);
 }
 
+// PCG iteration k = s->it_k: spmv (z dots, beta, p, Ap), in_a0 (alpha, update, V-cycle input),
+// the V-cycle (level 0's down kernel starts with the norm and stop test)
 static void pcg_iteration(tq_sim* s) {
   const int C = s->C;
-  const int nb = blocks_of(C);
+  const int nbs = (C + 256 * SPMV_CELLS - 1) / (256 * SPMV_CELLS);
   const int pn = 1 - s->pc;
   int t = time_begin(s, "pcg_spmv", (1 + 4 + 4 + 4 + 4) * (double)C);
-  hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->zbuf[s->zc], s->pbuf[s->pc],
-                     s->pbuf[pn], s->ap, s->nx, C, s->bvec, s->labelSlot, s->partials, s->bpart, s->sc);
+  hipLaunchKernelGGL(k_spmv, dim3(nbs), dim3(256), 0, s->st, s->opMask, s->rot, s->zbuf[s->zc], s->pbuf[s->pc],
+                     s->pbuf[pn], s->ap, s->nx, C, bubargs(s), (const double*)(s->partials + ps_off(PS_Z)),
+                     (const double*)bpart_z(s), s->nb_z, it_par(s), s->it_k == 0 ? 1 : 0,
+                     s->partials + ps_off(PS_SPMV), bpart_a(s), s->sc);
   time_end(s, t);
-  t = time_begin(s, "pcg_fin_spmv", 0);
-  hipLaunchKernelGGL(k_fin_spmv, dim3(1), dim3(FIN_THREADS), 0, s->st, s->facesB, s->bvec, s->activeList,
-                     s->labelSlot, s->partials, s->bpart, nb, s->sc);
-  time_end(s, t);
+  s->nb_spmv = nbs;
   s->pc = pn;
   launch_in_a0(s, true);
-  const dim3 g0 = tile_grid(s->lv[0].nx, s->lv[0].ny, 0);
-  t = time_begin(s, "pcg_fin_update", 0);
-  hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(FIN_THREADS), 0, s->st, s->partials, (int)(g0.x * g0.y), s->sc);
-  time_end(s, t);
-  precondition_rest(s, 0);
+  precondition_rest(s);
+  s->it_k++;
 }
 
 // Two PCG iterations (the buffer ping-pong period) captured once into a HIP graph and
@@ -2125,11 +2263,13 @@ static int graph_pair(tq_sim* s) {
   if (!s->graph_exec || s->graph_key != key) {
     graph_drop(s);
     hipGraph_t g = nullptr;
+    const int k0 = s->it_k;
     s->capturing = true;
     TQ_HIP(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
     pcg_iteration(s);
     pcg_iteration(s);
     const hipError_t e = hipStreamEndCapture(s->st, &g);
+    s->it_k = k0;
     s->capturing = false;
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
     hipGraphExec_t ex = nullptr;
@@ -2141,17 +2281,23 @@ static int graph_pair(tq_sim* s) {
     // the capture flipped the ping-pong state twice: back where it started
   }
   TQ_HIP(hipGraphLaunch((hipGraphExec_t)s->graph_exec, s->st));
+  s->it_k += 2;
   return 0;
 }
 
+// The captured pair's parity-dependent buffers are fixed, so pairs start at odd iterations;
+// iteration 0 (beta = 0) and any unpaired iteration run as eager launches.
 static int run_iterations(tq_sim* s, int n) {
-  if (s->d.use_graph && !s->timing) {
-    for (; n >= 2; n -= 2) {
-      const int rc = graph_pair(s);
-      if (rc) return rc;
+  const bool graph = s->d.use_graph && !s->timing;
+  while (n > 0) {
+    if (graph && n >= 2 && (s->it_k & 1)) {
+      if (const int rc = graph_pair(s)) return rc;
+      n -= 2;
+    } else {
+      pcg_iteration(s);
+      n--;
     }
   }
-  for (; n > 0; n--) pcg_iteration(s);
   TQ_LAUNCH_CHECK();
   return 0;
 }
@@ -2234,8 +2380,10 @@ int pressure_solve(tq_sim* s) {
                      s->labelSlot, s->partials, s->bpart, nb, s->sc);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
+  s->it_k = -1;
   launch_in_a0(s, false);
-  precondition_rest(s, 1);
+  precondition_rest(s);
+  s->it_k = 0;
   TQ_LAUNCH_CHECK();
   if (tol <= 0) {
     const int rc = run_iterations(s, maxit);
@@ -2264,7 +2412,7 @@ int pressure_solve(tq_sim* s) {
 int true_residual(tq_sim* s, double* out) {
   const int C = s->C;
   const int nb = blocks_of(C);
-  double* dres = s->partials + 8 * (size_t)MAX_RED_BLOCKS - 1;
+  double* dres = s->partials + ps_off(PS_LAST) + MAX_RED_BLOCKS - 1;
   hipLaunchKernelGGL(k_true_res, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->div, s->q, s->nx, C, s->bvec,
                      s->labelSlot, s->partials, s->bpart, s->sc);
   hipLaunchKernelGGL(k_fin_true_res, dim3(1), dim3(FIN_THREADS), 0, s->st, s->facesB, s->bvec, s->rhsB, s->activeList,
@@ -2293,10 +2441,11 @@ int debug_vcycle(tq_sim* s, const float* hb, float* hz) {
   TQ_HIP(hipMemcpyAsync(s->r, hb, sizeof(float) * C, hipMemcpyHostToDevice, s->st));
   TQ_HIP(hipMemcpyAsync(&s->sc->done, &zero, sizeof(int), hipMemcpyHostToDevice, s->st));
   TQ_HIP(hipMemcpyAsync(&s->sc->proj_mean, &dzero, sizeof(double), hipMemcpyHostToDevice, s->st));
-  TQ_HIP(hipMemcpyAsync(&s->sc->mg_scale, &scale, sizeof(float), hipMemcpyHostToDevice, s->st));
+  TQ_HIP(hipMemcpyAsync(&s->sc->mgs[1], &scale, sizeof(float), hipMemcpyHostToDevice, s->st));
   TQ_HIP(hipMemcpyAsync(&s->sc->mg_mult, &mult, sizeof(float), hipMemcpyHostToDevice, s->st));
+  s->it_k = -1;
   launch_in_a0(s, false);
-  precondition_rest(s, 1);
+  precondition_rest(s);
   TQ_LAUNCH_CHECK();
   TQ_HIP(hipMemcpyAsync(hz, s->zbuf[s->zc], sizeof(float) * C, hipMemcpyDeviceToHost, s->st));
   TQ_HIP(hipStreamSynchronize(s->st));

@@ -116,10 +116,10 @@ static int alloc_all(tq_sim* s) {
   DALLOC(s->pBolha, MAX_BOLHAS); DALLOC(s->volB, MAX_BOLHAS); DALLOC(s->facesB, MAX_BOLHAS); DALLOC(s->cellsB, MAX_BOLHAS);
   DALLOC(s->rhsB, MAX_BOLHAS); DALLOC(s->rhsB32, MAX_BOLHAS); DALLOC(s->yB32, MAX_BOLHAS); DALLOC(s->volAlvo, MAX_BOLHAS);
   DALLOC(s->vivo, MAX_BOLHAS); DALLOC(s->vivoNovo, MAX_BOLHAS);
-  DALLOC(s->bvec, 8 * (size_t)MAX_BOLHAS); DALLOC(s->activeList, MAX_BOLHAS);
+  DALLOC(s->bvec, BV_COUNT * (size_t)MAX_BOLHAS); DALLOC(s->activeList, MAX_BOLHAS);
   DALLOC(s->opMask, C); DALLOC(s->r, C); DALLOC(s->ap, C);
   for (int b = 0; b < 2; b++) { DALLOC(s->zbuf[b], C); DALLOC(s->pbuf[b], C); }
-  DALLOC(s->partials, 8 * (size_t)MAX_RED_BLOCKS);
+  DALLOC(s->partials, RED_SLOTS * (size_t)MAX_RED_BLOCKS);
   DALLOC(s->sc, 1);
   {
     // block sums of the largest scan (particle slots, binning keys or cells), two levels
@@ -144,7 +144,7 @@ static int alloc_all(tq_sim* s) {
   TQ_HIP(z(s->pBolha, 4 * MAX_BOLHAS)); TQ_HIP(z(s->volB, 4 * MAX_BOLHAS)); TQ_HIP(z(s->facesB, 4 * MAX_BOLHAS));
   TQ_HIP(z(s->cellsB, 4 * MAX_BOLHAS)); TQ_HIP(z(s->rhsB, 8 * MAX_BOLHAS)); TQ_HIP(z(s->rhsB32, 4 * MAX_BOLHAS));
   TQ_HIP(z(s->yB32, 4 * MAX_BOLHAS)); TQ_HIP(z(s->volAlvo, 4 * MAX_BOLHAS)); TQ_HIP(z(s->vivo, MAX_BOLHAS));
-  TQ_HIP(z(s->bvec, 8 * 8 * (size_t)MAX_BOLHAS));
+  TQ_HIP(z(s->bvec, 8 * BV_COUNT * (size_t)MAX_BOLHAS));
   for (int b = 0; b < 2; b++) { TQ_HIP(z(s->zbuf[b], 4 * C)); TQ_HIP(z(s->pbuf[b], 4 * C)); }
   TQ_HIP(z(s->r, 4 * C)); TQ_HIP(z(s->ap, 4 * C));
   TQ_HIP(z(s->sc, sizeof(DevScalars)));

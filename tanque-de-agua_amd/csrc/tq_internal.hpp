@@ -52,6 +52,8 @@ constexpr double FX_SCALE = 70368744177664.0;  // 2^46
 constexpr double FX_INV = 1.0 / 70368744177664.0;
 constexpr int RED_BLOCK = 256;           // block size of reduction kernels
 constexpr int MAX_RED_BLOCKS = 65536;    // partial slots per reduction (one per block)
+constexpr int RED_SLOTS = 10;            // reductions with their own partial arrays (solver layout)
+constexpr int BV_COUNT = 13;             // solver bubble vectors of MAX_BOLHAS doubles each
 
 // mask byte of a multigrid cell / operator cell
 //   bits 0-3: free neighbours (1 left, 2 right, 4 down, 8 up) as prepararVizinhos
@@ -87,6 +89,9 @@ struct DevScalars {
   float mg_scale;       // mixed precision: scale applied to the V-cycle input of this iteration
   float mg_scale_next;  // mg_mult/||r|| of the newest residual, the scale of the next iteration
   float mg_mult;        // numerator of the scale: FP16 values of the input sit this far above 1/||r||
+  double rzp[2];        // r.z of the last two iterations, by iteration parity
+  int itp[2];           // iterations completed, by the parity of the iteration that completed them
+  float mgs[2];         // mixed precision: V-cycle input scale of the iterations of each parity
   int op_dirty;         // this solve's one-workgroup level masks differ from the last operator build
   unsigned long long op_builds;  // operator builds (coarsest and level-lc matrices) since creation
 };
@@ -176,7 +181,7 @@ struct tq_sim {
   float* volAlvo = nullptr;    // volumeAlvo
   uint8_t* vivo = nullptr;
   uint8_t* vivoNovo = nullptr;
-  double* bvec = nullptr;      // bubble CG vectors: yb rb zb zvb pb apb acc0 acc1 (8 x MAX_BOLHAS)
+  double* bvec = nullptr;      // bubble CG vectors (BV_COUNT x MAX_BOLHAS; layout in k_solver.hip)
   int* activeList = nullptr;   // active bubble labels (count in sc->active_bubbles)
 
   // solver
@@ -186,8 +191,10 @@ struct tq_sim {
   float* zbuf[2] = {};
   float* pbuf[2] = {};
   int zc = 0, pc = 0;
-  double* partials = nullptr;   // per-block partial sums: 8 x MAX_RED_BLOCKS
-  double* bpart = nullptr;      // per-block partial sums of the slotted bubbles: 8 x MAX_RED_BLOCKS
+  double* partials = nullptr;   // per-block partial sums: RED_SLOTS x MAX_RED_BLOCKS
+  double* bpart = nullptr;      // per-block sums of the slotted bubbles: 2 x BSLOTS x MAX_RED_BLOCKS
+  int it_k = -1;                // index of the PCG iteration being launched (-1: the initial V-cycle)
+  int nb_spmv = 0, nb_upd = 0, nb_z = 0;  // partial counts of the last spmv, update and z launches
   int* labelSlot = nullptr;     // bubble label -> reduction slot (-1: FP64 atomics)
   int lc = 0;                   // first multigrid level solved inside the one-workgroup kernel
   int mg_scaled = 0;            // this solve stores some multigrid level in FP16 (input scaled)

@@ -130,3 +130,22 @@ def cmp_close(name, a, b, rtol):
     e = rel_err(a, b)
     log("  close %-14s n=%-7d max|d|/max|ref|=%.3e (tol %.1e)" % (name, np.asarray(a).size, e, rtol))
     return e <= rtol
+
+
+def tanque_com_bolhas(orc, n=12, mode=0):
+    """A 120x80 tank of water (up to 0.62 m) with n enclosed round air holes of radius 2.5 cm
+    on a grid, more bubbles than the solver's per-block slots."""
+    h = 0.01
+    cena = orc.Cena(120, 80, h)
+    sim = orc.Simulacao(cena, mode=mode)
+    m = MOLDURA * h
+    agua = orc.semear_retangulo(m, m, 1.2 - m, 0.62, h, (0, 0, 0), orc.Semente(13)).reshape(-1, 12)
+    x, y = agua[:, 0].astype(np.float64), agua[:, 1].astype(np.float64)
+    keep = np.ones(len(agua), bool)
+    cols = 6
+    for k in range(n):
+        cx = 0.12 + 0.19 * (k % cols)
+        cy = 0.14 + 0.2 * (k // cols)
+        keep &= ~((x - cx) ** 2 + (y - cy) ** 2 < 0.025 ** 2)
+    sim.adicionar(np.ascontiguousarray(agua[keep]).reshape(-1), False)
+    return sim

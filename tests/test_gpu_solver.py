@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from gpu_common import (ampulheta, clone_state, dam, hip_sim_like, log, oracle_fp64_clone, osim_snapshot, rel_err,
-                        substep_dt, tanque_com_bolha)
+                        substep_dt, tanque_com_bolha, tanque_com_bolhas)
 
 pytestmark = pytest.mark.gpu
 
@@ -43,6 +43,8 @@ def prepared(orc, build, frames):
     ("dam160", lambda o: dam(o, nx=160, ny=90, h=0.8 / 90, seed=0), 12),
     # 256x144: level 1 (128x72) is large enough for FP16 storage in the mixed preconditioner
     ("dam256", lambda o: dam(o, nx=256, ny=144, h=1.0 / 144, seed=0), 8),
+    # twelve bubbles: more than the per-block reduction slots, the rest through FP64 atomics
+    ("bolhas12", lambda o: tanque_com_bolhas(o), 2),
 ])
 def test_solve_to_tolerance(oracle, tq, name, build, frames):
     osim = prepared(oracle, build, frames)
