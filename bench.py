@@ -4,12 +4,14 @@
 Workload: the Ultra sealed hourglass (896x504 cells, 1.59 mm, ~0.75 M particles after the
 wall check), frames of 1/480 s split into substeps by the reference's adaptive bound, the
 pressure solved by MGPCG to a relative residual of 1e-6.  One bench step = one frame (air
-update + its substeps).  The reported value is the mixed-precision run (multigrid vectors stored
-in FP16, computed in FP32, FP32 PCG, FP64 dots); the FP32 run replays the same substep schedule.
+update + its substeps).  The reported value is the FP32 run (FP32 PCG and multigrid, FP64 dots
+and bubble unknowns), the library's default and fastest mode; the mixed-precision run (multigrid
+vectors stored in FP16) replays the same substep schedule and is reported beside it with the
+ratio the metric names.
 
 Timed runs use eager launches with HIP-event timing of the dominant kernel on the library's
-stream (the kernel with the largest share of device time in a short untimed probe of the same
-workload); the same runs are repeated with the PCG iterations replayed from a captured HIP graph
+stream (the launch with the largest share of device time in a short untimed probe of the same
+workload); the same runs are repeated with the PCG iterations replayed from captured HIP graphs
 and reported beside them.  A bounded sample of the same workload on the oracle (single-threaded
 C++ restatement of the reference CPU engine) is the CPU baseline.
 
@@ -142,16 +144,16 @@ def copy_calibration(nbytes=256 << 20):
     log("copy calibration: %d bytes device to device" % nbytes)
 
 
-def traffic_for(kernel_symbol_hint):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy of known size), when it covers this kernel."""
+def profile_for(name):
+    """The committed rocprofv3 figures of one timing name (profiles/traffic_ultra_hourglass.json):
+    HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated on a copy of known size) and the
+    kernel-trace average duration of the same launches."""
     try:
         with open(TRAFFIC_FILE) as f:
             d = json.load(f)
     except (OSError, ValueError):
         return None
-    k = d.get("kernels", {}).get(kernel_symbol_hint)
-    return k.get("hbm_bytes_per_launch") if k else None
+    return d.get("kernels", {}).get(name)
 
 
 def main():
@@ -161,7 +163,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10, help="untimed frames")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision runs")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay runs")
     ap.add_argument("--no-variants", action="store_true", help="skip the partial-FP16 variants")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -181,73 +183,82 @@ def main():
     work = scenes.ultra_hourglass() if (args.nx, args.ny) == (896, 504) else scenes.scaled_hourglass(args.nx, args.ny)
     log("workload %s: %dx%d cells, %d particles seeded" % (work.name, work.nx, work.ny, len(work.particles) // 12))
 
-    dom, probe = dominant_kernel(tq, work, tq.PREC_MIXED)
-    log("dominant kernel (mixed): %s (%.1f%% of device time in the probe)" % (
+    dom, probe = dominant_kernel(tq, work, tq.PREC_FP32)
+    log("dominant kernel (fp32): %s (%.1f%% of device time in the probe)" % (
         dom, 100 * probe[dom]["ms"] / sum(v["ms"] for v in probe.values())))
 
-    mixed = run_gpu(tq, work, tq.PREC_MIXED, args.warmup, args.steps, time_kernel=dom)
-    log("mixed: %.1f substeps/s, %.2f PCG it/solve, last rel resid %.2e (true %.2e)" % (
-        mixed["substeps_per_s"], mixed["pcg_iters_per_solve"], mixed["last_rel_residual"],
-        mixed["last_true_rel_residual"]))
-    sched = mixed["schedule"]
-    fp32 = None
-    if not args.no_fp32:
-        fp32 = run_gpu(tq, work, tq.PREC_FP32, args.warmup, args.steps, schedule=sched, time_kernel=dom)
-        log("fp32:  %.1f substeps/s, %.2f PCG it/solve, last rel resid %.2e" % (
-            fp32["substeps_per_s"], fp32["pcg_iters_per_solve"], fp32["last_rel_residual"]))
+    fp32 = run_gpu(tq, work, tq.PREC_FP32, args.warmup, args.steps)
+    log("fp32: %.1f substeps/s, %.2f PCG it/solve, last rel resid %.2e (true %.2e)" % (
+        fp32["substeps_per_s"], fp32["pcg_iters_per_solve"], fp32["last_rel_residual"],
+        fp32["last_true_rel_residual"]))
+    sched = fp32["schedule"]
+    # the same schedule again with the dominant kernel's launches bracketed by HIP events
+    timed = run_gpu(tq, work, tq.PREC_FP32, args.warmup, args.steps, schedule=sched, time_kernel=dom)
+    log("fp32 with %s timed: %.1f substeps/s" % (dom, timed["substeps_per_s"]))
+    mixed = None
+    if not args.no_mixed:
+        mixed = run_gpu(tq, work, tq.PREC_MIXED, args.warmup, args.steps, schedule=sched)
+        log("mixed: %.1f substeps/s, %.2f PCG it/solve, last rel resid %.2e" % (
+            mixed["substeps_per_s"], mixed["pcg_iters_per_solve"], mixed["last_rel_residual"]))
     graph = {}
     if not args.no_graph:
-        for lab, prec in (("mixed", tq.PREC_MIXED), ("fp32", tq.PREC_FP32)):
-            if lab == "fp32" and args.no_fp32:
+        for lab, prec in (("fp32", tq.PREC_FP32), ("mixed", tq.PREC_MIXED)):
+            if lab == "mixed" and args.no_mixed:
                 continue
             r = run_gpu(tq, work, prec, args.warmup, args.steps, schedule=sched, use_graph=True)
             graph[lab] = {"substeps_per_s": round(r["substeps_per_s"], 2),
                           "pcg_iters_per_solve": round(r["pcg_iters_per_solve"], 3)}
             log("graph %s: %.1f substeps/s" % (lab, r["substeps_per_s"]))
     variants = {}
-    if not args.no_variants:
+    if not args.no_variants and not args.no_mixed:
         for lab, prec, hl in (("fp16_b_only", tq.PREC_MIXED_B, -1), ("fp16_x_only_from_level2", tq.PREC_MIXED_X, 2)):
-            r = run_gpu(tq, work, prec, args.warmup, args.steps, schedule=sched, use_graph=True, mg_half_level=hl)
+            r = run_gpu(tq, work, prec, args.warmup, args.steps, schedule=sched, mg_half_level=hl)
             variants[lab] = {"substeps_per_s": round(r["substeps_per_s"], 2),
                              "pcg_iters_per_solve": round(r["pcg_iters_per_solve"], 3)}
             log("variant %s: %.1f substeps/s, %.2f it/solve" % (lab, r["substeps_per_s"], r["pcg_iters_per_solve"]))
 
-    kt = mixed["timing"].get(dom)
+    kt = timed["timing"].get(dom)
     roofline = None
     if kt and kt["launches"]:
         avg_s = kt["ms"] / kt["launches"] * 1e-3
         achieved = kt["bytes_per_launch"] / avg_s / 1e9
+        prof = profile_for(dom) or {}
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
-                    "traffic": traffic_for(dom), "avg_launch_us": round(avg_s * 1e6, 3),
-                    "algorithmic_bytes_per_launch": kt["bytes_per_launch"], "launches": kt["launches"]}
+                    "traffic": prof.get("hbm_bytes_per_launch"), "avg_launch_us": round(avg_s * 1e6, 3),
+                    "algorithmic_bytes_per_launch": kt["bytes_per_launch"], "launches": kt["launches"],
+                    "rocprof_avg_us": prof.get("rocprof_avg_us"),
+                    "frac_at_rocprof_duration": (round(kt["bytes_per_launch"] / (prof["rocprof_avg_us"] * 1e-6) / 1e9
+                                                       / HBM_PEAK_GBS, 6) if prof.get("rocprof_avg_us") else None)}
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(work, args.cpu_budget)
         log("cpu baseline: %.3f substeps/s (%s)" % (cpu["value"], cpu["sample"]))
 
-    ms_per_step = mixed["seconds"] / args.steps * 1e3
+    ms_per_step = fp32["seconds"] / args.steps * 1e3
     line = {
         "metric": "substeps/s @~450k cells, mixed vs FP32 MGPCG at 1e-6 resid; % HBM peak",
-        "value": round(mixed["substeps_per_s"], 2),
+        "value": round(fp32["substeps_per_s"], 2),
         "unit": "substeps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (MG storage f16, dots f64)",
+        "dtype": "f32 (PCG and multigrid f32; dots, norms and bubble unknowns f64)",
         "data": "synthetic",
         "config": {"workload": "%s: %s, frames of %.6f s, PCG to rel. residual 1e-6" % (
             work.name, work.description, work.dt_frame),
-            "cells": work.nx * work.ny, "particles": mixed["particles_start"],
-            "substeps_timed": mixed["substeps"], "pcg_iters_per_solve": round(mixed["pcg_iters_per_solve"], 3),
-            "last_rel_residual": mixed["last_rel_residual"], "last_true_rel_residual": mixed["last_true_rel_residual"],
-            "active_bubbles_end": mixed["active_bubbles"], "mg_levels": mixed["levels"],
-            "launches": "eager, dominant kernel timed with HIP events"},
-        "particle_steps_per_s": round(mixed["particle_steps_per_s"], 1),
-        "fp32_substeps_per_s": round(fp32["substeps_per_s"], 2) if fp32 else None,
-        "fp32_pcg_iters_per_solve": round(fp32["pcg_iters_per_solve"], 3) if fp32 else None,
-        "mixed_over_fp32": round(mixed["substeps_per_s"] / fp32["substeps_per_s"], 4) if fp32 else None,
+            "precision": "fp32 (value); mixed reported beside it",
+            "cells": work.nx * work.ny, "particles": fp32["particles_start"],
+            "substeps_timed": fp32["substeps"], "pcg_iters_per_solve": round(fp32["pcg_iters_per_solve"], 3),
+            "last_rel_residual": fp32["last_rel_residual"], "last_true_rel_residual": fp32["last_true_rel_residual"],
+            "active_bubbles_end": fp32["active_bubbles"], "mg_levels": fp32["levels"],
+            "launches": "eager; a second run of the same schedule times the dominant kernel with HIP events"},
+        "particle_steps_per_s": round(fp32["particle_steps_per_s"], 1),
+        "timed_run_substeps_per_s": round(timed["substeps_per_s"], 2),
+        "mixed_substeps_per_s": round(mixed["substeps_per_s"], 2) if mixed else None,
+        "mixed_pcg_iters_per_solve": round(mixed["pcg_iters_per_solve"], 3) if mixed else None,
+        "mixed_over_fp32": round(mixed["substeps_per_s"] / fp32["substeps_per_s"], 4) if mixed else None,
         "graph_replay": graph or None,
         "mixed_variants": variants or None,
         "roofline": roofline,

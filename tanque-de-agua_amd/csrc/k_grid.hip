@@ -24,7 +24,9 @@ __global__ __launch_bounds__(SF_THREADS) void k_surface(const uint8_t* __restric
   __shared__ float f[SF_N], t[SF_N], ga[SF_N], gb[SF_N];
   __shared__ uint8_t ty[SF_N], fl[SF_N];
   const int tid = threadIdx.x;
-  const int I0 = blockIdx.x * SF_T - SF_H, J0 = blockIdx.y * SF_T - SF_H;
+  int bxl, byl;
+  xcd_block2(&bxl, &byl);
+  const int I0 = bxl * SF_T - SF_H, J0 = byl * SF_T - SF_H;
   // this thread's region cells, fixed for every pass
   int lxk[SF_K], lyk[SF_K];
 #pragma unroll

@@ -426,7 +426,8 @@ __global__ __launch_bounds__(256) void k_g2p(float2* __restrict__ pos, float2* _
                                              int ny, double h, double dt, double decai, double mistura,
                                              unsigned long long* __restrict__ vmax_bits) {
 #pragma clang fp contract(off)
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  // particles are in cell order: consecutive blocks on one XCD gather from one band of rows
+  const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   double maior = 0;
   if (k < scr->n_slots) {
     const float2 P = pos[k];

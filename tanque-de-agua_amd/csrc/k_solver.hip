@@ -741,8 +741,10 @@ __device__ inline void store_cells(const Cells4& c, SX* __restrict__ dst, int ox
 // tile of wave w of block (bx, by): 2x2 tiles per 256-thread block
 __device__ inline void block_tile(int* tx, int* ty) {
   const int w = threadIdx.x >> 6;
-  *tx = 2 * blockIdx.x + (w & 1);
-  *ty = 2 * blockIdx.y + (w >> 1);
+  int bx, by;
+  xcd_block2(&bx, &by);
+  *tx = 2 * bx + (w & 1);
+  *ty = 2 * by + (w >> 1);
 }
 
 // ================================================================== level 0 input + first pass
@@ -1047,7 +1049,9 @@ __global__ __launch_bounds__(64 * BRA_WAVES) void k_bra(const SXF* __restrict__ 
   __shared__ float R[34 * 34];
   __shared__ float Bt[256];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int CI0 = blockIdx.x * 16, CJ0 = blockIdx.y * 16;
+  int bxl, byl;
+  xcd_block2(&bxl, &byl);
+  const int CI0 = bxl * 16, CJ0 = byl * 16;
   const int wx0 = 2 * CI0 - 8, wy0 = 2 * CJ0 - 8;   // window origin (fine)
   const int ox = wx0 + 16 * (wave % 3), oy = wy0 + 16 * (wave / 3);  // this wave's B tile
   float* T = Tt[wave];
@@ -1186,7 +1190,9 @@ __global__ __launch_bounds__(256) void k_pcd(const SX* __restrict__ xb, SX* __re
   __shared__ float W[CW * CW];
   __shared__ double sh[BSLOTS * 16];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int ax = blockIdx.x * 16, ay = blockIdx.y * 16;     // tile T
+  int bxl, byl;
+  xcd_block2(&bxl, &byl);
+  const int ax = bxl * 16, ay = byl * 16;  // tile T
   const int wx0 = ax - 8, wy0 = ay - 8;                      // window origin
   const int ox = wx0 + 16 * (wave & 1), oy = wy0 + 16 * (wave >> 1);  // this wave's C tile
   float* T = Tt[wave];
@@ -1712,7 +1718,7 @@ __global__ __launch_bounds__(256) void k_spmv(const uint8_t* __restrict__ opMask
   float zv[SPMV_CELLS][5], pv[SPMV_CELLS][5];
 #pragma unroll
   for (int q = 0; q < SPMV_CELLS; q++) {
-    const int c = blockIdx.x * (256 * SPMV_CELLS) + q * 256 + threadIdx.x;
+    const int c = xcd_block(blockIdx.x, gridDim.x) * (256 * SPMV_CELLS) + q * 256 + threadIdx.x;
     const int cc = c < C ? c : C - 1;
     const int nb0[5] = {cc, cc - 1, cc + 1, cc - nx, cc + nx};
 #pragma unroll
@@ -1744,7 +1750,7 @@ __global__ __launch_bounds__(256) void k_spmv(const uint8_t* __restrict__ opMask
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int q = 0; q < SPMV_CELLS; q++) {
-    const int c = blockIdx.x * (256 * SPMV_CELLS) + q * 256 + threadIdx.x;
+    const int c = xcd_block(blockIdx.x, gridDim.x) * (256 * SPMV_CELLS) + q * 256 + threadIdx.x;
     if (c < C) {
       const uint8_t m = mk[q][0];
       const float pc = peff(c, m, zv[q][0], pv[q][0]);
@@ -2061,7 +2067,10 @@ static void launch_bra(tq_sim* s, int l) {
   const int fold = (l == 0 && s->it_k >= 0) ? 1 : 0;
   const double* rr = s->partials + ps_off(PS_RR);
   // x(l) and b(l) and mask(l) in, x_B(l) out; coarse mask in, b (and x) out
-  int t = time_begin(s, l == 0 ? "mg_down_l0" : "mg_down",
+  static const char* const names[12] = {"mg_down_l0", "mg_down_l1", "mg_down_l2", "mg_down_l3",
+                                        "mg_down_l4", "mg_down_l5", "mg_down_l6", "mg_down_l7",
+                                        "mg_down_l8", "mg_down_l9", "mg_down_l10", "mg_down_l11"};
+  int t = time_begin(s, names[l],
                      (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n +
                          (1 + vbytes(G.hb) + (pa ? vbytes(G.hx) : 0)) * (double)G.n);
   with_types(F, [&](auto fx, auto fb) {
@@ -2095,7 +2104,9 @@ static int launch_pcd(tq_sim* s, int l) {
   const ZArgs za = zargs(s);
   // x_B, b, mask in; x out (or z with r, z_old, opMask in); coarse x and mask in
   const double fine = zd ? (vbytes(F.hx) + vbytes(F.hb) + 1 + 1 + 4 + 4 + 4) : (2 * vbytes(F.hx) + vbytes(F.hb) + 1);
-  int t = time_begin(s, zd ? "mg_up_z_l0" : "mg_up", fine * F.n + (vbytes(G.hx) + 1) * (double)G.n);
+  static const char* const names[12] = {"mg_up_z_l0", "mg_up_l1", "mg_up_l2", "mg_up_l3", "mg_up_l4", "mg_up_l5",
+                                        "mg_up_l6", "mg_up_l7", "mg_up_l8", "mg_up_l9", "mg_up_l10", "mg_up_l11"};
+  int t = time_begin(s, names[l], fine * F.n + (vbytes(G.hx) + 1) * (double)G.n);
   with_types(F, [&](auto tx, auto tb) {
     TQ_TYPES(tx, tb);
     auto go = [&](auto tc) {

@@ -27,6 +27,22 @@ __device__ inline double wave_maxd(double v) {
   return v;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin by linear id, and each XCD has its own L2.
+// Blocks are renumbered so that XCD k runs one contiguous range of logical blocks: neighbouring
+// tiles, whose halo windows overlap, then mostly share an L2.  A bijection on [0, n) for any n.
+constexpr int NXCD = 8;
+__device__ inline int xcd_block(int id, int n) {
+  const int q = n / NXCD, r = n % NXCD;
+  const int x = id % NXCD, slot = id / NXCD;
+  return x * q + (x < r ? x : r) + slot;
+}
+__device__ inline void xcd_block2(int* bx, int* by) {
+  const int gx = gridDim.x;
+  const int l = xcd_block(blockIdx.x + blockIdx.y * gx, gx * gridDim.y);
+  *by = l / gx;
+  *bx = l - *by * gx;
+}
+
 // Block sum of NV doubles per thread; result valid in thread 0.  `sh` holds >= NV*16 doubles.
 template <int NV>
 __device__ inline void block_sum(double (&v)[NV], double* sh) {
