@@ -310,6 +310,37 @@ __device__ inline bool fold_update(const double* __restrict__ part, int nb, int 
   return stop;
 }
 
+// Large grids: nb per-block partials of nv quantities (and of the slotted bubbles) summed into
+// PRE_R partials over contiguous block ranges, so each block of the consumer re-sums only PRE_R
+// (the consumers' redundant sums would otherwise grow as the square of the block count).
+constexpr int PRE_R = 64;
+constexpr int FOLD_DIRECT = 512;  // consumers re-sum up to this many partials themselves
+__global__ __launch_bounds__(256) void k_prereduce(const double* __restrict__ src, int nb, int nv,
+                                                   const double* __restrict__ bsrc, double* __restrict__ dst,
+                                                   double* __restrict__ bdst, const DevScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double sh[(2 + BSLOTS) * 16];
+  const int per = (nb + PRE_R - 1) / PRE_R;
+  const int b0 = blockIdx.x * per, b1 = b0 + per < nb ? b0 + per : nb;
+  const int ns = bsrc ? nslots(sc->active_bubbles) : 0;
+  double v[2 + BSLOTS];
+#pragma unroll
+  for (int k = 0; k < 2 + BSLOTS; k++) v[k] = 0;
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+      if (k < nv) v[k] += src[ps_off(k) + b];
+#pragma unroll
+    for (int k = 0; k < BSLOTS; k++)
+      if (k < ns) v[2 + k] += bsrc[ps_off(k) + b];
+  }
+  block_sum<2 + BSLOTS>(v, sh);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < nv; k++) dst[ps_off(k) + blockIdx.x] = v[k];
+    for (int k = 0; k < ns; k++) bdst[ps_off(k) + blockIdx.x] = v[2 + k];
+  }
+}
+
 // ================================================================== setup (types and neighbour masks)
 // Level 0: operator mask (true types, bubble bits) and MG mask (air with a bubble label as water,
 // pressao-cpu.js:305-308; colour bit).  Levels 1..PYR_LEVELS: their type bits (tiposGrossos,
@@ -1023,6 +1054,157 @@ __global__ __launch_bounds__(256) void k_pass_z(const float* __restrict__ src, c
   }
   store_partials<2>(dv, 2, za.partials, sh);
   if (ns > 0) store_partials<BSLOTS>(sl, ns, za.bpart, sh);
+}
+
+// ================================================================== large levels: one kernel per pass
+// Above SPLIT_CELLS cells the fused down/up kernels' recomputed halo tiles (9/4 of pass B, 4x
+// pass C) cost more than a kernel boundary, so those levels run each pass once: pass B, then
+// restriction + the coarse pass A; prolongation + pass C, then pass D.
+constexpr int SPLIT_CELLS = 1 << 20;
+
+// one smoothing pass of level l (offset desl, colour order from C0); on level 0 of an iteration
+// it first does the norm and stop test (fold_update)
+template <class SX, class SB, int C0>
+__global__ __launch_bounds__(256) void k_pass(const SX* __restrict__ src, SX* __restrict__ dst,
+                                              const SB* __restrict__ b, const uint8_t* __restrict__ mask, int nx,
+                                              int ny, int desl, int ntx, int nty, const double* __restrict__ partRR,
+                                              int nbu, int par, int fold, DevScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ float Tt[4][TCELLS];
+  const int lane = threadIdx.x & 63;
+  int tx, ty;
+  block_tile(&tx, &ty);
+  const bool valid = tx < ntx && ty < nty;
+  const int ox = tx * 16 - desl, oy = ty * 16 - desl;
+  float* T = Tt[threadIdx.x >> 6];
+  Own4 own;
+  if (valid) {
+    prefetch_cells<SB>(own, b, mask, ox, oy, nx, ny, lane);
+    fill_tile<SX>(T, src, ox, oy, nx, ny, lane);
+  }
+  if (fold) {
+    __shared__ double fsh[16], fres[1];
+    if (fold_update(partRR, nbu, par, fsh, fres, sc)) return;
+  }
+  if (!valid) return;
+  wave_sync();
+  Cells4 cc;
+  init_cells(cc, own, T, lane);
+  tile_sweeps<C0>(T, cc, lane);
+  store_cells<SX>(cc, dst, ox, oy, nx, ny, lane);
+}
+
+// restringir (pressao-cpu.js:159-188): the block owns a 16x16 coarse tile; fine residuals
+// r = b - A x_B on the 34x34 fine cells it gathers from (2I-1 .. 2I+2 per axis, weights
+// 1/4 3/4 3/4 1/4), from x_B on 36x36, in LDS.  Then wave 0 runs the coarse level's pass A
+// (offset 0 = this tile) from x = 0 unless the coarse level is solved otherwise.
+template <class SXF, class SBF, class SCB, class SCX, bool PASSA>
+__global__ __launch_bounds__(256) void k_restrict_a(const SXF* __restrict__ fx, const SBF* __restrict__ fb,
+                                                    const uint8_t* __restrict__ fm, int fnx, int fny,
+                                                    SCB* __restrict__ cb, SCX* __restrict__ cxo,
+                                                    const uint8_t* __restrict__ cmk, int cnx, int cny,
+                                                    const DevScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ float X[36 * 36], R[34 * 34], Bt[256], T[TCELLS];
+  const int tid = threadIdx.x;
+  int bxl, byl;
+  xcd_block2(&bxl, &byl);
+  const int CI0 = bxl * 16, CJ0 = byl * 16;
+  const int fx0 = 2 * CI0 - 2, fy0 = 2 * CJ0 - 2;  // X origin
+  // every global load first: the residual cells' masks and b, the coarse masks, then x_B
+  uint8_t mr[5];
+  float br[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const int e = tid + 256 * q;
+    const int ly = e / 34, lx = e - ly * 34;
+    const int i = fx0 + 1 + lx, j = fy0 + 1 + ly;
+    const bool in = e < 34 * 34 && i >= 0 && j >= 0 && i < fnx && j < fny;
+    const size_t c = in ? (size_t)i + (size_t)j * fnx : 0;
+    mr[q] = in ? fm[c] : 0;
+    br[q] = in ? ld<SBF>(fb, c) : 0.f;
+  }
+  const int Im = CI0 + (tid & 15), Jm = CJ0 + (tid >> 4);
+  const bool cin = Im < cnx && Jm < cny;
+  const size_t Cm = cin ? (size_t)Im + (size_t)Jm * cnx : 0;
+  const uint8_t cmine = cin ? cmk[Cm] : 0;
+  uint8_t cma[2][2];
+#pragma unroll
+  for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+    for (int rw = 0; rw < 2; rw++) {
+      const int I = CI0 + cell_lx(tid & 63, cq), J = CJ0 + cell_ly(tid & 63, rw);
+      cma[cq][rw] = (PASSA && tid < 64 && I < cnx && J < cny) ? cmk[(size_t)I + (size_t)J * cnx] : 0;
+    }
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const int e = tid + 256 * q;
+    if (e < 36 * 36) {
+      const int ly = e / 36, lx = e - ly * 36;
+      const int i = fx0 + lx, j = fy0 + ly;
+      X[e] = (i >= 0 && j >= 0 && i < fnx && j < fny) ? ld<SXF>(fx, (size_t)i + (size_t)j * fnx) : 0.f;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const int e = tid + 256 * q;
+    if (e < 34 * 34) {
+      const int ly = e / 34, lx = e - ly * 34;
+      const uint8_t m = mr[q];
+      float rr = 0.f;
+      if (mask_type(m) == AGUA) {
+        const int ex = (lx + 1) + (ly + 1) * 36;
+        float sn = (m & 1) ? X[ex - 1] : 0.f;
+        sn += (m & 2) ? X[ex + 1] : 0.f;
+        sn += (m & 4) ? X[ex - 36] : 0.f;
+        sn += (m & 8) ? X[ex + 36] : 0.f;
+        rr = br[q] - ((float)mask_cnt(m) * X[ex] - sn);
+      }
+      R[e] = rr;
+    }
+  }
+  __syncthreads();
+  {
+    float bc = 0.f;
+    if (cin) {
+      if (mask_type(cmine) == AGUA) {
+        const float w[4] = {0.25f, 0.75f, 0.75f, 0.25f};
+        // fine cell 2I-1+bi sits at R index (2I-1+bi) - (fx0+1) = 2(I-CI0) + bi
+        const int rx = 2 * (tid & 15), ry = 2 * (tid >> 4);
+        float acc = 0.f;
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++) {
+          float racc = 0.f;
+#pragma unroll
+          for (int bi = 0; bi < 4; bi++) racc += R[(rx + bi) + (ry + bj) * 34] * w[bi];
+          acc += racc * w[bj];
+        }
+        bc = rnd<SCB>(acc);
+      }
+      st<SCB>(cb, Cm, bc);
+    }
+    Bt[tid] = bc;
+  }
+  if (!PASSA) return;
+  __syncthreads();
+  if (tid >= 64) return;
+  const int lane = tid;
+  zero_tile(T, lane);
+  Cells4 cc;
+#pragma unroll
+  for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+    for (int rw = 0; rw < 2; rw++) {
+      const int li = cell_lx(lane, cq), lj = cell_ly(lane, rw);
+      const int I = CI0 + li, J = CJ0 + lj;
+      cc.v[cq][rw] = 0.f;
+      if (I < cnx && J < cny) cell_setup(cc, cq, rw, cma[cq][rw], Bt[li + 16 * lj]);
+      else cell_clear(cc, cq, rw);
+    }
+  wave_sync();
+  tile_sweeps<0>(T, cc, lane);
+  store_cells<SCX>(cc, cxo, CI0, CJ0, cnx, cny, lane);
 }
 
 // ================================================================== down: pass B + restriction + pass A
@@ -1940,6 +2122,7 @@ int solver_alloc(tq_sim* s) {
     }
   }
   TQ_HIP(hipMalloc(&s->bpart, sizeof(double) * 2 * BSLOTS * MAX_RED_BLOCKS));  // spmv's, then the z dots'
+  TQ_HIP(hipMalloc(&s->bpart_r, sizeof(double) * 2 * BSLOTS * MAX_RED_BLOCKS));
   TQ_HIP(hipMalloc(&s->labelSlot, sizeof(int) * MAX_BOLHAS));
   return 0;
 }
@@ -1957,6 +2140,8 @@ void solver_free(tq_sim* s) {
   }
   s->lv.clear();
   if (s->bpart) (void)hipFree(s->bpart);
+  if (s->bpart_r) (void)hipFree(s->bpart_r);
+  s->bpart_r = nullptr;
   if (s->labelSlot) (void)hipFree(s->labelSlot);
   if (s->dense) (void)hipFree(s->dense);
   s->dense = nullptr;
@@ -2014,6 +2199,25 @@ static BubArgs bubargs(tq_sim* s) {
 static int it_par(const tq_sim* s) { return s->it_k & 1; }
 static int z_par(const tq_sim* s) { return (s->it_k + 1) & 1; }
 
+// The partials a folding consumer reads: the producer's own, or (many blocks) PRE_R sums of them
+struct RedSrc {
+  const double* part;
+  const double* bpart;
+  int nb;
+};
+static RedSrc red_src(tq_sim* s, int slot, int nv, int bhalf, int nb) {
+  const double* src = s->partials + ps_off(slot);
+  const double* bsrc = bhalf < 0 ? nullptr : s->bpart + (size_t)bhalf * BSLOTS * MAX_RED_BLOCKS;
+  if (nb <= FOLD_DIRECT) return {src, bsrc, nb};
+  double* dst = s->partials_r + ps_off(slot);
+  double* bdst = bhalf < 0 ? nullptr : s->bpart_r + (size_t)bhalf * BSLOTS * MAX_RED_BLOCKS;
+  int t = time_begin(s, "pcg_prereduce", 8.0 * nv * nb);
+  hipLaunchKernelGGL(k_prereduce, dim3(PRE_R), dim3(256), 0, s->st, src, nb, nv, bsrc, dst, bdst,
+                     (const DevScalars*)s->sc);
+  time_end(s, t);
+  return {dst, bdst, PRE_R};
+}
+
 static void launch_in_a0(tq_sim* s, bool upd) {
   Level& F = s->lv[0];
   const int ntx = tiles_of(F.nx, 0), nty = tiles_of(F.ny, 0);
@@ -2022,6 +2226,7 @@ static void launch_in_a0(tq_sim* s, bool upd) {
   const float* pp = s->pbuf[s->pc];
   // bytes: opMask, mg0, b0 write, x0 write; update: q, p, r, Ap read + q, r write
   const double by = (2 + vbytes(F.hb) + (pa ? 4 : 0) + (upd ? 24 : 4)) * (double)F.n;
+  const RedSrc rs = upd ? red_src(s, PS_SPMV, 1, 0, s->nb_spmv) : RedSrc{s->partials + ps_off(PS_SPMV), bpart_a(s), 0};
   int t = time_begin(s, upd ? "pcg_update_mg_in_l0" : "pcg_mg_in_l0", by);
   // the first cycle reads the bubble residuals of parity 1 (par 0 selects them)
   const int par = upd ? it_par(s) : 0;
@@ -2029,8 +2234,7 @@ static void launch_in_a0(tq_sim* s, bool upd) {
   const float* mgs = &s->sc->mgs[it_par(s)];
 #define TQ_IN_A0(SB, U, P)                                                                                        \
   hipLaunchKernelGGL((k_in_a0<SB, U, P>), g, dim3(256), 0, s->st, s->opMask, F.mask, s->rot, s->cellsB, pp, s->ap, \
-                     s->q, s->r, F.nx, F.ny, ntx, nty, B, (const double*)(s->partials + ps_off(PS_SPMV)),        \
-                     (const double*)bpart_a(s), s->nb_spmv, par, (SB*)F.b, (float*)F.x,                          \
+                     s->q, s->r, F.nx, F.ny, ntx, nty, B, rs.part, rs.bpart, rs.nb, par, (SB*)F.b, (float*)F.x,  \
                      s->partials + ps_off(PS_RR), s->mg_scaled, mgs, s->sc)
   if (F.hb) {
     if (upd) { if (pa) TQ_IN_A0(__half, true, true); else TQ_IN_A0(__half, true, false); }
@@ -2065,8 +2269,44 @@ static void launch_bra(tq_sim* s, int l) {
   const dim3 g((G.nx + 15) / 16, (G.ny + 15) / 16);
   // level 0 of an iteration (not the first cycle) starts with the norm and stop test
   const int fold = (l == 0 && s->it_k >= 0) ? 1 : 0;
-  const double* rr = s->partials + ps_off(PS_RR);
+  const RedSrc rsu = fold ? red_src(s, PS_RR, 1, -1, s->nb_upd) : RedSrc{s->partials + ps_off(PS_RR), nullptr, 0};
+  const double* rr = rsu.part;
   // x(l) and b(l) and mask(l) in, x_B(l) out; coarse mask in, b (and x) out
+  if (F.n > SPLIT_CELLS) {  // pass B, then restriction + the coarse pass A
+    static const char* const pnames[12] = {"mg_pass_b_l0", "mg_pass_b_l1", "mg_pass_b_l2", "mg_pass_b_l3",
+                                           "mg_pass_b_l4", "mg_pass_b_l5", "mg_pass_b_l6", "mg_pass_b_l7",
+                                           "mg_pass_b_l8", "mg_pass_b_l9", "mg_pass_b_l10", "mg_pass_b_l11"};
+    static const char* const rnames[12] = {"mg_restrict_l0", "mg_restrict_l1", "mg_restrict_l2", "mg_restrict_l3",
+                                           "mg_restrict_l4", "mg_restrict_l5", "mg_restrict_l6", "mg_restrict_l7",
+                                           "mg_restrict_l8", "mg_restrict_l9", "mg_restrict_l10", "mg_restrict_l11"};
+    const int ntx8 = tiles_of(F.nx, 8), nty8 = tiles_of(F.ny, 8);
+    int t = time_begin(s, pnames[l], (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n);
+    with_types(F, [&](auto tx, auto tb) {
+      TQ_TYPES(tx, tb);
+      hipLaunchKernelGGL((k_pass<SX, SB, 0>), tile_grid(F.nx, F.ny, 8), dim3(256), 0, s->st, (const SX*)F.x,
+                         (SX*)F.tmp, (const SB*)F.b, F.mask, F.nx, F.ny, 8, ntx8, nty8, rr, rsu.nb, it_par(s), fold,
+                         s->sc);
+    });
+    time_end(s, t);
+    t = time_begin(s, rnames[l], (vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n +
+                                     (1 + vbytes(G.hb) + (pa ? vbytes(G.hx) : 0)) * (double)G.n);
+    with_types(F, [&](auto fx, auto fb) {
+      using SXF = typename decltype(fx)::type;
+      using SBF = typename decltype(fb)::type;
+      with_types(G, [&](auto cx, auto cb) {
+        using SCX = typename decltype(cx)::type;
+        using SCB = typename decltype(cb)::type;
+        if (pa)
+          hipLaunchKernelGGL((k_restrict_a<SXF, SBF, SCB, SCX, true>), g, dim3(256), 0, s->st, (const SXF*)F.tmp,
+                             (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny, s->sc);
+        else
+          hipLaunchKernelGGL((k_restrict_a<SXF, SBF, SCB, SCX, false>), g, dim3(256), 0, s->st, (const SXF*)F.tmp,
+                             (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny, s->sc);
+      });
+    });
+    time_end(s, t);
+    return;
+  }
   static const char* const names[12] = {"mg_down_l0", "mg_down_l1", "mg_down_l2", "mg_down_l3",
                                         "mg_down_l4", "mg_down_l5", "mg_down_l6", "mg_down_l7",
                                         "mg_down_l8", "mg_down_l9", "mg_down_l10", "mg_down_l11"};
@@ -2082,7 +2322,7 @@ static void launch_bra(tq_sim* s, int l) {
       if (pa)
         hipLaunchKernelGGL((k_bra<SXF, SBF, SCB, SCX, true>), g, dim3(64 * BRA_WAVES), 0, s->st, (const SXF*)F.x,
                            (SXF*)F.tmp, (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny,
-                           rr, s->nb_upd, it_par(s), fold, s->sc);
+                           rr, rsu.nb, it_par(s), fold, s->sc);
       else
         hipLaunchKernelGGL((k_bra<SXF, SBF, SCB, SCX, false>), g, dim3(64 * BRA_WAVES), 0, s->st, (const SXF*)F.x,
                            (SXF*)F.tmp, (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny,
@@ -2106,6 +2346,41 @@ static int launch_pcd(tq_sim* s, int l) {
   const double fine = zd ? (vbytes(F.hx) + vbytes(F.hb) + 1 + 1 + 4 + 4 + 4) : (2 * vbytes(F.hx) + vbytes(F.hb) + 1);
   static const char* const names[12] = {"mg_up_z_l0", "mg_up_l1", "mg_up_l2", "mg_up_l3", "mg_up_l4", "mg_up_l5",
                                         "mg_up_l6", "mg_up_l7", "mg_up_l8", "mg_up_l9", "mg_up_l10", "mg_up_l11"};
+  if (!zd && F.n > SPLIT_CELLS) {  // prolongation + pass C (tmp -> x), then pass D (x -> tmp)
+    static const char* const cnames[12] = {"mg_prolong_c_l0", "mg_prolong_c_l1", "mg_prolong_c_l2",
+                                           "mg_prolong_c_l3", "mg_prolong_c_l4", "mg_prolong_c_l5",
+                                           "mg_prolong_c_l6", "mg_prolong_c_l7", "mg_prolong_c_l8",
+                                           "mg_prolong_c_l9", "mg_prolong_c_l10", "mg_prolong_c_l11"};
+    static const char* const dnames[12] = {"mg_pass_d_l0", "mg_pass_d_l1", "mg_pass_d_l2", "mg_pass_d_l3",
+                                           "mg_pass_d_l4", "mg_pass_d_l5", "mg_pass_d_l6", "mg_pass_d_l7",
+                                           "mg_pass_d_l8", "mg_pass_d_l9", "mg_pass_d_l10", "mg_pass_d_l11"};
+    const int ntx8 = tiles_of(F.nx, 8), nty8 = tiles_of(F.ny, 8);
+    const int ntx = tiles_of(F.nx, 0), nty = tiles_of(F.ny, 0);
+    int t = time_begin(s, cnames[l], (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n +
+                                         (vbytes(G.hx) + 1) * (double)G.n);
+    with_types(F, [&](auto tx, auto tb) {
+      TQ_TYPES(tx, tb);
+      if (G.hx)
+        hipLaunchKernelGGL((k_prolong_pass<SX, SB, __half>), tile_grid(F.nx, F.ny, 8), dim3(256), 0, s->st,
+                           (const SX*)F.tmp, (SX*)F.x, (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8,
+                           (const __half*)G.res, G.mask, G.nx, G.ny, s->sc);
+      else
+        hipLaunchKernelGGL((k_prolong_pass<SX, SB, float>), tile_grid(F.nx, F.ny, 8), dim3(256), 0, s->st,
+                           (const SX*)F.tmp, (SX*)F.x, (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8,
+                           (const float*)G.res, G.mask, G.nx, G.ny, s->sc);
+    });
+    time_end(s, t);
+    t = time_begin(s, dnames[l], (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n);
+    with_types(F, [&](auto tx, auto tb) {
+      TQ_TYPES(tx, tb);
+      hipLaunchKernelGGL((k_pass<SX, SB, 1>), tile_grid(F.nx, F.ny, 0), dim3(256), 0, s->st, (const SX*)F.x,
+                         (SX*)F.tmp, (const SB*)F.b, F.mask, F.nx, F.ny, 0, ntx, nty, (const double*)nullptr, 0, 0, 0,
+                         s->sc);
+    });
+    time_end(s, t);
+    F.res = F.tmp;
+    return 0;
+  }
   int t = time_begin(s, names[l], fine * F.n + (vbytes(G.hx) + 1) * (double)G.n);
   with_types(F, [&](auto tx, auto tb) {
     TQ_TYPES(tx, tb);
@@ -2113,14 +2388,15 @@ static int launch_pcd(tq_sim* s, int l) {
       using SC = typename decltype(tc)::type;
       if (zd)
         hipLaunchKernelGGL((k_pcd<SX, SB, SC, true>), g, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
-                           (const SB*)F.b, F.mask, F.nx, F.ny, (const SC*)G.x, G.mask, G.nx, G.ny, za, s->sc);
+                           (const SB*)F.b, F.mask, F.nx, F.ny, (const SC*)G.res, G.mask, G.nx, G.ny, za, s->sc);
       else
         hipLaunchKernelGGL((k_pcd<SX, SB, SC, false>), g, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
-                           (const SB*)F.b, F.mask, F.nx, F.ny, (const SC*)G.x, G.mask, G.nx, G.ny, za, s->sc);
+                           (const SB*)F.b, F.mask, F.nx, F.ny, (const SC*)G.res, G.mask, G.nx, G.ny, za, s->sc);
     };
     if (G.hx) go(Tag<__half>{}); else go(Tag<float>{});
   });
   time_end(s, t);
+  F.res = F.x;
   return (int)(g.x * g.y);
 }
 
@@ -2202,11 +2478,11 @@ static int launch_up0(tq_sim* s) {
     TQ_TYPES(tx, tb);
     if (G.hx)
       hipLaunchKernelGGL((k_prolong_pass<SX, SB, __half>), g8, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
-                         (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8, (const __half*)G.x, G.mask, G.nx, G.ny,
+                         (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8, (const __half*)G.res, G.mask, G.nx, G.ny,
                          s->sc);
     else
       hipLaunchKernelGGL((k_prolong_pass<SX, SB, float>), g8, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
-                         (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8, (const float*)G.x, G.mask, G.nx, G.ny,
+                         (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8, (const float*)G.res, G.mask, G.nx, G.ny,
                          s->sc);
   });
   time_end(s, t);
@@ -2235,6 +2511,7 @@ static void precondition_rest(tq_sim* s) {
   }
   for (int l = 0; l < lc; l++) launch_bra(s, l);
   launch_coarse_wg(s);
+  s->lv[lc].res = s->lv[lc].x;
   if (lc > 0) {
     for (int l = lc - 1; l >= 1; l--) launch_pcd(s, l);
     s->nb_z = launch_up0(s);
@@ -2253,11 +2530,11 @@ static void pcg_iteration(tq_sim* s) {
   const int C = s->C;
   const int nbs = (C + 256 * SPMV_CELLS - 1) / (256 * SPMV_CELLS);
   const int pn = 1 - s->pc;
+  const RedSrc rz = red_src(s, PS_Z, 2, 1, s->nb_z);
   int t = time_begin(s, "pcg_spmv", (1 + 4 + 4 + 4 + 4) * (double)C);
   hipLaunchKernelGGL(k_spmv, dim3(nbs), dim3(256), 0, s->st, s->opMask, s->rot, s->zbuf[s->zc], s->pbuf[s->pc],
-                     s->pbuf[pn], s->ap, s->nx, C, bubargs(s), (const double*)(s->partials + ps_off(PS_Z)),
-                     (const double*)bpart_z(s), s->nb_z, it_par(s), s->it_k == 0 ? 1 : 0,
-                     s->partials + ps_off(PS_SPMV), bpart_a(s), s->sc);
+                     s->pbuf[pn], s->ap, s->nx, C, bubargs(s), rz.part, rz.bpart, rz.nb, it_par(s),
+                     s->it_k == 0 ? 1 : 0, s->partials + ps_off(PS_SPMV), bpart_a(s), s->sc);
   time_end(s, t);
   s->nb_spmv = nbs;
   s->pc = pn;

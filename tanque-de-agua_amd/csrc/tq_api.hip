@@ -120,6 +120,7 @@ static int alloc_all(tq_sim* s) {
   DALLOC(s->opMask, C); DALLOC(s->r, C); DALLOC(s->ap, C);
   for (int b = 0; b < 2; b++) { DALLOC(s->zbuf[b], C); DALLOC(s->pbuf[b], C); }
   DALLOC(s->partials, RED_SLOTS * (size_t)MAX_RED_BLOCKS);
+  DALLOC(s->partials_r, RED_SLOTS * (size_t)MAX_RED_BLOCKS);
   DALLOC(s->sc, 1);
   {
     // block sums of the largest scan (particle slots, binning keys or cells), two levels
@@ -164,7 +165,7 @@ static void free_all(tq_sim* s) {
   f(s->cellsB); f(s->rhsB); f(s->rhsB32); f(s->yB32); f(s->volAlvo); f(s->vivo); f(s->vivoNovo); f(s->bvec);
   f(s->activeList); f(s->opMask); f(s->r); f(s->ap);
   for (int b = 0; b < 2; b++) { f(s->zbuf[b]); f(s->pbuf[b]); }
-  f(s->partials); f(s->sc); f(s->scanBlocks);
+  f(s->partials); f(s->partials_r); f(s->sc); f(s->scanBlocks);
   if (s->sc_host) (void)hipHostFree(s->sc_host);
   solver_free(s);
   for (auto& e : s->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }

@@ -102,6 +102,7 @@ struct Level {
   void* b = nullptr;
   void* x = nullptr;
   void* tmp = nullptr;
+  void* res = nullptr;          // where this level's V-cycle result is (x, or tmp after a two-kernel up path)
   bool hx = false, hb = false;  // FP16 storage of x/tmp and of b on this level (this solve)
 };
 
@@ -193,6 +194,8 @@ struct tq_sim {
   int zc = 0, pc = 0;
   double* partials = nullptr;   // per-block partial sums: RED_SLOTS x MAX_RED_BLOCKS
   double* bpart = nullptr;      // per-block sums of the slotted bubbles: 2 x BSLOTS x MAX_RED_BLOCKS
+  double* partials_r = nullptr; // the same layouts, pre-reduced to a few partials for large grids
+  double* bpart_r = nullptr;
   int it_k = -1;                // index of the PCG iteration being launched (-1: the initial V-cycle)
   int nb_spmv = 0, nb_upd = 0, nb_z = 0;  // partial counts of the last spmv, update and z launches
   int* labelSlot = nullptr;     // bubble label -> reduction slot (-1: FP64 atomics)

@@ -45,6 +45,8 @@ def prepared(orc, build, frames):
     ("dam256", lambda o: dam(o, nx=256, ny=144, h=1.0 / 144, seed=0), 8),
     # twelve bubbles: more than the per-block reduction slots, the rest through FP64 atomics
     ("bolhas12", lambda o: tanque_com_bolhas(o), 2),
+    # 1024x640 (655k cells): more than 512 blocks per reduction, so the partials are pre-reduced
+    ("dam1024", lambda o: dam(o, nx=1024, ny=640, h=0.8 / 640, seed=0), 1),
 ])
 def test_solve_to_tolerance(oracle, tq, name, build, frames):
     osim = prepared(oracle, build, frames)
@@ -113,6 +115,7 @@ def test_mixed_matches_fp32_over_steps(tq, oracle):
 @pytest.mark.parametrize("name,build", [
     ("bolha", lambda o: tanque_com_bolha(o)),
     ("dam160", lambda o: dam(o, nx=160, ny=90, h=0.8 / 90, seed=0)),
+    ("dam1024", lambda o: dam(o, nx=1024, ny=640, h=0.8 / 640, seed=0)),  # pre-reduced partials
 ])
 def test_graph_replay_matches_eager(tq, oracle, prec, name, build):
     """PCG iterations replayed from the captured HIP graph give bit-identical states to eager

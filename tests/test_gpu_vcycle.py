@@ -45,8 +45,11 @@ def state(orc, build, frames):
     ("bolha", lambda o: tanque_com_bolha(o), 6),                        # an enclosed bubble
     ("ampulheta", lambda o: ampulheta(o), 12),                          # walls, air pockets
     ("dam256", lambda o: dam(o, nx=256, ny=144, h=1.0 / 144, seed=0), 4),
-    # 1600x800: 8 levels, the last one past the 64x64-tile type pyramid
+    # 1600x800: 8 levels, the last one past the 64x64-tile type pyramid; level 0 over 1M cells
+    # runs its down passes as separate kernels
     ("dam1600", lambda o: dam(o, nx=1600, ny=800, h=1.0 / 800, seed=0), 0),
+    # 2400x1800: level 1 (1200x900) over 1M cells too, so its up passes are separate kernels
+    ("dam2400", lambda o: dam(o, nx=2400, ny=1800, h=1.8 / 1800, seed=0), 0),
 ])
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
 def test_vcycle_matches_oracle(oracle, tq, name, build, frames, prec):
