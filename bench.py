@@ -85,6 +85,42 @@ def run_gpu(tq, work, precision, warmup, steps, schedule=None, time_kernel=None,
     return res
 
 
+def run_alto(tq, work, precision, tol=1e-6, maxit=200, schedule=None):
+    """The Alto config end to end: each frame the faucet emits (while on) and the frame's
+    substeps run, as the app does (principal.js:301-321, motor.js:98-107)."""
+    from tanque_hip.scenes import passo_frame
+    f = work.extra["faucet"]
+    cena = work.cena()
+    sim = tq.SimulacaoHIP(cena, precision=precision, tol=tol, iteracoes=maxit)
+    sem = tq.Semente(f["seed"])
+    sim.sync()
+    st0 = sim.solver_stats()
+    t0 = time.perf_counter()
+    subs = 0
+    psteps = 0.0
+    emitted = 0
+    sched = []
+    for k in range(work.frames):
+        if k < f["frames"]:
+            n, _ = sim.emitirTorneira(f["x"], f["y"], f["pvx"], f["pvy"], f["raio"], f["absorcao"], work.dt_frame, sem)
+            emitted += n
+        sub = schedule[k] if schedule is not None else passo_frame(work.h, cena.altura, sim.vMax, work.dt_frame)
+        sched.append(sub)
+        n0 = sim.n
+        sim.avancar(work.dt_frame, sub)
+        subs += sub
+        psteps += sub * n0
+    sim.sync()
+    t1 = time.perf_counter()
+    st1 = sim.solver_stats()
+    res = {"substeps_per_s": round(subs / (t1 - t0), 2), "particle_steps_per_s": round(psteps / (t1 - t0), 1),
+           "frames": work.frames, "substeps": subs, "emitted": emitted, "particles_end": sim.n,
+           "pcg_iters_per_solve": round((st1["total_iterations"] - st0["total_iterations"]) /
+                                        max(1, st1["solves"] - st0["solves"]), 3)}
+    sim.close()
+    return res, sched
+
+
 def dominant_kernel(tq, work, precision, frames=3):
     """Name of the kernel family with the largest device time on a few frames of the workload."""
     from tanque_hip.scenes import passo_frame
@@ -166,6 +202,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision runs")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay runs")
     ap.add_argument("--no-variants", action="store_true", help="skip the partial-FP16 variants")
+    ap.add_argument("--no-alto", action="store_true", help="skip the Alto (cup + faucet) runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="start with a 256 MiB device copy (reference for rocprofv3 --pmc byte counters)")
     ap.add_argument("--nx", type=int, default=896)
@@ -231,6 +268,17 @@ def main():
                     "frac_at_rocprof_duration": (round(kt["bytes_per_launch"] / (prof["rocprof_avg_us"] * 1e-6) / 1e9
                                                        / HBM_PEAK_GBS, 6) if prof.get("rocprof_avg_us") else None)}
 
+    alto = None
+    if not args.no_alto:
+        wa = scenes.alto_cup()
+        r32, sched_a = run_alto(tq, wa, tq.PREC_FP32)
+        alto = {"workload": wa.description, "fp32": r32}
+        log("alto fp32: %s" % alto["fp32"])
+        if not args.no_mixed:
+            alto["mixed"], _ = run_alto(tq, wa, tq.PREC_MIXED, schedule=sched_a)
+            alto["mixed_over_fp32"] = round(alto["mixed"]["substeps_per_s"] / alto["fp32"]["substeps_per_s"], 4)
+            log("alto mixed: %s" % alto["mixed"])
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(work, args.cpu_budget)
@@ -261,6 +309,7 @@ def main():
         "mixed_over_fp32": round(mixed["substeps_per_s"] / fp32["substeps_per_s"], 4) if mixed else None,
         "graph_replay": graph or None,
         "mixed_variants": variants or None,
+        "alto": alto,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -140,6 +140,21 @@ int tq_run_phase(tq_sim* s, int phase, double dt);
  * water cells (bubble cells take the setup's zero bubble residual). */
 int tq_debug_vcycle(tq_sim* s, const float* b, float* z);
 
+/* The faucet tool (emitirTorneira, principal.js:301-321) with the seeded LCG of the reference's
+ * tests (testes/apoio.js:4-9) in place of Math.random: velocity v = (pvx/2, pvy/2 - 0.7) from
+ * the pointer's, clamped to 4 m/s; n = min(40000, round(2 raio max(0.7, |v|) dt / h^2 * 4))
+ * particles uniform in the disc of radius raio around (x, y) (two draws each: angle, then
+ * radius), with the dye absorption, added through adicionar's checks (fisica-cpu.js:92-111)
+ * under the particle cap.  *seed advances by 2n draws; *emitted = n, *accepted = particles kept
+ * (either may be null). */
+typedef struct tq_faucet {
+  double x, y;            /* pointer position (m) */
+  double pvx, pvy;        /* pointer velocity (m/s) */
+  double raio;            /* brush radius (m) */
+  double absorcao[3];     /* dye absorption of the emitted water */
+} tq_faucet;
+int tq_emit_faucet(tq_sim* s, const tq_faucet* f, double dt, uint32_t* seed, int* emitted, int* accepted);
+
 /* Statistics of the last pressure solve. */
 typedef struct tq_solver_stats {
   int iterations;          /* PCG iterations run */

@@ -67,6 +67,7 @@ def lib():
             "or_sim_set_tol": (None, [vp, d, i]),
             "or_sim_set_gravity": (None, [vp, d, d]),
             "or_sim_add": (i, [vp, vp, i, i]),
+            "or_sim_emit_faucet": (i, [vp, d, d, d, d, d, C.POINTER(d), d, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]),
             "or_sim_step": (None, [vp, d, i]),
             "or_sim_substep": (None, [vp, d]),
             "or_sim_phase": (None, [vp, i, d]),
@@ -205,6 +206,13 @@ class Simulacao:
     def adicionar(self, lista, checar=True):
         lista = np.ascontiguousarray(lista, dtype=np.float32)
         return lib().or_sim_add(self._p, lista.ctypes.data, lista.size // CAMPOS, 1 if checar else 0)
+
+    def emitir_torneira(self, x, y, pvx, pvy, raio, absorcao, dt, semente):
+        """emitirTorneira (principal.js:301-321) with the seeded LCG: returns (emitted, accepted)."""
+        a = (C.c_double * 3)(*absorcao)
+        n = C.c_int()
+        acc = lib().or_sim_emit_faucet(self._p, x, y, pvx, pvy, raio, a, dt, C.byref(semente.state), C.byref(n))
+        return n.value, acc
 
     def avancar(self, dt, subpassos=1):
         lib().or_sim_step(self._p, dt, subpassos)

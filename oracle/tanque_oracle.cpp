@@ -911,6 +911,36 @@ struct Sim {
     }
     return aceitas;
   }
+  // emitirTorneira (principal.js:301-321) with the seeded LCG (testes/apoio.js:4-9) in place of
+  // Math.random: the list the app builds (velocity from the pointer's, clamped to 4 m/s; count
+  // from the flow; uniform in the disc: angle then radius), then adicionar(lista, true)
+  int emitir_torneira(double px, double py, double pvx, double pvy, double raio, const double* abs3, double dt,
+                      uint32_t* seed, int* emitted) {
+    double vx = pvx * 0.5, vy = pvy * 0.5 - 0.7;
+    const double v = js_hypot(vx, vy);
+    if (v > 4) {
+      vx *= 4 / v;
+      vy *= 4 / v;
+    }
+    const double vazao = 2 * raio * std::max(0.7, js_hypot(vx, vy));
+    const double nd = std::min(40000.0, js_round((vazao * dt) / (h * h) * PARTICULAS_POR_CELULA));
+    const int nn = nd > 0 ? (int)nd : 0;
+    *emitted = nn;
+    if (nn <= 0) return 0;
+    std::vector<float> lista((size_t)nn * CAMPOS, 0.f);
+    for (int k = 0; k < nn; k++) {
+      const double a = lcg_next(seed) * M_PI * 2, r = std::sqrt(lcg_next(seed)) * raio;
+      float* o = lista.data() + (size_t)k * CAMPOS;
+      o[0] = (float)(px + std::cos(a) * r);
+      o[1] = (float)(py + std::sin(a) * r);
+      o[2] = (float)vx;
+      o[3] = (float)vy;
+      o[9] = (float)abs3[0];
+      o[10] = (float)abs3[1];
+      o[11] = (float)abs3[2];
+    }
+    return adicionar(lista.data(), nn, true);
+  }
   // remover (fisica-cpu.js:114-126) with a predicate on (x, y, index)
   template <class F>
   int remover(F fn) {
@@ -1506,6 +1536,10 @@ void or_sim_set_gravity(OrSim* s, double gx, double gy) {
   DISPATCH(s, (S->gravidade[0] = gx, S->gravidade[1] = gy, 0));
 }
 int or_sim_add(OrSim* s, const float* lista, int total, int check) { return DISPATCH(s, S->adicionar(lista, total, check != 0)); }
+int or_sim_emit_faucet(OrSim* s, double x, double y, double pvx, double pvy, double raio, const double* abs3,
+                       double dt, uint32_t* seed, int* emitted) {
+  return DISPATCH(s, S->emitir_torneira(x, y, pvx, pvy, raio, abs3, dt, seed, emitted));
+}
 void or_sim_step(OrSim* s, double dt, int sub) { DISPATCH(s, (S->avancar(dt, sub), 0)); }
 void or_sim_substep(OrSim* s, double dt) { DISPATCH(s, (S->subpasso(dt), 0)); }
 void or_sim_phase(OrSim* s, int ph, double dt) { DISPATCH(s, (S->phase(ph, dt), 0)); }

@@ -598,6 +598,43 @@ int add_particles_dev(tq_sim* s, int count, int check) {
   return 0;
 }
 
+// ================================================================== emitirTorneira (principal.js:301-321)
+// Particle k of an emission takes draws 2k+1 (angle) and 2k+2 (radius) of the LCG stream that
+// starts at `seed`; records as the reference builds them (binary64 arithmetic, f32 stores).
+__global__ __launch_bounds__(256) void k_faucet(float* __restrict__ out, int n, uint32_t seed, double x, double y,
+                                                double raio, double vx, double vy, float a0, float a1, float a2) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t s1 = lcg_jump(seed, 2ull * (uint64_t)k + 1);
+  const uint32_t s2 = s1 * LCG_A + LCG_C;
+  const double ang = (double)s1 / 4294967296.0 * M_PI * 2;
+  const double r = sqrt((double)s2 / 4294967296.0) * raio;
+  float* o = out + 12 * (size_t)k;
+  o[0] = (float)(x + cos(ang) * r);
+  o[1] = (float)(y + sin(ang) * r);
+  o[2] = (float)vx;
+  o[3] = (float)vy;
+#pragma unroll
+  for (int q = 4; q < 9; q++) o[q] = 0.f;
+  o[9] = a0;
+  o[10] = a1;
+  o[11] = a2;
+}
+
+// n particles generated into the staging list (the first `keep` of them are added); returns
+// through *seed the stream advanced past all n
+int faucet_dev(tq_sim* s, const tq_faucet* f, double vx, double vy, int n, int keep, uint32_t* seed) {
+  if (keep > 0) {
+    hipLaunchKernelGGL(k_faucet, dim3((keep + 255) / 256), dim3(256), 0, s->st, s->stage, keep, *seed, f->x, f->y,
+                       f->raio, vx, vy, (float)f->absorcao[0], (float)f->absorcao[1], (float)f->absorcao[2]);
+    TQ_LAUNCH_CHECK();
+    if (int e = add_particles_dev(s, keep, 1)) return e;
+  }
+  *seed = lcg_jump(*seed, 2ull * (uint64_t)n);
+  return 0;
+}
+
 // ================================================================== aplicarFerramenta (fisica-cpu.js:129-152)
 __global__ __launch_bounds__(256) void k_tool(float2* __restrict__ pos, float2* __restrict__ vel, float4* __restrict__ ext,
                                               const DevScalars* __restrict__ sc, int kind, double fx, double fy,

@@ -222,6 +222,45 @@ napi_value m_tool(napi_env env, napi_callback_info info) {
   return undefined(env);
 }
 
+// emitirTorneira (principal.js:301-321) with the seeded LCG: ({x, y, vx, vy, raio, absorcao, dt,
+// semente}) -> {emitidas, aceitas, semente}; vx, vy are the pointer's velocity
+napi_value m_faucet(napi_env env, napi_callback_info info) {
+  napi_value argv[1], self;
+  if (!get_args(env, info, 1, argv, &self)) return nullptr;
+  Native* n = unwrap(env, self);
+  if (!n) return nullptr;
+  napi_value o = argv[0];
+  tq_faucet f{};
+  f.x = num_or(env, o, "x", 0);
+  f.y = num_or(env, o, "y", 0);
+  f.pvx = num_or(env, o, "vx", 0);
+  f.pvy = num_or(env, o, "vy", 0);
+  f.raio = num_or(env, o, "raio", 0);
+  bool has = false;
+  napi_has_named_property(env, o, "absorcao", &has);
+  if (has) {
+    napi_value a;
+    napi_get_named_property(env, o, "absorcao", &a);
+    for (uint32_t c = 0; c < 3; c++) {
+      napi_value e;
+      if (napi_get_element(env, a, c, &e) == napi_ok) napi_get_value_double(env, e, &f.absorcao[c]);
+    }
+  }
+  const double dt = num_or(env, o, "dt", 0);
+  uint32_t seed = (uint32_t)num_or(env, o, "semente", 1);
+  int emitted = 0, accepted = 0;
+  if (!throw_tq(env, tq_emit_faucet(n->s, &f, dt, &seed, &emitted, &accepted), "tq_emit_faucet")) return nullptr;
+  napi_value r, v;
+  napi_create_object(env, &r);
+  napi_create_int32(env, emitted, &v);
+  napi_set_named_property(env, r, "emitidas", v);
+  napi_create_int32(env, accepted, &v);
+  napi_set_named_property(env, r, "aceitas", v);
+  napi_create_uint32(env, seed, &v);
+  napi_set_named_property(env, r, "semente", v);
+  return r;
+}
+
 // avancar(dt, subpassos) (fisica-cpu.js:155-159)
 napi_value m_step(napi_env env, napi_callback_info info) {
   napi_value argv[2], self;
@@ -469,6 +508,7 @@ napi_value init(napi_env env, napi_value exports) {
       {"definirSdf", nullptr, m_set_sdf, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"adicionar", nullptr, m_add, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"aplicarFerramenta", nullptr, m_tool, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"emitirTorneira", nullptr, m_faucet, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"avancar", nullptr, m_step, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"subpasso", nullptr, m_substep, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"atualizarAr", nullptr, m_simple<tq_update_air>, nullptr, nullptr, nullptr, napi_default, nullptr},

@@ -355,6 +355,39 @@ int tq_add_particles(tq_sim* s, const float* list, int count, int check, int* ac
   return 0;
 }
 
+int tq_emit_faucet(tq_sim* s, const tq_faucet* f, double dt, uint32_t* seed, int* emitted, int* accepted) {
+  TQ_CHECK_SIM(s);
+  if (!f || !seed) { set_error("tq_emit_faucet: null argument"); return TQ_E_ARG; }
+  if (emitted) *emitted = 0;
+  if (accepted) *accepted = 0;
+  // emitirTorneira's velocity, flow and count (principal.js:301-306)
+  double vx = f->pvx * 0.5, vy = f->pvy * 0.5 - 0.7;
+  const double v = js_hypot(vx, vy);
+  if (v > 4) { vx *= 4 / v; vy *= 4 / v; }
+  const double vazao = 2 * f->raio * std::max(0.7, js_hypot(vx, vy));
+  const double nd = std::min(40000.0, js_round((vazao * dt) / (s->h * s->h) * PPC));
+  const int n = nd > 0 ? (int)nd : 0;
+  if (n <= 0) return 0;
+  int keep = n;
+  if ((long long)s->n_upper + keep > s->cap) {
+    // reclaim dead slots first, then cap the list as MotorCPU does (motor.js:100-104)
+    if (int e = bin_and_sort(s)) return e;
+    if (int e = read_scalars(s)) return e;
+    s->n_upper = (int)s->sc_host->n_slots;
+    keep = std::max(0, std::min(keep, s->cap - s->n_upper));
+  }
+  if ((unsigned long long)s->seq_next + keep >= 0xFFFFFFF0ull) { set_error("tq_emit_faucet: insertion counter exhausted"); return TQ_E_CAPACITY; }
+  if (int e = faucet_dev(s, f, vx, vy, n, keep, seed)) return e;
+  s->seq_next += (uint32_t)keep;
+  s->n_upper += keep;
+  if (emitted) *emitted = n;
+  if (accepted) {
+    if (int e = read_scalars(s)) return e;
+    *accepted = keep > 0 ? (int)s->sc_host->add_count : 0;
+  }
+  return 0;
+}
+
 int tq_apply_tool(tq_sim* s, const tq_tool* t) {
   TQ_CHECK_SIM(s);
   if (!t) return 0;

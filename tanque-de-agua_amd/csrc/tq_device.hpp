@@ -106,11 +106,11 @@ __device__ inline bool grid_sum_last(double (&v)[NV], double* partials, uint32_t
 
 // ------------------------------------------------------------------ binary64 number helpers
 // Math.hypot as V8 computes it (scaled, Kahan-compensated sum of squares).
-__device__ inline double js_hypot(double a, double b) {
+__host__ __device__ inline double js_hypot(double a, double b) {
 #pragma clang fp contract(off)
   double xa = fabs(a), xb = fabs(b);
-  if (isinf(xa) || isinf(xb)) return INFINITY;
-  if (isnan(xa) || isnan(xb)) return NAN;
+  if (xa == INFINITY || xb == INFINITY) return INFINITY;
+  if (xa != xa || xb != xb) return NAN;
   double mx = fmax(xa, xb);
   if (mx == 0) return 0;
   double sum = 0, comp = 0;
@@ -129,6 +129,30 @@ __device__ inline double js_hypot(double a, double b) {
     sum = prelim;
   }
   return sqrt(sum) * mx;
+}
+
+// Math.round: nearest integer, ties toward +infinity (ECMA-262)
+__host__ __device__ inline double js_round(double x) {
+  const double f = floor(x);
+  return (x - f >= 0.5) ? f + 1 : f;
+}
+
+// The seeded LCG of the reference's tests (testes/apoio.js:4-9), s -> (s*1664525 + 1013904223)
+// mod 2^32 (exact in binary64 there, so uint32 arithmetic reproduces it), advanced k steps at
+// once by composing powers of the affine map.
+constexpr uint32_t LCG_A = 1664525u, LCG_C = 1013904223u;
+__host__ __device__ inline uint32_t lcg_jump(uint32_t s, uint64_t k) {
+  uint32_t A = 1, C = 0, a = LCG_A, c = LCG_C;
+  while (k) {
+    if (k & 1) {
+      A = a * A;
+      C = a * C + c;
+    }
+    c = a * c + c;
+    a = a * a;
+    k >>= 1;
+  }
+  return A * s + C;
 }
 
 // clampi (fisica-cpu.js:697-699) on a double

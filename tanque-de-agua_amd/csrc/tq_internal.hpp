@@ -257,7 +257,8 @@ int scan_u32(tq_sim* s, const uint32_t* in, uint32_t* out, int n);  // exclusive
 int bin_and_sort(tq_sim* s);                  // keys, counting sort into the other buffer
 int p2g(tq_sim* s, double dt, int gravity);   // fused P2G + gravity/boundary + types
 int g2p(tq_sim* s, double dt);
-int add_particles_dev(tq_sim* s, int count, int check);  // from s->stage
+int add_particles_dev(tq_sim* s, int count, int check);
+int faucet_dev(tq_sim* s, const tq_faucet* f, double vx, double vy, int n, int keep, uint32_t* seed);  // from s->stage
 int apply_tool_dev(tq_sim* s, const tq_tool* t);
 int count_live(tq_sim* s, int* n);
 int solid_cells(tq_sim* s);

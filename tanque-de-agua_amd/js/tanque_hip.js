@@ -83,6 +83,14 @@ class SimulacaoHIP {
   adicionar(lista, checarCheia = true) { return this._n.adicionar(lista, checarCheia); }
   /** aplicarFerramenta (fisica-cpu.js:129-152). */
   aplicarFerramenta(f) { if (f && f.tipo && f.tipo !== 'nenhuma') this._n.aplicarFerramenta(f); }
+  /** The faucet tool (emitirTorneira, principal.js:301-321) with a Semente in place of
+   *  Math.random: ponteiro {x, y, vx, vy}; the semente advances.  Returns {emitidas, aceitas}. */
+  emitirTorneira(ponteiro, raio, dt, absorcao = [0, 0, 0], semente = new Semente(1)) {
+    const r = this._n.emitirTorneira({ x: ponteiro.x, y: ponteiro.y, vx: ponteiro.vx, vy: ponteiro.vy, raio, absorcao,
+      dt, semente: semente.estado });
+    semente.estado = r.semente >>> 0;
+    return { emitidas: r.emitidas, aceitas: r.aceitas };
+  }
   /** avancar (fisica-cpu.js:155-159). */
   avancar(dt, subpassos = 1) { this._n.avancar(dt, subpassos); }
   /** subpasso (fisica-cpu.js:176-189). */

@@ -48,6 +48,11 @@ class SolverStats(C.Structure):
                 ("total_iterations", C.c_longlong), ("solves", C.c_longlong), ("operator_builds", C.c_longlong)]
 
 
+class Faucet(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("pvx", C.c_double), ("pvy", C.c_double),
+                ("raio", C.c_double), ("absorcao", C.c_double * 3)]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("total_ms", C.c_double), ("launches", C.c_long),
                 ("bytes_per_launch", C.c_double)]
@@ -61,6 +66,8 @@ SIGNATURES = {
     "tq_set_sdf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
     "tq_add_particles": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "tq_apply_tool": (C.c_int, [C.c_void_p, C.POINTER(Tool)]),
+    "tq_emit_faucet": (C.c_int, [C.c_void_p, C.POINTER(Faucet), C.c_double, C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "tq_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "tq_substep": (C.c_int, [C.c_void_p, C.c_double]),
     "tq_update_air": (C.c_int, [C.c_void_p]),
@@ -232,6 +239,14 @@ class SimulacaoHIP:
         for k, a in enumerate(f.get("absorcao", (0, 0, 0))):
             t.absorcao[k] = a
         _chk(load().tq_apply_tool(self._h, C.byref(t)))
+
+    def emitirTorneira(self, x, y, pvx, pvy, raio, absorcao, dt, semente):
+        """The faucet tool (emitirTorneira, principal.js:301-321) with the seeded LCG in place of
+        Math.random; `semente` advances.  Returns (emitted, accepted)."""
+        f = Faucet(x, y, pvx, pvy, raio, (C.c_double * 3)(*absorcao))
+        n, acc = C.c_int(), C.c_int()
+        _chk(load().tq_emit_faucet(self._h, C.byref(f), dt, C.byref(semente.state), C.byref(n), C.byref(acc)))
+        return n.value, acc.value
 
     def avancar(self, dt, subpassos=1):
         _chk(load().tq_step(self._h, dt, subpassos))

@@ -79,6 +79,31 @@ def ultra_hourglass(seed=0, nx=896, ny=504):
                     "sealed hourglass %dx%d h=%.4g m, 10-cell neck, upper chamber full at 4 ppc" % (nx, ny, h))
 
 
+def alto_cup(seed=0, nx=480, ny=270):
+    """Alto ~130k cells: 480x270 (2.96 mm cells), a U-shaped cup (three strokes of radius 3h: 6
+    cells thick, 120 cells between the inner faces, centred, from the floor to 46 % of the
+    stage height) and a faucet above it: the faucet tool (emitirTorneira) of radius 6h (12 cells)
+    at the top, the pointer moving down at 1.6 m/s (water at 1.5 m/s), seeded with `seed`, on
+    for the first second; 480 frames of 1/240 s.  No particles at the start."""
+    h = TANK_H / ny
+    W, H = nx * h, ny * h
+    r = 3 * h
+    m = FRAME * h
+    cx = W / 2
+    half = 60 * h + r                      # stroke centre line: inner face 60 cells from the centre
+    base = m + r
+    top = m + 0.46 * (H - 2 * m)
+    strokes = [(cx - half, top, cx - half, base, r, False), (cx - half, base, cx + half, base, r, False),
+               (cx + half, base, cx + half, top, r, False)]
+    raio = 6 * h
+    faucet = {"x": cx, "y": H - m - raio - 2 * h, "pvx": 0.0, "pvy": -1.6, "raio": raio,
+              "absorcao": (0.0, 0.0, 0.0), "seed": seed, "frames": 240}
+    w = Workload("alto", nx, ny, h, strokes, np.zeros(0, np.float32), 1 / 240, 480,
+                 "cup 480x270 h=%.4g m (U, 6 cells thick, 120 inner), faucet 12 cells at 1.5 m/s for 1 s" % h,
+                 extra={"faucet": faucet})
+    return w
+
+
 def scaled_hourglass(nx, ny, seed=0):
     """The Ultra scene at another resolution (size sweeps past the 256 MB last-level cache)."""
     w = ultra_hourglass(seed, nx, ny)
