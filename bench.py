@@ -121,6 +121,42 @@ def run_alto(tq, work, precision, tol=1e-6, maxit=200, schedule=None):
     return res, sched
 
 
+def run_batched(tq, frames=20, threads=8):
+    """The batched configuration (64 maze tanks, fixed 3 PCG iterations as the reference): the
+    same frames stepped tank after tank (sequential launches, each tank waited for) and as one
+    batch (tq_batch_step), each from fresh identical states; tank-steps/s = tanks x frames / s."""
+    from tanque_hip import scenes
+    from tanque_hip.scenes import passo_frame
+    ts = scenes.batched_mazes()
+    res = {"tanks": len(ts), "cells": sum(t.nx * t.ny for t in ts), "frames_timed": frames,
+           "of_frames": ts[0].frames, "threads": threads, "pcg": "3 iterations (reference)"}
+    for mode in ("sequential", "batched"):
+        cenas = [t.cena() for t in ts]
+        b = tq.Batch(cenas, [{"iteracoes": 3}] * len(ts), threads=threads)
+        for t, s in zip(ts, b.tanks):
+            s.adicionar(t.particles, False)
+        res["particles"] = sum(s.n for s in b.tanks)
+        dt = ts[0].dt_frame
+
+        def frame():
+            subs = [passo_frame(t.h, c.altura, s.vMax, dt) for t, c, s in zip(ts, cenas, b.tanks)]
+            if mode == "sequential":
+                for s, k in zip(b.tanks, subs):
+                    s.avancar(dt, k)
+            else:
+                b.step(dt, subs)
+            return sum(subs)
+
+        frame()
+        t0 = time.perf_counter()
+        subs = sum(frame() for _ in range(frames))
+        el = time.perf_counter() - t0
+        res[mode] = {"tank_steps_per_s": round(len(ts) * frames / el, 1), "tank_substeps_per_s": round(subs / el, 1)}
+        b.close()
+    res["batched_over_sequential"] = round(res["batched"]["tank_steps_per_s"] / res["sequential"]["tank_steps_per_s"], 3)
+    return res
+
+
 def dominant_kernel(tq, work, precision, frames=3):
     """Name of the kernel family with the largest device time on a few frames of the workload."""
     from tanque_hip.scenes import passo_frame
@@ -203,6 +239,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay runs")
     ap.add_argument("--no-variants", action="store_true", help="skip the partial-FP16 variants")
     ap.add_argument("--no-alto", action="store_true", help="skip the Alto (cup + faucet) runs")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched (64 tanks) runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="start with a 256 MiB device copy (reference for rocprofv3 --pmc byte counters)")
     ap.add_argument("--nx", type=int, default=896)
@@ -279,6 +316,11 @@ def main():
             alto["mixed_over_fp32"] = round(alto["mixed"]["substeps_per_s"] / alto["fp32"]["substeps_per_s"], 4)
             log("alto mixed: %s" % alto["mixed"])
 
+    batched = None
+    if not args.no_batched:
+        batched = run_batched(tq)
+        log("batched: %s" % batched)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(work, args.cpu_budget)
@@ -310,6 +352,7 @@ def main():
         "graph_replay": graph or None,
         "mixed_variants": variants or None,
         "alto": alto,
+        "batched": batched,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

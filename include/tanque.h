@@ -155,6 +155,19 @@ typedef struct tq_faucet {
 } tq_faucet;
 int tq_emit_faucet(tq_sim* s, const tq_faucet* f, double dt, uint32_t* seed, int* emitted, int* accepted);
 
+/* A batch of independent tanks stepped together (the batched configuration; the reference steps
+ * one SimulacaoCPU per tank, fisica-cpu.js:29-77, 155-159, so there is no batch counterpart).
+ * Every tank is a tq_sim with its own stream, set up through the tq_* calls on tq_batch_tank(b, k);
+ * tq_batch_step does avancar(dt, substeps[k]) on every tank from `threads` host threads (tank k
+ * on thread k mod threads) and returns when all are done; the device overlaps the tanks'
+ * kernels.  Do not call tq_* on a tank while tq_batch_step runs. */
+typedef struct tq_batch tq_batch;
+int tq_batch_create(const tq_desc* descs, int n, int threads, tq_batch** out);
+int tq_batch_size(tq_batch* b, int* n);
+tq_sim* tq_batch_tank(tq_batch* b, int k);   /* NULL when k is out of range */
+int tq_batch_step(tq_batch* b, double dt, const int* substeps);
+int tq_batch_destroy(tq_batch* b);
+
 /* Statistics of the last pressure solve. */
 typedef struct tq_solver_stats {
   int iterations;          /* PCG iterations run */

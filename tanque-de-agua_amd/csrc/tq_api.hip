@@ -259,6 +259,20 @@ using namespace tq;
 
 #define TQ_CHECK_SIM(s) do { if (!(s)) { set_error("null handle"); return TQ_E_ARG; } } while (0)
 
+namespace tq {
+// avancar's work enqueued on the tank's stream without waiting for it (tq_batch_step)
+int step_enqueue(tq_sim* s, double dt, int substeps) {
+  if (!s) { set_error("null handle"); return TQ_E_ARG; }
+  if (!(dt > 0) || substeps < 0) { set_error("tq_step: bad dt or substep count"); return TQ_E_ARG; }
+  if (s->d.ar) if (int e = update_air(s)) return e;
+  const double d = dt / substeps;
+  for (int k = 0; k < substeps; k++)
+    if (int e = substep(s, d)) return e;
+  return 0;
+}
+int stream_wait(tq_sim* s) { return sync(s); }
+}  // namespace tq
+
 extern "C" {
 
 const char* tq_last_error(void) { return g_err.c_str(); }
@@ -403,12 +417,7 @@ int tq_substep(tq_sim* s, double dt) {
 }
 
 int tq_step(tq_sim* s, double dt, int substeps) {
-  TQ_CHECK_SIM(s);
-  if (!(dt > 0) || substeps < 0) { set_error("tq_step: bad dt or substep count"); return TQ_E_ARG; }
-  if (s->d.ar) if (int e = update_air(s)) return e;
-  const double d = dt / substeps;
-  for (int k = 0; k < substeps; k++)
-    if (int e = substep(s, d)) return e;
+  if (int e = tq::step_enqueue(s, dt, substeps)) return e;
   return sync(s);
 }
 

@@ -258,6 +258,8 @@ int bin_and_sort(tq_sim* s);                  // keys, counting sort into the ot
 int p2g(tq_sim* s, double dt, int gravity);   // fused P2G + gravity/boundary + types
 int g2p(tq_sim* s, double dt);
 int add_particles_dev(tq_sim* s, int count, int check);
+int step_enqueue(tq_sim* s, double dt, int substeps);  // tq_step without the final wait
+int stream_wait(tq_sim* s);
 int faucet_dev(tq_sim* s, const tq_faucet* f, double vx, double vy, int n, int keep, uint32_t* seed);  // from s->stage
 int apply_tool_dev(tq_sim* s, const tq_tool* t);
 int count_live(tq_sim* s, int* n);

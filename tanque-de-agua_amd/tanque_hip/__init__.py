@@ -68,6 +68,11 @@ SIGNATURES = {
     "tq_apply_tool": (C.c_int, [C.c_void_p, C.POINTER(Tool)]),
     "tq_emit_faucet": (C.c_int, [C.c_void_p, C.POINTER(Faucet), C.c_double, C.POINTER(C.c_uint32),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "tq_batch_create": (C.c_int, [C.POINTER(Desc), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "tq_batch_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "tq_batch_tank": (C.c_void_p, [C.c_void_p, C.c_int]),
+    "tq_batch_step": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_int)]),
+    "tq_batch_destroy": (C.c_int, [C.c_void_p]),
     "tq_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "tq_substep": (C.c_int, [C.c_void_p, C.c_double]),
     "tq_update_air": (C.c_int, [C.c_void_p]),
@@ -187,10 +192,22 @@ class SimulacaoHIP:
         self.arAtivo = bool(ar)
         self.atualizarParedes()
 
+    @classmethod
+    def _view(cls, handle, cena, desc, ar=True):
+        """A SimulacaoHIP on a handle owned elsewhere (a batch's tank); closing it does nothing."""
+        self = cls.__new__(cls)
+        self._desc = desc
+        self._h = C.c_void_p(handle)
+        self._owned = False
+        self.cena, self.nx, self.ny, self.h = cena, cena.nx, cena.ny, cena.h
+        self.arAtivo = bool(ar)
+        self.atualizarParedes()
+        return self
+
     def close(self):
-        if getattr(self, "_h", None) and _lib is not None:
+        if getattr(self, "_h", None) and _lib is not None and getattr(self, "_owned", True):
             _lib.tq_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         self.close()
@@ -369,6 +386,50 @@ class SimulacaoHIP:
 
     def sync(self):
         _chk(load().tq_sync(self._h))
+
+
+class Batch:
+    """Independent tanks stepped together (tq_batch_*): `cenas` with one options dict each
+    (SimulacaoHIP's keywords); `tanks[k]` is a SimulacaoHIP view of tank k for set-up and reads;
+    step(dt, substeps) runs avancar(dt, substeps[k]) on every tank from `threads` host threads."""
+
+    def __init__(self, cenas, opcoes, threads=8):
+        L = load()
+        n = len(cenas)
+        descs = (Desc * n)()
+        for k, (cena, o) in enumerate(zip(cenas, opcoes)):
+            L.tq_desc_default(C.byref(descs[k]), cena.nx, cena.ny, cena.h)
+            d = descs[k]
+            d.gx, d.gy = o.get("gravidade", (0.0, -9.81))
+            d.iteracoes = o.get("iteracoes", 3)
+            d.tau_densidade = o.get("tauDensidade", 0.1)
+            d.tensao = o.get("tensao", 0.0728)
+            d.ar = 1 if o.get("ar", True) else 0
+            d.precision = o.get("precision", PREC_FP32)
+            d.tol = o.get("tol", 0.0)
+            d.use_graph = 1 if o.get("use_graph", False) else 0
+        h = C.c_void_p()
+        _chk(L.tq_batch_create(descs, n, threads, C.byref(h)))
+        self._h = h
+        self.tanks = []
+        for k, cena in enumerate(cenas):
+            dk = Desc()
+            C.memmove(C.byref(dk), C.byref(descs[k]), C.sizeof(Desc))
+            self.tanks.append(SimulacaoHIP._view(L.tq_batch_tank(h, k), cena, dk, bool(dk.ar)))
+
+    def step(self, dt, substeps):
+        arr = (C.c_int * len(self.tanks))(*[int(x) for x in substeps])
+        _chk(load().tq_batch_step(self._h, dt, arr))
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            for t in self.tanks:
+                t._h = None
+            _lib.tq_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def rodar(sim, segundos, a_cada=None, quadro=1 / 60):

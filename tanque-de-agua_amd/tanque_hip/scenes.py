@@ -104,6 +104,78 @@ def alto_cup(seed=0, nx=480, ny=270):
     return w
 
 
+def maze_strokes(nx, ny, h, aleatorio, lattice=8, wall_cells=3):
+    """The maze of modelos.js:133-185 (recursive backtracker from the top-left cell, entrance at
+    the top of the first column and exit at the bottom of the last) on a lattice of `lattice`
+    cells filling the tank inside its frame, walls `wall_cells` thick."""
+    W, H = nx * h, ny * h
+    tam = lattice * h
+    cols = max(2, (nx - 2 * FRAME) // lattice)
+    rows = max(2, (ny - 2 * FRAME) // lattice)
+    x0, y0 = (W - cols * tam) / 2, (H - rows * tam) / 2
+    r = wall_cells * h / 2
+    dir_ = [[True] * rows for _ in range(cols)]
+    baixo = [[True] * rows for _ in range(cols)]
+    visto = [[False] * rows for _ in range(cols)]
+    pilha = [(0, rows - 1)]
+    visto[0][rows - 1] = True
+    while pilha:
+        c, l = pilha[-1]
+        viz = [(a, b) for a, b in ((c + 1, l), (c - 1, l), (c, l + 1), (c, l - 1))
+               if 0 <= a < cols and 0 <= b < rows and not visto[a][b]]
+        if not viz:
+            pilha.pop()
+            continue
+        a, b = viz[int(math.floor(aleatorio() * len(viz)))]
+        if a == c + 1:
+            dir_[c][l] = False
+        elif a == c - 1:
+            dir_[a][l] = False
+        elif b == l + 1:
+            baixo[c][b] = False
+        else:
+            baixo[c][l] = False
+        visto[a][b] = True
+        pilha.append((a, b))
+    X = lambda c: x0 + c * tam
+    Y = lambda l: y0 + l * tam
+    t = []
+    for c in range(cols):
+        for l in range(rows):
+            if dir_[c][l] and c < cols - 1:
+                t.append((X(c + 1), Y(l), X(c + 1), Y(l + 1), r, False))
+            if baixo[c][l] and l > 0:
+                t.append((X(c), Y(l), X(c + 1), Y(l), r, False))
+    t += [(X(0), Y(0), X(0), Y(rows), r, False), (X(cols), Y(0), X(cols), Y(rows), r, False),
+          (X(1), Y(rows), X(cols), Y(rows), r, False), (X(0), Y(0), X(cols - 1), Y(0), r, False)]
+    return t
+
+
+def batched_mazes(n=64, seed=0, h=TANK_H / 270):
+    """The batched configuration: n tanks with widths uniform in [64, 384] and heights in [48, 256]
+    cells (h = 2.963 mm), each a maze (maze_strokes) and a water block over the full width and a
+    uniform 30-60 % of the height at 4 ppc; all draws from the seeded LCG; frames of 1/240 s."""
+    from . import Semente as _Semente
+    rng = _Semente(seed)
+
+    def u():
+        rng.state.value = (rng.state.value * 1664525 + 1013904223) & 0xFFFFFFFF
+        return rng.state.value / 4294967296.0
+
+    tanks = []
+    for k in range(n):
+        nx = 64 + int(math.floor(u() * 321))
+        ny = 48 + int(math.floor(u() * 209))
+        strokes = maze_strokes(nx, ny, h, u)
+        frac = 0.3 + 0.3 * u()
+        W, H = nx * h, ny * h
+        m = FRAME * h
+        P = semear_retangulo(m, m, W - m, m + frac * (H - 2 * m), h, (0, 0, 0), _Semente(seed * 1000 + k + 1))
+        tanks.append(Workload("maze%d" % k, nx, ny, h, strokes, P, 1 / 240, 500,
+                              "maze %dx%d, water %.0f %% of the height" % (nx, ny, 100 * frac)))
+    return tanks
+
+
 def scaled_hourglass(nx, ny, seed=0):
     """The Ultra scene at another resolution (size sweeps past the 256 MB last-level cache)."""
     w = ultra_hourglass(seed, nx, ny)

@@ -1,0 +1,57 @@
+"""Batched tanks (tq_batch_*): stepping tanks together from several host threads gives, tank by
+tank, exactly the states of stepping each alone (a tank's work stays on its own stream).
+
+The comparison is bit-exact, so it runs the maze tanks without air tracking: with air, a maze
+traps 100-200 bubbles, and bubbles beyond the 8 per-block reduction slots are summed with FP64
+atomics, whose order -- and last bits -- differ between any two runs, batched or not."""
+import numpy as np
+import pytest
+
+from gpu_common import log
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def tq():
+    import tanque_hip
+    tanque_hip.load()
+    return tanque_hip
+
+
+def test_batch_matches_tanks_stepped_alone(tq):
+    from tanque_hip import scenes
+    from tanque_hip.scenes import passo_frame
+    ts = scenes.batched_mazes(n=5, seed=3)
+    opts = {"iteracoes": 3, "ar": False}
+    cenas = [t.cena() for t in ts]
+    b = tq.Batch(cenas, [opts] * len(ts), threads=3)
+    alone = [tq.SimulacaoHIP(t.cena(), iteracoes=3, ar=False) for t in ts]
+    for t, sb, sa in zip(ts, b.tanks, alone):
+        sb.adicionar(t.particles, False)
+        sa.adicionar(t.particles, False)
+    for f in range(6):
+        subs = [passo_frame(t.h, c.altura, s.vMax, t.dt_frame) for t, c, s in zip(ts, cenas, b.tanks)]
+        b.step(ts[0].dt_frame, subs)
+        for sa, k in zip(alone, subs):
+            sa.avancar(ts[0].dt_frame, k)
+    for k, (t, sb, sa) in enumerate(zip(ts, b.tanks, alone)):
+        pb, pa = sb.particulas(), sa.particulas()
+        log("\n  tank %d (%dx%d): %d particles batched, %d alone" % (k, t.nx, t.ny, len(pb), len(pa)))
+        assert len(pb) == len(pa) > 0
+        assert np.array_equal(pb.view(np.uint32), pa.view(np.uint32))
+        assert np.array_equal(sb.read(12), sa.read(12))
+    for s in alone:
+        s.close()
+    b.close()
+
+
+def test_batch_rejects_bad_arguments(tq):
+    from tanque_hip import scenes
+    ts = scenes.batched_mazes(n=2, seed=4)
+    b = tq.Batch([t.cena() for t in ts], [{}] * 2, threads=2)
+    with pytest.raises(tq.TqError):
+        b.step(-1.0, [1, 1])
+    with pytest.raises(tq.TqError):
+        b.step(1 / 240, [1, -1])
+    b.close()
