@@ -186,3 +186,112 @@ def scaled_hourglass(nx, ny, seed=0):
 def passo_frame(h, altura, vmax, dt_frame):
     from . import passo_maximo
     return int(math.ceil(dt_frame / passo_maximo(h, altura, vmax, dt_frame)))
+
+
+# ------------------------------------------------------------------ the funnel of the adjoint workload
+FUNNEL_SEGMENTS = 16  # capsule segments per wall; one height parameter each
+
+
+def funnel_geometry(nx=256, ny=144, segments=FUNNEL_SEGMENTS):
+    """Centre lines of the two funnel walls (metres): the left wall from (0.38 W, 0.80 H) and the
+    right from (0.93 W, 0.80 H) down to a spout at x = 0.68 W, y = 0.40 H whose inner faces are
+    12 cells apart; walls 4 cells thick (capsules of radius 2h)."""
+    h = TANK_H / ny
+    W, H = nx * h, ny * h
+    r = 2 * h
+    xs, ys = 0.68 * W, 0.40 * H
+    left = ((0.38 * W, 0.80 * H), (xs - 6 * h - r, ys))
+    right = ((0.93 * W, 0.80 * H), (xs + 6 * h + r, ys))
+    return h, r, [left, right], segments
+
+
+def funnel_strokes(heights, nx=256, ny=144, segments=FUNNEL_SEGMENTS):
+    """The funnel's capsule strokes: each wall split into `segments` equal pieces, piece k of
+    wall w moved up by heights[w * segments + k] metres (both of its ends)."""
+    h, r, walls, S = funnel_geometry(nx, ny, segments)
+    heights = np.asarray(heights, np.float64)
+    assert heights.shape == (2 * S,)
+    strokes = []
+    for w, (a, b) in enumerate(walls):
+        for k in range(S):
+            t0, t1 = k / S, (k + 1) / S
+            dy = float(heights[w * S + k])
+            strokes.append((a[0] + (b[0] - a[0]) * t0, a[1] + (b[1] - a[1]) * t0 + dy,
+                            a[0] + (b[0] - a[0]) * t1, a[1] + (b[1] - a[1]) * t1 + dy, r, False))
+    return strokes
+
+
+def funnel(heights=None, vjet=(0.6, -1.5), seed=0, nx=256, ny=144, frames=120):
+    """Adjoint 256x144 (5.56 mm cells): the funnel of `funnel_strokes`, and a faucet of radius 6h
+    (a 12-cell jet) above the left half of the funnel emitting water at the jet velocity `vjet`
+    (m/s) every frame (the faucet tool with the pointer velocity that gives that jet:
+    principal.js:302 maps pointer velocity p to 0.5 p + (0, -0.7)), seeded with `seed`;
+    `frames` frames of 1/240 s.  The loss box is the 40x30 cells at the bottom right inside
+    the frame; the loss is (water fraction of the box - 0.5)^2."""
+    h = TANK_H / ny
+    W, H = nx * h, ny * h
+    if heights is None:
+        heights = np.zeros(2 * FUNNEL_SEGMENTS)
+    strokes = funnel_strokes(heights, nx, ny)
+    raio = 6 * h
+    vx, vy = float(vjet[0]), float(vjet[1])
+    faucet = {"x": 0.60 * W, "y": 0.90 * H, "pvx": 2 * vx, "pvy": 2 * (vy + 0.7), "raio": raio,
+              "absorcao": (0.0, 0.0, 0.0), "seed": seed, "frames": frames}
+    box = (nx - FRAME - 40, FRAME, nx - FRAME, FRAME + 30)
+    return Workload("funnel", nx, ny, h, strokes, np.zeros(0, np.float32), 1 / 240, frames,
+                    "funnel 256x144 h=%.4g m, 2x16 wall segments, faucet jet (%.2f, %.2f) m/s" % (h, vx, vy),
+                    extra={"faucet": faucet, "box": box, "target": 0.5, "heights": np.asarray(heights, np.float64)})
+
+
+def stroke_shift_vjp(nx, ny, h, strokes, gsdf, movable):
+    """Chain rule from the SDF lattice to vertical shifts of strokes: returns, for each stroke
+    index in `movable`, sum over lattice points of gsdf * d(sdf)/d(shift), for an SDF built as
+    Cena.tracar builds it (js/cena.js:43-72: the frame, then the running minimum of clamped
+    capsule distances, stored in binary32 after each stroke).  The point's value comes from the
+    last stroke that lowered it (none: the frame, no derivative); its distance d = (|p| - rp) / r
+    (r = 2 points per cell) moves with a shift s of both ends of that stroke by
+    (p_y / |p|) / h per metre, p the vector from the point to its nearest point on the segment,
+    and not at all where d is clamped to +-SDF_MAX."""
+    R = 2
+    mw, mh = nx * R, ny * R
+    gsdf = np.asarray(gsdf, np.float64).reshape(mh, mw)
+    X, Y = np.meshgrid(np.arange(mw, dtype=np.float64), np.arange(mh, dtype=np.float64))
+    m = FRAME * R - 0.5
+    cur = np.minimum(np.minimum(X - m, mw - 1 - m - X), np.minimum(Y - m, mh - 1 - m - Y)) / R
+    cur = np.clip(cur, -10, 10).astype(np.float32)
+    owner = np.full((mh, mw), -1, np.int64)
+    deriv = np.zeros((mh, mw))
+    s = R / h
+    for k, (x0, y0, x1, y1, raio, apagar) in enumerate(strokes):
+        assert not apagar
+        ax, ay, bx, by = x0 * s - 0.5, y0 * s - 0.5, x1 * s - 0.5, y1 * s - 0.5
+        rp = raio * s
+        alc = rp + 10 * R + 1
+        xa = max(0, int(math.floor(min(ax, bx) - alc))); xb = min(mw - 1, int(math.ceil(max(ax, bx) + alc)))
+        ya = max(0, int(math.floor(min(ay, by) - alc))); yb = min(mh - 1, int(math.ceil(max(ay, by) + alc)))
+        if xa > xb or ya > yb:
+            continue
+        px_, py_ = X[ya:yb + 1, xa:xb + 1], Y[ya:yb + 1, xa:xb + 1]
+        dx, dy = bx - ax, by - ay
+        len2 = dx * dx + dy * dy
+        t = ((px_ - ax) * dx + (py_ - ay) * dy) / len2 if len2 > 0 else np.zeros_like(px_)
+        t = np.clip(t, 0, 1)
+        qx, qy = ax + t * dx - px_, ay + t * dy - py_
+        nrm = np.sqrt(qx * qx + qy * qy)
+        d = (nrm - rp) / R
+        dc = np.clip(d, -10, 10)
+        c = cur[ya:yb + 1, xa:xb + 1]
+        new = np.minimum(c.astype(np.float64), dc).astype(np.float32)
+        low = new < c
+        with np.errstate(invalid="ignore", divide="ignore"):
+            dd = np.where((np.abs(d) < 10) & (nrm > 0), qy / np.where(nrm > 0, nrm, 1) / h, 0.0)
+        o = owner[ya:yb + 1, xa:xb + 1]
+        o[low] = k
+        dv = deriv[ya:yb + 1, xa:xb + 1]
+        dv[low] = dd[low]
+        cur[ya:yb + 1, xa:xb + 1] = new
+    out = np.zeros(len(movable))
+    for j, k in enumerate(movable):
+        sel = owner == k
+        out[j] = float(np.sum(gsdf[sel] * deriv[sel]))
+    return out
