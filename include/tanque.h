@@ -168,6 +168,36 @@ tq_sim* tq_batch_tank(tq_batch* b, int k);   /* NULL when k is out of range */
 int tq_batch_step(tq_batch* b, double dt, const int* substeps);
 int tq_batch_destroy(tq_batch* b);
 
+/* Differentiation of recorded frames (the reference has no gradient; it is the backward() that
+ * a torch.autograd.Function would give, for a JavaScript host).  tq_tape_begin starts recording
+ * the frames run from now on: tq_emit_faucet (principal.js:301-321) and tq_step
+ * (fisica-cpu.js:155-159) calls; every other call that changes the state returns TQ_E_STATE
+ * until tq_tape_end.  checkpoint_every > 0 keeps the tank's state every that many tq_step calls
+ * and re-runs the frames in between during tq_backward; 0 keeps every substep's record instead.
+ * tq_backward takes dL/d(final state) for the live particles in tq_read_particles order, 8
+ * floats each (x y vx vy cux cuy cvx cvy), and returns dL/d(jet velocity) of each recorded
+ * tq_emit_faucet call (2 doubles each, the velocity after principal.js:302-304's mapping),
+ * dL/d(SDF) (4*nx*ny floats, may be NULL) and dL/d(state at tq_tape_begin) (n0 x 8 floats in
+ * that moment's tq_read_particles order, may be NULL).  The derivative is that of the recorded
+ * run with its discrete decisions held (cell types, bubble labels, micro-step counts, wall
+ * push-outs and velocity projections, removals); the adjoint pressure solves run the same
+ * MGPCG to relative residual `tol`.  The tank is left in its end state. */
+int tq_tape_begin(tq_sim* s, int checkpoint_every);
+int tq_tape_end(tq_sim* s);
+int tq_tape_info(tq_sim* s, int* frames, int* faucet_calls, int* checkpoints, int* n_begin);
+int tq_backward(tq_sim* s, const float* grad_state, int n, double tol, double* grad_faucet, int n_faucet,
+                float* grad_sdf, float* grad_state0, int n0);
+/* Water fraction of the cell box [i0,i1) x [j0,j1): the mean of min(1, densidade/PPC) over its
+ * cells (fisica-cpu.js:305), densidade the P2G density splat (:241-250) of the current particles;
+ * grad (may be NULL) gets d(fraction)/d(state), n x 8 floats in tq_read_particles order. */
+int tq_water_fraction(tq_sim* s, int i0, int j0, int i1, int j1, double* frac, float* grad, int n);
+/* Test hooks: substep records of a keep-all tape (checkpoint_every = 0): their count, and field
+ * `field` of record k (0 particle count u32, 1 types C u8, 2 labels C u32, 3 qAr C f32,
+ * 4 volumeAlvo and 5 yBolha MAX_BOLHAS f32, 6 decision words n u16 and 7 insertion numbers
+ * n u32 in P2G order, 8 dt f64). */
+int tq_tape_records(tq_sim* s, int* n);
+int tq_tape_record(tq_sim* s, int k, int field, void* dst, size_t bytes);
+
 /* Statistics of the last pressure solve. */
 typedef struct tq_solver_stats {
   int iterations;          /* PCG iterations run */

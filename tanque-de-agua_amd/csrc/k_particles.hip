@@ -424,7 +424,8 @@ __global__ __launch_bounds__(256) void k_g2p(float2* __restrict__ pos, float2* _
                                              const float* __restrict__ dens, const uint32_t* __restrict__ countCell,
                                              const float* __restrict__ dye, const float* __restrict__ sdf, int nx,
                                              int ny, double h, double dt, double decai, double mistura,
-                                             unsigned long long* __restrict__ vmax_bits) {
+                                             unsigned long long* __restrict__ vmax_bits,
+                                             uint16_t* __restrict__ flags) {
 #pragma clang fp contract(off)
   // particles are in cell order: consecutive blocks on one XCD gather from one band of rows
   const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -485,6 +486,9 @@ __global__ __launch_bounds__(256) void k_g2p(float2* __restrict__ pos, float2* _
       const int micro = (int)fmin((double)MAX_MICROPASSOS, fmax(1.0, ceil((fmin(velm, vmax) * dt) / (MICROPASSO * h))));
       const double dm = dt / micro;
       bool morta = false;
+      // decision word for the reverse pass: micro - 1, push-out steps (bit 2 + m), inward
+      // velocity removed (bit 6 + m), removal (bit 10)
+      uint32_t fw = (uint32_t)(micro - 1);
       for (int m = 0; m < micro; m++) {
         double wx = vx, wy = vy;
         if (micro > 1) velocidade_em(u, v, nx, ny, h, x, y, vmax, &wx, &wy);
@@ -492,16 +496,18 @@ __global__ __launch_bounds__(256) void k_g2p(float2* __restrict__ pos, float2* _
         y += wy * dm;
         double d = sdf_distancia(sdf, w, hh, h, x, y);
         if (d < RAIO_PARTICULA) {
-          if (d < -SDF_MAX + 0.5) { morta = true; break; }
+          fw |= 1u << (2 + m);
+          if (d < -SDF_MAX + 0.5) { morta = true; fw |= 1u << 10; break; }
           double nX, nY;
           sdf_normal(sdf, w, hh, h, x, y, &nX, &nY);
           double empurra = (RAIO_PARTICULA - d) * h;
           x += nX * empurra;
           y += nY * empurra;
           double vn = vx * nX + vy * nY;
-          if (vn < 0) { vx -= vn * nX; vy -= vn * nY; }
+          if (vn < 0) { vx -= vn * nX; vy -= vn * nY; fw |= 1u << (6 + m); }
         }
       }
+      if (flags) flags[k] = (uint16_t)fw;
       if (morta) {
         pos[k] = make_float2(__builtin_nanf(""), P.y);
       } else {
@@ -530,7 +536,7 @@ int g2p(tq_sim* s, double dt) {
     int t = time_begin(s, "g2p", bytes);
     hipLaunchKernelGGL(k_g2p, dim3((n + 255) / 256), dim3(256), 0, s->st, s->pos[o], s->vel[o], s->cm[o], s->ext[o],
                        s->sc, s->u, s->v, s->dens, s->countCell, s->dye, s->sdf, s->nx, s->ny, s->h, dt, decai,
-                       mistura, &s->sc->vmax_bits);
+                       mistura, &s->sc->vmax_bits, s->g2p_flags);
     time_end(s, t);
     TQ_LAUNCH_CHECK();
   }

@@ -85,7 +85,11 @@ static int dalloc(T** p, size_t n) {
   TQ_HIP(hipMalloc((void**)p, n * sizeof(T)));
   return 0;
 }
-#define DALLOC(p, n) do { if (int e_ = dalloc(&(p), (n))) return e_; } while (0)
+#define DALLOC(p, n)                                                                   \
+  do {                                                                                 \
+    if (int e_ = dalloc(&(p), (n))) return e_;                                         \
+    s->bufs.emplace_back((void*)(p), sizeof(*(p)) * std::max<size_t>((size_t)(n), 1)); \
+  } while (0)
 
 static void frame_sdf(std::vector<float>& sdf, int nx, int ny) { tq_scene_frame(sdf.data(), nx, ny); }
 
@@ -173,18 +177,23 @@ static void free_all(tq_sim* s) {
 }
 
 // ------------------------------------------------------------------ the substep
+// (with a substep record open, the hooks copy what the reverse pass needs)
 static int substep(tq_sim* s, double dt) {
   if (int e = p2g(s, dt, 1)) return e;           // #paraGrade + #gravidadeEFronteira + #tipos
   s->tipos_prontos = true;
+  if (s->rec) if (int e = rec_hook(s, REC_P2G, dt)) return e;
   if (s->d.ar) if (int e = propagate_labels(s)) return e;
   if (int e = surface(s)) return e;
   if (int e = air_pressure_and_div(s, dt, 3)) return e;  // #pressaoDoAr + #classificar
   if (s->d.ar) if (int e = bubble_rhs(s, dt)) return e;
+  if (s->rec) if (int e = rec_hook(s, REC_SOLVE, dt)) return e;
   if (int e = pressure_solve(s)) return e;
   if (int e = pressure_post(s, dt)) return e;
   if (int e = gradient(s)) return e;
   if (int e = extrapolate(s)) return e;
+  if (s->rec) if (int e = rec_hook(s, REC_EXTRAP, dt)) return e;
   if (int e = g2p(s, dt)) return e;
+  if (s->rec) if (int e = rec_hook(s, REC_G2P, dt)) return e;
   return 0;
 }
 
@@ -258,6 +267,13 @@ static size_t kind_bytes(int k) { return k == 1 ? 1 : 4; }
 using namespace tq;
 
 #define TQ_CHECK_SIM(s) do { if (!(s)) { set_error("null handle"); return TQ_E_ARG; } } while (0)
+// calls that change the state other than by the recorded operations (faucet, step) would make the
+// recorded frames unreproducible for tq_backward
+#define TQ_NOT_TAPING(s, what)                                                                 \
+  do {                                                                                         \
+    TQ_CHECK_SIM(s);                                                                           \
+    if ((s)->tape) { set_error(what ": not allowed while frames are recorded"); return TQ_E_STATE; } \
+  } while (0)
 
 namespace tq {
 // avancar's work enqueued on the tank's stream without waiting for it (tq_batch_step)
@@ -271,6 +287,75 @@ int step_enqueue(tq_sim* s, double dt, int substeps) {
   return 0;
 }
 int stream_wait(tq_sim* s) { return sync(s); }
+}  // namespace tq
+
+namespace tq {
+int emit_faucet_impl(tq_sim* s, const tq_faucet* f, double dt, uint32_t* seed, int* emitted, int* accepted) {
+  if (!f || !seed) { set_error("tq_emit_faucet: null argument"); return TQ_E_ARG; }
+  if (emitted) *emitted = 0;
+  if (accepted) *accepted = 0;
+  // emitirTorneira's velocity, flow and count (principal.js:301-306)
+  double vx = f->pvx * 0.5, vy = f->pvy * 0.5 - 0.7;
+  const double v = js_hypot(vx, vy);
+  if (v > 4) { vx *= 4 / v; vy *= 4 / v; }
+  const double vazao = 2 * f->raio * std::max(0.7, js_hypot(vx, vy));
+  const double nd = std::min(40000.0, js_round((vazao * dt) / (s->h * s->h) * PPC));
+  const int n = nd > 0 ? (int)nd : 0;
+  if (n <= 0) return 0;
+  int keep = n;
+  if ((long long)s->n_upper + keep > s->cap) {
+    // reclaim dead slots first, then cap the list as MotorCPU does (motor.js:100-104)
+    if (int e = bin_and_sort(s)) return e;
+    if (int e = read_scalars(s)) return e;
+    s->n_upper = (int)s->sc_host->n_slots;
+    keep = std::max(0, std::min(keep, s->cap - s->n_upper));
+  }
+  if ((unsigned long long)s->seq_next + keep >= 0xFFFFFFF0ull) { set_error("tq_emit_faucet: insertion counter exhausted"); return TQ_E_CAPACITY; }
+  if (int e = faucet_dev(s, f, vx, vy, n, keep, seed)) return e;
+  s->seq_next += (uint32_t)keep;
+  s->n_upper += keep;
+  if (emitted) *emitted = n;
+  if (accepted) {
+    if (int e = read_scalars(s)) return e;
+    *accepted = keep > 0 ? (int)s->sc_host->add_count : 0;
+  }
+  return 0;
+}
+}  // namespace tq
+
+namespace tq {
+// downloads the live particles ordered by insertion number (the reference's order)
+int download_ordered(tq_sim* s, std::vector<float>& out, std::vector<uint32_t>* seqs) {
+  if (int e = read_scalars(s)) return e;
+  const size_t n = s->sc_host->n_slots;
+  const int o = s->cur;
+  std::vector<float2> pos(n), vel(n);
+  std::vector<float4> cm(n), ext(n);
+  std::vector<uint32_t> sq(n);
+  if (n) {
+    TQ_HIP(hipMemcpyAsync(pos.data(), s->pos[o], n * sizeof(float2), hipMemcpyDeviceToHost, s->st));
+    TQ_HIP(hipMemcpyAsync(vel.data(), s->vel[o], n * sizeof(float2), hipMemcpyDeviceToHost, s->st));
+    TQ_HIP(hipMemcpyAsync(cm.data(), s->cm[o], n * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+    TQ_HIP(hipMemcpyAsync(ext.data(), s->ext[o], n * sizeof(float4), hipMemcpyDeviceToHost, s->st));
+    TQ_HIP(hipMemcpyAsync(sq.data(), s->seq[o], n * sizeof(uint32_t), hipMemcpyDeviceToHost, s->st));
+    TQ_HIP(hipStreamSynchronize(s->st));
+  }
+  std::vector<uint32_t> idx;
+  idx.reserve(n);
+  for (size_t k = 0; k < n; k++) if (pos[k].x == pos[k].x) idx.push_back((uint32_t)k);
+  std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return sq[a] < sq[b]; });
+  out.resize(idx.size() * 12);
+  if (seqs) seqs->resize(idx.size());
+  for (size_t m = 0; m < idx.size(); m++) {
+    const uint32_t k = idx[m];
+    float* p = &out[m * 12];
+    p[0] = pos[k].x; p[1] = pos[k].y; p[2] = vel[k].x; p[3] = vel[k].y;
+    p[4] = cm[k].x; p[5] = cm[k].y; p[6] = cm[k].z; p[7] = cm[k].w;
+    p[8] = ext[k].x; p[9] = ext[k].y; p[10] = ext[k].z; p[11] = ext[k].w;
+    if (seqs) (*seqs)[m] = sq[k];
+  }
+  return 0;
+}
 }  // namespace tq
 
 extern "C" {
@@ -329,12 +414,14 @@ int tq_create(const tq_desc* d, tq_sim** out) {
 int tq_destroy(tq_sim* s) {
   if (!s) return 0;
   (void)hipStreamSynchronize(s->st);
+  tape_free(s);
   free_all(s);
   delete s;
   return 0;
 }
 
 int tq_set_sdf(tq_sim* s, const float* sdf, int row0, int row1) {
+  TQ_NOT_TAPING(s, "tq_set_sdf");
   TQ_CHECK_SIM(s);
   const int mw = 2 * s->nx, mh = 2 * s->ny;
   if (!sdf || row0 < 0 || row1 >= mh || row0 > row1) { set_error("tq_set_sdf: bad rows"); return TQ_E_ARG; }
@@ -345,6 +432,7 @@ int tq_set_sdf(tq_sim* s, const float* sdf, int row0, int row1) {
 }
 
 int tq_add_particles(tq_sim* s, const float* list, int count, int check, int* accepted) {
+  TQ_NOT_TAPING(s, "tq_add_particles");
   TQ_CHECK_SIM(s);
   if (count < 0 || (count > 0 && !list)) { set_error("tq_add_particles: bad list"); return TQ_E_ARG; }
   if (accepted) *accepted = 0;
@@ -372,37 +460,14 @@ int tq_add_particles(tq_sim* s, const float* list, int count, int check, int* ac
 int tq_emit_faucet(tq_sim* s, const tq_faucet* f, double dt, uint32_t* seed, int* emitted, int* accepted) {
   TQ_CHECK_SIM(s);
   if (!f || !seed) { set_error("tq_emit_faucet: null argument"); return TQ_E_ARG; }
-  if (emitted) *emitted = 0;
-  if (accepted) *accepted = 0;
-  // emitirTorneira's velocity, flow and count (principal.js:301-306)
-  double vx = f->pvx * 0.5, vy = f->pvy * 0.5 - 0.7;
-  const double v = js_hypot(vx, vy);
-  if (v > 4) { vx *= 4 / v; vy *= 4 / v; }
-  const double vazao = 2 * f->raio * std::max(0.7, js_hypot(vx, vy));
-  const double nd = std::min(40000.0, js_round((vazao * dt) / (s->h * s->h) * PPC));
-  const int n = nd > 0 ? (int)nd : 0;
-  if (n <= 0) return 0;
-  int keep = n;
-  if ((long long)s->n_upper + keep > s->cap) {
-    // reclaim dead slots first, then cap the list as MotorCPU does (motor.js:100-104)
-    if (int e = bin_and_sort(s)) return e;
-    if (int e = read_scalars(s)) return e;
-    s->n_upper = (int)s->sc_host->n_slots;
-    keep = std::max(0, std::min(keep, s->cap - s->n_upper));
-  }
-  if ((unsigned long long)s->seq_next + keep >= 0xFFFFFFF0ull) { set_error("tq_emit_faucet: insertion counter exhausted"); return TQ_E_CAPACITY; }
-  if (int e = faucet_dev(s, f, vx, vy, n, keep, seed)) return e;
-  s->seq_next += (uint32_t)keep;
-  s->n_upper += keep;
-  if (emitted) *emitted = n;
-  if (accepted) {
-    if (int e = read_scalars(s)) return e;
-    *accepted = keep > 0 ? (int)s->sc_host->add_count : 0;
-  }
+  const uint32_t seed0 = *seed, seq0 = s->seq_next;
+  if (int e = emit_faucet_impl(s, f, dt, seed, emitted, accepted)) return e;
+  if (s->tape) return tape_log_emit(s, f, dt, seed0, seq0, s->seq_next);
   return 0;
 }
 
 int tq_apply_tool(tq_sim* s, const tq_tool* t) {
+  TQ_NOT_TAPING(s, "tq_apply_tool");
   TQ_CHECK_SIM(s);
   if (!t) return 0;
   if (int e = apply_tool_dev(s, t)) return e;
@@ -410,6 +475,7 @@ int tq_apply_tool(tq_sim* s, const tq_tool* t) {
 }
 
 int tq_substep(tq_sim* s, double dt) {
+  TQ_NOT_TAPING(s, "tq_substep");
   TQ_CHECK_SIM(s);
   if (!(dt > 0)) { set_error("tq_substep: dt must be > 0"); return TQ_E_ARG; }
   if (int e = substep(s, dt)) return e;
@@ -418,22 +484,27 @@ int tq_substep(tq_sim* s, double dt) {
 
 int tq_step(tq_sim* s, double dt, int substeps) {
   if (int e = tq::step_enqueue(s, dt, substeps)) return e;
-  return sync(s);
+  if (int e = sync(s)) return e;
+  if (s->tape) return tape_log_step(s, dt, substeps);
+  return 0;
 }
 
 int tq_update_air(tq_sim* s) {
+  TQ_NOT_TAPING(s, "tq_update_air");
   TQ_CHECK_SIM(s);
   if (int e = update_air(s)) return e;
   return sync(s);
 }
 
 int tq_reset_air(tq_sim* s) {
+  TQ_NOT_TAPING(s, "tq_reset_air");
   TQ_CHECK_SIM(s);
   if (int e = reset_air(s)) return e;
   return sync(s);
 }
 
 int tq_clear_water(tq_sim* s) {
+  TQ_NOT_TAPING(s, "tq_clear_water");
   TQ_CHECK_SIM(s);
   const uint32_t zero = 0;
   TQ_HIP(hipMemcpyAsync(&s->sc->n_slots, &zero, sizeof zero, hipMemcpyHostToDevice, s->st));
@@ -447,39 +518,6 @@ int tq_count(tq_sim* s, int* n) {
   TQ_CHECK_SIM(s);
   if (!n) { set_error("tq_count: null"); return TQ_E_ARG; }
   return count_live(s, n);
-}
-
-// downloads the live particles ordered by insertion number (the reference's order)
-static int download_ordered(tq_sim* s, std::vector<float>& out, std::vector<uint32_t>* seqs) {
-  if (int e = read_scalars(s)) return e;
-  const size_t n = s->sc_host->n_slots;
-  const int o = s->cur;
-  std::vector<float2> pos(n), vel(n);
-  std::vector<float4> cm(n), ext(n);
-  std::vector<uint32_t> sq(n);
-  if (n) {
-    TQ_HIP(hipMemcpyAsync(pos.data(), s->pos[o], n * sizeof(float2), hipMemcpyDeviceToHost, s->st));
-    TQ_HIP(hipMemcpyAsync(vel.data(), s->vel[o], n * sizeof(float2), hipMemcpyDeviceToHost, s->st));
-    TQ_HIP(hipMemcpyAsync(cm.data(), s->cm[o], n * sizeof(float4), hipMemcpyDeviceToHost, s->st));
-    TQ_HIP(hipMemcpyAsync(ext.data(), s->ext[o], n * sizeof(float4), hipMemcpyDeviceToHost, s->st));
-    TQ_HIP(hipMemcpyAsync(sq.data(), s->seq[o], n * sizeof(uint32_t), hipMemcpyDeviceToHost, s->st));
-    TQ_HIP(hipStreamSynchronize(s->st));
-  }
-  std::vector<uint32_t> idx;
-  idx.reserve(n);
-  for (size_t k = 0; k < n; k++) if (pos[k].x == pos[k].x) idx.push_back((uint32_t)k);
-  std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return sq[a] < sq[b]; });
-  out.resize(idx.size() * 12);
-  if (seqs) seqs->resize(idx.size());
-  for (size_t m = 0; m < idx.size(); m++) {
-    const uint32_t k = idx[m];
-    float* p = &out[m * 12];
-    p[0] = pos[k].x; p[1] = pos[k].y; p[2] = vel[k].x; p[3] = vel[k].y;
-    p[4] = cm[k].x; p[5] = cm[k].y; p[6] = cm[k].z; p[7] = cm[k].w;
-    p[8] = ext[k].x; p[9] = ext[k].y; p[10] = ext[k].z; p[11] = ext[k].w;
-    if (seqs) (*seqs)[m] = sq[k];
-  }
-  return 0;
 }
 
 int tq_read_particles(tq_sim* s, float* dst, int max, int* n) {
@@ -497,6 +535,7 @@ int tq_read_particles(tq_sim* s, float* dst, int max, int* n) {
 }
 
 int tq_write_particles(tq_sim* s, const float* src, int n) {
+  TQ_NOT_TAPING(s, "tq_write_particles");
   TQ_CHECK_SIM(s);
   if (n < 0 || n > s->cap || (n > 0 && !src)) { set_error("tq_write_particles: bad count"); return TQ_E_ARG; }
   std::vector<float2> pos(n), vel(n);
@@ -524,6 +563,7 @@ int tq_write_particles(tq_sim* s, const float* src, int n) {
 }
 
 int tq_truncate(tq_sim* s, int m) {
+  TQ_NOT_TAPING(s, "tq_truncate");
   TQ_CHECK_SIM(s);
   if (m < 0) { set_error("tq_truncate: negative"); return TQ_E_ARG; }
   std::vector<float> all;
@@ -553,6 +593,7 @@ int tq_area(tq_sim* s, double* a) {
 }
 
 int tq_set_options(tq_sim* s, const tq_desc* d) {
+  TQ_NOT_TAPING(s, "tq_set_options");
   TQ_CHECK_SIM(s);
   if (!d) { set_error("tq_set_options: null"); return TQ_E_ARG; }
   if (d->precision < TQ_PREC_FP32 || d->precision > TQ_PREC_MIXED_X) {
@@ -581,6 +622,7 @@ int tq_read_field(tq_sim* s, int f, void* dst, size_t bytes) {
 }
 
 int tq_write_field(tq_sim* s, int f, const void* src, size_t bytes) {
+  TQ_NOT_TAPING(s, "tq_write_field");
   TQ_CHECK_SIM(s);
   FieldRef r;
   if (!field_ref(s, f, &r)) { set_error("tq_write_field: unknown field"); return TQ_E_ARG; }
@@ -605,6 +647,7 @@ int tq_write_field(tq_sim* s, int f, const void* src, size_t bytes) {
 }
 
 int tq_set_tracker_first(tq_sim* s, int primeira) {
+  TQ_NOT_TAPING(s, "tq_set_tracker_first");
   TQ_CHECK_SIM(s);
   const int v = primeira ? 1 : 0;
   TQ_HIP(hipMemcpyAsync(&s->sc->primeira, &v, sizeof v, hipMemcpyHostToDevice, s->st));
@@ -614,6 +657,7 @@ int tq_set_tracker_first(tq_sim* s, int primeira) {
 // Derived per-bubble counts the solver needs when a test starts from the solve phase: faces
 // and cells per label come from the bubble-rhs pass, which a parity test may have skipped.
 int tq_run_phase(tq_sim* s, int ph, double dt) {
+  TQ_NOT_TAPING(s, "tq_run_phase");
   TQ_CHECK_SIM(s);
   int e = 0;
   switch (ph) {

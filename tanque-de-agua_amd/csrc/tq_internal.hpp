@@ -106,6 +106,9 @@ struct Level {
   bool hx = false, hb = false;  // FP16 storage of x/tmp and of b on this level (this solve)
 };
 
+struct Tape;     // recorded frames of one tank (tq_adjoint.hip)
+struct SubRec;   // one substep's record for the reverse pass
+
 struct KTime {
   std::string name;
   double total_ms = 0;
@@ -217,6 +220,12 @@ struct tq_sim {
   uint32_t* scanBlocks = nullptr;
   int scan_cap = 0;             // entries of scanBlocks
 
+  // differentiation (tq_adjoint.hip)
+  std::vector<std::pair<void*, size_t>> bufs;  // alloc_all's buffers (pointer, bytes): what a checkpoint copies
+  tq::Tape* tape = nullptr;       // frames being recorded for tq_backward
+  tq::SubRec* rec = nullptr;      // the substep record the substep's hooks fill (null: none)
+  uint16_t* g2p_flags = nullptr;  // where G2P writes each particle's decision word (null: nowhere)
+
   // host-side state
   bool tipos_prontos = false;
   bool has_vmax = false;
@@ -282,5 +291,14 @@ void solver_free(tq_sim* s);
 int pressure_solve(tq_sim* s);
 int true_residual(tq_sim* s, double* out);
 int debug_vcycle(tq_sim* s, const float* b, float* z);
+// ---- tq_api.hip
+int download_ordered(tq_sim* s, std::vector<float>& out, std::vector<uint32_t>* seqs);
+int emit_faucet_impl(tq_sim* s, const tq_faucet* f, double dt, uint32_t* seed, int* emitted, int* accepted);
+// ---- tq_adjoint.hip: substep hooks while a record is open, the operation log, the tape's end
+constexpr int REC_P2G = 0, REC_SOLVE = 1, REC_EXTRAP = 2, REC_G2P = 3;
+int rec_hook(tq_sim* s, int point, double dt);
+int tape_log_emit(tq_sim* s, const tq_faucet* f, double dt, uint32_t seed, uint32_t seq0, uint32_t seq1);
+int tape_log_step(tq_sim* s, double dt, int substeps);
+void tape_free(tq_sim* s);
 
 }  // namespace tq

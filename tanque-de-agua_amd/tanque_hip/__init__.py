@@ -90,6 +90,16 @@ SIGNATURES = {
     "tq_set_tracker_first": (C.c_int, [C.c_void_p, C.c_int]),
     "tq_run_phase": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "tq_debug_vcycle": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tq_tape_begin": (C.c_int, [C.c_void_p, C.c_int]),
+    "tq_tape_end": (C.c_int, [C.c_void_p]),
+    "tq_tape_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                               C.POINTER(C.c_int)]),
+    "tq_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_void_p,
+                              C.c_void_p, C.c_int]),
+    "tq_water_fraction": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                    C.c_void_p, C.c_int]),
+    "tq_tape_records": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "tq_tape_record": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t]),
     "tq_solver_stats_get": (C.c_int, [C.c_void_p, C.POINTER(SolverStats), C.c_int]),
     "tq_timing_enable": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p]),
     "tq_timing_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int, C.POINTER(C.c_int)]),
@@ -267,6 +277,74 @@ class SimulacaoHIP:
 
     def avancar(self, dt, subpassos=1):
         _chk(load().tq_step(self._h, dt, subpassos))
+
+    # ---- differentiation of recorded frames
+    def tape_begin(self, checkpoint_every=10):
+        """Records the frames run from now on (emitirTorneira and avancar calls) for backward();
+        checkpoint_every = 0 keeps every substep's record instead of checkpoints."""
+        _chk(load().tq_tape_begin(self._h, checkpoint_every))
+
+    def tape_end(self):
+        _chk(load().tq_tape_end(self._h))
+
+    def tape_info(self):
+        f, e, c, n = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _chk(load().tq_tape_info(self._h, C.byref(f), C.byref(e), C.byref(c), C.byref(n)))
+        return {"frames": f.value, "faucet_calls": e.value, "checkpoints": c.value, "n_begin": n.value}
+
+    def backward(self, grad_state, tol=1e-6, sdf=True, state0=True):
+        """dL/d(inputs) of the recorded frames from dL/d(final state) (n x 8: x y vx vy cux cuy cvx
+        cvy per live particle, particulas() order): returns dict(faucet=(calls x 2) jet velocity
+        gradients, sdf=(4 nx ny) or None, state0=(n0 x 8) or None)."""
+        g = np.ascontiguousarray(np.asarray(grad_state, np.float32).reshape(-1, 8))
+        info = self.tape_info()
+        gf = np.zeros(2 * max(1, info["faucet_calls"]))
+        gs = np.zeros(4 * self.cena.nx * self.cena.ny, np.float32) if sdf else None
+        g0 = np.zeros(8 * max(1, info["n_begin"]), np.float32) if state0 else None
+        _chk(load().tq_backward(self._h, g.ctypes.data, len(g), tol, gf.ctypes.data, info["faucet_calls"],
+                                gs.ctypes.data if sdf else None, g0.ctypes.data if state0 else None,
+                                info["n_begin"]))
+        return {"faucet": gf[: 2 * info["faucet_calls"]].reshape(-1, 2), "sdf": gs,
+                "state0": g0[: 8 * info["n_begin"]].reshape(-1, 8) if state0 else None}
+
+    def water_fraction(self, box, grad=False):
+        """Mean of min(1, densidade/PPC) over the cell box (i0, j0, i1, j1); with grad, also its
+        gradient with respect to the state (n x 8, particulas() order)."""
+        f = C.c_double()
+        if not grad:
+            _chk(load().tq_water_fraction(self._h, *[int(v) for v in box], C.byref(f), None, 0))
+            return f.value
+        n = self.n
+        g = np.zeros((max(n, 1), 8), np.float32)
+        _chk(load().tq_water_fraction(self._h, *[int(v) for v in box], C.byref(f), g.ctypes.data, n))
+        return f.value, g[:n]
+
+    def tape_records(self):
+        n = C.c_int()
+        _chk(load().tq_tape_records(self._h, C.byref(n)))
+        return n.value
+
+    def tape_record(self, k):
+        """Test hook: record k of a keep-all tape as a dict of arrays."""
+        L = load()
+        nx, ny = self.cena.nx, self.cena.ny
+        out = {}
+        cnt = np.zeros(1, np.uint32)
+        _chk(L.tq_tape_record(self._h, k, 0, cnt.ctypes.data, 4))
+        n = int(cnt[0])
+        spec = {"tipo": (1, np.uint8, nx * ny), "rot": (2, np.uint32, nx * ny), "qAr": (3, np.float32, nx * ny),
+                "volAlvo": (4, np.float32, 4096), "yB": (5, np.float32, 4096), "flags": (6, np.uint16, n),
+                "seq": (7, np.uint32, n)}
+        for name, (fid, dt, m) in spec.items():
+            a = np.zeros(max(m, 1), dt)
+            if m:
+                _chk(L.tq_tape_record(self._h, k, fid, a.ctypes.data, a.itemsize * m))
+            out[name] = a[:m]
+        d = np.zeros(1, np.float64)
+        _chk(L.tq_tape_record(self._h, k, 8, d.ctypes.data, 8))
+        out["dt"] = float(d[0])
+        out["n"] = n
+        return out
 
     def subpasso(self, dt):
         _chk(load().tq_substep(self._h, dt))

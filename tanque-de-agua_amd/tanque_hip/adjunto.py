@@ -1,0 +1,58 @@
+"""The adjoint workload on the HIP path: the funnel run forward with its frames recorded, the
+loss (water fraction of the bottom-right box - 0.5)^2, and its gradient with respect to the
+faucet jet velocity and the funnel's 32 wall heights (tq_tape_begin / tq_backward).
+
+The wall heights act through the SDF (the walls' push-out of particles, cena.js:102-117); the
+cell types the SDF gives are held with the other discrete decisions of the recorded run."""
+import numpy as np
+
+from . import Cena, Semente, SimulacaoHIP
+from .scenes import FUNNEL_SEGMENTS, passo_frame, stroke_shift_vjp
+
+
+def run_funnel(work, checkpoint_every=10, tol=1e-6, solve_tol=1e-6, maxit=200, keep_emitted=False, grad=True,
+               sim_options=None):
+    """Runs the funnel workload (scenes.funnel) forward and, with grad, backward.  Returns a dict:
+    loss, frac, g_vjet (2), g_heights (32), g_sdf, the sim (left in its end state, tape open),
+    the substep schedule, and with keep_emitted the emitted particles' positions per frame."""
+    nx, ny, h = work.nx, work.ny, work.h
+    cena = Cena(nx, ny, h)
+    for s in work.strokes:
+        cena.tracar(*s)
+    opts = dict(iteracoes=maxit, tol=solve_tol)
+    opts.update(sim_options or {})
+    sim = SimulacaoHIP(cena, **opts)
+    if len(work.particles):
+        sim.adicionar(np.asarray(work.particles, np.float32), False)
+    f = work.extra["faucet"]
+    sem = Semente(f["seed"])
+    if grad:
+        sim.tape_begin(checkpoint_every)
+    H = ny * h
+    schedule, emitted = [], []
+    for k in range(work.frames):
+        if k < f["frames"]:
+            n0 = sim.n
+            sim.emitirTorneira(f["x"], f["y"], f["pvx"], f["pvy"], f["raio"], f["absorcao"], work.dt_frame, sem)
+            if keep_emitted:
+                emitted.append(sim.particulas()[n0:, :2].astype(np.float64))
+        elif keep_emitted:
+            emitted.append(np.zeros((0, 2)))
+        nsub = passo_frame(h, H, sim.vMax, work.dt_frame)
+        schedule.append(nsub)
+        sim.avancar(work.dt_frame, nsub)
+    box = work.extra["box"]
+    target = work.extra["target"]
+    out = {"sim": sim, "schedule": schedule, "emitted": emitted, "cena": cena}
+    if not grad:
+        frac = sim.water_fraction(box)
+        out.update(loss=(frac - target) ** 2, frac=frac)
+        return out
+    frac, g = sim.water_fraction(box, grad=True)
+    loss = (frac - target) ** 2
+    res = sim.backward(2 * (frac - target) * g, tol=tol)
+    g_sdf = res["sdf"].astype(np.float64)
+    g_heights = stroke_shift_vjp(nx, ny, h, work.strokes, g_sdf, list(range(2 * FUNNEL_SEGMENTS)))
+    out.update(loss=loss, frac=frac, g_vjet=res["faucet"].sum(axis=0), g_faucet_calls=res["faucet"],
+               g_heights=g_heights, g_sdf=g_sdf, g_state0=res["state0"])
+    return out

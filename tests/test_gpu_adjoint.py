@@ -1,0 +1,117 @@
+"""tq_backward on the GPU against the FP64 replay adjoint of the oracle on the GPU's own tape.
+
+The GPU run records its decisions (keep-all tape); the oracle replays the same decisions in
+FP64 (exact pressure solves) from the same inputs and reverses them.  The two gradients differ
+by the FP32 forward (MGPCG to 1e-6) against the FP64 replay: the tolerances below are stated
+for that.  Checkpointed recording (re-running frames between checkpoints) must give the
+keep-all gradient, and backward() must leave the tank in its end state.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tanque-de-agua_amd"))
+
+pytestmark = pytest.mark.gpu
+
+NX, NY = 96, 54
+BOX = (30, 2, 94, 52)         # most of the 96x54 tank: the funnel, its spout and the water below
+FRAMES = 80                   # long enough for water to slide along the funnel walls
+RTOL_JET = 2e-3               # jet-velocity gradient, GPU (FP32 forward) vs FP64 replay
+RTOL_SDF = 2e-2               # SDF gradient vector, relative L2
+
+
+def small_funnel(frames):
+    from tanque_hip import scenes
+    w = scenes.funnel(nx=NX, ny=NY, frames=frames)
+    w.extra["box"] = BOX
+    return w
+
+
+def oracle_tape(sim, work, out):
+    """The GPU run's tape in the oracle's layout (particles in insertion order)."""
+    from oracle import adjoint as A
+    h = work.h
+    T = A.Tape(NX, NY, h)
+    for f, nsub in enumerate(out["schedule"]):
+        T.add_frame([tuple(p) for p in out["emitted"][f]], nsub, work.dt_frame)
+    nps, flags = [], []
+    for k in range(sim.tape_records()):
+        r = sim.tape_record(k)
+        bub, qextra, rhsfix = A.bubble_fields(NX, NY, h, r["dt"], r["tipo"], r["rot"], r["qAr"], r["volAlvo"], r["yB"])
+        T.add_substep(r["tipo"], bub, r["qAr"], qextra, rhsfix)
+        o = np.argsort(r["seq"], kind="stable")
+        nps.append(r["n"])
+        flags.append(r["flags"][o])
+    T.np = np.asarray(nps, np.int32)
+    T.flags = np.concatenate(flags).astype(np.uint16)
+    return T
+
+
+def vjet_of(work):
+    f = work.extra["faucet"]
+    return np.array([0.5 * f["pvx"], 0.5 * f["pvy"] - 0.7])
+
+
+@pytest.fixture(scope="module")
+def keep_all():
+    from tanque_hip.adjunto import run_funnel
+    w = small_funnel(FRAMES)
+    print("adjoint: funnel %dx%d, %d frames, keep-all tape" % (NX, NY, FRAMES), flush=True)
+    out = run_funnel(w, checkpoint_every=0, keep_emitted=True)
+    return w, out
+
+
+def test_backward_matches_fp64_replay_adjoint(keep_all):
+    from oracle import adjoint as A
+    w, out = keep_all
+    sim = out["sim"]
+    T = oracle_tape(sim, w, out)
+    sdf = np.asarray(out["cena"].sdf, np.float64)
+    r = A.run(T, sdf, np.zeros((0, 8)), vjet_of(w), BOX, target=w.extra["target"])
+    print("gpu: loss %.9g frac %.9g g_vjet %s" % (out["loss"], out["frac"], out["g_vjet"]))
+    print("fp64 replay: loss %.9g frac %.9g g_vjet %s" % (r["loss"], r["frac"], r["g_vjet"]))
+    assert 0.02 < out["frac"] < 0.98
+    assert abs(out["frac"] - r["frac"]) < 1e-4
+    for k in range(2):
+        assert abs(out["g_vjet"][k] - r["g_vjet"][k]) <= RTOL_JET * np.abs(r["g_vjet"]).max(), (k, out["g_vjet"], r["g_vjet"])
+    e = np.linalg.norm(out["g_sdf"] - r["g_sdf"]) / max(np.linalg.norm(r["g_sdf"]), 1e-300)
+    print("sdf gradient relative L2 error %.3g (|g| %.3g)" % (e, np.linalg.norm(r["g_sdf"])))
+    assert np.linalg.norm(r["g_sdf"]) > 0
+    assert e < RTOL_SDF
+
+
+def test_backward_leaves_end_state(keep_all):
+    w, out = keep_all
+    sim = out["sim"]
+    P0 = sim.particulas().copy()
+    frac, g = sim.water_fraction(BOX, grad=True)
+    sim.backward(g, tol=1e-6)
+    assert np.array_equal(sim.particulas(), P0)
+    assert sim.water_fraction(BOX) == frac
+
+
+def test_checkpointed_backward_equals_keep_all(keep_all):
+    from tanque_hip.adjunto import run_funnel
+    w, out = keep_all
+    print("adjoint: the same run with checkpoints every 10 frames", flush=True)
+    cp = run_funnel(w, checkpoint_every=10)
+    assert cp["sim"].tape_info()["checkpoints"] == FRAMES // 10 + 1
+    assert cp["frac"] == out["frac"]
+    np.testing.assert_allclose(cp["g_vjet"], out["g_vjet"], rtol=1e-4, atol=1e-4 * np.abs(out["g_vjet"]).max())
+    n = np.linalg.norm(out["g_sdf"])
+    assert np.linalg.norm(cp["g_sdf"] - out["g_sdf"]) <= 1e-4 * max(n, 1e-30)
+
+
+def test_recording_refuses_other_state_changes(keep_all):
+    import tanque_hip as tq
+    w, out = keep_all
+    sim = out["sim"]
+    with pytest.raises(tq.TqError):
+        sim.aplicarFerramenta({"tipo": "mao", "x": 0.5, "y": 0.5, "raio": 0.05})
+    sim.tape_end()
+    sim.aplicarFerramenta({"tipo": "mao", "x": 0.5, "y": 0.5, "raio": 0.05})
