@@ -24,6 +24,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "tanque-de-agua_amd"))
 sys.path.insert(0, ROOT)
@@ -157,6 +159,55 @@ def run_batched(tq, frames=20, threads=8):
     return res
 
 
+def run_adjoint(tq, fd_h=1e-3, checkpoint_every=10):
+    """The adjoint configuration (funnel 256x144, 120 frames of 1/240 s, faucet jet + 32 wall
+    heights, loss (box water fraction - 0.5)^2): the forward alone, the forward recorded with
+    checkpoints every 10 frames, the backward (recompute between checkpoints + reverse kernels),
+    and central differences of the plain forward at step fd_h for the two jet components and the
+    heights of the two wall segments at the spout."""
+    from tanque_hip import scenes
+    from tanque_hip.adjunto import run_funnel
+    w = scenes.funnel()
+    plain = run_funnel(w, grad=False)
+    subs = int(sum(plain["schedule"]))
+    plain["sim"].close()
+    out = run_funnel(w, checkpoint_every=checkpoint_every)
+    n_end = out["sim"].n
+    info = out["sim"].tape_info()
+    out["sim"].close()
+    res = {"workload": w.description + ", loss box %s, target 0.5" % (w.extra["box"],),
+           "frames": w.frames, "substeps": subs, "particles_end": n_end, "checkpoints": info["checkpoints"],
+           "forward_s": round(plain["t_forward"], 4), "forward_substeps_per_s": round(subs / plain["t_forward"], 1),
+           "recorded_forward_s": round(out["t_forward"], 4), "backward_s": round(out["t_backward"], 4),
+           "backward_over_forward": round(out["t_backward"] / plain["t_forward"], 3),
+           "loss": out["loss"], "frac": out["frac"],
+           "grad_jet": [float(v) for v in out["g_vjet"]],
+           "grad_heights_norm": float(np.linalg.norm(out["g_heights"])),
+           "grad_heights_first4": [float(v) for v in out["g_heights"][:4]]}
+    fd = {}
+    v0 = np.array([0.5 * w.extra["faucet"]["pvx"], 0.5 * w.extra["faucet"]["pvy"] - 0.7])
+    for k, name in ((0, "jet_vx"), (1, "jet_vy")):
+        L = []
+        for sg in (1, -1):
+            v = v0.copy()
+            v[k] += sg * fd_h
+            r = run_funnel(scenes.funnel(vjet=tuple(v)), grad=False)
+            r["sim"].close()
+            L.append(r["loss"])
+        fd[name] = {"fd": (L[0] - L[1]) / (2 * fd_h), "adjoint": float(out["g_vjet"][k])}
+    for k in (15, 31):
+        L = []
+        for sg in (1, -1):
+            th = np.zeros(2 * scenes.FUNNEL_SEGMENTS)
+            th[k] = sg * fd_h
+            r = run_funnel(scenes.funnel(heights=th), grad=False)
+            r["sim"].close()
+            L.append(r["loss"])
+        fd["height_%d" % k] = {"fd": (L[0] - L[1]) / (2 * fd_h), "adjoint": float(out["g_heights"][k])}
+    res["central_differences"] = {"h": fd_h, "of": "the plain forward (every decision free)", "values": fd}
+    return res
+
+
 def dominant_kernel(tq, work, precision, frames=3):
     """Name of the kernel family with the largest device time on a few frames of the workload."""
     from tanque_hip.scenes import passo_frame
@@ -240,6 +291,7 @@ def main():
     ap.add_argument("--no-variants", action="store_true", help="skip the partial-FP16 variants")
     ap.add_argument("--no-alto", action="store_true", help="skip the Alto (cup + faucet) runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (64 tanks) runs")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (funnel) runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="start with a 256 MiB device copy (reference for rocprofv3 --pmc byte counters)")
     ap.add_argument("--nx", type=int, default=896)
@@ -321,6 +373,11 @@ def main():
         batched = run_batched(tq)
         log("batched: %s" % batched)
 
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = run_adjoint(tq)
+        log("adjoint: %s" % adjoint)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(work, args.cpu_budget)
@@ -353,6 +410,7 @@ def main():
         "mixed_variants": variants or None,
         "alto": alto,
         "batched": batched,
+        "adjoint": adjoint,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -6,6 +6,7 @@
 #define NAPI_VERSION 4
 #include <node_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -261,6 +262,79 @@ napi_value m_faucet(napi_env env, napi_callback_info info) {
   return r;
 }
 
+// gravar(checkpointACada): start recording frames for retropropagar (tq_tape_begin); fimGravacao()
+napi_value m_tape_begin(napi_env env, napi_callback_info info) {
+  napi_value argv[1], self;
+  if (!get_args(env, info, 1, argv, &self)) return nullptr;
+  Native* n = unwrap(env, self);
+  if (!n) return nullptr;
+  double every = 10;
+  if (!is_nullish(env, argv[0]) && !num(env, argv[0], &every)) return nullptr;
+  if (!throw_tq(env, tq_tape_begin(n->s, (int)every), "tq_tape_begin")) return nullptr;
+  return undefined(env);
+}
+
+// fracaoAgua(i0, j0, i1, j1, comGradiente) -> {fracao, gradiente: Float32Array n*8 | undefined}
+napi_value m_fraction(napi_env env, napi_callback_info info) {
+  napi_value argv[5], self;
+  if (!get_args(env, info, 5, argv, &self)) return nullptr;
+  Native* n = unwrap(env, self);
+  if (!n) return nullptr;
+  double b[4];
+  for (int k = 0; k < 4; k++)
+    if (!num(env, argv[k], &b[k])) return nullptr;
+  bool want = false;
+  if (!is_nullish(env, argv[4])) napi_get_value_bool(env, argv[4], &want);
+  double frac = 0;
+  napi_value r;
+  napi_create_object(env, &r);
+  if (want) {
+    int c = 0;
+    if (!throw_tq(env, tq_count(n->s, &c), "tq_count")) return nullptr;
+    void* data = nullptr;
+    napi_value g = make_typed(env, napi_float32_array, (size_t)c * 8, 4, &data);
+    if (!throw_tq(env, tq_water_fraction(n->s, (int)b[0], (int)b[1], (int)b[2], (int)b[3], &frac, static_cast<float*>(data), c),
+                  "tq_water_fraction"))
+      return nullptr;
+    napi_set_named_property(env, r, "gradiente", g);
+  } else if (!throw_tq(env, tq_water_fraction(n->s, (int)b[0], (int)b[1], (int)b[2], (int)b[3], &frac, nullptr, 0),
+                       "tq_water_fraction")) {
+    return nullptr;
+  }
+  napi_set_named_property(env, r, "fracao", number(env, frac));
+  return r;
+}
+
+// retropropagar(gradEstado: Float32Array n*8, tol) -> {torneira: Float64Array 2 per faucet call,
+// sdf: Float32Array, estadoInicial: Float32Array n0*8} (tq_backward)
+napi_value m_backward(napi_env env, napi_callback_info info) {
+  napi_value argv[2], self;
+  if (!get_args(env, info, 2, argv, &self)) return nullptr;
+  Native* n = unwrap(env, self);
+  if (!n) return nullptr;
+  float* g = nullptr;
+  size_t len = 0;
+  if (!f32_array(env, argv[0], &g, &len)) return nullptr;
+  double tol = 1e-6;
+  if (!is_nullish(env, argv[1]) && !num(env, argv[1], &tol)) return nullptr;
+  int frames = 0, calls = 0, cps = 0, n0 = 0;
+  if (!throw_tq(env, tq_tape_info(n->s, &frames, &calls, &cps, &n0), "tq_tape_info")) return nullptr;
+  void *pf = nullptr, *ps = nullptr, *p0 = nullptr;
+  napi_value tf = make_typed(env, napi_float64_array, (size_t)std::max(calls, 0) * 2, 8, &pf);
+  napi_value ts = make_typed(env, napi_float32_array, (size_t)4 * n->nx * n->ny, 4, &ps);
+  napi_value t0 = make_typed(env, napi_float32_array, (size_t)n0 * 8, 4, &p0);
+  if (!throw_tq(env, tq_backward(n->s, g, (int)(len / 8), tol, static_cast<double*>(pf), calls, static_cast<float*>(ps),
+                                 static_cast<float*>(p0), n0),
+                "tq_backward"))
+    return nullptr;
+  napi_value r;
+  napi_create_object(env, &r);
+  napi_set_named_property(env, r, "torneira", tf);
+  napi_set_named_property(env, r, "sdf", ts);
+  napi_set_named_property(env, r, "estadoInicial", t0);
+  return r;
+}
+
 // avancar(dt, subpassos) (fisica-cpu.js:155-159)
 napi_value m_step(napi_env env, napi_callback_info info) {
   napi_value argv[2], self;
@@ -510,6 +584,10 @@ napi_value init(napi_env env, napi_value exports) {
       {"aplicarFerramenta", nullptr, m_tool, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"emitirTorneira", nullptr, m_faucet, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"avancar", nullptr, m_step, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"gravar", nullptr, m_tape_begin, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"fimGravacao", nullptr, m_simple<tq_tape_end>, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"fracaoAgua", nullptr, m_fraction, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"retropropagar", nullptr, m_backward, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"subpasso", nullptr, m_substep, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"atualizarAr", nullptr, m_simple<tq_update_air>, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"reiniciarAr", nullptr, m_simple<tq_reset_air>, nullptr, nullptr, nullptr, napi_default, nullptr},

@@ -123,11 +123,17 @@ int tq_batch_step(tq_batch* b, double dt, const int* substeps) {
     tq::set_error("tq_batch_step: bad arguments");
     return TQ_E_ARG;
   }
-  for (size_t k = 0; k < b->tanks.size(); k++)
+  for (size_t k = 0; k < b->tanks.size(); k++) {
     if (substeps[k] < 0) {
       tq::set_error("tq_batch_step: negative substep count");
       return TQ_E_ARG;
     }
+    if (b->tanks[k]->tape) {
+      // a recorded tank's frames go through tq_step, which logs them
+      tq::set_error("tq_batch_step: a tank is recording frames (tq_tape_begin)");
+      return TQ_E_STATE;
+    }
+  }
   std::unique_lock<std::mutex> lk(b->mu);
   b->dt = dt;
   b->subs.assign(substeps, substeps + b->tanks.size());

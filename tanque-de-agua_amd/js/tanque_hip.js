@@ -93,6 +93,17 @@ class SimulacaoHIP {
   }
   /** avancar (fisica-cpu.js:155-159). */
   avancar(dt, subpassos = 1) { this._n.avancar(dt, subpassos); }
+  /** Records the frames run from now on (emitirTorneira and avancar) for retropropagar: the
+   *  state is kept every checkpointACada frames and re-run in between (0: every substep kept). */
+  gravar(checkpointACada = 10) { this._n.gravar(checkpointACada); }
+  fimGravacao() { this._n.fimGravacao(); }
+  /** Water fraction of the cell box [i0,i1) x [j0,j1) (mean of min(1, densidade/4)); with
+   *  comGradiente, also d(fraction)/d(state) as a Float32Array of n*8 (x y vx vy cux cuy cvx cvy). */
+  fracaoAgua(i0, j0, i1, j1, comGradiente = false) { return this._n.fracaoAgua(i0, j0, i1, j1, comGradiente); }
+  /** The backward of the recorded frames: from dL/d(final state) (Float32Array n*8) to
+   *  {torneira: Float64Array of dL/d(jet vx, vy) per emitirTorneira call, sdf: Float32Array
+   *  dL/d(SDF), estadoInicial: Float32Array dL/d(state when gravar was called)}. */
+  retropropagar(gradEstado, tol = 1e-6) { return this._n.retropropagar(gradEstado, tol); }
   /** subpasso (fisica-cpu.js:176-189). */
   subpasso(dt) { this._n.subpasso(dt); }
   /** atualizarAr (fisica-cpu.js:162-167). */

@@ -4,6 +4,8 @@ faucet jet velocity and the funnel's 32 wall heights (tq_tape_begin / tq_backwar
 
 The wall heights act through the SDF (the walls' push-out of particles, cena.js:102-117); the
 cell types the SDF gives are held with the other discrete decisions of the recorded run."""
+import time
+
 import numpy as np
 
 from . import Cena, Semente, SimulacaoHIP
@@ -30,6 +32,7 @@ def run_funnel(work, checkpoint_every=10, tol=1e-6, solve_tol=1e-6, maxit=200, k
         sim.tape_begin(checkpoint_every)
     H = ny * h
     schedule, emitted = [], []
+    t0 = time.perf_counter()
     for k in range(work.frames):
         if k < f["frames"]:
             n0 = sim.n
@@ -41,16 +44,20 @@ def run_funnel(work, checkpoint_every=10, tol=1e-6, solve_tol=1e-6, maxit=200, k
         nsub = passo_frame(h, H, sim.vMax, work.dt_frame)
         schedule.append(nsub)
         sim.avancar(work.dt_frame, nsub)
+    sim.sync()
+    t_forward = time.perf_counter() - t0
     box = work.extra["box"]
     target = work.extra["target"]
-    out = {"sim": sim, "schedule": schedule, "emitted": emitted, "cena": cena}
+    out = {"sim": sim, "schedule": schedule, "emitted": emitted, "cena": cena, "t_forward": t_forward}
     if not grad:
         frac = sim.water_fraction(box)
         out.update(loss=(frac - target) ** 2, frac=frac)
         return out
     frac, g = sim.water_fraction(box, grad=True)
     loss = (frac - target) ** 2
+    t0 = time.perf_counter()
     res = sim.backward(2 * (frac - target) * g, tol=tol)
+    out["t_backward"] = time.perf_counter() - t0
     g_sdf = res["sdf"].astype(np.float64)
     g_heights = stroke_shift_vjp(nx, ny, h, work.strokes, g_sdf, list(range(2 * FUNNEL_SEGMENTS)))
     out.update(loss=loss, frac=frac, g_vjet=res["faucet"].sum(axis=0), g_faucet_calls=res["faucet"],

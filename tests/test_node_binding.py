@@ -61,3 +61,34 @@ def test_motor_hip_in_node_matches_ctypes_run():
     assert d["n"] == len(P)
     assert np.array_equal(d["P"], P)
     assert d["vMax"] == sim.vMax
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_backward_matches_ctypes_backward():
+    """The funnel's gradient through the addon (gravar, fracaoAgua, retropropagar) equals the
+    ctypes run's: same library and decisions.  The reverse pass's FP64 atomic sums are taken in
+    no fixed order and the adjoint solve's right-hand side is stored in FP32, so the two differ
+    by a few FP32 roundings (1e-5 relative here)."""
+    import tanque_hip as tq
+    from tanque_hip import scenes
+    from tanque_hip.adjunto import run_funnel
+    tq.load()
+    w = scenes.funnel(nx=96, ny=54, frames=40)
+    box = (30, 2, 94, 52)
+    w.extra["box"] = box
+    f = w.extra["faucet"]
+    cfg = {"nx": w.nx, "ny": w.ny, "h": w.h, "strokes": [list(s) for s in w.strokes], "frames": w.frames,
+           "dt": w.dt_frame, "box": list(box), "every": 10,
+           "faucet": {"x": f["x"], "y": f["y"], "pvx": f["pvx"], "pvy": f["pvy"], "raio": f["raio"], "seed": f["seed"]}}
+    script = os.path.join(ROOT, "tests", "js", "run_funnel_adjoint.js")
+    r = subprocess.run(["node", script, json.dumps(cfg)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    out = run_funnel(w, checkpoint_every=10)
+    print("\n  node: frac %.9g jet %s; ctypes: frac %.9g jet %s" % (d["fracao"], d["jato"], out["frac"], out["g_vjet"]))
+    assert d["fracao"] == out["frac"]
+    assert d["chamadas"] == w.frames
+    np.testing.assert_allclose(d["jato"], out["g_vjet"], rtol=1e-5, atol=1e-5 * np.abs(out["g_vjet"]).max())
+    gs = np.frombuffer(base64.b64decode(d["sdf"]), np.float32)
+    assert np.linalg.norm(gs - out["g_sdf"]) <= 1e-5 * np.linalg.norm(out["g_sdf"]) + 1e-30
