@@ -159,7 +159,7 @@ def run_batched(tq, frames=20, threads=8):
     return res
 
 
-def run_adjoint(tq, fd_h=1e-3, checkpoint_every=10):
+def run_adjoint(tq, fd_h=1e-3, checkpoint_every=10, fd=True):
     """The adjoint configuration (funnel 256x144, 120 frames of 1/240 s, faucet jet + 32 wall
     heights, loss (box water fraction - 0.5)^2): the forward alone, the forward recorded with
     checkpoints every 10 frames, the backward (recompute between checkpoints + reverse kernels),
@@ -184,6 +184,8 @@ def run_adjoint(tq, fd_h=1e-3, checkpoint_every=10):
            "grad_jet": [float(v) for v in out["g_vjet"]],
            "grad_heights_norm": float(np.linalg.norm(out["g_heights"])),
            "grad_heights_first4": [float(v) for v in out["g_heights"][:4]]}
+    if not fd:
+        return res
     fd = {}
     v0 = np.array([0.5 * w.extra["faucet"]["pvx"], 0.5 * w.extra["faucet"]["pvy"] - 0.7])
     for k, name in ((0, "jet_vx"), (1, "jet_vy")):
@@ -292,6 +294,8 @@ def main():
     ap.add_argument("--no-alto", action="store_true", help="skip the Alto (cup + faucet) runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (64 tanks) runs")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (funnel) runs")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the adjoint configuration's forward and backward (for kernel profiles); prints its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="start with a 256 MiB device copy (reference for rocprofv3 --pmc byte counters)")
     ap.add_argument("--nx", type=int, default=896)
@@ -304,6 +308,9 @@ def main():
     import tanque_hip as tq
     from tanque_hip import scenes
     tq.load()
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": run_adjoint(tq, fd=False)}), flush=True)
+        return
     if args.copy_calibration:
         copy_calibration()
     work = scenes.ultra_hourglass() if (args.nx, args.ny) == (896, 504) else scenes.scaled_hourglass(args.nx, args.ny)
