@@ -233,6 +233,18 @@ const char* tq_last_error(void);
 /* ---------------------------------------------------------------- host helpers
  * Scene construction (js/cena.js) and seeding (fisica-cpu.js:705-727, testes/apoio.js:4-9),
  * on caller-owned host arrays of (2nx)*(2ny) floats. */
+/* The tank's walls built on the device from capsule strokes (n x 6 doubles: x0 y0 x1 y1 raio
+ * apagar, metres, applied in order): the frame, every stroke as Cena.tracar applies it
+ * (cena.js:43-72, binary32 store after each), then the solid cells -- Cena.refazer (cena.js:87-91)
+ * followed by atualizarParedes (fisica-cpu.js:80-85), bit-identical to tq_scene_stroke. */
+int tq_set_strokes(tq_sim* s, const double* strokes, int n);
+/* Chain rule of the same construction: out[5k..5k+4] = sum over lattice points of gsdf times
+ * d(sdf)/d(x0, y0, x1, y1, raio) of stroke k, the point's value being that of the last stroke that
+ * changed it (points set by the frame or clamped to +-10 cells have none).  For the adjoint's wall
+ * parameters: tq_backward's SDF gradient -> stroke gradients. */
+int tq_strokes_vjp(tq_sim* s, const double* strokes, int n, const float* gsdf, double* out);
+/* The tank's SDF lattice as the device holds it (4*nx*ny floats; Cena.sdf, cena.js:24). */
+int tq_read_sdf(tq_sim* s, float* dst, size_t bytes);
 void tq_scene_frame(float* sdf, int nx, int ny);                      /* Cena #zerar (cena.js:139-144) */
 void tq_scene_stroke(float* sdf, int nx, int ny, double h, double x0, double y0, double x1, double y1,
                      double raio, int apagar);                        /* Cena.tracar (cena.js:43-72) */

@@ -262,6 +262,54 @@ napi_value m_faucet(napi_env env, napi_callback_info info) {
   return r;
 }
 
+// a Float64Array argument: data and element count
+bool f64_array(napi_env env, napi_value v, double** data, size_t* n) {
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  if (!is) { throw_arg(env, "Float64Array expected"); return false; }
+  napi_typedarray_type t;
+  void* p = nullptr;
+  napi_value ab;
+  size_t off = 0;
+  napi_get_typedarray_info(env, v, &t, n, &p, &ab, &off);
+  if (t != napi_float64_array) { throw_arg(env, "Float64Array expected"); return false; }
+  *data = static_cast<double*>(p);
+  return true;
+}
+
+// definirTracos(Float64Array n*6: x0 y0 x1 y1 raio apagar): the walls built on the device
+// (Cena.refazer + atualizarParedes, tq_set_strokes)
+napi_value m_set_strokes(napi_env env, napi_callback_info info) {
+  napi_value argv[1], self;
+  if (!get_args(env, info, 1, argv, &self)) return nullptr;
+  Native* n = unwrap(env, self);
+  if (!n) return nullptr;
+  double* d = nullptr;
+  size_t len = 0;
+  if (!f64_array(env, argv[0], &d, &len)) return nullptr;
+  if (len % 6) { throw_arg(env, "6 numbers per stroke expected"); return nullptr; }
+  if (!throw_tq(env, tq_set_strokes(n->s, d, (int)(len / 6)), "tq_set_strokes")) return nullptr;
+  return undefined(env);
+}
+
+// tracosVjp(Float64Array strokes, Float32Array gsdf) -> Float64Array n*5 (tq_strokes_vjp)
+napi_value m_strokes_vjp(napi_env env, napi_callback_info info) {
+  napi_value argv[2], self;
+  if (!get_args(env, info, 2, argv, &self)) return nullptr;
+  Native* n = unwrap(env, self);
+  if (!n) return nullptr;
+  double* d = nullptr;
+  size_t len = 0, glen = 0;
+  float* g = nullptr;
+  if (!f64_array(env, argv[0], &d, &len) || !f32_array(env, argv[1], &g, &glen)) return nullptr;
+  if (len % 6 || glen != (size_t)4 * n->nx * n->ny) { throw_arg(env, "strokes n*6 and gsdf 4*nx*ny expected"); return nullptr; }
+  void* out = nullptr;
+  napi_value arr = make_typed(env, napi_float64_array, (len / 6) * 5, 8, &out);
+  if (!throw_tq(env, tq_strokes_vjp(n->s, d, (int)(len / 6), g, static_cast<double*>(out)), "tq_strokes_vjp"))
+    return nullptr;
+  return arr;
+}
+
 // gravar(checkpointACada): start recording frames for retropropagar (tq_tape_begin); fimGravacao()
 napi_value m_tape_begin(napi_env env, napi_callback_info info) {
   napi_value argv[1], self;
@@ -585,6 +633,8 @@ napi_value init(napi_env env, napi_value exports) {
       {"emitirTorneira", nullptr, m_faucet, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"avancar", nullptr, m_step, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"gravar", nullptr, m_tape_begin, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"definirTracos", nullptr, m_set_strokes, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"tracosVjp", nullptr, m_strokes_vjp, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"fimGravacao", nullptr, m_simple<tq_tape_end>, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"fracaoAgua", nullptr, m_fraction, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"retropropagar", nullptr, m_backward, nullptr, nullptr, nullptr, napi_default, nullptr},

@@ -96,6 +96,13 @@ class SimulacaoHIP {
   /** Records the frames run from now on (emitirTorneira and avancar) for retropropagar: the
    *  state is kept every checkpointACada frames and re-run in between (0: every substep kept). */
   gravar(checkpointACada = 10) { this._n.gravar(checkpointACada); }
+  /** The walls built on the device from strokes [{x0, y0, x1, y1, raio, apagar}, ...] (Cena.refazer
+   *  + atualizarParedes); this.cena is left as it is. */
+  definirTracos(tracos) { this._n.definirTracos(Float64Array.from(tracos.flatMap((t) => [t.x0, t.y0, t.x1, t.y1, t.raio, t.apagar ? 1 : 0]))); }
+  /** Per stroke, the sum over lattice points of gsdf * d(sdf)/d(x0, y0, x1, y1, raio): a Float64Array n*5. */
+  tracosVjp(tracos, gsdf) {
+    return this._n.tracosVjp(Float64Array.from(tracos.flatMap((t) => [t.x0, t.y0, t.x1, t.y1, t.raio, t.apagar ? 1 : 0])), gsdf);
+  }
   fimGravacao() { this._n.fimGravacao(); }
   /** Water fraction of the cell box [i0,i1) x [j0,j1) (mean of min(1, densidade/4)); with
    *  comGradiente, also d(fraction)/d(state) as a Float32Array of n*8 (x y vx vy cux cuy cvx cvy). */

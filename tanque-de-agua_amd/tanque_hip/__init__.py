@@ -99,6 +99,9 @@ SIGNATURES = {
     "tq_water_fraction": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                                     C.c_void_p, C.c_int]),
     "tq_tape_records": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "tq_set_strokes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "tq_read_sdf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "tq_strokes_vjp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "tq_tape_record": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t]),
     "tq_solver_stats_get": (C.c_int, [C.c_void_p, C.POINTER(SolverStats), C.c_int]),
     "tq_timing_enable": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p]),
@@ -277,6 +280,28 @@ class SimulacaoHIP:
 
     def avancar(self, dt, subpassos=1):
         _chk(load().tq_step(self._h, dt, subpassos))
+
+    # ---- walls built on the device
+    def definir_tracos(self, tracos):
+        """The walls from capsule strokes [(x0, y0, x1, y1, raio, apagar), ...] built on the
+        device (Cena.refazer + atualizarParedes); the host Cena is not updated."""
+        a = np.ascontiguousarray(np.asarray([[float(v) for v in t] for t in tracos], np.float64).reshape(-1, 6))
+        _chk(load().tq_set_strokes(self._h, a.ctypes.data if len(a) else None, len(a)))
+
+    def tracos_vjp(self, tracos, gsdf):
+        """sum(gsdf * d(sdf)/d(stroke)) per stroke: (n, 5) for x0 y0 x1 y1 raio."""
+        a = np.ascontiguousarray(np.asarray([[float(v) for v in t] for t in tracos], np.float64).reshape(-1, 6))
+        g = np.ascontiguousarray(np.asarray(gsdf, np.float32).reshape(-1))
+        out = np.zeros((max(len(a), 1), 5))
+        _chk(load().tq_strokes_vjp(self._h, a.ctypes.data if len(a) else None, len(a), g.ctypes.data, out.ctypes.data))
+        return out[: len(a)]
+
+    def ler_sdf(self):
+        """The device's SDF lattice (4 nx ny floats)."""
+        L = load()
+        out = np.zeros(4 * self.nx * self.ny, np.float32)
+        _chk(L.tq_read_sdf(self._h, out.ctypes.data, out.nbytes))
+        return out
 
     # ---- differentiation of recorded frames
     def tape_begin(self, checkpoint_every=10):

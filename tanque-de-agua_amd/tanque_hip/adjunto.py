@@ -2,14 +2,16 @@
 loss (water fraction of the bottom-right box - 0.5)^2, and its gradient with respect to the
 faucet jet velocity and the funnel's 32 wall heights (tq_tape_begin / tq_backward).
 
-The wall heights act through the SDF (the walls' push-out of particles, cena.js:102-117); the
-cell types the SDF gives are held with the other discrete decisions of the recorded run."""
+The walls are built on the device from the funnel's strokes (tq_set_strokes); the wall heights
+act through the SDF (the walls' push-out of particles, cena.js:102-117), and the SDF gradient
+goes back to them through tq_strokes_vjp; the cell types the SDF gives are held with the other
+discrete decisions of the recorded run."""
 import time
 
 import numpy as np
 
 from . import Cena, Semente, SimulacaoHIP
-from .scenes import FUNNEL_SEGMENTS, passo_frame, stroke_shift_vjp
+from .scenes import FUNNEL_SEGMENTS, passo_frame
 
 
 def run_funnel(work, checkpoint_every=10, tol=1e-6, solve_tol=1e-6, maxit=200, keep_emitted=False, grad=True,
@@ -18,12 +20,10 @@ def run_funnel(work, checkpoint_every=10, tol=1e-6, solve_tol=1e-6, maxit=200, k
     loss, frac, g_vjet (2), g_heights (32), g_sdf, the sim (left in its end state, tape open),
     the substep schedule, and with keep_emitted the emitted particles' positions per frame."""
     nx, ny, h = work.nx, work.ny, work.h
-    cena = Cena(nx, ny, h)
-    for s in work.strokes:
-        cena.tracar(*s)
     opts = dict(iteracoes=maxit, tol=solve_tol)
     opts.update(sim_options or {})
-    sim = SimulacaoHIP(cena, **opts)
+    sim = SimulacaoHIP(Cena(nx, ny, h), **opts)
+    sim.definir_tracos(work.strokes)  # the walls built on the device
     if len(work.particles):
         sim.adicionar(np.asarray(work.particles, np.float32), False)
     f = work.extra["faucet"]
@@ -48,7 +48,7 @@ def run_funnel(work, checkpoint_every=10, tol=1e-6, solve_tol=1e-6, maxit=200, k
     t_forward = time.perf_counter() - t0
     box = work.extra["box"]
     target = work.extra["target"]
-    out = {"sim": sim, "schedule": schedule, "emitted": emitted, "cena": cena, "t_forward": t_forward}
+    out = {"sim": sim, "schedule": schedule, "emitted": emitted, "sdf": sim.ler_sdf(), "t_forward": t_forward}
     if not grad:
         frac = sim.water_fraction(box)
         out.update(loss=(frac - target) ** 2, frac=frac)
@@ -59,7 +59,9 @@ def run_funnel(work, checkpoint_every=10, tol=1e-6, solve_tol=1e-6, maxit=200, k
     res = sim.backward(2 * (frac - target) * g, tol=tol)
     out["t_backward"] = time.perf_counter() - t0
     g_sdf = res["sdf"].astype(np.float64)
-    g_heights = stroke_shift_vjp(nx, ny, h, work.strokes, g_sdf, list(range(2 * FUNNEL_SEGMENTS)))
+    # a wall height moves both ends of its stroke: d/dy0 + d/dy1 (chain rule on the device)
+    gs = sim.tracos_vjp(work.strokes, res["sdf"])
+    g_heights = (gs[:, 1] + gs[:, 3])[: 2 * FUNNEL_SEGMENTS]
     out.update(loss=loss, frac=frac, g_vjet=res["faucet"].sum(axis=0), g_faucet_calls=res["faucet"],
                g_heights=g_heights, g_sdf=g_sdf, g_state0=res["state0"])
     return out

@@ -71,7 +71,7 @@ def test_backward_matches_fp64_replay_adjoint(keep_all):
     w, out = keep_all
     sim = out["sim"]
     T = oracle_tape(sim, w, out)
-    sdf = np.asarray(out["cena"].sdf, np.float64)
+    sdf = np.asarray(out["sdf"], np.float64)
     r = A.run(T, sdf, np.zeros((0, 8)), vjet_of(w), BOX, target=w.extra["target"])
     print("gpu: loss %.9g frac %.9g g_vjet %s" % (out["loss"], out["frac"], out["g_vjet"]))
     print("fp64 replay: loss %.9g frac %.9g g_vjet %s" % (r["loss"], r["frac"], r["g_vjet"]))
