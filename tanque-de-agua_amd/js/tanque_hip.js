@@ -51,6 +51,11 @@ class Cena {
 /** semente (testes/apoio.js:4-9): the seeded LCG; the state is advanced by semearRetangulo. */
 class Semente {
   constructor(s = 1) { this.estado = s >>> 0; }
+  /** The next number in [0, 1) of the tests' LCG (testes/apoio.js:4-9): s = s*1664525 + 1013904223 mod 2^32. */
+  proximo() {
+    this.estado = (Math.imul(this.estado, 1664525) + 1013904223) >>> 0;
+    return this.estado / 4294967296;
+  }
 }
 
 /** semearRetangulo (fisica-cpu.js:705-727) with a Semente. */
@@ -190,4 +195,61 @@ class MotorHIP {
   destruir() { this.sim.destruir(); }
 }
 
-module.exports = { SimulacaoHIP, MotorHIP, Cena, Semente, semearRetangulo, passoMaximo, CAMPOS, CAMPO };
+/** The app's per-frame substep plan (principal.js:528-575): the step bound from the measured
+ *  top speed (raised while the hand or the faucet throws water), the substep count, and the cap
+ *  from the measured cost per substep (an EMA of wall time, principal.js:545-550) so a busy
+ *  device slows the water instead of the screen.  A device motor uses SUB_MAX_GPU. */
+class Controlador {
+  constructor(tipo = 'gpu') {
+    this.tetoFixo = tipo === 'gpu' ? 24 : 6;   // SUB_MAX_GPU, SUB_MAX_CPU
+    this.orcamentoMs = 16;                      // ORCAMENTO_MS
+    this.custoParede = null;
+  }
+  /** registrarCusto (principal.js:545-550). */
+  registrarCusto(ms, sub) {
+    const custo = ms / sub;
+    if (sub <= 0 || !Number.isFinite(custo)) return;
+    this.custoParede = this.custoParede == null ? custo : this.custoParede + (custo - this.custoParede) * 0.15;
+  }
+  /** {dt, sub} for a frame of `real` seconds (already limited to 1/20 s by the caller), as
+   *  quadro() plans it (principal.js:556-571). */
+  planejar(real, cena, motor, ponteiro = null, ferramenta = null, fator = 1) {
+    let dt = real * fator;
+    let vMax = motor.vMax;
+    if (vMax != null && ponteiro && ponteiro.apertado && (ferramenta === 'mao' || ferramenta === 'torneira')) {
+      vMax = Math.max(vMax, Math.min(6, Math.hypot(ponteiro.vx, ponteiro.vy)) + 0.7);
+    }
+    const dtMax = passoMaximo(cena.h, cena.altura, vMax, dt);
+    let sub = dt > 0 ? Math.ceil(dt / dtMax - 1e-6) : 0;
+    const custo = motor.custoPorSubpasso != null ? motor.custoPorSubpasso : this.custoParede;
+    const teto = custo > 0 ? Math.max(2, Math.min(this.tetoFixo, Math.floor(this.orcamentoMs / custo))) : this.tetoFixo;
+    if (sub > teto) { sub = teto; dt = sub * dtMax; }
+    return { dt, sub };
+  }
+}
+
+/** reamostrar (principal.js:93-112): particles of a tank resized from cell hAntigo to hNovo,
+ *  (hAntigo/hNovo)^2 copies each on average, extra copies jittered by half a cell, those
+ *  within the frame of the new tank dropped; `aleatorio` replaces Math.random. */
+function reamostrar(lista, hAntigo, hNovo, largura, altura, aleatorio = Math.random) {
+  const razao = (hAntigo / hNovo) ** 2;
+  const n = lista.length / CAMPOS;
+  const m = 2 * hNovo;  // MOLDURA cells
+  const saida = [];
+  for (let k = 0; k < n; k++) {
+    const o = k * CAMPOS;
+    const copias = Math.floor(razao) + (aleatorio() < razao % 1 ? 1 : 0);
+    for (let c = 0; c < copias; c++) {
+      const p = lista.slice(o, o + CAMPOS);
+      if (c > 0) { p[0] += (aleatorio() - 0.5) * hNovo * 0.5; p[1] += (aleatorio() - 0.5) * hNovo * 0.5; }
+      if (p[0] <= m || p[1] <= m || p[0] >= largura - m || p[1] >= altura - m) continue;
+      saida.push(p);
+    }
+  }
+  const junta = new Float32Array(saida.length * CAMPOS);
+  saida.forEach((p, k) => junta.set(p, k * CAMPOS));
+  return junta;
+}
+
+module.exports = { SimulacaoHIP, MotorHIP, Controlador, reamostrar, Cena, Semente, semearRetangulo, passoMaximo, CAMPOS,
+                   CAMPO };
