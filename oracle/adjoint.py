@@ -43,7 +43,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         vp, d, i = C.c_void_p, C.c_double, C.c_int
         L.adj_run.restype = i
-        L.adj_run.argtypes = [i, i, d, d, d, d, vp, i, vp, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+        L.adj_run.argtypes = [i, i, d, d, d, d, vp, i, vp, i, vp, vp, vp, vp, vp, vp, vp, d, vp, vp, vp,
                               i, C.c_long, vp, vp, vp, d, d, vp, i, vp, vp, vp]
         L.adj_apply_operator.restype = None
         L.adj_apply_operator.argtypes = [i, i, vp, vp, i, vp, vp]
@@ -60,8 +60,9 @@ def bubble_fields(nx, ny, h, dt, tipo, rot, qar, vol_alvo, y_bolha):
     #ladoDireitoDasBolhas (fisica-cpu.js:430-456) and before #pressao (:487-504): the active
     bubbles (labels with a water face, in first-seen raster order as montarBolhas,
     pressao-cpu.js:271-288), bub[c] for their air cells, qextra (the warm-start y of bubbles
-    with cells but no water face: #pressao adds it to their cells' q) and rhsfix, the part of
-    each active bubble's rhs that is not a face velocity."""
+    with cells but no water face: #pressao adds it to their cells' q) and rhsfix, the volume
+    return of each active bubble's rhs (the face velocities and air pressures are the replay's
+    own).  qar is unused (the replay computes the air pressures)."""
     C_ = nx * ny
     tipo = np.asarray(tipo).reshape(-1)
     rot = np.asarray(rot).reshape(-1).astype(np.int64)
@@ -70,7 +71,6 @@ def bubble_fields(nx, ny, h, dt, tipo, rot, qar, vol_alvo, y_bolha):
     qextra = np.zeros(C_)
     order, idx = [], {}
     vol = {}
-    rsum = {}
     for j in range(1, ny - 1):
         for i in range(1, nx - 1):
             c = i + j * nx
@@ -78,16 +78,13 @@ def bubble_fields(nx, ny, h, dt, tipo, rot, qar, vol_alvo, y_bolha):
             if tipo[c] != AR or r < 2:
                 continue
             vol[r] = vol.get(r, 0) + 1
-            t = qar[c]
             for nbh in (c - 1, c + 1, c - nx, c + nx):
-                if tipo[nbh] == AGUA:
-                    if r not in idx:
-                        idx[r] = len(order)
-                        order.append(r)
-                    rsum[r] = rsum.get(r, 0.0) - t
+                if tipo[nbh] == AGUA and r not in idx:
+                    idx[r] = len(order)
+                    order.append(r)
     rhsfix = np.zeros(len(order))
     for r, b in idx.items():
-        rhsfix[b] = rsum[r] + VOLTA_VOLUME_BOLHA * (float(vol_alvo[r]) - vol[r]) * h / dt
+        rhsfix[b] = VOLTA_VOLUME_BOLHA * (float(vol_alvo[r]) - vol[r]) * h / dt
     air = (tipo == AR) & (rot >= 2)
     for c in np.nonzero(air)[0]:
         r = int(rot[c])
@@ -101,8 +98,8 @@ def bubble_fields(nx, ny, h, dt, tipo, rot, qar, vol_alvo, y_bolha):
 class Tape:
     """Tape of one run (arrays as adj_run takes them)."""
 
-    def __init__(self, nx, ny, h, gx=0.0, gy=-9.81, tau=0.1):
-        self.nx, self.ny, self.h, self.gx, self.gy, self.tau = nx, ny, h, gx, gy, tau
+    def __init__(self, nx, ny, h, gx=0.0, gy=-9.81, tau=0.1, sigma=0.0728):
+        self.nx, self.ny, self.h, self.gx, self.gy, self.tau, self.sigma = nx, ny, h, gx, gy, tau, sigma
         self.f_nemit, self.f_nsub, self.f_dt, self.emit_xy = [], [], [], []
         self.tipo, self.bub, self.qar, self.qextra, self.nb, self.rhsfix = [], [], [], [], [], []
         self.np = None
@@ -155,7 +152,7 @@ def run(tape, sdf, p0, vjet, box, target=0.5, record=False, grad=True, cg_tol=1e
     g_p0 = np.zeros(max(1, p0.size)) if grad else None
     rc = lib().adj_run(tape.nx, tape.ny, tape.h, tape.gx, tape.gy, tape.tau, _p(sdf), len(p0), _p(p0),
                        len(a["f_nsub"]), _p(a["f_nemit"]), _p(a["f_nsub"]), _p(a["f_dt"]), _p(a["emit_xy"]),
-                       _p(vjet), _p(a["tipo"]), _p(a["bub"]), _p(a["qar"]), _p(a["qextra"]), _p(a["nb"]),
+                       _p(vjet), _p(a["tipo"]), _p(a["bub"]), tape.sigma, _p(a["qextra"]), _p(a["nb"]),
                        _p(a["rhsfix"]), 1 if record else 0, cap, _p(np_), _p(flags), _p(box), target, cg_tol,
                        _p(out), 1 if grad else 0, _p(g_vf), _p(g_sdf), _p(g_p0))
     if rc != 0:

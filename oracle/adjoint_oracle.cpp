@@ -8,15 +8,16 @@
 // vy, cux, cuy, cvx, cvy), of the faucet jet velocity given to the emitted particles
 // (js/principal.js:301-321) and of the wall SDF:
 //   frozen (read from the tape)  cell types incl. solid cells, the active bubbles of each cell,
-//       the air pressures of surface tension (qAr, fisica-cpu.js:362-384), the frozen pressure
-//       of bubbles without water faces, the frozen part of each bubble's right-hand side
-//       (-sum qAr over its water faces + the volume return, fisica-cpu.js:430-456), every
+//       the frozen pressure of bubbles without water faces, the volume-return part of each
+//       bubble's right-hand side (fisica-cpu.js:445-450), every
 //       particle's micro-step count, collision steps, velocity projections (vn < 0: a particle
 //       sliding along a wall sits at vn = 0, where the projection has a kink) and removal
 //       (fisica-cpu.js:595-620), and
 //       the emitted particles' positions and count;
-//   live  P2G with its weights and the affine term (:198-267), gravity (:271-291), the
-//       divergence with the density-excess term (:458-483), the bubble face sums, the
+//   live  P2G with its weights and the affine term (:198-267), gravity (:271-291), surface
+//       tension (the water fraction, its extension into walls and blurs, the normals, the
+//       curvature and the air pressures: :306-403), the divergence with the density-excess and
+//       air-pressure terms (:458-483), the bubble face sums with their air pressures, the
 //       pressure solve (solved exactly, FP64 CG: the converged MGPCG of pressao-cpu.js:296-402
 //       is the same A^-1), the gradient (:522-549), the extrapolation (:551-556, 660-678) and
 //       G2P with its clamp, micro-steps, grid velocity lookups and the SDF push-out and
@@ -53,7 +54,7 @@ static inline int clampi(int v, int a, int b) { return v < a ? a : v > b ? b : v
 
 struct Ctx {
   int nx, ny, C, nu, nv;
-  double h, inv, gx, gy, tau;
+  double h, inv, gx, gy, tau, sigma;
   int mw, mh;
   const double* sdf;
   double* gsdf;  // SDF adjoint (may be null)
@@ -592,12 +593,177 @@ static void extrap_layer_adj(std::vector<double>& ab, const std::vector<uint8_t>
   ab = out;
 }
 
+// ------------------------------------------------------------------ surface tension (fisica-cpu.js:306-403)
+// The chain water fraction -> wall extension (3 layers) -> blur x, blur y -> extension -> blur x,
+// blur y -> extension -> normals -> curvature -> air pressures, every step linear in its input
+// except the normal's normalisation and the pressure clamp; the types (solid cells, water, air)
+// are held.
+struct Tension {
+  std::vector<std::vector<double>> fs;      // the fraction before each step of the chain, and after the last
+  std::vector<int> ops;                     // 0 extension layer, 1 blur x, 2 blur y
+  std::vector<std::vector<uint8_t>> full;   // extension layers: the cells filled before the layer
+  std::vector<double> nx, ny, g, kap;
+  std::vector<uint8_t> clamped;             // air pressure clamped to +-1/h (no derivative)
+  std::vector<double> qar;
+};
+
+static void ext_layer(const Ctx& c, const std::vector<double>& f, std::vector<uint8_t>& full, std::vector<double>& out) {
+  const int nx = c.nx, ny = c.ny;
+  out = f;
+  std::vector<int> fresh;
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      const int k = i + j * nx;
+      if (full[k]) continue;
+      double sum = 0;
+      int n = 0;
+      if (i > 0 && full[k - 1]) { sum += f[k - 1]; n++; }
+      if (i < nx - 1 && full[k + 1]) { sum += f[k + 1]; n++; }
+      if (j > 0 && full[k - nx]) { sum += f[k - nx]; n++; }
+      if (j < ny - 1 && full[k + nx]) { sum += f[k + nx]; n++; }
+      if (n) { out[k] = sum / n; fresh.push_back(k); }
+    }
+  for (int k : fresh) full[k] = 1;
+}
+
+static void tension_fwd(const Ctx& c, const uint8_t* t, const std::vector<double>& dens, double sigma, double dt,
+                        Tension& T) {
+  const int nx = c.nx, ny = c.ny, C = c.C;
+  std::vector<double> f(C);
+  for (int k = 0; k < C; k++) f[k] = t[k] == SOLIDO ? 0 : std::min(1.0, dens[k] / PPC);
+  T.fs.clear(); T.ops.clear(); T.full.clear();
+  for (int pass = 0; pass <= 2; pass++) {
+    std::vector<uint8_t> full(C);
+    for (int k = 0; k < C; k++) full[k] = t[k] == SOLIDO ? 0 : 1;
+    for (int layer = 0; layer < 3; layer++) {
+      T.fs.push_back(f); T.ops.push_back(0); T.full.push_back(full);
+      std::vector<double> o;
+      ext_layer(c, f, full, o);
+      f.swap(o);
+    }
+    if (pass == 2) break;
+    for (int axis = 0; axis < 2; axis++) {
+      T.fs.push_back(f); T.ops.push_back(1 + axis); T.full.emplace_back();
+      std::vector<double> o(C);
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          const int k = i + j * nx;
+          int a, b;
+          if (axis == 0) { a = i > 0 ? k - 1 : k; b = i < nx - 1 ? k + 1 : k; }
+          else { a = j > 0 ? k - nx : k; b = j < ny - 1 ? k + nx : k; }
+          o[k] = (f[a] + 2 * f[k] + f[b]) * 0.25;
+        }
+      f.swap(o);
+    }
+  }
+  T.fs.push_back(f);
+  T.nx.assign(C, 0); T.ny.assign(C, 0); T.g.assign(C, 0); T.kap.assign(C, 0);
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      const int k = i + j * nx;
+      const double gx = (f[k + 1] - f[k - 1]) * 0.5, gy = (f[k + nx] - f[k - nx]) * 0.5;
+      const double g = std::hypot(gx, gy);
+      T.g[k] = g;
+      if (g > 1e-3) { T.nx[k] = gx / g; T.ny[k] = gy / g; }
+    }
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      const int k = i + j * nx;
+      T.kap[k] = -((T.nx[k + 1] - T.nx[k - 1]) + (T.ny[k + nx] - T.ny[k - nx])) / (2 * c.h);
+    }
+  const double escala = dt / (1000.0 * c.h), kmax = 1 / c.h;
+  T.qar.assign(C, 0);
+  T.clamped.assign(C, 1);
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      const int k = i + j * nx;
+      if (t[k] != AR || !(sigma > 0)) continue;
+      double sum = T.kap[k];
+      int n = 1;
+      const int nb[4] = {k - 1, k + 1, k - nx, k + nx};
+      for (int q = 0; q < 4; q++)
+        if (t[nb[q]] == AGUA) { sum += T.kap[nb[q]]; n++; }
+      if (n > 1) {
+        const double m = sum / n;
+        T.clamped[k] = !(m > -kmax && m < kmax);
+        T.qar[k] = sigma * std::max(-kmax, std::min(kmax, m)) * escala;
+      }
+    }
+}
+
+// qbar (per cell) -> the density adjoint, added to db
+static void tension_adj(const Ctx& c, const uint8_t* t, const std::vector<double>& dens, const Tension& T, double sigma,
+                        double dt, const std::vector<double>& qbar, std::vector<double>& db) {
+  const int nx = c.nx, ny = c.ny, C = c.C;
+  const double escala = dt / (1000.0 * c.h);
+  std::vector<double> kb(C, 0.0), nxb(C, 0.0), nyb(C, 0.0), fb(C, 0.0);
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      const int k = i + j * nx;
+      if (t[k] != AR || !(sigma > 0) || T.clamped[k] || qbar[k] == 0) continue;
+      const int nb[4] = {k - 1, k + 1, k - nx, k + nx};
+      int n = 1;
+      for (int q = 0; q < 4; q++) n += t[nb[q]] == AGUA;
+      const double g = qbar[k] * sigma * escala / n;
+      kb[k] += g;
+      for (int q = 0; q < 4; q++)
+        if (t[nb[q]] == AGUA) kb[nb[q]] += g;
+    }
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      const int k = i + j * nx;
+      const double g = kb[k] / (2 * c.h);
+      nxb[k + 1] -= g; nxb[k - 1] += g; nyb[k + nx] -= g; nyb[k - nx] += g;
+    }
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      const int k = i + j * nx;
+      if (!(T.g[k] > 1e-3)) continue;
+      const double dot = nxb[k] * T.nx[k] + nyb[k] * T.ny[k];
+      const double gxb = (nxb[k] - dot * T.nx[k]) / T.g[k], gyb = (nyb[k] - dot * T.ny[k]) / T.g[k];
+      fb[k + 1] += 0.5 * gxb; fb[k - 1] -= 0.5 * gxb;
+      fb[k + nx] += 0.5 * gyb; fb[k - nx] -= 0.5 * gyb;
+    }
+  for (int s = (int)T.ops.size() - 1; s >= 0; s--) {
+    std::vector<double> o(C, 0.0);
+    if (T.ops[s] == 0) {
+      const std::vector<uint8_t>& full = T.full[s];
+      o = fb;
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          const int k = i + j * nx;
+          if (full[k]) continue;
+          int n = (i > 0 && full[k - 1]) + (i < nx - 1 && full[k + 1]) + (j > 0 && full[k - nx]) + (j < ny - 1 && full[k + nx]);
+          if (!n) continue;
+          const double g = fb[k] / n;
+          if (i > 0 && full[k - 1]) o[k - 1] += g;
+          if (i < nx - 1 && full[k + 1]) o[k + 1] += g;
+          if (j > 0 && full[k - nx]) o[k - nx] += g;
+          if (j < ny - 1 && full[k + nx]) o[k + nx] += g;
+          o[k] -= fb[k];
+        }
+    } else {
+      const int axis = T.ops[s] - 1;
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          const int k = i + j * nx;
+          int a, b;
+          if (axis == 0) { a = i > 0 ? k - 1 : k; b = i < nx - 1 ? k + 1 : k; }
+          else { a = j > 0 ? k - nx : k; b = j < ny - 1 ? k + nx : k; }
+          o[a] += 0.25 * fb[k]; o[k] += 0.5 * fb[k]; o[b] += 0.25 * fb[k];
+        }
+    }
+    fb.swap(o);
+  }
+  for (int k = 0; k < C; k++)
+    if (t[k] != SOLIDO && dens[k] / PPC < 1) db[k] += fb[k] / PPC;
+}
+
 // ------------------------------------------------------------------ one substep: record and reverse
 struct Sub {
   double dt;
   const uint8_t* tipo;
   const int32_t* bub;
-  const double* qar;
   const double* qextra;
   int nb;
   const double* rhsfix;
@@ -610,6 +776,7 @@ struct Sub {
   std::vector<uint8_t> uval0, vval0, uval1, vval1;  // validity before extrapolation layers 1, 2
   std::vector<int> ufr[2], vfr[2];
   System sys;
+  Tension tn;
 };
 
 static int substep_fwd(const Ctx& c, Sub& S, std::vector<double>& P, bool record, double cg_tol) {
@@ -635,6 +802,8 @@ static int substep_fwd(const Ctx& c, Sub& S, std::vector<double>& P, bool record
       V[k] = S.V0[k] + (S.pg.VW[k] > 0 ? c.gy * S.dt : 0);
       if (v_wall(c, t, i, j)) V[k] = 0;
     }
+  tension_fwd(c, t, S.pg.dens, c.sigma, S.dt, S.tn);
+  const std::vector<double>& qar = S.tn.qar;
   // right-hand side [-div ; bubble rhs]
   const double alvo = c.h / std::max(c.tau, S.dt);
   S.sys.c = &c; S.sys.tipo = t; S.sys.bub = S.bub; S.sys.nb = S.nb;
@@ -648,10 +817,10 @@ static int substep_fwd(const Ctx& c, Sub& S, std::vector<double>& P, bool record
       double dd;
       const double e = excess(c, t, i, j, S.pg.dens[k], &dd);
       double ar = 0;
-      if (t[k - 1] == AR) ar += S.qar[k - 1];
-      if (t[k + 1] == AR) ar += S.qar[k + 1];
-      if (t[k - nx] == AR) ar += S.qar[k - nx];
-      if (t[k + nx] == AR) ar += S.qar[k + nx];
+      if (t[k - 1] == AR) ar += qar[k - 1];
+      if (t[k + 1] == AR) ar += qar[k + 1];
+      if (t[k - nx] == AR) ar += qar[k - nx];
+      if (t[k + nx] == AR) ar += qar[k + nx];
       rhs[k] = -(d - alvo * e - ar);
     }
   for (int b = 0; b < S.nb; b++) rhs[C + b] = S.rhsfix[b];
@@ -660,17 +829,17 @@ static int substep_fwd(const Ctx& c, Sub& S, std::vector<double>& P, bool record
       const int a = i + j * nx;
       if (t[a] != AR || S.bub[a] < 0) continue;
       double& r = rhs[C + S.bub[a]];
-      if (t[a - 1] == AGUA) r += U[i + j * (nx + 1)];
-      if (t[a + 1] == AGUA) r -= U[i + 1 + j * (nx + 1)];
-      if (t[a - nx] == AGUA) r += V[i + j * nx];
-      if (t[a + nx] == AGUA) r -= V[i + (j + 1) * nx];
+      if (t[a - 1] == AGUA) r += U[i + j * (nx + 1)] - qar[a];
+      if (t[a + 1] == AGUA) r -= U[i + 1 + j * (nx + 1)] + qar[a];
+      if (t[a - nx] == AGUA) r += V[i + j * nx] - qar[a];
+      if (t[a + nx] == AGUA) r -= V[i + (j + 1) * nx] + qar[a];
     }
   std::vector<double> z;
   if (S.sys.solve(rhs, z, cg_tol)) return 2;
   std::vector<double> q(C, 0.0);
   for (int k = 0; k < C; k++) {
     if (t[k] == AGUA) q[k] = z[k];
-    else if (t[k] == AR) q[k] = S.qar[k] + S.qextra[k] + (S.bub[k] >= 0 ? z[C + S.bub[k]] : 0);
+    else if (t[k] == AR) q[k] = qar[k] + S.qextra[k] + (S.bub[k] >= 0 ? z[C + S.bub[k]] : 0);
   }
   // gradient (fisica-cpu.js:522-549)
   S.uval0.assign(c.nu, 0); S.vval0.assign(c.nv, 0);
@@ -757,6 +926,26 @@ static int substep_adj(const Ctx& c, Sub& S, std::vector<double>& Pb, double cg_
   }
   std::vector<double> s;
   if (S.sys.solve(rb, s, cg_tol)) return 2;
+  // air pressures: the air cells' q (#pressao), the water cells' div (air neighbours), the
+  // bubbles' face terms
+  std::vector<double> qarb(C, 0.0);
+  for (int k = 0; k < C; k++) if (t[k] == AR) qarb[k] += qb[k];
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      const int k = i + j * nx;
+      if (t[k] != AGUA) continue;
+      if (t[k - 1] == AR) qarb[k - 1] += s[k];
+      if (t[k + 1] == AR) qarb[k + 1] += s[k];
+      if (t[k - nx] == AR) qarb[k - nx] += s[k];
+      if (t[k + nx] == AR) qarb[k + nx] += s[k];
+    }
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      const int a = i + j * nx;
+      if (t[a] != AR || S.bub[a] < 0) continue;
+      const int nf = (t[a - 1] == AGUA) + (t[a + 1] == AGUA) + (t[a - nx] == AGUA) + (t[a + nx] == AGUA);
+      qarb[a] -= nf * s[C + S.bub[a]];
+    }
   // rhs[k] = -(d - alvo e - ar) for water cells; bubble rhs face sums
   const double alvo = c.h / std::max(c.tau, S.dt);
   std::vector<double> db(C, 0.0);
@@ -783,6 +972,7 @@ static int substep_adj(const Ctx& c, Sub& S, std::vector<double>& Pb, double cg_
       if (t[a - nx] == AGUA) Vb[i + j * nx] += g;
       if (t[a + nx] == AGUA) Vb[i + (j + 1) * nx] -= g;
     }
+  tension_adj(c, t, S.pg.dens, S.tn, c.sigma, S.dt, qarb, db);
   // gravity and walls, then the normalisation U0 = Un / W
   std::vector<double> ua(c.nu, 0.0), uw(c.nu, 0.0), va(c.nv, 0.0), vw(c.nv, 0.0);
   for (int j = 0; j < ny; j++)
@@ -831,19 +1021,19 @@ extern "C" {
 
 // Runs the replay forward over the frames and, if want_grad, the reverse pass.
 //   p0: n0 x 8 initial state; frames: f_nemit emitted particles (positions in emit_xy, velocity
-//   vf) then f_nsub substeps of f_dt / f_nsub; per substep s: tipo/bub/qar/qextra (C each),
+//   vf) then f_nsub substeps of f_dt / f_nsub; per substep s: tipo/bub/qextra (C each),
 //   nb[s] bubbles with rhsfix; np[s] particles at its start and flags (concatenated).
 //   record != 0: the particle decisions are made here and written to np/flags (flags_cap words).
 //   out: [loss, fraction].  Returns 0, 1 (flags_cap too small), 2 (CG did not converge),
 //   3 (tape mismatch).
 int adj_run(int nx, int ny, double h, double gx, double gy, double tau, const double* sdf, int n0, const double* p0,
             int F, const int* f_nemit, const int* f_nsub, const double* f_dt, const double* emit_xy, const double* vf,
-            const uint8_t* tipo, const int32_t* bub, const double* qar, const double* qextra, const int* nb,
+            const uint8_t* tipo, const int32_t* bub, double sigma, const double* qextra, const int* nb,
             const double* rhsfix, int record, long flags_cap, int* np, uint16_t* flags, const int* box,
             double target, double cg_tol, double* out, int want_grad, double* g_vf, double* g_sdf, double* g_p0) {
   Ctx c;
   c.nx = nx; c.ny = ny; c.C = nx * ny; c.nu = (nx + 1) * ny; c.nv = nx * (ny + 1);
-  c.h = h; c.inv = 1 / h; c.gx = gx; c.gy = gy; c.tau = tau;
+  c.h = h; c.inv = 1 / h; c.gx = gx; c.gy = gy; c.tau = tau; c.sigma = sigma;
   c.mw = nx * RES; c.mh = ny * RES;
   c.sdf = sdf;
   c.gsdf = nullptr;
@@ -865,7 +1055,7 @@ int adj_run(int nx, int ny, double h, double gx, double gy, double tau, const do
       Sub& B = subs[s];
       B.dt = f_dt[f] / f_nsub[f];
       const size_t off = (size_t)s * c.C;
-      B.tipo = tipo + off; B.bub = bub + off; B.qar = qar + off; B.qextra = qextra + off;
+      B.tipo = tipo + off; B.bub = bub + off; B.qextra = qextra + off;
       B.nb = nb[s]; B.rhsfix = rhsfix + bo;
       bo += nb[s];
       const int n = (int)(P.size() / 8);

@@ -22,11 +22,16 @@
 //   k_adj_faces    the divergence and bubble face sums (fisica-cpu.js:430-483) reversed, then
 //                  gravity/walls and the P2G normalisation u = sum(w v)/sum(w)
 //   k_adj_dens     the density-excess term (fisica-cpu.js:465-470) reversed
+//   k_ten_*        surface tension (fisica-cpu.js:306-403) re-run from the record's density
+//                  and reversed: the air pressures' adjoints (from the air cells' q, the water
+//                  cells' divergence and the bubbles' face terms) back through the clamp, the
+//                  curvature, the normals, the blurs and wall extensions to the density
 //   k_adj_p2g      P2G of one particle (fisica-cpu.js:198-267) reversed, with its density splat
 //   k_adj_emit     the emitted particles' velocity adjoints summed per faucet call
 // Every discrete decision (types, labels, micro-step counts, push-outs, projections, removals)
 // is the one the recorded run took: the result is the derivative of the run with those
-// decisions held, which is smooth in positions, velocities, the jet velocity and the SDF.
+// decisions held, which is smooth in positions, velocities, the jet velocity and the SDF
+// (surface tension included; the bubbles' volume-return terms are functions of the labels).
 // Particle adjoints travel between substeps indexed by insertion number (seq), since the
 // counting sort reorders particles every substep.
 #include <algorithm>
@@ -95,6 +100,11 @@ struct Tape {
   double *ub = nullptr, *vb = nullptr, *ub2 = nullptr, *vb2 = nullptr;
   double *ua = nullptr, *uwb = nullptr, *va = nullptr, *vwb = nullptr, *db = nullptr;
   double *ybar = nullptr, *gsdf = nullptr, *gemit = nullptr;
+  double *qarb = nullptr;        // air-pressure adjoint per cell
+  double* tf = nullptr;          // surface tension: the water fraction before each of its 13 steps, and after
+  uint8_t* tfull = nullptr;      // the filled cells before each of the 9 extension layers
+  double *tnx = nullptr, *tny = nullptr, *tg = nullptr, *tcoef = nullptr;
+  double *tkb = nullptr, *tnxb = nullptr, *tnyb = nullptr, *tfb = nullptr, *tfb2 = nullptr;
   uint8_t *val0u = nullptr, *val1u = nullptr, *val0v = nullptr, *val1v = nullptr;
 };
 
@@ -394,7 +404,8 @@ __global__ __launch_bounds__(256) void k_adj_extrap(const double* __restrict__ a
 __global__ __launch_bounds__(256) void k_adj_gradq(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
                                                    const uint32_t* __restrict__ facesB, int ar, int nx, int ny,
                                                    const double* __restrict__ ub, const double* __restrict__ vb,
-                                                   float* __restrict__ div, double* __restrict__ ybar) {
+                                                   float* __restrict__ div, double* __restrict__ ybar,
+                                                   double* __restrict__ qarb) {
 #pragma clang fp contract(off)
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nx * ny) return;
@@ -406,6 +417,7 @@ __global__ __launch_bounds__(256) void k_adj_gradq(const uint8_t* __restrict__ t
   if (v_class(tipo, nx, ny, i, j + 1) == 1) qb += vb[i + (j + 1) * nx];
   const uint8_t t = tipo[c];
   div[c] = t == AGUA ? (float)(-qb) : 0.f;
+  qarb[c] = t == AR ? qb : 0.0;  // an air cell's q is qAr + its bubble's y (#pressao)
   if (ar && t == AR && qb != 0) {
     const uint32_t r = rot[c];
     if (r >= 2 && r < (uint32_t)MAX_BOLHAS && facesB[r] > 0) atomicAdd(&ybar[r], qb);
@@ -559,6 +571,176 @@ __global__ __launch_bounds__(256) void k_adj_p2g(const float2* __restrict__ pos,
   }
   double* o = adjS + 8 * (size_t)seq[i];
   for (int q = 0; q < 8; q++) o[q] = pb[q];
+}
+
+// ------------------------------------------------------------------ surface tension (fisica-cpu.js:306-403)
+// The water fraction's chain: 3 passes of (wall extension, 3 layers), the first two followed by a
+// (1 2 1) blur in x then in y; then normals, curvature and the air pressures.  13 steps.
+constexpr int TEN_LAYERS = 9, TEN_STEPS = 13, TEN_STEPS_P1 = 14;
+
+__global__ __launch_bounds__(256) void k_ten_init(const uint8_t* __restrict__ tipo, const float* __restrict__ dens, int C,
+                                                  double* __restrict__ f, uint8_t* __restrict__ full) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  f[c] = tipo[c] == SOLIDO ? 0.0 : fmin(1.0, (double)dens[c] / PPC);
+  full[c] = tipo[c] != SOLIDO;
+}
+__global__ __launch_bounds__(256) void k_ten_full(const uint8_t* __restrict__ tipo, int C, uint8_t* __restrict__ full) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < C) full[c] = tipo[c] != SOLIDO;
+}
+// one extension layer (#estenderNaParede, fisica-cpu.js:351-370): fin/full before, fout/fullout after
+__global__ __launch_bounds__(256) void k_ten_ext(const double* __restrict__ fin, const uint8_t* __restrict__ full, int nx,
+                                                 int ny, double* __restrict__ fout, uint8_t* __restrict__ fullout) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  double v = fin[c];
+  uint8_t fl = full[c];
+  if (!fl) {
+    double sum = 0;
+    int n = 0;
+    if (i > 0 && full[c - 1]) { sum += fin[c - 1]; n++; }
+    if (i < nx - 1 && full[c + 1]) { sum += fin[c + 1]; n++; }
+    if (j > 0 && full[c - nx]) { sum += fin[c - nx]; n++; }
+    if (j < ny - 1 && full[c + nx]) { sum += fin[c + nx]; n++; }
+    if (n) { v = sum / n; fl = 1; }
+  }
+  fout[c] = v;
+  fullout[c] = fl;
+}
+// the (1 2 1)/4 blur along one axis with the edge cell standing in for its missing neighbour; it
+// is symmetric, so the same kernel is its own transpose
+__global__ __launch_bounds__(256) void k_ten_blur(const double* __restrict__ fin, int nx, int ny, int axis,
+                                                  double* __restrict__ fout) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  int a, b;
+  if (axis == 0) { a = i > 0 ? c - 1 : c; b = i < nx - 1 ? c + 1 : c; }
+  else { a = j > 0 ? c - nx : c; b = j < ny - 1 ? c + nx : c; }
+  fout[c] = (fin[a] + 2 * fin[c] + fin[b]) * 0.25;
+}
+__device__ inline bool interior(int i, int j, int nx, int ny) { return i >= 1 && j >= 1 && i < nx - 1 && j < ny - 1; }
+__global__ __launch_bounds__(256) void k_ten_normals(const double* __restrict__ f, int nx, int ny, double* __restrict__ tnx,
+                                                     double* __restrict__ tny, double* __restrict__ tg) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  double x = 0, y = 0, g = 0;
+  if (interior(c % nx, c / nx, nx, ny)) {
+    const double gx = (f[c + 1] - f[c - 1]) * 0.5, gy = (f[c + nx] - f[c - nx]) * 0.5;
+    g = js_hypot(gx, gy);
+    if (g > 1e-3) { x = gx / g; y = gy / g; }
+  }
+  tnx[c] = x; tny[c] = y; tg[c] = g;
+}
+// per air cell, d(qAr)/d(kappa) of each of its terms: sigma * escala / (terms) when unclamped
+__global__ __launch_bounds__(256) void k_ten_coef(const uint8_t* __restrict__ tipo, const double* __restrict__ tnx,
+                                                  const double* __restrict__ tny, int nx, int ny, double h, double sigma,
+                                                  double escala, double* __restrict__ coef) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  double r = 0;
+  const int i = c % nx, j = c / nx;
+  if (sigma > 0 && tipo[c] == AR && interior(i, j, nx, ny)) {
+    auto kap = [&](int k) {
+      const int ki = k % nx, kj = k / nx;
+      if (!interior(ki, kj, nx, ny)) return 0.0;
+      return -((tnx[k + 1] - tnx[k - 1]) + (tny[k + nx] - tny[k - nx])) / (2 * h);
+    };
+    double sum = kap(c);
+    int n = 1;
+    const int nb[4] = {c - 1, c + 1, c - nx, c + nx};
+    for (int q = 0; q < 4; q++)
+      if (tipo[nb[q]] == AGUA) { sum += kap(nb[q]); n++; }
+    const double m = sum / n, kmax = 1 / h;
+    if (n > 1 && m > -kmax && m < kmax) r = sigma * escala / n;
+  }
+  coef[c] = r;
+}
+// air-pressure adjoints from the adjoint solve: each water cell's rhs has + qAr of its air
+// neighbours, each active bubble's rhs - qAr per water face of its (interior) cells
+__global__ __launch_bounds__(256) void k_adj_qar_rhs(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
+                                                     const uint32_t* __restrict__ facesB, int ar, int nx, int ny,
+                                                     const float* __restrict__ q, const float* __restrict__ yB,
+                                                     double* __restrict__ qarb) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny || tipo[c] != AR) return;
+  const int i = c % nx, j = c / nx;
+  double g = 0;
+  int nf = 0;
+  if (i > 0 && tipo[c - 1] == AGUA) { g += q[c - 1]; nf++; }
+  if (i < nx - 1 && tipo[c + 1] == AGUA) { g += q[c + 1]; nf++; }
+  if (j > 0 && tipo[c - nx] == AGUA) { g += q[c - nx]; nf++; }
+  if (j < ny - 1 && tipo[c + nx] == AGUA) { g += q[c + nx]; nf++; }
+  uint32_t r;
+  if (ar && bubble_cell(tipo, rot, facesB, nx, ny, i, j, &r)) g -= nf * (double)yB[r];
+  qarb[c] += g;
+}
+// curvature adjoints: an air cell's own term and its water neighbours' terms
+__global__ __launch_bounds__(256) void k_ten_kb(const uint8_t* __restrict__ tipo, const double* __restrict__ coef,
+                                                const double* __restrict__ qarb, int nx, int ny, double* __restrict__ kb) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  double g = coef[c] * qarb[c];
+  if (tipo[c] == AGUA) {
+    const int nb[4] = {c - 1, c + 1, c - nx, c + nx};
+    const bool in[4] = {i > 0, i < nx - 1, j > 0, j < ny - 1};
+    for (int q = 0; q < 4; q++)
+      if (in[q]) g += coef[nb[q]] * qarb[nb[q]];
+  }
+  kb[c] = interior(i, j, nx, ny) ? g : 0.0;
+}
+// normal adjoints from kappa_k = -(nx[k+1] - nx[k-1] + ny[k+nx] - ny[k-nx]) / 2h, then the
+// normalisation's adjoint (n = g/|g| above 1e-3)
+__global__ __launch_bounds__(256) void k_ten_nb(const double* __restrict__ kb, const double* __restrict__ tnx,
+                                                const double* __restrict__ tny, const double* __restrict__ tg, int nx,
+                                                int ny, double h, double* __restrict__ gxb, double* __restrict__ gyb) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  double bx = 0, by = 0;
+  if (i > 0 && interior(i - 1, j, nx, ny)) bx -= kb[c - 1];
+  if (i < nx - 1 && interior(i + 1, j, nx, ny)) bx += kb[c + 1];
+  if (j > 0 && interior(i, j - 1, nx, ny)) by -= kb[c - nx];
+  if (j < ny - 1 && interior(i, j + 1, nx, ny)) by += kb[c + nx];
+  bx /= 2 * h;
+  by /= 2 * h;
+  double rx = 0, ry = 0;
+  if (interior(i, j, nx, ny) && tg[c] > 1e-3) {
+    const double dot = bx * tnx[c] + by * tny[c];
+    rx = (bx - dot * tnx[c]) / tg[c];
+    ry = (by - dot * tny[c]) / tg[c];
+  }
+  gxb[c] = rx;
+  gyb[c] = ry;
+}
+// fraction adjoints from the central differences g = (f[k+1] - f[k-1], f[k+nx] - f[k-nx]) / 2
+__global__ __launch_bounds__(256) void k_ten_fb(const double* __restrict__ gxb, const double* __restrict__ gyb, int nx,
+                                                int ny, double* __restrict__ fb) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  double g = 0;
+  if (i > 0 && interior(i - 1, j, nx, ny)) g += 0.5 * gxb[c - 1];
+  if (i < nx - 1 && interior(i + 1, j, nx, ny)) g -= 0.5 * gxb[c + 1];
+  if (j > 0 && interior(i, j - 1, nx, ny)) g += 0.5 * gyb[c - nx];
+  if (j < ny - 1 && interior(i, j + 1, nx, ny)) g -= 0.5 * gyb[c + nx];
+  fb[c] = g;
+}
+__global__ __launch_bounds__(256) void k_ten_dens(const uint8_t* __restrict__ tipo, const float* __restrict__ dens,
+                                                  const double* __restrict__ fb, int C, double* __restrict__ db) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  if (tipo[c] != SOLIDO && (double)dens[c] / PPC < 1) db[c] += fb[c] / PPC;
 }
 
 // sum of the velocity adjoints of insertion numbers [s0, s1) (one faucet call's particles)
@@ -747,6 +929,8 @@ void tape_free(tq_sim* s) {
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   f(T->adjS); f(T->xg); f(T->ub); f(T->vb); f(T->ub2); f(T->vb2); f(T->ua); f(T->uwb); f(T->va); f(T->vwb);
   f(T->db); f(T->ybar); f(T->gsdf); f(T->gemit); f(T->val0u); f(T->val1u); f(T->val0v); f(T->val1v);
+  f(T->qarb); f(T->tf); f(T->tfull); f(T->tnx); f(T->tny); f(T->tg); f(T->tcoef); f(T->tkb); f(T->tnxb); f(T->tnyb);
+  f(T->tfb); f(T->tfb2);
   delete T;
   s->tape = nullptr;
   s->rec = nullptr;
@@ -843,7 +1027,7 @@ static int reverse_substep(tq_sim* s, SubRec* r, double tol) {
   TQ_HIP(hipMemcpyAsync(s->cellsB, r->cellsB, 4 * (size_t)MAX_BOLHAS, hipMemcpyDeviceToDevice, s->st));
   TQ_HIP(hipMemsetAsync(T->ybar, 0, 8 * (size_t)MAX_BOLHAS, s->st));
   hipLaunchKernelGGL(k_adj_gradq, dim3(blocks(C)), dim3(256), 0, s->st, r->tipo, r->rot, r->facesB, s->d.ar, nx, ny, T->ub,
-                     T->vb, s->div, T->ybar);
+                     T->vb, s->div, T->ybar, T->qarb);
   TQ_LAUNCH_CHECK();
   TQ_HIP(hipMemcpyAsync(s->rhsB, T->ybar, 8 * (size_t)MAX_BOLHAS, hipMemcpyDeviceToDevice, s->st));
   TQ_HIP(hipMemsetAsync(s->q, 0, 4 * (size_t)C, s->st));
@@ -861,6 +1045,52 @@ static int reverse_substep(tq_sim* s, SubRec* r, double tol) {
                      nx, ny, s->q, s->yB32, T->ub, T->vb, r->u1, r->uw, r->v1, r->vw, s->d.gx * r->dt, s->d.gy * r->dt,
                      T->ua, T->uwb, T->va, T->vwb);
   hipLaunchKernelGGL(k_adj_dens, dim3(blocks(C)), dim3(256), 0, s->st, r->tipo, r->dens, s->q, nx, ny, alvo, T->db);
+  if (s->d.tensao > 0) {
+    // surface tension: the air pressures' adjoints back to the density
+    hipLaunchKernelGGL(k_adj_qar_rhs, dim3(blocks(C)), dim3(256), 0, s->st, r->tipo, r->rot, r->facesB, s->d.ar, nx, ny,
+                       s->q, s->yB32, T->qarb);
+    double* F = T->tf;
+    uint8_t* FU = T->tfull;
+    auto f_at = [&](int k) { return F + (size_t)k * C; };
+    auto m_at = [&](int k) { return FU + (size_t)k * C; };
+    // the chain forward, keeping every step's input
+    hipLaunchKernelGGL(k_ten_init, dim3(blocks(C)), dim3(256), 0, s->st, r->tipo, r->dens, C, f_at(0), m_at(0));
+    int step = 0;
+    int kinds[TEN_STEPS];  // -1 - layer index for extension layers, 0/1 blur axis
+    for (int pass = 0; pass <= 2; pass++) {
+      if (pass > 0) hipLaunchKernelGGL(k_ten_full, dim3(blocks(C)), dim3(256), 0, s->st, r->tipo, C, m_at(3 * pass));
+      for (int layer = 0; layer < 3; layer++, step++) {
+        const int L = 3 * pass + layer;
+        uint8_t* mout = layer < 2 ? m_at(L + 1) : m_at(TEN_LAYERS);
+        hipLaunchKernelGGL(k_ten_ext, dim3(blocks(C)), dim3(256), 0, s->st, f_at(step), m_at(L), nx, ny, f_at(step + 1), mout);
+        kinds[step] = -1 - L;
+      }
+      if (pass == 2) break;
+      for (int axis = 0; axis < 2; axis++, step++) {
+        hipLaunchKernelGGL(k_ten_blur, dim3(blocks(C)), dim3(256), 0, s->st, f_at(step), nx, ny, axis, f_at(step + 1));
+        kinds[step] = axis;
+      }
+    }
+    const double escala = r->dt / (DENSIDADE_AGUA * s->h);
+    hipLaunchKernelGGL(k_ten_normals, dim3(blocks(C)), dim3(256), 0, s->st, f_at(TEN_STEPS), nx, ny, T->tnx, T->tny, T->tg);
+    hipLaunchKernelGGL(k_ten_coef, dim3(blocks(C)), dim3(256), 0, s->st, r->tipo, T->tnx, T->tny, nx, ny, s->h,
+                       s->d.tensao, escala, T->tcoef);
+    // and backwards
+    hipLaunchKernelGGL(k_ten_kb, dim3(blocks(C)), dim3(256), 0, s->st, r->tipo, T->tcoef, T->qarb, nx, ny, T->tkb);
+    hipLaunchKernelGGL(k_ten_nb, dim3(blocks(C)), dim3(256), 0, s->st, T->tkb, T->tnx, T->tny, T->tg, nx, ny, s->h, T->tnxb,
+                       T->tnyb);
+    hipLaunchKernelGGL(k_ten_fb, dim3(blocks(C)), dim3(256), 0, s->st, T->tnxb, T->tnyb, nx, ny, T->tfb);
+    double *a = T->tfb, *b = T->tfb2;
+    for (int k = TEN_STEPS - 1; k >= 0; k--) {
+      if (kinds[k] < 0)
+        hipLaunchKernelGGL(k_adj_extrap, dim3(blocks(C)), dim3(256), 0, s->st, a, m_at(-1 - kinds[k]), nx, ny, b);
+      else
+        hipLaunchKernelGGL(k_ten_blur, dim3(blocks(C)), dim3(256), 0, s->st, a, nx, ny, kinds[k], b);
+      std::swap(a, b);
+    }
+    hipLaunchKernelGGL(k_ten_dens, dim3(blocks(C)), dim3(256), 0, s->st, r->tipo, r->dens, a, C, T->db);
+    TQ_LAUNCH_CHECK();
+  }
   if (r->n_up > 0)
     hipLaunchKernelGGL(k_adj_p2g, dim3(blocks(r->n_up)), dim3(256), 0, s->st, r->pos, r->vel, r->cm, r->seq, r->n_dev, nx,
                        ny, s->h, T->ua, T->uwb, T->va, T->vwb, T->db, T->xg, T->adjS);
@@ -877,6 +1107,10 @@ static int tape_scratch(tq_sim* s) {
   TQ_HIP(hipMalloc((void**)&T->ua, 8 * nu)); TQ_HIP(hipMalloc((void**)&T->uwb, 8 * nu));
   TQ_HIP(hipMalloc((void**)&T->va, 8 * nv)); TQ_HIP(hipMalloc((void**)&T->vwb, 8 * nv));
   TQ_HIP(hipMalloc((void**)&T->db, 8 * C)); TQ_HIP(hipMalloc((void**)&T->ybar, 8 * (size_t)MAX_BOLHAS));
+  TQ_HIP(hipMalloc((void**)&T->qarb, 8 * C));
+  TQ_HIP(hipMalloc((void**)&T->tf, 8 * C * TEN_STEPS_P1)); TQ_HIP(hipMalloc((void**)&T->tfull, C * (TEN_LAYERS + 1)));
+  for (double** p : {&T->tnx, &T->tny, &T->tg, &T->tcoef, &T->tkb, &T->tnxb, &T->tnyb, &T->tfb, &T->tfb2})
+    TQ_HIP(hipMalloc((void**)p, 8 * C));
   TQ_HIP(hipMalloc((void**)&T->gsdf, 8 * 4 * C));
   TQ_HIP(hipMalloc((void**)&T->val0u, nu)); TQ_HIP(hipMalloc((void**)&T->val1u, nu));
   TQ_HIP(hipMalloc((void**)&T->val0v, nv)); TQ_HIP(hipMalloc((void**)&T->val1v, nv));
