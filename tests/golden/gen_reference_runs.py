@@ -117,7 +117,7 @@ if (!Array.prototype.at) {
 DRIVER = r"""
 const crypto = require('crypto');
 const { Cena } = __m.cena;
-const { SimulacaoCPU, CAMPOS } = __m.fisica;
+const { SimulacaoCPU, CAMPOS, semearRetangulo } = __m.fisica;
 const { MODELOS, semearAgua } = __m.modelos;
 const { semente } = __m.apoio;
 const { passoMaximo } = __m.constantes;
@@ -154,8 +154,20 @@ function run(sc) {
                 frames: [], checkpoints: {} };
   let sub_index = 0;
   for (let f = 0; f < sc.quadros; f++) {
+    // tools and added water between frames (fisica-cpu.js:92-111, 129-152), as the app applies them
+    const ev = [];
+    for (const e of (sc.eventos || []).filter((e) => e.quadro === f)) {
+      let aceitas = null;
+      if (e.tipo === 'agua') {
+        const r = e.retangulo;
+        aceitas = sim.adicionar(semearRetangulo(r[0], r[1], r[2], r[3], sc.h, e.absorcao, semente(e.semente)), e.checar);
+      } else {
+        sim.aplicarFerramenta(e.ferramenta);
+      }
+      ev.push(Object.assign({ aceitas }, record(sim)));
+    }
     const sub = Math.ceil(QUADRO / passoMaximo(sim.h, sim.cena.altura, sim.vMax == null ? null : sim.vMax, QUADRO));
-    const fr = { sub, ar: null, phases: [] };
+    const fr = { sub, ar: null, phases: [], eventos: ev };
     // avancar (fisica-cpu.js:155-159), phase by phase as subpasso (176-189) calls them
     if (sim.arAtivo) { sim.atualizarAr(); fr.ar = record(sim); }
     const d = QUADRO / sub;
@@ -203,6 +215,22 @@ SCENES = [
      "opcoes": {}, "checkpoints": [30]},
     {"nome": "labirinto", "modelo": "labirinto", "nx": 64, "ny": 36, "h": 0.8 / 36, "semente": 6, "quadros": 30,
      "opcoes": {}, "checkpoints": [40]},
+    # the tools and checked additions (A18): dye, hand, sponge, water poured onto water
+    {"nome": "ferramentas", "modelo": "copo", "nx": 64, "ny": 36, "h": 0.8 / 36, "semente": 7, "quadros": 36,
+     "opcoes": {}, "checkpoints": [60], "eventos": [
+         {"quadro": 4, "tipo": "ferramenta", "ferramenta": {"tipo": "corante", "x": 0.71, "y": 0.3, "raio": 0.09,
+                                                             "absorcao": [60, 26, 3]}},
+         {"quadro": 8, "tipo": "ferramenta", "ferramenta": {"tipo": "mao", "x": 0.66, "y": 0.32, "raio": 0.08,
+                                                             "vx": 1.4, "vy": 0.6}},
+         {"quadro": 12, "tipo": "agua", "retangulo": [0.6, 0.25, 0.82, 0.45], "semente": 11, "checar": True,
+          "absorcao": [2, 55, 42]},
+         {"quadro": 16, "tipo": "ferramenta", "ferramenta": {"tipo": "esponja", "x": 0.75, "y": 0.28, "raio": 0.06}},
+         {"quadro": 20, "tipo": "agua", "retangulo": [0.3, 0.5, 0.5, 0.7], "semente": 12, "checar": True,
+          "absorcao": [1, 7, 60]},
+         {"quadro": 24, "tipo": "ferramenta", "ferramenta": {"tipo": "mao", "x": 0.72, "y": 0.2, "raio": 0.1,
+                                                             "vx": -2.0, "vy": 1.0}},
+         {"quadro": 28, "tipo": "ferramenta", "ferramenta": {"tipo": "nenhuma"}},
+     ]},
 ]
 
 

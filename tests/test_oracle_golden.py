@@ -7,8 +7,10 @@ identical particles at the checkpoints.
 
 This pins, bit for bit over whole runs, what no closed form pins: the multigrid smoother and
 V-cycle, the 3-iteration warm-started PCG, #superficie, the tracker's label and volume history
-and the G2P collision trajectories.  The foam column is compared on its own: foam uses Math.exp
-(V8's fdlibm port) where the oracle uses the C library's exp.
+and the G2P collision trajectories; one run adds the tools (dye, hand, sponge) and water added
+with the full-cell check between frames (adicionar's accepted counts included).  The foam
+column is compared on its own: foam uses Math.exp (V8's fdlibm port) where the oracle uses the
+C library's exp.
 """
 import base64
 import hashlib
@@ -67,7 +69,22 @@ def test_oracle_reproduces_the_reference_run(nome):
     quadro = 1 / 60
     foam_diff = 0
     sub_index = 0
+    eventos = [e for e in cfg.get("eventos", [])]
+    n_ev = 0
     for fi, fr in enumerate(run["frames"]):
+        mine = [e for e in eventos if e["quadro"] == fi]
+        assert len(mine) == len(fr.get("eventos", [])), (nome, fi)
+        n_ev += len(mine)
+        for e, want in zip(mine, fr.get("eventos", [])):
+            if e["tipo"] == "agua":
+                r = e["retangulo"]
+                lista = orc.semear_retangulo(r[0], r[1], r[2], r[3], h, e["absorcao"], orc.Semente(e["semente"]))
+                assert sim.adicionar(lista, e["checar"]) == want["aceitas"], (nome, fi, "adicionar")
+            else:
+                t = e["ferramenta"]
+                sim.ferramenta(t["tipo"], t.get("x", 0.0), t.get("y", 0.0), t.get("raio", 0.0), t.get("vx", 0.0),
+                               t.get("vy", 0.0), t.get("absorcao", (0, 0, 0)))
+            assert record(orc, sim)["s"] == want["s"], (nome, fi, e["tipo"])
         # the substep count the reference took from passoMaximo(vMax) must be ours too
         vmax = sim.vmax
         assert int(np.ceil(quadro / orc.passo_maximo(h, ny * h, vmax, quadro))) == fr["sub"], (nome, fi)
@@ -88,5 +105,6 @@ def test_oracle_reproduces_the_reference_run(nome):
                 want = np.frombuffer(base64.b64decode(cp), np.float32).reshape(-1, 12)
                 P = sim.particulas()
                 assert np.array_equal(np.delete(P, 8, axis=1), np.delete(want, 8, axis=1)), (nome, sub_index)
+    assert n_ev == len(eventos)
     print("%s: %d substeps bit-identical; foam differed after %d phases" % (nome, sub_index, foam_diff))
     assert foam_diff == 0
