@@ -58,8 +58,8 @@ def run_gpu(tq, work, precision, warmup, steps, schedule=None, time_kernel=None,
         frame(k)
     sim.sync()
     st0 = sim.solver_stats()
-    if time_kernel:
-        sim.timing_enable(True, only=time_kernel)
+    if time_kernel:  # "*" times every kernel
+        sim.timing_enable(True, only=None if time_kernel == "*" else time_kernel)
     t0 = time.perf_counter()
     subs = 0
     for k in range(warmup, warmup + steps):
@@ -210,6 +210,38 @@ def run_adjoint(tq, fd_h=1e-3, checkpoint_every=10, fd=True):
     return res
 
 
+def run_size_sweep(tq, sizes=((896, 504), (1792, 1008), (4096, 2304)), warmup=2, frames=3):
+    """The hourglass at growing sizes (past the 256 MB last-level cache at 4096x2304): substeps/s
+    in FP32 and in the mixed mode on the same schedule, and, from a run with every kernel timed by
+    HIP events on the library stream, the five kernels with the most time and their algorithmic
+    bandwidth against the 8 TB/s peak."""
+    from tanque_hip import scenes
+    out = []
+    for nx, ny in sizes:
+        w = scenes.ultra_hourglass() if (nx, ny) == (896, 504) else scenes.scaled_hourglass(nx, ny)
+        r32 = run_gpu(tq, w, tq.PREC_FP32, warmup, frames)
+        timed = run_gpu(tq, w, tq.PREC_FP32, warmup, frames, schedule=r32["schedule"], time_kernel="*")
+        rmx = run_gpu(tq, w, tq.PREC_MIXED, warmup, frames, schedule=r32["schedule"])
+        row = {"grid": "%dx%d" % (nx, ny), "cells": nx * ny, "particles": r32["particles_start"],
+               "substeps": r32["substeps"], "fp32_substeps_per_s": round(r32["substeps_per_s"], 2),
+               "fp32_pcg_iters_per_solve": round(r32["pcg_iters_per_solve"], 2),
+               "mixed_substeps_per_s": round(rmx["substeps_per_s"], 2),
+               "mixed_pcg_iters_per_solve": round(rmx["pcg_iters_per_solve"], 2)}
+        tim = timed["timing"]
+        total = sum(v["ms"] for v in tim.values())
+        top = []
+        for name, v in sorted(tim.items(), key=lambda kv: -kv[1]["ms"])[:5]:
+            avg = v["ms"] / max(v["launches"], 1) * 1e-3
+            gbs = v["bytes_per_launch"] / avg / 1e9 if avg > 0 else 0.0
+            top.append({"kernel": name, "share": round(v["ms"] / total, 3), "us": round(avg * 1e6, 2),
+                        "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        row["timed_kernel_ms_per_substep"] = round(total / max(timed["substeps"], 1), 3)
+        row["top_kernels"] = top
+        log("size sweep %s" % row)
+        out.append(row)
+    return out
+
+
 def dominant_kernel(tq, work, precision, frames=3):
     """Name of the kernel family with the largest device time on a few frames of the workload."""
     from tanque_hip.scenes import passo_frame
@@ -294,6 +326,9 @@ def main():
     ap.add_argument("--no-alto", action="store_true", help="skip the Alto (cup + faucet) runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (64 tanks) runs")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (funnel) runs")
+    ap.add_argument("--size-sweep", action="store_true",
+                    help="only the hourglass at 896x504, 1792x1008 and 4096x2304 (FP32 and mixed, with the level-0 "
+                         "down kernel timed); prints its JSON")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the adjoint configuration's forward and backward (for kernel profiles); prints its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -310,6 +345,9 @@ def main():
     tq.load()
     if args.adjoint_only:
         print(json.dumps({"adjoint": run_adjoint(tq, fd=False)}), flush=True)
+        return
+    if args.size_sweep:
+        print(json.dumps({"size_sweep": run_size_sweep(tq)}), flush=True)
         return
     if args.copy_calibration:
         copy_calibration()

@@ -314,7 +314,7 @@ __device__ inline bool fold_update(const double* __restrict__ part, int nb, int 
 // PRE_R partials over contiguous block ranges, so each block of the consumer re-sums only PRE_R
 // (the consumers' redundant sums would otherwise grow as the square of the block count).
 constexpr int PRE_R = 64;
-constexpr int FOLD_DIRECT = 512;  // consumers re-sum up to this many partials themselves
+constexpr int FOLD_DIRECT = 1280;  // consumers re-sum up to this many partials themselves
 __global__ __launch_bounds__(256) void k_prereduce(const double* __restrict__ src, int nb, int nv,
                                                    const double* __restrict__ bsrc, double* __restrict__ dst,
                                                    double* __restrict__ bdst, const DevScalars* __restrict__ sc) {
@@ -782,13 +782,16 @@ __device__ inline void block_tile(int* tx, int* ty) {
 // UPD: q += alpha p, r -= alpha Ap and |r|^2 (PCG update, pressao-cpu.js:373-381); else the
 // first cycle (r minus the projected mean).  b0 = scale * r on water, rb/cells on bubble cells
 // (pressao-cpu.js:326-332), then pass A (offset 0, colours 0,1) from x = 0 (cicloV,
-// pressao-cpu.js:243) when level 0 is not the coarsest.
+// pressao-cpu.js:243) when level 0 is not the coarsest.  One-dimensional grid over the 2x2-tile
+// groups (vgx of them per row, nvb in all); large grids launch at most IN_A0_GRID blocks, each
+// walking its groups after one fold.
+constexpr int IN_A0_GRID = 1280;  // 5 blocks per CU at this kernel's 92 VGPRs
 template <class SB, bool UPD, bool PASSA>
 __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMask, const uint8_t* __restrict__ mg0,
                                                const uint32_t* __restrict__ rot, const uint32_t* __restrict__ cellsB,
                                                const float* __restrict__ p, const float* __restrict__ ap,
                                                float* __restrict__ q, float* __restrict__ r, int nx, int ny, int ntx,
-                                               int nty, BubArgs B, const double* __restrict__ partA,
+                                               int nty, int vgx, int nvb, BubArgs B, const double* __restrict__ partA,
                                                const double* __restrict__ bpartA, int nbs, int par,
                                                SB* __restrict__ b0, float* __restrict__ x0,
                                                double* __restrict__ partRR, int scaled, const float* __restrict__ mgs,
@@ -796,11 +799,10 @@ __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMas
   if (sc->done) return;
   __shared__ float Tt[4][TCELLS];
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], aps[BSLOTS], rbs[BSLOTS];
-  const int lane = threadIdx.x & 63;
-  int tx, ty;
-  block_tile(&tx, &ty);
-  const bool valid = tx < ntx && ty < nty;
-  const int ox = tx * 16, oy = ty * 16;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int vb = xcd_block(blockIdx.x, gridDim.x);
+  int ox, oy;
+  bool valid;
   const double mean = UPD ? 0.0 : sc->proj_mean;
   const float scale = scaled ? *mgs : 1.f;
   const double* rb = bv(B.bvec, BV_R0 + 1 - par);
@@ -810,66 +812,80 @@ __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMas
   float qv[2][2], pv[2][2], rv[2][2], av[2][2];
   size_t ci[2][2];
   bool in[2][2];
+  auto load = [&](int g) {
+    const int by = g / vgx, bx = g - by * vgx;
+    const int tx = 2 * bx + (w & 1), ty = 2 * by + (w >> 1);
+    valid = tx < ntx && ty < nty;
+    ox = tx * 16;
+    oy = ty * 16;
 #pragma unroll
-  for (int cq = 0; cq < 2; cq++)
+    for (int cq = 0; cq < 2; cq++)
 #pragma unroll
-    for (int rw = 0; rw < 2; rw++) {
-      const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
-      in[cq][rw] = valid && i < nx && j < ny;
-      const size_t c = in[cq][rw] ? (size_t)i + (size_t)j * nx : 0;
-      ci[cq][rw] = c;
-      om[cq][rw] = in[cq][rw] ? opMask[c] : 0;
-      gm[cq][rw] = in[cq][rw] ? mg0[c] : 0;
-      rv[cq][rw] = r[c];
-      if (UPD) {
-        qv[cq][rw] = q[c];
-        pv[cq][rw] = p[c];
-        av[cq][rw] = ap[c];
+      for (int rw = 0; rw < 2; rw++) {
+        const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
+        in[cq][rw] = valid && i < nx && j < ny;
+        const size_t c = in[cq][rw] ? (size_t)i + (size_t)j * nx : 0;
+        ci[cq][rw] = c;
+        om[cq][rw] = in[cq][rw] ? opMask[c] : 0;
+        gm[cq][rw] = in[cq][rw] ? mg0[c] : 0;
+        rv[cq][rw] = r[c];
+        if (UPD) {
+          qv[cq][rw] = q[c];
+          pv[cq][rw] = p[c];
+          av[cq][rw] = ap[c];
+        }
       }
-    }
+  };
+  load(vb);
   // UPD: alpha and the bubbles' new residuals from spmv's partials (fold_spmv, while the loads
   // above are in flight); else (the first cycle) the current bubble residuals are those of
   // parity 1 - par
   const double alpha = UPD ? fold_spmv(partA, bpartA, nbs, par, B, sh, res, aps, rbs, sc) : 0.0;
   const double beta = UPD ? sc->beta : 0.0;
-  Cells4 cc;
+  for (;;) {
+    Cells4 cc;
 #pragma unroll
-  for (int cq = 0; cq < 2; cq++)
+    for (int cq = 0; cq < 2; cq++)
 #pragma unroll
-    for (int rw = 0; rw < 2; rw++) {
-      cc.v[cq][rw] = 0.f;
-      if (in[cq][rw]) {
-        const size_t c = ci[cq][rw];
-        const uint8_t m = om[cq][rw];
-        float bval = 0.f;
-        if (mask_type(m) == AGUA) {
-          float rn;
-          if (UPD) {
-            q[c] = (float)((double)qv[cq][rw] + alpha * (double)pv[cq][rw]);
-            rn = (float)((double)rv[cq][rw] - alpha * (double)av[cq][rw]);
-            r[c] = rn;
-            rr[0] += (double)rn * rn;
-          } else {
-            rn = rv[cq][rw];
-            if (mean != 0) { rn = (float)((double)rn - mean); r[c] = rn; }
+      for (int rw = 0; rw < 2; rw++) {
+        cc.v[cq][rw] = 0.f;
+        if (in[cq][rw]) {
+          const size_t c = ci[cq][rw];
+          const uint8_t m = om[cq][rw];
+          float bval = 0.f;
+          if (mask_type(m) == AGUA) {
+            float rn;
+            if (UPD) {
+              q[c] = (float)((double)qv[cq][rw] + alpha * (double)pv[cq][rw]);
+              rn = (float)((double)rv[cq][rw] - alpha * (double)av[cq][rw]);
+              r[c] = rn;
+              rr[0] += (double)rn * rn;
+            } else {
+              rn = rv[cq][rw];
+              if (mean != 0) { rn = (float)((double)rn - mean); r[c] = rn; }
+            }
+            bval = rn;
+          } else if (m & MK_BUB) {
+            const uint32_t lb = rot[c];
+            const double rbl = UPD ? bubble_r(B, lb, par, alpha, beta, rbs) : rb[lb];
+            bval = (float)(rbl / cellsB[lb]);
           }
-          bval = rn;
-        } else if (m & MK_BUB) {
-          const uint32_t lb = rot[c];
-          const double rbl = UPD ? bubble_r(B, lb, par, alpha, beta, rbs) : rb[lb];
-          bval = (float)(rbl / cellsB[lb]);
-        }
-        bval = rnd<SB>(bval * scale);
-        st<SB>(b0, c, bval);
-        cell_setup(cc, cq, rw, gm[cq][rw], bval);
-      } else cell_clear(cc, cq, rw);
+          bval = rnd<SB>(bval * scale);
+          st<SB>(b0, c, bval);
+          cell_setup(cc, cq, rw, gm[cq][rw], bval);
+        } else cell_clear(cc, cq, rw);
+      }
+    if (PASSA && valid) {
+      float* T = Tt[w];
+      zero_tile(T, lane);
+      wave_sync();
+      tile_sweeps<0>(T, cc, lane);
+      store_cells<float>(cc, x0, ox, oy, nx, ny, lane);
+      wave_sync();  // the wave's tile is free for its next group
     }
-  if (PASSA && valid) {
-    float* T = Tt[threadIdx.x >> 6];
-    zero_tile(T, lane);
-    wave_sync();
-    tile_sweeps<0>(T, cc, lane);
-    store_cells<float>(cc, x0, ox, oy, nx, ny, lane);
+    vb += gridDim.x;
+    if (vb >= nvb) break;
+    load(vb);
   }
   if (UPD) store_partials<1>(rr, 1, partRR, sh);
 }
@@ -1883,7 +1899,11 @@ __global__ __launch_bounds__(256) void k_vcycle_apply(const float* __restrict__ 
 // First kernel of iteration k: the z dots and beta (fold_zdots), then per cell p = z + beta p_old
 // (bubble cells: their new z and old p), Ap and the p.Ap partials (produto / aplicar,
 // pressao-cpu.js:84-93, 257-261); SPMV_CELLS cells per thread so the partials stay few.
+// Large grids launch at most SPMV_GRID blocks (the ones that fit on the chip at once) and each
+// block walks its chunks after one fold: the fold's chain of dependent loads and barriers is
+// paid once per block, not once per chunk.
 constexpr int SPMV_CELLS = 4;
+constexpr int SPMV_GRID = 1024;  // 4 blocks per CU at this kernel's 123 VGPRs
 __global__ __launch_bounds__(256) void k_spmv(const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
                                               const float* __restrict__ z, const float* __restrict__ pold,
                                               float* __restrict__ pnew, float* __restrict__ ap, int nx, int C,
@@ -1893,25 +1913,31 @@ __global__ __launch_bounds__(256) void k_spmv(const uint8_t* __restrict__ opMask
                                               DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], zs[BSLOTS];
+  constexpr int CH = 256 * SPMV_CELLS;
+  const int nch = (C + CH - 1) / CH;
+  int ch = xcd_block(blockIdx.x, gridDim.x);
   // every load of the cells first (clamped indices; masks decide what is used): they do not
   // depend on beta, so their latency overlaps the fold's reductions
   int nbr[SPMV_CELLS][5];
   uint8_t mk[SPMV_CELLS][5];
   float zv[SPMV_CELLS][5], pv[SPMV_CELLS][5];
+  auto load = [&](int chunk) {
 #pragma unroll
-  for (int q = 0; q < SPMV_CELLS; q++) {
-    const int c = xcd_block(blockIdx.x, gridDim.x) * (256 * SPMV_CELLS) + q * 256 + threadIdx.x;
-    const int cc = c < C ? c : C - 1;
-    const int nb0[5] = {cc, cc - 1, cc + 1, cc - nx, cc + nx};
+    for (int q = 0; q < SPMV_CELLS; q++) {
+      const int c = chunk * CH + q * 256 + threadIdx.x;
+      const int cc = c < C ? c : C - 1;
+      const int nb0[5] = {cc, cc - 1, cc + 1, cc - nx, cc + nx};
 #pragma unroll
-    for (int k = 0; k < 5; k++) {
-      const int d = nb0[k] < 0 ? 0 : (nb0[k] >= C ? C - 1 : nb0[k]);
-      nbr[q][k] = d;
-      mk[q][k] = opMask[d];
-      zv[q][k] = z[d];
-      pv[q][k] = pold[d];
+      for (int k = 0; k < 5; k++) {
+        const int d = nb0[k] < 0 ? 0 : (nb0[k] >= C ? C - 1 : nb0[k]);
+        nbr[q][k] = d;
+        mk[q][k] = opMask[d];
+        zv[q][k] = z[d];
+        pv[q][k] = pold[d];
+      }
     }
-  }
+  };
+  load(ch);
   const double beta = fold_zdots(partZ, bpartZ, nbz, par, first, B, sh, res, zs, sc);
   const int ns = nslots(sc->active_bubbles);
   const double* Po = bv(B.bvec, BV_P0 + 1 - par);
@@ -1930,27 +1956,32 @@ __global__ __launch_bounds__(256) void k_spmv(const uint8_t* __restrict__ opMask
   };
   double v[1] = {0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (;;) {
 #pragma unroll
-  for (int q = 0; q < SPMV_CELLS; q++) {
-    const int c = xcd_block(blockIdx.x, gridDim.x) * (256 * SPMV_CELLS) + q * 256 + threadIdx.x;
-    if (c < C) {
-      const uint8_t m = mk[q][0];
-      const float pc = peff(c, m, zv[q][0], pv[q][0]);
-      pnew[c] = pc;
-      if (mask_type(m) == AGUA) {
-        double s = 0;
+    for (int q = 0; q < SPMV_CELLS; q++) {
+      const int c = ch * CH + q * 256 + threadIdx.x;
+      if (c < C) {
+        const uint8_t m = mk[q][0];
+        const float pc = peff(c, m, zv[q][0], pv[q][0]);
+        pnew[c] = pc;
+        if (mask_type(m) == AGUA) {
+          double s = 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          if (!(m & (1 << k))) continue;
-          const uint8_t md = mk[q][k + 1];
-          s += (double)peff(nbr[q][k + 1], md, zv[q][k + 1], pv[q][k + 1]);
-          if (md & MK_BUBIN) bub_add(sl, B.labelSlot, accA, rot[nbr[q][k + 1]], -(double)pc);
-        }
-        const float a = (float)(mask_cnt(m) * (double)pc - s);
-        ap[c] = a;
-        v[0] += (double)pc * a;
-      } else ap[c] = 0.f;
+          for (int k = 0; k < 4; k++) {
+            if (!(m & (1 << k))) continue;
+            const uint8_t md = mk[q][k + 1];
+            s += (double)peff(nbr[q][k + 1], md, zv[q][k + 1], pv[q][k + 1]);
+            if (md & MK_BUBIN) bub_add(sl, B.labelSlot, accA, rot[nbr[q][k + 1]], -(double)pc);
+          }
+          const float a = (float)(mask_cnt(m) * (double)pc - s);
+          ap[c] = a;
+          v[0] += (double)pc * a;
+        } else ap[c] = 0.f;
+      }
     }
+    ch += gridDim.x;
+    if (ch >= nch) break;
+    load(ch);
   }
   store_partials<1>(v, 1, partA, sh);
   if (ns > 0) store_partials<BSLOTS>(sl, ns, bpartA, sh);
@@ -2221,7 +2252,9 @@ static RedSrc red_src(tq_sim* s, int slot, int nv, int bhalf, int nb) {
 static void launch_in_a0(tq_sim* s, bool upd) {
   Level& F = s->lv[0];
   const int ntx = tiles_of(F.nx, 0), nty = tiles_of(F.ny, 0);
-  const dim3 g = tile_grid(F.nx, F.ny, 0);
+  const dim3 tg = tile_grid(F.nx, F.ny, 0);
+  const int vgx = (int)tg.x, nvb = (int)(tg.x * tg.y);
+  const dim3 g(std::min(nvb, IN_A0_GRID));
   const bool pa = s->lv.size() > 1;
   const float* pp = s->pbuf[s->pc];
   // bytes: opMask, mg0, b0 write, x0 write; update: q, p, r, Ap read + q, r write
@@ -2234,7 +2267,8 @@ static void launch_in_a0(tq_sim* s, bool upd) {
   const float* mgs = &s->sc->mgs[it_par(s)];
 #define TQ_IN_A0(SB, U, P)                                                                                        \
   hipLaunchKernelGGL((k_in_a0<SB, U, P>), g, dim3(256), 0, s->st, s->opMask, F.mask, s->rot, s->cellsB, pp, s->ap, \
-                     s->q, s->r, F.nx, F.ny, ntx, nty, B, rs.part, rs.bpart, rs.nb, par, (SB*)F.b, (float*)F.x,  \
+                     s->q, s->r, F.nx, F.ny, ntx, nty, vgx, nvb, B, rs.part, rs.bpart, rs.nb, par, (SB*)F.b,      \
+                     (float*)F.x,                                                                                  \
                      s->partials + ps_off(PS_RR), s->mg_scaled, mgs, s->sc)
   if (F.hb) {
     if (upd) { if (pa) TQ_IN_A0(__half, true, true); else TQ_IN_A0(__half, true, false); }
@@ -2528,7 +2562,7 @@ static void precondition_rest(tq_sim* s) {
 // the V-cycle (level 0's down kernel starts with the norm and stop test)
 static void pcg_iteration(tq_sim* s) {
   const int C = s->C;
-  const int nbs = (C + 256 * SPMV_CELLS - 1) / (256 * SPMV_CELLS);
+  const int nbs = std::min((C + 256 * SPMV_CELLS - 1) / (256 * SPMV_CELLS), SPMV_GRID);
   const int pn = 1 - s->pc;
   const RedSrc rz = red_src(s, PS_Z, 2, 1, s->nb_z);
   int t = time_begin(s, "pcg_spmv", (1 + 4 + 4 + 4 + 4) * (double)C);

@@ -45,8 +45,11 @@ def prepared(orc, build, frames):
     ("dam256", lambda o: dam(o, nx=256, ny=144, h=1.0 / 144, seed=0), 8),
     # twelve bubbles: more than the per-block reduction slots, the rest through FP64 atomics
     ("bolhas12", lambda o: tanque_com_bolhas(o), 2),
-    # 1024x640 (655k cells): more than 512 blocks per reduction, so the partials are pre-reduced
+    # 1024x640 (655k cells): 640 spmv and update blocks, their partials summed by every consumer block
     ("dam1024", lambda o: dam(o, nx=1024, ny=640, h=0.8 / 640, seed=0), 1),
+    # 2048x1152 (2.36M cells): spmv and the update launch capped grids whose blocks walk several
+    # chunks after one fold; the z partials (one per up-kernel block) are pre-reduced
+    ("dam2048", lambda o: dam(o, nx=2048, ny=1152, h=0.8 / 1152, seed=0), 1),
 ])
 def test_solve_to_tolerance(oracle, tq, name, build, frames):
     osim = prepared(oracle, build, frames)
@@ -115,7 +118,8 @@ def test_mixed_matches_fp32_over_steps(tq, oracle):
 @pytest.mark.parametrize("name,build", [
     ("bolha", lambda o: tanque_com_bolha(o)),
     ("dam160", lambda o: dam(o, nx=160, ny=90, h=0.8 / 90, seed=0)),
-    ("dam1024", lambda o: dam(o, nx=1024, ny=640, h=0.8 / 640, seed=0)),  # pre-reduced partials
+    ("dam1024", lambda o: dam(o, nx=1024, ny=640, h=0.8 / 640, seed=0)),
+    ("dam2048", lambda o: dam(o, nx=2048, ny=1152, h=0.8 / 1152, seed=0)),  # capped grids, pre-reduced partials
 ])
 def test_graph_replay_matches_eager(tq, oracle, prec, name, build):
     """PCG iterations replayed from the captured HIP graph give bit-identical states to eager
