@@ -327,8 +327,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (64 tanks) runs")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (funnel) runs")
     ap.add_argument("--size-sweep", action="store_true",
-                    help="only the hourglass at 896x504, 1792x1008 and 4096x2304 (FP32 and mixed, with the level-0 "
-                         "down kernel timed); prints its JSON")
+                    help="only the hourglass at 896x504, 1792x1008 and 4096x2304: FP32 and mixed on one schedule, "
+                         "and the five kernels with the most time (every kernel timed by HIP events); prints its JSON")
+    ap.add_argument("--sweep-grid", action="append", default=None, metavar="WxH",
+                    help="with --size-sweep: only these grids (repeatable), e.g. 4096x2304")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the adjoint configuration's forward and backward (for kernel profiles); prints its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -346,11 +348,13 @@ def main():
     if args.adjoint_only:
         print(json.dumps({"adjoint": run_adjoint(tq, fd=False)}), flush=True)
         return
-    if args.size_sweep:
-        print(json.dumps({"size_sweep": run_size_sweep(tq)}), flush=True)
-        return
     if args.copy_calibration:
         copy_calibration()
+    if args.size_sweep:
+        sizes = tuple(tuple(int(v) for v in g.lower().split("x")) for g in args.sweep_grid) if args.sweep_grid \
+            else ((896, 504), (1792, 1008), (4096, 2304))
+        print(json.dumps({"size_sweep": run_size_sweep(tq, sizes=sizes)}), flush=True)
+        return
     work = scenes.ultra_hourglass() if (args.nx, args.ny) == (896, 504) else scenes.scaled_hourglass(args.nx, args.ny)
     log("workload %s: %dx%d cells, %d particles seeded" % (work.name, work.nx, work.ny, len(work.particles) // 12))
 
