@@ -237,6 +237,9 @@ def run_size_sweep(tq, sizes=((896, 504), (1792, 1008), (4096, 2304)), warmup=2,
                         "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
         row["timed_kernel_ms_per_substep"] = round(total / max(timed["substeps"], 1), 3)
         row["top_kernels"] = top
+        row["kernels_us_GBs"] = {name: [round(v["ms"] / max(v["launches"], 1) * 1e3, 2),
+                                        round(v["bytes_per_launch"] / (v["ms"] / max(v["launches"], 1) * 1e-3) / 1e9, 1)
+                                        if v["ms"] > 0 else 0.0] for name, v in sorted(tim.items())}
         log("size sweep %s" % row)
         out.append(row)
     return out

@@ -225,6 +225,15 @@ typedef struct tq_kernel_time {
 int tq_timing_enable(tq_sim* s, int enable, const char* only);
 int tq_timing_read(tq_sim* s, tq_kernel_time* out, int max, int* n);
 
+/* Launch shapes of the solver's level-0 kernels, by name: "spmv_grid" (most blocks of the
+ * spmv kernel, default 1024) and "update_grid" (most blocks of the PCG update / V-cycle input
+ * kernel, default 1280); 0 restores the default.  Grids larger than the cap walk their cells
+ * with fewer blocks.  The cap decides how the dot products' partial sums are grouped, so results
+ * are bit-reproducible for a fixed setting and can differ in the last bits between settings.
+ * No counterpart in the reference (launch tuning). */
+int tq_set_launch(tq_sim* s, const char* name, int value);
+int tq_get_launch(tq_sim* s, const char* name, int* value);
+
 /* Waits for all queued device work of this handle. */
 int tq_sync(tq_sim* s);
 

@@ -308,6 +308,14 @@ __device__ inline long long wave_sum_i64(long long v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// Per-bubble sums over the air cells: cell count, interior cells, faces to water and the face
+// terms of the rhs (#ladoDireitoDasBolhas, fisica-cpu.js:429-454) as 2^-40 fixed point, all
+// integers, so the order of the adds does not matter.  Each wave sums per label (ballot
+// segments); the wave leaders add into a block table of BRHS_SLOTS labels in LDS; the block
+// flushes the table once.  One bubble of millions of cells is then a few thousand global
+// atomics instead of one per wave.  Labels the full table cannot take go to global atomics.
+constexpr int BRHS_SLOTS = 32;
+constexpr int BRHS_GRID = 2048;
 __global__ __launch_bounds__(256) void k_bubble_rhs(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
                                                     const float* __restrict__ u, const float* __restrict__ v,
                                                     const float* __restrict__ qAr, int nx, int ny,
@@ -315,45 +323,82 @@ __global__ __launch_bounds__(256) void k_bubble_rhs(const uint8_t* __restrict__ 
                                                     uint32_t* __restrict__ cellsB, double* __restrict__ rhsB,
                                                     int counts_only) {
 #pragma clang fp contract(off)
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t r = 0;
-  int interior = 0, faces = 0;
-  double rhs = 0;
-  if (c < nx * ny) {
-    const int i = c % nx, j = c / nx;
-    if (tipo[c] == AR && rot[c] >= 2) {
-      r = rot[c];
-      if (i >= 1 && j >= 1 && i < nx - 1 && j < ny - 1) {
-        interior = 1;
-        const double tq = qAr[c];
-        if (tipo[c - 1] == AGUA) { rhs += (double)u[(size_t)i + (size_t)j * (nx + 1)] - tq; faces++; }
-        if (tipo[c + 1] == AGUA) { rhs += -(double)u[(size_t)i + 1 + (size_t)j * (nx + 1)] - tq; faces++; }
-        if (tipo[c - nx] == AGUA) { rhs += (double)v[(size_t)i + (size_t)j * nx] - tq; faces++; }
-        if (tipo[c + nx] == AGUA) { rhs += -(double)v[(size_t)i + (size_t)(j + 1) * nx] - tq; faces++; }
+  __shared__ uint32_t key[BRHS_SLOTS], kc[BRHS_SLOTS], kv[BRHS_SLOTS], kf[BRHS_SLOTS];
+  __shared__ unsigned long long ks[BRHS_SLOTS];
+  for (int k = threadIdx.x; k < BRHS_SLOTS; k += blockDim.x) {
+    key[k] = 0;  // labels of bubbles are >= 2: 0 marks a free slot
+    kc[k] = kv[k] = kf[k] = 0;
+    ks[k] = 0;
+  }
+  __syncthreads();
+  const int C = nx * ny;
+  const int lane = threadIdx.x & 63;
+  for (int base = blockIdx.x * blockDim.x; base < C; base += gridDim.x * blockDim.x) {
+    const int c = base + threadIdx.x;
+    uint32_t r = 0;
+    int interior = 0, faces = 0;
+    double rhs = 0;
+    if (c < C) {
+      const int i = c % nx, j = c / nx;
+      if (tipo[c] == AR && rot[c] >= 2) {
+        r = rot[c];
+        if (i >= 1 && j >= 1 && i < nx - 1 && j < ny - 1) {
+          interior = 1;
+          const double tq = qAr[c];
+          if (tipo[c - 1] == AGUA) { rhs += (double)u[(size_t)i + (size_t)j * (nx + 1)] - tq; faces++; }
+          if (tipo[c + 1] == AGUA) { rhs += -(double)u[(size_t)i + 1 + (size_t)j * (nx + 1)] - tq; faces++; }
+          if (tipo[c - nx] == AGUA) { rhs += (double)v[(size_t)i + (size_t)j * nx] - tq; faces++; }
+          if (tipo[c + nx] == AGUA) { rhs += -(double)v[(size_t)i + (size_t)(j + 1) * nx] - tq; faces++; }
+        }
       }
+    }
+    bool pending = r >= 2;
+    while (true) {
+      unsigned long long b = __ballot(pending);
+      if (!b) break;
+      const int leader = __ffsll((long long)b) - 1;
+      const uint32_t lr = __shfl(r, leader, 64);
+      const bool m = pending && r == lr;
+      const long long sr = wave_sum_i64(m ? (long long)__builtin_rint(rhs * RHS_FX) : 0LL);
+      unsigned long long bm = __ballot(m);
+      unsigned long long bi = __ballot(m && interior);
+      int sf = (int)wave_sum(m ? (double)faces : 0.0);
+      if (lane == leader) {
+        int sl = -1;
+        for (int k = 0; k < BRHS_SLOTS; k++) {  // open addressing from the label's home slot
+          const int hs = (int)((lr + (uint32_t)k) % BRHS_SLOTS);
+          const uint32_t old = atomicCAS(&key[hs], 0u, lr);
+          if (old == 0u || old == lr) { sl = hs; break; }
+        }
+        if (sl >= 0) {
+          atomicAdd(&kc[sl], (uint32_t)__popcll(bm));
+          if (bi) {
+            atomicAdd(&kv[sl], (uint32_t)__popcll(bi));
+            atomicAdd(&kf[sl], (uint32_t)sf);
+            atomicAdd(&ks[sl], (unsigned long long)sr);
+          }
+        } else {
+          atomicAdd(&cellsB[lr], (uint32_t)__popcll(bm));
+          if (bi) {
+            atomicAdd(&volB[lr], (uint32_t)__popcll(bi));
+            atomicAdd(&facesB[lr], (uint32_t)sf);
+            if (!counts_only) atomicAdd((unsigned long long*)&rhsB[lr], (unsigned long long)sr);
+          }
+        }
+      }
+      if (m) pending = false;
     }
   }
-  const int lane = threadIdx.x & 63;
-  bool pending = r >= 2;
-  while (true) {
-    unsigned long long b = __ballot(pending);
-    if (!b) break;
-    const int leader = __ffsll((long long)b) - 1;
-    const uint32_t lr = __shfl(r, leader, 64);
-    const bool m = pending && r == lr;
-    const long long sr = wave_sum_i64(m ? (long long)__builtin_rint(rhs * RHS_FX) : 0LL);
-    unsigned long long bm = __ballot(m);
-    unsigned long long bi = __ballot(m && interior);
-    int sf = (int)wave_sum(m ? (double)faces : 0.0);
-    if (lane == leader) {
-      atomicAdd(&cellsB[lr], (uint32_t)__popcll(bm));
-      if (bi) {
-        atomicAdd(&volB[lr], (uint32_t)__popcll(bi));
-        atomicAdd(&facesB[lr], (uint32_t)sf);
-        if (!counts_only) atomicAdd((unsigned long long*)&rhsB[lr], (unsigned long long)sr);
-      }
+  __syncthreads();
+  for (int k = threadIdx.x; k < BRHS_SLOTS; k += blockDim.x) {
+    const uint32_t lr = key[k];
+    if (!lr) continue;
+    atomicAdd(&cellsB[lr], kc[k]);
+    if (kv[k]) {
+      atomicAdd(&volB[lr], kv[k]);
+      atomicAdd(&facesB[lr], kf[k]);
+      if (!counts_only) atomicAdd((unsigned long long*)&rhsB[lr], ks[k]);
     }
-    if (m) pending = false;
   }
 }
 
@@ -381,8 +426,8 @@ int bubble_rhs(tq_sim* s, double dt) {
   TQ_HIP(hipMemsetAsync(s->cellsB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
   TQ_HIP(hipMemsetAsync(s->rhsB, 0, sizeof(double) * MAX_BOLHAS, s->st));
   int t = time_begin(s, "bubble_rhs", (1 + 4 + 4 + 4 + 4) * (double)s->C);
-  hipLaunchKernelGGL(k_bubble_rhs, dim3((s->C + 255) / 256), dim3(256), 0, s->st, s->tipo, s->rot, s->u, s->v, s->qAr,
-                     s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB, 0);
+  hipLaunchKernelGGL(k_bubble_rhs, dim3(std::min((s->C + 255) / 256, BRHS_GRID)), dim3(256), 0, s->st, s->tipo, s->rot,
+                     s->u, s->v, s->qAr, s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB, 0);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_bubble_finish, dim3(1), dim3(1024), 0, s->st, s->volB, s->rhsB, s->rhsB32, s->yB32, s->pBolha,
@@ -397,8 +442,8 @@ int bubble_counts(tq_sim* s) {
   TQ_HIP(hipMemsetAsync(s->volB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
   TQ_HIP(hipMemsetAsync(s->facesB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
   TQ_HIP(hipMemsetAsync(s->cellsB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
-  hipLaunchKernelGGL(k_bubble_rhs, dim3((s->C + 255) / 256), dim3(256), 0, s->st, s->tipo, s->rot, s->u, s->v, s->qAr,
-                     s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB, 1);
+  hipLaunchKernelGGL(k_bubble_rhs, dim3(std::min((s->C + 255) / 256, BRHS_GRID)), dim3(256), 0, s->st, s->tipo, s->rot,
+                     s->u, s->v, s->qAr, s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB, 1);
   TQ_LAUNCH_CHECK();
   return 0;
 }

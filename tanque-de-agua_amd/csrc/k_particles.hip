@@ -112,18 +112,40 @@ int scan_u32(tq_sim* s, const uint32_t* in, uint32_t* out, int n) {
 }
 
 // ================================================================== binning + counting sort
+// Cell key and rank within the cell of every particle.  Particles arrive in cell order (the
+// previous substep's sort), so runs of lanes share a key: the run's first lane takes the run's
+// ranks with one atomic and the others count from it, instead of one same-address atomic per
+// lane.  Ranks within a cell depend on the order the atomics land; every consumer of the sorted
+// order sums in fixed point or works per particle, so results do not.
 __global__ __launch_bounds__(256) void k_bin(const float2* __restrict__ pos, const DevScalars* __restrict__ sc, int nx, int ny,
                                              double inv, int ntx, uint32_t* __restrict__ countKey,
                                              uint32_t* __restrict__ pkey, uint32_t* __restrict__ prank) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= sc->n_slots) return;
-  float2 p = pos[k];
-  if (p.x != p.x) { pkey[k] = DEAD_KEY; return; }
-  int ci, cj;
-  count_cell(p.x, p.y, inv, nx, ny, &ci, &cj);
-  uint32_t key = cell_key(ci, cj, ntx);
-  pkey[k] = key;
-  prank[k] = atomicAdd(&countKey[key], 1u);
+  const bool slot = k < sc->n_slots;
+  uint32_t key = DEAD_KEY;
+  if (slot) {
+    const float2 p = pos[k];
+    if (p.x == p.x) {
+      int ci, cj;
+      count_cell(p.x, p.y, inv, nx, ny, &ci, &cj);
+      key = cell_key(ci, cj, ntx);
+    }
+    pkey[k] = key;
+  }
+  const int lane = threadIdx.x & 63;
+  const uint32_t prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0 || key != prev;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
+  const int hl = 63 - __clzll(heads & upto);  // this lane's run head
+  uint32_t base = 0;
+  if (head && key != DEAD_KEY) {
+    const unsigned long long after = heads & ~upto;
+    const int next = after ? __ffsll((long long)after) - 1 : 64;
+    base = atomicAdd(&countKey[key], (uint32_t)(next - lane));
+  }
+  base = __shfl(base, hl, 64);
+  if (slot && key != DEAD_KEY) prank[k] = base + (uint32_t)(lane - hl);
 }
 
 __global__ __launch_bounds__(256) void k_scatter(const float2* __restrict__ pos, const float2* __restrict__ vel,

@@ -2164,6 +2164,9 @@ static void graph_drop(tq_sim* s) {
   s->graph_key = -1;
 }
 
+// a launch cap changed: the captured PCG iterations hold the old grids
+void solver_launch_changed(tq_sim* s) { graph_drop(s); }
+
 void solver_free(tq_sim* s) {
   graph_drop(s);
   for (auto& L : s->lv) {
@@ -2254,7 +2257,7 @@ static void launch_in_a0(tq_sim* s, bool upd) {
   const int ntx = tiles_of(F.nx, 0), nty = tiles_of(F.ny, 0);
   const dim3 tg = tile_grid(F.nx, F.ny, 0);
   const int vgx = (int)tg.x, nvb = (int)(tg.x * tg.y);
-  const dim3 g(std::min(nvb, IN_A0_GRID));
+  const dim3 g(std::min(nvb, s->launch_update_grid > 0 ? s->launch_update_grid : IN_A0_GRID));
   const bool pa = s->lv.size() > 1;
   const float* pp = s->pbuf[s->pc];
   // bytes: opMask, mg0, b0 write, x0 write; update: q, p, r, Ap read + q, r write
@@ -2562,7 +2565,8 @@ static void precondition_rest(tq_sim* s) {
 // the V-cycle (level 0's down kernel starts with the norm and stop test)
 static void pcg_iteration(tq_sim* s) {
   const int C = s->C;
-  const int nbs = std::min((C + 256 * SPMV_CELLS - 1) / (256 * SPMV_CELLS), SPMV_GRID);
+  const int nbs = std::min((C + 256 * SPMV_CELLS - 1) / (256 * SPMV_CELLS),
+                           s->launch_spmv_grid > 0 ? s->launch_spmv_grid : SPMV_GRID);
   const int pn = 1 - s->pc;
   const RedSrc rz = red_src(s, PS_Z, 2, 1, s->nb_z);
   int t = time_begin(s, "pcg_spmv", (1 + 4 + 4 + 4 + 4) * (double)C);

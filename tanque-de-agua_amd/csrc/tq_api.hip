@@ -723,6 +723,34 @@ int tq_timing_enable(tq_sim* s, int enable, const char* only) {
   return 0;
 }
 
+// launch caps by name: the field and the largest value a grid may take
+static int* launch_field(tq_sim* s, const char* name) {
+  if (!name) return nullptr;
+  if (!std::strcmp(name, "spmv_grid")) return &s->launch_spmv_grid;
+  if (!std::strcmp(name, "update_grid")) return &s->launch_update_grid;
+  return nullptr;
+}
+
+int tq_set_launch(tq_sim* s, const char* name, int value) {
+  TQ_CHECK_SIM(s);
+  int* f = launch_field(s, name);
+  if (!f) { set_error("tq_set_launch: unknown launch parameter"); return TQ_E_ARG; }
+  if (value < 0 || value > MAX_RED_BLOCKS) { set_error("tq_set_launch: value out of range"); return TQ_E_ARG; }
+  if (*f != value) {
+    *f = value;
+    solver_launch_changed(s);
+  }
+  return 0;
+}
+
+int tq_get_launch(tq_sim* s, const char* name, int* value) {
+  TQ_CHECK_SIM(s);
+  int* f = launch_field(s, name);
+  if (!f || !value) { set_error("tq_get_launch: unknown launch parameter or null"); return TQ_E_ARG; }
+  *value = *f;
+  return 0;
+}
+
 int tq_timing_read(tq_sim* s, tq_kernel_time* out, int max, int* n) {
   TQ_CHECK_SIM(s);
   timing_flush(s);

@@ -201,6 +201,8 @@ struct tq_sim {
   double* bpart_r = nullptr;
   int it_k = -1;                // index of the PCG iteration being launched (-1: the initial V-cycle)
   int nb_spmv = 0, nb_upd = 0, nb_z = 0;  // partial counts of the last spmv, update and z launches
+  int launch_spmv_grid = 0;     // most blocks of spmv (0: the built-in cap)
+  int launch_update_grid = 0;   // most blocks of the PCG update / level-0 input kernel (0: the built-in cap)
   int* labelSlot = nullptr;     // bubble label -> reduction slot (-1: FP64 atomics)
   int lc = 0;                   // first multigrid level solved inside the one-workgroup kernel
   int mg_scaled = 0;            // this solve stores some multigrid level in FP16 (input scaled)
@@ -289,6 +291,7 @@ int tracker_reset(tq_sim* s);
 int solver_alloc(tq_sim* s);
 void solver_free(tq_sim* s);
 int pressure_solve(tq_sim* s);
+void solver_launch_changed(tq_sim* s);
 int true_residual(tq_sim* s, double* out);
 int debug_vcycle(tq_sim* s, const float* b, float* z);
 // ---- tq_api.hip

@@ -106,6 +106,8 @@ SIGNATURES = {
     "tq_solver_stats_get": (C.c_int, [C.c_void_p, C.POINTER(SolverStats), C.c_int]),
     "tq_timing_enable": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p]),
     "tq_timing_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int, C.POINTER(C.c_int)]),
+    "tq_set_launch": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "tq_get_launch": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int)]),
     "tq_sync": (C.c_int, [C.c_void_p]),
     "tq_last_error": (C.c_char_p, []),
     "tq_scene_frame": (None, [C.c_void_p, C.c_int, C.c_int]),
@@ -478,6 +480,15 @@ class SimulacaoHIP:
 
     def timing_enable(self, on=True, only=None):
         _chk(load().tq_timing_enable(self._h, 1 if on else 0, only.encode() if only else None))
+
+    def set_launch(self, name, value):
+        """Launch cap of a solver kernel ("spmv_grid", "update_grid"; 0 = default)."""
+        _chk(load().tq_set_launch(self._h, name.encode(), int(value)))
+
+    def get_launch(self, name):
+        v = C.c_int()
+        _chk(load().tq_get_launch(self._h, name.encode(), C.byref(v)))
+        return v.value
 
     def timing_read(self):
         n = C.c_int()

@@ -213,3 +213,35 @@ def test_dense_operators_rebuilt_when_masks_change(tq, oracle):
     log("  operator builds %d over %d solves" % (st["operator_builds"], st["solves"]))
     assert 1 <= st["operator_builds"] <= st["solves"]
     assert worst < 1e-3, worst
+
+
+def test_launch_caps(tq, oracle):
+    """tq_set_launch: a 2.36M-cell solve with the spmv / update grids capped lower than the
+    defaults walks more cells per block; the same caps twice give identical bits, other caps
+    the same pressure within the solve's tolerance.  Unknown names and negative caps are refused."""
+    osim = prepared(oracle, lambda o: dam(o, nx=2048, ny=1152, h=0.8 / 1152, seed=0), 1)
+    dt = substep_dt(oracle, osim)
+    for ph in (0, 1, 2, 3, 4, 5, 6, 7):
+        osim.fase(ph, dt)
+    out = {}
+    for caps in ((0, 0), (256, 640), (256, 640), (4096, 4096)):
+        g = hip_sim_like(tq, osim, iteracoes=200, tol=TOL)
+        g.set_launch("spmv_grid", caps[0])
+        g.set_launch("update_grid", caps[1])
+        assert (g.get_launch("spmv_grid"), g.get_launch("update_grid")) == caps
+        clone_state(osim, g)
+        g.fase(8, dt)
+        st = g.solver_stats()
+        q = g.read(12)
+        if caps in out:
+            assert np.array_equal(q, out[caps][0]), caps
+        out[caps] = (q, st["iterations"])
+        log("\n  caps %s: %d iterations" % (caps, st["iterations"]))
+        if caps == (4096, 4096):
+            with pytest.raises(tq.TqError):
+                g.set_launch("no_such_kernel", 1)
+            with pytest.raises(tq.TqError):
+                g.set_launch("spmv_grid", -1)
+    q0 = out[(0, 0)][0]
+    for caps, (q, it) in out.items():
+        assert rel_err(q, q0) <= RTOL_Q, (caps, rel_err(q, q0))
