@@ -541,8 +541,19 @@ __global__ __launch_bounds__(256) void k_g2p(float2* __restrict__ pos, float2* _
       }
     }
   }
+  // top speed: block maximum, then one atomic per block and only when it beats the value already
+  // there (every block's atomic on the one word would serialise in the cache)
+  __shared__ double wmax[4];
   maior = wave_maxd(maior);
-  if ((threadIdx.x & 63) == 0 && maior > 0) atomicMax(vmax_bits, (unsigned long long)__double_as_longlong(maior));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = maior;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmax[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); q++) m = fmax(m, wmax[q]);
+    const unsigned long long cur = __hip_atomic_load(vmax_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (m > 0 && (unsigned long long)__double_as_longlong(m) > cur)
+      atomicMax(vmax_bits, (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 int g2p(tq_sim* s, double dt) {
@@ -707,10 +718,18 @@ int apply_tool_dev(tq_sim* s, const tq_tool* t) {
 // ================================================================== live count
 __global__ __launch_bounds__(256) void k_count_live(const float2* __restrict__ pos, const DevScalars* __restrict__ sc,
                                                     uint32_t* __restrict__ out) {
+  // one atomic per block (per-wave atomics on the one counter serialise in the cache)
+  __shared__ uint32_t wc[4];
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t live = (k < sc->n_slots && pos[k].x == pos[k].x) ? 1u : 0u;
   unsigned long long b = __ballot(live);
-  if ((threadIdx.x & 63) == 0 && b) atomicAdd(out, (uint32_t)__popcll(b));
+  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = (uint32_t)__popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += wc[q];
+    if (t) atomicAdd(out, t);
+  }
 }
 
 int count_live(tq_sim* s, int* n) {
