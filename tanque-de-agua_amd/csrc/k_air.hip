@@ -160,7 +160,35 @@ __device__ inline uint64_t hmix(uint64_t k) {
   return k;
 }
 
-// per-cell: component index, per-component statistics, overlap table
+// one (component, old label) group of `cnt` cells whose first cell in raster order is `first`
+// into the overlap hash table (open addressing; new entries listed in hlist)
+__device__ inline void overlap_add(unsigned long long key, uint32_t cnt, uint32_t first,
+                                   unsigned long long* __restrict__ hkeys, uint32_t* __restrict__ hcount,
+                                   uint32_t* __restrict__ hfirst, uint32_t* __restrict__ hlist, int hmask,
+                                   DevScalars* __restrict__ sc) {
+  uint32_t h = (uint32_t)hmix(key) & (uint32_t)hmask;
+  while (true) {
+    const unsigned long long prev = atomicCAS(&hkeys[h], HEMPTY, key);
+    if (prev == HEMPTY) {
+      const uint32_t idx = atomicAdd(&sc->npairs, 1u);
+      hlist[idx] = h;
+      break;
+    }
+    if (prev == key) break;
+    h = (h + 1) & (uint32_t)hmask;
+  }
+  atomicAdd(&hcount[h], cnt);
+  atomicMin(&hfirst[h], first);
+}
+
+// per-cell: component index, per-component statistics, overlap table.  Each wave groups its
+// cells by component (and by component and old label); the wave leaders add into block tables
+// in LDS (CS_SLOTS entries each), flushed to the global arrays once per block, so a component
+// of millions of cells costs a few thousand global atomics instead of several per wave.
+// Groups the full tables cannot take go to the global arrays directly.  Every sum is an
+// integer count, a flag or a minimum, so the order of the adds does not matter.
+constexpr int CS_SLOTS = 32;
+constexpr int CS_GRID = 2048;
 __global__ __launch_bounds__(256) void k_comp_stats(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
                                                     int nx, int ny, const int32_t* __restrict__ pai,
                                                     const uint32_t* __restrict__ compIdx, int32_t* __restrict__ comp,
@@ -169,64 +197,101 @@ __global__ __launch_bounds__(256) void k_comp_stats(const uint8_t* __restrict__ 
                                                     uint32_t* __restrict__ hcount, uint32_t* __restrict__ hfirst,
                                                     uint32_t* __restrict__ hlist, int hmask,
                                                     DevScalars* __restrict__ sc) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int ck[CS_SLOTS];
+  __shared__ uint32_t cc[CS_SLOTS], ca[CS_SLOTS], ct[CS_SLOTS];
+  __shared__ unsigned long long ok[CS_SLOTS];
+  __shared__ uint32_t oc[CS_SLOTS], of[CS_SLOTS];
+  for (int e = threadIdx.x; e < CS_SLOTS; e += blockDim.x) {
+    ck[e] = -1;
+    cc[e] = ca[e] = ct[e] = 0;
+    ok[e] = HEMPTY;
+    oc[e] = 0;
+    of[e] = 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  const int C = nx * ny;
   const int lane = threadIdx.x & 63;
-  int k = -1;
-  uint32_t old = 0;
-  bool istop = false;
-  if (c < nx * ny) {
-    if (tipo[c] == AR) {
-      int r = pai[c];
-      while (pai[r] != r) r = pai[r];
-      k = (int)compIdx[r];
-      old = rot[c];
-      istop = (c / nx) >= ny - MOLDURA - 1;
-    }
-    comp[c] = k;
-  }
-  // statistics grouped by component within the wavefront
-  bool pending = k >= 0;
-  while (true) {
-    unsigned long long b = __ballot(pending);
-    if (!b) break;
-    const int leader = __ffsll((long long)b) - 1;
-    const int lk = __shfl(k, leader, 64);
-    const bool m = pending && k == lk;
-    const unsigned long long bm = __ballot(m), ba = __ballot(m && old == 1), bt = __ballot(m && istop);
-    if (lane == leader) {
-      atomicAdd(&cells[lk], (uint32_t)__popcll(bm));
-      if (ba) atomicAdd(&atm[lk], (uint32_t)__popcll(ba));
-      if (bt) atomicOr(&topo[lk], 1u);
-    }
-    if (m) pending = false;
-  }
-  // overlap table entries for old bubble labels
-  pending = k >= 0 && old >= 2;
-  while (true) {
-    unsigned long long b = __ballot(pending);
-    if (!b) break;
-    const int leader = __ffsll((long long)b) - 1;
-    const int lk = __shfl(k, leader, 64);
-    const uint32_t lo = __shfl(old, leader, 64);
-    const bool m = pending && k == lk && old == lo;
-    const unsigned long long bm = __ballot(m);
-    if (lane == leader) {
-      const unsigned long long key = ((unsigned long long)(uint32_t)lk << 12) | lo;
-      uint32_t h = (uint32_t)hmix(key) & (uint32_t)hmask;
-      while (true) {
-        const unsigned long long prev = atomicCAS(&hkeys[h], HEMPTY, key);
-        if (prev == HEMPTY) {
-          const uint32_t idx = atomicAdd(&sc->npairs, 1u);
-          hlist[idx] = h;
-          break;
-        }
-        if (prev == key) break;
-        h = (h + 1) & (uint32_t)hmask;
+  for (int base = blockIdx.x * blockDim.x; base < C; base += gridDim.x * blockDim.x) {
+    const int c = base + threadIdx.x;
+    int k = -1;
+    uint32_t old = 0;
+    bool istop = false;
+    if (c < C) {
+      if (tipo[c] == AR) {
+        int r = pai[c];
+        while (pai[r] != r) r = pai[r];
+        k = (int)compIdx[r];
+        old = rot[c];
+        istop = (c / nx) >= ny - MOLDURA - 1;
       }
-      atomicAdd(&hcount[h], (uint32_t)__popcll(bm));
-      atomicMin(&hfirst[h], (uint32_t)c);  // the leader is the group's first cell in raster order
+      comp[c] = k;
     }
-    if (m) pending = false;
+    // statistics grouped by component within the wavefront
+    bool pending = k >= 0;
+    while (true) {
+      unsigned long long b = __ballot(pending);
+      if (!b) break;
+      const int leader = __ffsll((long long)b) - 1;
+      const int lk = __shfl(k, leader, 64);
+      const bool m = pending && k == lk;
+      const unsigned long long bm = __ballot(m), ba = __ballot(m && old == 1), bt = __ballot(m && istop);
+      if (lane == leader) {
+        int sl = -1;
+        for (int q = 0; q < CS_SLOTS; q++) {
+          const int hs = (int)(((uint32_t)lk + (uint32_t)q) % CS_SLOTS);
+          const int prev = atomicCAS(&ck[hs], -1, lk);
+          if (prev == -1 || prev == lk) { sl = hs; break; }
+        }
+        if (sl >= 0) {
+          atomicAdd(&cc[sl], (uint32_t)__popcll(bm));
+          if (ba) atomicAdd(&ca[sl], (uint32_t)__popcll(ba));
+          if (bt) atomicOr(&ct[sl], 1u);
+        } else {
+          atomicAdd(&cells[lk], (uint32_t)__popcll(bm));
+          if (ba) atomicAdd(&atm[lk], (uint32_t)__popcll(ba));
+          if (bt) atomicOr(&topo[lk], 1u);
+        }
+      }
+      if (m) pending = false;
+    }
+    // overlap table entries for old bubble labels
+    pending = k >= 0 && old >= 2;
+    while (true) {
+      unsigned long long b = __ballot(pending);
+      if (!b) break;
+      const int leader = __ffsll((long long)b) - 1;
+      const int lk = __shfl(k, leader, 64);
+      const uint32_t lo = __shfl(old, leader, 64);
+      const bool m = pending && k == lk && old == lo;
+      const unsigned long long bm = __ballot(m);
+      if (lane == leader) {
+        const unsigned long long key = ((unsigned long long)(uint32_t)lk << 12) | lo;
+        int sl = -1;
+        for (int q = 0; q < CS_SLOTS; q++) {
+          const int hs = (int)(((uint32_t)hmix(key) + (uint32_t)q) % CS_SLOTS);
+          const unsigned long long prev = atomicCAS(&ok[hs], HEMPTY, key);
+          if (prev == HEMPTY || prev == key) { sl = hs; break; }
+        }
+        // the leader is the group's first cell in raster order
+        if (sl >= 0) {
+          atomicAdd(&oc[sl], (uint32_t)__popcll(bm));
+          atomicMin(&of[sl], (uint32_t)c);
+        } else {
+          overlap_add(key, (uint32_t)__popcll(bm), (uint32_t)c, hkeys, hcount, hfirst, hlist, hmask, sc);
+        }
+      }
+      if (m) pending = false;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < CS_SLOTS; e += blockDim.x) {
+    const int lk = ck[e];
+    if (lk >= 0) {
+      atomicAdd(&cells[lk], cc[e]);
+      if (ca[e]) atomicAdd(&atm[lk], ca[e]);
+      if (ct[e]) atomicOr(&topo[lk], 1u);
+    }
+    if (ok[e] != HEMPTY) overlap_add(ok[e], oc[e], of[e], hkeys, hcount, hfirst, hlist, hmask, sc);
   }
 }
 
@@ -384,7 +449,8 @@ int tracker_update(tq_sim* s) {
   hipLaunchKernelGGL(k_uf_flatten, g, b, 0, s->st, s->tipo, C, s->pai, s->rootFlag);
   TQ_LAUNCH_CHECK();
   if (int e = scan_u32(s, s->rootFlag, s->compIdx, C)) return e;
-  hipLaunchKernelGGL(k_comp_stats, g, b, 0, s->st, s->tipo, s->rot, s->nx, s->ny, s->pai, s->compIdx, s->comp,
+  hipLaunchKernelGGL(k_comp_stats, dim3(std::min((C + 255) / 256, CS_GRID)), b, 0, s->st, s->tipo, s->rot, s->nx, s->ny,
+                     s->pai, s->compIdx, s->comp,
                      s->compCells, s->compAtm, s->compTopo, s->hkeys, s->hcount, s->hfirst, s->hlist, s->hsize - 1, s->sc);
   hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(1024), 0, s->st, s->compIdx + C, s->compCells, s->compAtm, s->compTopo,
                      s->compAlvo, s->compDono, s->compLabel, s->hkeys, s->hcount, s->hfirst, s->hlist,
