@@ -420,11 +420,20 @@ __global__ __launch_bounds__(1024) void k_bubble_finish(const uint32_t* __restri
   }
 }
 
+// the per-bubble sums of k_bubble_rhs start at zero (one launch instead of four fills)
+__global__ __launch_bounds__(256) void k_bubble_clear(uint32_t* __restrict__ volB, uint32_t* __restrict__ facesB,
+                                                      uint32_t* __restrict__ cellsB, double* __restrict__ rhsB) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= MAX_BOLHAS) return;
+  volB[r] = 0;
+  facesB[r] = 0;
+  cellsB[r] = 0;
+  if (rhsB) rhsB[r] = 0.0;
+}
+
 int bubble_rhs(tq_sim* s, double dt) {
-  TQ_HIP(hipMemsetAsync(s->volB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
-  TQ_HIP(hipMemsetAsync(s->facesB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
-  TQ_HIP(hipMemsetAsync(s->cellsB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
-  TQ_HIP(hipMemsetAsync(s->rhsB, 0, sizeof(double) * MAX_BOLHAS, s->st));
+  hipLaunchKernelGGL(k_bubble_clear, dim3((MAX_BOLHAS + 255) / 256), dim3(256), 0, s->st, s->volB, s->facesB, s->cellsB,
+                     s->rhsB);
   int t = time_begin(s, "bubble_rhs", (1 + 4 + 4 + 4 + 4) * (double)s->C);
   hipLaunchKernelGGL(k_bubble_rhs, dim3(std::min((s->C + 255) / 256, BRHS_GRID)), dim3(256), 0, s->st, s->tipo, s->rot,
                      s->u, s->v, s->qAr, s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB, 0);
@@ -439,9 +448,8 @@ int bubble_rhs(tq_sim* s, double dt) {
 // Per-bubble cell and face counts only (the rhs is kept): lets a solve start from a state whose
 // bubble sums were computed elsewhere.
 int bubble_counts(tq_sim* s) {
-  TQ_HIP(hipMemsetAsync(s->volB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
-  TQ_HIP(hipMemsetAsync(s->facesB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
-  TQ_HIP(hipMemsetAsync(s->cellsB, 0, sizeof(uint32_t) * MAX_BOLHAS, s->st));
+  hipLaunchKernelGGL(k_bubble_clear, dim3((MAX_BOLHAS + 255) / 256), dim3(256), 0, s->st, s->volB, s->facesB, s->cellsB,
+                     (double*)nullptr);
   hipLaunchKernelGGL(k_bubble_rhs, dim3(std::min((s->C + 255) / 256, BRHS_GRID)), dim3(256), 0, s->st, s->tipo, s->rot,
                      s->u, s->v, s->qAr, s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB, 1);
   TQ_LAUNCH_CHECK();

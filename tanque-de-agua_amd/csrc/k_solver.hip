@@ -477,7 +477,7 @@ __global__ __launch_bounds__(256) void k_viz_all(VizPlan P, uint8_t* __restrict_
 __global__ __launch_bounds__(1024) void k_bub_init(const uint32_t* __restrict__ facesB, const float* __restrict__ yB32,
                                                    int ar, double* __restrict__ bvec, int* __restrict__ activeList,
                                                    int* __restrict__ labelSlot, DevScalars* __restrict__ sc) {
-  __shared__ int cnt[1024];
+  __shared__ int cnt[16];
   const int tid = threadIdx.x;
   int mine = 0;
   for (int k = 0; k < 4; k++) {
@@ -488,16 +488,25 @@ __global__ __launch_bounds__(1024) void k_bub_init(const uint32_t* __restrict__ 
     if (act) bv(bvec, BV_Y)[r] = (double)yB32[r];
     labelSlot[r] = -1;
   }
-  cnt[tid] = mine;
+  // exclusive scan of the per-thread counts in label order: within each wavefront by shuffles,
+  // then over the 16 wavefront totals
+  const int lane = tid & 63, w = tid >> 6;
+  int incl = mine;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int v = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += v;
+  }
+  if (lane == 63) cnt[w] = incl;
   __syncthreads();
   if (tid == 0) {
     int acc = 0;
-    for (int t = 0; t < 1024; t++) { int c = cnt[t]; cnt[t] = acc; acc += c; }
+    for (int q = 0; q < 16; q++) { const int c = cnt[q]; cnt[q] = acc; acc += c; }
     sc->active_bubbles = acc;
     sc->op_dirty = 0;
   }
   __syncthreads();
-  int o = cnt[tid];
+  int o = cnt[w] + incl - mine;
   for (int k = 0; k < 4; k++) {
     const int r = tid * 4 + k;
     if (ar && r >= 2 && facesB[r] > 0) {
