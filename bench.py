@@ -126,36 +126,41 @@ def run_alto(tq, work, precision, tol=1e-6, maxit=200, schedule=None):
 def run_batched(tq, frames=20, threads=8):
     """The batched configuration (64 maze tanks, fixed 3 PCG iterations as the reference): the
     same frames stepped tank after tank (sequential launches, each tank waited for) and as one
-    batch (tq_batch_step), each from fresh identical states; tank-steps/s = tanks x frames / s."""
+    batch (tq_batch_step), each from fresh identical states; tank-steps/s = tanks x frames / s.
+    FP32, then the mixed-precision preconditioner (under "mixed")."""
     from tanque_hip import scenes
     from tanque_hip.scenes import passo_frame
     ts = scenes.batched_mazes()
     res = {"tanks": len(ts), "cells": sum(t.nx * t.ny for t in ts), "frames_timed": frames,
            "of_frames": ts[0].frames, "threads": threads, "pcg": "3 iterations (reference)"}
-    for mode in ("sequential", "batched"):
-        cenas = [t.cena() for t in ts]
-        b = tq.Batch(cenas, [{"iteracoes": 3}] * len(ts), threads=threads)
-        for t, s in zip(ts, b.tanks):
-            s.adicionar(t.particles, False)
-        res["particles"] = sum(s.n for s in b.tanks)
-        dt = ts[0].dt_frame
+    for prec, key in ((tq.PREC_FP32, None), (tq.PREC_MIXED, "mixed")):
+        out = res if key is None else res.setdefault(key, {})
+        for mode in ("sequential", "batched"):
+            cenas = [t.cena() for t in ts]
+            b = tq.Batch(cenas, [{"iteracoes": 3, "precision": prec}] * len(ts), threads=threads)
+            for t, s in zip(ts, b.tanks):
+                s.adicionar(t.particles, False)
+            res["particles"] = sum(s.n for s in b.tanks)
+            dt = ts[0].dt_frame
 
-        def frame():
-            subs = [passo_frame(t.h, c.altura, s.vMax, dt) for t, c, s in zip(ts, cenas, b.tanks)]
-            if mode == "sequential":
-                for s, k in zip(b.tanks, subs):
-                    s.avancar(dt, k)
-            else:
-                b.step(dt, subs)
-            return sum(subs)
+            def frame():
+                subs = [passo_frame(t.h, c.altura, s.vMax, dt) for t, c, s in zip(ts, cenas, b.tanks)]
+                if mode == "sequential":
+                    for s, k in zip(b.tanks, subs):
+                        s.avancar(dt, k)
+                else:
+                    b.step(dt, subs)
+                return sum(subs)
 
-        frame()
-        t0 = time.perf_counter()
-        subs = sum(frame() for _ in range(frames))
-        el = time.perf_counter() - t0
-        res[mode] = {"tank_steps_per_s": round(len(ts) * frames / el, 1), "tank_substeps_per_s": round(subs / el, 1)}
-        b.close()
-    res["batched_over_sequential"] = round(res["batched"]["tank_steps_per_s"] / res["sequential"]["tank_steps_per_s"], 3)
+            frame()
+            t0 = time.perf_counter()
+            subs = sum(frame() for _ in range(frames))
+            el = time.perf_counter() - t0
+            out[mode] = {"tank_steps_per_s": round(len(ts) * frames / el, 1), "tank_substeps_per_s": round(subs / el, 1)}
+            b.close()
+        out["batched_over_sequential"] = round(out["batched"]["tank_steps_per_s"] / out["sequential"]["tank_steps_per_s"], 3)
+    res["mixed_batched_over_fp32_batched"] = round(res["mixed"]["batched"]["tank_steps_per_s"] /
+                                                   res["batched"]["tank_steps_per_s"], 3)
     return res
 
 

@@ -19,14 +19,16 @@ def tq():
     return tanque_hip
 
 
-def test_batch_matches_tanks_stepped_alone(tq):
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+def test_batch_matches_tanks_stepped_alone(tq, prec):
     from tanque_hip import scenes
     from tanque_hip.scenes import passo_frame
+    p = tq.PREC_FP32 if prec == "fp32" else tq.PREC_MIXED
     ts = scenes.batched_mazes(n=5, seed=3)
-    opts = {"iteracoes": 3, "ar": False}
+    opts = {"iteracoes": 3, "ar": False, "precision": p}
     cenas = [t.cena() for t in ts]
     b = tq.Batch(cenas, [opts] * len(ts), threads=3)
-    alone = [tq.SimulacaoHIP(t.cena(), iteracoes=3, ar=False) for t in ts]
+    alone = [tq.SimulacaoHIP(t.cena(), iteracoes=3, ar=False, precision=p) for t in ts]
     for t, sb, sa in zip(ts, b.tanks, alone):
         sb.adicionar(t.particles, False)
         sa.adicionar(t.particles, False)
@@ -37,7 +39,7 @@ def test_batch_matches_tanks_stepped_alone(tq):
             sa.avancar(ts[0].dt_frame, k)
     for k, (t, sb, sa) in enumerate(zip(ts, b.tanks, alone)):
         pb, pa = sb.particulas(), sa.particulas()
-        log("\n  tank %d (%dx%d): %d particles batched, %d alone" % (k, t.nx, t.ny, len(pb), len(pa)))
+        log("\n  %s tank %d (%dx%d): %d particles batched, %d alone" % (prec, k, t.nx, t.ny, len(pb), len(pa)))
         assert len(pb) == len(pa) > 0
         assert np.array_equal(pb.view(np.uint32), pa.view(np.uint32))
         assert np.array_equal(sb.read(12), sa.read(12))
