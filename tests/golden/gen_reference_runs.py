@@ -1,6 +1,7 @@
 """Writes tests/golden/reference_runs.json: golden vectors of the reference's own CPU engine.
 
-Run in the build container only (the reference is not on the GPU box):
+Run where the reference's sources are checked out (they are not part of this repository; the
+committed fixture is what the tests read):
     python tests/golden/gen_reference_runs.py [/root/reference]
 
 The reference (js/constantes.js, cena.js, fisica-cpu.js, pressao-cpu.js, ar.js, modelos.js,
