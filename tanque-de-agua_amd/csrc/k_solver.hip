@@ -33,13 +33,41 @@
 namespace tq {
 
 // bubble vectors (MAX_BOLHAS doubles each): y, ap, and by iteration parity r, z, p and the
-// atomic accumulators of spmv (A) and of the z dots (Z); I: the accumulator outside the iterations
+// accumulators of spmv (A) and of the z dots (Z); I: the accumulator outside the iterations.
+// Each accumulator holds the low 64 bits of a 128-bit fixed-point sum; its high 64 bits sit
+// BV_FX_HI vectors further on (see fx_add).
 enum {
   BV_Y = 0, BV_AP, BV_R0, BV_R1, BV_Z0, BV_Z1, BV_P0, BV_P1,
-  BV_ACCA0, BV_ACCA1, BV_ACCZ0, BV_ACCZ1, BV_ACCI, BV_N
+  BV_ACCA0, BV_ACCA1, BV_ACCZ0, BV_ACCZ1, BV_ACCI,
+  BV_ACCA0_HI, BV_ACCA1_HI, BV_ACCZ0_HI, BV_ACCZ1_HI, BV_ACCI_HI, BV_N
 };
+constexpr int BV_FX_HI = BV_ACCA0_HI - BV_ACCA0;
 static_assert(BV_N == BV_COUNT, "bubble vector count");
+static_assert(BV_ACCI_HI - BV_ACCI == BV_FX_HI, "accumulator high words");
 __device__ __host__ inline double* bv(double* base, int k) { return base + (size_t)k * MAX_BOLHAS; }
+
+// Sums of bubbles without a reduction slot, in 128-bit fixed point with 2^-64 resolution: an
+// addend v = hi + lo 2^-64 (hi = floor(v), lo the fraction's top 64 bits) is added with two
+// 64-bit integer atomics and the carry out of the low word, so the total does not depend on the
+// order of the adds (FP64 atomics would).  |sums| < 2^63; addends below 2^-64 are dropped.
+__device__ inline void fx_add(double* acc, uint32_t lab, double v) {
+  const double fl = floor(v);
+  const unsigned long long hi = (unsigned long long)(long long)fl;
+  const unsigned long long lo = (unsigned long long)((v - fl) * 18446744073709551616.0);
+  unsigned long long* L = reinterpret_cast<unsigned long long*>(acc) + lab;
+  unsigned long long* H = reinterpret_cast<unsigned long long*>(acc + (size_t)BV_FX_HI * MAX_BOLHAS) + lab;
+  const unsigned long long old = atomicAdd(L, lo);
+  atomicAdd(H, hi + (old + lo < old ? 1ull : 0ull));
+}
+__device__ inline double fx_read(const double* acc, uint32_t lab) {
+  const unsigned long long lo = reinterpret_cast<const unsigned long long*>(acc)[lab];
+  const long long hi = reinterpret_cast<const long long*>(acc + (size_t)BV_FX_HI * MAX_BOLHAS)[lab];
+  return (double)hi + (double)lo * 5.421010862427522e-20;  // 2^-64
+}
+__device__ inline void fx_clear(double* acc, uint32_t lab) {
+  reinterpret_cast<unsigned long long*>(acc)[lab] = 0;
+  reinterpret_cast<unsigned long long*>(acc + (size_t)BV_FX_HI * MAX_BOLHAS)[lab] = 0;
+}
 
 // partial-sum arrays (slots of MAX_RED_BLOCKS doubles): 0..4 the solve's set-up and the true
 // residual, then the products of one iteration, each read by the following kernel
@@ -89,7 +117,7 @@ __device__ inline void bub_add(double (&sl)[BSLOTS], const int* __restrict__ lab
                                double val) {
   const int s = labelSlot[lab];
   if (s >= 0) slot_add(sl, s, val);
-  else atomicAdd(&accG[lab], val);
+  else fx_add(accG, lab, val);
 }
 
 // Block sum of the first nv of v[]; thread 0 stores partial `bid` of each quantity.  nv must
@@ -192,12 +220,12 @@ __device__ inline double fold_zdots(const double* __restrict__ part, const doubl
   for (int k = threadIdx.x; k < na; k += blockDim.x) {
     const int b = B.activeList[k];
     const int sl = B.labelSlot[b];
-    const double zn = sl >= 0 ? zs[sl] : accZ[b];
+    const double zn = sl >= 0 ? zs[sl] : fx_read(accZ, b);
     bacc[0] += Rc[b] * zn;
     bacc[1] += Rc[b] * Zo[b];
     if (b0) {
       Zn[b] = zn;
-      accZnext[b] = 0;
+      fx_clear(accZnext, b);
     }
   }
   block_sum_all<2>(bacc, sh, res);
@@ -236,7 +264,7 @@ __device__ inline double fold_spmv(const double* __restrict__ part, const double
     const int sl = B.labelSlot[b];
     const double pn = Zn[b] + beta * Po[b];
     *pn_out = pn;
-    return (double)B.facesB[b] * pn + (sl >= 0 ? aps[sl] : accA[b]);
+    return (double)B.facesB[b] * pn + (sl >= 0 ? aps[sl] : fx_read(accA, b));
   };
   double bacc[1] = {0};
   for (int k = threadIdx.x; k < na; k += blockDim.x) {
@@ -267,7 +295,7 @@ __device__ inline double fold_spmv(const double* __restrict__ part, const double
       APb[b] = apb;
       Yb[b] += alpha * pn;
       Rn[b] = rbn;
-      accAnext[b] = 0;
+      fx_clear(accAnext, b);
     }
   }
   block_sum<1>(rr, sh);
@@ -284,7 +312,7 @@ __device__ inline double bubble_r(const BubArgs& B, int b, int par, double alpha
   const int sl = B.labelSlot[b];
   if (sl >= 0) return rbs[sl];
   const double pn = bv(B.bvec, BV_Z0 + par)[b] + beta * bv(B.bvec, BV_P0 + 1 - par)[b];
-  const double apb = (double)B.facesB[b] * pn + bv(B.bvec, BV_ACCA0 + par)[b];
+  const double apb = (double)B.facesB[b] * pn + fx_read(bv(B.bvec, BV_ACCA0 + par), b);
   return bv(B.bvec, BV_R0 + 1 - par)[b] - alpha * apb;
 }
 
@@ -596,7 +624,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_init(const uint32_t* __rest
     const int s = labelSlot[b];
     double accB;
     if (s >= 0) accB = slotTot[s];
-    else { accB = acc0[b]; acc0[b] = 0; }
+    else { accB = fx_read(acc0, b); fx_clear(acc0, b); }
     rb[b] = rhsB[b] - ((double)facesB[b] * yb[b] + accB);
     bacc[0] += rb[b] * rb[b];
     bacc[1] += rhsB[b] * rhsB[b];
@@ -907,7 +935,7 @@ struct ZArgs {  // level-0 outputs: z = MG result and the dots r.z, r.z_old
   const float* r;
   const float* zold;
   float* znew;
-  double* accZ;       // bubble z sums of labels without a slot (FP64 atomics)
+  double* accZ;       // bubble z sums of labels without a slot (128-bit fixed point, fx_add)
   double* partials;   // r.z, r.z_old partials (two slots)
   double* bpart;      // bubble z sums of the slotted labels
   const float* mgs;   // the V-cycle input scale of this cycle
@@ -1958,7 +1986,7 @@ __global__ __launch_bounds__(256) void k_spmv(const uint8_t* __restrict__ opMask
     if (md & MK_BUB) {
       const uint32_t lb = rot[d];
       const int sl = B.labelSlot[lb];
-      const double zn = sl >= 0 ? zs[sl] : accZ[lb];
+      const double zn = sl >= 0 ? zs[sl] : fx_read(accZ, lb);
       return (float)(zn + beta * Po[lb]);
     }
     return 0.f;
@@ -2086,7 +2114,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_true_res(const uint32_t* __
     const int s = labelSlot[b];
     double accB;
     if (s >= 0) accB = slotTot[s];
-    else { accB = acc0[b]; acc0[b] = 0; }
+    else { accB = fx_read(acc0, b); fx_clear(acc0, b); }
     const double e = (rhsB[b] - sc->proj_mean) - ((double)facesB[b] * yb[b] + accB);
     bacc[0] += e * e;
   }

@@ -53,7 +53,7 @@ constexpr double FX_INV = 1.0 / 70368744177664.0;
 constexpr int RED_BLOCK = 256;           // block size of reduction kernels
 constexpr int MAX_RED_BLOCKS = 65536;    // partial slots per reduction (one per block)
 constexpr int RED_SLOTS = 10;            // reductions with their own partial arrays (solver layout)
-constexpr int BV_COUNT = 13;             // solver bubble vectors of MAX_BOLHAS doubles each
+constexpr int BV_COUNT = 18;             // solver bubble vectors of MAX_BOLHAS doubles each
 
 // mask byte of a multigrid cell / operator cell
 //   bits 0-3: free neighbours (1 left, 2 right, 4 down, 8 up) as prepararVizinhos

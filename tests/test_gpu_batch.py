@@ -1,9 +1,7 @@
 """Batched tanks (tq_batch_*): stepping tanks together from several host threads gives, tank by
-tank, exactly the states of stepping each alone (a tank's work stays on its own stream).
-
-The comparison is bit-exact, so it runs the maze tanks without air tracking: with air, a maze
-traps 100-200 bubbles, and bubbles beyond the 8 per-block reduction slots are summed with FP64
-atomics, whose order -- and last bits -- differ between any two runs, batched or not."""
+tank, exactly the states of stepping each alone (a tank's work stays on its own stream), bit
+for bit, with air tracking on: a maze traps 100-200 bubbles, and the bubbles beyond the 8
+per-block reduction slots are summed in order-independent fixed point."""
 import numpy as np
 import pytest
 
@@ -25,10 +23,10 @@ def test_batch_matches_tanks_stepped_alone(tq, prec):
     from tanque_hip.scenes import passo_frame
     p = tq.PREC_FP32 if prec == "fp32" else tq.PREC_MIXED
     ts = scenes.batched_mazes(n=5, seed=3)
-    opts = {"iteracoes": 3, "ar": False, "precision": p}
+    opts = {"iteracoes": 3, "precision": p}
     cenas = [t.cena() for t in ts]
     b = tq.Batch(cenas, [opts] * len(ts), threads=3)
-    alone = [tq.SimulacaoHIP(t.cena(), iteracoes=3, ar=False, precision=p) for t in ts]
+    alone = [tq.SimulacaoHIP(t.cena(), iteracoes=3, precision=p) for t in ts]
     for t, sb, sa in zip(ts, b.tanks, alone):
         sb.adicionar(t.particles, False)
         sa.adicionar(t.particles, False)

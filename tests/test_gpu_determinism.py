@@ -32,3 +32,31 @@ def test_hourglass_run_twice_identical(prec, graph):
     assert a[4]["total_iterations"] == b[4]["total_iterations"]
     for k in range(4):
         assert np.array_equal(a[k], b[k])
+
+
+@pytest.mark.parametrize("iters", [3, 200])
+def test_maze_with_many_bubbles_run_twice_identical(iters):
+    """A maze tank traps far more bubbles than the 8 per-block reduction slots; the others are
+    summed in 128-bit fixed point, so two runs still agree bit for bit (fixed 3 PCG iterations as
+    the reference, and solved to 1e-6)."""
+    import tanque_hip as tq
+    from tanque_hip import scenes
+    from tanque_hip.scenes import passo_frame
+    tq.load()
+    t = scenes.batched_mazes(n=3, seed=3)[2]
+    out = []
+    for _ in range(2):
+        cena = t.cena()
+        sim = tq.SimulacaoHIP(cena, iteracoes=iters, tol=1e-6 if iters > 3 else 0.0)
+        sim.adicionar(t.particles, False)
+        for _ in range(8):
+            sim.avancar(t.dt_frame, passo_frame(t.h, cena.altura, sim.vMax, t.dt_frame))
+        out.append((sim.particulas(), sim.read(12), sim.read(22), sim.read(19), sim.solver_stats()))
+        sim.close()
+    a, b = out
+    log("\n  maze %dx%d, %d PCG iterations: n %d, active bubbles %d, iterations %d / %d" % (
+        t.nx, t.ny, iters, len(a[0]), a[4]["active_bubbles"], a[4]["total_iterations"], b[4]["total_iterations"]))
+    assert a[4]["active_bubbles"] > 8
+    assert a[4]["total_iterations"] == b[4]["total_iterations"]
+    for k in range(4):
+        assert np.array_equal(a[k], b[k])
