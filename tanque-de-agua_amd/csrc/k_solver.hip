@@ -6,7 +6,7 @@
 // of same-address atomics: each block stores one partial per quantity (and one per slotted
 // bubble); the next kernel's blocks each re-sum them in block order -- so every block holds
 // bit-identical scalars -- and redo the per-bubble scalar work, block 0 alone storing the
-// bubble vectors.  Bubbles beyond BSLOTS fall back to FP64 atomics.  What one kernel reads and
+// bubble vectors.  Bubbles beyond BSLOTS go to 128-bit fixed-point atomics.  What one kernel reads and
 // rewrites alternates by iteration parity (bubble z, p, r, the atomic accumulators, rz).
 //
 // Multigrid.  The reference smoother (suavizar, pressao-cpu.js:99-150) runs 16x16 tiles that

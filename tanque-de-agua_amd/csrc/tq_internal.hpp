@@ -203,7 +203,7 @@ struct tq_sim {
   int nb_spmv = 0, nb_upd = 0, nb_z = 0;  // partial counts of the last spmv, update and z launches
   int launch_spmv_grid = 0;     // most blocks of spmv (0: the built-in cap)
   int launch_update_grid = 0;   // most blocks of the PCG update / level-0 input kernel (0: the built-in cap)
-  int* labelSlot = nullptr;     // bubble label -> reduction slot (-1: FP64 atomics)
+  int* labelSlot = nullptr;     // bubble label -> reduction slot (-1: 128-bit fixed-point atomics)
   int lc = 0;                   // first multigrid level solved inside the one-workgroup kernel
   int mg_scaled = 0;            // this solve stores some multigrid level in FP16 (input scaled)
   float* dense = nullptr;       // coarsest-level operator (when that level is small)

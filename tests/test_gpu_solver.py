@@ -43,7 +43,7 @@ def prepared(orc, build, frames):
     ("dam160", lambda o: dam(o, nx=160, ny=90, h=0.8 / 90, seed=0), 12),
     # 256x144: level 1 (128x72) is large enough for FP16 storage in the mixed preconditioner
     ("dam256", lambda o: dam(o, nx=256, ny=144, h=1.0 / 144, seed=0), 8),
-    # twelve bubbles: more than the per-block reduction slots, the rest through FP64 atomics
+    # twelve bubbles: more than the per-block reduction slots, the rest in 128-bit fixed point
     ("bolhas12", lambda o: tanque_com_bolhas(o), 2),
     # 1024x640 (655k cells): 640 spmv and update blocks, their partials summed by every consumer block
     ("dam1024", lambda o: dam(o, nx=1024, ny=640, h=0.8 / 640, seed=0), 1),
