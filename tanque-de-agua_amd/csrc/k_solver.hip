@@ -1114,37 +1114,55 @@ __global__ __launch_bounds__(256) void k_pass_z(const float* __restrict__ src, c
 // pass C) cost more than a kernel boundary, so those levels run each pass once: pass B, then
 // restriction + the coarse pass A; prolongation + pass C, then pass D.
 constexpr int SPLIT_CELLS = 1 << 20;
+constexpr int PASS_FOLD_GRID = 2048;  // 8 blocks per CU at k_pass's register count
 
 // one smoothing pass of level l (offset desl, colour order from C0); on level 0 of an iteration
 // it first does the norm and stop test (fold_update)
 template <class SX, class SB, int C0>
 __global__ __launch_bounds__(256) void k_pass(const SX* __restrict__ src, SX* __restrict__ dst,
                                               const SB* __restrict__ b, const uint8_t* __restrict__ mask, int nx,
-                                              int ny, int desl, int ntx, int nty, const double* __restrict__ partRR,
-                                              int nbu, int par, int fold, DevScalars* __restrict__ sc) {
+                                              int ny, int desl, int ntx, int nty, int vgx, int nvb,
+                                              const double* __restrict__ partRR, int nbu, int par, int fold,
+                                              DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ float Tt[4][TCELLS];
-  const int lane = threadIdx.x & 63;
-  int tx, ty;
-  block_tile(&tx, &ty);
-  const bool valid = tx < ntx && ty < nty;
-  const int ox = tx * 16 - desl, oy = ty * 16 - desl;
-  float* T = Tt[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  float* T = Tt[w];
+  // one-dimensional grid over the 2x2-tile groups (vgx per row, nvb in all); a folding launch
+  // is capped and its blocks walk several groups after one fold
+  int vb = xcd_block(blockIdx.x, gridDim.x);
+  bool valid;
+  int ox, oy;
   Own4 own;
-  if (valid) {
-    prefetch_cells<SB>(own, b, mask, ox, oy, nx, ny, lane);
-    fill_tile<SX>(T, src, ox, oy, nx, ny, lane);
-  }
+  auto load = [&](int g) {
+    const int by = g / vgx, bx = g - by * vgx;
+    const int tx = 2 * bx + (w & 1), ty = 2 * by + (w >> 1);
+    valid = tx < ntx && ty < nty;
+    ox = tx * 16 - desl;
+    oy = ty * 16 - desl;
+    if (valid) {
+      prefetch_cells<SB>(own, b, mask, ox, oy, nx, ny, lane);
+      fill_tile<SX>(T, src, ox, oy, nx, ny, lane);
+    }
+  };
+  load(vb);
   if (fold) {
     __shared__ double fsh[16], fres[1];
     if (fold_update(partRR, nbu, par, fsh, fres, sc)) return;
   }
-  if (!valid) return;
-  wave_sync();
-  Cells4 cc;
-  init_cells(cc, own, T, lane);
-  tile_sweeps<C0>(T, cc, lane);
-  store_cells<SX>(cc, dst, ox, oy, nx, ny, lane);
+  for (;;) {
+    if (valid) {
+      wave_sync();
+      Cells4 cc;
+      init_cells(cc, own, T, lane);
+      tile_sweeps<C0>(T, cc, lane);
+      store_cells<SX>(cc, dst, ox, oy, nx, ny, lane);
+      wave_sync();  // the wave's tile is free for its next group
+    }
+    vb += gridDim.x;
+    if (vb >= nvb) break;
+    load(vb);
+  }
 }
 
 // restringir (pressao-cpu.js:159-188): the block owns a 16x16 coarse tile; fine residuals
@@ -1940,8 +1958,8 @@ __global__ __launch_bounds__(256) void k_vcycle_apply(const float* __restrict__ 
 // block walks its chunks after one fold: the fold's chain of dependent loads and barriers is
 // paid once per block, not once per chunk.
 constexpr int SPMV_CELLS = 4;
-constexpr int SPMV_GRID = 1024;  // 4 blocks per CU at this kernel's 123 VGPRs
-__global__ __launch_bounds__(256) void k_spmv(const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
+constexpr int SPMV_GRID = 1024;  // 4 blocks per CU: the kernel is held to 128 VGPRs (4 waves per SIMD)
+__global__ __launch_bounds__(256, 4) void k_spmv(const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
                                               const float* __restrict__ z, const float* __restrict__ pold,
                                               float* __restrict__ pnew, float* __restrict__ ap, int nx, int C,
                                               BubArgs B, const double* __restrict__ partZ,
@@ -2357,9 +2375,13 @@ static void launch_bra(tq_sim* s, int l) {
     int t = time_begin(s, pnames[l], (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n);
     with_types(F, [&](auto tx, auto tb) {
       TQ_TYPES(tx, tb);
-      hipLaunchKernelGGL((k_pass<SX, SB, 0>), tile_grid(F.nx, F.ny, 8), dim3(256), 0, s->st, (const SX*)F.x,
-                         (SX*)F.tmp, (const SB*)F.b, F.mask, F.nx, F.ny, 8, ntx8, nty8, rr, rsu.nb, it_par(s), fold,
-                         s->sc);
+      const dim3 tg = tile_grid(F.nx, F.ny, 8);
+      const int nvb = (int)(tg.x * tg.y);
+      // the folding launch (level 0 of an iteration) walks several groups per block after one fold
+      const int nbk = fold ? std::min(nvb, PASS_FOLD_GRID) : nvb;
+      hipLaunchKernelGGL((k_pass<SX, SB, 0>), dim3(nbk), dim3(256), 0, s->st, (const SX*)F.x, (SX*)F.tmp,
+                         (const SB*)F.b, F.mask, F.nx, F.ny, 8, ntx8, nty8, (int)tg.x, nvb, rr, rsu.nb, it_par(s),
+                         fold, s->sc);
     });
     time_end(s, t);
     t = time_begin(s, rnames[l], (vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n +
@@ -2447,9 +2469,10 @@ static int launch_pcd(tq_sim* s, int l) {
     t = time_begin(s, dnames[l], (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n);
     with_types(F, [&](auto tx, auto tb) {
       TQ_TYPES(tx, tb);
-      hipLaunchKernelGGL((k_pass<SX, SB, 1>), tile_grid(F.nx, F.ny, 0), dim3(256), 0, s->st, (const SX*)F.x,
-                         (SX*)F.tmp, (const SB*)F.b, F.mask, F.nx, F.ny, 0, ntx, nty, (const double*)nullptr, 0, 0, 0,
-                         s->sc);
+      const dim3 tg = tile_grid(F.nx, F.ny, 0);
+      hipLaunchKernelGGL((k_pass<SX, SB, 1>), dim3(tg.x * tg.y), dim3(256), 0, s->st, (const SX*)F.x,
+                         (SX*)F.tmp, (const SB*)F.b, F.mask, F.nx, F.ny, 0, ntx, nty, (int)tg.x, (int)(tg.x * tg.y),
+                         (const double*)nullptr, 0, 0, 0, s->sc);
     });
     time_end(s, t);
     F.res = F.tmp;
