@@ -270,23 +270,51 @@ def dominant_kernel(tq, work, precision, frames=3):
 
 def cpu_baseline(work, budget_s=15.0, tol=1e-6, maxit=200):
     """The oracle (single-threaded C++ restatement of the reference CPU engine) on a bounded
-    sample of the same workload: the first frames of the same scene, same particles."""
+    sample of the same workload: the first frames of the same scene, same particles.  Driven from
+    Node, the reference's host (oracle/bench_oracle.js over the oracle's Node-API addon), as a
+    child process; from Python where node or the addon is missing."""
+    import shutil
+    import subprocess
+    import tempfile
     from oracle import oracle as orc
     orc.build()
-    cena = orc.Cena(work.nx, work.ny, work.h)
-    for s in work.strokes:
-        cena.tracar(*s)
-    sim = orc.Simulacao(cena, mode=0, tol=tol, iteracoes=maxit)
-    sim.adicionar(work.particles, False)
-    t0 = time.perf_counter()
-    subs = frames = 0
-    while time.perf_counter() - t0 < budget_s and frames < work.frames:
-        sub = int(math.ceil(work.dt_frame / orc.passo_maximo(work.h, cena.altura, sim.vmax, work.dt_frame)))
-        sim.avancar(work.dt_frame, sub)
-        subs += sub
-        frames += 1
-    el = time.perf_counter() - t0
-    return {"value": round(subs / el, 4), "unit": "substeps/s", "cores": 1, "kind": "port",
+    here = os.path.dirname(os.path.abspath(__file__))
+    harness = os.path.join(here, "oracle", "bench_oracle.js")
+    node = shutil.which("node")
+    if node and os.path.exists(orc.NODE_ADDON):
+        with tempfile.TemporaryDirectory() as d:
+            pf = os.path.join(d, "particles.f32")
+            np.ascontiguousarray(work.particles, np.float32).tofile(pf)
+            wf = os.path.join(d, "workload.json")
+            with open(wf, "w") as f:
+                json.dump({"nx": work.nx, "ny": work.ny, "h": work.h,
+                           "strokes": [[float(v) for v in s[:5]] + [bool(s[5]) if len(s) > 5 else False]
+                                       for s in work.strokes],
+                           "particles_file": pf, "dt_frame": work.dt_frame, "frames": work.frames,
+                           "budget_s": budget_s, "tol": tol, "maxit": maxit}, f)
+            r = subprocess.run([node, harness, wf], capture_output=True, text=True, timeout=budget_s + 300)
+        if r.returncode != 0:
+            raise RuntimeError("bench_oracle.js failed: %s" % r.stderr[-2000:])
+        out = json.loads(r.stdout.strip().splitlines()[-1])
+        frames, subs, el = out["frames"], out["substeps"], out["seconds"]
+        driver = "node %s (oracle/bench_oracle.js over oracle/oracle_node.node)" % (
+            subprocess.run([node, "--version"], capture_output=True, text=True).stdout.strip())
+    else:
+        cena = orc.Cena(work.nx, work.ny, work.h)
+        for s in work.strokes:
+            cena.tracar(*s)
+        sim = orc.Simulacao(cena, mode=0, tol=tol, iteracoes=maxit)
+        sim.adicionar(work.particles, False)
+        t0 = time.perf_counter()
+        subs = frames = 0
+        while time.perf_counter() - t0 < budget_s and frames < work.frames:
+            sub = int(math.ceil(work.dt_frame / orc.passo_maximo(work.h, cena.altura, sim.vmax, work.dt_frame)))
+            sim.avancar(work.dt_frame, sub)
+            subs += sub
+            frames += 1
+        el = time.perf_counter() - t0
+        driver = "python (ctypes over oracle/liboracle.so)"
+    return {"value": round(subs / el, 4), "unit": "substeps/s", "cores": 1, "kind": "port", "driver": driver,
             "sample": "first %d frames (%d substeps) of the same workload on the oracle (JS-exact binary64 "
                       "compute / binary32 storage, PCG to 1e-6), %.1f s on one host core" % (frames, subs, el)}
 

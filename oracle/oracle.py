@@ -37,7 +37,27 @@ def build(force=False):
         subprocess.check_call(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
                                "-fPIC", "-shared", src, "-o", LIB_PATH + ".tmp"])
         os.replace(LIB_PATH + ".tmp", LIB_PATH)
+    build_node(force)
     return LIB_PATH
+
+
+NODE_ADDON = os.path.join(HERE, "oracle_node.node")
+
+
+def build_node(force=False):
+    """The Node-API addon over this library (node_oracle.cpp), for timing the oracle from Node
+    (bench_oracle.js); skipped when node's headers are absent."""
+    import subprocess
+    src = os.path.join(HERE, "node_oracle.cpp")
+    if not os.path.exists("/usr/include/node/node_api.h"):
+        return None
+    if force or not os.path.exists(NODE_ADDON) or os.path.getmtime(NODE_ADDON) < max(
+            os.path.getmtime(src), os.path.getmtime(LIB_PATH)):
+        subprocess.check_call(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-I/usr/include/node", src, "-o",
+                               NODE_ADDON + ".tmp", "-L" + HERE, "-loracle", "-Wl,-rpath,$ORIGIN",
+                               "-Wl,--unresolved-symbols=ignore-all"])
+        os.replace(NODE_ADDON + ".tmp", NODE_ADDON)
+    return NODE_ADDON
 
 
 def lib():
