@@ -51,9 +51,13 @@ __device__ __host__ inline double* bv(double* base, int k) { return base + (size
 // 64-bit integer atomics and the carry out of the low word, so the total does not depend on the
 // order of the adds (FP64 atomics would).  |sums| < 2^63; addends below 2^-64 are dropped.
 __device__ inline void fx_add(double* acc, uint32_t lab, double v) {
-  const double fl = floor(v);
+  double fl = floor(v), fr = v - fl;
+  if (fr >= 1.0) {  // v in (-2^-54, 0): 1 + v rounds to 1; the addend is below the resolution
+    fl += 1.0;
+    fr = 0.0;
+  }
   const unsigned long long hi = (unsigned long long)(long long)fl;
-  const unsigned long long lo = (unsigned long long)((v - fl) * 18446744073709551616.0);
+  const unsigned long long lo = (unsigned long long)(fr * 18446744073709551616.0);
   unsigned long long* L = reinterpret_cast<unsigned long long*>(acc) + lab;
   unsigned long long* H = reinterpret_cast<unsigned long long*>(acc + (size_t)BV_FX_HI * MAX_BOLHAS) + lab;
   const unsigned long long old = atomicAdd(L, lo);
