@@ -234,6 +234,15 @@ int tq_timing_read(tq_sim* s, tq_kernel_time* out, int max, int* n);
 int tq_set_launch(tq_sim* s, const char* name, int value);
 int tq_get_launch(tq_sim* s, const char* name, int* value);
 
+/* The render hand-off: the water layer of the reference's 2D renderer (js/render-2d.js:110-162)
+ * computed on the device from the particle buffer, at 3 pixels per cell (w = 3 nx, h = 3 ny,
+ * rows from the top).  tint_rgba: the Beer-Lambert water tint multiplied over the panel
+ * (255,255,255,255 where the water covers less than the threshold); foam_rgba: (248, 251, 252,
+ * alpha) foam over it (all zero where uncovered; the reference leaves those RGB bytes as they
+ * were).  Sorts the particles by cell for their current positions (particle order is not part of
+ * the state: tq_read_particles returns insertion order).  Not allowed while frames are recorded. */
+int tq_render_water(tq_sim* s, int w, int h, uint8_t* tint_rgba, uint8_t* foam_rgba);
+
 /* Waits for all queued device work of this handle. */
 int tq_sync(tq_sim* s);
 

@@ -292,6 +292,33 @@ napi_value m_set_strokes(napi_env env, napi_callback_info info) {
   return undefined(env);
 }
 
+// renderizarAgua() -> {largura, altura, tinta, espuma}: the water layer of the 2D renderer
+// (render-2d.js:110-162) as two Uint8ClampedArray RGBA images of 3 nx x 3 ny pixels (tq_render_water)
+napi_value m_render_water(napi_env env, napi_callback_info info) {
+  napi_value argv[1], self;
+  if (!get_args(env, info, 0, argv, &self)) return nullptr;
+  Native* n = unwrap(env, self);
+  if (!n) return nullptr;
+  const int w = 3 * n->nx, h = 3 * n->ny;
+  const size_t bytes = (size_t)w * h * 4;
+  void *tp = nullptr, *ep = nullptr;
+  napi_value tint = make_typed(env, napi_uint8_clamped_array, bytes, 1, &tp);
+  napi_value foam = make_typed(env, napi_uint8_clamped_array, bytes, 1, &ep);
+  if (!tint || !foam) return nullptr;
+  if (!throw_tq(env, tq_render_water(n->s, w, h, static_cast<uint8_t*>(tp), static_cast<uint8_t*>(ep)),
+                "tq_render_water"))
+    return nullptr;
+  napi_value out, vw, vh;
+  napi_create_object(env, &out);
+  napi_create_int32(env, w, &vw);
+  napi_create_int32(env, h, &vh);
+  napi_set_named_property(env, out, "largura", vw);
+  napi_set_named_property(env, out, "altura", vh);
+  napi_set_named_property(env, out, "tinta", tint);
+  napi_set_named_property(env, out, "espuma", foam);
+  return out;
+}
+
 // tracosVjp(Float64Array strokes, Float32Array gsdf) -> Float64Array n*5 (tq_strokes_vjp)
 napi_value m_strokes_vjp(napi_env env, napi_callback_info info) {
   napi_value argv[2], self;
@@ -635,6 +662,7 @@ napi_value init(napi_env env, napi_value exports) {
       {"gravar", nullptr, m_tape_begin, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"definirTracos", nullptr, m_set_strokes, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"tracosVjp", nullptr, m_strokes_vjp, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"renderizarAgua", nullptr, m_render_water, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"fimGravacao", nullptr, m_simple<tq_tape_end>, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"fracaoAgua", nullptr, m_fraction, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"retropropagar", nullptr, m_backward, nullptr, nullptr, nullptr, napi_default, nullptr},

@@ -160,7 +160,7 @@ static int alloc_all(tq_sim* s) {
 static void free_all(tq_sim* s) {
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   for (int b = 0; b < 2; b++) { f(s->pos[b]); f(s->vel[b]); f(s->cm[b]); f(s->ext[b]); f(s->seq[b]); }
-  f(s->pkey); f(s->prank); f(s->addflag); f(s->stage); f(s->sdf); f(s->solid); f(s->countKey); f(s->startKey);
+  f(s->pkey); f(s->prank); f(s->addflag); f(s->stage); f(s->render_img); f(s->sdf); f(s->solid); f(s->countKey); f(s->startKey);
   f(s->countCell); f(s->u); f(s->v); f(s->uw); f(s->vw); f(s->uTmp); f(s->vTmp); f(s->uVal); f(s->vVal);
   f(s->uValTmp); f(s->vValTmp); f(s->dens); f(s->dye); f(s->tipo); f(s->fase); f(s->nX); f(s->nY); f(s->kappa);
   f(s->qAr); f(s->div); f(s->q); f(s->rot); f(s->cflag); f(s->cand); f(s->pai); f(s->comp); f(s->rootFlag);

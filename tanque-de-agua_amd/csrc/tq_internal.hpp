@@ -139,6 +139,8 @@ struct tq_sim {
   uint32_t* addflag = nullptr;  // add/compaction scratch (cap+1)
   uint32_t seq_next = 0;
   float* stage = nullptr;       // staging for uploads of particle lists (cap*12)
+  uint8_t* render_img = nullptr;  // tq_render_water's two RGBA images, allocated on its first call
+  size_t render_bytes = 0;
 
   // scene / grid
   float* sdf = nullptr;

@@ -109,6 +109,10 @@ class SimulacaoHIP {
     return this._n.tracosVjp(Float64Array.from(tracos.flatMap((t) => [t.x0, t.y0, t.x1, t.y1, t.raio, t.apagar ? 1 : 0])), gsdf);
   }
   fimGravacao() { this._n.fimGravacao(); }
+  /** The render hand-off: the water layer of Render2D.desenhar (render-2d.js:110-162) computed on
+   *  the device, {largura: 3 nx, altura: 3 ny, tinta, espuma} with two Uint8ClampedArray RGBA
+   *  images, ready for ImageData (tinta multiplied over the panel, espuma drawn over it). */
+  renderizarAgua() { return this._n.renderizarAgua(); }
   /** Water fraction of the cell box [i0,i1) x [j0,j1) (mean of min(1, densidade/4)); with
    *  comGradiente, also d(fraction)/d(state) as a Float32Array of n*8 (x y vx vy cux cuy cvx cvy). */
   fracaoAgua(i0, j0, i1, j1, comGradiente = false) { return this._n.fracaoAgua(i0, j0, i1, j1, comGradiente); }

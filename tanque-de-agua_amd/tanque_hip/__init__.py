@@ -106,6 +106,7 @@ SIGNATURES = {
     "tq_solver_stats_get": (C.c_int, [C.c_void_p, C.POINTER(SolverStats), C.c_int]),
     "tq_timing_enable": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p]),
     "tq_timing_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int, C.POINTER(C.c_int)]),
+    "tq_render_water": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "tq_set_launch": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "tq_get_launch": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int)]),
     "tq_sync": (C.c_int, [C.c_void_p]),
@@ -480,6 +481,15 @@ class SimulacaoHIP:
 
     def timing_enable(self, on=True, only=None):
         _chk(load().tq_timing_enable(self._h, 1 if on else 0, only.encode() if only else None))
+
+    def render_water(self):
+        """The water layer of the reference's 2D renderer (js/render-2d.js:110-162) at 3 pixels per
+        cell: (tint, foam), two uint8 RGBA images of shape (3 ny, 3 nx, 4), rows from the top."""
+        w, h = 3 * self.nx, 3 * self.ny
+        tint = np.empty((h, w, 4), np.uint8)
+        foam = np.empty((h, w, 4), np.uint8)
+        _chk(load().tq_render_water(self._h, w, h, tint.ctypes.data, foam.ctypes.data))
+        return tint, foam
 
     def set_launch(self, name, value):
         """Launch cap of a solver kernel ("spmv_grid", "update_grid"; 0 = default)."""
