@@ -2,12 +2,14 @@
 // Drives the library through the Node-API addon the way the reference's tests drive
 // SimulacaoCPU (testes/fisica.test.js:12-29): a dam-break tank stepped with the adaptive
 // substep bound; prints one JSON line with the particle records (base64 of the Float32Array).
-// Usage: node tests/js/run_dam.js <nx> <ny> <h> <seconds> [host-only]
+// Usage: node tests/js/run_dam.js <nx> <ny> <h> <seconds> [host-only | render]
+// (render: also the water layer of the end state, base64 of the two RGBA images)
 const path = require('path');
 const tq = require(path.join(__dirname, '..', '..', 'tanque-de-agua_amd', 'js', 'tanque_hip.js'));
 
 const [nx, ny, h, segundos] = process.argv.slice(2, 6).map(Number);
 const hostOnly = process.argv[6] === 'host-only';
+const render = process.argv[6] === 'render';
 const MOLDURA = 2;
 const m = MOLDURA * h;
 const semente = new tq.Semente(9);
@@ -34,6 +36,13 @@ function main() {
   out.area = motor.sim.area();
   out.custoPorSubpasso = motor.custoPorSubpasso;
   out.particulas = Buffer.from(P.buffer, P.byteOffset, P.byteLength).toString('base64');
+  if (render) {
+    const img = motor.sim.renderizarAgua();
+    out.largura = img.largura;
+    out.altura = img.altura;
+    out.tinta = Buffer.from(img.tinta.buffer).toString('base64');
+    out.espuma = Buffer.from(img.espuma.buffer).toString('base64');
+  }
   motor.destruir();
 }
 main();

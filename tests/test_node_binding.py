@@ -65,6 +65,35 @@ def test_motor_hip_in_node_matches_ctypes_run():
 
 @needs_node
 @pytest.mark.gpu
+def test_node_render_water_matches_ctypes():
+    """renderizarAgua() through the addon returns the images render_water() returns through
+    ctypes for the same run (sizes 3 nx x 3 ny RGBA; bytes equal up to a one-unit rounding edge
+    where the per-cell particle order, which the sort's atomics set, changes a binary64 sum)."""
+    import tanque_hip as tq
+    tq.load()
+    nx, ny, h, seconds = 80, 60, 0.01, 0.25
+    r = subprocess.run(["node", DRIVER] + [str(v) for v in (nx, ny, h, seconds, "render")], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    assert (d["largura"], d["altura"]) == (3 * nx, 3 * ny)
+    tint = np.frombuffer(base64.b64decode(d["tinta"]), np.uint8).reshape(3 * ny, 3 * nx, 4)
+    foam = np.frombuffer(base64.b64decode(d["espuma"]), np.uint8).reshape(3 * ny, 3 * nx, 4)
+    cena = tq.Cena(nx, ny, h)
+    sim = tq.SimulacaoHIP(cena)
+    sim.adicionar(tq.semear_retangulo(2 * h, 2 * h, 0.3, 0.42, h, (0, 0, 0), tq.Semente(9)), False)
+    quadro = 1 / 60
+    for _ in range(int(math.floor(seconds / quadro + 0.5))):
+        sim.avancar(quadro, math.ceil(quadro / tq.passo_maximo(h, cena.altura, sim.vMax, quadro)))
+    t2, f2 = sim.render_water()
+    for a, b in ((tint, t2), (foam, f2)):
+        diff = np.abs(a.astype(int) - b.astype(int))
+        assert diff.max() <= 1 and (diff > 0).sum() <= 1e-3 * diff.size
+    assert (tint[..., 0] < 250).any()
+
+
+@needs_node
+@pytest.mark.gpu
 def test_node_backward_matches_ctypes_backward():
     """The funnel's gradient through the addon (gravar, fracaoAgua, retropropagar) equals the
     ctypes run's: same library and decisions.  The reverse pass's FP64 atomic sums are taken in
