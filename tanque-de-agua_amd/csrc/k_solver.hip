@@ -2538,6 +2538,13 @@ static void launch_coarse_wg(tq_sim* s) {
   const int L = (int)s->lv.size();
   Level& G = s->lv[s->lc];
   if (vop_active(s)) {
+    if (s->vop_pending) {  // the matrix of this solve is built on the second stream
+      if (hipStreamWaitEvent(s->st, s->ev_vop, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(s->st2);
+      }
+      s->vop_pending = false;
+    }
     // x(lc) = C b(lc): the matrix and b in, x out
     const int waves = (G.n + VOP_ROWS - 1) / VOP_ROWS;
     int t = time_begin(s, "mg_coarse_mv", 4.0 * s->vop_ld * G.n + 8.0 * G.n);
@@ -2558,13 +2565,25 @@ static void launch_coarse_wg(tq_sim* s) {
 }
 
 // builds this solve's matrix of level lc's V-cycle (after the level masks and the coarsest operator)
-static void launch_vcycle_op(tq_sim* s) {
+// On the second stream (when there is one, and outside kernel timing and graph capture) the
+// build runs beside the solve's set-up, first residual and first down kernels; the first dense
+// product waits for it (launch_coarse_wg).
+static int launch_vcycle_op(tq_sim* s) {
   const WGPlan P = wg_plan(s);
   const Level& G = s->lv[s->lc];
+  const size_t lds = P.pool + VOP_WAVES * TCELLS * sizeof(float);
+  if (s->st2 && !s->timing && !s->capturing) {
+    TQ_HIP(hipEventRecord(s->ev_masks, s->st));
+    TQ_HIP(hipStreamWaitEvent(s->st2, s->ev_masks, 0));
+    hipLaunchKernelGGL(k_vcycle_op, dim3(G.n), dim3(64 * VOP_WAVES), lds, s->st2, P, s->vop_ld, s->vop, s->sc);
+    TQ_HIP(hipEventRecord(s->ev_vop, s->st2));
+    s->vop_pending = true;
+    return 0;
+  }
   int t = time_begin(s, "mg_vcycle_op", 4.0 * s->vop_ld * G.n);
-  hipLaunchKernelGGL(k_vcycle_op, dim3(G.n), dim3(64 * VOP_WAVES), P.pool + VOP_WAVES * TCELLS * sizeof(float), s->st,
-                     P, s->vop_ld, s->vop, s->sc);
+  hipLaunchKernelGGL(k_vcycle_op, dim3(G.n), dim3(64 * VOP_WAVES), lds, s->st, P, s->vop_ld, s->vop, s->sc);
   time_end(s, t);
+  return 0;
 }
 
 // level 0 up: prolongation from level 1 (x(1)) + pass C, tmp(0) -> x(0); then pass D on x(0)
@@ -2742,7 +2761,8 @@ static int solve_setup(tq_sim* s) {
                        (const DevScalars*)s->sc);
   }
   time_end(s, t);
-  if (vop_active(s)) launch_vcycle_op(s);
+  if (vop_active(s))
+    if (int e = launch_vcycle_op(s)) return e;
   hipLaunchKernelGGL(k_op_count, dim3(1), dim3(64), 0, s->st, s->sc);
   TQ_LAUNCH_CHECK();
   return 0;

@@ -95,6 +95,7 @@ int tq_batch_create(const tq_desc* descs, int n, int threads, tq_batch** out) {
       tq::set_error(msg);
       return e;
     }
+    tq::drop_second_stream(s);
     b->tanks.push_back(s);
   }
   b->nthreads = std::max(1, std::min(threads, std::min(n, 64)));
