@@ -78,8 +78,10 @@ def run_gpu(tq, work, precision, warmup, steps, schedule=None, time_kernel=None,
         "particles_start": n0, "particles_end": n1,
         "particle_steps_per_s": subs * 0.5 * (n0 + n1) / (t1 - t0),
         "pcg_iters_per_solve": iters / max(solves, 1),
-        "last_true_rel_residual": st1["true_resid"] / st1["norm_b"] if st1["norm_b"] > 0 else 0.0,
+        "last_true_rel_residual": st1["true_resid_iterate"] / st1["norm_b"] if st1["norm_b"] > 0 else 0.0,
+        "last_true_rel_residual_binary32_q": st1["true_resid"] / st1["norm_b"] if st1["norm_b"] > 0 else 0.0,
         "last_rel_residual": st1["norm"] / st1["norm_b"] if st1["norm_b"] > 0 else 0.0,
+        "restarts_last_solve": st1["restarts"],
         "active_bubbles": st1["active_bubbles"], "levels": st1["levels"],
         "schedule": sched, "timing": timing,
     }
@@ -268,6 +270,27 @@ def dominant_kernel(tq, work, precision, frames=3):
     return name, t
 
 
+def host_cpu_info():
+    """nproc and the lscpu summary of the host the CPU baseline ran on."""
+    import subprocess
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keep = ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)", "CPU max MHz",
+                "L3 cache")
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keep:
+                info["lscpu " + k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        info["lscpu"] = "unavailable"
+    return info
+
+
 def cpu_baseline(work, budget_s=15.0, tol=1e-6, maxit=200):
     """The oracle (single-threaded C++ restatement of the reference CPU engine) on a bounded
     sample of the same workload: the first frames of the same scene, same particles.  Driven from
@@ -315,6 +338,7 @@ def cpu_baseline(work, budget_s=15.0, tol=1e-6, maxit=200):
         el = time.perf_counter() - t0
         driver = "python (ctypes over oracle/liboracle.so)"
     return {"value": round(subs / el, 4), "unit": "substeps/s", "cores": 1, "kind": "port", "driver": driver,
+            "host": host_cpu_info(),
             "sample": "first %d frames (%d substeps) of the same workload on the oracle (JS-exact binary64 "
                       "compute / binary32 storage, PCG to 1e-6), %.1f s on one host core" % (frames, subs, el)}
 
@@ -409,8 +433,9 @@ def main():
     mixed = None
     if not args.no_mixed:
         mixed = run_gpu(tq, work, tq.PREC_MIXED, args.warmup, args.steps, schedule=sched)
-        log("mixed: %.1f substeps/s, %.2f PCG it/solve, last rel resid %.2e" % (
-            mixed["substeps_per_s"], mixed["pcg_iters_per_solve"], mixed["last_rel_residual"]))
+        log("mixed: %.1f substeps/s, %.2f PCG it/solve, last rel resid %.2e (true %.2e)" % (
+            mixed["substeps_per_s"], mixed["pcg_iters_per_solve"], mixed["last_rel_residual"],
+            mixed["last_true_rel_residual"]))
     graph = {}
     if not args.no_graph:
         for lab, prec in (("fp32", tq.PREC_FP32), ("mixed", tq.PREC_MIXED)):
@@ -483,7 +508,13 @@ def main():
             "precision": "fp32 (value); mixed reported beside it",
             "cells": work.nx * work.ny, "particles": fp32["particles_start"],
             "substeps_timed": fp32["substeps"], "pcg_iters_per_solve": round(fp32["pcg_iters_per_solve"], 3),
-            "last_rel_residual": fp32["last_rel_residual"], "last_true_rel_residual": fp32["last_true_rel_residual"],
+            "last_rel_residual": fp32["last_rel_residual"],
+            "last_true_rel_residual": fp32["last_true_rel_residual"],
+            "last_true_rel_residual_binary32_q": fp32["last_true_rel_residual_binary32_q"],
+            "last_restarts": fp32["restarts_last_solve"],
+            "mixed_last_true_rel_residual": mixed["last_true_rel_residual"] if mixed else None,
+            "stop_test": ("recursive residual <= 1e-6, then the binary64 true residual of the compensated iterate "
+                          "(q + q_lo) recomputed; restart from it while above 1e-6"),
             "active_bubbles_end": fp32["active_bubbles"], "mg_levels": fp32["levels"],
             "launches": "eager; a second run of the same schedule times the dominant kernel with HIP events"},
         "particle_steps_per_s": round(fp32["particle_steps_per_s"], 1),

@@ -210,6 +210,14 @@ typedef struct tq_solver_stats {
   long long total_iterations; /* PCG iterations since creation */
   long long solves;           /* pressure solves since creation */
   long long operator_builds;  /* solves that rebuilt the dense coarse operators (their masks changed) */
+  /* Residual-terminated mode (tol > 0): the iterate is kept as a compensated pair of binary32
+   * words (q + q_lo) and its true residual is recomputed in binary64 when the recursive residual
+   * reaches tol; while it is above tol the solve restarts from it (residual replacement, at most
+   * 3 times).  true_resid above is that of the binary32 q the solve hands on (what the gradient
+   * reads: the storage floor of binary32 pressures). */
+  double true_resid_iterate;  /* binary64 true residual of the compensated iterate at the stop; -1 if not computed */
+  int first_iterations;       /* iterations of the first segment (before any restart) */
+  int restarts;               /* residual replacements of the last solve */
 } tq_solver_stats;
 int tq_solver_stats_get(tq_sim* s, tq_solver_stats* st, int compute_true_residual);
 

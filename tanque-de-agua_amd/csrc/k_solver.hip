@@ -549,10 +549,20 @@ __global__ __launch_bounds__(1024) void k_bub_init(const uint32_t* __restrict__ 
   }
 }
 
+// a restart of the compensated iterate: the bubble directions and the fixed-point accumulators
+// start from zero as at the solve's set-up; y (the bubble unknowns) is kept
+__global__ __launch_bounds__(1024) void k_bub_restart(double* __restrict__ bvec) {
+  for (int r = threadIdx.x; r < MAX_BOLHAS; r += blockDim.x)
+    for (int q = BV_AP; q < BV_N; q++) bv(bvec, q)[r] = 0.0;
+}
+
 // ================================================================== init: warm-start residual
 // q := 0 off water, bubble cells := y (difundir); r = -div - A q; bubble rows A_b [q;y].
+// qmode 0: q alone (the reference's binary32 iterate); 1: compensated iterate q + qlo starting
+// with qlo = 0; 2: a restart of the compensated iterate (its residual from q + qlo in binary64).
 __global__ __launch_bounds__(256) void k_pcg_init(const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
                                                   const float* __restrict__ div, float* __restrict__ q,
+                                                  float* __restrict__ qlo, int qmode,
                                                   float* __restrict__ r, float* __restrict__ pbuf0, int nx, int C,
                                                   double* __restrict__ bvec, const int* __restrict__ labelSlot,
                                                   double* __restrict__ partials, double* __restrict__ bpart,
@@ -563,15 +573,24 @@ __global__ __launch_bounds__(256) void k_pcg_init(const uint8_t* __restrict__ op
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
   const double* yb = bv(bvec, BV_Y);
   double* acc0 = bv(bvec, BV_ACCI);
+  // the cell's value as the operator sees it: binary32 q (bubble cells: their y), plus qlo on a restart
   auto qe = [&](int d, uint8_t md) -> float {
     if (mask_type(md) == AGUA) return q[d];
     if (md & MK_BUB) return (float)yb[rot[d]];
     return 0.f;
   };
+  auto qd = [&](int d, uint8_t md) -> double {
+    if (qmode == 2) {  // a restart: water q + qlo, bubble y in binary64
+      if (mask_type(md) == AGUA) return (double)q[d] + (double)qlo[d];
+      if (md & MK_BUB) return yb[rot[d]];
+    }
+    return (double)qe(d, md);
+  };
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < C) {
     const uint8_t m = opMask[c];
     const float qc = qe(c, m);
+    const double qcd = qd(c, m);
     pbuf0[c] = 0.f;
     if (mask_type(m) == AGUA) {
       double s = 0;
@@ -580,14 +599,14 @@ __global__ __launch_bounds__(256) void k_pcg_init(const uint8_t* __restrict__ op
       for (int k = 0; k < 4; k++) {
         if (!(m & (1 << k))) continue;
         const uint8_t md = opMask[nb[k]];
-        s += (double)qe(nb[k], md);
+        s += qd(nb[k], md);
         if (mask_type(md) == AR) {
           if (md & MK_BUB) {
-            if (md & MK_BUBIN) bub_add(sl, labelSlot, acc0, rot[nb[k]], -(double)qc);
+            if (md & MK_BUBIN) bub_add(sl, labelSlot, acc0, rot[nb[k]], -qcd);
           } else anchored = true;
         }
       }
-      const double aq = mask_cnt(m) * (double)qc - s;
+      const double aq = mask_cnt(m) * qcd - s;
       const float rr = (float)(-(double)div[c] - aq);
       r[c] = rr;
       v[0] = (double)rr * rr;
@@ -597,18 +616,41 @@ __global__ __launch_bounds__(256) void k_pcg_init(const uint8_t* __restrict__ op
       v[4] = anchored ? 1 : 0;
     } else r[c] = 0.f;
     q[c] = qc;
+    if (qmode == 1 || (qmode == 2 && mask_type(m) != AGUA)) qlo[c] = 0.f;
   }
   store_partials<5>(v, 5, partials, sh);
   if (ns > 0) store_partials<BSLOTS>(sl, ns, bpart, sh);
 }
 
+// restart = 0 starts a solve (tolerance, iteration cap and the mixed modes' scale numerator are
+// set here, as arguments, so no host copy precedes the solve); restart = 1 restarts the
+// compensated iterate from its true residual, adding the finished segment's iterations to the count.
 __global__ __launch_bounds__(FIN_THREADS) void k_fin_init(const uint32_t* __restrict__ facesB, double* __restrict__ bvec,
                                                           const double* __restrict__ rhsB,
                                                           const int* __restrict__ activeList,
                                                           const int* __restrict__ labelSlot,
                                                           const double* __restrict__ partials,
-                                                          const double* __restrict__ bpart, int nb,
+                                                          const double* __restrict__ bpart, int nb, double tol,
+                                                          int maxit, float mult, int restart,
                                                           DevScalars* __restrict__ sc) {
+  if (threadIdx.x == 0) {
+    if (restart) {
+      const int seg = sc->itp[0] > sc->itp[1] ? sc->itp[0] : sc->itp[1];
+      if (sc->restarts == 0) sc->iters_first = seg;
+      sc->iters_base += seg;
+      sc->restarts += 1;
+    } else {
+      sc->tol = tol;
+      sc->maxit = maxit;
+      sc->mg_mult = mult;
+      sc->iters_base = 0;
+      sc->iters_first = 0;
+      sc->restarts = 0;
+    }
+    sc->true_norm = -1;
+    sc->true_done = 0;
+  }
+  __syncthreads();
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], slotTot[BSLOTS];
   __shared__ double mean_s;
   const int na = sc->active_bubbles;
@@ -659,7 +701,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_init(const uint32_t* __rest
     sc->norm_last = sc->norm0;
     sc->normb = sqrt(bb);
     sc->iters = 0;
-    sc->solves += 1;
+    if (!restart) sc->solves += 1;
     sc->rz = 0;
     sc->beta = 0;
     sc->alpha = 0;
@@ -831,7 +873,8 @@ template <class SB, bool UPD, bool PASSA>
 __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMask, const uint8_t* __restrict__ mg0,
                                                const uint32_t* __restrict__ rot, const uint32_t* __restrict__ cellsB,
                                                const float* __restrict__ p, const float* __restrict__ ap,
-                                               float* __restrict__ q, float* __restrict__ r, int nx, int ny, int ntx,
+                                               float* __restrict__ q, float* __restrict__ qlo, int comp,
+                                               float* __restrict__ r, int nx, int ny, int ntx,
                                                int nty, int vgx, int nvb, BubArgs B, const double* __restrict__ partA,
                                                const double* __restrict__ bpartA, int nbs, int par,
                                                SB* __restrict__ b0, float* __restrict__ x0,
@@ -850,7 +893,7 @@ __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMas
   double rr[1] = {0};
   // every load of the four cells first (one memory latency), then the arithmetic
   uint8_t om[2][2], gm[2][2];
-  float qv[2][2], pv[2][2], rv[2][2], av[2][2];
+  float qv[2][2], pv[2][2], rv[2][2], av[2][2], lv[2][2];
   size_t ci[2][2];
   bool in[2][2];
   auto load = [&](int g) {
@@ -874,6 +917,7 @@ __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMas
           qv[cq][rw] = q[c];
           pv[cq][rw] = p[c];
           av[cq][rw] = ap[c];
+          lv[cq][rw] = comp ? qlo[c] : 0.f;
         }
       }
   };
@@ -897,7 +941,14 @@ __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMas
           if (mask_type(m) == AGUA) {
             float rn;
             if (UPD) {
-              q[c] = (float)((double)qv[cq][rw] + alpha * (double)pv[cq][rw]);
+              if (comp) {  // the compensated iterate: q + qlo keeps the update's binary64 sum
+                const double sx = (double)qv[cq][rw] + (double)lv[cq][rw] + alpha * (double)pv[cq][rw];
+                const float hi = (float)sx;
+                q[c] = hi;
+                qlo[c] = (float)(sx - (double)hi);
+              } else {
+                q[c] = (float)((double)qv[cq][rw] + alpha * (double)pv[cq][rw]);
+              }
               rn = (float)((double)rv[cq][rw] - alpha * (double)av[cq][rw]);
               r[c] = rn;
               rr[0] += (double)rn * rn;
@@ -2062,7 +2113,9 @@ __global__ __launch_bounds__(256) void k_pcg_final(const uint8_t* __restrict__ t
 __global__ __launch_bounds__(1024) void k_pcg_final_b(const double* __restrict__ bvec, const int* __restrict__ activeList,
                                                       float* __restrict__ yB32, DevScalars* __restrict__ sc) {
   if (threadIdx.x == 0) {
-    const int it = sc->itp[0] > sc->itp[1] ? sc->itp[0] : sc->itp[1];
+    const int seg = sc->itp[0] > sc->itp[1] ? sc->itp[0] : sc->itp[1];
+    if (sc->restarts == 0) sc->iters_first = seg;
+    const int it = sc->iters_base + seg;
     sc->iters = it;
     sc->total_iters += (unsigned long long)it;
   }
@@ -2072,12 +2125,19 @@ __global__ __launch_bounds__(1024) void k_pcg_final_b(const double* __restrict__
   }
 }
 
-// true residual ||b' - A[q;y]|| (diagnostics; b' is the projected rhs that was solved)
+// true residual ||b' - A[q;y]|| in binary64 (b' is the projected rhs that was solved), of the
+// binary32 q or, with qlo, of the compensated iterate q + qlo.  gated: the stop check of the
+// residual-terminated mode, appended to every chunk of iterations -- it runs once, after the
+// segment's stop, and returns at once otherwise.
+__device__ inline bool true_res_skip(const DevScalars* sc, int gated) { return gated && (!sc->done || sc->true_done); }
+
 __global__ __launch_bounds__(256) void k_true_res(const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
-                                                  const float* __restrict__ div, const float* __restrict__ q, int nx,
-                                                  int C, double* __restrict__ bvec, const int* __restrict__ labelSlot,
-                                                  double* __restrict__ partials, double* __restrict__ bpart,
+                                                  const float* __restrict__ div, const float* __restrict__ q,
+                                                  const float* __restrict__ qlo, int nx, int C,
+                                                  double* __restrict__ bvec, const int* __restrict__ labelSlot,
+                                                  double* __restrict__ partials, double* __restrict__ bpart, int gated,
                                                   const DevScalars* __restrict__ sc) {
+  if (true_res_skip(sc, gated)) return;
   __shared__ double sh[BSLOTS * 16];
   const int ns = nslots(sc->active_bubbles);
   const double* yb = bv(bvec, BV_Y);
@@ -2085,25 +2145,25 @@ __global__ __launch_bounds__(256) void k_true_res(const uint8_t* __restrict__ op
   const double mean = sc->proj_mean;
   double v[1] = {0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-  auto qe = [&](int d, uint8_t md) -> float {
-    if (mask_type(md) == AGUA) return q[d];
-    if (md & MK_BUB) return (float)yb[rot[d]];
-    return 0.f;
+  auto qe = [&](int d, uint8_t md) -> double {
+    if (mask_type(md) == AGUA) return qlo ? (double)q[d] + (double)qlo[d] : (double)q[d];
+    if (md & MK_BUB) return qlo ? yb[rot[d]] : (double)(float)yb[rot[d]];  // compensated: y in binary64
+    return 0.0;
   };
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < C) {
     const uint8_t m = opMask[c];
     if (mask_type(m) == AGUA) {
-      const float qc = qe(c, m);
+      const double qc = qe(c, m);
       double s = 0;
       const int nb[4] = {c - 1, c + 1, c - nx, c + nx};
       for (int k = 0; k < 4; k++) {
         if (!(m & (1 << k))) continue;
         const uint8_t md = opMask[nb[k]];
-        s += (double)qe(nb[k], md);
-        if (md & MK_BUBIN) bub_add(sl, labelSlot, acc0, rot[nb[k]], -(double)qc);
+        s += qe(nb[k], md);
+        if (md & MK_BUBIN) bub_add(sl, labelSlot, acc0, rot[nb[k]], -qc);
       }
-      const double e = (-(double)div[c] - mean) - (mask_cnt(m) * (double)qc - s);
+      const double e = (-(double)div[c] - mean) - (mask_cnt(m) * qc - s);
       v[0] = e * e;
     }
   }
@@ -2116,9 +2176,10 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_true_res(const uint32_t* __
                                                               const int* __restrict__ activeList,
                                                               const int* __restrict__ labelSlot,
                                                               const double* __restrict__ partials,
-                                                              const double* __restrict__ bpart, int nb,
-                                                              const DevScalars* __restrict__ sc,
+                                                              const double* __restrict__ bpart, int nb, int gated,
+                                                              DevScalars* __restrict__ sc,
                                                               double* __restrict__ out) {
+  if (true_res_skip(sc, gated)) return;
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], slotTot[BSLOTS];
   const int na = sc->active_bubbles;
   double tot[1];
@@ -2141,7 +2202,14 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_true_res(const uint32_t* __
     bacc[0] += e * e;
   }
   block_sum<1>(bacc, sh);
-  if (threadIdx.x == 0) *out = sqrt(tot[0] + bacc[0]);
+  if (threadIdx.x == 0) {
+    const double t = sqrt(tot[0] + bacc[0]);
+    if (out) *out = t;
+    if (gated) {
+      sc->true_norm = t;
+      sc->true_done = 1;
+    }
+  }
 }
 
 // ================================================================== host side
@@ -2329,7 +2397,8 @@ static void launch_in_a0(tq_sim* s, bool upd) {
   const float* mgs = &s->sc->mgs[it_par(s)];
 #define TQ_IN_A0(SB, U, P)                                                                                        \
   hipLaunchKernelGGL((k_in_a0<SB, U, P>), g, dim3(256), 0, s->st, s->opMask, F.mask, s->rot, s->cellsB, pp, s->ap, \
-                     s->q, s->r, F.nx, F.ny, ntx, nty, vgx, nvb, B, rs.part, rs.bpart, rs.nb, par, (SB*)F.b,      \
+                     s->q, s->qlo, s->qcomp, s->r, F.nx, F.ny, ntx, nty, vgx, nvb, B, rs.part, rs.bpart, rs.nb, par, \
+                     (SB*)F.b,                                                                                     \
                      (float*)F.x,                                                                                  \
                      s->partials + ps_off(PS_RR), s->mg_scaled, mgs, s->sc)
   if (F.hb) {
@@ -2667,8 +2736,8 @@ static void pcg_iteration(tq_sim* s) {
 // Two PCG iterations (the buffer ping-pong period) captured once into a HIP graph and
 // replayed; the capture depends on the level plan, so a plan change recaptures.
 static int graph_pair(tq_sim* s) {
-  const int key = (s->d.precision << 8) | ((s->d.mg_half_level + 1) << 2) | (vop_active(s) ? 2 : 0) |
-                  (s->mg_scaled ? 1 : 0);
+  const int key = (s->qcomp << 16) | (s->d.precision << 8) | ((s->d.mg_half_level + 1) << 2) |
+                  (vop_active(s) ? 2 : 0) | (s->mg_scaled ? 1 : 0);
   if (!s->graph_exec || s->graph_key != key) {
     graph_drop(s);
     hipGraph_t g = nullptr;
@@ -2715,6 +2784,12 @@ static int run_iterations(tq_sim* s, int n) {
 static int solve_setup(tq_sim* s) {
   const int C = s->C;
   const int L = (int)s->lv.size();
+  // a matrix build of an earlier solve that left early (an error return) may still be running on
+  // the second stream: the set-up below rewrites the masks and flags it reads
+  if (s->vop_pending) {
+    TQ_HIP(hipStreamWaitEvent(s->st, s->ev_vop, 0));
+    s->vop_pending = false;
+  }
   int t = time_begin(s, "pcg_setup", (1 + 4 + 1 + 1) * (double)C);
   {
     PyrPlan Pp{};
@@ -2768,50 +2843,73 @@ static int solve_setup(tq_sim* s) {
   return 0;
 }
 
+// the stop check of the residual-terminated mode, queued after every chunk of iterations: the
+// binary64 true residual of the compensated iterate, computed once after the segment stops
+static void launch_true_check(tq_sim* s) {
+  const int C = s->C;
+  const int nb = blocks_of(C);
+  hipLaunchKernelGGL(k_true_res, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->div, s->q, (const float*)s->qlo,
+                     s->nx, C, s->bvec, s->labelSlot, s->partials, s->bpart, 1, (const DevScalars*)s->sc);
+  hipLaunchKernelGGL(k_fin_true_res, dim3(1), dim3(FIN_THREADS), 0, s->st, s->facesB, s->bvec, s->rhsB, s->activeList,
+                     s->labelSlot, s->partials, s->bpart, nb, 1, s->sc, (double*)nullptr);
+}
+
+constexpr int MAX_RESTARTS = 3;  // residual replacements per solve (each one re-runs the set-up residual)
+
 int pressure_solve(tq_sim* s) {
   const int C = s->C;
   s->mg_scaled = plan_precision(s) ? 1 : 0;
-  s->pc = 0;
-  s->zc = 0;
-  if (int e = solve_setup(s)) return e;
   const double tol = s->d.tol;
   const int maxit = s->d.iteracoes;
-  TQ_HIP(hipMemcpyAsync(&s->sc->tol, &tol, sizeof(double), hipMemcpyHostToDevice, s->st));
-  TQ_HIP(hipMemcpyAsync(&s->sc->maxit, &maxit, sizeof(int), hipMemcpyHostToDevice, s->st));
+  // residual-terminated mode: the iterate is kept compensated (q + qlo) so its true residual can
+  // reach tol; the fixed-iteration mode keeps the reference's binary32 iterate exactly
+  s->qcomp = tol > 0 ? 1 : 0;
+  if (int e = solve_setup(s)) return e;
   // FP16 right-hand sides with FP32 x can sit 2^12 above unit norm (more of the residual stays
   // in the normal FP16 range); FP16 x keeps unit-norm input so the V-cycle output stays in range
   const float mult = s->d.precision == TQ_PREC_MIXED_B ? 4096.f : 1.f;
-  TQ_HIP(hipMemcpyAsync(&s->sc->mg_mult, &mult, sizeof(float), hipMemcpyHostToDevice, s->st));
   const int nb = blocks_of(C);
-  int t = time_begin(s, "pcg_init", (1 + 1 + 4 + 4 + 4 + 4 + 4) * (double)C);
-  hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->div, s->q, s->r, s->pbuf[s->pc],
-                     s->nx, C, s->bvec, s->labelSlot, s->partials, s->bpart, s->sc);
-  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(FIN_THREADS), 0, s->st, s->facesB, s->bvec, s->rhsB, s->activeList,
-                     s->labelSlot, s->partials, s->bpart, nb, s->sc);
-  time_end(s, t);
-  TQ_LAUNCH_CHECK();
-  s->it_k = -1;
-  launch_in_a0(s, false);
-  precondition_rest(s);
-  s->it_k = 0;
-  TQ_LAUNCH_CHECK();
-  if (tol <= 0) {
-    const int rc = run_iterations(s, maxit);
-    if (rc) return rc;
-  } else {
-    // launch in chunks and poll the device stop flag (kernels after the stop exit at once)
+  for (int restart = 0;; restart++) {
+    s->pc = 0;
+    s->zc = 0;
+    int t = time_begin(s, "pcg_init", (1 + 1 + 4 + 4 + 4 + 4 + 4 + (s->qcomp ? 4 : 0)) * (double)C);
+    if (restart) hipLaunchKernelGGL(k_bub_restart, dim3(1), dim3(1024), 0, s->st, s->bvec);
+    hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->div, s->q, s->qlo,
+                       s->qcomp ? (restart ? 2 : 1) : 0, s->r, s->pbuf[s->pc], s->nx, C, s->bvec, s->labelSlot,
+                       s->partials, s->bpart, s->sc);
+    hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(FIN_THREADS), 0, s->st, s->facesB, s->bvec, s->rhsB, s->activeList,
+                       s->labelSlot, s->partials, s->bpart, nb, tol, maxit, mult, restart ? 1 : 0, s->sc);
+    time_end(s, t);
+    TQ_LAUNCH_CHECK();
+    s->it_k = -1;
+    launch_in_a0(s, false);
+    precondition_rest(s);
+    s->it_k = 0;
+    TQ_LAUNCH_CHECK();
+    if (tol <= 0) {
+      const int rc = run_iterations(s, maxit);
+      if (rc) return rc;
+      break;
+    }
+    // launch in chunks, each followed by the gated stop check, and poll the device flags
+    // (kernels after the stop exit at once); the first chunk is the last solve's first segment
     int launched = 0;
-    int chunk = std::max(2, s->sc_host->iters > 0 ? (s->sc_host->iters + 1) & ~1 : 8);
+    int chunk = restart ? 4 : std::max(2, s->sc_host->iters_first > 0 ? (s->sc_host->iters_first + 1) & ~1 : 8);
     while (launched < maxit) {
       const int n = std::min(chunk, maxit - launched);
       const int rc = run_iterations(s, n);
       if (rc) return rc;
       launched += n;
+      launch_true_check(s);
+      TQ_LAUNCH_CHECK();
       TQ_HIP(hipMemcpyAsync(s->sc_host, s->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s->st));
       TQ_HIP(hipStreamSynchronize(s->st));
       if (s->sc_host->done) break;
       chunk = 4;
     }
+    const DevScalars& h = *s->sc_host;
+    const bool converged = h.norm_last <= tol * h.normb;  // the segment stopped on its recursive residual
+    if (!converged || !(h.true_norm > tol * h.normb) || restart >= MAX_RESTARTS) break;
   }
   hipLaunchKernelGGL(k_pcg_final, dim3(blocks_of(C)), dim3(256), 0, s->st, s->tipo, s->q, C);
   hipLaunchKernelGGL(k_pcg_final_b, dim3(1), dim3(1024), 0, s->st, s->bvec, s->activeList, s->yB32, s->sc);
@@ -2823,10 +2921,10 @@ int true_residual(tq_sim* s, double* out) {
   const int C = s->C;
   const int nb = blocks_of(C);
   double* dres = s->partials + ps_off(PS_LAST) + MAX_RED_BLOCKS - 1;
-  hipLaunchKernelGGL(k_true_res, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->div, s->q, s->nx, C, s->bvec,
-                     s->labelSlot, s->partials, s->bpart, s->sc);
+  hipLaunchKernelGGL(k_true_res, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->div, s->q, (const float*)nullptr,
+                     s->nx, C, s->bvec, s->labelSlot, s->partials, s->bpart, 0, (const DevScalars*)s->sc);
   hipLaunchKernelGGL(k_fin_true_res, dim3(1), dim3(FIN_THREADS), 0, s->st, s->facesB, s->bvec, s->rhsB, s->activeList,
-                     s->labelSlot, s->partials, s->bpart, nb, s->sc, dres);
+                     s->labelSlot, s->partials, s->bpart, nb, 0, s->sc, dres);
   TQ_LAUNCH_CHECK();
   TQ_HIP(hipMemcpyAsync(out, dres, sizeof(double), hipMemcpyDeviceToHost, s->st));
   TQ_HIP(hipStreamSynchronize(s->st));

@@ -121,7 +121,7 @@ static int alloc_all(tq_sim* s) {
   DALLOC(s->rhsB, MAX_BOLHAS); DALLOC(s->rhsB32, MAX_BOLHAS); DALLOC(s->yB32, MAX_BOLHAS); DALLOC(s->volAlvo, MAX_BOLHAS);
   DALLOC(s->vivo, MAX_BOLHAS); DALLOC(s->vivoNovo, MAX_BOLHAS);
   DALLOC(s->bvec, BV_COUNT * (size_t)MAX_BOLHAS); DALLOC(s->activeList, MAX_BOLHAS);
-  DALLOC(s->opMask, C); DALLOC(s->r, C); DALLOC(s->ap, C);
+  DALLOC(s->opMask, C); DALLOC(s->r, C); DALLOC(s->ap, C); DALLOC(s->qlo, C);
   for (int b = 0; b < 2; b++) { DALLOC(s->zbuf[b], C); DALLOC(s->pbuf[b], C); }
   DALLOC(s->partials, RED_SLOTS * (size_t)MAX_RED_BLOCKS);
   DALLOC(s->partials_r, RED_SLOTS * (size_t)MAX_RED_BLOCKS);
@@ -151,7 +151,7 @@ static int alloc_all(tq_sim* s) {
   TQ_HIP(z(s->yB32, 4 * MAX_BOLHAS)); TQ_HIP(z(s->volAlvo, 4 * MAX_BOLHAS)); TQ_HIP(z(s->vivo, MAX_BOLHAS));
   TQ_HIP(z(s->bvec, 8 * BV_COUNT * (size_t)MAX_BOLHAS));
   for (int b = 0; b < 2; b++) { TQ_HIP(z(s->zbuf[b], 4 * C)); TQ_HIP(z(s->pbuf[b], 4 * C)); }
-  TQ_HIP(z(s->r, 4 * C)); TQ_HIP(z(s->ap, 4 * C));
+  TQ_HIP(z(s->r, 4 * C)); TQ_HIP(z(s->ap, 4 * C)); TQ_HIP(z(s->qlo, 4 * C));
   TQ_HIP(z(s->sc, sizeof(DevScalars)));
   std::memset(s->sc_host, 0, sizeof(DevScalars));
   return 0;
@@ -181,7 +181,7 @@ static void free_all(tq_sim* s) {
   f(s->compIdx); f(s->compCells); f(s->compAtm); f(s->compTopo); f(s->compAlvo); f(s->compDono); f(s->compLabel);
   f(s->hkeys); f(s->hcount); f(s->hfirst); f(s->hlist); f(s->ordBuf); f(s->pBolha); f(s->volB); f(s->facesB);
   f(s->cellsB); f(s->rhsB); f(s->rhsB32); f(s->yB32); f(s->volAlvo); f(s->vivo); f(s->vivoNovo); f(s->bvec);
-  f(s->activeList); f(s->opMask); f(s->r); f(s->ap);
+  f(s->activeList); f(s->opMask); f(s->r); f(s->ap); f(s->qlo);
   for (int b = 0; b < 2; b++) { f(s->zbuf[b]); f(s->pbuf[b]); }
   f(s->partials); f(s->partials_r); f(s->sc); f(s->scanBlocks);
   if (s->sc_host) (void)hipHostFree(s->sc_host);
@@ -733,6 +733,9 @@ int tq_solver_stats_get(tq_sim* s, tq_solver_stats* st, int compute_true) {
   st->total_iterations = (long long)h.total_iters;
   st->solves = (long long)h.solves;
   st->operator_builds = (long long)h.op_builds;
+  st->true_resid_iterate = h.true_done ? h.true_norm : -1.0;
+  st->first_iterations = h.restarts > 0 ? h.iters_first : h.iters;
+  st->restarts = h.restarts;
   st->true_resid = -1;
   if (compute_true) {
     double tr = 0;

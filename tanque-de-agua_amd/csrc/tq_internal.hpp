@@ -94,6 +94,14 @@ struct DevScalars {
   float mgs[2];         // mixed precision: V-cycle input scale of the iterations of each parity
   int op_dirty;         // this solve's one-workgroup level masks differ from the last operator build
   unsigned long long op_builds;  // operator builds (coarsest and level-lc matrices) since creation
+  // residual-terminated mode: the iterate is the pair q + qlo (binary32 each); at the stop its true
+  // residual ||b - A[q + qlo; y]|| is recomputed in binary64, and a restart from that residual
+  // follows while it is above tol
+  double true_norm;     // true residual of the iterate at the last stop (-1: not computed yet)
+  int true_done;        // true_norm belongs to the current segment's stop
+  int iters_base;       // iterations of the segments before the current one (restarts)
+  int iters_first;      // iterations of the first segment
+  int restarts;         // restarts of the last solve
 };
 
 struct Level {
@@ -197,6 +205,7 @@ struct tq_sim {
   std::vector<tq::Level> lv;
   uint8_t* opMask = nullptr;
   float *r = nullptr, *ap = nullptr;
+  float* qlo = nullptr;          // low word of the compensated iterate (residual-terminated mode)
   float* zbuf[2] = {};
   float* pbuf[2] = {};
   int zc = 0, pc = 0;
@@ -211,6 +220,7 @@ struct tq_sim {
   int* labelSlot = nullptr;     // bubble label -> reduction slot (-1: 128-bit fixed-point atomics)
   int lc = 0;                   // first multigrid level solved inside the one-workgroup kernel
   int mg_scaled = 0;            // this solve stores some multigrid level in FP16 (input scaled)
+  int qcomp = 0;                // this solve keeps the compensated iterate q + qlo (tolerance mode)
   float* dense = nullptr;       // coarsest-level operator (when that level is small)
   float* vop = nullptr;         // level lc's V-cycle as a dense matrix (rows of vop_ld floats)
   int vop_ld = 0;               // 0: level lc has no dense V-cycle operator
