@@ -191,6 +191,14 @@ int tq_backward(tq_sim* s, const float* grad_state, int n, double tol, double* g
  * cells (fisica-cpu.js:305), densidade the P2G density splat (:241-250) of the current particles;
  * grad (may be NULL) gets d(fraction)/d(state), n x 8 floats in tq_read_particles order. */
 int tq_water_fraction(tq_sim* s, int i0, int j0, int i1, int j1, double* frac, float* grad, int n);
+/* tq_water_fraction's value (no gradient) without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight optimisation
+ * loops).  Its working memory (the box's FP64 density, the block partials, the result) lives in
+ * a fixed 8.4 MB device array that each GPU the library runs on holds until the process ends, so
+ * boxes are limited to 1048576 cells (TQ_E_CAPACITY above).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (two handles, two threads) race.  Blocks until the result
+ * is on the host. */
+int tq_water_fraction_noalloc(tq_sim* s, int i0, int j0, int i1, int j1, double* frac);
 /* Test hooks: substep records of a keep-all tape (checkpoint_every = 0): their count, and field
  * `field` of record k (0 particle count u32, 1 types C u8, 2 labels C u32, 3 qAr C f32,
  * 4 volumeAlvo and 5 yBolha MAX_BOLHAS f32, 6 decision words n u16 and 7 insertion numbers

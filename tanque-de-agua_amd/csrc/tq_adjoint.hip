@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "tq_device.hpp"
@@ -798,6 +799,57 @@ __global__ __launch_bounds__(256) void k_frac_grad(const float2* __restrict__ po
   out[i] = make_float2((float)gx, (float)gy);
 }
 
+// ------------------------------------------------------------------ water fraction without allocation
+// tq_water_fraction_noalloc keeps its working memory in `scratch`, a file-scope device array
+// (each GPU the library runs on holds its own copy until the process ends): the density of the
+// box's cells (FP64, box-relative index), one partial per 256-cell block and the result.  Host
+// code never passes `scratch` itself to a kernel: it passes the address hipGetSymbolAddress gives
+// for the current device.
+constexpr int WF_MAX_CELLS = 1 << 20;            // largest box (cells)
+constexpr int WF_MAX_PARTS = WF_MAX_CELLS / 256; // block partials of the box sum
+constexpr size_t WF_SCRATCH = (size_t)WF_MAX_CELLS + WF_MAX_PARTS + 8;
+static __device__ __align__(256) double scratch[WF_SCRATCH];  // 8.4 MB per device
+
+__global__ __launch_bounds__(256) void k_wf_clear(double* __restrict__ box, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) box[i] = 0.0;
+}
+// k_frac_dens restricted to the box: the same splat weights, added only to the box's cells
+__global__ __launch_bounds__(256) void k_wf_dens(const float2* __restrict__ pos, const DevScalars* __restrict__ sc,
+                                                 int nx, int ny, double h, int i0, int j0, int i1, int j1,
+                                                 double* __restrict__ box) {
+#pragma clang fp contract(off)
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= sc->n_slots) return;
+  const float2 P = pos[i];
+  if (P.x != P.x) return;
+  const DSten st = sten_c(nx, ny, 1 / h, P.x, P.y);
+  const int bw = i1 - i0;
+  for (int b = 0; b < 2; b++)
+    for (int a = 0; a < 2; a++) {
+      const int ci = st.i0 + a, cj = st.j0 + b;
+      if (ci < i0 || ci >= i1 || cj < j0 || cj >= j1) continue;
+      atomicAdd(&box[(ci - i0) + (cj - j0) * bw], wa(st.tx, a) * wa(st.ty, b));
+    }
+}
+// per 256 box cells: the sum of min(1, dens / PPC)
+__global__ __launch_bounds__(256) void k_wf_sum(const double* __restrict__ box, int n, double* __restrict__ part) {
+  __shared__ double sh[16];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[1] = {i < n ? fmin(1.0, box[i] / PPC) : 0.0};
+  block_sum<1>(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+// the box mean from the partials, summed in block order (one block)
+__global__ __launch_bounds__(256) void k_wf_final(const double* __restrict__ part, int nb, double cells,
+                                                  double* __restrict__ out) {
+  __shared__ double sh[16];
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += part[b];
+  block_sum<1>(v, sh);
+  if (threadIdx.x == 0) *out = v[0] / cells;
+}
+
 // ------------------------------------------------------------------ host: records
 static int blocks(long n) { return (int)std::max(1L, (n + 255) / 256); }
 
@@ -1262,6 +1314,58 @@ int tq_water_fraction(tq_sim* s, int i0, int j0, int i1, int j1, double* frac, f
   (void)hipFree(dens);
   (void)hipFree(g);
   return rc;
+}
+
+// The address of `scratch` on the current device, looked up once per device (each GPU holds its
+// own copy at its own address); null (with the error set) when the lookup fails.
+static double* wf_scratch_ptr() {
+  static std::mutex mu;
+  static double* ptrs[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) { (void)hip_fail(e, "hipGetDevice", __FILE__, __LINE__); return nullptr; }
+  if (dev < 0 || dev >= 64) { set_error("tq_water_fraction_noalloc: device index out of range"); return nullptr; }
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(tq::scratch));
+    if (e != hipSuccess || !p) {
+      (void)hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+      return nullptr;
+    }
+    ptrs[dev] = (double*)p;
+  }
+  return ptrs[dev];
+}
+
+int tq_water_fraction_noalloc(tq_sim* s, int i0, int j0, int i1, int j1, double* frac) {
+  if (!s) { set_error("null handle"); return TQ_E_ARG; }
+  if (!frac || i0 < 0 || j0 < 0 || i1 > s->nx || j1 > s->ny || i0 >= i1 || j0 >= j1) {
+    set_error("tq_water_fraction_noalloc: bad box");
+    return TQ_E_ARG;
+  }
+  const long cells = (long)(i1 - i0) * (j1 - j0);
+  if (cells > WF_MAX_CELLS) {
+    set_error("tq_water_fraction_noalloc: the box exceeds the 1048576-cell scratch (use tq_water_fraction)");
+    return TQ_E_CAPACITY;
+  }
+  double* base = wf_scratch_ptr();
+  if (!base) return TQ_E_HIP;
+  double* box = base;
+  double* part = base + WF_MAX_CELLS;
+  double* out = part + WF_MAX_PARTS;
+  const int nb = blocks(cells);
+  // the box is cleared by a kernel (no memset node if a caller captures this into a graph)
+  hipLaunchKernelGGL(k_wf_clear, dim3(nb), dim3(256), 0, s->st, box, (int)cells);
+  if (s->n_upper > 0)
+    hipLaunchKernelGGL(k_wf_dens, dim3(blocks(s->n_upper)), dim3(256), 0, s->st, s->pos[s->cur], s->sc, s->nx, s->ny,
+                       s->h, i0, j0, i1, j1, box);
+  hipLaunchKernelGGL(k_wf_sum, dim3(nb), dim3(256), 0, s->st, (const double*)box, (int)cells, part);
+  hipLaunchKernelGGL(k_wf_final, dim3(1), dim3(256), 0, s->st, (const double*)part, nb, (double)cells, out);
+  TQ_LAUNCH_CHECK();
+  TQ_HIP(hipMemcpyAsync(frac, out, sizeof(double), hipMemcpyDeviceToHost, s->st));
+  TQ_HIP(hipStreamSynchronize(s->st));
+  return 0;
 }
 
 int tq_backward(tq_sim* s, const float* grad_state, int n, double tol, double* grad_faucet, int n_faucet,

@@ -99,6 +99,8 @@ SIGNATURES = {
                               C.c_void_p, C.c_int]),
     "tq_water_fraction": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                                     C.c_void_p, C.c_int]),
+    "tq_water_fraction_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                            C.POINTER(C.c_double)]),
     "tq_tape_records": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "tq_set_strokes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "tq_read_sdf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
