@@ -213,8 +213,65 @@ def run_adjoint(tq, fd_h=1e-3, checkpoint_every=10, fd=True):
             r["sim"].close()
             L.append(r["loss"])
         fd["height_%d" % k] = {"fd": (L[0] - L[1]) / (2 * fd_h), "adjoint": float(out["g_heights"][k])}
-    res["central_differences"] = {"h": fd_h, "of": "the plain forward (every decision free)", "values": fd}
+    res["central_differences"] = {"h": fd_h, "of": "the plain forward (every decision free), whole run, configured box",
+                                  "values": fd}
+    res["window"] = adjoint_window(tq, fd_h=fd_h)
     return res
+
+
+FUNNEL_BOX = (100, 55, 240, 120)  # the funnel itself: the jet's water sits there within 20 frames
+
+
+def adjoint_window(tq, W=5, fd_h=1e-3, box=FUNNEL_BOX, descent_steps=4, step=0.05):
+    """A truncated-window objective the adjoint differentiates: the same 120-frame funnel run with
+    the substep schedule of the base run, the jet of the last W frames a free parameter and the
+    loss (water fraction of `box` - 0.5)^2 at the end.  Its gradient is the sum of the recorded
+    faucet calls' gradients of a backward over the window (tape from frame 120 - W).  Checked
+    against central differences of the forward at step fd_h, then a few normalised gradient
+    steps of `step` m/s on the window's jet, each with the loss change the gradient predicted."""
+    from tanque_hip import scenes
+    from tanque_hip.adjunto import run_funnel
+    w = scenes.funnel()
+    w.extra["box"] = box
+    base = run_funnel(w, grad=False)
+    sched = base["schedule"]
+    base["sim"].close()
+    v0 = np.array([0.5 * w.extra["faucet"]["pvx"], 0.5 * w.extra["faucet"]["pvy"] - 0.7])
+    F0 = w.frames - W
+
+    def evaluate(vw, grad):
+        jets = [tuple(v0)] * F0 + [tuple(vw)] * W
+        r = run_funnel(w, checkpoint_every=0, grad=grad, jets=jets, schedule=sched, tape_from=F0)
+        r["sim"].close()
+        return r
+
+    r0 = evaluate(v0, True)
+    g = np.asarray(r0["g_vjet"], np.float64)
+    fd = []
+    for k in range(2):
+        L = []
+        for sg in (1, -1):
+            v = v0.copy()
+            v[k] += sg * fd_h
+            L.append(evaluate(v, False)["loss"])
+        fd.append((L[0] - L[1]) / (2 * fd_h))
+    fd = np.array(fd)
+    err = float(np.abs(fd - g).max() / max(np.abs(g).max(), 1e-300))
+    steps = []
+    v, L, gv = v0.copy(), r0["loss"], g
+    for _ in range(descent_steps):
+        d = -gv / max(np.linalg.norm(gv), 1e-300) * step
+        pred = float(gv @ d)
+        v = v + d
+        r = evaluate(v, True)
+        steps.append({"jet": [round(float(x), 5) for x in v], "loss": r["loss"], "predicted_change": pred,
+                      "actual_change": r["loss"] - L})
+        L, gv = r["loss"], np.asarray(r["g_vjet"], np.float64)
+    return {"objective": "loss at frame %d with the jet of frames %d..%d free (box %s, target 0.5, base substep "
+                         "schedule)" % (w.frames, F0, w.frames - 1, box),
+            "loss": r0["loss"], "frac": r0["frac"], "adjoint": [float(x) for x in g],
+            "fd": [float(x) for x in fd], "fd_h": fd_h, "max_abs_err_over_max_component": err,
+            "descent": {"step_m_per_s": step, "loss_start": r0["loss"], "steps": steps}}
 
 
 def run_size_sweep(tq, sizes=((896, 504), (1792, 1008), (4096, 2304)), warmup=2, frames=3):
@@ -393,6 +450,8 @@ def main():
                     help="with --size-sweep: only these grids (repeatable), e.g. 4096x2304")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the adjoint configuration's forward and backward (for kernel profiles); prints its JSON")
+    ap.add_argument("--adjoint-check", action="store_true",
+                    help="only the adjoint configuration with its central differences and windowed check")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="start with a 256 MiB device copy (reference for rocprofv3 --pmc byte counters)")
     ap.add_argument("--nx", type=int, default=896)
@@ -405,8 +464,8 @@ def main():
     import tanque_hip as tq
     from tanque_hip import scenes
     tq.load()
-    if args.adjoint_only:
-        print(json.dumps({"adjoint": run_adjoint(tq, fd=False)}), flush=True)
+    if args.adjoint_only or args.adjoint_check:
+        print(json.dumps({"adjoint": run_adjoint(tq, fd=args.adjoint_check)}), flush=True)
         return
     if args.copy_calibration:
         copy_calibration()

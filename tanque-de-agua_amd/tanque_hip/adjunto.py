@@ -15,10 +15,16 @@ from .scenes import FUNNEL_SEGMENTS, passo_frame
 
 
 def run_funnel(work, checkpoint_every=10, tol=1e-6, solve_tol=1e-6, maxit=200, keep_emitted=False, grad=True,
-               sim_options=None):
+               sim_options=None, jets=None, schedule=None, tape_from=0):
     """Runs the funnel workload (scenes.funnel) forward and, with grad, backward.  Returns a dict:
     loss, frac, g_vjet (2), g_heights (32), g_sdf, the sim (left in its end state, tape open),
-    the substep schedule, and with keep_emitted the emitted particles' positions per frame."""
+    the substep schedule, and with keep_emitted the emitted particles' positions per frame.
+
+    jets: per-frame jet velocities (vx, vy) in place of the workload's one jet; schedule: the
+    substep count of every frame (default: passoMaximo from the running top speed, as the app);
+    tape_from: the frame at which recording starts, so the gradient is that of the loss with
+    respect to the jets of frames tape_from.. and the state at that frame (a truncated window;
+    g_vjet sums the recorded calls, g_faucet_calls has one row per recorded call)."""
     nx, ny, h = work.nx, work.ny, work.h
     opts = dict(iteracoes=maxit, tol=solve_tol)
     opts.update(sim_options or {})
@@ -28,20 +34,24 @@ def run_funnel(work, checkpoint_every=10, tol=1e-6, solve_tol=1e-6, maxit=200, k
         sim.adicionar(np.asarray(work.particles, np.float32), False)
     f = work.extra["faucet"]
     sem = Semente(f["seed"])
-    if grad:
-        sim.tape_begin(checkpoint_every)
     H = ny * h
+    fixed = schedule
     schedule, emitted = [], []
     t0 = time.perf_counter()
     for k in range(work.frames):
+        if grad and k == tape_from:
+            sim.tape_begin(checkpoint_every)
         if k < f["frames"]:
+            pvx, pvy = f["pvx"], f["pvy"]
+            if jets is not None:  # principal.js:302: jet = 0.5 pointer + (0, -0.7)
+                pvx, pvy = 2 * jets[k][0], 2 * (jets[k][1] + 0.7)
             n0 = sim.n
-            sim.emitirTorneira(f["x"], f["y"], f["pvx"], f["pvy"], f["raio"], f["absorcao"], work.dt_frame, sem)
-            if keep_emitted:
+            sim.emitirTorneira(f["x"], f["y"], pvx, pvy, f["raio"], f["absorcao"], work.dt_frame, sem)
+            if keep_emitted and k >= tape_from:
                 emitted.append(sim.particulas()[n0:, :2].astype(np.float64))
-        elif keep_emitted:
+        elif keep_emitted and k >= tape_from:
             emitted.append(np.zeros((0, 2)))
-        nsub = passo_frame(h, H, sim.vMax, work.dt_frame)
+        nsub = fixed[k] if fixed is not None else passo_frame(h, H, sim.vMax, work.dt_frame)
         schedule.append(nsub)
         sim.avancar(work.dt_frame, nsub)
     sim.sync()

@@ -35,14 +35,14 @@ def small_funnel(frames):
 def oracle_tape(sim, work, out):
     """The GPU run's tape in the oracle's layout (particles in insertion order)."""
     from oracle import adjoint as A
-    h = work.h
-    T = A.Tape(NX, NY, h)
+    h, nx, ny = work.h, work.nx, work.ny
+    T = A.Tape(nx, ny, h)
     for f, nsub in enumerate(out["schedule"]):
         T.add_frame([tuple(p) for p in out["emitted"][f]], nsub, work.dt_frame)
     nps, flags = [], []
     for k in range(sim.tape_records()):
         r = sim.tape_record(k)
-        bub, qextra, rhsfix = A.bubble_fields(NX, NY, h, r["dt"], r["tipo"], r["rot"], r["qAr"], r["volAlvo"], r["yB"])
+        bub, qextra, rhsfix = A.bubble_fields(nx, ny, h, r["dt"], r["tipo"], r["rot"], r["qAr"], r["volAlvo"], r["yB"])
         T.add_substep(r["tipo"], bub, r["qAr"], qextra, rhsfix)
         o = np.argsort(r["seq"], kind="stable")
         nps.append(r["n"])
@@ -115,3 +115,42 @@ def test_recording_refuses_other_state_changes(keep_all):
         sim.aplicarFerramenta({"tipo": "mao", "x": 0.5, "y": 0.5, "raio": 0.05})
     sim.tape_end()
     sim.aplicarFerramenta({"tipo": "mao", "x": 0.5, "y": 0.5, "raio": 0.05})
+
+
+# The configured funnel (256x144, 5.56 mm cells).  Its own loss box (bottom right) receives no
+# water before ~75 frames, and the GPU run (FP32, MGPCG to 1e-6) and the FP64 replay of its tape
+# separate after ~30 frames of this splashing flow, so the comparison at the configured grid uses a
+# box over the funnel, where the jet's water sits within 20 frames, at the longest horizon at
+# which the two forwards agree: 30 frames (at 40 frames the jet gradients already differ by 3x).
+FULL_BOX = (100, 55, 240, 120)
+FULL_FRAMES = 30
+
+
+def test_backward_matches_fp64_replay_at_configured_grid():
+    from oracle import adjoint as A
+    from tanque_hip import scenes
+    from tanque_hip.adjunto import run_funnel
+    w = scenes.funnel(frames=FULL_FRAMES)
+    w.extra["box"] = FULL_BOX
+    print("adjoint: funnel %dx%d, %d frames, keep-all tape, box %s" % (w.nx, w.ny, FULL_FRAMES, FULL_BOX), flush=True)
+    out = run_funnel(w, checkpoint_every=0, keep_emitted=True)
+    tape = oracle_tape(out["sim"], w, out)
+    sdf = np.asarray(out["sdf"], np.float64)
+    r = A.run(tape, sdf, np.zeros((0, 8)), vjet_of(w), FULL_BOX, target=w.extra["target"])
+    print("gpu: frac %.9g g_vjet %s | fp64 replay: frac %.9g g_vjet %s" % (out["frac"], out["g_vjet"], r["frac"],
+                                                                          r["g_vjet"]))
+    assert 0.01 < out["frac"] < 0.99
+    assert abs(out["frac"] - r["frac"]) < 1e-5
+    scale = np.abs(r["g_vjet"]).max()
+    assert scale > 0
+    err = np.abs(out["g_vjet"] - r["g_vjet"]).max() / scale
+    ng, ngg = np.linalg.norm(r["g_sdf"]), np.linalg.norm(out["g_sdf"])
+    e = np.linalg.norm(out["g_sdf"] - r["g_sdf"]) / max(ng, 1e-300)
+    print("jet gradient error %.3g of its largest component; |sdf gradient| gpu %.3g fp64 %.3g, relative L2 error "
+          "%.3g" % (err, ngg, ng, e))
+    assert err <= RTOL_JET
+    if ng > 0:
+        assert e < RTOL_SDF
+    else:  # no particle was pushed out of a wall in this window: both gradients vanish
+        assert ngg == 0
+    out["sim"].close()

@@ -55,3 +55,36 @@ def test_batch_rejects_bad_arguments(tq):
     with pytest.raises(tq.TqError):
         b.step(1 / 240, [1, -1])
     b.close()
+
+
+def test_full_batched_configuration_matches_tanks_stepped_alone(tq):
+    """The configured batch: 64 maze tanks of uniform random sizes (64-384 x 48-256 cells, 2.39 M
+    cells and 2.07 M particles in all), 3 PCG iterations as the reference; three frames batched
+    equal the same frames stepped tank after tank, bit for bit."""
+    from tanque_hip import scenes
+    from tanque_hip.scenes import passo_frame
+    ts = scenes.batched_mazes()
+    assert len(ts) == 64
+    cenas = [t.cena() for t in ts]
+    b = tq.Batch(cenas, [{"iteracoes": 3}] * len(ts), threads=8)
+    alone = [tq.SimulacaoHIP(t.cena(), iteracoes=3) for t in ts]
+    for t, sb, sa in zip(ts, b.tanks, alone):
+        sb.adicionar(t.particles, False)
+        sa.adicionar(t.particles, False)
+    for f in range(3):
+        subs = [passo_frame(t.h, c.altura, s.vMax, t.dt_frame) for t, c, s in zip(ts, cenas, b.tanks)]
+        b.step(ts[0].dt_frame, subs)
+        for sa, k in zip(alone, subs):
+            sa.avancar(ts[0].dt_frame, k)
+    cells = parts = 0
+    for k, (t, sb, sa) in enumerate(zip(ts, b.tanks, alone)):
+        pb, pa = sb.particulas(), sa.particulas()
+        assert len(pb) == len(pa) > 0, k
+        assert np.array_equal(pb.view(np.uint32), pa.view(np.uint32)), k
+        assert np.array_equal(sb.read(12), sa.read(12)), k
+        cells += t.nx * t.ny
+        parts += len(pb)
+    log("\n  64 tanks, %d cells, %d particles: batched == alone after 3 frames" % (cells, parts))
+    for s in alone:
+        s.close()
+    b.close()
