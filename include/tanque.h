@@ -79,13 +79,29 @@ int tq_set_sdf(tq_sim* s, const float* sdf, int row0, int row1);
  * >= 3 particles are skipped; the rest are appended in order.  `accepted` may be NULL. */
 int tq_add_particles(tq_sim* s, const float* list, int count, int check, int* accepted);
 
-/* aplicarFerramenta (fisica-cpu.js:129-152). kind: 0 none, 1 esponja, 2 mao, 3 corante */
+/* aplicarFerramenta (fisica-cpu.js:129-152). kind: 0 none, 1 esponja, 2 mao, 3 corante.
+ * Queued on the handle's stream (reads of the state afterwards see it). */
 typedef struct tq_tool {
   int kind;
   double x, y, raio, vx, vy;
   double absorcao[3];
 } tq_tool;
 int tq_apply_tool(tq_sim* s, const tq_tool* t);
+
+/* MotorGPU's non-blocking frame (motor.js:61-74, where quadro submits and contagem/vMax are read
+ * back later): tq_step's work queued on the handle's stream without waiting, followed by a
+ * status readback (tq_status_request).  With tol > 0 the pressure solves still poll the device
+ * for their stop test; with the reference's fixed iteration count nothing waits.  Not allowed
+ * while frames are recorded. */
+int tq_step_async(tq_sim* s, double dt, int substeps);
+/* Queues behind the handle's queued work a copy of the live particle count and vMax into pinned
+ * host memory (MotorGPU.lerContagem / its vMax readback, js/gpu/fisica.js).  No wait. */
+int tq_status_request(tq_sim* s);
+/* The last requested status without waiting: *ready = 0 while the device has not reached it;
+ * else the live count, vMax (*has_vmax = 0 before the first G2P) and the device time in ms of
+ * the last frame tq_step_async queued (-1 if none; MotorGPU's custoPorSubpasso measures the same
+ * with timestamp queries).  Any output pointer but `ready` may be NULL. */
+int tq_status_poll(tq_sim* s, int* ready, int* n, double* vmax, int* has_vmax, double* gpu_ms);
 
 /* avancar(dt, subpassos) (fisica-cpu.js:155-159): air labelling once, then `substeps`
  * substeps of dt/substeps.  Blocks until the device is done. */

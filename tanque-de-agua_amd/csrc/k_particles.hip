@@ -746,6 +746,26 @@ int count_live(tq_sim* s, int* n) {
   return 0;
 }
 
+// the status words of a non-blocking readback: the count is cleared and vMax copied (one thread);
+// k_count_live adds the live particles behind it
+__global__ __launch_bounds__(64) void k_status_begin(unsigned long long* __restrict__ st,
+                                                     const DevScalars* __restrict__ sc) {
+  if (threadIdx.x == 0) {
+    st[0] = 0ull;
+    st[1] = sc->vmax_bits;
+  }
+}
+
+int status_enqueue(tq_sim* s) {
+  hipLaunchKernelGGL(k_status_begin, dim3(1), dim3(64), 0, s->st, (unsigned long long*)s->status_dev,
+                     (const DevScalars*)s->sc);
+  if (s->n_upper > 0)
+    hipLaunchKernelGGL(k_count_live, dim3((s->n_upper + 255) / 256), dim3(256), 0, s->st, s->pos[s->cur], s->sc,
+                       (uint32_t*)s->status_dev);
+  TQ_LAUNCH_CHECK();
+  return 0;
+}
+
 // ================================================================== atualizarParedes (fisica-cpu.js:80-85)
 // Cena.celulaSolida (cena.js:119-127): the sum of the cell's 4 SDF samples, in binary64, < 0.
 __global__ __launch_bounds__(256) void k_solid(const float* __restrict__ sdf, int nx, int ny, uint8_t* __restrict__ solid) {

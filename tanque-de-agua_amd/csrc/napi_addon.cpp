@@ -423,6 +423,56 @@ napi_value m_step(napi_env env, napi_callback_info info) {
   return undefined(env);
 }
 
+// avancarAssincrono(dt, subpassos): the frame queued without waiting, its status readback behind it
+napi_value m_step_async(napi_env env, napi_callback_info info) {
+  napi_value argv[2], self;
+  if (!get_args(env, info, 2, argv, &self)) return nullptr;
+  Native* n = unwrap(env, self);
+  if (!n) return nullptr;
+  double dt, sub = 1;
+  if (!num(env, argv[0], &dt)) return nullptr;
+  if (!is_nullish(env, argv[1]) && !num(env, argv[1], &sub)) return nullptr;
+  throw_tq(env, tq_step_async(n->s, dt, (int)sub), "tq_step_async");
+  return undefined(env);
+}
+
+// pedirEstado(): a status readback queued behind the queued work
+napi_value m_status_request(napi_env env, napi_callback_info info) {
+  napi_value argv[1], self;
+  if (!get_args(env, info, 0, argv, &self)) return nullptr;
+  Native* n = unwrap(env, self);
+  if (!n) return nullptr;
+  throw_tq(env, tq_status_request(n->s), "tq_status_request");
+  return undefined(env);
+}
+
+// estado() -> null while the device has not reached the last request, else {n, vMax, msGpu}
+napi_value m_status_poll(napi_env env, napi_callback_info info) {
+  napi_value argv[1], self;
+  if (!get_args(env, info, 0, argv, &self)) return nullptr;
+  Native* n = unwrap(env, self);
+  if (!n) return nullptr;
+  int ready = 0, c = 0, has = 0;
+  double v = 0, ms = -1;
+  if (!throw_tq(env, tq_status_poll(n->s, &ready, &c, &v, &has, &ms), "tq_status_poll")) return nullptr;
+  napi_value out;
+  if (!ready) {
+    napi_get_null(env, &out);
+    return out;
+  }
+  napi_create_object(env, &out);
+  napi_set_named_property(env, out, "n", number(env, c));
+  napi_value vm;
+  if (has) vm = number(env, v);
+  else napi_get_null(env, &vm);
+  napi_set_named_property(env, out, "vMax", vm);
+  napi_value mv;
+  if (ms >= 0) mv = number(env, ms);
+  else napi_get_null(env, &mv);
+  napi_set_named_property(env, out, "msGpu", mv);
+  return out;
+}
+
 // subpasso(dt) (fisica-cpu.js:176-189)
 napi_value m_substep(napi_env env, napi_callback_info info) {
   napi_value argv[1], self;
@@ -659,6 +709,9 @@ napi_value init(napi_env env, napi_value exports) {
       {"aplicarFerramenta", nullptr, m_tool, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"emitirTorneira", nullptr, m_faucet, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"avancar", nullptr, m_step, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"avancarAssincrono", nullptr, m_step_async, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"pedirEstado", nullptr, m_status_request, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"estado", nullptr, m_status_poll, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"gravar", nullptr, m_tape_begin, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"definirTracos", nullptr, m_set_strokes, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"tracosVjp", nullptr, m_strokes_vjp, nullptr, nullptr, nullptr, napi_default, nullptr},

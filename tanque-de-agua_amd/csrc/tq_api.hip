@@ -185,6 +185,11 @@ static void free_all(tq_sim* s) {
   for (int b = 0; b < 2; b++) { f(s->zbuf[b]); f(s->pbuf[b]); }
   f(s->partials); f(s->partials_r); f(s->sc); f(s->scanBlocks);
   if (s->sc_host) (void)hipHostFree(s->sc_host);
+  if (s->status_dev) (void)hipFree(s->status_dev);
+  if (s->status_host) (void)hipHostFree(s->status_host);
+  if (s->ev_status) (void)hipEventDestroy(s->ev_status);
+  if (s->ev_frame0) (void)hipEventDestroy(s->ev_frame0);
+  if (s->ev_frame1) (void)hipEventDestroy(s->ev_frame1);
   solver_free(s);
   for (auto& e : s->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
   if (s->st2) (void)hipStreamDestroy(s->st2);
@@ -498,8 +503,64 @@ int tq_apply_tool(tq_sim* s, const tq_tool* t) {
   TQ_NOT_TAPING(s, "tq_apply_tool");
   TQ_CHECK_SIM(s);
   if (!t) return 0;
-  if (int e = apply_tool_dev(s, t)) return e;
-  return sync(s);
+  // queued behind the tank's work: the tool's parameters travel by value, nothing to wait for
+  return apply_tool_dev(s, t);
+}
+
+static int status_buffers(tq_sim* s) {
+  if (s->status_dev) return 0;
+  TQ_HIP(hipMalloc((void**)&s->status_dev, 2 * sizeof(uint64_t)));
+  TQ_HIP(hipHostMalloc((void**)&s->status_host, 2 * sizeof(uint64_t), hipHostMallocDefault));
+  TQ_HIP(hipEventCreate(&s->ev_status));
+  TQ_HIP(hipEventCreate(&s->ev_frame0));
+  TQ_HIP(hipEventCreate(&s->ev_frame1));
+  return 0;
+}
+
+int tq_status_request(tq_sim* s) {
+  TQ_CHECK_SIM(s);
+  if (int e = status_buffers(s)) return e;
+  if (int e = status_enqueue(s)) return e;
+  TQ_HIP(hipMemcpyAsync(s->status_host, s->status_dev, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s->st));
+  TQ_HIP(hipEventRecord(s->ev_status, s->st));
+  s->status_pending = true;
+  s->status_has = s->has_vmax;
+  return 0;
+}
+
+int tq_step_async(tq_sim* s, double dt, int substeps) {
+  TQ_NOT_TAPING(s, "tq_step_async");
+  TQ_CHECK_SIM(s);
+  if (int e = status_buffers(s)) return e;
+  TQ_HIP(hipEventRecord(s->ev_frame0, s->st));
+  if (int e = tq::step_enqueue(s, dt, substeps)) return e;
+  TQ_HIP(hipEventRecord(s->ev_frame1, s->st));
+  s->status_timed = true;
+  return tq_status_request(s);
+}
+
+int tq_status_poll(tq_sim* s, int* ready, int* n, double* vmax, int* has_vmax, double* gpu_ms) {
+  TQ_CHECK_SIM(s);
+  if (!ready) { set_error("tq_status_poll: null ready"); return TQ_E_ARG; }
+  *ready = 0;
+  if (!s->status_pending) return 0;
+  const hipError_t q = hipEventQuery(s->ev_status);
+  if (q == hipErrorNotReady) return 0;
+  if (q != hipSuccess) return hip_fail(q, "hipEventQuery", __FILE__, __LINE__);
+  *ready = 1;
+  if (n) *n = (int)(uint32_t)s->status_host[0];
+  if (vmax) {
+    const uint64_t b = s->status_host[1];
+    std::memcpy(vmax, &b, sizeof(double));
+  }
+  if (has_vmax) *has_vmax = s->status_has ? 1 : 0;
+  if (gpu_ms) {
+    float ms = -1.f;
+    // the last tq_step_async frame, complete once the request queued after it is
+    if (s->status_timed) TQ_HIP(hipEventElapsedTime(&ms, s->ev_frame0, s->ev_frame1));
+    *gpu_ms = ms;
+  }
+  return 0;
 }
 
 int tq_substep(tq_sim* s, double dt) {

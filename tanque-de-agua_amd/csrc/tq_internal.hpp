@@ -243,6 +243,16 @@ struct tq_sim {
   tq::SubRec* rec = nullptr;      // the substep record the substep's hooks fill (null: none)
   uint16_t* g2p_flags = nullptr;  // where G2P writes each particle's decision word (null: nowhere)
 
+  // non-blocking status readback (tq_step_async / tq_status_request / tq_status_poll)
+  uint64_t* status_dev = nullptr;   // [0] live particles (low 32 bits), [1] vMax (binary64 bits)
+  uint64_t* status_host = nullptr;  // pinned copy of status_dev
+  hipEvent_t ev_status = nullptr;   // recorded after the copy
+  hipEvent_t ev_frame0 = nullptr;   // recorded before the last frame tq_step_async queued
+  hipEvent_t ev_frame1 = nullptr;   // recorded after it
+  bool status_pending = false;      // a request was queued (ev_status belongs to it)
+  bool status_timed = false;        // a tq_step_async frame was queued before the last request
+  bool status_has = false;          // vMax had been measured when the request was queued
+
   // host-side state
   bool tipos_prontos = false;
   bool has_vmax = false;
@@ -289,6 +299,7 @@ int stream_wait(tq_sim* s);
 int faucet_dev(tq_sim* s, const tq_faucet* f, double vx, double vy, int n, int keep, uint32_t* seed);  // from s->stage
 int apply_tool_dev(tq_sim* s, const tq_tool* t);
 int count_live(tq_sim* s, int* n);
+int status_enqueue(tq_sim* s);  // live count and vMax into s->status_dev (queued, no wait)
 int solid_cells(tq_sim* s);
 // ---- launchers (k_grid.hip)
 int surface(tq_sim* s);

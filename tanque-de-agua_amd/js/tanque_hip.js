@@ -156,7 +156,23 @@ class SimulacaoHIP {
   destruir() { this._n.destruir(); }
 }
 
-/** MotorCPU's surface (js/motor.js:79-120) on SimulacaoHIP; drawing is the caller's. */
+/** Resolves once `pronto()` returns a value other than null, polling between event-loop turns
+ *  (the host thread stays free while the device works). */
+function aguardar(pronto) {
+  return new Promise((resolve, reject) => {
+    const volta = () => {
+      let v;
+      try { v = pronto(); } catch (e) { reject(e); return; }
+      if (v !== null) resolve(v); else setImmediate(volta);
+    };
+    volta();
+  });
+}
+
+/** MotorCPU's surface (js/motor.js:79-120) on SimulacaoHIP, with MotorGPU's timing (motor.js:
+ *  54-74): quadro queues the frame and returns, contagem/particulas/esperar resolve when the
+ *  device has reached them, vMax and custoPorSubpasso are those of the last frame whose status
+ *  has arrived.  Drawing is the caller's. */
 class MotorHIP {
   constructor(canvas, cena, opcoes = {}) {
     this.tipo = 'hip';
@@ -166,11 +182,14 @@ class MotorHIP {
     this.sim = new SimulacaoHIP(cena, opcoes);
     this.pendentes = [];
     this.custo = null;
+    this.vMaxMedido = null;
+    this.subpassosPedidos = 0;
   }
 
   get h() { return this.cena.h; }
   get maxParticulas() { return Math.ceil(this.sim.nx * this.sim.ny * PARTICULAS_MAX_POR_CELULA); }
-  get vMax() { const v = this.sim.vMax; return v === undefined ? null : v; }
+  /** Top speed of the last frame whose status has arrived (m/s), or null (MotorGPU.vMax). */
+  get vMax() { this._colher(); return this.vMaxMedido; }
   paredesMudaram() { this.sim.atualizarParedes(); }
   adicionar(lista, checar = true) { this.pendentes.push([lista, checar]); }
   limparAgua() { this.sim.limparAgua(); this.pendentes = []; }
@@ -178,82 +197,40 @@ class MotorHIP {
 
   quadro(dt, subpassos, ferramenta) {
     this.sim.aplicarFerramenta(ferramenta);
-    const teto = this.maxParticulas;
-    for (const [l, c] of this.pendentes) {
-      const cabe = Math.max(0, teto - this.sim.n) * CAMPOS;
-      if (cabe > 0) this.sim.adicionar(l.length > cabe ? l.subarray(0, cabe) : l, c);
-    }
+    // the library cuts each list to the room left under maxParticulas (motor.js:100-104)
+    for (const [l, c] of this.pendentes) this.sim.adicionar(l, c);
     this.pendentes = [];
     if (subpassos > 0) {
-      const t0 = process.hrtime.bigint();
-      this.sim.avancar(dt, subpassos);
-      const ms = Number(process.hrtime.bigint() - t0) / 1e6 / subpassos;
-      this.custo = this.custo === null ? ms : this.custo + (ms - this.custo) * 0.2;
+      this._colher();
+      this.sim._n.avancarAssincrono(dt, subpassos);
+      this.subpassosPedidos = subpassos;
     }
   }
 
-  get custoPorSubpasso() { return this.custo; }
-  async contagem() { return this.sim.n; }
-  async particulas() { return this.sim.particulas(); }
-  async esperar() { this.sim.esperar(); }
+  /** Takes the last status if it has arrived: vMax and, once per queued frame, its device time
+   *  per substep (an EMA as MotorCPU's, motor.js:108-110). */
+  _colher() {
+    const e = this.sim._n.estado();
+    if (e === null) return null;
+    this.vMaxMedido = e.vMax;
+    if (e.msGpu !== null && this.subpassosPedidos > 0) {
+      const ms = e.msGpu / this.subpassosPedidos;
+      this.custo = this.custo === null ? ms : this.custo + (ms - this.custo) * 0.2;
+      this.subpassosPedidos = 0;
+    }
+    return e;
+  }
+
+  get custoPorSubpasso() { this._colher(); return this.custo; }
+  /** The live particle count once the device has finished the queued frames. */
+  contagem() {
+    this.sim._n.pedirEstado();
+    return aguardar(() => { const e = this._colher(); return e === null ? null : e.n; });
+  }
+  async particulas() { await this.esperar(); return this.sim.particulas(); }
+  /** Resolves when the device has finished what was queued (motor.js:72). */
+  async esperar() { await this.contagem(); }
   destruir() { this.sim.destruir(); }
 }
 
-/** The app's per-frame substep plan (principal.js:528-575): the step bound from the measured
- *  top speed (raised while the hand or the faucet throws water), the substep count, and the cap
- *  from the measured cost per substep (an EMA of wall time, principal.js:545-550) so a busy
- *  device slows the water instead of the screen.  A device motor uses SUB_MAX_GPU. */
-class Controlador {
-  constructor(tipo = 'gpu') {
-    this.tetoFixo = tipo === 'gpu' ? 24 : 6;   // SUB_MAX_GPU, SUB_MAX_CPU
-    this.orcamentoMs = 16;                      // ORCAMENTO_MS
-    this.custoParede = null;
-  }
-  /** registrarCusto (principal.js:545-550). */
-  registrarCusto(ms, sub) {
-    const custo = ms / sub;
-    if (sub <= 0 || !Number.isFinite(custo)) return;
-    this.custoParede = this.custoParede == null ? custo : this.custoParede + (custo - this.custoParede) * 0.15;
-  }
-  /** {dt, sub} for a frame of `real` seconds (already limited to 1/20 s by the caller), as
-   *  quadro() plans it (principal.js:556-571). */
-  planejar(real, cena, motor, ponteiro = null, ferramenta = null, fator = 1) {
-    let dt = real * fator;
-    let vMax = motor.vMax;
-    if (vMax != null && ponteiro && ponteiro.apertado && (ferramenta === 'mao' || ferramenta === 'torneira')) {
-      vMax = Math.max(vMax, Math.min(6, Math.hypot(ponteiro.vx, ponteiro.vy)) + 0.7);
-    }
-    const dtMax = passoMaximo(cena.h, cena.altura, vMax, dt);
-    let sub = dt > 0 ? Math.ceil(dt / dtMax - 1e-6) : 0;
-    const custo = motor.custoPorSubpasso != null ? motor.custoPorSubpasso : this.custoParede;
-    const teto = custo > 0 ? Math.max(2, Math.min(this.tetoFixo, Math.floor(this.orcamentoMs / custo))) : this.tetoFixo;
-    if (sub > teto) { sub = teto; dt = sub * dtMax; }
-    return { dt, sub };
-  }
-}
-
-/** reamostrar (principal.js:93-112): particles of a tank resized from cell hAntigo to hNovo,
- *  (hAntigo/hNovo)^2 copies each on average, extra copies jittered by half a cell, those
- *  within the frame of the new tank dropped; `aleatorio` replaces Math.random. */
-function reamostrar(lista, hAntigo, hNovo, largura, altura, aleatorio = Math.random) {
-  const razao = (hAntigo / hNovo) ** 2;
-  const n = lista.length / CAMPOS;
-  const m = 2 * hNovo;  // MOLDURA cells
-  const saida = [];
-  for (let k = 0; k < n; k++) {
-    const o = k * CAMPOS;
-    const copias = Math.floor(razao) + (aleatorio() < razao % 1 ? 1 : 0);
-    for (let c = 0; c < copias; c++) {
-      const p = lista.slice(o, o + CAMPOS);
-      if (c > 0) { p[0] += (aleatorio() - 0.5) * hNovo * 0.5; p[1] += (aleatorio() - 0.5) * hNovo * 0.5; }
-      if (p[0] <= m || p[1] <= m || p[0] >= largura - m || p[1] >= altura - m) continue;
-      saida.push(p);
-    }
-  }
-  const junta = new Float32Array(saida.length * CAMPOS);
-  saida.forEach((p, k) => junta.set(p, k * CAMPOS));
-  return junta;
-}
-
-module.exports = { SimulacaoHIP, MotorHIP, Controlador, reamostrar, Cena, Semente, semearRetangulo, passoMaximo, CAMPOS,
-                   CAMPO };
+module.exports = { SimulacaoHIP, MotorHIP, Cena, Semente, semearRetangulo, passoMaximo, CAMPOS, CAMPO };

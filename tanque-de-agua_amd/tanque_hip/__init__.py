@@ -101,6 +101,10 @@ SIGNATURES = {
                                     C.c_void_p, C.c_int]),
     "tq_water_fraction_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                             C.POINTER(C.c_double)]),
+    "tq_step_async": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "tq_status_request": (C.c_int, [C.c_void_p]),
+    "tq_status_poll": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "tq_tape_records": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "tq_set_strokes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "tq_read_sdf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -286,6 +290,23 @@ class SimulacaoHIP:
 
     def avancar(self, dt, subpassos=1):
         _chk(load().tq_step(self._h, dt, subpassos))
+
+    def avancar_assincrono(self, dt, subpassos=1):
+        """The frame queued without waiting, a status readback behind it (tq_step_async)."""
+        _chk(load().tq_step_async(self._h, dt, subpassos))
+
+    def pedir_estado(self):
+        _chk(load().tq_status_request(self._h))
+
+    def estado(self):
+        """None while the device has not reached the last status request, else a dict with n,
+        vMax (None before the first G2P) and ms_gpu (the last queued frame's device time)."""
+        ready, n, has = C.c_int(), C.c_int(), C.c_int()
+        v, ms = C.c_double(), C.c_double()
+        _chk(load().tq_status_poll(self._h, C.byref(ready), C.byref(n), C.byref(v), C.byref(has), C.byref(ms)))
+        if not ready.value:
+            return None
+        return {"n": n.value, "vMax": v.value if has.value else None, "ms_gpu": ms.value if ms.value >= 0 else None}
 
     # ---- walls built on the device
     def definir_tracos(self, tracos):

@@ -16,7 +16,7 @@ const semente = new tq.Semente(9);
 const agua = tq.semearRetangulo(m, m, 0.3, 0.42, h, [0, 0, 0], semente);
 const out = { n_seeded: agua.length / tq.CAMPOS, passo: tq.passoMaximo(h, ny * h, null, 1 / 60),
               semente: semente.estado };
-function main() {
+async function main() {
   if (hostOnly) {
     out.particulas = Buffer.from(agua.buffer).toString('base64');
     return;
@@ -27,9 +27,13 @@ function main() {
   const quadro = 1 / 60;
   const quadros = Math.floor(segundos / quadro + 0.5);
   for (let f = 0; f < quadros; f++) {
+    // the plan reads the previous frame's top speed: wait for its status (the app plans with
+    // whatever arrived, as with MotorGPU)
+    await motor.esperar();
     const sub = Math.ceil(quadro / tq.passoMaximo(h, cena.altura, motor.vMax, quadro));
     motor.quadro(quadro, sub, null);
   }
+  await motor.esperar();
   const P = motor.sim.particulas();
   out.n = motor.sim.n;
   out.vMax = motor.vMax;
@@ -45,5 +49,4 @@ function main() {
   }
   motor.destruir();
 }
-main();
-process.stdout.write(JSON.stringify(out) + '\n');
+main().then(() => process.stdout.write(JSON.stringify(out) + '\n'), (e) => { console.error(e); process.exit(1); });

@@ -123,46 +123,25 @@ def test_node_backward_matches_ctypes_backward():
     assert np.linalg.norm(gs - out["g_sdf"]) <= 1e-5 * np.linalg.norm(out["g_sdf"]) + 1e-30
 
 
-def lcg_advance(s, k):
-    for _ in range(k):
-        s = (s * 1664525 + 1013904223) & 0xFFFFFFFF
-    return s
-
-
 @needs_node
-def test_app_substep_plan_and_resampling():
-    """Controlador (principal.js:528-575) and reamostrar (principal.js:93-112) in the JS host:
-    the plan against passoMaximo computed by the library, the budget cap, the cost average;
-    resampling at ratio 1 and 4 (exact copies and draws) and 2.25 (mean count)."""
+@pytest.mark.gpu
+def test_motor_hip_asynchronous_frames():
+    """MotorHIP as MotorGPU (motor.js:54-74): 30 frames queued without waiting, then contagem()
+    resolves to the live count and vMax/custoPorSubpasso are those of the last frame, equal to
+    the blocking reads of the same state; lists beyond maxParticulas are cut to the room left
+    (motor.js:100-104), so the tank fills exactly to the cap and stays there."""
     import tanque_hip as tq
     tq.load()
-    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "check_app.js")], capture_output=True, text=True,
-                       timeout=120)
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "check_motor.js")], capture_output=True, text=True,
+                       timeout=300)
     assert r.returncode == 0, r.stderr
     d = json.loads(r.stdout)
-    h, H = 0.01, 0.8
-    p = tq.passo_maximo(h, H, 2.0, 1 / 20)
-    assert d["passo"] == p
-    assert d["semCusto"] == {"dt": 0.05, "sub": math.ceil(0.05 / p - 1e-6)}
-    assert d["comCusto"]["sub"] == 4 and d["comCusto"]["dt"] == 4 * p  # floor(16 ms / 4 ms)
-    assert d["cpu"]["sub"] == 6 and d["cpu"]["dt"] == 6 * p            # SUB_MAX_CPU
-    pm = tq.passo_maximo(h, H, 5.7, 1 / 60)                           # hand at 5 m/s: 5 + 0.7
-    assert d["passoMao"] == pm and d["mao"]["sub"] == math.ceil((1 / 60) / pm - 1e-6)
-    assert d["semVmax"]["sub"] == math.ceil((1 / 60) / tq.passo_maximo(h, H, None, 1 / 60) - 1e-6)
-    assert d["custoParede"] == 5 + (10 - 5) * 0.15
-    # ratio 1: the particle inside the new frame (x = 0.015 <= 2 h) goes, the others stay as they
-    # were; one draw per particle
-    P = np.asarray(d["igual"], np.float32).reshape(-1, 12)
-    assert len(P) == 2 and list(P[:, 0]) == [np.float32(0.5), np.float32(0.7)] and list(P[:, 2]) == [1, 3]
-    assert d["igualSemente"] == lcg_advance(7, 3)
-    # ratio 4: four copies each, the first unchanged, the others within a quarter cell; 1 + 3*2
-    # draws per particle
-    P = np.asarray(d["quatro"], np.float32).reshape(-1, 12)
-    assert len(P) == 8
-    assert P[0, 0] == np.float32(0.5) and P[4, 0] == np.float32(0.7)
-    for a, b in ((0, 4), (4, 8)):
-        assert np.all(np.abs(P[a:b, 0] - P[a, 0]) <= 0.0025 + 1e-7) and np.all(np.abs(P[a:b, 1] - P[a, 1]) <= 0.0025 + 1e-7)
-        assert np.all(P[a:b, 2] == P[a, 2])
-    assert d["quatroSemente"] == lcg_advance(7, 21)
-    # ratio 2.25: 2 or 3 copies, 9000 on average for 4000 particles (sd 27)
-    assert abs(d["fracionario"] - 9000) < 5 * 27
+    print("\n  30 frames queued in %.1f ms host time, count back after %.1f ms (ready at once: %s); n %d, vMax %.4f, "
+          "%.4f ms/substep; cap %d -> %d, %d" % (d["msFila"], d["msAteContagem"], d["prontoLogo"], d["n"], d["vMax"],
+                                                d["custo"], d["teto"], d["nNoTeto"], d["nDepois"]))
+    assert d["vMaxAntes"] is None
+    assert d["n"] == d["nBloqueante"] > 0
+    assert d["vMax"] == d["vMaxBloqueante"] and d["vMax"] > 0
+    assert d["custo"] is not None and 0 < d["custo"] < 1000
+    assert d["teto"] == math.ceil(240 * 135 * 4.6)
+    assert d["nNoTeto"] == d["teto"] == d["nDepois"] == d["nParticulas"]
