@@ -89,6 +89,23 @@ def run_gpu(tq, work, precision, warmup, steps, schedule=None, time_kernel=None,
     return res
 
 
+def run_autotune(tq, work, precision, frames=8, iterations=10):
+    """tq_autotune on the workload's pressure system after `frames` frames: the chosen launch
+    shapes (cached by tq_create for this grid, so the runs that follow use them) and the solve
+    time of the defaults and of the choice."""
+    from tanque_hip.scenes import passo_frame
+    cena = work.cena()
+    sim = tq.SimulacaoHIP(cena, precision=precision, tol=1e-6, iteracoes=200)
+    sim.adicionar(work.particles, False)
+    for _ in range(frames):
+        sim.avancar(work.dt_frame, passo_frame(work.h, cena.altura, sim.vMax, work.dt_frame))
+    r = sim.autotune(iterations)
+    sim.close()
+    r["gain"] = round(r["default_us"] / r["tuned_us"], 4) if r["tuned_us"] > 0 else None
+    r["iterations_per_trial_solve"] = iterations
+    return r
+
+
 def run_alto(tq, work, precision, tol=1e-6, maxit=200, schedule=None):
     """The Alto config end to end: each frame the faucet emits (while on) and the frame's
     substeps run, as the app does (principal.js:301-321, motor.js:98-107)."""
@@ -274,7 +291,7 @@ def adjoint_window(tq, W=5, fd_h=1e-3, box=FUNNEL_BOX, descent_steps=4, step=0.0
             "descent": {"step_m_per_s": step, "loss_start": r0["loss"], "steps": steps}}
 
 
-def run_size_sweep(tq, sizes=((896, 504), (1792, 1008), (4096, 2304)), warmup=2, frames=3):
+def run_size_sweep(tq, sizes=((896, 504), (1792, 1008), (4096, 2304)), warmup=2, frames=3, autotune=True):
     """The hourglass at growing sizes (past the 256 MB last-level cache at 4096x2304): substeps/s
     in FP32 and in the mixed mode on the same schedule, and, from a run with every kernel timed by
     HIP events on the library stream, the five kernels with the most time and their algorithmic
@@ -283,6 +300,7 @@ def run_size_sweep(tq, sizes=((896, 504), (1792, 1008), (4096, 2304)), warmup=2,
     out = []
     for nx, ny in sizes:
         w = scenes.ultra_hourglass() if (nx, ny) == (896, 504) else scenes.scaled_hourglass(nx, ny)
+        tune = run_autotune(tq, w, tq.PREC_FP32, frames=2) if autotune else None
         r32 = run_gpu(tq, w, tq.PREC_FP32, warmup, frames)
         timed = run_gpu(tq, w, tq.PREC_FP32, warmup, frames, schedule=r32["schedule"], time_kernel="*")
         rmx = run_gpu(tq, w, tq.PREC_MIXED, warmup, frames, schedule=r32["schedule"])
@@ -290,7 +308,7 @@ def run_size_sweep(tq, sizes=((896, 504), (1792, 1008), (4096, 2304)), warmup=2,
                "substeps": r32["substeps"], "fp32_substeps_per_s": round(r32["substeps_per_s"], 2),
                "fp32_pcg_iters_per_solve": round(r32["pcg_iters_per_solve"], 2),
                "mixed_substeps_per_s": round(rmx["substeps_per_s"], 2),
-               "mixed_pcg_iters_per_solve": round(rmx["pcg_iters_per_solve"], 2)}
+               "mixed_pcg_iters_per_solve": round(rmx["pcg_iters_per_solve"], 2), "autotune": tune}
         tim = timed["timing"]
         total = sum(v["ms"] for v in tim.values())
         top = []
@@ -443,6 +461,7 @@ def main():
     ap.add_argument("--no-alto", action="store_true", help="skip the Alto (cup + faucet) runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (64 tanks) runs")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (funnel) runs")
+    ap.add_argument("--no-autotune", action="store_true", help="run with the built-in launch shapes")
     ap.add_argument("--size-sweep", action="store_true",
                     help="only the hourglass at 896x504, 1792x1008 and 4096x2304: FP32 and mixed on one schedule, "
                          "and the five kernels with the most time (every kernel timed by HIP events); prints its JSON")
@@ -472,11 +491,17 @@ def main():
     if args.size_sweep:
         sizes = tuple(tuple(int(v) for v in g.lower().split("x")) for g in args.sweep_grid) if args.sweep_grid \
             else ((896, 504), (1792, 1008), (4096, 2304))
-        print(json.dumps({"size_sweep": run_size_sweep(tq, sizes=sizes)}), flush=True)
+        print(json.dumps({"size_sweep": run_size_sweep(tq, sizes=sizes, autotune=not args.no_autotune)}), flush=True)
         return
     work = scenes.ultra_hourglass() if (args.nx, args.ny) == (896, 504) else scenes.scaled_hourglass(args.nx, args.ny)
     log("workload %s: %dx%d cells, %d particles seeded" % (work.name, work.nx, work.ny, len(work.particles) // 12))
 
+    tune = None
+    if not args.no_autotune:
+        tune = {"fp32": run_autotune(tq, work, tq.PREC_FP32)}
+        if not args.no_mixed:
+            tune["mixed"] = run_autotune(tq, work, tq.PREC_MIXED)
+        log("autotune: %s" % tune)
     dom, probe = dominant_kernel(tq, work, tq.PREC_FP32)
     log("dominant kernel (fp32): %s (%.1f%% of device time in the probe)" % (
         dom, 100 * probe[dom]["ms"] / sum(v["ms"] for v in probe.values())))
@@ -581,6 +606,7 @@ def main():
         "mixed_substeps_per_s": round(mixed["substeps_per_s"], 2) if mixed else None,
         "mixed_pcg_iters_per_solve": round(mixed["pcg_iters_per_solve"], 3) if mixed else None,
         "mixed_over_fp32": round(mixed["substeps_per_s"] / fp32["substeps_per_s"], 4) if mixed else None,
+        "autotune": tune,
         "graph_replay": graph or None,
         "mixed_variants": variants or None,
         "alto": alto,

@@ -257,14 +257,34 @@ typedef struct tq_kernel_time {
 int tq_timing_enable(tq_sim* s, int enable, const char* only);
 int tq_timing_read(tq_sim* s, tq_kernel_time* out, int max, int* n);
 
-/* Launch shapes of the solver's level-0 kernels, by name: "spmv_grid" (most blocks of the
- * spmv kernel, default 1024) and "update_grid" (most blocks of the PCG update / V-cycle input
- * kernel, default 1280); 0 restores the default.  Grids larger than the cap walk their cells
- * with fewer blocks.  The cap decides how the dot products' partial sums are grouped, so results
+/* Launch shapes of the solver, by name: "spmv_grid" (most blocks of the spmv kernel, default
+ * 1024), "update_grid" (most blocks of the PCG update / V-cycle input kernel, default 1280),
+ * "split_cells" (multigrid levels above this many cells run each pass as its own kernel instead
+ * of the fused down/up kernels, default 2^20) and "pass_fold_grid" (most blocks of a split level
+ * 0's folding pass-B kernel, default 2048); 0 restores the default.  Grids larger than a cap
+ * walk their cells with fewer blocks.  The cap decides how the dot products' partial sums are grouped, so results
  * are bit-reproducible for a fixed setting and can differ in the last bits between settings.
  * No counterpart in the reference (launch tuning). */
 int tq_set_launch(tq_sim* s, const char* name, int value);
 int tq_get_launch(tq_sim* s, const char* name, int* value);
+
+/* The solver's autotuner (no counterpart in the reference, whose WebGPU engine fixes its dispatch
+ * shapes, js/gpu/multigrid.js:283-287).  Times pressure solves of `iterations` PCG iterations on
+ * the tank's current pressure system (call it after a few frames) for each candidate of, in turn,
+ * the dense/one-workgroup coarse V-cycle (mg_dense), the levels run one pass per kernel
+ * (tq_set_launch "split_cells": levels above that many cells; default 2^20), "spmv_grid",
+ * "update_grid" and "pass_fold_grid" (most blocks of a split level 0's folding pass-B kernel,
+ * default 2048), keeping a change only when it is faster by more than 1 %; the state a solve
+ * changes is restored after every trial.  The choice is applied to `s` and cached for the process
+ * by (nx, ny, precision): tq_create applies a cached choice.  `out` (may be NULL) receives it with
+ * the mean solve time of the defaults and of the choice. */
+typedef struct tq_tune_result {
+  int spmv_grid, update_grid, split_cells, pass_fold_grid, mg_dense;  /* 0: the built-in default */
+  double default_us;  /* one tuning solve with the defaults (device time, us) */
+  double tuned_us;    /* with the chosen shapes */
+  int trials;         /* settings timed */
+} tq_tune_result;
+int tq_autotune(tq_sim* s, int iterations, tq_tune_result* out);
 
 /* The render hand-off: the water layer of the reference's 2D renderer (js/render-2d.js:110-162)
  * computed on the device from the particle buffer, at 3 pixels per cell (w = 3 nx, h = 3 ny,

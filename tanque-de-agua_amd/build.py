@@ -17,7 +17,7 @@ LIB = os.path.join(LIB_DIR, "libtanque.so")
 NODE_ADDON = os.path.join(LIB_DIR, "tanque_node.node")
 
 HIP_SOURCES = ["k_particles.hip", "k_grid.hip", "k_air.hip", "k_solver.hip", "tq_api.hip", "tq_batch.hip",
-               "tq_adjoint.hip", "k_scene.hip", "k_render.hip"]
+               "tq_adjoint.hip", "k_scene.hip", "k_render.hip", "tq_tune.hip"]
 CPP_SOURCES = ["scene.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed-FP32 instructions are disabled for all device code (known-bad on these boxes).

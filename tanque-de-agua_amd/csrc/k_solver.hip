@@ -1170,6 +1170,10 @@ __global__ __launch_bounds__(256) void k_pass_z(const float* __restrict__ src, c
 // restriction + the coarse pass A; prolongation + pass C, then pass D.
 constexpr int SPLIT_CELLS = 1 << 20;
 constexpr int PASS_FOLD_GRID = 2048;  // 8 blocks per CU at k_pass's register count
+static int split_cells(const tq_sim* s) { return s->launch_split_cells > 0 ? s->launch_split_cells : SPLIT_CELLS; }
+static int pass_fold_grid(const tq_sim* s) {
+  return s->launch_pass_fold_grid > 0 ? s->launch_pass_fold_grid : PASS_FOLD_GRID;
+}
 
 // one smoothing pass of level l (offset desl, colour order from C0); on level 0 of an iteration
 // it first does the norm and stop test (fold_update)
@@ -2437,7 +2441,7 @@ static void launch_bra(tq_sim* s, int l) {
   const RedSrc rsu = fold ? red_src(s, PS_RR, 1, -1, s->nb_upd) : RedSrc{s->partials + ps_off(PS_RR), nullptr, 0};
   const double* rr = rsu.part;
   // x(l) and b(l) and mask(l) in, x_B(l) out; coarse mask in, b (and x) out
-  if (F.n > SPLIT_CELLS) {  // pass B, then restriction + the coarse pass A
+  if (F.n > split_cells(s)) {  // pass B, then restriction + the coarse pass A
     static const char* const pnames[12] = {"mg_pass_b_l0", "mg_pass_b_l1", "mg_pass_b_l2", "mg_pass_b_l3",
                                            "mg_pass_b_l4", "mg_pass_b_l5", "mg_pass_b_l6", "mg_pass_b_l7",
                                            "mg_pass_b_l8", "mg_pass_b_l9", "mg_pass_b_l10", "mg_pass_b_l11"};
@@ -2451,7 +2455,7 @@ static void launch_bra(tq_sim* s, int l) {
       const dim3 tg = tile_grid(F.nx, F.ny, 8);
       const int nvb = (int)(tg.x * tg.y);
       // the folding launch (level 0 of an iteration) walks several groups per block after one fold
-      const int nbk = fold ? std::min(nvb, PASS_FOLD_GRID) : nvb;
+      const int nbk = fold ? std::min(nvb, pass_fold_grid(s)) : nvb;
       hipLaunchKernelGGL((k_pass<SX, SB, 0>), dim3(nbk), dim3(256), 0, s->st, (const SX*)F.x, (SX*)F.tmp,
                          (const SB*)F.b, F.mask, F.nx, F.ny, 8, ntx8, nty8, (int)tg.x, nvb, rr, rsu.nb, it_par(s),
                          fold, s->sc);
@@ -2515,7 +2519,7 @@ static int launch_pcd(tq_sim* s, int l) {
   const double fine = zd ? (vbytes(F.hx) + vbytes(F.hb) + 1 + 1 + 4 + 4 + 4) : (2 * vbytes(F.hx) + vbytes(F.hb) + 1);
   static const char* const names[12] = {"mg_up_z_l0", "mg_up_l1", "mg_up_l2", "mg_up_l3", "mg_up_l4", "mg_up_l5",
                                         "mg_up_l6", "mg_up_l7", "mg_up_l8", "mg_up_l9", "mg_up_l10", "mg_up_l11"};
-  if (!zd && F.n > SPLIT_CELLS) {  // prolongation + pass C (tmp -> x), then pass D (x -> tmp)
+  if (!zd && F.n > split_cells(s)) {  // prolongation + pass C (tmp -> x), then pass D (x -> tmp)
     static const char* const cnames[12] = {"mg_prolong_c_l0", "mg_prolong_c_l1", "mg_prolong_c_l2",
                                            "mg_prolong_c_l3", "mg_prolong_c_l4", "mg_prolong_c_l5",
                                            "mg_prolong_c_l6", "mg_prolong_c_l7", "mg_prolong_c_l8",

@@ -440,6 +440,7 @@ int tq_create(const tq_desc* d, tq_sim** out) {
     return TQ_E_HIP;
   }
   s->tipos_prontos = false;
+  tune_apply_cached(s);
   *out = s;
   return 0;
 }
@@ -820,6 +821,8 @@ static int* launch_field(tq_sim* s, const char* name) {
   if (!name) return nullptr;
   if (!std::strcmp(name, "spmv_grid")) return &s->launch_spmv_grid;
   if (!std::strcmp(name, "update_grid")) return &s->launch_update_grid;
+  if (!std::strcmp(name, "split_cells")) return &s->launch_split_cells;
+  if (!std::strcmp(name, "pass_fold_grid")) return &s->launch_pass_fold_grid;
   return nullptr;
 }
 
@@ -827,7 +830,8 @@ int tq_set_launch(tq_sim* s, const char* name, int value) {
   TQ_CHECK_SIM(s);
   int* f = launch_field(s, name);
   if (!f) { set_error("tq_set_launch: unknown launch parameter"); return TQ_E_ARG; }
-  if (value < 0 || value > MAX_RED_BLOCKS) { set_error("tq_set_launch: value out of range"); return TQ_E_ARG; }
+  const int top = f == &s->launch_split_cells ? (1 << 27) : MAX_RED_BLOCKS;
+  if (value < 0 || value > top) { set_error("tq_set_launch: value out of range"); return TQ_E_ARG; }
   if (*f != value) {
     *f = value;
     solver_launch_changed(s);

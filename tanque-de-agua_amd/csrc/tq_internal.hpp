@@ -217,6 +217,8 @@ struct tq_sim {
   int nb_spmv = 0, nb_upd = 0, nb_z = 0;  // partial counts of the last spmv, update and z launches
   int launch_spmv_grid = 0;     // most blocks of spmv (0: the built-in cap)
   int launch_update_grid = 0;   // most blocks of the PCG update / level-0 input kernel (0: the built-in cap)
+  int launch_split_cells = 0;   // levels above this many cells run each multigrid pass as its own kernel (0: 2^20)
+  int launch_pass_fold_grid = 0; // most blocks of a split level 0's folding pass-B kernel (0: the built-in cap)
   int* labelSlot = nullptr;     // bubble label -> reduction slot (-1: 128-bit fixed-point atomics)
   int lc = 0;                   // first multigrid level solved inside the one-workgroup kernel
   int mg_scaled = 0;            // this solve stores some multigrid level in FP16 (input scaled)
@@ -318,6 +320,7 @@ int solver_alloc(tq_sim* s);
 void solver_free(tq_sim* s);
 int pressure_solve(tq_sim* s);
 void solver_launch_changed(tq_sim* s);
+void tune_apply_cached(tq_sim* s);  // tq_tune.hip: a tq_autotune result cached for this grid
 void drop_second_stream(tq_sim* s);  // build the coarse matrix on the tank's own stream
 int true_residual(tq_sim* s, double* out);
 int debug_vcycle(tq_sim* s, const float* b, float* z);

@@ -37,6 +37,12 @@ class Desc(C.Structure):
                 ("mg_dense", C.c_int)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("spmv_grid", C.c_int), ("update_grid", C.c_int), ("split_cells", C.c_int),
+                ("pass_fold_grid", C.c_int), ("mg_dense", C.c_int), ("default_us", C.c_double),
+                ("tuned_us", C.c_double), ("trials", C.c_int)]
+
+
 class Tool(C.Structure):
     _fields_ = [("kind", C.c_int), ("x", C.c_double), ("y", C.c_double), ("raio", C.c_double),
                 ("vx", C.c_double), ("vy", C.c_double), ("absorcao", C.c_double * 3)]
@@ -101,6 +107,7 @@ SIGNATURES = {
                                     C.c_void_p, C.c_int]),
     "tq_water_fraction_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                             C.POINTER(C.c_double)]),
+    "tq_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(TuneResult)]),
     "tq_step_async": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "tq_status_request": (C.c_int, [C.c_void_p]),
     "tq_status_poll": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
@@ -290,6 +297,13 @@ class SimulacaoHIP:
 
     def avancar(self, dt, subpassos=1):
         _chk(load().tq_step(self._h, dt, subpassos))
+
+    def autotune(self, iterations=10):
+        """tq_autotune: the solver's launch shapes tuned on the current pressure system; returns
+        the choice and the solve times of the defaults and of the choice."""
+        r = TuneResult()
+        _chk(load().tq_autotune(self._h, iterations, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in TuneResult._fields_}
 
     def avancar_assincrono(self, dt, subpassos=1):
         """The frame queued without waiting, a status readback behind it (tq_step_async)."""

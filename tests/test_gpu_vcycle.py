@@ -75,3 +75,28 @@ def test_vcycle_matches_oracle(oracle, tq, name, build, frames, prec):
                 bad.append((dense, k, "nonzero z off the water cells"))
         g.close()
     assert not bad, bad
+
+
+@pytest.mark.parametrize("name,build,frames", [
+    ("dam256", lambda o: dam(o, nx=256, ny=144, h=1.0 / 144, seed=0), 4),
+    ("ampulheta", lambda o: ampulheta(o), 12),
+])
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+def test_split_levels_match_oracle(oracle, tq, name, build, frames, prec):
+    """Every tiled level run one pass per kernel (tq_set_launch "split_cells" = 1, a shape the
+    autotuner may choose) gives the oracle's V-cycle as the fused kernels do."""
+    osim = state(oracle, build, frames)
+    water = osim.read(oracle.F_TIPO) == AGUA
+    p = {"fp32": tq.PREC_FP32, "mixed": tq.PREC_MIXED}[prec]
+    g = hip_sim_like(tq, osim, precision=p)
+    clone_state(osim, g)
+    g.set_launch("split_cells", 1)
+    rng = np.random.default_rng(11)
+    for k in range(3):
+        b = np.where(water, rng.standard_normal(water.size), 0.0).astype(np.float32)
+        zg = g.vcycle(b)
+        zo = osim.vcycle(b.astype(np.float64))
+        e = rel_err(zg[water], zo[water])
+        log("\n  [%s %s split] vector %d: z vs oracle %.2e" % (name, prec, k, e))
+        assert e <= RTOL[prec]
+    g.close()
