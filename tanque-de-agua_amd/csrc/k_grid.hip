@@ -27,6 +27,7 @@ __global__ __launch_bounds__(SF_THREADS) void k_surface(const uint8_t* __restric
   int bxl, byl;
   xcd_block2(&bxl, &byl);
   const int I0 = bxl * SF_T - SF_H, J0 = byl * SF_T - SF_H;
+  int solid_here = 0, fluid_here = 0;  // solid / filled cells in this thread's part of the region
   // this thread's region cells, fixed for every pass
   int lxk[SF_K], lyk[SF_K];
 #pragma unroll
@@ -45,41 +46,48 @@ __global__ __launch_bounds__(SF_THREADS) void k_surface(const uint8_t* __restric
       const uint8_t tt = tipo[c];
       ty[e] = tt;
       f[e] = tt == SOLIDO ? 0.f : (float)fmin(1.0, (double)dens[c] / PPC);
+      solid_here |= tt == SOLIDO;
+      fluid_here |= tt != SOLIDO;
     } else {
       ty[e] = 255;
       f[e] = 0.f;
     }
   }
-  __syncthreads();
+  // The wall extension fills solid cells (cells outside the grid are never filled) from filled
+  // neighbours: a region without solid cells, or without a filled one, is left as it is by all its
+  // layers, which are skipped (the tank's interior, the inside of thick walls).
+  const bool walls = (__syncthreads_or(solid_here) != 0) && (__syncthreads_or(fluid_here) != 0);
   for (int passada = 0; passada <= 2; passada++) {
-    // #estenderNaParede: 3 layers of "unfilled cell := mean of filled 4-neighbours"
-#pragma unroll
-    for (int k = 0; k < SF_K; k++) {
-      const int e = tid + SF_THREADS * k;
-      if (e < SF_N) fl[e] = (ty[e] != SOLIDO && ty[e] != 255) ? 1 : 0;
-    }
-    __syncthreads();
-    for (int camada = 0; camada < 3; camada++) {
+    if (walls) {
+      // #estenderNaParede: 3 layers of "unfilled cell := mean of filled 4-neighbours"
 #pragma unroll
       for (int k = 0; k < SF_K; k++) {
         const int e = tid + SF_THREADS * k;
-        if (e >= SF_N || fl[e] || ty[e] == 255) continue;
-        const int lx = lxk[k], ly = lyk[k], gi = I0 + lx, gj = J0 + ly;
-        double sum = 0;
-        int n = 0;
-        if (gi > 0 && lx > 0 && fl[e - 1] == 1) { sum += (double)f[e - 1]; n++; }
-        if (gi < nx - 1 && lx < SF_R - 1 && fl[e + 1] == 1) { sum += (double)f[e + 1]; n++; }
-        if (gj > 0 && ly > 0 && fl[e - SF_R] == 1) { sum += (double)f[e - SF_R]; n++; }
-        if (gj < ny - 1 && ly < SF_R - 1 && fl[e + SF_R] == 1) { sum += (double)f[e + SF_R]; n++; }
-        if (n) { t[e] = (float)(sum / n); fl[e] = 2; }
+        if (e < SF_N) fl[e] = (ty[e] != SOLIDO && ty[e] != 255) ? 1 : 0;
       }
       __syncthreads();
+      for (int camada = 0; camada < 3; camada++) {
 #pragma unroll
-      for (int k = 0; k < SF_K; k++) {
-        const int e = tid + SF_THREADS * k;
-        if (e < SF_N && fl[e] == 2) { f[e] = t[e]; fl[e] = 1; }
+        for (int k = 0; k < SF_K; k++) {
+          const int e = tid + SF_THREADS * k;
+          if (e >= SF_N || fl[e] || ty[e] == 255) continue;
+          const int lx = lxk[k], ly = lyk[k], gi = I0 + lx, gj = J0 + ly;
+          double sum = 0;
+          int n = 0;
+          if (gi > 0 && lx > 0 && fl[e - 1] == 1) { sum += (double)f[e - 1]; n++; }
+          if (gi < nx - 1 && lx < SF_R - 1 && fl[e + 1] == 1) { sum += (double)f[e + 1]; n++; }
+          if (gj > 0 && ly > 0 && fl[e - SF_R] == 1) { sum += (double)f[e - SF_R]; n++; }
+          if (gj < ny - 1 && ly < SF_R - 1 && fl[e + SF_R] == 1) { sum += (double)f[e + SF_R]; n++; }
+          if (n) { t[e] = (float)(sum / n); fl[e] = 2; }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SF_K; k++) {
+          const int e = tid + SF_THREADS * k;
+          if (e < SF_N && fl[e] == 2) { f[e] = t[e]; fl[e] = 1; }
+        }
+        __syncthreads();
       }
-      __syncthreads();
     }
     if (passada == 2) break;
     // separable (1 2 1) blur

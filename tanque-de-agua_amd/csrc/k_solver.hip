@@ -2012,43 +2012,52 @@ __global__ __launch_bounds__(256) void k_vcycle_apply(const float* __restrict__ 
 
 // First kernel of iteration k: the z dots and beta (fold_zdots), then per cell p = z + beta p_old
 // (bubble cells: their new z and old p), Ap and the p.Ap partials (produto / aplicar,
-// pressao-cpu.js:84-93, 257-261); SPMV_CELLS cells per thread so the partials stay few.
-// Large grids launch at most SPMV_GRID blocks (the ones that fit on the chip at once) and each
-// block walks its chunks after one fold: the fold's chain of dependent loads and barriers is
-// paid once per block, not once per chunk.
-constexpr int SPMV_CELLS = 4;
+// pressao-cpu.js:84-93, 257-261).  A block owns a 64x16 tile of cells (4 per thread, one row
+// segment of 64 per wavefront) and stages the tile with its one-cell halo in LDS: mask, z and
+// p_old are read once per cell (1.16x the tile, not 5x), p is formed once per cell and the
+// stencil reads it from LDS.  Large grids launch at most SPMV_GRID blocks (the ones that fit on
+// the chip at once) and each block walks its tiles after one fold: the fold's chain of dependent
+// loads and barriers is paid once per block, not once per tile.
+constexpr int SPMV_TW = 64, SPMV_TH = 16;            // tile (cells)
+constexpr int SPMV_HW = SPMV_TW + 2, SPMV_HN = SPMV_HW * (SPMV_TH + 2);  // with its halo: 66 x 18
+constexpr int SPMV_LOADS = (SPMV_HN + 255) / 256;    // halo-tile cells per thread (5)
 constexpr int SPMV_GRID = 1024;  // 4 blocks per CU: the kernel is held to 128 VGPRs (4 waves per SIMD)
+static int spmv_tiles(int nx, int ny) { return ((nx + SPMV_TW - 1) / SPMV_TW) * ((ny + SPMV_TH - 1) / SPMV_TH); }
 __global__ __launch_bounds__(256, 4) void k_spmv(const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
                                               const float* __restrict__ z, const float* __restrict__ pold,
-                                              float* __restrict__ pnew, float* __restrict__ ap, int nx, int C,
+                                              float* __restrict__ pnew, float* __restrict__ ap, int nx, int ny,
                                               BubArgs B, const double* __restrict__ partZ,
                                               const double* __restrict__ bpartZ, int nbz, int par, int first,
                                               double* __restrict__ partA, double* __restrict__ bpartA,
                                               DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], zs[BSLOTS];
-  constexpr int CH = 256 * SPMV_CELLS;
-  const int nch = (C + CH - 1) / CH;
+  __shared__ float P[SPMV_HN];
+  __shared__ uint8_t M[SPMV_HN];
+  const int tid = threadIdx.x;
+  const int ntx = (nx + SPMV_TW - 1) / SPMV_TW, nch = ntx * ((ny + SPMV_TH - 1) / SPMV_TH);
   int ch = xcd_block(blockIdx.x, gridDim.x);
-  // every load of the cells first (clamped indices; masks decide what is used): they do not
-  // depend on beta, so their latency overlaps the fold's reductions
-  int nbr[SPMV_CELLS][5];
-  uint8_t mk[SPMV_CELLS][5];
-  float zv[SPMV_CELLS][5], pv[SPMV_CELLS][5];
+  // every load of the halo tile first: it does not depend on beta, so its latency overlaps the
+  // fold's reductions (cells outside the grid: mask 0, never a free neighbour)
+  int i0 = 0, j0 = 0;
+  int gidx[SPMV_LOADS];
+  uint8_t mk[SPMV_LOADS];
+  float zv[SPMV_LOADS], pv[SPMV_LOADS];
   auto load = [&](int chunk) {
+    const int ty = chunk / ntx, tx = chunk - ty * ntx;
+    i0 = tx * SPMV_TW;
+    j0 = ty * SPMV_TH;
 #pragma unroll
-    for (int q = 0; q < SPMV_CELLS; q++) {
-      const int c = chunk * CH + q * 256 + threadIdx.x;
-      const int cc = c < C ? c : C - 1;
-      const int nb0[5] = {cc, cc - 1, cc + 1, cc - nx, cc + nx};
-#pragma unroll
-      for (int k = 0; k < 5; k++) {
-        const int d = nb0[k] < 0 ? 0 : (nb0[k] >= C ? C - 1 : nb0[k]);
-        nbr[q][k] = d;
-        mk[q][k] = opMask[d];
-        zv[q][k] = z[d];
-        pv[q][k] = pold[d];
-      }
+    for (int q = 0; q < SPMV_LOADS; q++) {
+      const int e = tid + 256 * q;
+      const int hy = e / SPMV_HW, hx = e - hy * SPMV_HW;
+      const int i = i0 - 1 + hx, j = j0 - 1 + hy;
+      const bool in = e < SPMV_HN && i >= 0 && j >= 0 && i < nx && j < ny;
+      const int c = in ? i + j * nx : -1;
+      gidx[q] = c;
+      mk[q] = in ? opMask[c] : 0;
+      zv[q] = in ? z[c] : 0.f;
+      pv[q] = in ? pold[c] : 0.f;
     }
   };
   load(ch);
@@ -2070,22 +2079,37 @@ __global__ __launch_bounds__(256, 4) void k_spmv(const uint8_t* __restrict__ opM
   };
   double v[1] = {0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int lx = tid & (SPMV_TW - 1), w = tid >> 6;
   for (;;) {
+    __syncthreads();  // the previous tile's P and M are no longer read
 #pragma unroll
-    for (int q = 0; q < SPMV_CELLS; q++) {
-      const int c = ch * CH + q * 256 + threadIdx.x;
-      if (c < C) {
-        const uint8_t m = mk[q][0];
-        const float pc = peff(c, m, zv[q][0], pv[q][0]);
+    for (int q = 0; q < SPMV_LOADS; q++) {
+      const int e = tid + 256 * q;
+      if (e < SPMV_HN) {
+        P[e] = gidx[q] >= 0 ? peff(gidx[q], mk[q], zv[q], pv[q]) : 0.f;
+        M[e] = mk[q];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SPMV_TH / 4; r++) {
+      const int ly = w + 4 * r;
+      const int i = i0 + lx, j = j0 + ly;
+      if (i < nx && j < ny) {
+        const int c = i + j * nx;
+        const int e = (lx + 1) + (ly + 1) * SPMV_HW;
+        const uint8_t m = M[e];
+        const float pc = P[e];
         pnew[c] = pc;
         if (mask_type(m) == AGUA) {
+          const int ne[4] = {e - 1, e + 1, e - SPMV_HW, e + SPMV_HW};
+          const int nc[4] = {c - 1, c + 1, c - nx, c + nx};
           double s = 0;
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             if (!(m & (1 << k))) continue;
-            const uint8_t md = mk[q][k + 1];
-            s += (double)peff(nbr[q][k + 1], md, zv[q][k + 1], pv[q][k + 1]);
-            if (md & MK_BUBIN) bub_add(sl, B.labelSlot, accA, rot[nbr[q][k + 1]], -(double)pc);
+            s += (double)P[ne[k]];
+            if (M[ne[k]] & MK_BUBIN) bub_add(sl, B.labelSlot, accA, rot[nc[k]], -(double)pc);
           }
           const float a = (float)(mask_cnt(m) * (double)pc - s);
           ap[c] = a;
@@ -2721,13 +2745,12 @@ static void precondition_rest(tq_sim* s) {
 // the V-cycle (level 0's down kernel starts with the norm and stop test)
 static void pcg_iteration(tq_sim* s) {
   const int C = s->C;
-  const int nbs = std::min((C + 256 * SPMV_CELLS - 1) / (256 * SPMV_CELLS),
-                           s->launch_spmv_grid > 0 ? s->launch_spmv_grid : SPMV_GRID);
+  const int nbs = std::min(spmv_tiles(s->nx, s->ny), s->launch_spmv_grid > 0 ? s->launch_spmv_grid : SPMV_GRID);
   const int pn = 1 - s->pc;
   const RedSrc rz = red_src(s, PS_Z, 2, 1, s->nb_z);
   int t = time_begin(s, "pcg_spmv", (1 + 4 + 4 + 4 + 4) * (double)C);
   hipLaunchKernelGGL(k_spmv, dim3(nbs), dim3(256), 0, s->st, s->opMask, s->rot, s->zbuf[s->zc], s->pbuf[s->pc],
-                     s->pbuf[pn], s->ap, s->nx, C, bubargs(s), rz.part, rz.bpart, rz.nb, it_par(s),
+                     s->pbuf[pn], s->ap, s->nx, s->ny, bubargs(s), rz.part, rz.bpart, rz.nb, it_par(s),
                      s->it_k == 0 ? 1 : 0, s->partials + ps_off(PS_SPMV), bpart_a(s), s->sc);
   time_end(s, t);
   s->nb_spmv = nbs;

@@ -145,7 +145,6 @@ int tq_autotune(tq_sim* s, int iterations, tq_tune_result* out) {
   };
   // candidate values that change what is launched on this grid
   const int L = (int)s->lv.size();
-  const int C = s->C;
   if (!rc && s->vop != nullptr) {
     TuneSet t = best;
     t.mg_dense = 1 - best.mg_dense;
@@ -177,7 +176,7 @@ int tq_autotune(tq_sim* s, int iterations, tq_tune_result* out) {
     }
   }
   if (!rc) {
-    const int spmv_blocks = (C + 1023) / 1024;  // chunks of 256 threads x 4 cells
+    const int spmv_blocks = ((s->nx + 63) / 64) * ((s->ny + 15) / 16);  // spmv's 64x16 tiles
     const TuneSet base = best;
     for (int cap : {256, 512, 2048, 4096}) {
       if (rc || cap >= spmv_blocks) continue;
