@@ -448,6 +448,136 @@ def profile_for(name):
     return d.get("kernels", {}).get(name)
 
 
+# ------------------------------------------------------------------ rocprofv3 summaries (profiles/)
+def _csv_rows(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _find(d, name):
+    """The rocprofv3 CSV of `name` (kernel_trace, counter_collection) under directory d."""
+    for root, _, files in os.walk(d):
+        for fn in files:
+            if fn.endswith(name + ".csv"):
+                return os.path.join(root, fn)
+    raise FileNotFoundError("%s: no *%s.csv" % (d, name))
+
+
+def _grid_total(r):
+    if "Grid_Size" in r:
+        return int(r["Grid_Size"])
+    return int(r.get("Grid_Size_X", 1)) * int(r.get("Grid_Size_Y", 1)) * int(r.get("Grid_Size_Z", 1))
+
+
+def _kname(r):
+    n = r["Kernel_Name"].split("(")[0].strip()
+    return n[5:] if n.startswith("void ") else n
+
+
+def solver_kernel_map(nx, ny):
+    """Timing name -> (kernel symbol, total grid in threads) of the FP32 solver kernels as the
+    library launches them on an nx x ny grid with the default launch shapes (levels halve while
+    min(nx, ny) > 12; the fused down kernel has 9 waves per block, the others 256 threads)."""
+    dims = [(nx, ny)]
+    while min(dims[-1]) > 12:
+        dims.append(((dims[-1][0] + 1) // 2, (dims[-1][1] + 1) // 2))
+    t16 = lambda n: (n + 15) // 16
+    groups = lambda a, b, d: ((a + d + 15) // 16 + 1) // 2 * (((b + d + 15) // 16 + 1) // 2)
+    m = {"pcg_spmv": ("tq::k_spmv", min(((nx + 63) // 64) * ((ny + 15) // 16), 1024) * 256),
+         "pcg_update_mg_in_l0": ("tq::k_in_a0<float, true, true>", min(groups(nx, ny, 0), 1280) * 256),
+         "mg_pass_z_l0": ("tq::k_pass_z<float>", groups(nx, ny, 0) * 256),
+         "mg_prolong_c_l0": ("tq::k_prolong_pass<float, float, float>", groups(nx, ny, 8) * 256)}
+    for l in range(len(dims) - 1):
+        a, b = dims[l + 1]
+        for passa in ("true", "false"):
+            m.setdefault("mg_down_l%d" % l, None)
+            key = ("tq::k_bra<float, float, float, float, %s>" % passa, t16(a) * t16(b) * 576)
+            m["mg_down_l%d:%s" % (l, passa)] = key
+        if l > 0:
+            m["mg_up_l%d" % l] = ("tq::k_pcd<float, float, float, false>", t16(dims[l][0]) * t16(dims[l][1]) * 256)
+    return {k: v for k, v in m.items() if v is not None}
+
+
+def summarize_profiles(fetch_dir, write_dir, trace_dir, nx, ny, workload, commands, copy_bytes=256 << 20):
+    """The traffic (FETCH_SIZE, WRITE_SIZE in KiB per dispatch, scaled so a device copy of
+    copy_bytes reads and writes copy_bytes) and kernel-trace durations of every (kernel symbol,
+    grid) of a run, with the solver kernels also under their timing names."""
+    import statistics
+    def counters(d, counter):
+        per = {}
+        calib = None
+        for r in _csv_rows(_find(d, "counter_collection")):
+            if r.get("Counter_Name") != counter:
+                continue
+            v = float(r["Counter_Value"])
+            k = (_kname(r), _grid_total(r))
+            if "copyBuffer" in k[0] and (calib is None or v > calib):
+                calib = v
+            per.setdefault(k, []).append(v)
+        return per, calib
+    fetch, fcal = counters(fetch_dir, "FETCH_SIZE")
+    write, wcal = counters(write_dir, "WRITE_SIZE")
+    ffac = copy_bytes / (fcal * 1024.0) if fcal else 1.0
+    wfac = copy_bytes / (wcal * 1024.0) if wcal else 1.0
+    dur = {}
+    for r in _csv_rows(_find(trace_dir, "kernel_trace")):
+        k = (_kname(r), _grid_total(r))
+        dur.setdefault(k, []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    table = []
+    for k, ds in dur.items():
+        f = fetch.get(k, [])
+        w = write.get(k, [])
+        table.append({"symbol": k[0], "grid_total": k[1], "dispatches": len(ds),
+                      "total_ms": round(sum(ds) / 1e3, 3), "avg_us": round(sum(ds) / len(ds), 3),
+                      "median_us": round(statistics.median(ds), 3),
+                      "fetch_bytes_per_launch": round(sum(f) / len(f) * 1024 * ffac) if f else None,
+                      "write_bytes_per_launch": round(sum(w) / len(w) * 1024 * wfac) if w else None})
+    table.sort(key=lambda e: -e["total_ms"])
+    return {"workload": workload, "commands": commands,
+            "calibration": {"copy_bytes": copy_bytes, "fetch_kib": fcal, "write_kib": wcal,
+                            "fetch_factor": ffac, "write_factor": wfac},
+            "kernels": named_kernels(table, nx, ny), "by_symbol_grid": table}
+
+
+def named_kernels(table, nx, ny):
+    """The solver kernels of a (symbol, grid) table under their timing names."""
+    byk = {(e["symbol"][5:] if e["symbol"].startswith("void ") else e["symbol"], e["grid_total"]): e for e in table}
+    kernels = {}
+    for name, key in solver_kernel_map(nx, ny).items():
+        e = byk.get(key)
+        if not e:
+            continue
+        base = name.split(":")[0]
+        if base in kernels:
+            continue
+        hb = (e["fetch_bytes_per_launch"] or 0) + (e["write_bytes_per_launch"] or 0)
+        kernels[base] = {"symbol": key[0], "grid_total": key[1], "dispatches": e["dispatches"],
+                         "fetch_bytes_per_launch": e["fetch_bytes_per_launch"],
+                         "write_bytes_per_launch": e["write_bytes_per_launch"],
+                         "hbm_bytes_per_launch": hb or None, "rocprof_avg_us": e["avg_us"],
+                         "rocprof_median_us": e["median_us"]}
+    return kernels
+
+
+def profile_markdown(summary, title, top=30):
+    """The kernel table of a summary as markdown (profiles/*.md)."""
+    lines = ["# " + title, "", "Commands (from the repository root, on one MI355X):", "", "```"]
+    lines += summary["commands"] + ["```", "",
+                                    "FETCH_SIZE / WRITE_SIZE scaled by a %d-byte device copy in the same runs (factors %.3f / "
+                                    "%.3f).  Per (kernel, grid in threads):" % (
+                                        summary["calibration"]["copy_bytes"], summary["calibration"]["fetch_factor"],
+                                        summary["calibration"]["write_factor"]), "",
+                                    "| kernel | grid | calls | total ms | avg us | median us | fetch MB/launch | write MB/launch |",
+                                    "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    for e in summary["by_symbol_grid"][:top]:
+        mb = lambda v: "%.3f" % (v / 1e6) if v is not None else "-"
+        lines.append("| `%s` | %d | %d | %.2f | %.2f | %.2f | %s | %s |" % (
+            e["symbol"], e["grid_total"], e["dispatches"], e["total_ms"], e["avg_us"], e["median_us"],
+            mb(e["fetch_bytes_per_launch"]), mb(e["write_bytes_per_launch"])))
+    return "\n".join(lines) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -462,6 +592,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (64 tanks) runs")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (funnel) runs")
     ap.add_argument("--no-autotune", action="store_true", help="run with the built-in launch shapes")
+    ap.add_argument("--summarize-profiles", nargs=5, metavar=("FETCH_DIR", "WRITE_DIR", "TRACE_DIR", "OUT_JSON", "OUT_MD"),
+                    help="write the traffic/duration summary of three rocprofv3 runs of this workload (--nx/--ny)")
+    ap.add_argument("--profile-commands", action="append", default=[], help="the commands of the summarized runs")
     ap.add_argument("--size-sweep", action="store_true",
                     help="only the hourglass at 896x504, 1792x1008 and 4096x2304: FP32 and mixed on one schedule, "
                          "and the five kernels with the most time (every kernel timed by HIP events); prints its JSON")
@@ -483,6 +616,16 @@ def main():
     import tanque_hip as tq
     from tanque_hip import scenes
     tq.load()
+    if args.summarize_profiles:
+        fd, wd, td, oj, om = args.summarize_profiles
+        name = "hourglass %dx%d" % (args.nx, args.ny)
+        sm = summarize_profiles(fd, wd, td, args.nx, args.ny, name, args.profile_commands)
+        with open(oj, "w") as f:
+            json.dump(sm, f, indent=1)
+        with open(om, "w") as f:
+            f.write(profile_markdown(sm, "Kernel time and HBM traffic, %s" % name))
+        print(json.dumps({"summary": oj, "kernels": sm["kernels"]}), flush=True)
+        return
     if args.adjoint_only or args.adjoint_check:
         print(json.dumps({"adjoint": run_adjoint(tq, fd=args.adjoint_check)}), flush=True)
         return

@@ -2415,8 +2415,9 @@ static void launch_in_a0(tq_sim* s, bool upd) {
   const dim3 g(std::min(nvb, s->launch_update_grid > 0 ? s->launch_update_grid : IN_A0_GRID));
   const bool pa = s->lv.size() > 1;
   const float* pp = s->pbuf[s->pc];
-  // bytes: opMask, mg0, b0 write, x0 write; update: q, p, r, Ap read + q, r write
-  const double by = (2 + vbytes(F.hb) + (pa ? 4 : 0) + (upd ? 24 : 4)) * (double)F.n;
+  // bytes: opMask, mg0, b0 write, x0 write; update: q, p, r, Ap read + q, r write (and q_lo read
+  // and written for the compensated iterate)
+  const double by = (2 + vbytes(F.hb) + (pa ? 4 : 0) + (upd ? (s->qcomp ? 32 : 24) : 4)) * (double)F.n;
   const RedSrc rs = upd ? red_src(s, PS_SPMV, 1, 0, s->nb_spmv) : RedSrc{s->partials + ps_off(PS_SPMV), bpart_a(s), 0};
   int t = time_begin(s, upd ? "pcg_update_mg_in_l0" : "pcg_mg_in_l0", by);
   // the first cycle reads the bubble residuals of parity 1 (par 0 selects them)
