@@ -604,6 +604,8 @@ def main():
                     help="only the adjoint configuration's forward and backward (for kernel profiles); prints its JSON")
     ap.add_argument("--adjoint-check", action="store_true",
                     help="only the adjoint configuration with its central differences and windowed check")
+    ap.add_argument("--sync-probe", action="store_true",
+                    help="only the kernel-boundary / grid-barrier cost probe (tanque-de-agua_amd/lib/bench_sync)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="start with a 256 MiB device copy (reference for rocprofv3 --pmc byte counters)")
     ap.add_argument("--nx", type=int, default=896)
@@ -616,6 +618,16 @@ def main():
     import tanque_hip as tq
     from tanque_hip import scenes
     tq.load()
+    if args.sync_probe:
+        import subprocess
+        exe = os.path.join(ROOT, "tanque-de-agua_amd", "lib", "bench_sync")
+        if not os.path.exists(exe):
+            raise SystemExit("bench_sync is not built (__graft_entry__.build())")
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            raise SystemExit("bench_sync failed: %s" % r.stderr)
+        print(json.dumps({"sync_probe": json.loads(r.stdout)}), flush=True)
+        return
     if args.summarize_profiles:
         fd, wd, td, oj, om = args.summarize_profiles
         name = "hourglass %dx%d" % (args.nx, args.ny)

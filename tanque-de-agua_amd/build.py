@@ -67,6 +67,23 @@ def build_lib(force=False):
     return LIB
 
 
+SYNC_BENCH = os.path.join(LIB_DIR, "bench_sync")
+
+
+def build_sync_bench(force=False):
+    """The kernel-boundary / grid-barrier cost probe (csrc/bench_sync.hip; bench.py --sync-probe)."""
+    src = os.path.join(CSRC, "bench_sync.hip")
+    if not force and not _newer(SYNC_BENCH, [src]):
+        return SYNC_BENCH
+    tmp = SYNC_BENCH + ".tmp"
+    r = subprocess.run([HIPCC] + [f for f in HIP_FLAGS if f != "-fPIC"] + [src, "-o", tmp], capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError("bench_sync build failed:\n%s\n%s" % (r.stdout, r.stderr))
+    os.replace(tmp, SYNC_BENCH)
+    return SYNC_BENCH
+
+
 def build_node_addon(force=False):
     """The thin Node-API addon over the C ABI (needs node's headers; skipped if absent)."""
     src = os.path.join(CSRC, "napi_addon.cpp")
@@ -88,6 +105,7 @@ def build_node_addon(force=False):
 def build(force=False):
     lib = build_lib(force)
     addon = build_node_addon(force)
+    build_sync_bench(force)
     return lib, addon
 
 
