@@ -55,10 +55,13 @@ typedef struct tq_desc {
                              FP16 (level 0 keeps x in FP32 and stores only b in FP16); levels
                              under 4096 cells and those in LDS stay FP32; -1 = default (1) */
   int use_graph;          /* 1: PCG iterations are replayed from a captured HIP graph */
-  int mg_dense;           /* 1 (default): when the first one-workgroup level has at most 2048
-                             cells and is not the coarsest, its V-cycle is built once per solve
-                             as a dense matrix and applied by a many-workgroup product;
-                             0: that V-cycle runs in one workgroup every iteration */
+  int mg_dense;           /* 1: when the first one-workgroup level has at most 2048 cells and is
+                             not the coarsest, its V-cycle is built once per solve as a dense
+                             matrix (one V-cycle per row) and applied by a many-workgroup
+                             product; 0: that V-cycle runs in one workgroup every iteration;
+                             -1 (default): the matrix when the solve runs to a tolerance or for
+                             at least 8 fixed iterations (the reference's 3 apply the V-cycle
+                             too few times for the build to pay) */
 } tq_desc;
 
 /* Fills `d` with the reference defaults (constantes.js) for an nx x ny grid of cell size h. */
@@ -173,15 +176,37 @@ int tq_emit_faucet(tq_sim* s, const tq_faucet* f, double dt, uint32_t* seed, int
 
 /* A batch of independent tanks stepped together (the batched configuration; the reference steps
  * one SimulacaoCPU per tank, fisica-cpu.js:29-77, 155-159, so there is no batch counterpart).
- * Every tank is a tq_sim with its own stream, set up through the tq_* calls on tq_batch_tank(b, k);
- * tq_batch_step does avancar(dt, substeps[k]) on every tank from `threads` host threads (tank k
- * on thread k mod threads) and returns when all are done; the device overlaps the tanks'
- * kernels.  Do not call tq_* on a tank while tq_batch_step runs. */
+ * Every tank is a tq_sim set up through the tq_* calls on tq_batch_tank(b, k); tq_batch_step does
+ * avancar(dt, substeps[k]) on every tank and returns when all are done, each tank ending exactly
+ * as if stepped alone.  Two launch models:
+ *   tq_batch_create_table: the tanks share one stream; a step records every tank's frame on
+ *     `threads` host threads and runs each kernel of the step as ONE table launch over all tanks
+ *     (blocks = the concatenation of the tanks' grids).  Needs tol = 0 (the fixed PCG iteration
+ *     count) on every tank: a step on a tank with tol > 0 returns TQ_E_STATE.
+ *   tq_batch_create: every tank on its own stream, its frame enqueued from `threads` host threads
+ *     (tank k on thread k mod threads); the device overlaps the tanks' kernels.
+ * Do not call tq_* on a tank while tq_batch_step runs; after a failed step the tanks' states are
+ * undefined. */
 typedef struct tq_batch tq_batch;
+int tq_batch_create_table(const tq_desc* descs, int n, int threads, tq_batch** out);
 int tq_batch_create(const tq_desc* descs, int n, int threads, tq_batch** out);
 int tq_batch_size(tq_batch* b, int* n);
 tq_sim* tq_batch_tank(tq_batch* b, int k);   /* NULL when k is out of range */
 int tq_batch_step(tq_batch* b, double dt, const int* substeps);
+/* every tank's vMax (tq_vmax; has[k] as its `has`), v and has of n entries (either may be null):
+ * a table step leaves them in host memory, so after one they are read without a device wait */
+int tq_batch_vmax(tq_batch* b, double* v, int* has);
+/* where a table batch's steps spend host time (seconds, summed over steps since the last reset):
+ * recording the tanks' frames, merging the rounds, copying the records into the upload buffer,
+ * issuing the copies and table launches, waiting for the device; and the last step's table
+ * launches and uploaded record bytes */
+typedef struct tq_batch_stats {
+  int steps;
+  int launches;
+  double upload_bytes;
+  double record_s, plan_s, stage_s, issue_s, wait_s;
+} tq_batch_stats;
+int tq_batch_stats_get(tq_batch* b, tq_batch_stats* out, int reset);
 int tq_batch_destroy(tq_batch* b);
 
 /* Differentiation of recorded frames (the reference has no gradient; it is the backward() that

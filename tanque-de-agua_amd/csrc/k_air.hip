@@ -46,10 +46,10 @@ __device__ inline void uf_unite(int32_t* pai, int a, int b) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_uf_init(int C, uint32_t* __restrict__ cells, uint32_t* __restrict__ atm,
+__device__ __forceinline__ void k_uf_init(Blk B_, int C, uint32_t* __restrict__ cells, uint32_t* __restrict__ atm,
                                                  uint32_t* __restrict__ topo, int32_t* __restrict__ dono,
                                                  uint32_t* __restrict__ label, float* __restrict__ alvo) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = B_.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
   cells[c] = 0; atm[c] = 0; topo[c] = 0; dono[c] = -1; label[c] = 0; alvo[c] = 0.f;
 }
@@ -82,10 +82,10 @@ __device__ inline void lds_unite(int* lp, int a, int b) {
 
 // Every cell of the tile gets pai = the global index of its tile-component's root (-1: not air).
 // Local indices are row-major in the tile, so the smallest local index is the smallest global one.
-__global__ __launch_bounds__(256) void k_uf_local(const uint8_t* __restrict__ tipo, int nx, int ny,
+__device__ __forceinline__ void k_uf_local(Blk B_, const uint8_t* __restrict__ tipo, int nx, int ny,
                                                   int32_t* __restrict__ pai) {
   __shared__ int lp[UF_T * UF_T];
-  const int ox = blockIdx.x * UF_T, oy = blockIdx.y * UF_T;
+  const int ox = B_.x * UF_T, oy = B_.y * UF_T;
   bool air[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -122,9 +122,9 @@ __global__ __launch_bounds__(256) void k_uf_local(const uint8_t* __restrict__ ti
 
 // ---- 1b. tile-border cells unite with their neighbour across the border (global array):
 // first the left borders (i = 32a, a > 0), then the top borders (j = 32b, b > 0)
-__global__ __launch_bounds__(256) void k_uf_border(const uint8_t* __restrict__ tipo, int nx, int ny,
+__device__ __forceinline__ void k_uf_border(Blk B_, const uint8_t* __restrict__ tipo, int nx, int ny,
                                                    int32_t* __restrict__ pai) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = B_.x * blockDim.x + threadIdx.x;
   const int ntx = (nx + UF_T - 1) / UF_T, nty = (ny + UF_T - 1) / UF_T;
   const int nv = (ntx - 1) * ny;
   int c, d;
@@ -143,9 +143,9 @@ __global__ __launch_bounds__(256) void k_uf_border(const uint8_t* __restrict__ t
   if (tipo[c] == AR && tipo[d] == AR) uf_unite(pai, c, d);
 }
 
-__global__ __launch_bounds__(256) void k_uf_flatten(const uint8_t* __restrict__ tipo, int C, int32_t* __restrict__ pai,
+__device__ __forceinline__ void k_uf_flatten(Blk B_, const uint8_t* __restrict__ tipo, int C, int32_t* __restrict__ pai,
                                                     uint32_t* __restrict__ rootFlag) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = B_.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
   uint32_t f = 0;
   if (tipo[c] == AR) {
@@ -189,7 +189,7 @@ __device__ inline void overlap_add(unsigned long long key, uint32_t cnt, uint32_
 // integer count, a flag or a minimum, so the order of the adds does not matter.
 constexpr int CS_SLOTS = 32;
 constexpr int CS_GRID = 2048;
-__global__ __launch_bounds__(256) void k_comp_stats(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
+__device__ __forceinline__ void k_comp_stats(Blk B_, const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
                                                     int nx, int ny, const int32_t* __restrict__ pai,
                                                     const uint32_t* __restrict__ compIdx, int32_t* __restrict__ comp,
                                                     uint32_t* __restrict__ cells, uint32_t* __restrict__ atm,
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void k_comp_stats(const uint8_t* __restrict__ 
   __syncthreads();
   const int C = nx * ny;
   const int lane = threadIdx.x & 63;
-  for (int base = blockIdx.x * blockDim.x; base < C; base += gridDim.x * blockDim.x) {
+  for (int base = B_.x * blockDim.x; base < C; base += B_.gx * blockDim.x) {
     const int c = base + threadIdx.x;
     int k = -1;
     uint32_t old = 0;
@@ -318,7 +318,7 @@ __device__ void block_bitonic(unsigned long long* keys, uint32_t* vals, int n) {
     }
 }
 
-__global__ __launch_bounds__(1024) void k_bookkeep(const uint32_t* __restrict__ compIdxTotal, const uint32_t* __restrict__ cells,
+__device__ __forceinline__ void k_bookkeep(Blk B_, const uint32_t* __restrict__ compIdxTotal, const uint32_t* __restrict__ cells,
                                                    const uint32_t* __restrict__ atm, const uint32_t* __restrict__ topo,
                                                    float* __restrict__ alvo, int32_t* __restrict__ dono,
                                                    uint32_t* __restrict__ label, unsigned long long* __restrict__ hkeys,
@@ -427,9 +427,9 @@ __global__ __launch_bounds__(1024) void k_bookkeep(const uint32_t* __restrict__ 
   if (tid == 0) { sc->npairs = 0; sc->primeira = 0; }
 }
 
-__global__ __launch_bounds__(256) void k_label_write(const int32_t* __restrict__ comp, const uint32_t* __restrict__ label,
+__device__ __forceinline__ void k_label_write(Blk B_, const int32_t* __restrict__ comp, const uint32_t* __restrict__ label,
                                                      int C, uint32_t* __restrict__ rot) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = B_.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
   const int k = comp[c];
   rot[c] = k >= 0 ? label[k] : 0u;
@@ -439,24 +439,24 @@ int tracker_update(tq_sim* s) {
   const int C = s->C;
   const dim3 g((C + 255) / 256), b(256);
   int t = time_begin(s, "air_tracker", 16.0 * C);
-  hipLaunchKernelGGL(k_uf_init, g, b, 0, s->st, C, s->compCells, s->compAtm, s->compTopo, s->compDono, s->compLabel,
+  launch<256, 0, k_uf_init>(s, s->st, g, b, 0, C, s->compCells, s->compAtm, s->compTopo, s->compDono, s->compLabel,
                      s->compAlvo);
   const int ntx = (s->nx + UF_T - 1) / UF_T, nty = (s->ny + UF_T - 1) / UF_T;
-  hipLaunchKernelGGL(k_uf_local, dim3(ntx, nty), b, 0, s->st, s->tipo, s->nx, s->ny, s->pai);
+  launch<256, 0, k_uf_local>(s, s->st, dim3(ntx, nty), b, 0, s->tipo, s->nx, s->ny, s->pai);
   const int nborder = (ntx - 1) * s->ny + (nty - 1) * s->nx;
   if (nborder > 0)
-    hipLaunchKernelGGL(k_uf_border, dim3((nborder + 255) / 256), b, 0, s->st, s->tipo, s->nx, s->ny, s->pai);
-  hipLaunchKernelGGL(k_uf_flatten, g, b, 0, s->st, s->tipo, C, s->pai, s->rootFlag);
+    launch<256, 0, k_uf_border>(s, s->st, dim3((nborder + 255) / 256), b, 0, s->tipo, s->nx, s->ny, s->pai);
+  launch<256, 0, k_uf_flatten>(s, s->st, g, b, 0, s->tipo, C, s->pai, s->rootFlag);
   TQ_LAUNCH_CHECK();
   if (int e = scan_u32(s, s->rootFlag, s->compIdx, C)) return e;
-  hipLaunchKernelGGL(k_comp_stats, dim3(std::min((C + 255) / 256, CS_GRID)), b, 0, s->st, s->tipo, s->rot, s->nx, s->ny,
+  launch<256, 0, k_comp_stats>(s, s->st, dim3(std::min((C + 255) / 256, CS_GRID)), b, 0, s->tipo, s->rot, s->nx, s->ny,
                      s->pai, s->compIdx, s->comp,
                      s->compCells, s->compAtm, s->compTopo, s->hkeys, s->hcount, s->hfirst, s->hlist, s->hsize - 1, s->sc);
-  hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(1024), 0, s->st, s->compIdx + C, s->compCells, s->compAtm, s->compTopo,
+  launch<1024, 0, k_bookkeep>(s, s->st, dim3(1), dim3(1024), 0, s->compIdx + C, s->compCells, s->compAtm, s->compTopo,
                      s->compAlvo, s->compDono, s->compLabel, s->hkeys, s->hcount, s->hfirst, s->hlist,
                      (unsigned long long*)s->ordBuf, s->ordBuf + 2 * (size_t)s->hsize, s->volAlvo, s->vivo, s->pBolha,
                      s->sc);
-  hipLaunchKernelGGL(k_label_write, g, b, 0, s->st, s->comp, s->compLabel, C, s->rot);
+  launch<256, 0, k_label_write>(s, s->st, g, b, 0, s->comp, s->compLabel, C, s->rot);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   return 0;

@@ -16,7 +16,7 @@ constexpr int SF_THREADS = 1024;
 constexpr int SF_K = (SF_N + SF_THREADS - 1) / SF_THREADS;   // region cells per thread (4)
 static_assert(SF_T * SF_T == SF_THREADS, "one owned cell per thread");
 
-__global__ __launch_bounds__(SF_THREADS) void k_surface(const uint8_t* __restrict__ tipo, const float* __restrict__ dens,
+__device__ __forceinline__ void k_surface(Blk B_, const uint8_t* __restrict__ tipo, const float* __restrict__ dens,
                                                         int nx, int ny, double h, float* __restrict__ fase,
                                                         float* __restrict__ nXo, float* __restrict__ nYo,
                                                         float* __restrict__ kap) {
@@ -25,7 +25,7 @@ __global__ __launch_bounds__(SF_THREADS) void k_surface(const uint8_t* __restric
   __shared__ uint8_t ty[SF_N], fl[SF_N];
   const int tid = threadIdx.x;
   int bxl, byl;
-  xcd_block2(&bxl, &byl);
+  xcd_block2(B_, &bxl, &byl);
   const int I0 = bxl * SF_T - SF_H, J0 = byl * SF_T - SF_H;
   int solid_here = 0, fluid_here = 0;  // solid / filled cells in this thread's part of the region
   // this thread's region cells, fixed for every pass
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(SF_THREADS) void k_surface(const uint8_t* __restric
 int surface(tq_sim* s) {
   dim3 g((s->nx + SF_T - 1) / SF_T, (s->ny + SF_T - 1) / SF_T);
   int t = time_begin(s, "surface", (1 + 4 + 4 * 4) * (double)s->C);
-  hipLaunchKernelGGL(k_surface, g, dim3(SF_THREADS), 0, s->st, s->tipo, s->dens, s->nx, s->ny, s->h, s->fase, s->nX,
+  launch<SF_THREADS, 0, k_surface>(s, s->st, g, dim3(SF_THREADS), 0, s->tipo, s->dens, s->nx, s->ny, s->h, s->fase, s->nX,
                      s->nY, s->kappa);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
@@ -177,7 +177,7 @@ __device__ inline float qar_at(const uint8_t* __restrict__ tipo, const float* __
   return (float)(p * escala);
 }
 
-__global__ __launch_bounds__(256) void k_air_div(const uint8_t* __restrict__ tipo, const float* __restrict__ kap,
+__device__ __forceinline__ void k_air_div(Blk B_, const uint8_t* __restrict__ tipo, const float* __restrict__ kap,
                                                  const float* __restrict__ u, const float* __restrict__ v,
                                                  const float* __restrict__ dens, const uint32_t* __restrict__ count,
                                                  const uint8_t* __restrict__ solid, int nx, int ny, double tensao,
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void k_air_div(const uint8_t* __restrict__ tip
                                                  float* __restrict__ qAr, float* __restrict__ div,
                                                  float* __restrict__ q) {
 #pragma clang fp contract(off)
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = B_.x * blockDim.x + threadIdx.x;
   if (c >= nx * ny) return;
   const int i = c % nx, j = c / nx;
   if (which & 1) qAr[c] = qar_at(tipo, kap, nx, ny, i, j, tensao, kMax, escala);
@@ -217,7 +217,7 @@ int air_pressure_and_div(tq_sim* s, double dt, int which) {
   const double escala = dt / (DENSIDADE_AGUA * s->h), kMax = 1 / s->h;
   const double alvo = s->h / std::max(s->d.tau_densidade, dt);
   int t = time_begin(s, "air_div", (1 + 4 + 4 + 4 + 4 + 1 + 4 + 4 + 4) * (double)s->C);
-  hipLaunchKernelGGL(k_air_div, dim3((s->C + 255) / 256), dim3(256), 0, s->st, s->tipo, s->kappa, s->u, s->v, s->dens,
+  launch<256, 0, k_air_div>(s, s->st, dim3((s->C + 255) / 256), dim3(256), 0, s->tipo, s->kappa, s->u, s->v, s->dens,
                      s->countCell, s->solid, s->nx, s->ny, s->d.tensao, kMax, escala, alvo, which, s->qAr, s->div, s->q);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
@@ -231,10 +231,10 @@ int air_pressure_and_div(tq_sim* s, double dt, int which) {
 // live values (already processed in raster order) and R, U the values at the start of the
 // pass (0 for candidates).  The dependency graph is acyclic (raster order), so the fixpoint is
 // unique and equals the sequential result.
-__global__ __launch_bounds__(256) void k_label_prep(const uint8_t* __restrict__ tipo, uint32_t* __restrict__ rot,
+__device__ __forceinline__ void k_label_prep(Blk B_, const uint8_t* __restrict__ tipo, uint32_t* __restrict__ rot,
                                                     uint8_t* __restrict__ cflag, uint32_t* __restrict__ cand,
                                                     uint32_t* __restrict__ candCount, int nx, int ny) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = B_.x * blockDim.x + threadIdx.x;
   bool is_c = false;
   if (c < nx * ny) {
     const int i = c % nx, j = c / nx;
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void k_label_prep(const uint8_t* __restrict__ 
   }
 }
 
-__global__ __launch_bounds__(1024) void k_label_prop(uint32_t* __restrict__ rot, uint8_t* __restrict__ cflag,
+__device__ __forceinline__ void k_label_prop(Blk B_, uint32_t* __restrict__ rot, uint8_t* __restrict__ cflag,
                                                      const uint32_t* __restrict__ cand,
                                                      const uint32_t* __restrict__ candCount, int nx) {
   __shared__ int changed;
@@ -291,14 +291,14 @@ __global__ __launch_bounds__(1024) void k_label_prop(uint32_t* __restrict__ rot,
 }
 
 int propagate_labels(tq_sim* s) {
-  TQ_HIP(hipMemsetAsync(&s->sc->cand_count, 0, sizeof(uint32_t), s->st));
+  if (int e = zero_async(s, s->st, &s->sc->cand_count, sizeof(uint32_t))) return e;
   int t = time_begin(s, "label_prep", 6.0 * s->C);
-  hipLaunchKernelGGL(k_label_prep, dim3((s->C + 255) / 256), dim3(256), 0, s->st, s->tipo, s->rot, s->cflag, s->cand,
+  launch<256, 0, k_label_prep>(s, s->st, dim3((s->C + 255) / 256), dim3(256), 0, s->tipo, s->rot, s->cflag, s->cand,
                      &s->sc->cand_count, s->nx, s->ny);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   t = time_begin(s, "label_prop", 0);
-  hipLaunchKernelGGL(k_label_prop, dim3(1), dim3(1024), 0, s->st, s->rot, s->cflag, s->cand, &s->sc->cand_count, s->nx);
+  launch<1024, 0, k_label_prop>(s, s->st, dim3(1), dim3(1024), 0, s->rot, s->cflag, s->cand, &s->sc->cand_count, s->nx);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   return 0;
@@ -324,7 +324,7 @@ __device__ inline long long wave_sum_i64(long long v) {
 // atomics instead of one per wave.  Labels the full table cannot take go to global atomics.
 constexpr int BRHS_SLOTS = 32;
 constexpr int BRHS_GRID = 2048;
-__global__ __launch_bounds__(256) void k_bubble_rhs(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
+__device__ __forceinline__ void k_bubble_rhs(Blk B_, const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
                                                     const float* __restrict__ u, const float* __restrict__ v,
                                                     const float* __restrict__ qAr, int nx, int ny,
                                                     uint32_t* __restrict__ volB, uint32_t* __restrict__ facesB,
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void k_bubble_rhs(const uint8_t* __restrict__ 
   __syncthreads();
   const int C = nx * ny;
   const int lane = threadIdx.x & 63;
-  for (int base = blockIdx.x * blockDim.x; base < C; base += gridDim.x * blockDim.x) {
+  for (int base = B_.x * blockDim.x; base < C; base += B_.gx * blockDim.x) {
     const int c = base + threadIdx.x;
     uint32_t r = 0;
     int interior = 0, faces = 0;
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(256) void k_bubble_rhs(const uint8_t* __restrict__ 
   }
 }
 
-__global__ __launch_bounds__(1024) void k_bubble_finish(const uint32_t* __restrict__ volB, double* __restrict__ rhsB,
+__device__ __forceinline__ void k_bubble_finish(Blk B_, const uint32_t* __restrict__ volB, double* __restrict__ rhsB,
                                                         float* __restrict__ rhsB32, float* __restrict__ yB32,
                                                         const float* __restrict__ pBolha,
                                                         const float* __restrict__ volAlvo, double h, double dt) {
@@ -429,9 +429,9 @@ __global__ __launch_bounds__(1024) void k_bubble_finish(const uint32_t* __restri
 }
 
 // the per-bubble sums of k_bubble_rhs start at zero (one launch instead of four fills)
-__global__ __launch_bounds__(256) void k_bubble_clear(uint32_t* __restrict__ volB, uint32_t* __restrict__ facesB,
+__device__ __forceinline__ void k_bubble_clear(Blk B_, uint32_t* __restrict__ volB, uint32_t* __restrict__ facesB,
                                                       uint32_t* __restrict__ cellsB, double* __restrict__ rhsB) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = B_.x * blockDim.x + threadIdx.x;
   if (r >= MAX_BOLHAS) return;
   volB[r] = 0;
   facesB[r] = 0;
@@ -440,14 +440,14 @@ __global__ __launch_bounds__(256) void k_bubble_clear(uint32_t* __restrict__ vol
 }
 
 int bubble_rhs(tq_sim* s, double dt) {
-  hipLaunchKernelGGL(k_bubble_clear, dim3((MAX_BOLHAS + 255) / 256), dim3(256), 0, s->st, s->volB, s->facesB, s->cellsB,
+  launch<256, 0, k_bubble_clear>(s, s->st, dim3((MAX_BOLHAS + 255) / 256), dim3(256), 0, s->volB, s->facesB, s->cellsB,
                      s->rhsB);
   int t = time_begin(s, "bubble_rhs", (1 + 4 + 4 + 4 + 4) * (double)s->C);
-  hipLaunchKernelGGL(k_bubble_rhs, dim3(std::min((s->C + 255) / 256, BRHS_GRID)), dim3(256), 0, s->st, s->tipo, s->rot,
+  launch<256, 0, k_bubble_rhs>(s, s->st, dim3(std::min((s->C + 255) / 256, BRHS_GRID)), dim3(256), 0, s->tipo, s->rot,
                      s->u, s->v, s->qAr, s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB, 0);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_bubble_finish, dim3(1), dim3(1024), 0, s->st, s->volB, s->rhsB, s->rhsB32, s->yB32, s->pBolha,
+  launch<1024, 0, k_bubble_finish>(s, s->st, dim3(1), dim3(1024), 0, s->volB, s->rhsB, s->rhsB32, s->yB32, s->pBolha,
                      s->volAlvo, s->h, dt);
   TQ_LAUNCH_CHECK();
   return 0;
@@ -456,27 +456,27 @@ int bubble_rhs(tq_sim* s, double dt) {
 // Per-bubble cell and face counts only (the rhs is kept): lets a solve start from a state whose
 // bubble sums were computed elsewhere.
 int bubble_counts(tq_sim* s) {
-  hipLaunchKernelGGL(k_bubble_clear, dim3((MAX_BOLHAS + 255) / 256), dim3(256), 0, s->st, s->volB, s->facesB, s->cellsB,
+  launch<256, 0, k_bubble_clear>(s, s->st, dim3((MAX_BOLHAS + 255) / 256), dim3(256), 0, s->volB, s->facesB, s->cellsB,
                      (double*)nullptr);
-  hipLaunchKernelGGL(k_bubble_rhs, dim3(std::min((s->C + 255) / 256, BRHS_GRID)), dim3(256), 0, s->st, s->tipo, s->rot,
+  launch<256, 0, k_bubble_rhs>(s, s->st, dim3(std::min((s->C + 255) / 256, BRHS_GRID)), dim3(256), 0, s->tipo, s->rot,
                      s->u, s->v, s->qAr, s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB, 1);
   TQ_LAUNCH_CHECK();
   return 0;
 }
 
 // ================================================================== #pressao post (fisica-cpu.js:503-514)
-__global__ __launch_bounds__(256) void k_pressure_post(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
+__device__ __forceinline__ void k_pressure_post(Blk B_, const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
                                                        const float* __restrict__ qAr, const uint32_t* __restrict__ volB,
                                                        const float* __restrict__ yB32, int C, int ar,
                                                        float* __restrict__ q) {
 #pragma clang fp contract(off)
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = B_.x * blockDim.x + threadIdx.x;
   if (c >= C || tipo[c] != AR) return;
   const uint32_t r = rot[c];
   q[c] = (float)((double)qAr[c] + ((ar && r >= 2 && volB[r]) ? (double)yB32[r] : 0.0));
 }
 
-__global__ __launch_bounds__(1024) void k_bubble_pressure(const uint32_t* __restrict__ volB, const float* __restrict__ yB32,
+__device__ __forceinline__ void k_bubble_pressure(Blk B_, const uint32_t* __restrict__ volB, const float* __restrict__ yB32,
                                                           float* __restrict__ pBolha, double paraPa) {
 #pragma clang fp contract(off)
   for (int r = threadIdx.x; r < MAX_BOLHAS; r += blockDim.x)
@@ -485,11 +485,11 @@ __global__ __launch_bounds__(1024) void k_bubble_pressure(const uint32_t* __rest
 }
 
 int pressure_post(tq_sim* s, double dt) {
-  hipLaunchKernelGGL(k_pressure_post, dim3((s->C + 255) / 256), dim3(256), 0, s->st, s->tipo, s->rot, s->qAr, s->volB,
+  launch<256, 0, k_pressure_post>(s, s->st, dim3((s->C + 255) / 256), dim3(256), 0, s->tipo, s->rot, s->qAr, s->volB,
                      s->yB32, s->C, s->d.ar, s->q);
   TQ_LAUNCH_CHECK();
   if (s->d.ar) {
-    hipLaunchKernelGGL(k_bubble_pressure, dim3(1), dim3(1024), 0, s->st, s->volB, s->yB32, s->pBolha,
+    launch<1024, 0, k_bubble_pressure>(s, s->st, dim3(1), dim3(1024), 0, s->volB, s->yB32, s->pBolha,
                        (DENSIDADE_AGUA * s->h) / dt);
     TQ_LAUNCH_CHECK();
   }
@@ -497,11 +497,11 @@ int pressure_post(tq_sim* s, double dt) {
 }
 
 // ================================================================== #gradiente (fisica-cpu.js:519-545)
-__global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ tipo, const float* __restrict__ q, int nx,
+__device__ __forceinline__ void k_gradient(Blk B_, const uint8_t* __restrict__ tipo, const float* __restrict__ q, int nx,
                                                   int ny, float* __restrict__ u, uint8_t* __restrict__ uVal,
                                                   float* __restrict__ v, uint8_t* __restrict__ vVal) {
 #pragma clang fp contract(off)
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = B_.x * blockDim.x + threadIdx.x;
   const int nu = (nx + 1) * ny, nv = nx * (ny + 1);
   if (k < nu) {
     const int i = k % (nx + 1), j = k / (nx + 1);
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ ti
 int gradient(tq_sim* s) {
   const int n = std::max(s->nu, s->nv);
   int t = time_begin(s, "gradient", 9.0 * (s->nu + s->nv));
-  hipLaunchKernelGGL(k_gradient, dim3((n + 255) / 256), dim3(256), 0, s->st, s->tipo, s->q, s->nx, s->ny, s->u, s->uVal,
+  launch<256, 0, k_gradient>(s, s->st, dim3((n + 255) / 256), dim3(256), 0, s->tipo, s->q, s->nx, s->ny, s->u, s->uVal,
                      s->v, s->vVal);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
@@ -537,12 +537,12 @@ int gradient(tq_sim* s) {
 
 // ================================================================== #extrapolar (fisica-cpu.js:547-552, 673-690)
 // One layer for both components: an invalid face with valid 4-neighbours takes their mean.
-__global__ __launch_bounds__(256) void k_extrap(const float* __restrict__ ua, const uint8_t* __restrict__ uva,
+__device__ __forceinline__ void k_extrap(Blk B_, const float* __restrict__ ua, const uint8_t* __restrict__ uva,
                                                 float* __restrict__ ub, uint8_t* __restrict__ uvb,
                                                 const float* __restrict__ va, const uint8_t* __restrict__ vva,
                                                 float* __restrict__ vb, uint8_t* __restrict__ vvb, int nx, int ny) {
 #pragma clang fp contract(off)
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = B_.x * blockDim.x + threadIdx.x;
   const int nu = (nx + 1) * ny, nv = nx * (ny + 1);
   for (int comp = 0; comp < 2; comp++) {
     const float* a = comp ? va : ua;
@@ -571,9 +571,9 @@ __global__ __launch_bounds__(256) void k_extrap(const float* __restrict__ ua, co
 int extrapolate(tq_sim* s) {
   const int n = std::max(s->nu, s->nv);
   int t = time_begin(s, "extrapolate", 10.0 * (s->nu + s->nv));
-  hipLaunchKernelGGL(k_extrap, dim3((n + 255) / 256), dim3(256), 0, s->st, s->u, s->uVal, s->uTmp, s->uValTmp, s->v,
+  launch<256, 0, k_extrap>(s, s->st, dim3((n + 255) / 256), dim3(256), 0, s->u, s->uVal, s->uTmp, s->uValTmp, s->v,
                      s->vVal, s->vTmp, s->vValTmp, s->nx, s->ny);
-  hipLaunchKernelGGL(k_extrap, dim3((n + 255) / 256), dim3(256), 0, s->st, s->uTmp, s->uValTmp, s->u, s->uVal, s->vTmp,
+  launch<256, 0, k_extrap>(s, s->st, dim3((n + 255) / 256), dim3(256), 0, s->uTmp, s->uValTmp, s->u, s->uVal, s->vTmp,
                      s->vValTmp, s->v, s->vVal, s->nx, s->ny);
   time_end(s, t);
   TQ_LAUNCH_CHECK();

@@ -32,18 +32,18 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t* t
   return r;
 }
 
-__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const uint32_t* __restrict__ in, int n, uint32_t* __restrict__ blocks) {
+__device__ __forceinline__ void k_scan_reduce(Blk B_, const uint32_t* __restrict__ in, int n, uint32_t* __restrict__ blocks) {
   __shared__ uint32_t sh[17];
-  const size_t base = (size_t)blockIdx.x * SCAN_B + (size_t)threadIdx.x * SCAN_V;
+  const size_t base = (size_t)B_.x * SCAN_B + (size_t)threadIdx.x * SCAN_V;
   uint32_t s = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_V; k++) if (base + k < (size_t)n) s += in[base + k];
   uint32_t tot;
   block_excl_scan(s, sh, &tot);
-  if (threadIdx.x == 0) blocks[blockIdx.x] = tot;
+  if (threadIdx.x == 0) blocks[B_.x] = tot;
 }
 
-__global__ __launch_bounds__(SCAN_T) void k_scan_blocks(uint32_t* __restrict__ blocks, int nb, uint32_t* __restrict__ out_total) {
+__device__ __forceinline__ void k_scan_blocks(Blk B_, uint32_t* __restrict__ blocks, int nb, uint32_t* __restrict__ out_total) {
   __shared__ uint32_t sh[17];
   uint32_t v[SCAN_V], s = 0;
 #pragma unroll
@@ -63,10 +63,10 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_blocks(uint32_t* __restrict__ b
   if (threadIdx.x == 0) *out_total = tot;
 }
 
-__global__ __launch_bounds__(SCAN_T) void k_scan_apply(const uint32_t* __restrict__ in, int n, const uint32_t* __restrict__ blocks,
+__device__ __forceinline__ void k_scan_apply(Blk B_, const uint32_t* __restrict__ in, int n, const uint32_t* __restrict__ blocks,
                                                        uint32_t* __restrict__ out) {
   __shared__ uint32_t sh[17];
-  const size_t base = (size_t)blockIdx.x * SCAN_B + (size_t)threadIdx.x * SCAN_V;
+  const size_t base = (size_t)B_.x * SCAN_B + (size_t)threadIdx.x * SCAN_V;
   uint32_t v[SCAN_V], s = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_V; k++) {
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(const uint32_t* __restric
     s += v[k];
   }
   uint32_t tot;
-  uint32_t off = block_excl_scan(s, sh, &tot) + blocks[blockIdx.x];
+  uint32_t off = block_excl_scan(s, sh, &tot) + blocks[B_.x];
 #pragma unroll
   for (int k = 0; k < SCAN_V; k++) {
     if (base + k < (size_t)n) out[base + k] = off;
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(const uint32_t* __restric
 // (itself two-level past 4096 blocks), then the per-block scans with their offsets.
 int scan_u32(tq_sim* s, const uint32_t* in, uint32_t* out, int n) {
   if (n <= 0) {
-    TQ_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), s->st));
+    if (int e = zero_async(s, s->st, out, sizeof(uint32_t))) return e;
     return 0;
   }
   const int nb = (n + SCAN_B - 1) / SCAN_B;
@@ -96,16 +96,16 @@ int scan_u32(tq_sim* s, const uint32_t* in, uint32_t* out, int n) {
   uint32_t* blocks2 = blocks + nb;             // nb2 sums of those (two-level case)
   uint32_t* offs = blocks2 + nb2 + 1;          // nb block offsets (two-level case)
   int t = time_begin(s, "scan", 8.0 * n);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, s->st, in, n, blocks);
+  launch<SCAN_T, 0, k_scan_reduce>(s, s->st, dim3(nb), dim3(SCAN_T), 0, in, n, blocks);
   if (nb <= SCAN_B) {
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_T), 0, s->st, blocks, nb, out + n);
+    launch<SCAN_T, 0, k_scan_blocks>(s, s->st, dim3(1), dim3(SCAN_T), 0, blocks, nb, out + n);
   } else {
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb2), dim3(SCAN_T), 0, s->st, blocks, nb, blocks2);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_T), 0, s->st, blocks2, nb2, out + n);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb2), dim3(SCAN_T), 0, s->st, blocks, nb, blocks2, offs);
+    launch<SCAN_T, 0, k_scan_reduce>(s, s->st, dim3(nb2), dim3(SCAN_T), 0, blocks, nb, blocks2);
+    launch<SCAN_T, 0, k_scan_blocks>(s, s->st, dim3(1), dim3(SCAN_T), 0, blocks2, nb2, out + n);
+    launch<SCAN_T, 0, k_scan_apply>(s, s->st, dim3(nb2), dim3(SCAN_T), 0, blocks, nb, blocks2, offs);
     blocks = offs;
   }
-  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_T), 0, s->st, in, n, blocks, out);
+  launch<SCAN_T, 0, k_scan_apply>(s, s->st, dim3(nb), dim3(SCAN_T), 0, in, n, blocks, out);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   return 0;
@@ -117,10 +117,10 @@ int scan_u32(tq_sim* s, const uint32_t* in, uint32_t* out, int n) {
 // ranks with one atomic and the others count from it, instead of one same-address atomic per
 // lane.  Ranks within a cell depend on the order the atomics land; every consumer of the sorted
 // order sums in fixed point or works per particle, so results do not.
-__global__ __launch_bounds__(256) void k_bin(const float2* __restrict__ pos, const DevScalars* __restrict__ sc, int nx, int ny,
+__device__ __forceinline__ void k_bin(Blk B_, const float2* __restrict__ pos, const DevScalars* __restrict__ sc, int nx, int ny,
                                              double inv, int ntx, uint32_t* __restrict__ countKey,
                                              uint32_t* __restrict__ pkey, uint32_t* __restrict__ prank) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = B_.x * blockDim.x + threadIdx.x;
   const bool slot = k < sc->n_slots;
   uint32_t key = DEAD_KEY;
   if (slot) {
@@ -148,14 +148,14 @@ __global__ __launch_bounds__(256) void k_bin(const float2* __restrict__ pos, con
   if (slot && key != DEAD_KEY) prank[k] = base + (uint32_t)(lane - hl);
 }
 
-__global__ __launch_bounds__(256) void k_scatter(const float2* __restrict__ pos, const float2* __restrict__ vel,
+__device__ __forceinline__ void k_scatter(Blk B_, const float2* __restrict__ pos, const float2* __restrict__ vel,
                                                  const float4* __restrict__ cm, const float4* __restrict__ ext,
                                                  const uint32_t* __restrict__ seq, const DevScalars* __restrict__ sc,
                                                  const uint32_t* __restrict__ pkey, const uint32_t* __restrict__ prank,
                                                  const uint32_t* __restrict__ startKey, float2* __restrict__ pos2,
                                                  float2* __restrict__ vel2, float4* __restrict__ cm2,
                                                  float4* __restrict__ ext2, uint32_t* __restrict__ seq2) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = B_.x * blockDim.x + threadIdx.x;
   if (k >= sc->n_slots) return;
   uint32_t key = pkey[k];
   if (key == DEAD_KEY) return;
@@ -167,16 +167,16 @@ __global__ __launch_bounds__(256) void k_scatter(const float2* __restrict__ pos,
   seq2[d] = seq[k];
 }
 
-__global__ void k_set_slots(DevScalars* sc, const uint32_t* __restrict__ total) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) { sc->n_slots = *total; sc->n_live = *total; }
+__device__ __forceinline__ void k_set_slots(Blk B_, DevScalars* sc, const uint32_t* __restrict__ total) {
+  if (threadIdx.x == 0 && B_.x == 0) { sc->n_slots = *total; sc->n_live = *total; }
 }
 
 int bin_and_sort(tq_sim* s) {
   const int n = s->n_upper;
-  TQ_HIP(hipMemsetAsync(s->countKey, 0, sizeof(uint32_t) * s->K, s->st));
+  if (int e = zero_async(s, s->st, s->countKey, sizeof(uint32_t) * s->K)) return e;
   if (n > 0) {
     int t = time_begin(s, "bin", 16.0 * n);
-    hipLaunchKernelGGL(k_bin, dim3((n + 255) / 256), dim3(256), 0, s->st, s->pos[s->cur], s->sc, s->nx, s->ny,
+    launch<256, 0, k_bin>(s, s->st, dim3((n + 255) / 256), dim3(256), 0, s->pos[s->cur], s->sc, s->nx, s->ny,
                        1.0 / s->h, s->ntx, s->countKey, s->pkey, s->prank);
     time_end(s, t);
     TQ_LAUNCH_CHECK();
@@ -185,14 +185,14 @@ int bin_and_sort(tq_sim* s) {
   if (n > 0) {
     int o = s->cur, d = 1 - s->cur;
     int t = time_begin(s, "scatter", 2.0 * 60 * n);
-    hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, s->st, s->pos[o], s->vel[o], s->cm[o], s->ext[o],
+    launch<256, 0, k_scatter>(s, s->st, dim3((n + 255) / 256), dim3(256), 0, s->pos[o], s->vel[o], s->cm[o], s->ext[o],
                        s->seq[o], s->sc, s->pkey, s->prank, s->startKey, s->pos[d], s->vel[d], s->cm[d], s->ext[d],
                        s->seq[d]);
     time_end(s, t);
     TQ_LAUNCH_CHECK();
     s->cur = d;
   }
-  hipLaunchKernelGGL(k_set_slots, dim3(1), dim3(1), 0, s->st, s->sc, s->startKey + s->K);
+  launch<64, 0, k_set_slots>(s, s->st, dim3(1), dim3(1), 0, s->sc, s->startKey + s->K);
   TQ_LAUNCH_CHECK();
   return 0;
 }
@@ -205,7 +205,7 @@ int bin_and_sort(tq_sim* s) {
 // fixed-point integer atomics, so the sums are exact and order-independent, then finalises
 // the owned nodes: u = sum(w v)/sum(w), + g dt where weighted, zero at solid/domain faces
 // (#paraGrade + #gravidadeEFronteira + #tipos, fisica-cpu.js:198-299).
-__global__ __launch_bounds__(256) void k_p2g(const float2* __restrict__ pos, const float2* __restrict__ vel,
+__device__ __forceinline__ void k_p2g(Blk B_, const float2* __restrict__ pos, const float2* __restrict__ vel,
                                              const float4* __restrict__ cm, const float4* __restrict__ ext,
                                              const uint32_t* __restrict__ startKey, int nx, int ny, int ntx, int K,
                                              double h, const uint8_t* __restrict__ solid, double gdx, double gdy,
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(256) void k_p2g(const float2* __restrict__ pos, con
   __shared__ uint32_t rStart[68], rOff[69];
   __shared__ uint32_t ownStart, ownCount;
   const int tid = threadIdx.x;
-  const int I0 = blockIdx.x * TILE, J0 = blockIdx.y * TILE;
+  const int I0 = B_.x * TILE, J0 = B_.y * TILE;
   const int I1 = min(I0 + TILE, nx), J1 = min(J0 + TILE, ny);
   const double inv = 1 / h;
 
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void k_p2g(const float2* __restrict__ pos, con
     rOff[tid] = cnt;
   }
   if (tid == 0) {
-    uint32_t base = (uint32_t)(blockIdx.y * ntx + blockIdx.x) * 256u;
+    uint32_t base = (uint32_t)(B_.y * ntx + B_.x) * 256u;
     ownStart = startKey[base];
     ownCount = startKey[base + 256] - ownStart;
   }
@@ -397,7 +397,7 @@ int p2g(tq_sim* s, double dt, int gravity) {
   // count, type out; solid in
   double bytes = 44.0 * s->n_upper + 8.0 * (s->nu + s->nv) + (4 + 12 + 4 + 1 + 1) * (double)s->C;
   int t = time_begin(s, "p2g", bytes);
-  hipLaunchKernelGGL(k_p2g, dim3(s->ntx, s->nty), dim3(256), 0, s->st, s->pos[o], s->vel[o], s->cm[o], s->ext[o],
+  launch<256, 0, k_p2g>(s, s->st, dim3(s->ntx, s->nty), dim3(256), 0, s->pos[o], s->vel[o], s->cm[o], s->ext[o],
                      s->startKey, s->nx, s->ny, s->ntx, s->K, s->h, s->solid, s->d.gx * dt, s->d.gy * dt, gravity,
                      s->u, s->uw, s->v, s->vw, s->dens, s->dye, s->countCell, s->tipo, s->sc);
   time_end(s, t);
@@ -440,7 +440,7 @@ __device__ inline void velocidade_em(const float* __restrict__ u, const float* _
   if (vv > vmax) { *wx = vx * vmax / vv; *wy = vy * vmax / vv; } else { *wx = vx; *wy = vy; }
 }
 
-__global__ __launch_bounds__(256) void k_g2p(float2* __restrict__ pos, float2* __restrict__ vel, float4* __restrict__ cm,
+__device__ __forceinline__ void k_g2p(Blk B_, float2* __restrict__ pos, float2* __restrict__ vel, float4* __restrict__ cm,
                                              float4* __restrict__ ext, const DevScalars* __restrict__ scr,
                                              const float* __restrict__ u, const float* __restrict__ v,
                                              const float* __restrict__ dens, const uint32_t* __restrict__ countCell,
@@ -450,7 +450,7 @@ __global__ __launch_bounds__(256) void k_g2p(float2* __restrict__ pos, float2* _
                                              uint16_t* __restrict__ flags) {
 #pragma clang fp contract(off)
   // particles are in cell order: consecutive blocks on one XCD gather from one band of rows
-  const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const uint32_t k = xcd_block(B_.x, B_.gx) * blockDim.x + threadIdx.x;
   double maior = 0;
   if (k < scr->n_slots) {
     const float2 P = pos[k];
@@ -558,7 +558,7 @@ __global__ __launch_bounds__(256) void k_g2p(float2* __restrict__ pos, float2* _
 
 int g2p(tq_sim* s, double dt) {
   const int n = s->n_upper;
-  TQ_HIP(hipMemsetAsync(&s->sc->vmax_bits, 0, sizeof(unsigned long long), s->st));
+  if (int e = zero_async(s, s->st, &s->sc->vmax_bits, sizeof(unsigned long long))) return e;
   if (n > 0) {
     const int o = s->cur;
     double decai = std::exp(-dt / ESPUMA_VIDA);
@@ -567,7 +567,7 @@ int g2p(tq_sim* s, double dt) {
     // grid reads (u, v 4x4 B each, cell 24 B) are cache-resident gathers
     double bytes = (8 + 16 + 16 + 16 + 16) * (double)n + 24.0 * s->C;
     int t = time_begin(s, "g2p", bytes);
-    hipLaunchKernelGGL(k_g2p, dim3((n + 255) / 256), dim3(256), 0, s->st, s->pos[o], s->vel[o], s->cm[o], s->ext[o],
+    launch<256, 0, k_g2p>(s, s->st, dim3((n + 255) / 256), dim3(256), 0, s->pos[o], s->vel[o], s->cm[o], s->ext[o],
                        s->sc, s->u, s->v, s->dens, s->countCell, s->dye, s->sdf, s->nx, s->ny, s->h, dt, decai,
                        mistura, &s->sc->vmax_bits, s->g2p_flags);
     time_end(s, t);
@@ -578,11 +578,11 @@ int g2p(tq_sim* s, double dt) {
 }
 
 // ================================================================== adicionar (fisica-cpu.js:92-111)
-__global__ __launch_bounds__(256) void k_add_flags(const float* __restrict__ list, int count, int check,
+__device__ __forceinline__ void k_add_flags(Blk B_, const float* __restrict__ list, int count, int check,
                                                    const float* __restrict__ sdf, int nx, int ny, double h,
                                                    const uint32_t* __restrict__ countCell, uint32_t* __restrict__ flag) {
 #pragma clang fp contract(off)
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = B_.x * blockDim.x + threadIdx.x;
   if (k >= count) return;
   const double x = list[(size_t)k * 12], y = list[(size_t)k * 12 + 1];
   uint32_t ok = 1;
@@ -596,13 +596,13 @@ __global__ __launch_bounds__(256) void k_add_flags(const float* __restrict__ lis
   flag[k] = ok;
 }
 
-__global__ __launch_bounds__(256) void k_add_append(const float* __restrict__ list, int count,
+__device__ __forceinline__ void k_add_append(Blk B_, const float* __restrict__ list, int count,
                                                     const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off,
                                                     DevScalars* __restrict__ sc, uint32_t seq0, int cap,
                                                     float2* __restrict__ pos, float2* __restrict__ vel,
                                                     float4* __restrict__ cm, float4* __restrict__ ext,
                                                     uint32_t* __restrict__ seq) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = B_.x * blockDim.x + threadIdx.x;
   if (k >= count || !flag[k]) return;
   const uint32_t base = sc->n_slots;
   const uint32_t d = base + off[k];
@@ -615,7 +615,7 @@ __global__ __launch_bounds__(256) void k_add_append(const float* __restrict__ li
   seq[d] = seq0 + off[k];
 }
 
-__global__ void k_add_finish(DevScalars* sc, const uint32_t* __restrict__ total, int cap) {
+__device__ __forceinline__ void k_add_finish(Blk B_, DevScalars* sc, const uint32_t* __restrict__ total, int cap) {
   uint32_t t = *total;
   uint32_t n = sc->n_slots + t;
   sc->n_slots = n < (uint32_t)cap ? n : (uint32_t)cap;
@@ -624,15 +624,15 @@ __global__ void k_add_finish(DevScalars* sc, const uint32_t* __restrict__ total,
 
 int add_particles_dev(tq_sim* s, int count, int check) {
   if (count <= 0) return 0;
-  hipLaunchKernelGGL(k_add_flags, dim3((count + 255) / 256), dim3(256), 0, s->st, s->stage, count, check, s->sdf,
+  launch<256, 0, k_add_flags>(s, s->st, dim3((count + 255) / 256), dim3(256), 0, s->stage, count, check, s->sdf,
                      s->nx, s->ny, s->h, s->countCell, s->addflag);
   TQ_LAUNCH_CHECK();
   if (int e = scan_u32(s, s->addflag, s->pkey, count)) return e;
   const int o = s->cur;
-  hipLaunchKernelGGL(k_add_append, dim3((count + 255) / 256), dim3(256), 0, s->st, s->stage, count, s->addflag, s->pkey,
+  launch<256, 0, k_add_append>(s, s->st, dim3((count + 255) / 256), dim3(256), 0, s->stage, count, s->addflag, s->pkey,
                      s->sc, s->seq_next, s->cap, s->pos[o], s->vel[o], s->cm[o], s->ext[o], s->seq[o]);
   TQ_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_add_finish, dim3(1), dim3(1), 0, s->st, s->sc, s->pkey + count, s->cap);
+  launch<64, 0, k_add_finish>(s, s->st, dim3(1), dim3(1), 0, s->sc, s->pkey + count, s->cap);
   TQ_LAUNCH_CHECK();
   return 0;
 }
@@ -640,10 +640,10 @@ int add_particles_dev(tq_sim* s, int count, int check) {
 // ================================================================== emitirTorneira (principal.js:301-321)
 // Particle k of an emission takes draws 2k+1 (angle) and 2k+2 (radius) of the LCG stream that
 // starts at `seed`; records as the reference builds them (binary64 arithmetic, f32 stores).
-__global__ __launch_bounds__(256) void k_faucet(float* __restrict__ out, int n, uint32_t seed, double x, double y,
+__device__ __forceinline__ void k_faucet(Blk B_, float* __restrict__ out, int n, uint32_t seed, double x, double y,
                                                 double raio, double vx, double vy, float a0, float a1, float a2) {
 #pragma clang fp contract(off)
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = B_.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const uint32_t s1 = lcg_jump(seed, 2ull * (uint64_t)k + 1);
   const uint32_t s2 = s1 * LCG_A + LCG_C;
@@ -665,7 +665,7 @@ __global__ __launch_bounds__(256) void k_faucet(float* __restrict__ out, int n, 
 // through *seed the stream advanced past all n
 int faucet_dev(tq_sim* s, const tq_faucet* f, double vx, double vy, int n, int keep, uint32_t* seed) {
   if (keep > 0) {
-    hipLaunchKernelGGL(k_faucet, dim3((keep + 255) / 256), dim3(256), 0, s->st, s->stage, keep, *seed, f->x, f->y,
+    launch<256, 0, k_faucet>(s, s->st, dim3((keep + 255) / 256), dim3(256), 0, s->stage, keep, *seed, f->x, f->y,
                        f->raio, vx, vy, (float)f->absorcao[0], (float)f->absorcao[1], (float)f->absorcao[2]);
     TQ_LAUNCH_CHECK();
     if (int e = add_particles_dev(s, keep, 1)) return e;
@@ -675,11 +675,11 @@ int faucet_dev(tq_sim* s, const tq_faucet* f, double vx, double vy, int n, int k
 }
 
 // ================================================================== aplicarFerramenta (fisica-cpu.js:129-152)
-__global__ __launch_bounds__(256) void k_tool(float2* __restrict__ pos, float2* __restrict__ vel, float4* __restrict__ ext,
+__device__ __forceinline__ void k_tool(Blk B_, float2* __restrict__ pos, float2* __restrict__ vel, float4* __restrict__ ext,
                                               const DevScalars* __restrict__ sc, int kind, double fx, double fy,
                                               double raio, double fvx, double fvy, float a0, float a1, float a2) {
 #pragma clang fp contract(off)
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = B_.x * blockDim.x + threadIdx.x;
   if (k >= sc->n_slots) return;
   float2 P = pos[k];
   if (P.x != P.x) return;
@@ -708,7 +708,7 @@ __global__ __launch_bounds__(256) void k_tool(float2* __restrict__ pos, float2* 
 int apply_tool_dev(tq_sim* s, const tq_tool* t) {
   if (!t || t->kind == 0 || s->n_upper <= 0) return 0;
   const int o = s->cur;
-  hipLaunchKernelGGL(k_tool, dim3((s->n_upper + 255) / 256), dim3(256), 0, s->st, s->pos[o], s->vel[o], s->ext[o], s->sc,
+  launch<256, 0, k_tool>(s, s->st, dim3((s->n_upper + 255) / 256), dim3(256), 0, s->pos[o], s->vel[o], s->ext[o], s->sc,
                      t->kind, t->x, t->y, t->raio, t->vx, t->vy, (float)t->absorcao[0], (float)t->absorcao[1],
                      (float)t->absorcao[2]);
   TQ_LAUNCH_CHECK();
@@ -716,11 +716,11 @@ int apply_tool_dev(tq_sim* s, const tq_tool* t) {
 }
 
 // ================================================================== live count
-__global__ __launch_bounds__(256) void k_count_live(const float2* __restrict__ pos, const DevScalars* __restrict__ sc,
+__device__ __forceinline__ void k_count_live(Blk B_, const float2* __restrict__ pos, const DevScalars* __restrict__ sc,
                                                     uint32_t* __restrict__ out) {
   // one atomic per block (per-wave atomics on the one counter serialise in the cache)
   __shared__ uint32_t wc[4];
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = B_.x * blockDim.x + threadIdx.x;
   uint32_t live = (k < sc->n_slots && pos[k].x == pos[k].x) ? 1u : 0u;
   unsigned long long b = __ballot(live);
   if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = (uint32_t)__popcll(b);
@@ -736,7 +736,7 @@ int count_live(tq_sim* s, int* n) {
   uint32_t* tmp = s->addflag;
   TQ_HIP(hipMemsetAsync(tmp, 0, sizeof(uint32_t), s->st));
   if (s->n_upper > 0) {
-    hipLaunchKernelGGL(k_count_live, dim3((s->n_upper + 255) / 256), dim3(256), 0, s->st, s->pos[s->cur], s->sc, tmp);
+    launch<256, 0, k_count_live>(s, s->st, dim3((s->n_upper + 255) / 256), dim3(256), 0, s->pos[s->cur], s->sc, tmp);
     TQ_LAUNCH_CHECK();
   }
   uint32_t h = 0;
@@ -748,7 +748,7 @@ int count_live(tq_sim* s, int* n) {
 
 // the status words of a non-blocking readback: the count is cleared and vMax copied (one thread);
 // k_count_live adds the live particles behind it
-__global__ __launch_bounds__(64) void k_status_begin(unsigned long long* __restrict__ st,
+__device__ __forceinline__ void k_status_begin(Blk B_, unsigned long long* __restrict__ st,
                                                      const DevScalars* __restrict__ sc) {
   if (threadIdx.x == 0) {
     st[0] = 0ull;
@@ -757,10 +757,10 @@ __global__ __launch_bounds__(64) void k_status_begin(unsigned long long* __restr
 }
 
 int status_enqueue(tq_sim* s) {
-  hipLaunchKernelGGL(k_status_begin, dim3(1), dim3(64), 0, s->st, (unsigned long long*)s->status_dev,
+  launch<64, 0, k_status_begin>(s, s->st, dim3(1), dim3(64), 0, (unsigned long long*)s->status_dev,
                      (const DevScalars*)s->sc);
   if (s->n_upper > 0)
-    hipLaunchKernelGGL(k_count_live, dim3((s->n_upper + 255) / 256), dim3(256), 0, s->st, s->pos[s->cur], s->sc,
+    launch<256, 0, k_count_live>(s, s->st, dim3((s->n_upper + 255) / 256), dim3(256), 0, s->pos[s->cur], s->sc,
                        (uint32_t*)s->status_dev);
   TQ_LAUNCH_CHECK();
   return 0;
@@ -768,9 +768,9 @@ int status_enqueue(tq_sim* s) {
 
 // ================================================================== atualizarParedes (fisica-cpu.js:80-85)
 // Cena.celulaSolida (cena.js:119-127): the sum of the cell's 4 SDF samples, in binary64, < 0.
-__global__ __launch_bounds__(256) void k_solid(const float* __restrict__ sdf, int nx, int ny, uint8_t* __restrict__ solid) {
+__device__ __forceinline__ void k_solid(Blk B_, const float* __restrict__ sdf, int nx, int ny, uint8_t* __restrict__ solid) {
 #pragma clang fp contract(off)
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = B_.x * blockDim.x + threadIdx.x;
   if (c >= nx * ny) return;
   const int i = c % nx, j = c / nx, w = 2 * nx, a = 2 * i, b = 2 * j;
   double s = (double)sdf[(size_t)b * w + a] + (double)sdf[(size_t)b * w + a + 1];
@@ -780,7 +780,7 @@ __global__ __launch_bounds__(256) void k_solid(const float* __restrict__ sdf, in
 }
 
 int solid_cells(tq_sim* s) {
-  hipLaunchKernelGGL(k_solid, dim3((s->C + 255) / 256), dim3(256), 0, s->st, s->sdf, s->nx, s->ny, s->solid);
+  launch<256, 0, k_solid>(s, s->st, dim3((s->C + 255) / 256), dim3(256), 0, s->sdf, s->nx, s->ny, s->solid);
   TQ_LAUNCH_CHECK();
   return 0;
 }

@@ -127,7 +127,7 @@ __device__ inline void bub_add(double (&sl)[BSLOTS], const int* __restrict__ lab
 // Block sum of the first nv of v[]; thread 0 stores partial `bid` of each quantity.  nv must
 // be uniform over the block; `sh` holds NMAX*16 doubles.
 template <int NMAX>
-__device__ inline void store_partials(double (&v)[NMAX], int nv, double* __restrict__ part, double* sh) {
+__device__ inline void store_partials(const Blk& B_, double (&v)[NMAX], int nv, double* __restrict__ part, double* sh) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
   for (int k = 0; k < NMAX; k++)
@@ -139,7 +139,7 @@ __device__ inline void store_partials(double (&v)[NMAX], int nv, double* __restr
   if ((int)threadIdx.x < nv) {
     double t = 0;
     for (int q = 0; q < nw; q++) t += sh[threadIdx.x * 16 + q];
-    part[(size_t)threadIdx.x * MAX_RED_BLOCKS + blockIdx.x + blockIdx.y * gridDim.x] = t;
+    part[(size_t)threadIdx.x * MAX_RED_BLOCKS + B_.x + B_.y * B_.gx] = t;
   }
   __syncthreads();  // `sh` is free again for the caller's next reduction
 }
@@ -188,7 +188,7 @@ __device__ inline void block_sum_all(double (&v)[NV], double* sh, double* res) {
   __syncthreads();
 }
 
-__device__ inline bool first_block() { return blockIdx.x == 0 && blockIdx.y == 0; }
+__device__ inline bool first_block(const Blk& B_) { return B_.x == 0 && B_.y == 0; }
 
 // ================================================================== reductions folded into consumers
 struct BubArgs {
@@ -203,7 +203,7 @@ struct BubArgs {
 // accumulator) and Polak-Ribiere beta = max(0, (r.z - r.z_old) / rz_prev), 0 on the first
 // iteration.  Block 0 stores z of this parity, clears the accumulator the next z dots use
 // (the other parity) and records rz.
-__device__ inline double fold_zdots(const double* __restrict__ part, const double* __restrict__ bpartZ, int nb,
+__device__ inline double fold_zdots(const Blk& B_, const double* __restrict__ part, const double* __restrict__ bpartZ, int nb,
                                     int par, int first, const BubArgs& B, double* sh, double* res, double* zs,
                                     DevScalars* __restrict__ sc) {
   const int na = sc->active_bubbles;
@@ -219,7 +219,7 @@ __device__ inline double fold_zdots(const double* __restrict__ part, const doubl
   const double* Rc = bv(B.bvec, BV_R0 + 1 - par);
   const double* accZ = bv(B.bvec, BV_ACCZ0 + par);
   double* accZnext = bv(B.bvec, BV_ACCZ0 + 1 - par);
-  const bool b0 = first_block();
+  const bool b0 = first_block(B_);
   double bacc[2] = {0, 0};
   for (int k = threadIdx.x; k < na; k += blockDim.x) {
     const int b = B.activeList[k];
@@ -248,7 +248,7 @@ __device__ inline double fold_zdots(const double* __restrict__ part, const doubl
 // (slot totals, else the atomic accumulator) and their new residuals (rbs[] for the slotted
 // ones).  Block 0 stores p, Ap, y += alpha p, r -= alpha Ap of the bubbles, the bubble part of
 // |r|^2 and clears the accumulator of the next spmv (the other parity).
-__device__ inline double fold_spmv(const double* __restrict__ part, const double* __restrict__ bpartA, int nb, int par,
+__device__ inline double fold_spmv(const Blk& B_, const double* __restrict__ part, const double* __restrict__ bpartA, int nb, int par,
                                    const BubArgs& B, double* sh, double* res, double* aps, double* rbs,
                                    DevScalars* __restrict__ sc) {
   const int na = sc->active_bubbles;
@@ -279,7 +279,7 @@ __device__ inline double fold_spmv(const double* __restrict__ part, const double
   block_sum_all<1>(bacc, sh, res);
   const double pap = tot[0] + bacc[0];
   const double alpha = fabs(pap) < 1e-30 ? 0.0 : rz / pap;
-  const bool b0 = first_block();
+  const bool b0 = first_block(B_);
   double* Pn = bv(B.bvec, BV_P0 + par);
   double* Rn = bv(B.bvec, BV_R0 + par);
   double* Yb = bv(B.bvec, BV_Y);
@@ -323,7 +323,7 @@ __device__ inline double bubble_r(const BubArgs& B, int b, int par, double alpha
 // the norm and stop test after iteration k's update (pressao-cpu.js:382-386), at the start of
 // level 0's down kernel; block 0 records them, the iteration count and the next V-cycle input
 // scale.  Returns true when the solve stops.
-__device__ inline bool fold_update(const double* __restrict__ part, int nb, int par, double* sh, double* res,
+__device__ inline bool fold_update(const Blk& B_, const double* __restrict__ part, int nb, int par, double* sh, double* res,
                                    DevScalars* __restrict__ sc) {
   double tot[1];
   sum_partials<1>(part, nb, 1, tot, sh, res);
@@ -332,7 +332,7 @@ __device__ inline bool fold_update(const double* __restrict__ part, int nb, int 
   const int itn = sc->itp[1 - par] + 1;
   bool stop = itn >= sc->maxit;
   if (sc->tol > 0 && (nr == 0 || nr <= sc->tol * sc->normb)) stop = true;
-  if (first_block() && threadIdx.x == 0) {
+  if (first_block(B_) && threadIdx.x == 0) {
     sc->norm2 = n2;
     sc->norm_last = nr;
     sc->itp[par] = itn;
@@ -347,13 +347,13 @@ __device__ inline bool fold_update(const double* __restrict__ part, int nb, int 
 // (the consumers' redundant sums would otherwise grow as the square of the block count).
 constexpr int PRE_R = 64;
 constexpr int FOLD_DIRECT = 1280;  // consumers re-sum up to this many partials themselves
-__global__ __launch_bounds__(256) void k_prereduce(const double* __restrict__ src, int nb, int nv,
+__device__ __forceinline__ void k_prereduce(Blk B_, const double* __restrict__ src, int nb, int nv,
                                                    const double* __restrict__ bsrc, double* __restrict__ dst,
                                                    double* __restrict__ bdst, const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[(2 + BSLOTS) * 16];
   const int per = (nb + PRE_R - 1) / PRE_R;
-  const int b0 = blockIdx.x * per, b1 = b0 + per < nb ? b0 + per : nb;
+  const int b0 = B_.x * per, b1 = b0 + per < nb ? b0 + per : nb;
   const int ns = bsrc ? nslots(sc->active_bubbles) : 0;
   double v[2 + BSLOTS];
 #pragma unroll
@@ -368,8 +368,8 @@ __global__ __launch_bounds__(256) void k_prereduce(const double* __restrict__ sr
   }
   block_sum<2 + BSLOTS>(v, sh);
   if (threadIdx.x == 0) {
-    for (int k = 0; k < nv; k++) dst[ps_off(k) + blockIdx.x] = v[k];
-    for (int k = 0; k < ns; k++) bdst[ps_off(k) + blockIdx.x] = v[2 + k];
+    for (int k = 0; k < nv; k++) dst[ps_off(k) + B_.x] = v[k];
+    for (int k = 0; k < ns; k++) bdst[ps_off(k) + B_.x] = v[2 + k];
   }
 }
 
@@ -387,13 +387,13 @@ struct PyrPlan {
   uint8_t* mask[12];
 };
 
-__global__ __launch_bounds__(1024) void k_setup_pyr(const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
+__device__ __forceinline__ void k_setup_pyr(Blk B_, const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
                                                     const uint32_t* __restrict__ facesB, int ar,
-                                                    uint8_t* __restrict__ opMask, PyrPlan P) {
+                                                    uint8_t* __restrict__ opMask, const PyrPlan& P) {
   __shared__ uint8_t ta[PYR_T * PYR_T], tb[PYR_T * PYR_T / 4];
   const int tid = threadIdx.x;
   const int nx = P.nx[0], ny = P.ny[0];
-  const int ox = blockIdx.x * PYR_T, oy = blockIdx.y * PYR_T;
+  const int ox = B_.x * PYR_T, oy = B_.y * PYR_T;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int e = tid + 1024 * k;
@@ -450,9 +450,9 @@ __global__ __launch_bounds__(1024) void k_setup_pyr(const uint8_t* __restrict__ 
 }
 
 // tiposGrossos (pressao-cpu.js:63-81): air if any child is air, else water if any is water
-__global__ __launch_bounds__(256) void k_coarse_type(const uint8_t* __restrict__ fm, int fnx, int fny,
+__device__ __forceinline__ void k_coarse_type(Blk B_, const uint8_t* __restrict__ fm, int fnx, int fny,
                                                      uint8_t* __restrict__ cm, int cnx, int cny) {
-  const int C = blockIdx.x * blockDim.x + threadIdx.x;
+  const int C = B_.x * blockDim.x + threadIdx.x;
   if (C >= cnx * cny) return;
   const int J = C / cnx, I = C - J * cnx;
   bool temAr = false, temAgua = false;
@@ -477,9 +477,9 @@ struct VizPlan {
   int nx[12], ny[12];
   uint8_t* mask[12];
 };
-__global__ __launch_bounds__(256) void k_viz_all(VizPlan P, uint8_t* __restrict__ saved, int force,
+__device__ __forceinline__ void k_viz_all(Blk B_, const VizPlan& P, uint8_t* __restrict__ saved, int force,
                                                  DevScalars* __restrict__ sc) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = B_.x * blockDim.x + threadIdx.x;
   if (g >= P.off[P.L]) return;
   int l = P.v0;
   while (g >= P.off[l + 1]) l++;
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(256) void k_viz_all(VizPlan P, uint8_t* __restrict_
 }
 
 // bubble vectors, the active list (labels with water faces, in label order) and their slots
-__global__ __launch_bounds__(1024) void k_bub_init(const uint32_t* __restrict__ facesB, const float* __restrict__ yB32,
+__device__ __forceinline__ void k_bub_init(Blk B_, const uint32_t* __restrict__ facesB, const float* __restrict__ yB32,
                                                    int ar, double* __restrict__ bvec, int* __restrict__ activeList,
                                                    int* __restrict__ labelSlot, DevScalars* __restrict__ sc) {
   __shared__ int cnt[16];
@@ -551,7 +551,7 @@ __global__ __launch_bounds__(1024) void k_bub_init(const uint32_t* __restrict__ 
 
 // a restart of the compensated iterate: the bubble directions and the fixed-point accumulators
 // start from zero as at the solve's set-up; y (the bubble unknowns) is kept
-__global__ __launch_bounds__(1024) void k_bub_restart(double* __restrict__ bvec) {
+__device__ __forceinline__ void k_bub_restart(Blk B_, double* __restrict__ bvec) {
   for (int r = threadIdx.x; r < MAX_BOLHAS; r += blockDim.x)
     for (int q = BV_AP; q < BV_N; q++) bv(bvec, q)[r] = 0.0;
 }
@@ -560,7 +560,7 @@ __global__ __launch_bounds__(1024) void k_bub_restart(double* __restrict__ bvec)
 // q := 0 off water, bubble cells := y (difundir); r = -div - A q; bubble rows A_b [q;y].
 // qmode 0: q alone (the reference's binary32 iterate); 1: compensated iterate q + qlo starting
 // with qlo = 0; 2: a restart of the compensated iterate (its residual from q + qlo in binary64).
-__global__ __launch_bounds__(256) void k_pcg_init(const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
+__device__ __forceinline__ void k_pcg_init(Blk B_, const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
                                                   const float* __restrict__ div, float* __restrict__ q,
                                                   float* __restrict__ qlo, int qmode,
                                                   float* __restrict__ r, float* __restrict__ pbuf0, int nx, int C,
@@ -586,7 +586,7 @@ __global__ __launch_bounds__(256) void k_pcg_init(const uint8_t* __restrict__ op
     }
     return (double)qe(d, md);
   };
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = B_.x * blockDim.x + threadIdx.x;
   if (c < C) {
     const uint8_t m = opMask[c];
     const float qc = qe(c, m);
@@ -618,14 +618,14 @@ __global__ __launch_bounds__(256) void k_pcg_init(const uint8_t* __restrict__ op
     q[c] = qc;
     if (qmode == 1 || (qmode == 2 && mask_type(m) != AGUA)) qlo[c] = 0.f;
   }
-  store_partials<5>(v, 5, partials, sh);
-  if (ns > 0) store_partials<BSLOTS>(sl, ns, bpart, sh);
+  store_partials<5>(B_, v, 5, partials, sh);
+  if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, bpart, sh);
 }
 
 // restart = 0 starts a solve (tolerance, iteration cap and the mixed modes' scale numerator are
 // set here, as arguments, so no host copy precedes the solve); restart = 1 restarts the
 // compensated iterate from its true residual, adding the finished segment's iterations to the count.
-__global__ __launch_bounds__(FIN_THREADS) void k_fin_init(const uint32_t* __restrict__ facesB, double* __restrict__ bvec,
+__device__ __forceinline__ void k_fin_init(Blk B_, const uint32_t* __restrict__ facesB, double* __restrict__ bvec,
                                                           const double* __restrict__ rhsB,
                                                           const int* __restrict__ activeList,
                                                           const int* __restrict__ labelSlot,
@@ -853,10 +853,10 @@ __device__ inline void store_cells(const Cells4& c, SX* __restrict__ dst, int ox
 }
 
 // tile of wave w of block (bx, by): 2x2 tiles per 256-thread block
-__device__ inline void block_tile(int* tx, int* ty) {
+__device__ inline void block_tile(const Blk& B_, int* tx, int* ty) {
   const int w = threadIdx.x >> 6;
   int bx, by;
-  xcd_block2(&bx, &by);
+  xcd_block2(B_, &bx, &by);
   *tx = 2 * bx + (w & 1);
   *ty = 2 * by + (w >> 1);
 }
@@ -870,7 +870,7 @@ __device__ inline void block_tile(int* tx, int* ty) {
 // walking its groups after one fold.
 constexpr int IN_A0_GRID = 1280;  // 5 blocks per CU at this kernel's 92 VGPRs
 template <class SB, bool UPD, bool PASSA>
-__global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMask, const uint8_t* __restrict__ mg0,
+__device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMask, const uint8_t* __restrict__ mg0,
                                                const uint32_t* __restrict__ rot, const uint32_t* __restrict__ cellsB,
                                                const float* __restrict__ p, const float* __restrict__ ap,
                                                float* __restrict__ q, float* __restrict__ qlo, int comp,
@@ -884,7 +884,7 @@ __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMas
   __shared__ float Tt[4][TCELLS];
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], aps[BSLOTS], rbs[BSLOTS];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int vb = xcd_block(blockIdx.x, gridDim.x);
+  int vb = xcd_block(B_.x, B_.gx);
   int ox, oy;
   bool valid;
   const double mean = UPD ? 0.0 : sc->proj_mean;
@@ -925,7 +925,7 @@ __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMas
   // UPD: alpha and the bubbles' new residuals from spmv's partials (fold_spmv, while the loads
   // above are in flight); else (the first cycle) the current bubble residuals are those of
   // parity 1 - par
-  const double alpha = UPD ? fold_spmv(partA, bpartA, nbs, par, B, sh, res, aps, rbs, sc) : 0.0;
+  const double alpha = UPD ? fold_spmv(B_, partA, bpartA, nbs, par, B, sh, res, aps, rbs, sc) : 0.0;
   const double beta = UPD ? sc->beta : 0.0;
   for (;;) {
     Cells4 cc;
@@ -975,11 +975,11 @@ __global__ __launch_bounds__(256) void k_in_a0(const uint8_t* __restrict__ opMas
       store_cells<float>(cc, x0, ox, oy, nx, ny, lane);
       wave_sync();  // the wave's tile is free for its next group
     }
-    vb += gridDim.x;
+    vb += B_.gx;
     if (vb >= nvb) break;
     load(vb);
   }
-  if (UPD) store_partials<1>(rr, 1, partRR, sh);
+  if (UPD) store_partials<1>(B_, rr, 1, partRR, sh);
 }
 
 struct ZArgs {  // level-0 outputs: z = MG result and the dots r.z, r.z_old
@@ -1014,18 +1014,18 @@ __device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, f
 }
 
 // z and the dots when level 0 itself is solved inside the workgroup (small grids)
-__global__ __launch_bounds__(256) void k_zdots0(const float* __restrict__ x0, ZArgs za, int C,
+__device__ __forceinline__ void k_zdots0(Blk B_, const float* __restrict__ x0, ZArgs za, int C,
                                                 const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[BSLOTS * 16];
   const int ns = nslots(sc->active_bubbles);
   double dv[2] = {0, 0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = B_.x * blockDim.x + threadIdx.x;
   const float inv = za.scaled ? 1.f / *za.mgs : 1.f;
   if (c < C) z_cell(za, c, za.opMask[c], za.r[c], za.zold[c], x0[c], inv, dv, sl);
-  store_partials<2>(dv, 2, za.partials, sh);
-  if (ns > 0) store_partials<BSLOTS>(sl, ns, za.bpart, sh);
+  store_partials<2>(B_, dv, 2, za.partials, sh);
+  if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, za.bpart, sh);
 }
 
 // ================================================================== up on level 0 (unfused)
@@ -1034,7 +1034,7 @@ __global__ __launch_bounds__(256) void k_zdots0(const float* __restrict__ x0, ZA
 // prolongar (pressao-cpu.js:191-214) folded into the snapshot of the first up pass (offset 8,
 // colours 1,0): water cells read x + P e, with the coarse values of the tile's 10x10 patch
 template <class SX, class SB, class SC>
-__global__ __launch_bounds__(256) void k_prolong_pass(const SX* __restrict__ src, SX* __restrict__ dst,
+__device__ __forceinline__ void k_prolong_pass(Blk B_, const SX* __restrict__ src, SX* __restrict__ dst,
                                                       const SB* __restrict__ b, const uint8_t* __restrict__ mask,
                                                       int nx, int ny, int ntx, int nty, const SC* __restrict__ cx,
                                                       const uint8_t* __restrict__ cmk, int cnx, int cny,
@@ -1044,7 +1044,7 @@ __global__ __launch_bounds__(256) void k_prolong_pass(const SX* __restrict__ src
   __shared__ float CPt[4][100];
   const int lane = threadIdx.x & 63;
   int tx, ty;
-  block_tile(&tx, &ty);
+  block_tile(B_, &tx, &ty);
   if (tx >= ntx || ty >= nty) return;
   const int ox = tx * 16 - 8, oy = ty * 16 - 8;
   float* T = Tt[threadIdx.x >> 6];
@@ -1114,7 +1114,7 @@ __global__ __launch_bounds__(256) void k_prolong_pass(const SX* __restrict__ src
 
 // last up pass of level 0 (offset 0, colours 1,0) writing z and the dot partials
 template <class SB>
-__global__ __launch_bounds__(256) void k_pass_z(const float* __restrict__ src, const SB* __restrict__ b,
+__device__ __forceinline__ void k_pass_z(Blk B_, const float* __restrict__ src, const SB* __restrict__ b,
                                                 const uint8_t* __restrict__ mask, int nx, int ny, int ntx, int nty,
                                                 ZArgs za, const DevScalars* __restrict__ sc) {
   if (sc->done) return;
@@ -1123,7 +1123,7 @@ __global__ __launch_bounds__(256) void k_pass_z(const float* __restrict__ src, c
   const int lane = threadIdx.x & 63;
   const int ns = nslots(sc->active_bubbles);
   int tx, ty;
-  block_tile(&tx, &ty);
+  block_tile(B_, &tx, &ty);
   double dv[2] = {0, 0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (tx < ntx && ty < nty) {
@@ -1160,8 +1160,8 @@ __global__ __launch_bounds__(256) void k_pass_z(const float* __restrict__ src, c
                  (cc.m[cq][rw] & 32) ? cc.v[cq][rw] : 0.f, inv, dv, sl);
       }
   }
-  store_partials<2>(dv, 2, za.partials, sh);
-  if (ns > 0) store_partials<BSLOTS>(sl, ns, za.bpart, sh);
+  store_partials<2>(B_, dv, 2, za.partials, sh);
+  if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, za.bpart, sh);
 }
 
 // ================================================================== large levels: one kernel per pass
@@ -1178,7 +1178,7 @@ static int pass_fold_grid(const tq_sim* s) {
 // one smoothing pass of level l (offset desl, colour order from C0); on level 0 of an iteration
 // it first does the norm and stop test (fold_update)
 template <class SX, class SB, int C0>
-__global__ __launch_bounds__(256) void k_pass(const SX* __restrict__ src, SX* __restrict__ dst,
+__device__ __forceinline__ void k_pass(Blk B_, const SX* __restrict__ src, SX* __restrict__ dst,
                                               const SB* __restrict__ b, const uint8_t* __restrict__ mask, int nx,
                                               int ny, int desl, int ntx, int nty, int vgx, int nvb,
                                               const double* __restrict__ partRR, int nbu, int par, int fold,
@@ -1189,7 +1189,7 @@ __global__ __launch_bounds__(256) void k_pass(const SX* __restrict__ src, SX* __
   float* T = Tt[w];
   // one-dimensional grid over the 2x2-tile groups (vgx per row, nvb in all); a folding launch
   // is capped and its blocks walk several groups after one fold
-  int vb = xcd_block(blockIdx.x, gridDim.x);
+  int vb = xcd_block(B_.x, B_.gx);
   bool valid;
   int ox, oy;
   Own4 own;
@@ -1207,7 +1207,7 @@ __global__ __launch_bounds__(256) void k_pass(const SX* __restrict__ src, SX* __
   load(vb);
   if (fold) {
     __shared__ double fsh[16], fres[1];
-    if (fold_update(partRR, nbu, par, fsh, fres, sc)) return;
+    if (fold_update(B_, partRR, nbu, par, fsh, fres, sc)) return;
   }
   for (;;) {
     if (valid) {
@@ -1218,7 +1218,7 @@ __global__ __launch_bounds__(256) void k_pass(const SX* __restrict__ src, SX* __
       store_cells<SX>(cc, dst, ox, oy, nx, ny, lane);
       wave_sync();  // the wave's tile is free for its next group
     }
-    vb += gridDim.x;
+    vb += B_.gx;
     if (vb >= nvb) break;
     load(vb);
   }
@@ -1229,7 +1229,7 @@ __global__ __launch_bounds__(256) void k_pass(const SX* __restrict__ src, SX* __
 // 1/4 3/4 3/4 1/4), from x_B on 36x36, in LDS.  Then wave 0 runs the coarse level's pass A
 // (offset 0 = this tile) from x = 0 unless the coarse level is solved otherwise.
 template <class SXF, class SBF, class SCB, class SCX, bool PASSA>
-__global__ __launch_bounds__(256) void k_restrict_a(const SXF* __restrict__ fx, const SBF* __restrict__ fb,
+__device__ __forceinline__ void k_restrict_a(Blk B_, const SXF* __restrict__ fx, const SBF* __restrict__ fb,
                                                     const uint8_t* __restrict__ fm, int fnx, int fny,
                                                     SCB* __restrict__ cb, SCX* __restrict__ cxo,
                                                     const uint8_t* __restrict__ cmk, int cnx, int cny,
@@ -1238,7 +1238,7 @@ __global__ __launch_bounds__(256) void k_restrict_a(const SXF* __restrict__ fx, 
   __shared__ float X[36 * 36], R[34 * 34], Bt[256], T[TCELLS];
   const int tid = threadIdx.x;
   int bxl, byl;
-  xcd_block2(&bxl, &byl);
+  xcd_block2(B_, &bxl, &byl);
   const int CI0 = bxl * 16, CJ0 = byl * 16;
   const int fx0 = 2 * CI0 - 2, fy0 = 2 * CJ0 - 2;  // X origin
   // every global load first: the residual cells' masks and b, the coarse masks, then x_B
@@ -1349,7 +1349,7 @@ constexpr int BRA_WAVES = 9;
 constexpr int XBW = 48;  // x_B window: fine cells 2*CI0 - 8 .. 2*CI0 + 39
 
 template <class SXF, class SBF, class SCB, class SCX, bool PASSA>
-__global__ __launch_bounds__(64 * BRA_WAVES) void k_bra(const SXF* __restrict__ fx, SXF* __restrict__ fxb,
+__device__ __forceinline__ void k_bra(Blk B_, const SXF* __restrict__ fx, SXF* __restrict__ fxb,
                                                        const SBF* __restrict__ fb, const uint8_t* __restrict__ fm,
                                                        int fnx, int fny, SCB* __restrict__ cb, SCX* __restrict__ cxo,
                                                        const uint8_t* __restrict__ cmk, int cnx, int cny,
@@ -1362,7 +1362,7 @@ __global__ __launch_bounds__(64 * BRA_WAVES) void k_bra(const SXF* __restrict__ 
   __shared__ float Bt[256];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   int bxl, byl;
-  xcd_block2(&bxl, &byl);
+  xcd_block2(B_, &bxl, &byl);
   const int CI0 = bxl * 16, CJ0 = byl * 16;
   const int wx0 = 2 * CI0 - 8, wy0 = 2 * CJ0 - 8;   // window origin (fine)
   const int ox = wx0 + 16 * (wave % 3), oy = wy0 + 16 * (wave / 3);  // this wave's B tile
@@ -1398,7 +1398,7 @@ __global__ __launch_bounds__(64 * BRA_WAVES) void k_bra(const SXF* __restrict__ 
   fill_tile<SXF>(T, fx, ox, oy, fnx, fny, lane);
   if (fold) {  // level 0 of an iteration: the norm and stop test (fold_update) while the loads land
     __shared__ double fsh[16], fres[1];
-    if (fold_update(partRR, nbu, par, fsh, fres, sc)) return;
+    if (fold_update(B_, partRR, nbu, par, fsh, fres, sc)) return;
   }
   wave_sync();
   // ---- pass B on this wave's tile (colours 0,1), into the window and (own cells) to memory
@@ -1492,7 +1492,7 @@ __global__ __launch_bounds__(64 * BRA_WAVES) void k_bra(const SXF* __restrict__ 
 constexpr int CW = 32;  // pass-C window: cells 16a - 8 .. 16a + 23
 
 template <class SX, class SB, class SC, bool ZD>
-__global__ __launch_bounds__(256) void k_pcd(const SX* __restrict__ xb, SX* __restrict__ xout,
+__device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __restrict__ xout,
                                              const SB* __restrict__ b, const uint8_t* __restrict__ mask, int nx,
                                              int ny, const SC* __restrict__ cx, const uint8_t* __restrict__ cmk,
                                              int cnx, int cny, ZArgs za, const DevScalars* __restrict__ sc) {
@@ -1503,7 +1503,7 @@ __global__ __launch_bounds__(256) void k_pcd(const SX* __restrict__ xb, SX* __re
   __shared__ double sh[BSLOTS * 16];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   int bxl, byl;
-  xcd_block2(&bxl, &byl);
+  xcd_block2(B_, &bxl, &byl);
   const int ax = bxl * 16, ay = byl * 16;  // tile T
   const int wx0 = ax - 8, wy0 = ay - 8;                      // window origin
   const int ox = wx0 + 16 * (wave & 1), oy = wy0 + 16 * (wave >> 1);  // this wave's C tile
@@ -1630,8 +1630,8 @@ __global__ __launch_bounds__(256) void k_pcd(const SX* __restrict__ xb, SX* __re
   }
   if (ZD) {
     const int ns = nslots(sc->active_bubbles);
-    store_partials<2>(dv, 2, za.partials, sh);
-    if (ns > 0) store_partials<BSLOTS>(sl, ns, za.bpart, sh);
+    store_partials<2>(B_, dv, 2, za.partials, sh);
+    if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, za.bpart, sh);
   }
 }
 
@@ -1640,11 +1640,11 @@ __global__ __launch_bounds__(256) void k_pcd(const SX* __restrict__ xb, SX* __re
 // x = G b for a matrix G fixed by the level's masks.  Column j is the result for b = e_j: one
 // wavefront per column runs the sweeps on its own copy of the level (LDS), cells in registers.
 // Columns of cells that are not water are zero (their b is never read).
-__global__ __launch_bounds__(64) void k_coarsest_op(const uint8_t* __restrict__ mask, int nx, int n,
+__device__ __forceinline__ void k_coarsest_op(Blk B_, const uint8_t* __restrict__ mask, int nx, int n,
                                                     float* __restrict__ G, const DevScalars* __restrict__ sc) {
   if (!sc->op_dirty) return;  // masks as at the last build: G is current
   __shared__ float x[DENSE_MAX];
-  const int j = blockIdx.x, lane = threadIdx.x;
+  const int j = B_.x, lane = threadIdx.x;
   for (int c = lane; c < n; c += 64) x[c] = 0.f;
   if (mask_type(mask[j]) != AGUA) {
     for (int c = lane; c < n; c += 64) G[(size_t)c * n + j] = 0.f;
@@ -1924,7 +1924,7 @@ __device__ __forceinline__ void wg_load_masks(const WGPlan& P, char* pool) {
 // Levels lc..L-1 from LDS in one workgroup.  Level lc arrives with b and (unless it is the
 // coarsest) x after its pass A; the result of level lc's V-cycle is written to out_lc, which may
 // be x_lc (every input is in LDS before the first output is stored).
-__global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __restrict__ b_lc, const float* x_lc,
+__device__ __forceinline__ void k_coarse_wg(Blk B_, const WGPlan& P, const float* __restrict__ b_lc, const float* x_lc,
                                                     float* out_lc, const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ __attribute__((aligned(16))) char pool[WG_POOL];
@@ -1950,11 +1950,11 @@ __global__ __launch_bounds__(1024) void k_coarse_wg(WGPlan P, const float* __res
 // (C b)_i = sum_j (V e_i)_j b_j = (V b)_i.  Rows of cells that are not water are zero: V
 // neither reads b nor writes x there.
 constexpr int VOP_WAVES = 4;
-__global__ __launch_bounds__(64 * VOP_WAVES) void k_vcycle_op(WGPlan P, int ld, float* __restrict__ C,
+__device__ __forceinline__ void k_vcycle_op(Blk B_, const WGPlan& P, int ld, float* __restrict__ C,
                                                              DevScalars* __restrict__ sc) {
   if (!sc->op_dirty) return;  // masks as at the last build: C is current
   extern __shared__ __attribute__((aligned(16))) char vpool[];
-  const int lc = P.lc, n = P.nx[lc] * P.ny[lc], j = blockIdx.x;
+  const int lc = P.lc, n = P.nx[lc] * P.ny[lc], j = B_.x;
   float* row = C + (size_t)j * ld;
   if (mask_type(P.mask[lc][j]) != AGUA) {
     for (int c = threadIdx.x; c < ld; c += blockDim.x) row[c] = 0.f;
@@ -1969,18 +1969,18 @@ __global__ __launch_bounds__(64 * VOP_WAVES) void k_vcycle_op(WGPlan P, int ld, 
   for (int c = threadIdx.x; c < ld; c += blockDim.x) row[c] = c < n ? X[c] : 0.f;
 }
 
-__global__ __launch_bounds__(64) void k_op_count(DevScalars* __restrict__ sc) {
+__device__ __forceinline__ void k_op_count(Blk B_, DevScalars* __restrict__ sc) {
   if (threadIdx.x == 0 && sc->op_dirty) sc->op_builds += 1;
 }
 
 // x = C b on level lc: VOP_ROWS rows per wavefront, each lane reading float4 columns
 constexpr int VOP_ROWS = 2;
-__global__ __launch_bounds__(256) void k_vcycle_apply(const float* __restrict__ C, int ld, int n,
+__device__ __forceinline__ void k_vcycle_apply(Blk B_, const float* __restrict__ C, int ld, int n,
                                                       const float* __restrict__ b, float* __restrict__ x,
                                                       const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   const int lane = threadIdx.x & 63;
-  const int r0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * VOP_ROWS;
+  const int r0 = (B_.x * 4 + (threadIdx.x >> 6)) * VOP_ROWS;
   if (r0 >= n) return;
   const int nk = ld >> 8;  // float4 columns per lane (ld is a multiple of 256, at most 2048)
   float4 cv[VOP_ROWS][8];
@@ -2023,7 +2023,7 @@ constexpr int SPMV_HW = SPMV_TW + 2, SPMV_HN = SPMV_HW * (SPMV_TH + 2);  // with
 constexpr int SPMV_LOADS = (SPMV_HN + 255) / 256;    // halo-tile cells per thread (5)
 constexpr int SPMV_GRID = 1024;  // 4 blocks per CU: the kernel is held to 128 VGPRs (4 waves per SIMD)
 static int spmv_tiles(int nx, int ny) { return ((nx + SPMV_TW - 1) / SPMV_TW) * ((ny + SPMV_TH - 1) / SPMV_TH); }
-__global__ __launch_bounds__(256, 4) void k_spmv(const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
+__device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
                                               const float* __restrict__ z, const float* __restrict__ pold,
                                               float* __restrict__ pnew, float* __restrict__ ap, int nx, int ny,
                                               BubArgs B, const double* __restrict__ partZ,
@@ -2036,7 +2036,7 @@ __global__ __launch_bounds__(256, 4) void k_spmv(const uint8_t* __restrict__ opM
   __shared__ uint8_t M[SPMV_HN];
   const int tid = threadIdx.x;
   const int ntx = (nx + SPMV_TW - 1) / SPMV_TW, nch = ntx * ((ny + SPMV_TH - 1) / SPMV_TH);
-  int ch = xcd_block(blockIdx.x, gridDim.x);
+  int ch = xcd_block(B_.x, B_.gx);
   // every load of the halo tile first: it does not depend on beta, so its latency overlaps the
   // fold's reductions (cells outside the grid: mask 0, never a free neighbour)
   int i0 = 0, j0 = 0;
@@ -2061,7 +2061,7 @@ __global__ __launch_bounds__(256, 4) void k_spmv(const uint8_t* __restrict__ opM
     }
   };
   load(ch);
-  const double beta = fold_zdots(partZ, bpartZ, nbz, par, first, B, sh, res, zs, sc);
+  const double beta = fold_zdots(B_, partZ, bpartZ, nbz, par, first, B, sh, res, zs, sc);
   const int ns = nslots(sc->active_bubbles);
   const double* Po = bv(B.bvec, BV_P0 + 1 - par);
   const double* accZ = bv(B.bvec, BV_ACCZ0 + par);
@@ -2117,28 +2117,28 @@ __global__ __launch_bounds__(256, 4) void k_spmv(const uint8_t* __restrict__ opM
         } else ap[c] = 0.f;
       }
     }
-    ch += gridDim.x;
+    ch += B_.gx;
     if (ch >= nch) break;
     load(ch);
   }
-  store_partials<1>(v, 1, partA, sh);
-  if (ns > 0) store_partials<BSLOTS>(sl, ns, bpartA, sh);
+  store_partials<1>(B_, v, 1, partA, sh);
+  if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, bpartA, sh);
 }
 
 // the update fold as a kernel of its own, when level 0 has no down kernel (one-workgroup grids)
-__global__ __launch_bounds__(FIN_THREADS) void k_fold_update(const double* __restrict__ part, int nb, int par,
+__device__ __forceinline__ void k_fold_update(Blk B_, const double* __restrict__ part, int nb, int par,
                                                              DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[16], res[1];
-  (void)fold_update(part, nb, par, sh, res, sc);
+  (void)fold_update(B_, part, nb, par, sh, res, sc);
 }
 
 // ================================================================== end of the solve
-__global__ __launch_bounds__(256) void k_pcg_final(const uint8_t* __restrict__ tipo, float* __restrict__ q, int C) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_pcg_final(Blk B_, const uint8_t* __restrict__ tipo, float* __restrict__ q, int C) {
+  const int c = B_.x * blockDim.x + threadIdx.x;
   if (c < C && tipo[c] != AGUA) q[c] = 0.f;
 }
-__global__ __launch_bounds__(1024) void k_pcg_final_b(const double* __restrict__ bvec, const int* __restrict__ activeList,
+__device__ __forceinline__ void k_pcg_final_b(Blk B_, const double* __restrict__ bvec, const int* __restrict__ activeList,
                                                       float* __restrict__ yB32, DevScalars* __restrict__ sc) {
   if (threadIdx.x == 0) {
     const int seg = sc->itp[0] > sc->itp[1] ? sc->itp[0] : sc->itp[1];
@@ -2159,7 +2159,7 @@ __global__ __launch_bounds__(1024) void k_pcg_final_b(const double* __restrict__
 // segment's stop, and returns at once otherwise.
 __device__ inline bool true_res_skip(const DevScalars* sc, int gated) { return gated && (!sc->done || sc->true_done); }
 
-__global__ __launch_bounds__(256) void k_true_res(const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
+__device__ __forceinline__ void k_true_res(Blk B_, const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
                                                   const float* __restrict__ div, const float* __restrict__ q,
                                                   const float* __restrict__ qlo, int nx, int C,
                                                   double* __restrict__ bvec, const int* __restrict__ labelSlot,
@@ -2178,7 +2178,7 @@ __global__ __launch_bounds__(256) void k_true_res(const uint8_t* __restrict__ op
     if (md & MK_BUB) return qlo ? yb[rot[d]] : (double)(float)yb[rot[d]];  // compensated: y in binary64
     return 0.0;
   };
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = B_.x * blockDim.x + threadIdx.x;
   if (c < C) {
     const uint8_t m = opMask[c];
     if (mask_type(m) == AGUA) {
@@ -2195,11 +2195,11 @@ __global__ __launch_bounds__(256) void k_true_res(const uint8_t* __restrict__ op
       v[0] = e * e;
     }
   }
-  store_partials<1>(v, 1, partials, sh);
-  if (ns > 0) store_partials<BSLOTS>(sl, ns, bpart, sh);
+  store_partials<1>(B_, v, 1, partials, sh);
+  if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, bpart, sh);
 }
 
-__global__ __launch_bounds__(FIN_THREADS) void k_fin_true_res(const uint32_t* __restrict__ facesB,
+__device__ __forceinline__ void k_fin_true_res(Blk B_, const uint32_t* __restrict__ facesB,
                                                               double* __restrict__ bvec, const double* __restrict__ rhsB,
                                                               const int* __restrict__ activeList,
                                                               const int* __restrict__ labelSlot,
@@ -2244,7 +2244,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_true_res(const uint32_t* __
 static long wg_bytes(int n) { return 12L * n + ((n + 15) & ~15); }
 
 // level lc's V-cycle is applied as the dense matrix of this solve
-static bool vop_active(const tq_sim* s) { return s->vop_ld > 0 && s->d.mg_dense; }
+static bool vop_active(const tq_sim* s) { return s->vop_ld > 0 && dense_wanted(s); }
 
 int solver_alloc(tq_sim* s) {
   s->lv.clear();
@@ -2401,7 +2401,7 @@ static RedSrc red_src(tq_sim* s, int slot, int nv, int bhalf, int nb) {
   double* dst = s->partials_r + ps_off(slot);
   double* bdst = bhalf < 0 ? nullptr : s->bpart_r + (size_t)bhalf * BSLOTS * MAX_RED_BLOCKS;
   int t = time_begin(s, "pcg_prereduce", 8.0 * nv * nb);
-  hipLaunchKernelGGL(k_prereduce, dim3(PRE_R), dim3(256), 0, s->st, src, nb, nv, bsrc, dst, bdst,
+  launch<256, 0, k_prereduce>(s, s->st, dim3(PRE_R), dim3(256), 0, src, nb, nv, bsrc, dst, bdst,
                      (const DevScalars*)s->sc);
   time_end(s, t);
   return {dst, bdst, PRE_R};
@@ -2425,7 +2425,7 @@ static void launch_in_a0(tq_sim* s, bool upd) {
   const BubArgs B = bubargs(s);
   const float* mgs = &s->sc->mgs[it_par(s)];
 #define TQ_IN_A0(SB, U, P)                                                                                        \
-  hipLaunchKernelGGL((k_in_a0<SB, U, P>), g, dim3(256), 0, s->st, s->opMask, F.mask, s->rot, s->cellsB, pp, s->ap, \
+  launch<256, 0, k_in_a0<SB, U, P>>(s, s->st, g, dim3(256), 0, s->opMask, F.mask, s->rot, s->cellsB, pp, s->ap, \
                      s->q, s->qlo, s->qcomp, s->r, F.nx, F.ny, ntx, nty, vgx, nvb, B, rs.part, rs.bpart, rs.nb, par, \
                      (SB*)F.b,                                                                                     \
                      (float*)F.x,                                                                                  \
@@ -2481,7 +2481,7 @@ static void launch_bra(tq_sim* s, int l) {
       const int nvb = (int)(tg.x * tg.y);
       // the folding launch (level 0 of an iteration) walks several groups per block after one fold
       const int nbk = fold ? std::min(nvb, pass_fold_grid(s)) : nvb;
-      hipLaunchKernelGGL((k_pass<SX, SB, 0>), dim3(nbk), dim3(256), 0, s->st, (const SX*)F.x, (SX*)F.tmp,
+      launch<256, 0, k_pass<SX, SB, 0>>(s, s->st, dim3(nbk), dim3(256), 0, (const SX*)F.x, (SX*)F.tmp,
                          (const SB*)F.b, F.mask, F.nx, F.ny, 8, ntx8, nty8, (int)tg.x, nvb, rr, rsu.nb, it_par(s),
                          fold, s->sc);
     });
@@ -2495,10 +2495,10 @@ static void launch_bra(tq_sim* s, int l) {
         using SCX = typename decltype(cx)::type;
         using SCB = typename decltype(cb)::type;
         if (pa)
-          hipLaunchKernelGGL((k_restrict_a<SXF, SBF, SCB, SCX, true>), g, dim3(256), 0, s->st, (const SXF*)F.tmp,
+          launch<256, 0, k_restrict_a<SXF, SBF, SCB, SCX, true>>(s, s->st, g, dim3(256), 0, (const SXF*)F.tmp,
                              (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny, s->sc);
         else
-          hipLaunchKernelGGL((k_restrict_a<SXF, SBF, SCB, SCX, false>), g, dim3(256), 0, s->st, (const SXF*)F.tmp,
+          launch<256, 0, k_restrict_a<SXF, SBF, SCB, SCX, false>>(s, s->st, g, dim3(256), 0, (const SXF*)F.tmp,
                              (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny, s->sc);
       });
     });
@@ -2518,11 +2518,11 @@ static void launch_bra(tq_sim* s, int l) {
       using SCX = typename decltype(cx)::type;
       using SCB = typename decltype(cb)::type;
       if (pa)
-        hipLaunchKernelGGL((k_bra<SXF, SBF, SCB, SCX, true>), g, dim3(64 * BRA_WAVES), 0, s->st, (const SXF*)F.x,
+        launch<64 * BRA_WAVES, 0, k_bra<SXF, SBF, SCB, SCX, true>>(s, s->st, g, dim3(64 * BRA_WAVES), 0, (const SXF*)F.x,
                            (SXF*)F.tmp, (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny,
                            rr, rsu.nb, it_par(s), fold, s->sc);
       else
-        hipLaunchKernelGGL((k_bra<SXF, SBF, SCB, SCX, false>), g, dim3(64 * BRA_WAVES), 0, s->st, (const SXF*)F.x,
+        launch<64 * BRA_WAVES, 0, k_bra<SXF, SBF, SCB, SCX, false>>(s, s->st, g, dim3(64 * BRA_WAVES), 0, (const SXF*)F.x,
                            (SXF*)F.tmp, (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny,
                            rr, s->nb_upd, it_par(s), fold, s->sc);
     });
@@ -2559,11 +2559,11 @@ static int launch_pcd(tq_sim* s, int l) {
     with_types(F, [&](auto tx, auto tb) {
       TQ_TYPES(tx, tb);
       if (G.hx)
-        hipLaunchKernelGGL((k_prolong_pass<SX, SB, __half>), tile_grid(F.nx, F.ny, 8), dim3(256), 0, s->st,
+        launch<256, 0, k_prolong_pass<SX, SB, __half>>(s, s->st, tile_grid(F.nx, F.ny, 8), dim3(256), 0,
                            (const SX*)F.tmp, (SX*)F.x, (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8,
                            (const __half*)G.res, G.mask, G.nx, G.ny, s->sc);
       else
-        hipLaunchKernelGGL((k_prolong_pass<SX, SB, float>), tile_grid(F.nx, F.ny, 8), dim3(256), 0, s->st,
+        launch<256, 0, k_prolong_pass<SX, SB, float>>(s, s->st, tile_grid(F.nx, F.ny, 8), dim3(256), 0,
                            (const SX*)F.tmp, (SX*)F.x, (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8,
                            (const float*)G.res, G.mask, G.nx, G.ny, s->sc);
     });
@@ -2572,7 +2572,7 @@ static int launch_pcd(tq_sim* s, int l) {
     with_types(F, [&](auto tx, auto tb) {
       TQ_TYPES(tx, tb);
       const dim3 tg = tile_grid(F.nx, F.ny, 0);
-      hipLaunchKernelGGL((k_pass<SX, SB, 1>), dim3(tg.x * tg.y), dim3(256), 0, s->st, (const SX*)F.x,
+      launch<256, 0, k_pass<SX, SB, 1>>(s, s->st, dim3(tg.x * tg.y), dim3(256), 0, (const SX*)F.x,
                          (SX*)F.tmp, (const SB*)F.b, F.mask, F.nx, F.ny, 0, ntx, nty, (int)tg.x, (int)(tg.x * tg.y),
                          (const double*)nullptr, 0, 0, 0, s->sc);
     });
@@ -2586,10 +2586,10 @@ static int launch_pcd(tq_sim* s, int l) {
     auto go = [&](auto tc) {
       using SC = typename decltype(tc)::type;
       if (zd)
-        hipLaunchKernelGGL((k_pcd<SX, SB, SC, true>), g, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
+        launch<256, 0, k_pcd<SX, SB, SC, true>>(s, s->st, g, dim3(256), 0, (const SX*)F.tmp, (SX*)F.x,
                            (const SB*)F.b, F.mask, F.nx, F.ny, (const SC*)G.res, G.mask, G.nx, G.ny, za, s->sc);
       else
-        hipLaunchKernelGGL((k_pcd<SX, SB, SC, false>), g, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
+        launch<256, 0, k_pcd<SX, SB, SC, false>>(s, s->st, g, dim3(256), 0, (const SX*)F.tmp, (SX*)F.x,
                            (const SB*)F.b, F.mask, F.nx, F.ny, (const SC*)G.res, G.mask, G.nx, G.ny, za, s->sc);
     };
     if (G.hx) go(Tag<__half>{}); else go(Tag<float>{});
@@ -2646,7 +2646,7 @@ static void launch_coarse_wg(tq_sim* s) {
     // x(lc) = C b(lc): the matrix and b in, x out
     const int waves = (G.n + VOP_ROWS - 1) / VOP_ROWS;
     int t = time_begin(s, "mg_coarse_mv", 4.0 * s->vop_ld * G.n + 8.0 * G.n);
-    hipLaunchKernelGGL(k_vcycle_apply, dim3((waves + 3) / 4), dim3(256), 0, s->st, (const float*)s->vop, s->vop_ld,
+    launch<256, 0, k_vcycle_apply>(s, s->st, dim3((waves + 3) / 4), dim3(256), 0, (const float*)s->vop, s->vop_ld,
                        G.n, (const float*)G.b, (float*)G.x, s->sc);
     time_end(s, t);
     return;
@@ -2657,7 +2657,7 @@ static void launch_coarse_wg(tq_sim* s) {
   for (int l = s->lc; l < L; l++) bytes += s->lv[l].n;
   if (s->dense) bytes += 4.0 * s->lv[L - 1].n * s->lv[L - 1].n;
   int t = time_begin(s, "mg_coarse_wg", bytes);
-  hipLaunchKernelGGL(k_coarse_wg, dim3(1), dim3(1024), 0, s->st, P, (const float*)G.b, (const float*)G.x,
+  launch<1024, 0, k_coarse_wg>(s, s->st, dim3(1), dim3(1024), 0, P, (const float*)G.b, (const float*)G.x,
                      (float*)G.x, s->sc);
   time_end(s, t);
 }
@@ -2673,13 +2673,13 @@ static int launch_vcycle_op(tq_sim* s) {
   if (s->st2 && !s->timing && !s->capturing) {
     TQ_HIP(hipEventRecord(s->ev_masks, s->st));
     TQ_HIP(hipStreamWaitEvent(s->st2, s->ev_masks, 0));
-    hipLaunchKernelGGL(k_vcycle_op, dim3(G.n), dim3(64 * VOP_WAVES), lds, s->st2, P, s->vop_ld, s->vop, s->sc);
+    launch<64 * VOP_WAVES, 0, k_vcycle_op>(s, s->st2, dim3(G.n), dim3(64 * VOP_WAVES), lds, P, s->vop_ld, s->vop, s->sc);
     TQ_HIP(hipEventRecord(s->ev_vop, s->st2));
     s->vop_pending = true;
     return 0;
   }
   int t = time_begin(s, "mg_vcycle_op", 4.0 * s->vop_ld * G.n);
-  hipLaunchKernelGGL(k_vcycle_op, dim3(G.n), dim3(64 * VOP_WAVES), lds, s->st, P, s->vop_ld, s->vop, s->sc);
+  launch<64 * VOP_WAVES, 0, k_vcycle_op>(s, s->st, dim3(G.n), dim3(64 * VOP_WAVES), lds, P, s->vop_ld, s->vop, s->sc);
   time_end(s, t);
   return 0;
 }
@@ -2695,11 +2695,11 @@ static int launch_up0(tq_sim* s) {
   with_types(F, [&](auto tx, auto tb) {
     TQ_TYPES(tx, tb);
     if (G.hx)
-      hipLaunchKernelGGL((k_prolong_pass<SX, SB, __half>), g8, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
+      launch<256, 0, k_prolong_pass<SX, SB, __half>>(s, s->st, g8, dim3(256), 0, (const SX*)F.tmp, (SX*)F.x,
                          (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8, (const __half*)G.res, G.mask, G.nx, G.ny,
                          s->sc);
     else
-      hipLaunchKernelGGL((k_prolong_pass<SX, SB, float>), g8, dim3(256), 0, s->st, (const SX*)F.tmp, (SX*)F.x,
+      launch<256, 0, k_prolong_pass<SX, SB, float>>(s, s->st, g8, dim3(256), 0, (const SX*)F.tmp, (SX*)F.x,
                          (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8, (const float*)G.res, G.mask, G.nx, G.ny,
                          s->sc);
   });
@@ -2709,10 +2709,10 @@ static int launch_up0(tq_sim* s) {
   const ZArgs za = zargs(s);
   t = time_begin(s, "mg_pass_z_l0", (4 + vbytes(F.hb) + 1 + 1 + 4 + 4 + 4) * (double)F.n);
   if (F.hb)
-    hipLaunchKernelGGL((k_pass_z<__half>), g, dim3(256), 0, s->st, (const float*)F.x, (const __half*)F.b, F.mask, F.nx,
+    launch<256, 0, k_pass_z<__half>>(s, s->st, g, dim3(256), 0, (const float*)F.x, (const __half*)F.b, F.mask, F.nx,
                        F.ny, ntx, nty, za, s->sc);
   else
-    hipLaunchKernelGGL((k_pass_z<float>), g, dim3(256), 0, s->st, (const float*)F.x, (const float*)F.b, F.mask, F.nx,
+    launch<256, 0, k_pass_z<float>>(s, s->st, g, dim3(256), 0, (const float*)F.x, (const float*)F.b, F.mask, F.nx,
                        F.ny, ntx, nty, za, s->sc);
   time_end(s, t);
   return (int)(g.x * g.y);
@@ -2723,7 +2723,7 @@ static void precondition_rest(tq_sim* s) {
   const int lc = s->lc;
   if (lc == 0 && s->it_k >= 0) {  // no level-0 down kernel to carry the norm and stop test
     int t = time_begin(s, "pcg_update_fold", 0);
-    hipLaunchKernelGGL(k_fold_update, dim3(1), dim3(FIN_THREADS), 0, s->st,
+    launch<FIN_THREADS, 0, k_fold_update>(s, s->st, dim3(1), dim3(FIN_THREADS), 0,
                        (const double*)(s->partials + ps_off(PS_RR)), s->nb_upd, it_par(s), s->sc);
     time_end(s, t);
   }
@@ -2736,7 +2736,7 @@ static void precondition_rest(tq_sim* s) {
   } else {
     s->nb_z = blocks_of(s->C);
     int t = time_begin(s, "pcg_zdots", 16.0 * s->C);
-    hipLaunchKernelGGL(k_zdots0, dim3(s->nb_z), dim3(256), 0, s->st, (const float*)s->lv[0].x, zargs(s), s->C, s->sc);
+    launch<256, 0, k_zdots0>(s, s->st, dim3(s->nb_z), dim3(256), 0, (const float*)s->lv[0].x, zargs(s), s->C, s->sc);
     time_end(s, t);
   }
   s->zc = 1 - s->zc;
@@ -2750,7 +2750,7 @@ static void pcg_iteration(tq_sim* s) {
   const int pn = 1 - s->pc;
   const RedSrc rz = red_src(s, PS_Z, 2, 1, s->nb_z);
   int t = time_begin(s, "pcg_spmv", (1 + 4 + 4 + 4 + 4) * (double)C);
-  hipLaunchKernelGGL(k_spmv, dim3(nbs), dim3(256), 0, s->st, s->opMask, s->rot, s->zbuf[s->zc], s->pbuf[s->pc],
+  launch<256, 4, k_spmv>(s, s->st, dim3(nbs), dim3(256), 0, s->opMask, s->rot, s->zbuf[s->zc], s->pbuf[s->pc],
                      s->pbuf[pn], s->ap, s->nx, s->ny, bubargs(s), rz.part, rz.bpart, rz.nb, it_par(s),
                      s->it_k == 0 ? 1 : 0, s->partials + ps_off(PS_SPMV), bpart_a(s), s->sc);
   time_end(s, t);
@@ -2794,7 +2794,7 @@ static int graph_pair(tq_sim* s) {
 // The captured pair's parity-dependent buffers are fixed, so pairs start at odd iterations;
 // iteration 0 (beta = 0) and any unpaired iteration run as eager launches.
 static int run_iterations(tq_sim* s, int n) {
-  const bool graph = s->d.use_graph && !s->timing;
+  const bool graph = s->d.use_graph && !s->timing && !s->lrec;
   while (n > 0) {
     if (graph && n >= 2 && (s->it_k & 1)) {
       if (const int rc = graph_pair(s)) return rc;
@@ -2827,15 +2827,14 @@ static int solve_setup(tq_sim* s) {
       Pp.ny[l] = s->lv[l].ny;
       Pp.mask[l] = s->lv[l].mask;
     }
-    hipLaunchKernelGGL(k_setup_pyr, dim3((s->nx + PYR_T - 1) / PYR_T, (s->ny + PYR_T - 1) / PYR_T), dim3(1024), 0,
-                       s->st, s->tipo, s->rot, s->facesB, s->d.ar, s->opMask, Pp);
+    launch<1024, 0, k_setup_pyr>(s, s->st, dim3((s->nx + PYR_T - 1) / PYR_T, (s->ny + PYR_T - 1) / PYR_T), dim3(1024), 0, s->tipo, s->rot, s->facesB, s->d.ar, s->opMask, Pp);
   }
   for (int l = PYR_LEVELS + 1; l < L; l++) {  // levels past the pyramid (grids over 768 cells a side)
     const Level& F = s->lv[l - 1];
     const Level& G = s->lv[l];
-    hipLaunchKernelGGL(k_coarse_type, dim3(blocks_of(G.n)), dim3(256), 0, s->st, F.mask, F.nx, F.ny, G.mask, G.nx, G.ny);
+    launch<256, 0, k_coarse_type>(s, s->st, dim3(blocks_of(G.n)), dim3(256), 0, F.mask, F.nx, F.ny, G.mask, G.nx, G.ny);
   }
-  hipLaunchKernelGGL(k_bub_init, dim3(1), dim3(1024), 0, s->st, s->facesB, s->yB32, s->d.ar, s->bvec, s->activeList,
+  launch<1024, 0, k_bub_init>(s, s->st, dim3(1), dim3(1024), 0, s->facesB, s->yB32, s->d.ar, s->bvec, s->activeList,
                      s->labelSlot, s->sc);
   {
     // neighbour bits of levels 1.., and whether the dense operators must be rebuilt (their masks
@@ -2855,18 +2854,18 @@ static int solve_setup(tq_sim* s) {
     V.off[L] = off;
     const bool force = vop_active(s) && !s->vop_was_active;
     if (off > 0)
-      hipLaunchKernelGGL(k_viz_all, dim3(blocks_of(off)), dim3(256), 0, s->st, V, s->op_masks, force ? 1 : 0, s->sc);
+      launch<256, 0, k_viz_all>(s, s->st, dim3(blocks_of(off)), dim3(256), 0, V, s->op_masks, force ? 1 : 0, s->sc);
     s->vop_was_active = vop_active(s);
   }
   if (s->dense) {
     const Level& G = s->lv[L - 1];
-    hipLaunchKernelGGL(k_coarsest_op, dim3(G.n), dim3(64), 0, s->st, G.mask, G.nx, G.n, s->dense,
+    launch<64, 0, k_coarsest_op>(s, s->st, dim3(G.n), dim3(64), 0, G.mask, G.nx, G.n, s->dense,
                        (const DevScalars*)s->sc);
   }
   time_end(s, t);
   if (vop_active(s))
     if (int e = launch_vcycle_op(s)) return e;
-  hipLaunchKernelGGL(k_op_count, dim3(1), dim3(64), 0, s->st, s->sc);
+  launch<64, 0, k_op_count>(s, s->st, dim3(1), dim3(64), 0, s->sc);
   TQ_LAUNCH_CHECK();
   return 0;
 }
@@ -2876,9 +2875,9 @@ static int solve_setup(tq_sim* s) {
 static void launch_true_check(tq_sim* s) {
   const int C = s->C;
   const int nb = blocks_of(C);
-  hipLaunchKernelGGL(k_true_res, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->div, s->q, (const float*)s->qlo,
+  launch<256, 0, k_true_res>(s, s->st, dim3(nb), dim3(256), 0, s->opMask, s->rot, s->div, s->q, (const float*)s->qlo,
                      s->nx, C, s->bvec, s->labelSlot, s->partials, s->bpart, 1, (const DevScalars*)s->sc);
-  hipLaunchKernelGGL(k_fin_true_res, dim3(1), dim3(FIN_THREADS), 0, s->st, s->facesB, s->bvec, s->rhsB, s->activeList,
+  launch<FIN_THREADS, 0, k_fin_true_res>(s, s->st, dim3(1), dim3(FIN_THREADS), 0, s->facesB, s->bvec, s->rhsB, s->activeList,
                      s->labelSlot, s->partials, s->bpart, nb, 1, s->sc, (double*)nullptr);
 }
 
@@ -2901,11 +2900,11 @@ int pressure_solve(tq_sim* s) {
     s->pc = 0;
     s->zc = 0;
     int t = time_begin(s, "pcg_init", (1 + 1 + 4 + 4 + 4 + 4 + 4 + (s->qcomp ? 4 : 0)) * (double)C);
-    if (restart) hipLaunchKernelGGL(k_bub_restart, dim3(1), dim3(1024), 0, s->st, s->bvec);
-    hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->div, s->q, s->qlo,
+    if (restart) launch<1024, 0, k_bub_restart>(s, s->st, dim3(1), dim3(1024), 0, s->bvec);
+    launch<256, 0, k_pcg_init>(s, s->st, dim3(nb), dim3(256), 0, s->opMask, s->rot, s->div, s->q, s->qlo,
                        s->qcomp ? (restart ? 2 : 1) : 0, s->r, s->pbuf[s->pc], s->nx, C, s->bvec, s->labelSlot,
                        s->partials, s->bpart, s->sc);
-    hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(FIN_THREADS), 0, s->st, s->facesB, s->bvec, s->rhsB, s->activeList,
+    launch<FIN_THREADS, 0, k_fin_init>(s, s->st, dim3(1), dim3(FIN_THREADS), 0, s->facesB, s->bvec, s->rhsB, s->activeList,
                        s->labelSlot, s->partials, s->bpart, nb, tol, maxit, mult, restart ? 1 : 0, s->sc);
     time_end(s, t);
     TQ_LAUNCH_CHECK();
@@ -2939,8 +2938,8 @@ int pressure_solve(tq_sim* s) {
     const bool converged = h.norm_last <= tol * h.normb;  // the segment stopped on its recursive residual
     if (!converged || !(h.true_norm > tol * h.normb) || restart >= MAX_RESTARTS) break;
   }
-  hipLaunchKernelGGL(k_pcg_final, dim3(blocks_of(C)), dim3(256), 0, s->st, s->tipo, s->q, C);
-  hipLaunchKernelGGL(k_pcg_final_b, dim3(1), dim3(1024), 0, s->st, s->bvec, s->activeList, s->yB32, s->sc);
+  launch<256, 0, k_pcg_final>(s, s->st, dim3(blocks_of(C)), dim3(256), 0, s->tipo, s->q, C);
+  launch<1024, 0, k_pcg_final_b>(s, s->st, dim3(1), dim3(1024), 0, s->bvec, s->activeList, s->yB32, s->sc);
   TQ_LAUNCH_CHECK();
   return 0;
 }
@@ -2949,9 +2948,9 @@ int true_residual(tq_sim* s, double* out) {
   const int C = s->C;
   const int nb = blocks_of(C);
   double* dres = s->partials + ps_off(PS_LAST) + MAX_RED_BLOCKS - 1;
-  hipLaunchKernelGGL(k_true_res, dim3(nb), dim3(256), 0, s->st, s->opMask, s->rot, s->div, s->q, (const float*)nullptr,
+  launch<256, 0, k_true_res>(s, s->st, dim3(nb), dim3(256), 0, s->opMask, s->rot, s->div, s->q, (const float*)nullptr,
                      s->nx, C, s->bvec, s->labelSlot, s->partials, s->bpart, 0, (const DevScalars*)s->sc);
-  hipLaunchKernelGGL(k_fin_true_res, dim3(1), dim3(FIN_THREADS), 0, s->st, s->facesB, s->bvec, s->rhsB, s->activeList,
+  launch<FIN_THREADS, 0, k_fin_true_res>(s, s->st, dim3(1), dim3(FIN_THREADS), 0, s->facesB, s->bvec, s->rhsB, s->activeList,
                      s->labelSlot, s->partials, s->bpart, nb, 0, s->sc, dres);
   TQ_LAUNCH_CHECK();
   TQ_HIP(hipMemcpyAsync(out, dres, sizeof(double), hipMemcpyDeviceToHost, s->st));

@@ -64,6 +64,12 @@ double num_or(napi_env env, napi_value obj, const char* key, double dflt) {
   return d;
 }
 
+// mgDense: true/1 the dense coarse matrix, false/0 the one-workgroup V-cycle, -1 (or absent) auto
+int dense_opt(napi_env env, napi_value o, int dflt) {
+  const double v = num_or(env, o, "mgDense", dflt);
+  return v < 0 ? -1 : (v != 0 ? 1 : 0);
+}
+
 // a Float32Array argument: data and element count
 bool f32_array(napi_env env, napi_value v, float** data, size_t* n) {
   bool is = false;
@@ -142,7 +148,7 @@ napi_value construct(napi_env env, napi_callback_info info) {
     d.precision = (int)num_or(env, o, "precision", d.precision);
     d.tol = num_or(env, o, "tol", d.tol);
     d.use_graph = num_or(env, o, "useGraph", d.use_graph) != 0;
-    d.mg_dense = num_or(env, o, "mgDense", d.mg_dense) != 0;
+    d.mg_dense = dense_opt(env, o, d.mg_dense);
     d.max_particles = (int)num_or(env, o, "maxParticulas", d.max_particles);
   }
   Native* n = new Native();
@@ -612,7 +618,7 @@ napi_value m_options(napi_env env, napi_callback_info info) {
   d.precision = (int)num_or(env, o, "precision", d.precision);
   d.tol = num_or(env, o, "tol", d.tol);
   d.use_graph = num_or(env, o, "useGraph", d.use_graph) != 0;
-  d.mg_dense = num_or(env, o, "mgDense", d.mg_dense) != 0;
+  d.mg_dense = dense_opt(env, o, d.mg_dense);
   throw_tq(env, tq_set_options(n->s, &d), "tq_set_options");
   return undefined(env);
 }

@@ -41,7 +41,7 @@ static void timing_collect(tq_sim* s) {
 }
 
 int time_begin(tq_sim* s, const char* name, double bytes) {
-  if (!s->timing || s->capturing) return -1;
+  if (!s->timing || s->capturing || s->lrec) return -1;
   if (!s->timing_only.empty() && s->timing_only != name) return -1;
   int ki = -1;
   for (size_t i = 0; i < s->ktimes.size(); i++)
@@ -76,6 +76,25 @@ void time_end(tq_sim* s, int slot) {
 }
 
 static void timing_flush(tq_sim* s) { timing_collect(s); }
+
+// fills n 32-bit words with zero: the form of hipMemsetAsync that a launch log can record
+__device__ __forceinline__ void k_zero_words(Blk B_, uint32_t* __restrict__ p, int n) {
+  for (int i = B_.x * 256 + (int)threadIdx.x; i < n; i += B_.gx * 256) p[i] = 0u;
+}
+
+int zero_async(tq_sim* s, hipStream_t st, void* p, size_t bytes) {
+  if (!s->lrec) {
+    TQ_HIP(hipMemsetAsync(p, 0, bytes, st));
+    return 0;
+  }
+  if (bytes % 4 || (uintptr_t)p % 4 || bytes / 4 > (size_t)INT32_MAX) {
+    set_error("zero_async: the range is not whole aligned words");
+    return TQ_E_ARG;
+  }
+  const int n = (int)(bytes / 4);
+  launch<256, 0, k_zero_words>(s, st, dim3(std::max(1, std::min((n + 255) / 256, 256))), dim3(256), 0, (uint32_t*)p, n);
+  return 0;
+}
 
 // ------------------------------------------------------------------ allocation
 template <class T>
@@ -170,6 +189,16 @@ void drop_second_stream(tq_sim* s) {
   s->vop_pending = false;
 }
 
+// A tank of a table batch enqueues on the batch's stream (shared by every tank of the batch, which
+// then orders the tanks' own calls and the batch's table launches): its own stream is released.
+int share_stream(tq_sim* s, hipStream_t st) {
+  TQ_HIP(hipStreamSynchronize(s->st));
+  if (s->own_st) TQ_HIP(hipStreamDestroy(s->st));
+  s->st = st;
+  s->own_st = false;
+  return 0;
+}
+
 static void free_all(tq_sim* s) {
   if (s->st2) (void)hipStreamSynchronize(s->st2);  // a matrix build may still be writing
   auto f = [](void* p) { if (p) (void)hipFree(p); };
@@ -195,12 +224,13 @@ static void free_all(tq_sim* s) {
   if (s->st2) (void)hipStreamDestroy(s->st2);
   if (s->ev_masks) (void)hipEventDestroy(s->ev_masks);
   if (s->ev_vop) (void)hipEventDestroy(s->ev_vop);
-  if (s->st) (void)hipStreamDestroy(s->st);
+  if (s->st && s->own_st) (void)hipStreamDestroy(s->st);
 }
 
 // ------------------------------------------------------------------ the substep
 // (with a substep record open, the hooks copy what the reverse pass needs)
 static int substep(tq_sim* s, double dt) {
+  s->substeps_run++;
   if (int e = p2g(s, dt, 1)) return e;           // #paraGrade + #gravidadeEFronteira + #tipos
   s->tipos_prontos = true;
   if (s->rec) if (int e = rec_hook(s, REC_P2G, dt)) return e;
@@ -309,6 +339,30 @@ int step_enqueue(tq_sim* s, double dt, int substeps) {
   return 0;
 }
 int stream_wait(tq_sim* s) { return sync(s); }
+
+// The launches of the same frame appended to `log` instead of launched: round 0 holds the air
+// update, round k (1..substeps) substep k.  The host-side state (buffer parities, iteration
+// counters) advances as if the frame had been enqueued, so the log must be launched next.
+int step_record(tq_sim* s, double dt, int substeps, LaunchLog* log) {
+  if (!s || !log) { set_error("null handle"); return TQ_E_ARG; }
+  if (!(dt > 0) || substeps < 0) { set_error("tq_step: bad dt or substep count"); return TQ_E_ARG; }
+  if (s->tape || s->rec) { set_error("table batch: a tank is recording frames (tq_tape_begin)"); return TQ_E_STATE; }
+  if (s->timing) { set_error("table batch: kernel timing is on for a tank"); return TQ_E_STATE; }
+  if (s->d.tol > 0) {
+    set_error("table batch: a tank solves to a tolerance (a table frame needs a fixed PCG iteration count, tol = 0)");
+    return TQ_E_STATE;
+  }
+  s->lrec = log;
+  log->rounds.push_back((uint32_t)log->ops.size());
+  int e = s->d.ar ? update_air(s) : 0;
+  const double d = dt / substeps;
+  for (int k = 0; k < substeps && !e; k++) {
+    log->rounds.push_back((uint32_t)log->ops.size());
+    e = substep(s, d);
+  }
+  s->lrec = nullptr;
+  return e;
+}
 }  // namespace tq
 
 namespace tq {
@@ -398,7 +452,7 @@ void tq_desc_default(tq_desc* d, int nx, int ny, double h) {
   d->mg_min_lds_level = 0;
   d->mg_half_level = -1;
   d->use_graph = 0;
-  d->mg_dense = 1;
+  d->mg_dense = -1;
 }
 
 int tq_create(const tq_desc* d, tq_sim** out) {
@@ -688,6 +742,10 @@ int tq_set_options(tq_sim* s, const tq_desc* d) {
   if (!d) { set_error("tq_set_options: null"); return TQ_E_ARG; }
   if (d->precision < TQ_PREC_FP32 || d->precision > TQ_PREC_MIXED_X) {
     set_error("tq_set_options: precision");
+    return TQ_E_ARG;
+  }
+  if (d->mg_dense < -1 || d->mg_dense > 1) {
+    set_error("tq_set_options: mg_dense is -1, 0 or 1");
     return TQ_E_ARG;
   }
   s->d.gx = d->gx; s->d.gy = d->gy; s->d.iteracoes = d->iteracoes; s->d.tau_densidade = d->tau_densidade;

@@ -42,6 +42,12 @@ __device__ inline void xcd_block2(int* bx, int* by) {
   *by = l / gx;
   *bx = l - *by * gx;
 }
+// the same over a logical block (bodies launched alone or from a tank table)
+__device__ inline void xcd_block2(const Blk& B, int* bx, int* by) {
+  const int l = xcd_block(B.x + B.y * B.gx, B.gx * B.gy);
+  *by = l / B.gx;
+  *bx = l - *by * B.gx;
+}
 
 // Block sum of NV doubles per thread; result valid in thread 0.  `sh` holds >= NV*16 doubles.
 template <int NV>

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/tanque.h"
+#include "tq_launch.hpp"
 
 namespace tq {
 
@@ -232,6 +233,8 @@ struct tq_sim {
   void* graph_exec = nullptr;   // hipGraphExec_t of two PCG iterations
   int graph_key = -1;
   bool capturing = false;       // a stream capture is recording (no timing events)
+  tq::LaunchLog* lrec = nullptr; // launches are appended here instead of launched (tank-table batches)
+  bool own_st = true;           // st was created by this tank (a table batch shares its own stream)
   tq::DevScalars* sc = nullptr;
   tq::DevScalars* sc_host = nullptr;  // pinned mirror
 
@@ -258,6 +261,7 @@ struct tq_sim {
   // host-side state
   bool tipos_prontos = false;
   bool has_vmax = false;
+  uint64_t substeps_run = 0;    // substeps enqueued or recorded so far
   bool air_reset_pending = true;
 
   // timing
@@ -286,6 +290,22 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
     if (e_ != hipSuccess) return tq::hip_fail(e_, "kernel launch", __FILE__, __LINE__); \
   } while (0)
 
+// whether the solve builds level lc's V-cycle as a dense matrix (mg_dense; -1: when the solve runs
+// to a tolerance or for at least DENSE_MIN_ITERS fixed iterations, since the build runs one V-cycle
+// per row of the level and pays only over many applications)
+constexpr int DENSE_MIN_ITERS = 8;
+inline bool dense_wanted(const tq_sim* s) {
+  return s->d.mg_dense > 0 || (s->d.mg_dense < 0 && (s->d.tol > 0 || s->d.iteracoes >= DENSE_MIN_ITERS));
+}
+
+// a kernel body launched on `st`, or appended to the tank's launch log while one is recorded
+template <int LB, int W, auto Body, class... A>
+inline void launch(tq_sim* s, hipStream_t st, dim3 g, dim3 b, size_t shm, A&&... a) {
+  Kernel<LB, W, Body>::go(s->lrec, st, g, b, shm, std::forward<A>(a)...);
+}
+// zero `bytes` (a multiple of 4) at p: hipMemsetAsync, or a recorded fill launch
+int zero_async(tq_sim* s, hipStream_t st, void* p, size_t bytes);
+
 // timing brackets around launches (tq_api.cpp)
 int time_begin(tq_sim* s, const char* name, double bytes);
 void time_end(tq_sim* s, int slot);
@@ -297,6 +317,7 @@ int p2g(tq_sim* s, double dt, int gravity);   // fused P2G + gravity/boundary + 
 int g2p(tq_sim* s, double dt);
 int add_particles_dev(tq_sim* s, int count, int check);
 int step_enqueue(tq_sim* s, double dt, int substeps);  // tq_step without the final wait
+int step_record(tq_sim* s, double dt, int substeps, LaunchLog* log);  // the same frame's launches, logged
 int stream_wait(tq_sim* s);
 int faucet_dev(tq_sim* s, const tq_faucet* f, double vx, double vy, int n, int keep, uint32_t* seed);  // from s->stage
 int apply_tool_dev(tq_sim* s, const tq_tool* t);
@@ -322,6 +343,7 @@ int pressure_solve(tq_sim* s);
 void solver_launch_changed(tq_sim* s);
 void tune_apply_cached(tq_sim* s);  // tq_tune.hip: a tq_autotune result cached for this grid
 void drop_second_stream(tq_sim* s);  // build the coarse matrix on the tank's own stream
+int share_stream(tq_sim* s, hipStream_t st);  // enqueue on a stream the tank does not own
 int true_residual(tq_sim* s, double* out);
 int debug_vcycle(tq_sim* s, const float* b, float* z);
 // ---- tq_api.hip

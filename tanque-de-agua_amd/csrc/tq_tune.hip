@@ -116,6 +116,7 @@ int tq_autotune(tq_sim* s, int iterations, tq_tune_result* out) {
   if (iterations <= 0 || iterations > 1000) { set_error("tq_autotune: iterations out of range"); return TQ_E_ARG; }
   const tq_desc saved = s->d;
   const TuneSet start = current(s);
+  s->d.mg_dense = dense_wanted(s) ? 1 : 0;  // the caller's choice, explicit (the trials change tol)
   s->d.tol = 0;  // fixed iteration count: every trial does the same arithmetic
   s->d.iteracoes = iterations;
   SolveState st;
@@ -125,7 +126,7 @@ int tq_autotune(tq_sim* s, int iterations, tq_tune_result* out) {
     rc = hip_fail(hipErrorOutOfMemory, "hipEventCreate", __FILE__, __LINE__);
   const int reps = 5;
   int trials = 0;
-  TuneSet best = {0, 0, 0, 0, saved.mg_dense};
+  TuneSet best = {0, 0, 0, 0, s->d.mg_dense};
   double best_us = 0, default_us = 0;
   auto trial = [&](const TuneSet& t, double* us) -> int {
     apply(s, t);
@@ -211,7 +212,6 @@ int tq_autotune(tq_sim* s, int iterations, tq_tune_result* out) {
   // put the caller's solve options back, with the chosen shapes (or the starting ones on failure)
   const TuneSet keep = rc ? start : best;
   s->d = saved;
-  s->d.mg_dense = keep.mg_dense;
   apply(s, keep);
   if (rc) return rc;
   {

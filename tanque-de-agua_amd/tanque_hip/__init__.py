@@ -43,6 +43,11 @@ class TuneResult(C.Structure):
                 ("tuned_us", C.c_double), ("trials", C.c_int)]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("steps", C.c_int), ("launches", C.c_int), ("upload_bytes", C.c_double), ("record_s", C.c_double),
+                ("plan_s", C.c_double), ("stage_s", C.c_double), ("issue_s", C.c_double), ("wait_s", C.c_double)]
+
+
 class Tool(C.Structure):
     _fields_ = [("kind", C.c_int), ("x", C.c_double), ("y", C.c_double), ("raio", C.c_double),
                 ("vx", C.c_double), ("vy", C.c_double), ("absorcao", C.c_double * 3)]
@@ -75,10 +80,13 @@ SIGNATURES = {
     "tq_apply_tool": (C.c_int, [C.c_void_p, C.POINTER(Tool)]),
     "tq_emit_faucet": (C.c_int, [C.c_void_p, C.POINTER(Faucet), C.c_double, C.POINTER(C.c_uint32),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "tq_batch_create_table": (C.c_int, [C.POINTER(Desc), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "tq_batch_create": (C.c_int, [C.POINTER(Desc), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "tq_batch_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "tq_batch_tank": (C.c_void_p, [C.c_void_p, C.c_int]),
     "tq_batch_step": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_int)]),
+    "tq_batch_vmax": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "tq_batch_stats_get": (C.c_int, [C.c_void_p, C.POINTER(BatchStats), C.c_int]),
     "tq_batch_destroy": (C.c_int, [C.c_void_p]),
     "tq_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "tq_substep": (C.c_int, [C.c_void_p, C.c_double]),
@@ -203,7 +211,7 @@ class SimulacaoHIP:
 
     def __init__(self, cena, gravidade=(0.0, -9.81), iteracoes=3, tauDensidade=0.1, tensao=0.0728, ar=True,
                  precision=PREC_FP32, tol=0.0, max_particles=0, mg_min_lds_level=0, mg_half_level=-1,
-                 use_graph=False, mg_dense=True):
+                 use_graph=False, mg_dense=None):
         L = load()
         d = Desc()
         L.tq_desc_default(C.byref(d), cena.nx, cena.ny, cena.h)
@@ -213,7 +221,7 @@ class SimulacaoHIP:
         d.mg_min_lds_level = mg_min_lds_level
         d.mg_half_level = mg_half_level
         d.use_graph = 1 if use_graph else 0
-        d.mg_dense = 1 if mg_dense else 0
+        d.mg_dense = -1 if mg_dense is None else (1 if mg_dense else 0)
         self._desc = d
         h = C.c_void_p()
         _chk(L.tq_create(C.byref(d), C.byref(h)))
@@ -553,9 +561,12 @@ class SimulacaoHIP:
 class Batch:
     """Independent tanks stepped together (tq_batch_*): `cenas` with one options dict each
     (SimulacaoHIP's keywords); `tanks[k]` is a SimulacaoHIP view of tank k for set-up and reads;
-    step(dt, substeps) runs avancar(dt, substeps[k]) on every tank from `threads` host threads."""
+    step(dt, substeps) runs avancar(dt, substeps[k]) on every tank.  mode "table": one launch per
+    kernel of the step over all tanks (tq_batch_create_table; tol must be 0), the frames recorded
+    on `threads` host threads; mode "threads": every tank on its own stream, enqueued from
+    `threads` host threads (tq_batch_create)."""
 
-    def __init__(self, cenas, opcoes, threads=8):
+    def __init__(self, cenas, opcoes, threads=8, mode="table"):
         L = load()
         n = len(cenas)
         descs = (Desc * n)()
@@ -570,9 +581,15 @@ class Batch:
             d.precision = o.get("precision", PREC_FP32)
             d.tol = o.get("tol", 0.0)
             d.use_graph = 1 if o.get("use_graph", False) else 0
+            if o.get("mg_dense") is not None:
+                d.mg_dense = 1 if o["mg_dense"] else 0
+        if mode not in ("table", "threads"):
+            raise ValueError("Batch: mode is 'table' or 'threads'")
         h = C.c_void_p()
-        _chk(L.tq_batch_create(descs, n, threads, C.byref(h)))
+        create = L.tq_batch_create_table if mode == "table" else L.tq_batch_create
+        _chk(create(descs, n, threads, C.byref(h)))
         self._h = h
+        self.mode = mode
         self.tanks = []
         for k, cena in enumerate(cenas):
             dk = Desc()
@@ -582,6 +599,22 @@ class Batch:
     def step(self, dt, substeps):
         arr = (C.c_int * len(self.tanks))(*[int(x) for x in substeps])
         _chk(load().tq_batch_step(self._h, dt, arr))
+
+    def vmax(self):
+        """every tank's vMax (a list; None where not measured yet), read after a table step
+        without a device wait (tq_batch_vmax)"""
+        n = len(self.tanks)
+        v = (C.c_double * n)()
+        has = (C.c_int * n)()
+        _chk(load().tq_batch_vmax(self._h, v, has))
+        return [v[k] if has[k] else None for k in range(n)]
+
+    def stats(self, reset=False):
+        """host seconds of the table steps by part, steps, the last step's launches and upload
+        bytes (tq_batch_stats_get)"""
+        st = BatchStats()
+        _chk(load().tq_batch_stats_get(self._h, C.byref(st), 1 if reset else 0))
+        return {f: getattr(st, f) for f, _ in BatchStats._fields_}
 
     def close(self):
         if getattr(self, "_h", None) and _lib is not None:
