@@ -196,15 +196,19 @@ int tq_batch_step(tq_batch* b, double dt, const int* substeps);
 /* every tank's vMax (tq_vmax; has[k] as its `has`), v and has of n entries (either may be null):
  * a table step leaves them in host memory, so after one they are read without a device wait */
 int tq_batch_vmax(tq_batch* b, double* v, int* has);
+/* table mode: a frame's table launches are captured as a HIP graph and replayed while the launch
+ * list repeats (on: 1, the default; 0 launches them one by one) */
+int tq_batch_use_graphs(tq_batch* b, int on);
 /* where a table batch's steps spend host time (seconds, summed over steps since the last reset):
  * recording the tanks' frames, merging the rounds, copying the records into the upload buffer,
- * issuing the copies and table launches, waiting for the device; and the last step's table
- * launches and uploaded record bytes */
+ * issuing the copies and table launches (or the frame graph), waiting for the device; the last
+ * step's table launches and uploaded record bytes; frame graphs captured and replayed */
 typedef struct tq_batch_stats {
   int steps;
   int launches;
   double upload_bytes;
   double record_s, plan_s, stage_s, issue_s, wait_s;
+  int graph_builds, graph_replays;
 } tq_batch_stats;
 int tq_batch_stats_get(tq_batch* b, tq_batch_stats* out, int reset);
 int tq_batch_destroy(tq_batch* b);

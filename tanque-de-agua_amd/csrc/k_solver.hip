@@ -50,18 +50,73 @@ __device__ __host__ inline double* bv(double* base, int k) { return base + (size
 // addend v = hi + lo 2^-64 (hi = floor(v), lo the fraction's top 64 bits) is added with two
 // 64-bit integer atomics and the carry out of the low word, so the total does not depend on the
 // order of the adds (FP64 atomics would).  |sums| < 2^63; addends below 2^-64 are dropped.
-__device__ inline void fx_add(double* acc, uint32_t lab, double v) {
+__device__ inline void fx_split(double v, unsigned long long* hi, unsigned long long* lo) {
   double fl = floor(v), fr = v - fl;
   if (fr >= 1.0) {  // v in (-2^-54, 0): 1 + v rounds to 1; the addend is below the resolution
     fl += 1.0;
     fr = 0.0;
   }
-  const unsigned long long hi = (unsigned long long)(long long)fl;
-  const unsigned long long lo = (unsigned long long)(fr * 18446744073709551616.0);
-  unsigned long long* L = reinterpret_cast<unsigned long long*>(acc) + lab;
-  unsigned long long* H = reinterpret_cast<unsigned long long*>(acc + (size_t)BV_FX_HI * MAX_BOLHAS) + lab;
+  *hi = (unsigned long long)(long long)fl;
+  *lo = (unsigned long long)(fr * 18446744073709551616.0);
+}
+// adds the 128-bit value (hi, lo) to the pair of words (H, L) (global or LDS)
+__device__ inline void fx_add_words(unsigned long long* L, unsigned long long* H, unsigned long long hi,
+                                    unsigned long long lo) {
   const unsigned long long old = atomicAdd(L, lo);
   atomicAdd(H, hi + (old + lo < old ? 1ull : 0ull));
+}
+__device__ inline void fx_add(double* acc, uint32_t lab, double v) {
+  unsigned long long hi, lo;
+  fx_split(v, &hi, &lo);
+  fx_add_words(reinterpret_cast<unsigned long long*>(acc) + lab,
+               reinterpret_cast<unsigned long long*>(acc + (size_t)BV_FX_HI * MAX_BOLHAS) + lab, hi, lo);
+}
+
+// Per-block sums of the bubbles without a reduction slot, on their way to the global fixed-point
+// accumulators: an LDS table of BAGG labels, each summing its addends in the same 128-bit integer
+// arithmetic, flushed with one global add per label at the end of the block.  A cell of a bubble
+// (or a face next to one) then costs LDS atomics instead of two contended global ones.  Integer
+// sums are associative, so the global totals are bit-identical to adding every addend there; a
+// label that finds no free entry within BAGG_PROBE probes is added there directly.
+constexpr int BAGG = 64, BAGG_PROBE = 8;
+struct BubLds {
+  uint32_t key[BAGG];  // label (bubble labels are >= 2), 0 = free
+  unsigned long long lo[BAGG], hi[BAGG];
+  double* acc;         // the global accumulator these labels belong to
+};
+// every thread of the block; a block barrier must follow before the first bub_add
+__device__ inline void bagg_clear(BubLds& a, double* acc) {
+  for (int e = threadIdx.x; e < BAGG; e += blockDim.x) {
+    a.key[e] = 0u;
+    a.lo[e] = 0ull;
+    a.hi[e] = 0ull;
+  }
+  if (threadIdx.x == 0) a.acc = acc;
+}
+__device__ inline void bagg_add(BubLds& a, uint32_t lab, double v) {
+  unsigned long long hi, lo;
+  fx_split(v, &hi, &lo);
+  uint32_t e = (lab * 2654435761u) >> 26;  // 6 bits: BAGG entries
+  for (int p = 0; p < BAGG_PROBE && lab != 0u; p++, e = (e + 1) & (BAGG - 1)) {
+    const uint32_t k = atomicCAS(&a.key[e], 0u, lab);
+    if (k == 0u || k == lab) {
+      fx_add_words(&a.lo[e], &a.hi[e], hi, lo);
+      return;
+    }
+  }
+  fx_add_words(reinterpret_cast<unsigned long long*>(a.acc) + lab,
+               reinterpret_cast<unsigned long long*>(a.acc + (size_t)BV_FX_HI * MAX_BOLHAS) + lab, hi, lo);
+}
+// every thread of the block, after its last bub_add
+__device__ inline void bagg_flush(BubLds& a) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < BAGG; e += blockDim.x) {
+    const uint32_t lab = a.key[e];
+    if (lab)
+      fx_add_words(reinterpret_cast<unsigned long long*>(a.acc) + lab,
+                   reinterpret_cast<unsigned long long*>(a.acc + (size_t)BV_FX_HI * MAX_BOLHAS) + lab, a.hi[e],
+                   a.lo[e]);
+  }
 }
 __device__ inline double fx_read(const double* acc, uint32_t lab) {
   const unsigned long long lo = reinterpret_cast<const unsigned long long*>(acc)[lab];
@@ -115,13 +170,13 @@ __device__ inline void slot_add(double (&acc)[BSLOTS], int slot, double val) {
   for (int k = 0; k < BSLOTS; k++) acc[k] += (k == slot) ? val : 0.0;
 }
 
-// a bubble contribution: through the block's slot accumulators, or an FP64 atomic when the
-// label has no slot
-__device__ inline void bub_add(double (&sl)[BSLOTS], const int* __restrict__ labelSlot, double* accG, uint32_t lab,
+// a bubble contribution: through the block's slot accumulators, or the block's LDS table of the
+// fixed-point sums of labels without a slot
+__device__ inline void bub_add(double (&sl)[BSLOTS], const int* __restrict__ labelSlot, BubLds& agg, uint32_t lab,
                                double val) {
   const int s = labelSlot[lab];
   if (s >= 0) slot_add(sl, s, val);
-  else fx_add(accG, lab, val);
+  else bagg_add(agg, lab, val);
 }
 
 // Block sum of the first nv of v[]; thread 0 stores partial `bid` of each quantity.  nv must
@@ -568,11 +623,13 @@ __device__ __forceinline__ void k_pcg_init(Blk B_, const uint8_t* __restrict__ o
                                                   double* __restrict__ partials, double* __restrict__ bpart,
                                                   DevScalars* __restrict__ sc) {
   __shared__ double sh[BSLOTS * 16];
+  __shared__ BubLds agg;
   const int ns = nslots(sc->active_bubbles);
   double v[5] = {0, 0, 0, 0, 0};  // r.r, div.div, sum(b), n_water, anchored
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
   const double* yb = bv(bvec, BV_Y);
-  double* acc0 = bv(bvec, BV_ACCI);
+  bagg_clear(agg, bv(bvec, BV_ACCI));
+  __syncthreads();
   // the cell's value as the operator sees it: binary32 q (bubble cells: their y), plus qlo on a restart
   auto qe = [&](int d, uint8_t md) -> float {
     if (mask_type(md) == AGUA) return q[d];
@@ -602,7 +659,7 @@ __device__ __forceinline__ void k_pcg_init(Blk B_, const uint8_t* __restrict__ o
         s += qd(nb[k], md);
         if (mask_type(md) == AR) {
           if (md & MK_BUB) {
-            if (md & MK_BUBIN) bub_add(sl, labelSlot, acc0, rot[nb[k]], -qcd);
+            if (md & MK_BUBIN) bub_add(sl, labelSlot, agg, rot[nb[k]], -qcd);
           } else anchored = true;
         }
       }
@@ -618,6 +675,7 @@ __device__ __forceinline__ void k_pcg_init(Blk B_, const uint8_t* __restrict__ o
     q[c] = qc;
     if (qmode == 1 || (qmode == 2 && mask_type(m) != AGUA)) qlo[c] = 0.f;
   }
+  bagg_flush(agg);
   store_partials<5>(B_, v, 5, partials, sh);
   if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, bpart, sh);
 }
@@ -999,7 +1057,7 @@ struct ZArgs {  // level-0 outputs: z = MG result and the dots r.z, r.z_old
 
 // z and its dot/bubble contributions for one cell (x = MG output of level 0)
 __device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, float zoldc, float xv, float inv,
-                              double (&dv)[2], double (&sl)[BSLOTS]) {
+                              double (&dv)[2], double (&sl)[BSLOTS], BubLds& agg) {
   const float x = xv * inv;
   float z = 0.f;
   if (mask_type(om) == AGUA) {
@@ -1008,7 +1066,7 @@ __device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, f
     dv[1] += (double)rc * zoldc;
   } else if (om & MK_BUB) {
     const uint32_t lb = za.rot[c];
-    bub_add(sl, za.labelSlot, za.accZ, lb, (double)x / za.cellsB[lb]);
+    bub_add(sl, za.labelSlot, agg, lb, (double)x / za.cellsB[lb]);
   }
   za.znew[c] = z;
 }
@@ -1018,12 +1076,16 @@ __device__ __forceinline__ void k_zdots0(Blk B_, const float* __restrict__ x0, Z
                                                 const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[BSLOTS * 16];
+  __shared__ BubLds agg;
   const int ns = nslots(sc->active_bubbles);
   double dv[2] = {0, 0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  bagg_clear(agg, za.accZ);
+  __syncthreads();
   const int c = B_.x * blockDim.x + threadIdx.x;
   const float inv = za.scaled ? 1.f / *za.mgs : 1.f;
-  if (c < C) z_cell(za, c, za.opMask[c], za.r[c], za.zold[c], x0[c], inv, dv, sl);
+  if (c < C) z_cell(za, c, za.opMask[c], za.r[c], za.zold[c], x0[c], inv, dv, sl, agg);
+  bagg_flush(agg);
   store_partials<2>(B_, dv, 2, za.partials, sh);
   if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, za.bpart, sh);
 }
@@ -1120,8 +1182,11 @@ __device__ __forceinline__ void k_pass_z(Blk B_, const float* __restrict__ src, 
   if (sc->done) return;
   __shared__ float Tt[4][TCELLS];
   __shared__ double sh[BSLOTS * 16];
+  __shared__ BubLds agg;
   const int lane = threadIdx.x & 63;
   const int ns = nslots(sc->active_bubbles);
+  bagg_clear(agg, za.accZ);
+  __syncthreads();
   int tx, ty;
   block_tile(B_, &tx, &ty);
   double dv[2] = {0, 0};
@@ -1157,9 +1222,10 @@ __device__ __forceinline__ void k_pass_z(Blk B_, const float* __restrict__ src, 
         const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
         if (i < nx && j < ny)
           z_cell(za, (size_t)i + (size_t)j * nx, om[cq][rw], rv[cq][rw], zo[cq][rw],
-                 (cc.m[cq][rw] & 32) ? cc.v[cq][rw] : 0.f, inv, dv, sl);
+                 (cc.m[cq][rw] & 32) ? cc.v[cq][rw] : 0.f, inv, dv, sl, agg);
       }
   }
+  bagg_flush(agg);
   store_partials<2>(B_, dv, 2, za.partials, sh);
   if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, za.bpart, sh);
 }
@@ -1501,6 +1567,8 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
   __shared__ float CPt[4][100];
   __shared__ float W[CW * CW];
   __shared__ double sh[BSLOTS * 16];
+  __shared__ BubLds agg;
+  if (ZD) bagg_clear(agg, za.accZ);  // the barrier before pass D orders it before the adds
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   int bxl, byl;
   xcd_block2(B_, &bxl, &byl);
@@ -1624,12 +1692,13 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
           const int i = ax + cell_lx(lane, cq), j = ay + cell_ly(lane, rw);
           if (i < nx && j < ny)
             z_cell(za, (size_t)i + (size_t)j * nx, om[cq][rw], rv[cq][rw], zo[cq][rw],
-                   (cc.m[cq][rw] & 32) ? cc.v[cq][rw] : 0.f, inv, dv, sl);
+                   (cc.m[cq][rw] & 32) ? cc.v[cq][rw] : 0.f, inv, dv, sl, agg);
         }
     }
   }
   if (ZD) {
     const int ns = nslots(sc->active_bubbles);
+    bagg_flush(agg);
     store_partials<2>(B_, dv, 2, za.partials, sh);
     if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, za.bpart, sh);
   }
@@ -2034,6 +2103,7 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], zs[BSLOTS];
   __shared__ float P[SPMV_HN];
   __shared__ uint8_t M[SPMV_HN];
+  __shared__ BubLds agg;
   const int tid = threadIdx.x;
   const int ntx = (nx + SPMV_TW - 1) / SPMV_TW, nch = ntx * ((ny + SPMV_TH - 1) / SPMV_TH);
   int ch = xcd_block(B_.x, B_.gx);
@@ -2065,7 +2135,7 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
   const int ns = nslots(sc->active_bubbles);
   const double* Po = bv(B.bvec, BV_P0 + 1 - par);
   const double* accZ = bv(B.bvec, BV_ACCZ0 + par);
-  double* accA = bv(B.bvec, BV_ACCA0 + par);
+  bagg_clear(agg, bv(B.bvec, BV_ACCA0 + par));  // the loop's first barrier orders it before the adds
   // p of a cell from its loaded z, p_old and mask (bubble cells from the bubble vectors)
   auto peff = [&](int d, uint8_t md, float zd, float pd) -> float {
     if (mask_type(md) == AGUA) return (float)((double)zd + beta * (double)pd);
@@ -2109,7 +2179,7 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
           for (int k = 0; k < 4; k++) {
             if (!(m & (1 << k))) continue;
             s += (double)P[ne[k]];
-            if (M[ne[k]] & MK_BUBIN) bub_add(sl, B.labelSlot, accA, rot[nc[k]], -(double)pc);
+            if (M[ne[k]] & MK_BUBIN) bub_add(sl, B.labelSlot, agg, rot[nc[k]], -(double)pc);
           }
           const float a = (float)(mask_cnt(m) * (double)pc - s);
           ap[c] = a;
@@ -2121,6 +2191,7 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
     if (ch >= nch) break;
     load(ch);
   }
+  bagg_flush(agg);
   store_partials<1>(B_, v, 1, partA, sh);
   if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, bpartA, sh);
 }
@@ -2167,9 +2238,11 @@ __device__ __forceinline__ void k_true_res(Blk B_, const uint8_t* __restrict__ o
                                                   const DevScalars* __restrict__ sc) {
   if (true_res_skip(sc, gated)) return;
   __shared__ double sh[BSLOTS * 16];
+  __shared__ BubLds agg;
   const int ns = nslots(sc->active_bubbles);
   const double* yb = bv(bvec, BV_Y);
-  double* acc0 = bv(bvec, BV_ACCI);
+  bagg_clear(agg, bv(bvec, BV_ACCI));
+  __syncthreads();
   const double mean = sc->proj_mean;
   double v[1] = {0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -2189,12 +2262,13 @@ __device__ __forceinline__ void k_true_res(Blk B_, const uint8_t* __restrict__ o
         if (!(m & (1 << k))) continue;
         const uint8_t md = opMask[nb[k]];
         s += qe(nb[k], md);
-        if (md & MK_BUBIN) bub_add(sl, labelSlot, acc0, rot[nb[k]], -qc);
+        if (md & MK_BUBIN) bub_add(sl, labelSlot, agg, rot[nb[k]], -qc);
       }
       const double e = (-(double)div[c] - mean) - (mask_cnt(m) * qc - s);
       v[0] = e * e;
     }
   }
+  bagg_flush(agg);
   store_partials<1>(B_, v, 1, partials, sh);
   if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, bpart, sh);
 }

@@ -137,6 +137,16 @@ struct tq_batch {
   bool vmax_valid = false;
   std::vector<uint64_t> vmax_epoch;           // each tank's substep count when vmax_host was filled
   tq_batch_stats stats{};                     // host time per part of the steps (tq_batch_stats_get)
+  // a frame's table launches captured as a HIP graph and replayed while the launch list (bodies,
+  // grids, block offsets, record offsets) repeats; the records themselves are uploaded each frame
+  bool use_graphs = true;
+  struct FrameGraph {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec;
+    uint64_t used;
+  };
+  std::vector<FrameGraph> graphs;
+  uint64_t graph_clock = 0;
 };
 
 namespace {
@@ -288,6 +298,69 @@ int stage_round(tq_batch* b, const Plan& plan, const std::vector<int>& tank, con
   return 0;
 }
 
+int issue(tq_batch* b, Issue& is) {
+  const unsigned char* rec = b->up_dev + is.rec;
+  void* args[] = {&is.hdr, &rec};
+  TQ_HIP(hipLaunchKernel(is.tab, dim3(is.blocks), is.block, args, is.shm, b->st));
+  return 0;
+}
+
+void drop_graphs(tq_batch* b) {
+  for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
+  b->graphs.clear();
+}
+
+constexpr size_t GRAPH_CACHE = 8;  // frame graphs kept (substep-count patterns that recur)
+
+// the frame's launches from a captured graph: replayed when the launch list matches one captured
+// before, else captured now (the launches are recorded, not run, then the graph is launched)
+int launch_graph(tq_batch* b, std::vector<Issue>& issues) {
+  std::vector<uint64_t> key;
+  key.reserve(issues.size() * 8 + 1);
+  key.push_back((uint64_t)(uintptr_t)b->up_dev);
+  for (const Issue& is : issues) {
+    key.push_back((uint64_t)(uintptr_t)is.tab);
+    key.push_back(((uint64_t)is.block.x << 42) | ((uint64_t)is.block.y << 21) | is.block.z);
+    key.push_back(((uint64_t)is.shm << 32) | is.blocks);
+    key.push_back(((uint64_t)(uint32_t)is.hdr.ntank << 32) | (uint32_t)is.hdr.stride);
+    key.push_back(is.rec);
+    for (int t = 0; t <= is.hdr.ntank; t++) key.push_back(is.hdr.start[t]);
+  }
+  b->graph_clock++;
+  for (auto& g : b->graphs)
+    if (g.key == key) {
+      g.used = b->graph_clock;
+      b->stats.graph_replays++;
+      TQ_HIP(hipGraphLaunch(g.exec, b->st));
+      return 0;
+    }
+  hipGraph_t graph = nullptr;
+  TQ_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+  int e = 0;
+  for (Issue& is : issues)
+    if ((e = issue(b, is)) != 0) break;
+  const hipError_t ec = hipStreamEndCapture(b->st, &graph);
+  if (e) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return e;
+  }
+  if (ec != hipSuccess) return tq::hip_fail(ec, "hipStreamEndCapture", __FILE__, __LINE__);
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return tq::hip_fail(ei, "hipGraphInstantiate", __FILE__, __LINE__);
+  if (b->graphs.size() >= GRAPH_CACHE) {
+    auto old = std::min_element(b->graphs.begin(), b->graphs.end(),
+                                [](const tq_batch::FrameGraph& x, const tq_batch::FrameGraph& y) { return x.used < y.used; });
+    (void)hipGraphExecDestroy(old->exec);
+    b->graphs.erase(old);
+  }
+  b->graphs.push_back(tq_batch::FrameGraph{std::move(key), exec, b->graph_clock});
+  b->stats.graph_builds++;
+  TQ_HIP(hipGraphLaunch(exec, b->st));
+  return 0;
+}
+
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -311,6 +384,7 @@ int table_step(tq_batch* b) {
       for (const LaunchOp& o : L->ops) need += (tq::TAB_HDR + o.bytes + tq::TAB_ALIGN - 1) & ~(tq::TAB_ALIGN - 1);
   if (need > b->up_cap) {
     TQ_HIP(hipStreamSynchronize(b->st));
+    drop_graphs(b);  // they read the records at the old address
     if (b->up_host) (void)hipHostFree(b->up_host);
     if (b->up_dev) (void)hipFree(b->up_dev);
     b->up_host = nullptr;
@@ -351,10 +425,11 @@ int table_step(tq_batch* b) {
   }
   t1 = now_s();
   if (up > 0) TQ_HIP(hipMemcpyAsync(b->up_dev, b->up_host, up, hipMemcpyHostToDevice, b->st));
-  for (Issue& is : issues) {
-    const unsigned char* rec = b->up_dev + is.rec;
-    void* args[] = {&is.hdr, &rec};
-    TQ_HIP(hipLaunchKernel(is.tab, dim3(is.blocks), is.block, args, is.shm, b->st));
+  if (b->use_graphs) {
+    if (int e = launch_graph(b, issues)) return e;
+  } else {
+    for (Issue& is : issues)
+      if (int e = issue(b, is)) return e;
   }
   TQ_HIP(hipMemcpyAsync(b->vmax_host, b->vmax_dev, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, b->st));
   const double t2 = now_s();
@@ -371,6 +446,7 @@ int table_step(tq_batch* b) {
 
 void release(tq_batch* b) {
   stop_workers(b);
+  drop_graphs(b);
   for (tq_sim* s : b->tanks) tq_destroy(s);
   if (b->up_host) (void)hipHostFree(b->up_host);
   if (b->up_dev) (void)hipFree(b->up_dev);
@@ -481,6 +557,16 @@ int tq_batch_vmax(tq_batch* b, double* v, int* has) {
       return e;
     }
   }
+  return 0;
+}
+
+int tq_batch_use_graphs(tq_batch* b, int on) {
+  if (!b) {
+    tq::set_error("tq_batch_use_graphs: null handle");
+    return TQ_E_ARG;
+  }
+  b->use_graphs = on != 0;
+  if (!b->use_graphs) drop_graphs(b);
   return 0;
 }
 

@@ -45,7 +45,8 @@ class TuneResult(C.Structure):
 
 class BatchStats(C.Structure):
     _fields_ = [("steps", C.c_int), ("launches", C.c_int), ("upload_bytes", C.c_double), ("record_s", C.c_double),
-                ("plan_s", C.c_double), ("stage_s", C.c_double), ("issue_s", C.c_double), ("wait_s", C.c_double)]
+                ("plan_s", C.c_double), ("stage_s", C.c_double), ("issue_s", C.c_double), ("wait_s", C.c_double),
+                ("graph_builds", C.c_int), ("graph_replays", C.c_int)]
 
 
 class Tool(C.Structure):
@@ -87,6 +88,7 @@ SIGNATURES = {
     "tq_batch_step": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_int)]),
     "tq_batch_vmax": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "tq_batch_stats_get": (C.c_int, [C.c_void_p, C.POINTER(BatchStats), C.c_int]),
+    "tq_batch_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "tq_batch_destroy": (C.c_int, [C.c_void_p]),
     "tq_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "tq_substep": (C.c_int, [C.c_void_p, C.c_double]),
@@ -608,6 +610,11 @@ class Batch:
         has = (C.c_int * n)()
         _chk(load().tq_batch_vmax(self._h, v, has))
         return [v[k] if has[k] else None for k in range(n)]
+
+    def use_graphs(self, on):
+        """table mode: capture each frame's table launches as a HIP graph and replay it while the
+        launch list repeats (tq_batch_use_graphs; on by default)"""
+        _chk(load().tq_batch_use_graphs(self._h, 1 if on else 0))
 
     def stats(self, reset=False):
         """host seconds of the table steps by part, steps, the last step's launches and upload

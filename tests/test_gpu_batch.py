@@ -25,15 +25,19 @@ def frame_subs(ts, cenas, vmaxes):
     return [passo_frame(t.h, c.altura, v, t.dt_frame) for t, c, v in zip(ts, cenas, vmaxes)]
 
 
-@pytest.mark.parametrize("mode", ["table", "threads"])
+@pytest.mark.parametrize("mode", ["table", "table-eager", "threads"])
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
 def test_batch_matches_tanks_stepped_alone(tq, prec, mode):
+    """Frame by frame: in table mode the frames after the first replay the captured frame graph
+    (table-eager launches every table kernel itself), with host reads between the replays."""
     from tanque_hip import scenes
     p = tq.PREC_FP32 if prec == "fp32" else tq.PREC_MIXED
     ts = scenes.batched_mazes(n=5, seed=3)
     opts = {"iteracoes": 3, "precision": p}
     cenas = [t.cena() for t in ts]
-    b = tq.Batch(cenas, [opts] * len(ts), threads=3, mode=mode)
+    b = tq.Batch(cenas, [opts] * len(ts), threads=3, mode=mode.split("-")[0])
+    if mode == "table-eager":
+        b.use_graphs(False)
     alone = [tq.SimulacaoHIP(t.cena(), iteracoes=3, precision=p) for t in ts]
     for t, sb, sa in zip(ts, b.tanks, alone):
         sb.adicionar(t.particles, False)
@@ -42,8 +46,13 @@ def test_batch_matches_tanks_stepped_alone(tq, prec, mode):
         subs = frame_subs(ts, cenas, b.vmax())
         assert subs == frame_subs(ts, cenas, [s.vMax for s in alone])
         b.step(ts[0].dt_frame, subs)
-        for sa, k in zip(alone, subs):
-            sa.avancar(ts[0].dt_frame, k)
+        for k, (sa, sb, ns) in enumerate(zip(alone, b.tanks, subs)):
+            sa.avancar(ts[0].dt_frame, ns)
+            assert np.array_equal(sb.particulas().view(np.uint32), sa.particulas().view(np.uint32)), (f, k)
+    if mode == "table":
+        st = b.stats()
+        log("\n  frame graphs: %d captured, %d replayed" % (st["graph_builds"], st["graph_replays"]))
+        assert st["graph_replays"] >= 1
     for k, (t, sb, sa) in enumerate(zip(ts, b.tanks, alone)):
         pb, pa = sb.particulas(), sa.particulas()
         log("\n  %s %s tank %d (%dx%d): %d particles batched, %d alone" % (mode, prec, k, t.nx, t.ny, len(pb),
