@@ -142,44 +142,82 @@ def run_alto(tq, work, precision, tol=1e-6, maxit=200, schedule=None):
     return res, sched
 
 
-def run_batched(tq, frames=20, threads=8):
-    """The batched configuration (64 maze tanks, fixed 3 PCG iterations as the reference): the
-    same frames stepped tank after tank (sequential launches, each tank waited for) and as one
-    batch (tq_batch_step), each from fresh identical states; tank-steps/s = tanks x frames / s.
-    FP32, then the mixed-precision preconditioner (under "mixed")."""
+def run_batched(tq, frames=None, threads=8):
+    """The batched configuration (64 maze tanks of 64-384 x 48-256 cells, fixed 3 PCG iterations as
+    the reference, frames of 1/240 s) over all its frames, from fresh identical states, in three
+    launch models:
+      sequential  tank after tank from one thread, each tank's frame (avancar) waited for before
+                  the next tank's;
+      threads     every tank on its own stream, the frames enqueued from 8 host threads
+                  (tq_batch_create);
+      table       one table launch per kernel of the step over all 64 tanks
+                  (tq_batch_create_table), issued one by one; "table_graph" replays each frame
+                  from a HIP graph captured from the first frame with the same launch list.
+    Substep counts come from every tank's vMax after the previous frame (passo_frame).
+    tank-steps/s = tanks x frames / s.  The table run ends bit-equal to the sequential one (every
+    tank's particles compared).  Then the mixed-precision preconditioner in table mode."""
     from tanque_hip import scenes
     from tanque_hip.scenes import passo_frame
     ts = scenes.batched_mazes()
+    frames = frames or ts[0].frames
     res = {"tanks": len(ts), "cells": sum(t.nx * t.ny for t in ts), "frames_timed": frames,
            "of_frames": ts[0].frames, "threads": threads, "pcg": "3 iterations (reference)"}
-    for prec, key in ((tq.PREC_FP32, None), (tq.PREC_MIXED, "mixed")):
-        out = res if key is None else res.setdefault(key, {})
-        for mode in ("sequential", "batched"):
-            cenas = [t.cena() for t in ts]
-            b = tq.Batch(cenas, [{"iteracoes": 3, "precision": prec}] * len(ts), threads=threads)
-            for t, s in zip(ts, b.tanks):
-                s.adicionar(t.particles, False)
-            res["particles"] = sum(s.n for s in b.tanks)
-            dt = ts[0].dt_frame
+    dt = ts[0].dt_frame
 
-            def frame():
-                subs = [passo_frame(t.h, c.altura, s.vMax, dt) for t, c, s in zip(ts, cenas, b.tanks)]
-                if mode == "sequential":
-                    for s, k in zip(b.tanks, subs):
-                        s.avancar(dt, k)
-                else:
-                    b.step(dt, subs)
-                return sum(subs)
+    def run(mode, prec):
+        cenas = [t.cena() for t in ts]
+        b = tq.Batch(cenas, [{"iteracoes": 3, "precision": prec}] * len(ts), threads=threads,
+                     mode="threads" if mode in ("sequential", "threads") else "table")
+        if mode == "table_graph":
+            b.use_graphs(True)
+        for t, s in zip(ts, b.tanks):
+            s.adicionar(t.particles, False)
+        res["particles"] = sum(s.n for s in b.tanks)
+        table = b.mode == "table"
 
-            frame()
-            t0 = time.perf_counter()
-            subs = sum(frame() for _ in range(frames))
-            el = time.perf_counter() - t0
-            out[mode] = {"tank_steps_per_s": round(len(ts) * frames / el, 1), "tank_substeps_per_s": round(subs / el, 1)}
-            b.close()
-        out["batched_over_sequential"] = round(out["batched"]["tank_steps_per_s"] / out["sequential"]["tank_steps_per_s"], 3)
-    res["mixed_batched_over_fp32_batched"] = round(res["mixed"]["batched"]["tank_steps_per_s"] /
-                                                   res["batched"]["tank_steps_per_s"], 3)
+        def frame():
+            vm = b.vmax() if table else [s.vMax for s in b.tanks]
+            subs = [passo_frame(t.h, c.altura, v, dt) for t, c, v in zip(ts, cenas, vm)]
+            if mode == "sequential":
+                for s, k in zip(b.tanks, subs):
+                    s.avancar(dt, k)
+            else:
+                b.step(dt, subs)
+            return sum(subs)
+
+        if table:
+            b.stats(reset=True)
+        t0 = time.perf_counter()
+        subs = sum(frame() for _ in range(frames))
+        el = time.perf_counter() - t0
+        out = {"tank_steps_per_s": round(len(ts) * frames / el, 1), "tank_substeps_per_s": round(subs / el, 1),
+               "seconds": round(el, 3), "mean_substeps": round(subs / frames / len(ts), 3)}
+        if table:
+            st = b.stats()
+            out["host_ms_per_frame"] = {k[:-2]: round(st[k] / frames * 1e3, 3)
+                                        for k in ("record_s", "plan_s", "stage_s", "issue_s", "wait_s")}
+            out["table_launches_last_frame"] = st["launches"]
+            out["record_bytes_last_frame"] = int(st["upload_bytes"])
+            out["frame_graphs"] = {"captured": st["graph_builds"], "replayed": st["graph_replays"]}
+        return b, out
+
+    seq, res["sequential"] = run("sequential", tq.PREC_FP32)
+    seq_parts = [s.particulas().view(np.uint32).copy() for s in seq.tanks]
+    seq.close()
+    for mode in ("threads", "table_graph", "table"):
+        b, res[mode] = run(mode, tq.PREC_FP32)
+        if mode == "table":
+            res["table_bit_equal_to_sequential"] = all(
+                np.array_equal(s.particulas().view(np.uint32), p) for s, p in zip(b.tanks, seq_parts))
+        b.close()
+        log("batched %s: %s" % (mode, res[mode]))
+    res["batched"] = res["table"]
+    res["batched_over_sequential"] = round(res["table"]["tank_steps_per_s"] / res["sequential"]["tank_steps_per_s"], 3)
+    res["threads_over_sequential"] = round(res["threads"]["tank_steps_per_s"] / res["sequential"]["tank_steps_per_s"], 3)
+    b, res["mixed_table"] = run("table", tq.PREC_MIXED)
+    b.close()
+    res["mixed_batched_over_fp32_batched"] = round(res["mixed_table"]["tank_steps_per_s"] /
+                                                   res["table"]["tank_steps_per_s"], 3)
     return res
 
 
@@ -590,6 +628,7 @@ def main():
     ap.add_argument("--no-variants", action="store_true", help="skip the partial-FP16 variants")
     ap.add_argument("--no-alto", action="store_true", help="skip the Alto (cup + faucet) runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (64 tanks) runs")
+    ap.add_argument("--batched-frames", type=int, default=0, help="frames of the batched runs (0: all 500)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (funnel) runs")
     ap.add_argument("--no-autotune", action="store_true", help="run with the built-in launch shapes")
     ap.add_argument("--summarize-profiles", nargs=5, metavar=("FETCH_DIR", "WRITE_DIR", "TRACE_DIR", "OUT_JSON", "OUT_MD"),
@@ -719,7 +758,7 @@ def main():
 
     batched = None
     if not args.no_batched:
-        batched = run_batched(tq)
+        batched = run_batched(tq, frames=args.batched_frames or None)
         log("batched: %s" % batched)
 
     adjoint = None

@@ -196,8 +196,8 @@ int tq_batch_step(tq_batch* b, double dt, const int* substeps);
 /* every tank's vMax (tq_vmax; has[k] as its `has`), v and has of n entries (either may be null):
  * a table step leaves them in host memory, so after one they are read without a device wait */
 int tq_batch_vmax(tq_batch* b, double* v, int* has);
-/* table mode: a frame's table launches are captured as a HIP graph and replayed while the launch
- * list repeats (on: 1, the default; 0 launches them one by one) */
+/* table mode: on = 1 captures a frame's table launches as a HIP graph, replayed by later frames
+ * with the same launch list (the same substep counts); 0 (the default) issues them one by one */
 int tq_batch_use_graphs(tq_batch* b, int on);
 /* where a table batch's steps spend host time (seconds, summed over steps since the last reset):
  * recording the tanks' frames, merging the rounds, copying the records into the upload buffer,

@@ -12,8 +12,10 @@
 //      their order (a longest-common-subsequence merge, added one tank at a time; the merge of a
 //      round is cached by the tanks' sequences, which repeat from frame to frame);
 //   3. uploads every tank's records of the frame in one copy and issues each group as one table
-//      launch (k_tab, tq_launch.hpp) of up to TAB_MAX tanks, one after another on the stream,
-//      then gathers every tank's vMax into one host buffer (tq_batch_vmax) and waits.
+//      launch (k_tab, tq_launch.hpp) of up to TAB_MAX tanks, one after another on the stream (or
+//      replays a HIP graph of them captured from an earlier frame with the same launch list,
+//      tq_batch_use_graphs), then gathers every tank's vMax into one host buffer (tq_batch_vmax)
+//      and waits.
 // Each launch covers all tanks, so the device runs every phase across the batch's 2-3 M cells
 // instead of tank by tank, and a frame costs one launch per phase rather than one per phase and
 // tank.  The table runs each body with the tank's own logical block, grid and arguments, so every
@@ -138,8 +140,11 @@ struct tq_batch {
   std::vector<uint64_t> vmax_epoch;           // each tank's substep count when vmax_host was filled
   tq_batch_stats stats{};                     // host time per part of the steps (tq_batch_stats_get)
   // a frame's table launches captured as a HIP graph and replayed while the launch list (bodies,
-  // grids, block offsets, record offsets) repeats; the records themselves are uploaded each frame
-  bool use_graphs = true;
+  // grids, block offsets, record offsets) repeats; the records themselves are uploaded each frame.
+  // Off by default: over the 500-frame maze batch the substep counts vary from tank to tank and
+  // frame to frame, so most frames are a new pattern (386 captures in 500 frames), and a replayed
+  // frame runs no faster than the same launches issued one by one (the issue overlaps the device).
+  bool use_graphs = false;
   struct FrameGraph {
     std::vector<uint64_t> key;
     hipGraphExec_t exec;

@@ -25,19 +25,20 @@ def frame_subs(ts, cenas, vmaxes):
     return [passo_frame(t.h, c.altura, v, t.dt_frame) for t, c, v in zip(ts, cenas, vmaxes)]
 
 
-@pytest.mark.parametrize("mode", ["table", "table-eager", "threads"])
+@pytest.mark.parametrize("mode", ["table", "table-graph", "threads"])
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
 def test_batch_matches_tanks_stepped_alone(tq, prec, mode):
-    """Frame by frame: in table mode the frames after the first replay the captured frame graph
-    (table-eager launches every table kernel itself), with host reads between the replays."""
+    """Frame by frame: table-graph captures the first frame's table launches as a HIP graph and
+    replays it for the frames with the same substep counts, with host reads between the replays
+    (table launches every table kernel itself)."""
     from tanque_hip import scenes
     p = tq.PREC_FP32 if prec == "fp32" else tq.PREC_MIXED
     ts = scenes.batched_mazes(n=5, seed=3)
     opts = {"iteracoes": 3, "precision": p}
     cenas = [t.cena() for t in ts]
     b = tq.Batch(cenas, [opts] * len(ts), threads=3, mode=mode.split("-")[0])
-    if mode == "table-eager":
-        b.use_graphs(False)
+    if mode == "table-graph":
+        b.use_graphs(True)
     alone = [tq.SimulacaoHIP(t.cena(), iteracoes=3, precision=p) for t in ts]
     for t, sb, sa in zip(ts, b.tanks, alone):
         sb.adicionar(t.particles, False)
@@ -49,7 +50,7 @@ def test_batch_matches_tanks_stepped_alone(tq, prec, mode):
         for k, (sa, sb, ns) in enumerate(zip(alone, b.tanks, subs)):
             sa.avancar(ts[0].dt_frame, ns)
             assert np.array_equal(sb.particulas().view(np.uint32), sa.particulas().view(np.uint32)), (f, k)
-    if mode == "table":
+    if mode == "table-graph":
         st = b.stats()
         log("\n  frame graphs: %d captured, %d replayed" % (st["graph_builds"], st["graph_replays"]))
         assert st["graph_replays"] >= 1
