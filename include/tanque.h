@@ -271,6 +271,9 @@ typedef struct tq_solver_stats {
   double true_resid_iterate;  /* binary64 true residual of the compensated iterate at the stop; -1 if not computed */
   int first_iterations;       /* iterations of the first segment (before any restart) */
   int restarts;               /* residual replacements of the last solve */
+  /* the air tracker's last update (tq_update_air): air components, (component, old label)
+   * overlap pairs and bubble candidates (components of at least 4 cells of volume) */
+  int air_components, overlap_pairs, bubble_candidates;
 } tq_solver_stats;
 int tq_solver_stats_get(tq_sim* s, tq_solver_stats* st, int compute_true_residual);
 

@@ -318,6 +318,53 @@ __device__ void block_bitonic(unsigned long long* keys, uint32_t* vals, int n) {
     }
 }
 
+// Lists of overlap pairs and of candidate components up to this long are sorted, and labels
+// are assigned, in LDS; longer lists use global memory and the reference's sequential pass.
+constexpr int BK_LDS = 4096;
+static_assert(BK_LDS == MAX_BOLHAS, "the label scans cover BK_LDS entries");
+
+// Exclusive prefix counts of the flags f(i), i in [0, n), n <= 4 * 1024, in a 1024-thread
+// block: idx[i] = number of flagged entries before i (written where f(i)); returns the total in
+// every thread.  `ws` holds 17 ints of LDS.
+template <class F>
+__device__ inline int block_scan_flags(int n, F f, uint16_t* idx, int* ws) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int v[4], s = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int i = 4 * tid + q;
+    v[q] = (i < n && f(i)) ? 1 : 0;
+    s += v[q];
+  }
+  int x = s;  // inclusive scan over the wave
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) ws[w] = x;
+  __syncthreads();
+  if (tid == 0) {
+    int a = 0;
+    for (int k = 0; k < 16; k++) {
+      const int t = ws[k];
+      ws[k] = a;
+      a += t;
+    }
+    ws[16] = a;
+  }
+  __syncthreads();
+  int off = ws[w] + x - s;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    if (v[q]) idx[4 * tid + q] = (uint16_t)off;
+    off += v[q];
+  }
+  const int total = ws[16];
+  __syncthreads();
+  return total;
+}
+
 __device__ __forceinline__ void k_bookkeep(Blk B_, const uint32_t* __restrict__ compIdxTotal, const uint32_t* __restrict__ cells,
                                                    const uint32_t* __restrict__ atm, const uint32_t* __restrict__ topo,
                                                    float* __restrict__ alvo, int32_t* __restrict__ dono,
@@ -328,37 +375,49 @@ __device__ __forceinline__ void k_bookkeep(Blk B_, const uint32_t* __restrict__ 
                                                    uint8_t* __restrict__ vivo, float* __restrict__ pBolha,
                                                    DevScalars* __restrict__ sc) {
   __shared__ uint32_t total[MAX_BOLHAS];
-  __shared__ uint8_t vivoNovo[MAX_BOLHAS];
+  __shared__ uint8_t vivoNovo[MAX_BOLHAS], vivoV[MAX_BOLHAS];
+  __shared__ float pB[MAX_BOLHAS];
+  __shared__ unsigned long long lkeys[BK_LDS];
+  __shared__ uint32_t lvals[BK_LDS];
+  __shared__ int32_t cdono[BK_LDS];
+  __shared__ uint32_t clab[BK_LDS];
   __shared__ uint32_t ncand_s;
   const int tid = threadIdx.x;
   const uint32_t ncomp = *compIdxTotal;
   const uint32_t np = sc->npairs;
   const bool primeira = sc->primeira != 0;
-  for (int r = tid; r < MAX_BOLHAS; r += 1024) { total[r] = 0; vivoNovo[r] = 0; }
+  for (int r = tid; r < MAX_BOLHAS; r += 1024) {
+    total[r] = 0;
+    vivoNovo[r] = 0;
+    vivoV[r] = vivo[r];
+    pB[r] = pBolha[r];
+  }
   if (tid == 0) { sc->ncomp = ncomp; ncand_s = 0; }
   __syncthreads();
   // ---- overlap totals of the components that stay (ar.js:101-106)
+  unsigned long long* SK = np <= (uint32_t)BK_LDS ? lkeys : skeys;
+  uint32_t* SV = np <= (uint32_t)BK_LDS ? lvals : svals;
   for (uint32_t i = tid; i < np; i += 1024) {
     const uint32_t h = hlist[i];
     const unsigned long long key = hkeys[h];
     const uint32_t k = (uint32_t)(key >> 12), r = (uint32_t)(key & 4095);
     if (topo[k] || cells[k] >= MIN_BOLHA_CELULAS) atomicAdd(&total[r], hcount[h]);
-    skeys[i] = ((unsigned long long)k << 32) | hfirst[h];
-    svals[i] = h;
+    SK[i] = ((unsigned long long)k << 32) | hfirst[h];
+    SV[i] = h;
   }
   __syncthreads();
   // ---- per-component volume over old labels in first-appearance order (ar.js:108-125)
   if (np > 0 && !primeira) {
-    block_bitonic(skeys, svals, (int)np);
+    block_bitonic(SK, SV, (int)np);
     for (uint32_t i = tid; i < np; i += 1024) {
-      const uint32_t k = (uint32_t)(skeys[i] >> 32);
-      if (i > 0 && (uint32_t)(skeys[i - 1] >> 32) == k) continue;  // one thread per component segment
+      const uint32_t k = (uint32_t)(SK[i] >> 32);
+      if (i > 0 && (uint32_t)(SK[i - 1] >> 32) == k) continue;  // one thread per component segment
       if (topo[k] || cells[k] < MIN_BOLHA_CELULAS) continue;
       double v = atm[k];
       uint32_t melhor = 0;
       int dn = -1;
-      for (uint32_t e = i; e < np && (uint32_t)(skeys[e] >> 32) == k; e++) {
-        const uint32_t h = svals[e];
+      for (uint32_t e = i; e < np && (uint32_t)(SK[e] >> 32) == k; e++) {
+        const uint32_t h = SV[e];
         const uint32_t r = (uint32_t)(hkeys[h] & 4095);
         const uint32_t qv = hcount[h];
         v += (double)volAlvo[r] * ((double)qv / (double)total[r]);
@@ -368,9 +427,11 @@ __device__ __forceinline__ void k_bookkeep(Blk B_, const uint32_t* __restrict__ 
       dono[k] = dn;
       label[k] = 0xFFFFFFFFu;  // marks "volume computed from overlaps"
     }
-    __syncthreads();
   }
+  __syncthreads();  // the pair list is no longer read (its LDS holds the candidates next)
   // ---- volumes of the other components, candidates (ar.js:110-130)
+  unsigned long long* CK = ncomp <= (uint32_t)BK_LDS ? lkeys : skeys;
+  uint32_t* CV = ncomp <= (uint32_t)BK_LDS ? lvals : svals;
   for (uint32_t k = tid; k < ncomp; k += 1024) {
     float a = 0.f;
     if (topo[k]) { label[k] = 1u; alvo[k] = 0.f; continue; }
@@ -383,40 +444,106 @@ __device__ __forceinline__ void k_bookkeep(Blk B_, const uint32_t* __restrict__ 
     if ((double)a >= MIN_BOLHA_CELULAS) {
       const uint32_t slot = atomicAdd(&ncand_s, 1u);
       // descending volume, ascending component index (a stable sort of the raster order)
-      skeys[slot] = ((unsigned long long)(~__float_as_uint(a)) << 32) | k;
-      svals[slot] = k;
+      CK[slot] = ((unsigned long long)(~__float_as_uint(a)) << 32) | k;
+      CV[slot] = k;
     }
   }
   __syncthreads();
   const int ncand = (int)ncand_s;
-  if (ncand > 0) block_bitonic(skeys, svals, ncand);
-  __syncthreads();
-  // ---- stable labels, free labels, born bubbles (ar.js:131-145); sequential like the reference
-  if (tid == 0) {
-    for (int e = 0; e < ncand; e++) {
-      const uint32_t k = svals[e];
-      const int r = dono[k];
-      if (r >= 2 && !vivoNovo[r]) { label[k] = (uint32_t)r; vivoNovo[r] = 1; }
+  if (ncand > 0) block_bitonic(CK, CV, ncand);
+  // the sequential pass's inputs in LDS: every candidate's owner (the old label of largest overlap)
+  const bool lds = ncand <= BK_LDS;
+  if (lds)
+    for (int e = tid; e < ncand; e += 1024) {
+      cdono[e] = dono[CV[e]];
+      clab[e] = 0u;  // label[k] of a candidate is 0 here
     }
+  __syncthreads();
+  // ---- stable labels, free labels, born bubbles (ar.js:131-145).  The reference walks the sorted
+  // candidates twice: (1) a candidate takes its owner label r >= 2 unless an earlier candidate took
+  // it; (2) every candidate still unlabelled takes the smallest label that is neither taken nor
+  // alive before, in candidate order, until the labels run out, and a born bubble inherits its
+  // owner's pressure.  In LDS both passes run in parallel with the same result: (1) r goes to the
+  // smallest candidate index owning it (an atomic minimum); (2) the j-th unlabelled candidate
+  // takes the j-th free label (two prefix counts).  Pass 2 reads the owners' pressures while it
+  // writes the born ones; when an owner is itself a free label (never, while every owner is a
+  // label alive before) a later read could see an earlier write, so that case, and lists beyond
+  // LDS, keep the sequential pass.
+  __shared__ int ws[17];
+  __shared__ int odd;
+  uint16_t* F = reinterpret_cast<uint16_t*>(lkeys);  // the free labels in ascending order
+  uint16_t* rank = F + MAX_BOLHAS;                    // rank of an unlabelled candidate
+  uint16_t* fpos = rank + MAX_BOLHAS;                 // position of a free label in F
+  if (lds) {
+    int* win = reinterpret_cast<int*>(total);  // the overlap totals are no longer read
+    for (int r = tid; r < MAX_BOLHAS; r += 1024) win[r] = 0x7FFFFFFF;
+    if (tid == 0) odd = 0;
+    __syncthreads();
+    for (int e = tid; e < ncand; e += 1024)
+      if (cdono[e] >= 2) atomicMin(&win[cdono[e]], e);
+    __syncthreads();
+    for (int e = tid; e < ncand; e += 1024) {
+      const int r = cdono[e];
+      if (r >= 2 && win[r] == e) {
+        clab[e] = (uint32_t)r;
+        vivoNovo[r] = 1;
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < ncand; e += 1024) {
+      const int dn = cdono[e];
+      if (clab[e] == 0u && dn >= 2 && !vivoNovo[dn] && !vivoV[dn]) odd = 1;  // an owner that is a free label
+    }
+    __syncthreads();
+  }
+  if (lds && !odd) {
+    const int nfree = block_scan_flags(MAX_BOLHAS, [&](int r) { return r >= 2 && !vivoNovo[r] && !vivoV[r]; },
+                                       fpos, ws);
+    for (int r = 2 + tid; r < MAX_BOLHAS; r += 1024)
+      if (!vivoNovo[r] && !vivoV[r]) F[fpos[r]] = (uint16_t)r;
+    (void)block_scan_flags(ncand, [&](int e) { return clab[e] == 0u; }, rank, ws);  // its barriers order F
+    for (int e = tid; e < ncand; e += 1024) {
+      if (clab[e] != 0u || rank[e] >= nfree) continue;
+      const int lv = F[rank[e]];
+      clab[e] = (uint32_t)lv;
+      vivoNovo[lv] = 1;
+      const int dn = cdono[e];
+      const float p = dn >= 2 ? pB[dn] : 0.f;
+      pB[lv] = p;
+      pBolha[lv] = p;
+    }
+  } else if (tid == 0) {
+    if (!lds)
+      for (int e = 0; e < ncand; e++) {
+        const int r = dono[CV[e]];
+        if (r >= 2 && !vivoNovo[r]) {
+          label[CV[e]] = (uint32_t)r;
+          vivoNovo[r] = 1;
+        }
+      }
     int livre = 2;
     for (int e = 0; e < ncand; e++) {
-      const uint32_t k = svals[e];
-      if (label[k]) continue;
-      while (livre < MAX_BOLHAS && (vivoNovo[livre] || vivo[livre])) livre++;
+      if (lds ? clab[e] != 0u : label[CV[e]] != 0u) continue;
+      while (livre < MAX_BOLHAS && (vivoNovo[livre] || vivoV[livre])) livre++;
       if (livre >= MAX_BOLHAS) break;
-      label[k] = (uint32_t)livre;
+      if (lds) clab[e] = (uint32_t)livre;
+      else label[CV[e]] = (uint32_t)livre;
       vivoNovo[livre] = 1;
-      const int dn = dono[k];
-      pBolha[livre] = dn >= 2 ? pBolha[dn] : 0.f;
+      const int dn = lds ? cdono[e] : dono[CV[e]];
+      pB[livre] = dn >= 2 ? pB[dn] : 0.f;
+      pBolha[livre] = pB[livre];
     }
   }
   __syncthreads();
   // ---- volumeAlvo, vivo (ar.js:147-150)
   for (int r = tid; r < MAX_BOLHAS; r += 1024) { volAlvo[r] = 0.f; vivo[r] = vivoNovo[r]; }
+  if (lds)
+    for (int e = tid; e < ncand; e += 1024) label[CV[e]] = clab[e];
   __syncthreads();
   for (int e = tid; e < ncand; e += 1024) {
-    const uint32_t k = svals[e];
-    if (label[k] >= 2) volAlvo[label[k]] = alvo[k];
+    const uint32_t k = CV[e];
+    const uint32_t lb = lds ? clab[e] : label[k];
+    if (lb >= 2) volAlvo[lb] = alvo[k];
   }
   // ---- clear the overlap table for the next update
   for (uint32_t i = tid; i < np; i += 1024) {
@@ -424,7 +551,12 @@ __device__ __forceinline__ void k_bookkeep(Blk B_, const uint32_t* __restrict__ 
     hkeys[h] = HEMPTY; hcount[h] = 0; hfirst[h] = 0xFFFFFFFFu;
   }
   __syncthreads();
-  if (tid == 0) { sc->npairs = 0; sc->primeira = 0; }
+  if (tid == 0) {
+    sc->last_pairs = np;
+    sc->last_cand = (uint32_t)ncand;
+    sc->npairs = 0;
+    sc->primeira = 0;
+  }
 }
 
 __device__ __forceinline__ void k_label_write(Blk B_, const int32_t* __restrict__ comp, const uint32_t* __restrict__ label,

@@ -856,6 +856,9 @@ int tq_solver_stats_get(tq_sim* s, tq_solver_stats* st, int compute_true) {
   st->true_resid_iterate = h.true_done ? h.true_norm : -1.0;
   st->first_iterations = h.restarts > 0 ? h.iters_first : h.iters;
   st->restarts = h.restarts;
+  st->air_components = (int)h.ncomp;
+  st->overlap_pairs = (int)h.last_pairs;
+  st->bubble_candidates = (int)h.last_cand;
   st->true_resid = -1;
   if (compute_true) {
     double tr = 0;

@@ -75,7 +75,9 @@ struct DevScalars {
   uint32_t add_count;   // particles accepted by the last add
   uint32_t cand_count;  // label-propagation candidates
   uint32_t ncomp;       // air components of the last tracker update
-  uint32_t npairs;      // (component, old label) pairs of the last tracker update
+  uint32_t npairs;      // (component, old label) pairs of the tracker update in progress
+  uint32_t last_pairs;  // pairs of the last tracker update
+  uint32_t last_cand;   // bubble candidates (components of >= 4 cells of volume) of the last update
   uint32_t ticket[16];  // last-block counters of the reductions
   int done;             // PCG converged / stopped
   int iters;            // PCG iterations run

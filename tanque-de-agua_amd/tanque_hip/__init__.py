@@ -58,7 +58,8 @@ class SolverStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("norm0", C.c_double), ("norm", C.c_double), ("norm_b", C.c_double),
                 ("true_resid", C.c_double), ("levels", C.c_int), ("active_bubbles", C.c_int),
                 ("total_iterations", C.c_longlong), ("solves", C.c_longlong), ("operator_builds", C.c_longlong),
-                ("true_resid_iterate", C.c_double), ("first_iterations", C.c_int), ("restarts", C.c_int)]
+                ("true_resid_iterate", C.c_double), ("first_iterations", C.c_int), ("restarts", C.c_int),
+                ("air_components", C.c_int), ("overlap_pairs", C.c_int), ("bubble_candidates", C.c_int)]
 
 
 class Faucet(C.Structure):
