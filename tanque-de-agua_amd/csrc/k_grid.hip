@@ -8,13 +8,79 @@ namespace tq {
 // ================================================================== #superficie (fisica-cpu.js:306-374)
 // One workgroup computes a 32x32 tile from a 58x58 LDS region (halo 13 = the dependency radius
 // of ext3, blur, ext3, blur, ext3, normal, curvature), so the 15 full-grid passes of the
-// reference become one launch.  Cells of the halo are computed too (garbage near the region
-// edge) and discarded.
+// reference become one launch.  Each pass computes only the box of cells the rest of the chain
+// reads -- its radius around the owned tile shrinks from 12 (the first extension layer) to 1
+// (the normals) -- so the halo's cells are computed no more often than the owned tile needs.
 constexpr int SF_T = 32, SF_H = 13, SF_R = SF_T + 2 * SF_H;  // 58
 constexpr int SF_N = SF_R * SF_R;                            // 3364
 constexpr int SF_THREADS = 1024;
 constexpr int SF_K = (SF_N + SF_THREADS - 1) / SF_THREADS;   // region cells per thread (4)
 static_assert(SF_T * SF_T == SF_THREADS, "one owned cell per thread");
+
+// fn(e, lx, ly) for the region cells within RX columns and RY rows of the owned tile
+template <int RX, int RY, class Fn>
+__device__ __forceinline__ void sf_box(Fn fn) {
+  constexpr int W = SF_T + 2 * RX, H = SF_T + 2 * RY, N = W * H;
+  for (int i = threadIdx.x; i < N; i += SF_THREADS) {
+    const int by = i / W, bx = i - by * W;
+    const int lx = SF_H - RX + bx, ly = SF_H - RY + by;
+    fn(lx + ly * SF_R, lx, ly);
+  }
+}
+
+// one extension layer (#estenderNaParede): an unfilled cell takes the mean of its filled
+// 4-neighbours; computed on the box of radius R (it reads radius R + 1)
+template <int R>
+__device__ __forceinline__ void sf_layer(float* f, float* t, uint8_t* fl, const uint8_t* ty, int I0, int J0, int nx,
+                                         int ny) {
+#pragma clang fp contract(off)
+  sf_box<R, R>([&](int e, int lx, int ly) {
+    if (fl[e] || ty[e] == 255) return;
+    const int gi = I0 + lx, gj = J0 + ly;
+    double sum = 0;
+    int n = 0;
+    if (gi > 0 && fl[e - 1] == 1) { sum += (double)f[e - 1]; n++; }
+    if (gi < nx - 1 && fl[e + 1] == 1) { sum += (double)f[e + 1]; n++; }
+    if (gj > 0 && fl[e - SF_R] == 1) { sum += (double)f[e - SF_R]; n++; }
+    if (gj < ny - 1 && fl[e + SF_R] == 1) { sum += (double)f[e + SF_R]; n++; }
+    if (n) { t[e] = (float)(sum / n); fl[e] = 2; }
+  });
+  __syncthreads();
+  sf_box<R, R>([&](int e, int, int) {
+    if (fl[e] == 2) { f[e] = t[e]; fl[e] = 1; }
+  });
+  __syncthreads();
+}
+
+// the three layers of a passada whose last layer writes radius R (the filled flags first, on
+// the radius R + 3 the first layer reads)
+template <int R>
+__device__ __forceinline__ void sf_extend(float* f, float* t, uint8_t* fl, const uint8_t* ty, int I0, int J0, int nx,
+                                          int ny) {
+  sf_box<R + 3, R + 3>([&](int e, int, int) { fl[e] = (ty[e] != SOLIDO && ty[e] != 255) ? 1 : 0; });
+  __syncthreads();
+  sf_layer<R + 2>(f, t, fl, ty, I0, J0, nx, ny);
+  sf_layer<R + 1>(f, t, fl, ty, I0, J0, nx, ny);
+  sf_layer<R>(f, t, fl, ty, I0, J0, nx, ny);
+}
+
+// the separable (1 2 1) blur, writing radius R (the row pass on radius R columns and R + 1 rows)
+template <int R>
+__device__ __forceinline__ void sf_blur(float* f, float* t, int I0, int J0, int nx, int ny) {
+#pragma clang fp contract(off)
+  sf_box<R, R + 1>([&](int e, int lx, int) {
+    const int gi = I0 + lx;
+    const int em = gi > 0 ? e - 1 : e, ep = gi < nx - 1 ? e + 1 : e;
+    t[e] = (float)(((double)f[em] + 2 * (double)f[e] + (double)f[ep]) * 0.25);
+  });
+  __syncthreads();
+  sf_box<R, R>([&](int e, int, int ly) {
+    const int gj = J0 + ly;
+    const int em = gj > 0 ? e - SF_R : e, ep = gj < ny - 1 ? e + SF_R : e;
+    f[e] = (float)(((double)t[em] + 2 * (double)t[e] + (double)t[ep]) * 0.25);
+  });
+  __syncthreads();
+}
 
 __device__ __forceinline__ void k_surface(Blk B_, const uint8_t* __restrict__ tipo, const float* __restrict__ dens,
                                                         int nx, int ny, double h, float* __restrict__ fase,
@@ -28,19 +94,12 @@ __device__ __forceinline__ void k_surface(Blk B_, const uint8_t* __restrict__ ti
   xcd_block2(B_, &bxl, &byl);
   const int I0 = bxl * SF_T - SF_H, J0 = byl * SF_T - SF_H;
   int solid_here = 0, fluid_here = 0;  // solid / filled cells in this thread's part of the region
-  // this thread's region cells, fixed for every pass
-  int lxk[SF_K], lyk[SF_K];
-#pragma unroll
-  for (int k = 0; k < SF_K; k++) {
-    const int e = tid + SF_THREADS * k;
-    lyk[k] = e / SF_R;
-    lxk[k] = e - lyk[k] * SF_R;
-  }
 #pragma unroll
   for (int k = 0; k < SF_K; k++) {
     const int e = tid + SF_THREADS * k;
     if (e >= SF_N) continue;
-    const int gi = I0 + lxk[k], gj = J0 + lyk[k];
+    const int ly = e / SF_R, lx = e - ly * SF_R;
+    const int gi = I0 + lx, gj = J0 + ly;
     if (gi >= 0 && gj >= 0 && gi < nx && gj < ny) {
       const size_t c = (size_t)gi + (size_t)gj * nx;
       const uint8_t tt = tipo[c];
@@ -57,67 +116,18 @@ __device__ __forceinline__ void k_surface(Blk B_, const uint8_t* __restrict__ ti
   // neighbours: a region without solid cells, or without a filled one, is left as it is by all its
   // layers, which are skipped (the tank's interior, the inside of thick walls).
   const bool walls = (__syncthreads_or(solid_here) != 0) && (__syncthreads_or(fluid_here) != 0);
-  for (int passada = 0; passada <= 2; passada++) {
-    if (walls) {
-      // #estenderNaParede: 3 layers of "unfilled cell := mean of filled 4-neighbours"
-#pragma unroll
-      for (int k = 0; k < SF_K; k++) {
-        const int e = tid + SF_THREADS * k;
-        if (e < SF_N) fl[e] = (ty[e] != SOLIDO && ty[e] != 255) ? 1 : 0;
-      }
-      __syncthreads();
-      for (int camada = 0; camada < 3; camada++) {
-#pragma unroll
-        for (int k = 0; k < SF_K; k++) {
-          const int e = tid + SF_THREADS * k;
-          if (e >= SF_N || fl[e] || ty[e] == 255) continue;
-          const int lx = lxk[k], ly = lyk[k], gi = I0 + lx, gj = J0 + ly;
-          double sum = 0;
-          int n = 0;
-          if (gi > 0 && lx > 0 && fl[e - 1] == 1) { sum += (double)f[e - 1]; n++; }
-          if (gi < nx - 1 && lx < SF_R - 1 && fl[e + 1] == 1) { sum += (double)f[e + 1]; n++; }
-          if (gj > 0 && ly > 0 && fl[e - SF_R] == 1) { sum += (double)f[e - SF_R]; n++; }
-          if (gj < ny - 1 && ly < SF_R - 1 && fl[e + SF_R] == 1) { sum += (double)f[e + SF_R]; n++; }
-          if (n) { t[e] = (float)(sum / n); fl[e] = 2; }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < SF_K; k++) {
-          const int e = tid + SF_THREADS * k;
-          if (e < SF_N && fl[e] == 2) { f[e] = t[e]; fl[e] = 1; }
-        }
-        __syncthreads();
-      }
-    }
-    if (passada == 2) break;
-    // separable (1 2 1) blur
-#pragma unroll
-    for (int k = 0; k < SF_K; k++) {
-      const int e = tid + SF_THREADS * k;
-      if (e >= SF_N) continue;
-      const int lx = lxk[k], gi = I0 + lx;
-      const int em = (gi > 0 && lx > 0) ? e - 1 : e, ep = (gi < nx - 1 && lx < SF_R - 1) ? e + 1 : e;
-      t[e] = (float)(((double)f[em] + 2 * (double)f[e] + (double)f[ep]) * 0.25);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < SF_K; k++) {
-      const int e = tid + SF_THREADS * k;
-      if (e >= SF_N) continue;
-      const int ly = lyk[k], gj = J0 + ly;
-      const int em = (gj > 0 && ly > 0) ? e - SF_R : e, ep = (gj < ny - 1 && ly < SF_R - 1) ? e + SF_R : e;
-      f[e] = (float)(((double)t[em] + 2 * (double)t[e] + (double)t[ep]) * 0.25);
-    }
-    __syncthreads();
-  }
-  // normals (interior cells of the domain only; the reference never writes the border)
-#pragma unroll
-  for (int k = 0; k < SF_K; k++) {
-    const int e = tid + SF_THREADS * k;
-    if (e >= SF_N) continue;
-    const int lx = lxk[k], ly = lyk[k], gi = I0 + lx, gj = J0 + ly;
+  // ext3 (radius 10), blur (9), ext3 (6), blur (5), ext3 (2): what the normals (1) read
+  if (walls) sf_extend<10>(f, t, fl, ty, I0, J0, nx, ny);
+  sf_blur<9>(f, t, I0, J0, nx, ny);
+  if (walls) sf_extend<6>(f, t, fl, ty, I0, J0, nx, ny);
+  sf_blur<5>(f, t, I0, J0, nx, ny);
+  if (walls) sf_extend<2>(f, t, fl, ty, I0, J0, nx, ny);
+  // normals (interior cells of the domain only; the reference never writes the border), on the
+  // owned tile and its one-cell ring (what the curvature reads)
+  sf_box<1, 1>([&](int e, int lx, int ly) {
+    const int gi = I0 + lx, gj = J0 + ly;
     float a = 0.f, b = 0.f;
-    if (gi >= 1 && gj >= 1 && gi < nx - 1 && gj < ny - 1 && lx > 0 && ly > 0 && lx < SF_R - 1 && ly < SF_R - 1) {
+    if (gi >= 1 && gj >= 1 && gi < nx - 1 && gj < ny - 1) {
       const double gx = ((double)f[e + 1] - (double)f[e - 1]) * 0.5, gy = ((double)f[e + SF_R] - (double)f[e - SF_R]) * 0.5;
       const double g = js_hypot(gx, gy);
       a = g > 1e-3 ? (float)(gx / g) : 0.f;
@@ -125,7 +135,7 @@ __device__ __forceinline__ void k_surface(Blk B_, const uint8_t* __restrict__ ti
     }
     ga[e] = a;
     gb[e] = b;
-  }
+  });
   __syncthreads();
   // the owned 32x32 cells: one per thread
   {
