@@ -508,8 +508,50 @@ def _grid_total(r):
     return int(r.get("Grid_Size_X", 1)) * int(r.get("Grid_Size_Y", 1)) * int(r.get("Grid_Size_Z", 1))
 
 
+def body_name(mangled):
+    """The body of a launch wrapper (tq::k_one / k_one_w launch one tank, tq::k_tab / k_tab_w a
+    tank table) from its mangled symbol, which the profiler leaves mangled (the wrappers take the
+    body as an auto template argument): "tq::k_bra<float, float, float, float, true>", with
+    " [table]" for a table launch; None for other symbols.  The bodies' template arguments are
+    float, __half (its later uses are substitutions S<n>_), bool and int literals."""
+    import re
+    m = re.match(r"_ZN2tq\d+k_(one|tab)(?:_w)?I.*?XadL_ZNS_(\d+)", mangled)
+    if not m:
+        return None
+    pos, ln = m.end(), int(m.group(2))
+    body, rest = mangled[pos:pos + ln], mangled[pos + ln:]
+    args = []
+    if rest.startswith("I"):
+        i = 1
+        while i < len(rest) and rest[i] != "E":
+            if rest[i] == "f":
+                args.append("float")
+                i += 1
+            elif rest.startswith("6__half", i):
+                args.append("__half")
+                i += 7
+            elif rest[i] == "S":
+                args.append("__half")
+                i = rest.index("_", i) + 1
+            elif rest.startswith("Lb", i):
+                args.append("true" if rest[i + 2] == "1" else "false")
+                i = rest.index("E", i) + 1
+            elif rest.startswith("Li", i):
+                j = rest.index("E", i)
+                args.append(rest[i + 2:j])
+                i = j + 1
+            else:
+                args.append("?")
+                break
+    name = "tq::%s<%s>" % (body, ", ".join(args)) if args else "tq::" + body
+    return name + (" [table]" if m.group(1) == "tab" else "")
+
+
 def _kname(r):
     n = r["Kernel_Name"].split("(")[0].strip()
+    b = body_name(n)
+    if b:
+        return b
     return n[5:] if n.startswith("void ") else n
 
 
@@ -598,6 +640,55 @@ def named_kernels(table, nx, ny):
     return kernels
 
 
+def run_batched_table_only(tq, frames):
+    """The table launch model of the batched configuration alone, for a profiler run: the
+    configured 64 mazes stepped `frames` frames from fresh states after one untimed frame."""
+    from tanque_hip import scenes
+    from tanque_hip.scenes import passo_frame
+    ts = scenes.batched_mazes()
+    cenas = [t.cena() for t in ts]
+    b = tq.Batch(cenas, [{"iteracoes": 3}] * len(ts), threads=8, mode="table")
+    for t, s in zip(ts, b.tanks):
+        s.adicionar(t.particles, False)
+    dt = ts[0].dt_frame
+    subs = 0
+    t0 = 0.0
+    for f in range(frames + 1):
+        if f == 1:
+            t0 = time.perf_counter()
+        k = [passo_frame(t.h, c.altura, v, dt) for t, c, v in zip(ts, cenas, b.vmax())]
+        b.step(dt, k)
+        subs += sum(k) if f else 0
+    el = time.perf_counter() - t0
+    b.close()
+    return {"frames": frames, "tank_steps_per_s": round(len(ts) * frames / el, 1), "substeps": subs}
+
+
+def summarize_batch_trace(trace_dir, frames, commands):
+    """Markdown table of the kernels of a table-batch run's rocprofv3 kernel trace (the run of
+    run_batched_table_only with frames + 1 frames): per body, launches and device time per frame."""
+    import statistics
+    rows = _csv_rows(_find(trace_dir, "kernel_trace"))
+    per = {}
+    for r in rows:
+        k = _kname(r)
+        per.setdefault(k, []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    fr = frames + 1
+    tab = {k: v for k, v in per.items() if k.endswith("[table]")}
+    total = sum(sum(v) for v in tab.values()) / fr
+    lines = ["# Kernel time of the table-batched configuration (64 maze tanks, 2.39 M cells)", "",
+             "Commands (from the repository root, on one MI355X):", "", "```"] + commands + ["```", "",
+             "%d frames (the untimed first frame included); table launches only (the set-up's per-tank "
+             "launches are left out).  Device time of the table kernels: %.1f us per frame." % (fr, total), "",
+             "| body | launches per frame | avg us | median us | us per frame | share |",
+             "|---|---:|---:|---:|---:|---:|"]
+    for k, v in sorted(tab.items(), key=lambda kv: -sum(kv[1])):
+        s = sum(v) / fr
+        lines.append("| `%s` | %.1f | %.2f | %.2f | %.1f | %.1f %% |" % (
+            k.replace(" [table]", ""), len(v) / fr, sum(v) / len(v), statistics.median(v), s, 100 * s / total))
+    return "\n".join(lines) + "\n"
+
+
 def profile_markdown(summary, title, top=30):
     """The kernel table of a summary as markdown (profiles/*.md)."""
     lines = ["# " + title, "", "Commands (from the repository root, on one MI355X):", "", "```"]
@@ -629,6 +720,10 @@ def main():
     ap.add_argument("--no-alto", action="store_true", help="skip the Alto (cup + faucet) runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (64 tanks) runs")
     ap.add_argument("--batched-frames", type=int, default=0, help="frames of the batched runs (0: all 500)")
+    ap.add_argument("--batched-table-only", action="store_true",
+                    help="only the table-batched configuration for --batched-frames frames (default 20; for profiling)")
+    ap.add_argument("--summarize-batch-trace", nargs=2, metavar=("TRACE_DIR", "OUT_MD"),
+                    help="write the kernel table of a --batched-table-only rocprofv3 --kernel-trace run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (funnel) runs")
     ap.add_argument("--no-autotune", action="store_true", help="run with the built-in launch shapes")
     ap.add_argument("--summarize-profiles", nargs=5, metavar=("FETCH_DIR", "WRITE_DIR", "TRACE_DIR", "OUT_JSON", "OUT_MD"),
@@ -676,6 +771,15 @@ def main():
         with open(om, "w") as f:
             f.write(profile_markdown(sm, "Kernel time and HBM traffic, %s" % name))
         print(json.dumps({"summary": oj, "kernels": sm["kernels"]}), flush=True)
+        return
+    if args.summarize_batch_trace:
+        td, om = args.summarize_batch_trace
+        with open(om, "w") as f:
+            f.write(summarize_batch_trace(td, args.batched_frames or 20, args.profile_commands))
+        print(json.dumps({"summary": om}), flush=True)
+        return
+    if args.batched_table_only:
+        print(json.dumps({"batched_table": run_batched_table_only(tq, args.batched_frames or 20)}), flush=True)
         return
     if args.adjoint_only or args.adjoint_check:
         print(json.dumps({"adjoint": run_adjoint(tq, fd=args.adjoint_check)}), flush=True)

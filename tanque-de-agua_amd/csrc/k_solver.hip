@@ -82,42 +82,56 @@ constexpr int BAGG = 64, BAGG_PROBE = 8;
 struct BubLds {
   uint32_t key[BAGG];  // label (bubble labels are >= 2), 0 = free
   unsigned long long lo[BAGG], hi[BAGG];
-  double* acc;         // the global accumulator these labels belong to
 };
-// every thread of the block; a block barrier must follow before the first bub_add
-__device__ inline void bagg_clear(BubLds& a, double* acc) {
-  for (int e = threadIdx.x; e < BAGG; e += blockDim.x) {
-    a.key[e] = 0u;
-    a.lo[e] = 0ull;
-    a.hi[e] = 0ull;
-  }
-  if (threadIdx.x == 0) a.acc = acc;
+// a kernel's handle: its block's table, or none (every addend goes to `acc` directly: a solve
+// with at most BSLOTS bubbles has a slot for each, so nothing reaches the table)
+struct BubAgg {
+  BubLds* t;
+  double* acc;  // the global accumulator of the labels without a slot
+};
+// every thread of the block, with `on` uniform; when on, a block barrier must follow before the
+// first addend
+__device__ inline BubAgg bagg_begin(BubLds& tab, double* acc, bool on) {
+  if (on)
+    for (int e = threadIdx.x; e < BAGG; e += blockDim.x) {
+      tab.key[e] = 0u;
+      tab.lo[e] = 0ull;
+      tab.hi[e] = 0ull;
+    }
+  return BubAgg{on ? &tab : nullptr, acc};
 }
-__device__ inline void bagg_add(BubLds& a, uint32_t lab, double v) {
+// not inlined: it is the cold branch of the cells' loops (bubbles beyond the slots), and inlining
+// its probe loop at every call site of an unrolled stencil slowed spmv by ~2.7 us per launch
+// even where it never runs
+__device__ __attribute__((noinline)) void bagg_add(const BubAgg& a, uint32_t lab, double v) {
   unsigned long long hi, lo;
   fx_split(v, &hi, &lo);
-  uint32_t e = (lab * 2654435761u) >> 26;  // 6 bits: BAGG entries
-  for (int p = 0; p < BAGG_PROBE && lab != 0u; p++, e = (e + 1) & (BAGG - 1)) {
-    const uint32_t k = atomicCAS(&a.key[e], 0u, lab);
-    if (k == 0u || k == lab) {
-      fx_add_words(&a.lo[e], &a.hi[e], hi, lo);
-      return;
+  if (a.t) {
+    uint32_t e = (lab * 2654435761u) >> 26;  // 6 bits: BAGG entries
+    for (int p = 0; p < BAGG_PROBE && lab != 0u; p++, e = (e + 1) & (BAGG - 1)) {
+      const uint32_t k = atomicCAS(&a.t->key[e], 0u, lab);
+      if (k == 0u || k == lab) {
+        fx_add_words(&a.t->lo[e], &a.t->hi[e], hi, lo);
+        return;
+      }
     }
   }
   fx_add_words(reinterpret_cast<unsigned long long*>(a.acc) + lab,
                reinterpret_cast<unsigned long long*>(a.acc + (size_t)BV_FX_HI * MAX_BOLHAS) + lab, hi, lo);
 }
-// every thread of the block, after its last bub_add
-__device__ inline void bagg_flush(BubLds& a) {
+// every thread of the block, after its last addend
+__device__ inline void bagg_flush(const BubAgg& a) {
+  if (!a.t) return;
   __syncthreads();
   for (int e = threadIdx.x; e < BAGG; e += blockDim.x) {
-    const uint32_t lab = a.key[e];
+    const uint32_t lab = a.t->key[e];
     if (lab)
       fx_add_words(reinterpret_cast<unsigned long long*>(a.acc) + lab,
-                   reinterpret_cast<unsigned long long*>(a.acc + (size_t)BV_FX_HI * MAX_BOLHAS) + lab, a.hi[e],
-                   a.lo[e]);
+                   reinterpret_cast<unsigned long long*>(a.acc + (size_t)BV_FX_HI * MAX_BOLHAS) + lab, a.t->hi[e],
+                   a.t->lo[e]);
   }
 }
+
 __device__ inline double fx_read(const double* acc, uint32_t lab) {
   const unsigned long long lo = reinterpret_cast<const unsigned long long*>(acc)[lab];
   const long long hi = reinterpret_cast<const long long*>(acc + (size_t)BV_FX_HI * MAX_BOLHAS)[lab];
@@ -172,7 +186,7 @@ __device__ inline void slot_add(double (&acc)[BSLOTS], int slot, double val) {
 
 // a bubble contribution: through the block's slot accumulators, or the block's LDS table of the
 // fixed-point sums of labels without a slot
-__device__ inline void bub_add(double (&sl)[BSLOTS], const int* __restrict__ labelSlot, BubLds& agg, uint32_t lab,
+__device__ inline void bub_add(double (&sl)[BSLOTS], const int* __restrict__ labelSlot, const BubAgg& agg, uint32_t lab,
                                double val) {
   const int s = labelSlot[lab];
   if (s >= 0) slot_add(sl, s, val);
@@ -623,13 +637,13 @@ __device__ __forceinline__ void k_pcg_init(Blk B_, const uint8_t* __restrict__ o
                                                   double* __restrict__ partials, double* __restrict__ bpart,
                                                   DevScalars* __restrict__ sc) {
   __shared__ double sh[BSLOTS * 16];
-  __shared__ BubLds agg;
+  __shared__ BubLds aggt;
   const int ns = nslots(sc->active_bubbles);
   double v[5] = {0, 0, 0, 0, 0};  // r.r, div.div, sum(b), n_water, anchored
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
   const double* yb = bv(bvec, BV_Y);
-  bagg_clear(agg, bv(bvec, BV_ACCI));
-  __syncthreads();
+  const BubAgg agg = bagg_begin(aggt, bv(bvec, BV_ACCI), sc->active_bubbles > BSLOTS);
+  if (agg.t) __syncthreads();
   // the cell's value as the operator sees it: binary32 q (bubble cells: their y), plus qlo on a restart
   auto qe = [&](int d, uint8_t md) -> float {
     if (mask_type(md) == AGUA) return q[d];
@@ -1057,7 +1071,7 @@ struct ZArgs {  // level-0 outputs: z = MG result and the dots r.z, r.z_old
 
 // z and its dot/bubble contributions for one cell (x = MG output of level 0)
 __device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, float zoldc, float xv, float inv,
-                              double (&dv)[2], double (&sl)[BSLOTS], BubLds& agg) {
+                              double (&dv)[2], double (&sl)[BSLOTS], const BubAgg& agg) {
   const float x = xv * inv;
   float z = 0.f;
   if (mask_type(om) == AGUA) {
@@ -1076,12 +1090,12 @@ __device__ __forceinline__ void k_zdots0(Blk B_, const float* __restrict__ x0, Z
                                                 const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[BSLOTS * 16];
-  __shared__ BubLds agg;
+  __shared__ BubLds aggt;
   const int ns = nslots(sc->active_bubbles);
   double dv[2] = {0, 0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-  bagg_clear(agg, za.accZ);
-  __syncthreads();
+  const BubAgg agg = bagg_begin(aggt, za.accZ, sc->active_bubbles > BSLOTS);
+  if (agg.t) __syncthreads();
   const int c = B_.x * blockDim.x + threadIdx.x;
   const float inv = za.scaled ? 1.f / *za.mgs : 1.f;
   if (c < C) z_cell(za, c, za.opMask[c], za.r[c], za.zold[c], x0[c], inv, dv, sl, agg);
@@ -1182,11 +1196,11 @@ __device__ __forceinline__ void k_pass_z(Blk B_, const float* __restrict__ src, 
   if (sc->done) return;
   __shared__ float Tt[4][TCELLS];
   __shared__ double sh[BSLOTS * 16];
-  __shared__ BubLds agg;
+  __shared__ BubLds aggt;
   const int lane = threadIdx.x & 63;
   const int ns = nslots(sc->active_bubbles);
-  bagg_clear(agg, za.accZ);
-  __syncthreads();
+  const BubAgg agg = bagg_begin(aggt, za.accZ, sc->active_bubbles > BSLOTS);
+  if (agg.t) __syncthreads();
   int tx, ty;
   block_tile(B_, &tx, &ty);
   double dv[2] = {0, 0};
@@ -1567,8 +1581,9 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
   __shared__ float CPt[4][100];
   __shared__ float W[CW * CW];
   __shared__ double sh[BSLOTS * 16];
-  __shared__ BubLds agg;
-  if (ZD) bagg_clear(agg, za.accZ);  // the barrier before pass D orders it before the adds
+  __shared__ BubLds aggt;
+  // (the barrier before pass D orders the table's clearing before the adds)
+  const BubAgg agg = bagg_begin(aggt, za.accZ, ZD && sc->active_bubbles > BSLOTS);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   int bxl, byl;
   xcd_block2(B_, &bxl, &byl);
@@ -2103,7 +2118,7 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], zs[BSLOTS];
   __shared__ float P[SPMV_HN];
   __shared__ uint8_t M[SPMV_HN];
-  __shared__ BubLds agg;
+  __shared__ BubLds aggt;
   const int tid = threadIdx.x;
   const int ntx = (nx + SPMV_TW - 1) / SPMV_TW, nch = ntx * ((ny + SPMV_TH - 1) / SPMV_TH);
   int ch = xcd_block(B_.x, B_.gx);
@@ -2135,7 +2150,8 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
   const int ns = nslots(sc->active_bubbles);
   const double* Po = bv(B.bvec, BV_P0 + 1 - par);
   const double* accZ = bv(B.bvec, BV_ACCZ0 + par);
-  bagg_clear(agg, bv(B.bvec, BV_ACCA0 + par));  // the loop's first barrier orders it before the adds
+  // (the loop's first barrier orders the table's clearing before the adds)
+  const BubAgg agg = bagg_begin(aggt, bv(B.bvec, BV_ACCA0 + par), sc->active_bubbles > BSLOTS);
   // p of a cell from its loaded z, p_old and mask (bubble cells from the bubble vectors)
   auto peff = [&](int d, uint8_t md, float zd, float pd) -> float {
     if (mask_type(md) == AGUA) return (float)((double)zd + beta * (double)pd);
@@ -2238,11 +2254,11 @@ __device__ __forceinline__ void k_true_res(Blk B_, const uint8_t* __restrict__ o
                                                   const DevScalars* __restrict__ sc) {
   if (true_res_skip(sc, gated)) return;
   __shared__ double sh[BSLOTS * 16];
-  __shared__ BubLds agg;
+  __shared__ BubLds aggt;
   const int ns = nslots(sc->active_bubbles);
   const double* yb = bv(bvec, BV_Y);
-  bagg_clear(agg, bv(bvec, BV_ACCI));
-  __syncthreads();
+  const BubAgg agg = bagg_begin(aggt, bv(bvec, BV_ACCI), sc->active_bubbles > BSLOTS);
+  if (agg.t) __syncthreads();
   const double mean = sc->proj_mean;
   double v[1] = {0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};

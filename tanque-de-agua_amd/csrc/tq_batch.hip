@@ -154,13 +154,15 @@ struct tq_batch {
   uint64_t graph_clock = 0;
 };
 
-namespace {
-
+namespace tq {
 // gather of one tank's vMax bits (the word tq_vmax reads)
-__device__ __forceinline__ void k_vmax_put(tq::Blk B_, const tq::DevScalars* __restrict__ sc,
+__device__ __forceinline__ void k_vmax_put(Blk B_, const DevScalars* __restrict__ sc,
                                            unsigned long long* __restrict__ dst) {
   if (threadIdx.x == 0) *dst = sc->vmax_bits;
 }
+}  // namespace tq
+
+namespace {
 
 // a worker's share of a step: enqueue its tanks' frames (thread mode) or record them (table mode)
 int worker_job(tq_batch* b, int w, std::string* msg) {
@@ -379,7 +381,7 @@ int table_step(tq_batch* b) {
   b->stats.record_s += t1 - t0;
   for (int k = 0; k < n; k++) {
     b->tails[k].clear();
-    tq::Kernel<64, 0, k_vmax_put>::go(&b->tails[k], nullptr, dim3(1), dim3(64), 0,
+    tq::Kernel<64, 0, tq::k_vmax_put>::go(&b->tails[k], nullptr, dim3(1), dim3(64), 0,
                                       (const tq::DevScalars*)b->tanks[k]->sc, b->vmax_dev + k);
   }
   // upload size: every launch's record
