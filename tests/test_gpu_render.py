@@ -1,10 +1,12 @@
 """The render hand-off (tq_render_water) against a restatement of the reference's 2D renderer's
 water layer (js/render-2d.js:110-162) in numpy: every particle a (1 - r^2)^2 splat of radius
 2.1 cells at 3 pixels per cell into coverage, foam and dye, added in the reference's order
-(particle, then pixel row, then pixel column), then the Beer-Lambert tint and the foam alpha
+(particle, then pixel row, then pixel column) into Float32Array buffers as the reference does
+(each add in binary64, the sum rounded to binary32), then the Beer-Lambert tint and the foam alpha
 stored as a Uint8ClampedArray stores them (round half to even, clamp).  The device gathers per
-pixel in another order and uses the device exp, so a byte may differ by one where a value sits
-on a rounding edge.  Rendering between frames leaves the simulation bit-identical."""
+pixel in another order, sums in binary64 and uses the device exp, so a byte may differ by one
+where a value sits on a rounding edge; the test bounds how many.  Rendering between frames
+leaves the simulation bit-identical."""
 import math
 
 import numpy as np
@@ -48,13 +50,21 @@ def render_reference(P, nx, ny, h):
     sel = ok.ravel()
     idx, pw = idx[sel], peso.ravel()[sel]
     rep = np.repeat(np.arange(len(P)), span * span)[sel]
-    campo = np.zeros(w * hh)
-    espuma = np.zeros(w * hh)
-    cor = np.zeros((3, w * hh))
-    np.add.at(campo, idx, pw)
-    np.add.at(espuma, idx, pw * P[rep, 8].astype(np.float64))
-    for c in range(3):
-        np.add.at(cor[c], idx, pw * P[rep, 9 + c].astype(np.float64))
+    # Float32Array sums in the reference's order: per pixel, its addends in particle order (the
+    # stable sort keeps it), the j-th addend of every pixel added in step j
+    order = np.argsort(idx, kind="stable")
+    idx, pw, rep = idx[order], pw[order], rep[order]
+    first = np.r_[0, np.flatnonzero(np.diff(idx)) + 1]
+    nth = np.arange(len(idx)) - np.repeat(first, np.diff(np.r_[first, len(idx)]))
+    adds = [pw, pw * P[rep, 8].astype(np.float64)] + [pw * P[rep, 9 + c].astype(np.float64) for c in range(3)]
+    acc = np.zeros((5, w * hh), np.float32)
+    for j in range(int(nth.max()) + 1 if len(nth) else 0):
+        m = nth == j
+        for q in range(5):
+            acc[q, idx[m]] = (acc[q, idx[m]].astype(np.float64) + adds[q][m]).astype(np.float32)
+    campo = acc[0].astype(np.float64)
+    espuma = acc[1].astype(np.float64)
+    cor = acc[2:].astype(np.float64)
     phi0 = (PPC * math.pi * RAIO * RAIO) / 3
     phi = campo / phi0
     cobre = suave(0.26, 0.5, phi)
