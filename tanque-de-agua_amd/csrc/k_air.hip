@@ -47,8 +47,8 @@ __device__ inline void uf_unite(int32_t* pai, int a, int b) {
 }
 
 __device__ __forceinline__ void k_uf_init(Blk B_, int C, uint32_t* __restrict__ cells, uint32_t* __restrict__ atm,
-                                                 uint32_t* __restrict__ topo, int32_t* __restrict__ dono,
-                                                 uint32_t* __restrict__ label, float* __restrict__ alvo) {
+                                          uint32_t* __restrict__ topo, int32_t* __restrict__ dono,
+                                          uint32_t* __restrict__ label, float* __restrict__ alvo) {
   const int c = B_.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
   cells[c] = 0; atm[c] = 0; topo[c] = 0; dono[c] = -1; label[c] = 0; alvo[c] = 0.f;
@@ -83,7 +83,7 @@ __device__ inline void lds_unite(int* lp, int a, int b) {
 // Every cell of the tile gets pai = the global index of its tile-component's root (-1: not air).
 // Local indices are row-major in the tile, so the smallest local index is the smallest global one.
 __device__ __forceinline__ void k_uf_local(Blk B_, const uint8_t* __restrict__ tipo, int nx, int ny,
-                                                  int32_t* __restrict__ pai) {
+                                           int32_t* __restrict__ pai) {
   __shared__ int lp[UF_T * UF_T];
   const int ox = B_.x * UF_T, oy = B_.y * UF_T;
   bool air[4];
@@ -123,7 +123,7 @@ __device__ __forceinline__ void k_uf_local(Blk B_, const uint8_t* __restrict__ t
 // ---- 1b. tile-border cells unite with their neighbour across the border (global array):
 // first the left borders (i = 32a, a > 0), then the top borders (j = 32b, b > 0)
 __device__ __forceinline__ void k_uf_border(Blk B_, const uint8_t* __restrict__ tipo, int nx, int ny,
-                                                   int32_t* __restrict__ pai) {
+                                            int32_t* __restrict__ pai) {
   const int t = B_.x * blockDim.x + threadIdx.x;
   const int ntx = (nx + UF_T - 1) / UF_T, nty = (ny + UF_T - 1) / UF_T;
   const int nv = (ntx - 1) * ny;
@@ -144,7 +144,7 @@ __device__ __forceinline__ void k_uf_border(Blk B_, const uint8_t* __restrict__ 
 }
 
 __device__ __forceinline__ void k_uf_flatten(Blk B_, const uint8_t* __restrict__ tipo, int C, int32_t* __restrict__ pai,
-                                                    uint32_t* __restrict__ rootFlag) {
+                                             uint32_t* __restrict__ rootFlag) {
   const int c = B_.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
   uint32_t f = 0;
@@ -190,13 +190,13 @@ __device__ inline void overlap_add(unsigned long long key, uint32_t cnt, uint32_
 constexpr int CS_SLOTS = 32;
 constexpr int CS_GRID = 2048;
 __device__ __forceinline__ void k_comp_stats(Blk B_, const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
-                                                    int nx, int ny, const int32_t* __restrict__ pai,
-                                                    const uint32_t* __restrict__ compIdx, int32_t* __restrict__ comp,
-                                                    uint32_t* __restrict__ cells, uint32_t* __restrict__ atm,
-                                                    uint32_t* __restrict__ topo, unsigned long long* __restrict__ hkeys,
-                                                    uint32_t* __restrict__ hcount, uint32_t* __restrict__ hfirst,
-                                                    uint32_t* __restrict__ hlist, int hmask,
-                                                    DevScalars* __restrict__ sc) {
+                                             int nx, int ny, const int32_t* __restrict__ pai,
+                                             const uint32_t* __restrict__ compIdx, int32_t* __restrict__ comp,
+                                             uint32_t* __restrict__ cells, uint32_t* __restrict__ atm,
+                                             uint32_t* __restrict__ topo, unsigned long long* __restrict__ hkeys,
+                                             uint32_t* __restrict__ hcount, uint32_t* __restrict__ hfirst,
+                                             uint32_t* __restrict__ hlist, int hmask,
+                                             DevScalars* __restrict__ sc) {
   __shared__ int ck[CS_SLOTS];
   __shared__ uint32_t cc[CS_SLOTS], ca[CS_SLOTS], ct[CS_SLOTS];
   __shared__ unsigned long long ok[CS_SLOTS];
@@ -366,14 +366,14 @@ __device__ inline int block_scan_flags(int n, F f, uint16_t* idx, int* ws) {
 }
 
 __device__ __forceinline__ void k_bookkeep(Blk B_, const uint32_t* __restrict__ compIdxTotal, const uint32_t* __restrict__ cells,
-                                                   const uint32_t* __restrict__ atm, const uint32_t* __restrict__ topo,
-                                                   float* __restrict__ alvo, int32_t* __restrict__ dono,
-                                                   uint32_t* __restrict__ label, unsigned long long* __restrict__ hkeys,
-                                                   uint32_t* __restrict__ hcount, uint32_t* __restrict__ hfirst,
-                                                   const uint32_t* __restrict__ hlist, unsigned long long* __restrict__ skeys,
-                                                   uint32_t* __restrict__ svals, float* __restrict__ volAlvo,
-                                                   uint8_t* __restrict__ vivo, float* __restrict__ pBolha,
-                                                   DevScalars* __restrict__ sc) {
+                                           const uint32_t* __restrict__ atm, const uint32_t* __restrict__ topo,
+                                           float* __restrict__ alvo, int32_t* __restrict__ dono,
+                                           uint32_t* __restrict__ label, unsigned long long* __restrict__ hkeys,
+                                           uint32_t* __restrict__ hcount, uint32_t* __restrict__ hfirst,
+                                           const uint32_t* __restrict__ hlist, unsigned long long* __restrict__ skeys,
+                                           uint32_t* __restrict__ svals, float* __restrict__ volAlvo,
+                                           uint8_t* __restrict__ vivo, float* __restrict__ pBolha,
+                                           DevScalars* __restrict__ sc) {
   __shared__ uint32_t total[MAX_BOLHAS];
   __shared__ uint8_t vivoNovo[MAX_BOLHAS], vivoV[MAX_BOLHAS];
   __shared__ float pB[MAX_BOLHAS];
@@ -560,7 +560,7 @@ __device__ __forceinline__ void k_bookkeep(Blk B_, const uint32_t* __restrict__ 
 }
 
 __device__ __forceinline__ void k_label_write(Blk B_, const int32_t* __restrict__ comp, const uint32_t* __restrict__ label,
-                                                     int C, uint32_t* __restrict__ rot) {
+                                              int C, uint32_t* __restrict__ rot) {
   const int c = B_.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
   const int k = comp[c];
@@ -572,7 +572,7 @@ int tracker_update(tq_sim* s) {
   const dim3 g((C + 255) / 256), b(256);
   int t = time_begin(s, "air_tracker", 16.0 * C);
   launch<256, 0, k_uf_init>(s, s->st, g, b, 0, C, s->compCells, s->compAtm, s->compTopo, s->compDono, s->compLabel,
-                     s->compAlvo);
+                            s->compAlvo);
   const int ntx = (s->nx + UF_T - 1) / UF_T, nty = (s->ny + UF_T - 1) / UF_T;
   launch<256, 0, k_uf_local>(s, s->st, dim3(ntx, nty), b, 0, s->tipo, s->nx, s->ny, s->pai);
   const int nborder = (ntx - 1) * s->ny + (nty - 1) * s->nx;
@@ -582,12 +582,12 @@ int tracker_update(tq_sim* s) {
   TQ_LAUNCH_CHECK();
   if (int e = scan_u32(s, s->rootFlag, s->compIdx, C)) return e;
   launch<256, 0, k_comp_stats>(s, s->st, dim3(std::min((C + 255) / 256, CS_GRID)), b, 0, s->tipo, s->rot, s->nx, s->ny,
-                     s->pai, s->compIdx, s->comp,
-                     s->compCells, s->compAtm, s->compTopo, s->hkeys, s->hcount, s->hfirst, s->hlist, s->hsize - 1, s->sc);
+      s->pai, s->compIdx, s->comp,
+      s->compCells, s->compAtm, s->compTopo, s->hkeys, s->hcount, s->hfirst, s->hlist, s->hsize - 1, s->sc);
   launch<1024, 0, k_bookkeep>(s, s->st, dim3(1), dim3(1024), 0, s->compIdx + C, s->compCells, s->compAtm, s->compTopo,
-                     s->compAlvo, s->compDono, s->compLabel, s->hkeys, s->hcount, s->hfirst, s->hlist,
-                     (unsigned long long*)s->ordBuf, s->ordBuf + 2 * (size_t)s->hsize, s->volAlvo, s->vivo, s->pBolha,
-                     s->sc);
+      s->compAlvo, s->compDono, s->compLabel, s->hkeys, s->hcount, s->hfirst, s->hlist,
+      (unsigned long long*)s->ordBuf, s->ordBuf + 2 * (size_t)s->hsize, s->volAlvo, s->vivo, s->pBolha,
+      s->sc);
   launch<256, 0, k_label_write>(s, s->st, g, b, 0, s->comp, s->compLabel, C, s->rot);
   time_end(s, t);
   TQ_LAUNCH_CHECK();

@@ -83,9 +83,9 @@ __device__ __forceinline__ void sf_blur(float* f, float* t, int I0, int J0, int 
 }
 
 __device__ __forceinline__ void k_surface(Blk B_, const uint8_t* __restrict__ tipo, const float* __restrict__ dens,
-                                                        int nx, int ny, double h, float* __restrict__ fase,
-                                                        float* __restrict__ nXo, float* __restrict__ nYo,
-                                                        float* __restrict__ kap) {
+                                          int nx, int ny, double h, float* __restrict__ fase,
+                                          float* __restrict__ nXo, float* __restrict__ nYo,
+                                          float* __restrict__ kap) {
 #pragma clang fp contract(off)
   __shared__ float f[SF_N], t[SF_N], ga[SF_N], gb[SF_N];
   __shared__ uint8_t ty[SF_N], fl[SF_N];
@@ -160,7 +160,7 @@ int surface(tq_sim* s) {
   dim3 g((s->nx + SF_T - 1) / SF_T, (s->ny + SF_T - 1) / SF_T);
   int t = time_begin(s, "surface", (1 + 4 + 4 * 4) * (double)s->C);
   launch<SF_THREADS, 0, k_surface>(s, s->st, g, dim3(SF_THREADS), 0, s->tipo, s->dens, s->nx, s->ny, s->h, s->fase, s->nX,
-                     s->nY, s->kappa);
+                                   s->nY, s->kappa);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   return 0;
@@ -188,12 +188,12 @@ __device__ inline float qar_at(const uint8_t* __restrict__ tipo, const float* __
 }
 
 __device__ __forceinline__ void k_air_div(Blk B_, const uint8_t* __restrict__ tipo, const float* __restrict__ kap,
-                                                 const float* __restrict__ u, const float* __restrict__ v,
-                                                 const float* __restrict__ dens, const uint32_t* __restrict__ count,
-                                                 const uint8_t* __restrict__ solid, int nx, int ny, double tensao,
-                                                 double kMax, double escala, double alvo, int which,
-                                                 float* __restrict__ qAr, float* __restrict__ div,
-                                                 float* __restrict__ q) {
+                                          const float* __restrict__ u, const float* __restrict__ v,
+                                          const float* __restrict__ dens, const uint32_t* __restrict__ count,
+                                          const uint8_t* __restrict__ solid, int nx, int ny, double tensao,
+                                          double kMax, double escala, double alvo, int which,
+                                          float* __restrict__ qAr, float* __restrict__ div,
+                                          float* __restrict__ q) {
 #pragma clang fp contract(off)
   const int c = B_.x * blockDim.x + threadIdx.x;
   if (c >= nx * ny) return;
@@ -228,7 +228,7 @@ int air_pressure_and_div(tq_sim* s, double dt, int which) {
   const double alvo = s->h / std::max(s->d.tau_densidade, dt);
   int t = time_begin(s, "air_div", (1 + 4 + 4 + 4 + 4 + 1 + 4 + 4 + 4) * (double)s->C);
   launch<256, 0, k_air_div>(s, s->st, dim3((s->C + 255) / 256), dim3(256), 0, s->tipo, s->kappa, s->u, s->v, s->dens,
-                     s->countCell, s->solid, s->nx, s->ny, s->d.tensao, kMax, escala, alvo, which, s->qAr, s->div, s->q);
+      s->countCell, s->solid, s->nx, s->ny, s->d.tensao, kMax, escala, alvo, which, s->qAr, s->div, s->q);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   return 0;
@@ -242,8 +242,8 @@ int air_pressure_and_div(tq_sim* s, double dt, int which) {
 // pass (0 for candidates).  The dependency graph is acyclic (raster order), so the fixpoint is
 // unique and equals the sequential result.
 __device__ __forceinline__ void k_label_prep(Blk B_, const uint8_t* __restrict__ tipo, uint32_t* __restrict__ rot,
-                                                    uint8_t* __restrict__ cflag, uint32_t* __restrict__ cand,
-                                                    uint32_t* __restrict__ candCount, int nx, int ny) {
+                                             uint8_t* __restrict__ cflag, uint32_t* __restrict__ cand,
+                                             uint32_t* __restrict__ candCount, int nx, int ny) {
   const int c = B_.x * blockDim.x + threadIdx.x;
   bool is_c = false;
   if (c < nx * ny) {
@@ -266,8 +266,8 @@ __device__ __forceinline__ void k_label_prep(Blk B_, const uint8_t* __restrict__
 }
 
 __device__ __forceinline__ void k_label_prop(Blk B_, uint32_t* __restrict__ rot, uint8_t* __restrict__ cflag,
-                                                     const uint32_t* __restrict__ cand,
-                                                     const uint32_t* __restrict__ candCount, int nx) {
+                                             const uint32_t* __restrict__ cand,
+                                             const uint32_t* __restrict__ candCount, int nx) {
   __shared__ int changed;
   const int tid = threadIdx.x;
   const uint32_t m = *candCount;
@@ -304,7 +304,7 @@ int propagate_labels(tq_sim* s) {
   if (int e = zero_async(s, s->st, &s->sc->cand_count, sizeof(uint32_t))) return e;
   int t = time_begin(s, "label_prep", 6.0 * s->C);
   launch<256, 0, k_label_prep>(s, s->st, dim3((s->C + 255) / 256), dim3(256), 0, s->tipo, s->rot, s->cflag, s->cand,
-                     &s->sc->cand_count, s->nx, s->ny);
+                               &s->sc->cand_count, s->nx, s->ny);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   t = time_begin(s, "label_prop", 0);
@@ -335,11 +335,11 @@ __device__ inline long long wave_sum_i64(long long v) {
 constexpr int BRHS_SLOTS = 32;
 constexpr int BRHS_GRID = 2048;
 __device__ __forceinline__ void k_bubble_rhs(Blk B_, const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
-                                                    const float* __restrict__ u, const float* __restrict__ v,
-                                                    const float* __restrict__ qAr, int nx, int ny,
-                                                    uint32_t* __restrict__ volB, uint32_t* __restrict__ facesB,
-                                                    uint32_t* __restrict__ cellsB, double* __restrict__ rhsB,
-                                                    int counts_only) {
+                                             const float* __restrict__ u, const float* __restrict__ v,
+                                             const float* __restrict__ qAr, int nx, int ny,
+                                             uint32_t* __restrict__ volB, uint32_t* __restrict__ facesB,
+                                             uint32_t* __restrict__ cellsB, double* __restrict__ rhsB,
+                                             int counts_only) {
 #pragma clang fp contract(off)
   __shared__ uint32_t key[BRHS_SLOTS], kc[BRHS_SLOTS], kv[BRHS_SLOTS], kf[BRHS_SLOTS];
   __shared__ unsigned long long ks[BRHS_SLOTS];
@@ -421,9 +421,9 @@ __device__ __forceinline__ void k_bubble_rhs(Blk B_, const uint8_t* __restrict__
 }
 
 __device__ __forceinline__ void k_bubble_finish(Blk B_, const uint32_t* __restrict__ volB, double* __restrict__ rhsB,
-                                                        float* __restrict__ rhsB32, float* __restrict__ yB32,
-                                                        const float* __restrict__ pBolha,
-                                                        const float* __restrict__ volAlvo, double h, double dt) {
+                                                float* __restrict__ rhsB32, float* __restrict__ yB32,
+                                                const float* __restrict__ pBolha,
+                                                const float* __restrict__ volAlvo, double h, double dt) {
 #pragma clang fp contract(off)
   const double paraQ = dt / (DENSIDADE_AGUA * h);
   for (int r = threadIdx.x; r < MAX_BOLHAS; r += blockDim.x) {
@@ -440,7 +440,7 @@ __device__ __forceinline__ void k_bubble_finish(Blk B_, const uint32_t* __restri
 
 // the per-bubble sums of k_bubble_rhs start at zero (one launch instead of four fills)
 __device__ __forceinline__ void k_bubble_clear(Blk B_, uint32_t* __restrict__ volB, uint32_t* __restrict__ facesB,
-                                                      uint32_t* __restrict__ cellsB, double* __restrict__ rhsB) {
+                                               uint32_t* __restrict__ cellsB, double* __restrict__ rhsB) {
   const int r = B_.x * blockDim.x + threadIdx.x;
   if (r >= MAX_BOLHAS) return;
   volB[r] = 0;
@@ -451,14 +451,14 @@ __device__ __forceinline__ void k_bubble_clear(Blk B_, uint32_t* __restrict__ vo
 
 int bubble_rhs(tq_sim* s, double dt) {
   launch<256, 0, k_bubble_clear>(s, s->st, dim3((MAX_BOLHAS + 255) / 256), dim3(256), 0, s->volB, s->facesB, s->cellsB,
-                     s->rhsB);
+                                 s->rhsB);
   int t = time_begin(s, "bubble_rhs", (1 + 4 + 4 + 4 + 4) * (double)s->C);
   launch<256, 0, k_bubble_rhs>(s, s->st, dim3(std::min((s->C + 255) / 256, BRHS_GRID)), dim3(256), 0, s->tipo, s->rot,
-                     s->u, s->v, s->qAr, s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB, 0);
+                               s->u, s->v, s->qAr, s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB, 0);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   launch<1024, 0, k_bubble_finish>(s, s->st, dim3(1), dim3(1024), 0, s->volB, s->rhsB, s->rhsB32, s->yB32, s->pBolha,
-                     s->volAlvo, s->h, dt);
+                                   s->volAlvo, s->h, dt);
   TQ_LAUNCH_CHECK();
   return 0;
 }
@@ -467,18 +467,18 @@ int bubble_rhs(tq_sim* s, double dt) {
 // bubble sums were computed elsewhere.
 int bubble_counts(tq_sim* s) {
   launch<256, 0, k_bubble_clear>(s, s->st, dim3((MAX_BOLHAS + 255) / 256), dim3(256), 0, s->volB, s->facesB, s->cellsB,
-                     (double*)nullptr);
+                                 (double*)nullptr);
   launch<256, 0, k_bubble_rhs>(s, s->st, dim3(std::min((s->C + 255) / 256, BRHS_GRID)), dim3(256), 0, s->tipo, s->rot,
-                     s->u, s->v, s->qAr, s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB, 1);
+                               s->u, s->v, s->qAr, s->nx, s->ny, s->volB, s->facesB, s->cellsB, s->rhsB, 1);
   TQ_LAUNCH_CHECK();
   return 0;
 }
 
 // ================================================================== #pressao post (fisica-cpu.js:503-514)
 __device__ __forceinline__ void k_pressure_post(Blk B_, const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
-                                                       const float* __restrict__ qAr, const uint32_t* __restrict__ volB,
-                                                       const float* __restrict__ yB32, int C, int ar,
-                                                       float* __restrict__ q) {
+                                                const float* __restrict__ qAr, const uint32_t* __restrict__ volB,
+                                                const float* __restrict__ yB32, int C, int ar,
+                                                float* __restrict__ q) {
 #pragma clang fp contract(off)
   const int c = B_.x * blockDim.x + threadIdx.x;
   if (c >= C || tipo[c] != AR) return;
@@ -487,7 +487,7 @@ __device__ __forceinline__ void k_pressure_post(Blk B_, const uint8_t* __restric
 }
 
 __device__ __forceinline__ void k_bubble_pressure(Blk B_, const uint32_t* __restrict__ volB, const float* __restrict__ yB32,
-                                                          float* __restrict__ pBolha, double paraPa) {
+                                                  float* __restrict__ pBolha, double paraPa) {
 #pragma clang fp contract(off)
   for (int r = threadIdx.x; r < MAX_BOLHAS; r += blockDim.x)
     if (r >= 2 && volB[r])
@@ -496,11 +496,11 @@ __device__ __forceinline__ void k_bubble_pressure(Blk B_, const uint32_t* __rest
 
 int pressure_post(tq_sim* s, double dt) {
   launch<256, 0, k_pressure_post>(s, s->st, dim3((s->C + 255) / 256), dim3(256), 0, s->tipo, s->rot, s->qAr, s->volB,
-                     s->yB32, s->C, s->d.ar, s->q);
+                                  s->yB32, s->C, s->d.ar, s->q);
   TQ_LAUNCH_CHECK();
   if (s->d.ar) {
     launch<1024, 0, k_bubble_pressure>(s, s->st, dim3(1), dim3(1024), 0, s->volB, s->yB32, s->pBolha,
-                       (DENSIDADE_AGUA * s->h) / dt);
+                                       (DENSIDADE_AGUA * s->h) / dt);
     TQ_LAUNCH_CHECK();
   }
   return 0;
@@ -508,8 +508,8 @@ int pressure_post(tq_sim* s, double dt) {
 
 // ================================================================== #gradiente (fisica-cpu.js:519-545)
 __device__ __forceinline__ void k_gradient(Blk B_, const uint8_t* __restrict__ tipo, const float* __restrict__ q, int nx,
-                                                  int ny, float* __restrict__ u, uint8_t* __restrict__ uVal,
-                                                  float* __restrict__ v, uint8_t* __restrict__ vVal) {
+                                           int ny, float* __restrict__ u, uint8_t* __restrict__ uVal,
+                                           float* __restrict__ v, uint8_t* __restrict__ vVal) {
 #pragma clang fp contract(off)
   const int k = B_.x * blockDim.x + threadIdx.x;
   const int nu = (nx + 1) * ny, nv = nx * (ny + 1);
@@ -539,7 +539,7 @@ int gradient(tq_sim* s) {
   const int n = std::max(s->nu, s->nv);
   int t = time_begin(s, "gradient", 9.0 * (s->nu + s->nv));
   launch<256, 0, k_gradient>(s, s->st, dim3((n + 255) / 256), dim3(256), 0, s->tipo, s->q, s->nx, s->ny, s->u, s->uVal,
-                     s->v, s->vVal);
+                             s->v, s->vVal);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   return 0;
@@ -548,9 +548,9 @@ int gradient(tq_sim* s) {
 // ================================================================== #extrapolar (fisica-cpu.js:547-552, 673-690)
 // One layer for both components: an invalid face with valid 4-neighbours takes their mean.
 __device__ __forceinline__ void k_extrap(Blk B_, const float* __restrict__ ua, const uint8_t* __restrict__ uva,
-                                                float* __restrict__ ub, uint8_t* __restrict__ uvb,
-                                                const float* __restrict__ va, const uint8_t* __restrict__ vva,
-                                                float* __restrict__ vb, uint8_t* __restrict__ vvb, int nx, int ny) {
+                                         float* __restrict__ ub, uint8_t* __restrict__ uvb,
+                                         const float* __restrict__ va, const uint8_t* __restrict__ vva,
+                                         float* __restrict__ vb, uint8_t* __restrict__ vvb, int nx, int ny) {
 #pragma clang fp contract(off)
   const int k = B_.x * blockDim.x + threadIdx.x;
   const int nu = (nx + 1) * ny, nv = nx * (ny + 1);
@@ -582,9 +582,9 @@ int extrapolate(tq_sim* s) {
   const int n = std::max(s->nu, s->nv);
   int t = time_begin(s, "extrapolate", 10.0 * (s->nu + s->nv));
   launch<256, 0, k_extrap>(s, s->st, dim3((n + 255) / 256), dim3(256), 0, s->u, s->uVal, s->uTmp, s->uValTmp, s->v,
-                     s->vVal, s->vTmp, s->vValTmp, s->nx, s->ny);
+                           s->vVal, s->vTmp, s->vValTmp, s->nx, s->ny);
   launch<256, 0, k_extrap>(s, s->st, dim3((n + 255) / 256), dim3(256), 0, s->uTmp, s->uValTmp, s->u, s->uVal, s->vTmp,
-                     s->vValTmp, s->v, s->vVal, s->nx, s->ny);
+                           s->vValTmp, s->v, s->vVal, s->nx, s->ny);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   return 0;

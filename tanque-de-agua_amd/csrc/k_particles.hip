@@ -64,7 +64,7 @@ __device__ __forceinline__ void k_scan_blocks(Blk B_, uint32_t* __restrict__ blo
 }
 
 __device__ __forceinline__ void k_scan_apply(Blk B_, const uint32_t* __restrict__ in, int n, const uint32_t* __restrict__ blocks,
-                                                       uint32_t* __restrict__ out) {
+                                             uint32_t* __restrict__ out) {
   __shared__ uint32_t sh[17];
   const size_t base = (size_t)B_.x * SCAN_B + (size_t)threadIdx.x * SCAN_V;
   uint32_t v[SCAN_V], s = 0;
@@ -118,8 +118,8 @@ int scan_u32(tq_sim* s, const uint32_t* in, uint32_t* out, int n) {
 // lane.  Ranks within a cell depend on the order the atomics land; every consumer of the sorted
 // order sums in fixed point or works per particle, so results do not.
 __device__ __forceinline__ void k_bin(Blk B_, const float2* __restrict__ pos, const DevScalars* __restrict__ sc, int nx, int ny,
-                                             double inv, int ntx, uint32_t* __restrict__ countKey,
-                                             uint32_t* __restrict__ pkey, uint32_t* __restrict__ prank) {
+                                      double inv, int ntx, uint32_t* __restrict__ countKey,
+                                      uint32_t* __restrict__ pkey, uint32_t* __restrict__ prank) {
   const uint32_t k = B_.x * blockDim.x + threadIdx.x;
   const bool slot = k < sc->n_slots;
   uint32_t key = DEAD_KEY;
@@ -149,12 +149,12 @@ __device__ __forceinline__ void k_bin(Blk B_, const float2* __restrict__ pos, co
 }
 
 __device__ __forceinline__ void k_scatter(Blk B_, const float2* __restrict__ pos, const float2* __restrict__ vel,
-                                                 const float4* __restrict__ cm, const float4* __restrict__ ext,
-                                                 const uint32_t* __restrict__ seq, const DevScalars* __restrict__ sc,
-                                                 const uint32_t* __restrict__ pkey, const uint32_t* __restrict__ prank,
-                                                 const uint32_t* __restrict__ startKey, float2* __restrict__ pos2,
-                                                 float2* __restrict__ vel2, float4* __restrict__ cm2,
-                                                 float4* __restrict__ ext2, uint32_t* __restrict__ seq2) {
+                                          const float4* __restrict__ cm, const float4* __restrict__ ext,
+                                          const uint32_t* __restrict__ seq, const DevScalars* __restrict__ sc,
+                                          const uint32_t* __restrict__ pkey, const uint32_t* __restrict__ prank,
+                                          const uint32_t* __restrict__ startKey, float2* __restrict__ pos2,
+                                          float2* __restrict__ vel2, float4* __restrict__ cm2,
+                                          float4* __restrict__ ext2, uint32_t* __restrict__ seq2) {
   const uint32_t k = B_.x * blockDim.x + threadIdx.x;
   if (k >= sc->n_slots) return;
   uint32_t key = pkey[k];
@@ -177,7 +177,7 @@ int bin_and_sort(tq_sim* s) {
   if (n > 0) {
     int t = time_begin(s, "bin", 16.0 * n);
     launch<256, 0, k_bin>(s, s->st, dim3((n + 255) / 256), dim3(256), 0, s->pos[s->cur], s->sc, s->nx, s->ny,
-                       1.0 / s->h, s->ntx, s->countKey, s->pkey, s->prank);
+                          1.0 / s->h, s->ntx, s->countKey, s->pkey, s->prank);
     time_end(s, t);
     TQ_LAUNCH_CHECK();
   }
@@ -186,8 +186,8 @@ int bin_and_sort(tq_sim* s) {
     int o = s->cur, d = 1 - s->cur;
     int t = time_begin(s, "scatter", 2.0 * 60 * n);
     launch<256, 0, k_scatter>(s, s->st, dim3((n + 255) / 256), dim3(256), 0, s->pos[o], s->vel[o], s->cm[o], s->ext[o],
-                       s->seq[o], s->sc, s->pkey, s->prank, s->startKey, s->pos[d], s->vel[d], s->cm[d], s->ext[d],
-                       s->seq[d]);
+        s->seq[o], s->sc, s->pkey, s->prank, s->startKey, s->pos[d], s->vel[d], s->cm[d], s->ext[d],
+        s->seq[d]);
     time_end(s, t);
     TQ_LAUNCH_CHECK();
     s->cur = d;
@@ -206,13 +206,13 @@ int bin_and_sort(tq_sim* s) {
 // the owned nodes: u = sum(w v)/sum(w), + g dt where weighted, zero at solid/domain faces
 // (#paraGrade + #gravidadeEFronteira + #tipos, fisica-cpu.js:198-299).
 __device__ __forceinline__ void k_p2g(Blk B_, const float2* __restrict__ pos, const float2* __restrict__ vel,
-                                             const float4* __restrict__ cm, const float4* __restrict__ ext,
-                                             const uint32_t* __restrict__ startKey, int nx, int ny, int ntx, int K,
-                                             double h, const uint8_t* __restrict__ solid, double gdx, double gdy,
-                                             int gravity, float* __restrict__ u, float* __restrict__ uw,
-                                             float* __restrict__ v, float* __restrict__ vw, float* __restrict__ dens,
-                                             float* __restrict__ dye, uint32_t* __restrict__ countCell,
-                                             uint8_t* __restrict__ tipo, DevScalars* __restrict__ sc) {
+                                      const float4* __restrict__ cm, const float4* __restrict__ ext,
+                                      const uint32_t* __restrict__ startKey, int nx, int ny, int ntx, int K,
+                                      double h, const uint8_t* __restrict__ solid, double gdx, double gdy,
+                                      int gravity, float* __restrict__ u, float* __restrict__ uw,
+                                      float* __restrict__ v, float* __restrict__ vw, float* __restrict__ dens,
+                                      float* __restrict__ dye, uint32_t* __restrict__ countCell,
+                                      uint8_t* __restrict__ tipo, DevScalars* __restrict__ sc) {
 #pragma clang fp contract(off)
   __shared__ unsigned long long aU[17 * 16], wU[17 * 16], aV[16 * 17], wV[16 * 17], aD[256], aC[3 * 256];
   __shared__ uint32_t rStart[68], rOff[69];
@@ -398,8 +398,8 @@ int p2g(tq_sim* s, double dt, int gravity) {
   double bytes = 44.0 * s->n_upper + 8.0 * (s->nu + s->nv) + (4 + 12 + 4 + 1 + 1) * (double)s->C;
   int t = time_begin(s, "p2g", bytes);
   launch<256, 0, k_p2g>(s, s->st, dim3(s->ntx, s->nty), dim3(256), 0, s->pos[o], s->vel[o], s->cm[o], s->ext[o],
-                     s->startKey, s->nx, s->ny, s->ntx, s->K, s->h, s->solid, s->d.gx * dt, s->d.gy * dt, gravity,
-                     s->u, s->uw, s->v, s->vw, s->dens, s->dye, s->countCell, s->tipo, s->sc);
+                        s->startKey, s->nx, s->ny, s->ntx, s->K, s->h, s->solid, s->d.gx * dt, s->d.gy * dt, gravity,
+                        s->u, s->uw, s->v, s->vw, s->dens, s->dye, s->countCell, s->tipo, s->sc);
   time_end(s, t);
   TQ_LAUNCH_CHECK();
   return 0;
@@ -441,13 +441,13 @@ __device__ inline void velocidade_em(const float* __restrict__ u, const float* _
 }
 
 __device__ __forceinline__ void k_g2p(Blk B_, float2* __restrict__ pos, float2* __restrict__ vel, float4* __restrict__ cm,
-                                             float4* __restrict__ ext, const DevScalars* __restrict__ scr,
-                                             const float* __restrict__ u, const float* __restrict__ v,
-                                             const float* __restrict__ dens, const uint32_t* __restrict__ countCell,
-                                             const float* __restrict__ dye, const float* __restrict__ sdf, int nx,
-                                             int ny, double h, double dt, double decai, double mistura,
-                                             unsigned long long* __restrict__ vmax_bits,
-                                             uint16_t* __restrict__ flags) {
+                                      float4* __restrict__ ext, const DevScalars* __restrict__ scr,
+                                      const float* __restrict__ u, const float* __restrict__ v,
+                                      const float* __restrict__ dens, const uint32_t* __restrict__ countCell,
+                                      const float* __restrict__ dye, const float* __restrict__ sdf, int nx,
+                                      int ny, double h, double dt, double decai, double mistura,
+                                      unsigned long long* __restrict__ vmax_bits,
+                                      uint16_t* __restrict__ flags) {
 #pragma clang fp contract(off)
   // particles are in cell order: consecutive blocks on one XCD gather from one band of rows
   const uint32_t k = xcd_block(B_.x, B_.gx) * blockDim.x + threadIdx.x;
@@ -568,8 +568,8 @@ int g2p(tq_sim* s, double dt) {
     double bytes = (8 + 16 + 16 + 16 + 16) * (double)n + 24.0 * s->C;
     int t = time_begin(s, "g2p", bytes);
     launch<256, 0, k_g2p>(s, s->st, dim3((n + 255) / 256), dim3(256), 0, s->pos[o], s->vel[o], s->cm[o], s->ext[o],
-                       s->sc, s->u, s->v, s->dens, s->countCell, s->dye, s->sdf, s->nx, s->ny, s->h, dt, decai,
-                       mistura, &s->sc->vmax_bits, s->g2p_flags);
+                          s->sc, s->u, s->v, s->dens, s->countCell, s->dye, s->sdf, s->nx, s->ny, s->h, dt, decai,
+                          mistura, &s->sc->vmax_bits, s->g2p_flags);
     time_end(s, t);
     TQ_LAUNCH_CHECK();
   }
@@ -579,8 +579,8 @@ int g2p(tq_sim* s, double dt) {
 
 // ================================================================== adicionar (fisica-cpu.js:92-111)
 __device__ __forceinline__ void k_add_flags(Blk B_, const float* __restrict__ list, int count, int check,
-                                                   const float* __restrict__ sdf, int nx, int ny, double h,
-                                                   const uint32_t* __restrict__ countCell, uint32_t* __restrict__ flag) {
+                                            const float* __restrict__ sdf, int nx, int ny, double h,
+                                            const uint32_t* __restrict__ countCell, uint32_t* __restrict__ flag) {
 #pragma clang fp contract(off)
   const int k = B_.x * blockDim.x + threadIdx.x;
   if (k >= count) return;
@@ -597,11 +597,11 @@ __device__ __forceinline__ void k_add_flags(Blk B_, const float* __restrict__ li
 }
 
 __device__ __forceinline__ void k_add_append(Blk B_, const float* __restrict__ list, int count,
-                                                    const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off,
-                                                    DevScalars* __restrict__ sc, uint32_t seq0, int cap,
-                                                    float2* __restrict__ pos, float2* __restrict__ vel,
-                                                    float4* __restrict__ cm, float4* __restrict__ ext,
-                                                    uint32_t* __restrict__ seq) {
+                                             const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off,
+                                             DevScalars* __restrict__ sc, uint32_t seq0, int cap,
+                                             float2* __restrict__ pos, float2* __restrict__ vel,
+                                             float4* __restrict__ cm, float4* __restrict__ ext,
+                                             uint32_t* __restrict__ seq) {
   const int k = B_.x * blockDim.x + threadIdx.x;
   if (k >= count || !flag[k]) return;
   const uint32_t base = sc->n_slots;
@@ -625,12 +625,12 @@ __device__ __forceinline__ void k_add_finish(Blk B_, DevScalars* sc, const uint3
 int add_particles_dev(tq_sim* s, int count, int check) {
   if (count <= 0) return 0;
   launch<256, 0, k_add_flags>(s, s->st, dim3((count + 255) / 256), dim3(256), 0, s->stage, count, check, s->sdf,
-                     s->nx, s->ny, s->h, s->countCell, s->addflag);
+                              s->nx, s->ny, s->h, s->countCell, s->addflag);
   TQ_LAUNCH_CHECK();
   if (int e = scan_u32(s, s->addflag, s->pkey, count)) return e;
   const int o = s->cur;
   launch<256, 0, k_add_append>(s, s->st, dim3((count + 255) / 256), dim3(256), 0, s->stage, count, s->addflag, s->pkey,
-                     s->sc, s->seq_next, s->cap, s->pos[o], s->vel[o], s->cm[o], s->ext[o], s->seq[o]);
+                               s->sc, s->seq_next, s->cap, s->pos[o], s->vel[o], s->cm[o], s->ext[o], s->seq[o]);
   TQ_LAUNCH_CHECK();
   launch<64, 0, k_add_finish>(s, s->st, dim3(1), dim3(1), 0, s->sc, s->pkey + count, s->cap);
   TQ_LAUNCH_CHECK();
@@ -641,7 +641,7 @@ int add_particles_dev(tq_sim* s, int count, int check) {
 // Particle k of an emission takes draws 2k+1 (angle) and 2k+2 (radius) of the LCG stream that
 // starts at `seed`; records as the reference builds them (binary64 arithmetic, f32 stores).
 __device__ __forceinline__ void k_faucet(Blk B_, float* __restrict__ out, int n, uint32_t seed, double x, double y,
-                                                double raio, double vx, double vy, float a0, float a1, float a2) {
+                                         double raio, double vx, double vy, float a0, float a1, float a2) {
 #pragma clang fp contract(off)
   const int k = B_.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -666,7 +666,7 @@ __device__ __forceinline__ void k_faucet(Blk B_, float* __restrict__ out, int n,
 int faucet_dev(tq_sim* s, const tq_faucet* f, double vx, double vy, int n, int keep, uint32_t* seed) {
   if (keep > 0) {
     launch<256, 0, k_faucet>(s, s->st, dim3((keep + 255) / 256), dim3(256), 0, s->stage, keep, *seed, f->x, f->y,
-                       f->raio, vx, vy, (float)f->absorcao[0], (float)f->absorcao[1], (float)f->absorcao[2]);
+                             f->raio, vx, vy, (float)f->absorcao[0], (float)f->absorcao[1], (float)f->absorcao[2]);
     TQ_LAUNCH_CHECK();
     if (int e = add_particles_dev(s, keep, 1)) return e;
   }
@@ -676,8 +676,8 @@ int faucet_dev(tq_sim* s, const tq_faucet* f, double vx, double vy, int n, int k
 
 // ================================================================== aplicarFerramenta (fisica-cpu.js:129-152)
 __device__ __forceinline__ void k_tool(Blk B_, float2* __restrict__ pos, float2* __restrict__ vel, float4* __restrict__ ext,
-                                              const DevScalars* __restrict__ sc, int kind, double fx, double fy,
-                                              double raio, double fvx, double fvy, float a0, float a1, float a2) {
+                                       const DevScalars* __restrict__ sc, int kind, double fx, double fy,
+                                       double raio, double fvx, double fvy, float a0, float a1, float a2) {
 #pragma clang fp contract(off)
   const uint32_t k = B_.x * blockDim.x + threadIdx.x;
   if (k >= sc->n_slots) return;
@@ -709,15 +709,15 @@ int apply_tool_dev(tq_sim* s, const tq_tool* t) {
   if (!t || t->kind == 0 || s->n_upper <= 0) return 0;
   const int o = s->cur;
   launch<256, 0, k_tool>(s, s->st, dim3((s->n_upper + 255) / 256), dim3(256), 0, s->pos[o], s->vel[o], s->ext[o], s->sc,
-                     t->kind, t->x, t->y, t->raio, t->vx, t->vy, (float)t->absorcao[0], (float)t->absorcao[1],
-                     (float)t->absorcao[2]);
+                         t->kind, t->x, t->y, t->raio, t->vx, t->vy, (float)t->absorcao[0], (float)t->absorcao[1],
+                         (float)t->absorcao[2]);
   TQ_LAUNCH_CHECK();
   return 0;
 }
 
 // ================================================================== live count
 __device__ __forceinline__ void k_count_live(Blk B_, const float2* __restrict__ pos, const DevScalars* __restrict__ sc,
-                                                    uint32_t* __restrict__ out) {
+                                             uint32_t* __restrict__ out) {
   // one atomic per block (per-wave atomics on the one counter serialise in the cache)
   __shared__ uint32_t wc[4];
   const uint32_t k = B_.x * blockDim.x + threadIdx.x;
@@ -749,7 +749,7 @@ int count_live(tq_sim* s, int* n) {
 // the status words of a non-blocking readback: the count is cleared and vMax copied (one thread);
 // k_count_live adds the live particles behind it
 __device__ __forceinline__ void k_status_begin(Blk B_, unsigned long long* __restrict__ st,
-                                                     const DevScalars* __restrict__ sc) {
+                                               const DevScalars* __restrict__ sc) {
   if (threadIdx.x == 0) {
     st[0] = 0ull;
     st[1] = sc->vmax_bits;
@@ -758,10 +758,10 @@ __device__ __forceinline__ void k_status_begin(Blk B_, unsigned long long* __res
 
 int status_enqueue(tq_sim* s) {
   launch<64, 0, k_status_begin>(s, s->st, dim3(1), dim3(64), 0, (unsigned long long*)s->status_dev,
-                     (const DevScalars*)s->sc);
+                                (const DevScalars*)s->sc);
   if (s->n_upper > 0)
     launch<256, 0, k_count_live>(s, s->st, dim3((s->n_upper + 255) / 256), dim3(256), 0, s->pos[s->cur], s->sc,
-                       (uint32_t*)s->status_dev);
+                                 (uint32_t*)s->status_dev);
   TQ_LAUNCH_CHECK();
   return 0;
 }

@@ -417,8 +417,8 @@ __device__ inline bool fold_update(const Blk& B_, const double* __restrict__ par
 constexpr int PRE_R = 64;
 constexpr int FOLD_DIRECT = 1280;  // consumers re-sum up to this many partials themselves
 __device__ __forceinline__ void k_prereduce(Blk B_, const double* __restrict__ src, int nb, int nv,
-                                                   const double* __restrict__ bsrc, double* __restrict__ dst,
-                                                   double* __restrict__ bdst, const DevScalars* __restrict__ sc) {
+                                            const double* __restrict__ bsrc, double* __restrict__ dst,
+                                            double* __restrict__ bdst, const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[(2 + BSLOTS) * 16];
   const int per = (nb + PRE_R - 1) / PRE_R;
@@ -457,8 +457,8 @@ struct PyrPlan {
 };
 
 __device__ __forceinline__ void k_setup_pyr(Blk B_, const uint8_t* __restrict__ tipo, const uint32_t* __restrict__ rot,
-                                                    const uint32_t* __restrict__ facesB, int ar,
-                                                    uint8_t* __restrict__ opMask, const PyrPlan& P) {
+                                            const uint32_t* __restrict__ facesB, int ar,
+                                            uint8_t* __restrict__ opMask, const PyrPlan& P) {
   __shared__ uint8_t ta[PYR_T * PYR_T], tb[PYR_T * PYR_T / 4];
   const int tid = threadIdx.x;
   const int nx = P.nx[0], ny = P.ny[0];
@@ -520,7 +520,7 @@ __device__ __forceinline__ void k_setup_pyr(Blk B_, const uint8_t* __restrict__ 
 
 // tiposGrossos (pressao-cpu.js:63-81): air if any child is air, else water if any is water
 __device__ __forceinline__ void k_coarse_type(Blk B_, const uint8_t* __restrict__ fm, int fnx, int fny,
-                                                     uint8_t* __restrict__ cm, int cnx, int cny) {
+                                              uint8_t* __restrict__ cm, int cnx, int cny) {
   const int C = B_.x * blockDim.x + threadIdx.x;
   if (C >= cnx * cny) return;
   const int J = C / cnx, I = C - J * cnx;
@@ -547,7 +547,7 @@ struct VizPlan {
   uint8_t* mask[12];
 };
 __device__ __forceinline__ void k_viz_all(Blk B_, const VizPlan& P, uint8_t* __restrict__ saved, int force,
-                                                 DevScalars* __restrict__ sc) {
+                                          DevScalars* __restrict__ sc) {
   const int g = B_.x * blockDim.x + threadIdx.x;
   if (g >= P.off[P.L]) return;
   int l = P.v0;
@@ -576,8 +576,8 @@ __device__ __forceinline__ void k_viz_all(Blk B_, const VizPlan& P, uint8_t* __r
 
 // bubble vectors, the active list (labels with water faces, in label order) and their slots
 __device__ __forceinline__ void k_bub_init(Blk B_, const uint32_t* __restrict__ facesB, const float* __restrict__ yB32,
-                                                   int ar, double* __restrict__ bvec, int* __restrict__ activeList,
-                                                   int* __restrict__ labelSlot, DevScalars* __restrict__ sc) {
+                                           int ar, double* __restrict__ bvec, int* __restrict__ activeList,
+                                           int* __restrict__ labelSlot, DevScalars* __restrict__ sc) {
   __shared__ int cnt[16];
   const int tid = threadIdx.x;
   int mine = 0;
@@ -630,12 +630,12 @@ __device__ __forceinline__ void k_bub_restart(Blk B_, double* __restrict__ bvec)
 // qmode 0: q alone (the reference's binary32 iterate); 1: compensated iterate q + qlo starting
 // with qlo = 0; 2: a restart of the compensated iterate (its residual from q + qlo in binary64).
 __device__ __forceinline__ void k_pcg_init(Blk B_, const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
-                                                  const float* __restrict__ div, float* __restrict__ q,
-                                                  float* __restrict__ qlo, int qmode,
-                                                  float* __restrict__ r, float* __restrict__ pbuf0, int nx, int C,
-                                                  double* __restrict__ bvec, const int* __restrict__ labelSlot,
-                                                  double* __restrict__ partials, double* __restrict__ bpart,
-                                                  DevScalars* __restrict__ sc) {
+                                           const float* __restrict__ div, float* __restrict__ q,
+                                           float* __restrict__ qlo, int qmode,
+                                           float* __restrict__ r, float* __restrict__ pbuf0, int nx, int C,
+                                           double* __restrict__ bvec, const int* __restrict__ labelSlot,
+                                           double* __restrict__ partials, double* __restrict__ bpart,
+                                           DevScalars* __restrict__ sc) {
   __shared__ double sh[BSLOTS * 16];
   __shared__ BubLds aggt;
   const int ns = nslots(sc->active_bubbles);
@@ -698,13 +698,13 @@ __device__ __forceinline__ void k_pcg_init(Blk B_, const uint8_t* __restrict__ o
 // set here, as arguments, so no host copy precedes the solve); restart = 1 restarts the
 // compensated iterate from its true residual, adding the finished segment's iterations to the count.
 __device__ __forceinline__ void k_fin_init(Blk B_, const uint32_t* __restrict__ facesB, double* __restrict__ bvec,
-                                                          const double* __restrict__ rhsB,
-                                                          const int* __restrict__ activeList,
-                                                          const int* __restrict__ labelSlot,
-                                                          const double* __restrict__ partials,
-                                                          const double* __restrict__ bpart, int nb, double tol,
-                                                          int maxit, float mult, int restart,
-                                                          DevScalars* __restrict__ sc) {
+                                           const double* __restrict__ rhsB,
+                                           const int* __restrict__ activeList,
+                                           const int* __restrict__ labelSlot,
+                                           const double* __restrict__ partials,
+                                           const double* __restrict__ bpart, int nb, double tol,
+                                           int maxit, float mult, int restart,
+                                           DevScalars* __restrict__ sc) {
   if (threadIdx.x == 0) {
     if (restart) {
       const int seg = sc->itp[0] > sc->itp[1] ? sc->itp[0] : sc->itp[1];
@@ -943,15 +943,15 @@ __device__ inline void block_tile(const Blk& B_, int* tx, int* ty) {
 constexpr int IN_A0_GRID = 1280;  // 5 blocks per CU at this kernel's 92 VGPRs
 template <class SB, bool UPD, bool PASSA>
 __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMask, const uint8_t* __restrict__ mg0,
-                                               const uint32_t* __restrict__ rot, const uint32_t* __restrict__ cellsB,
-                                               const float* __restrict__ p, const float* __restrict__ ap,
-                                               float* __restrict__ q, float* __restrict__ qlo, int comp,
-                                               float* __restrict__ r, int nx, int ny, int ntx,
-                                               int nty, int vgx, int nvb, BubArgs B, const double* __restrict__ partA,
-                                               const double* __restrict__ bpartA, int nbs, int par,
-                                               SB* __restrict__ b0, float* __restrict__ x0,
-                                               double* __restrict__ partRR, int scaled, const float* __restrict__ mgs,
-                                               DevScalars* __restrict__ sc) {
+                                        const uint32_t* __restrict__ rot, const uint32_t* __restrict__ cellsB,
+                                        const float* __restrict__ p, const float* __restrict__ ap,
+                                        float* __restrict__ q, float* __restrict__ qlo, int comp,
+                                        float* __restrict__ r, int nx, int ny, int ntx,
+                                        int nty, int vgx, int nvb, BubArgs B, const double* __restrict__ partA,
+                                        const double* __restrict__ bpartA, int nbs, int par,
+                                        SB* __restrict__ b0, float* __restrict__ x0,
+                                        double* __restrict__ partRR, int scaled, const float* __restrict__ mgs,
+                                        DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ float Tt[4][TCELLS];
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], aps[BSLOTS], rbs[BSLOTS];
@@ -1087,7 +1087,7 @@ __device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, f
 
 // z and the dots when level 0 itself is solved inside the workgroup (small grids)
 __device__ __forceinline__ void k_zdots0(Blk B_, const float* __restrict__ x0, ZArgs za, int C,
-                                                const DevScalars* __restrict__ sc) {
+                                         const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[BSLOTS * 16];
   __shared__ BubLds aggt;
@@ -1111,10 +1111,10 @@ __device__ __forceinline__ void k_zdots0(Blk B_, const float* __restrict__ x0, Z
 // colours 1,0): water cells read x + P e, with the coarse values of the tile's 10x10 patch
 template <class SX, class SB, class SC>
 __device__ __forceinline__ void k_prolong_pass(Blk B_, const SX* __restrict__ src, SX* __restrict__ dst,
-                                                      const SB* __restrict__ b, const uint8_t* __restrict__ mask,
-                                                      int nx, int ny, int ntx, int nty, const SC* __restrict__ cx,
-                                                      const uint8_t* __restrict__ cmk, int cnx, int cny,
-                                                      const DevScalars* __restrict__ sc) {
+                                               const SB* __restrict__ b, const uint8_t* __restrict__ mask,
+                                               int nx, int ny, int ntx, int nty, const SC* __restrict__ cx,
+                                               const uint8_t* __restrict__ cmk, int cnx, int cny,
+                                               const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ float Tt[4][TCELLS];
   __shared__ float CPt[4][100];
@@ -1191,8 +1191,8 @@ __device__ __forceinline__ void k_prolong_pass(Blk B_, const SX* __restrict__ sr
 // last up pass of level 0 (offset 0, colours 1,0) writing z and the dot partials
 template <class SB>
 __device__ __forceinline__ void k_pass_z(Blk B_, const float* __restrict__ src, const SB* __restrict__ b,
-                                                const uint8_t* __restrict__ mask, int nx, int ny, int ntx, int nty,
-                                                ZArgs za, const DevScalars* __restrict__ sc) {
+                                         const uint8_t* __restrict__ mask, int nx, int ny, int ntx, int nty,
+                                         ZArgs za, const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ float Tt[4][TCELLS];
   __shared__ double sh[BSLOTS * 16];
@@ -1259,10 +1259,10 @@ static int pass_fold_grid(const tq_sim* s) {
 // it first does the norm and stop test (fold_update)
 template <class SX, class SB, int C0>
 __device__ __forceinline__ void k_pass(Blk B_, const SX* __restrict__ src, SX* __restrict__ dst,
-                                              const SB* __restrict__ b, const uint8_t* __restrict__ mask, int nx,
-                                              int ny, int desl, int ntx, int nty, int vgx, int nvb,
-                                              const double* __restrict__ partRR, int nbu, int par, int fold,
-                                              DevScalars* __restrict__ sc) {
+                                       const SB* __restrict__ b, const uint8_t* __restrict__ mask, int nx,
+                                       int ny, int desl, int ntx, int nty, int vgx, int nvb,
+                                       const double* __restrict__ partRR, int nbu, int par, int fold,
+                                       DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ float Tt[4][TCELLS];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1310,10 +1310,10 @@ __device__ __forceinline__ void k_pass(Blk B_, const SX* __restrict__ src, SX* _
 // (offset 0 = this tile) from x = 0 unless the coarse level is solved otherwise.
 template <class SXF, class SBF, class SCB, class SCX, bool PASSA>
 __device__ __forceinline__ void k_restrict_a(Blk B_, const SXF* __restrict__ fx, const SBF* __restrict__ fb,
-                                                    const uint8_t* __restrict__ fm, int fnx, int fny,
-                                                    SCB* __restrict__ cb, SCX* __restrict__ cxo,
-                                                    const uint8_t* __restrict__ cmk, int cnx, int cny,
-                                                    const DevScalars* __restrict__ sc) {
+                                             const uint8_t* __restrict__ fm, int fnx, int fny,
+                                             SCB* __restrict__ cb, SCX* __restrict__ cxo,
+                                             const uint8_t* __restrict__ cmk, int cnx, int cny,
+                                             const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ float X[36 * 36], R[34 * 34], Bt[256], T[TCELLS];
   const int tid = threadIdx.x;
@@ -1430,11 +1430,11 @@ constexpr int XBW = 48;  // x_B window: fine cells 2*CI0 - 8 .. 2*CI0 + 39
 
 template <class SXF, class SBF, class SCB, class SCX, bool PASSA>
 __device__ __forceinline__ void k_bra(Blk B_, const SXF* __restrict__ fx, SXF* __restrict__ fxb,
-                                                       const SBF* __restrict__ fb, const uint8_t* __restrict__ fm,
-                                                       int fnx, int fny, SCB* __restrict__ cb, SCX* __restrict__ cxo,
-                                                       const uint8_t* __restrict__ cmk, int cnx, int cny,
-                                                       const double* __restrict__ partRR, int nbu, int par, int fold,
-                                                       DevScalars* __restrict__ sc) {
+                                      const SBF* __restrict__ fb, const uint8_t* __restrict__ fm,
+                                      int fnx, int fny, SCB* __restrict__ cb, SCX* __restrict__ cxo,
+                                      const uint8_t* __restrict__ cmk, int cnx, int cny,
+                                      const double* __restrict__ partRR, int nbu, int par, int fold,
+                                      DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ float Tt[BRA_WAVES][TCELLS];
   __shared__ float XB[XBW * XBW];
@@ -1573,9 +1573,9 @@ constexpr int CW = 32;  // pass-C window: cells 16a - 8 .. 16a + 23
 
 template <class SX, class SB, class SC, bool ZD>
 __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __restrict__ xout,
-                                             const SB* __restrict__ b, const uint8_t* __restrict__ mask, int nx,
-                                             int ny, const SC* __restrict__ cx, const uint8_t* __restrict__ cmk,
-                                             int cnx, int cny, ZArgs za, const DevScalars* __restrict__ sc) {
+                                      const SB* __restrict__ b, const uint8_t* __restrict__ mask, int nx,
+                                      int ny, const SC* __restrict__ cx, const uint8_t* __restrict__ cmk,
+                                      int cnx, int cny, ZArgs za, const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ float Tt[4][TCELLS];
   __shared__ float CPt[4][100];
@@ -1725,7 +1725,7 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
 // wavefront per column runs the sweeps on its own copy of the level (LDS), cells in registers.
 // Columns of cells that are not water are zero (their b is never read).
 __device__ __forceinline__ void k_coarsest_op(Blk B_, const uint8_t* __restrict__ mask, int nx, int n,
-                                                    float* __restrict__ G, const DevScalars* __restrict__ sc) {
+                                              float* __restrict__ G, const DevScalars* __restrict__ sc) {
   if (!sc->op_dirty) return;  // masks as at the last build: G is current
   __shared__ float x[DENSE_MAX];
   const int j = B_.x, lane = threadIdx.x;
@@ -2009,7 +2009,7 @@ __device__ __forceinline__ void wg_load_masks(const WGPlan& P, char* pool) {
 // coarsest) x after its pass A; the result of level lc's V-cycle is written to out_lc, which may
 // be x_lc (every input is in LDS before the first output is stored).
 __device__ __forceinline__ void k_coarse_wg(Blk B_, const WGPlan& P, const float* __restrict__ b_lc, const float* x_lc,
-                                                    float* out_lc, const DevScalars* __restrict__ sc) {
+                                            float* out_lc, const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ __attribute__((aligned(16))) char pool[WG_POOL];
   __shared__ float tiles[WG_WAVES][TCELLS];
@@ -2035,7 +2035,7 @@ __device__ __forceinline__ void k_coarse_wg(Blk B_, const WGPlan& P, const float
 // neither reads b nor writes x there.
 constexpr int VOP_WAVES = 4;
 __device__ __forceinline__ void k_vcycle_op(Blk B_, const WGPlan& P, int ld, float* __restrict__ C,
-                                                             DevScalars* __restrict__ sc) {
+                                            DevScalars* __restrict__ sc) {
   if (!sc->op_dirty) return;  // masks as at the last build: C is current
   extern __shared__ __attribute__((aligned(16))) char vpool[];
   const int lc = P.lc, n = P.nx[lc] * P.ny[lc], j = B_.x;
@@ -2060,8 +2060,8 @@ __device__ __forceinline__ void k_op_count(Blk B_, DevScalars* __restrict__ sc) 
 // x = C b on level lc: VOP_ROWS rows per wavefront, each lane reading float4 columns
 constexpr int VOP_ROWS = 2;
 __device__ __forceinline__ void k_vcycle_apply(Blk B_, const float* __restrict__ C, int ld, int n,
-                                                      const float* __restrict__ b, float* __restrict__ x,
-                                                      const DevScalars* __restrict__ sc) {
+                                               const float* __restrict__ b, float* __restrict__ x,
+                                               const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   const int lane = threadIdx.x & 63;
   const int r0 = (B_.x * 4 + (threadIdx.x >> 6)) * VOP_ROWS;
@@ -2108,12 +2108,12 @@ constexpr int SPMV_LOADS = (SPMV_HN + 255) / 256;    // halo-tile cells per thre
 constexpr int SPMV_GRID = 1024;  // 4 blocks per CU: the kernel is held to 128 VGPRs (4 waves per SIMD)
 static int spmv_tiles(int nx, int ny) { return ((nx + SPMV_TW - 1) / SPMV_TW) * ((ny + SPMV_TH - 1) / SPMV_TH); }
 __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
-                                              const float* __restrict__ z, const float* __restrict__ pold,
-                                              float* __restrict__ pnew, float* __restrict__ ap, int nx, int ny,
-                                              BubArgs B, const double* __restrict__ partZ,
-                                              const double* __restrict__ bpartZ, int nbz, int par, int first,
-                                              double* __restrict__ partA, double* __restrict__ bpartA,
-                                              DevScalars* __restrict__ sc) {
+                                       const float* __restrict__ z, const float* __restrict__ pold,
+                                       float* __restrict__ pnew, float* __restrict__ ap, int nx, int ny,
+                                       BubArgs B, const double* __restrict__ partZ,
+                                       const double* __restrict__ bpartZ, int nbz, int par, int first,
+                                       double* __restrict__ partA, double* __restrict__ bpartA,
+                                       DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], zs[BSLOTS];
   __shared__ float P[SPMV_HN];
@@ -2214,7 +2214,7 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
 
 // the update fold as a kernel of its own, when level 0 has no down kernel (one-workgroup grids)
 __device__ __forceinline__ void k_fold_update(Blk B_, const double* __restrict__ part, int nb, int par,
-                                                             DevScalars* __restrict__ sc) {
+                                              DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[16], res[1];
   (void)fold_update(B_, part, nb, par, sh, res, sc);
@@ -2226,7 +2226,7 @@ __device__ __forceinline__ void k_pcg_final(Blk B_, const uint8_t* __restrict__ 
   if (c < C && tipo[c] != AGUA) q[c] = 0.f;
 }
 __device__ __forceinline__ void k_pcg_final_b(Blk B_, const double* __restrict__ bvec, const int* __restrict__ activeList,
-                                                      float* __restrict__ yB32, DevScalars* __restrict__ sc) {
+                                              float* __restrict__ yB32, DevScalars* __restrict__ sc) {
   if (threadIdx.x == 0) {
     const int seg = sc->itp[0] > sc->itp[1] ? sc->itp[0] : sc->itp[1];
     if (sc->restarts == 0) sc->iters_first = seg;
@@ -2247,11 +2247,11 @@ __device__ __forceinline__ void k_pcg_final_b(Blk B_, const double* __restrict__
 __device__ inline bool true_res_skip(const DevScalars* sc, int gated) { return gated && (!sc->done || sc->true_done); }
 
 __device__ __forceinline__ void k_true_res(Blk B_, const uint8_t* __restrict__ opMask, const uint32_t* __restrict__ rot,
-                                                  const float* __restrict__ div, const float* __restrict__ q,
-                                                  const float* __restrict__ qlo, int nx, int C,
-                                                  double* __restrict__ bvec, const int* __restrict__ labelSlot,
-                                                  double* __restrict__ partials, double* __restrict__ bpart, int gated,
-                                                  const DevScalars* __restrict__ sc) {
+                                           const float* __restrict__ div, const float* __restrict__ q,
+                                           const float* __restrict__ qlo, int nx, int C,
+                                           double* __restrict__ bvec, const int* __restrict__ labelSlot,
+                                           double* __restrict__ partials, double* __restrict__ bpart, int gated,
+                                           const DevScalars* __restrict__ sc) {
   if (true_res_skip(sc, gated)) return;
   __shared__ double sh[BSLOTS * 16];
   __shared__ BubLds aggt;
@@ -2290,13 +2290,13 @@ __device__ __forceinline__ void k_true_res(Blk B_, const uint8_t* __restrict__ o
 }
 
 __device__ __forceinline__ void k_fin_true_res(Blk B_, const uint32_t* __restrict__ facesB,
-                                                              double* __restrict__ bvec, const double* __restrict__ rhsB,
-                                                              const int* __restrict__ activeList,
-                                                              const int* __restrict__ labelSlot,
-                                                              const double* __restrict__ partials,
-                                                              const double* __restrict__ bpart, int nb, int gated,
-                                                              DevScalars* __restrict__ sc,
-                                                              double* __restrict__ out) {
+                                               double* __restrict__ bvec, const double* __restrict__ rhsB,
+                                               const int* __restrict__ activeList,
+                                               const int* __restrict__ labelSlot,
+                                               const double* __restrict__ partials,
+                                               const double* __restrict__ bpart, int nb, int gated,
+                                               DevScalars* __restrict__ sc,
+                                               double* __restrict__ out) {
   if (true_res_skip(sc, gated)) return;
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], slotTot[BSLOTS];
   const int na = sc->active_bubbles;
@@ -2492,7 +2492,7 @@ static RedSrc red_src(tq_sim* s, int slot, int nv, int bhalf, int nb) {
   double* bdst = bhalf < 0 ? nullptr : s->bpart_r + (size_t)bhalf * BSLOTS * MAX_RED_BLOCKS;
   int t = time_begin(s, "pcg_prereduce", 8.0 * nv * nb);
   launch<256, 0, k_prereduce>(s, s->st, dim3(PRE_R), dim3(256), 0, src, nb, nv, bsrc, dst, bdst,
-                     (const DevScalars*)s->sc);
+                              (const DevScalars*)s->sc);
   time_end(s, t);
   return {dst, bdst, PRE_R};
 }
@@ -2514,12 +2514,11 @@ static void launch_in_a0(tq_sim* s, bool upd) {
   const int par = upd ? it_par(s) : 0;
   const BubArgs B = bubargs(s);
   const float* mgs = &s->sc->mgs[it_par(s)];
-#define TQ_IN_A0(SB, U, P)                                                                                        \
+#define TQ_IN_A0(SB, U, P)                                                                                 \
   launch<256, 0, k_in_a0<SB, U, P>>(s, s->st, g, dim3(256), 0, s->opMask, F.mask, s->rot, s->cellsB, pp, s->ap, \
-                     s->q, s->qlo, s->qcomp, s->r, F.nx, F.ny, ntx, nty, vgx, nvb, B, rs.part, rs.bpart, rs.nb, par, \
-                     (SB*)F.b,                                                                                     \
-                     (float*)F.x,                                                                                  \
-                     s->partials + ps_off(PS_RR), s->mg_scaled, mgs, s->sc)
+                                    s->q, s->qlo, s->qcomp, s->r, F.nx, F.ny, ntx, nty, vgx, nvb, B, rs.part,   \
+                                    rs.bpart, rs.nb, par, (SB*)F.b, (float*)F.x, s->partials + ps_off(PS_RR),  \
+                                    s->mg_scaled, mgs, s->sc)
   if (F.hb) {
     if (upd) { if (pa) TQ_IN_A0(__half, true, true); else TQ_IN_A0(__half, true, false); }
     else { if (pa) TQ_IN_A0(__half, false, true); else TQ_IN_A0(__half, false, false); }
@@ -2572,8 +2571,8 @@ static void launch_bra(tq_sim* s, int l) {
       // the folding launch (level 0 of an iteration) walks several groups per block after one fold
       const int nbk = fold ? std::min(nvb, pass_fold_grid(s)) : nvb;
       launch<256, 0, k_pass<SX, SB, 0>>(s, s->st, dim3(nbk), dim3(256), 0, (const SX*)F.x, (SX*)F.tmp,
-                         (const SB*)F.b, F.mask, F.nx, F.ny, 8, ntx8, nty8, (int)tg.x, nvb, rr, rsu.nb, it_par(s),
-                         fold, s->sc);
+          (const SB*)F.b, F.mask, F.nx, F.ny, 8, ntx8, nty8, (int)tg.x, nvb, rr, rsu.nb, it_par(s),
+          fold, s->sc);
     });
     time_end(s, t);
     t = time_begin(s, rnames[l], (vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n +
@@ -2586,10 +2585,10 @@ static void launch_bra(tq_sim* s, int l) {
         using SCB = typename decltype(cb)::type;
         if (pa)
           launch<256, 0, k_restrict_a<SXF, SBF, SCB, SCX, true>>(s, s->st, g, dim3(256), 0, (const SXF*)F.tmp,
-                             (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny, s->sc);
+              (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny, s->sc);
         else
           launch<256, 0, k_restrict_a<SXF, SBF, SCB, SCX, false>>(s, s->st, g, dim3(256), 0, (const SXF*)F.tmp,
-                             (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny, s->sc);
+              (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny, s->sc);
       });
     });
     time_end(s, t);
@@ -2609,12 +2608,12 @@ static void launch_bra(tq_sim* s, int l) {
       using SCB = typename decltype(cb)::type;
       if (pa)
         launch<64 * BRA_WAVES, 0, k_bra<SXF, SBF, SCB, SCX, true>>(s, s->st, g, dim3(64 * BRA_WAVES), 0, (const SXF*)F.x,
-                           (SXF*)F.tmp, (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny,
-                           rr, rsu.nb, it_par(s), fold, s->sc);
+            (SXF*)F.tmp, (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny,
+            rr, rsu.nb, it_par(s), fold, s->sc);
       else
         launch<64 * BRA_WAVES, 0, k_bra<SXF, SBF, SCB, SCX, false>>(s, s->st, g, dim3(64 * BRA_WAVES), 0, (const SXF*)F.x,
-                           (SXF*)F.tmp, (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny,
-                           rr, s->nb_upd, it_par(s), fold, s->sc);
+            (SXF*)F.tmp, (const SBF*)F.b, F.mask, F.nx, F.ny, (SCB*)G.b, (SCX*)G.x, G.mask, G.nx, G.ny,
+            rr, s->nb_upd, it_par(s), fold, s->sc);
     });
   });
   time_end(s, t);
@@ -2650,12 +2649,12 @@ static int launch_pcd(tq_sim* s, int l) {
       TQ_TYPES(tx, tb);
       if (G.hx)
         launch<256, 0, k_prolong_pass<SX, SB, __half>>(s, s->st, tile_grid(F.nx, F.ny, 8), dim3(256), 0,
-                           (const SX*)F.tmp, (SX*)F.x, (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8,
-                           (const __half*)G.res, G.mask, G.nx, G.ny, s->sc);
+            (const SX*)F.tmp, (SX*)F.x, (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8,
+            (const __half*)G.res, G.mask, G.nx, G.ny, s->sc);
       else
         launch<256, 0, k_prolong_pass<SX, SB, float>>(s, s->st, tile_grid(F.nx, F.ny, 8), dim3(256), 0,
-                           (const SX*)F.tmp, (SX*)F.x, (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8,
-                           (const float*)G.res, G.mask, G.nx, G.ny, s->sc);
+            (const SX*)F.tmp, (SX*)F.x, (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8,
+            (const float*)G.res, G.mask, G.nx, G.ny, s->sc);
     });
     time_end(s, t);
     t = time_begin(s, dnames[l], (2 * vbytes(F.hx) + vbytes(F.hb) + 1) * (double)F.n);
@@ -2663,8 +2662,8 @@ static int launch_pcd(tq_sim* s, int l) {
       TQ_TYPES(tx, tb);
       const dim3 tg = tile_grid(F.nx, F.ny, 0);
       launch<256, 0, k_pass<SX, SB, 1>>(s, s->st, dim3(tg.x * tg.y), dim3(256), 0, (const SX*)F.x,
-                         (SX*)F.tmp, (const SB*)F.b, F.mask, F.nx, F.ny, 0, ntx, nty, (int)tg.x, (int)(tg.x * tg.y),
-                         (const double*)nullptr, 0, 0, 0, s->sc);
+          (SX*)F.tmp, (const SB*)F.b, F.mask, F.nx, F.ny, 0, ntx, nty, (int)tg.x, (int)(tg.x * tg.y),
+          (const double*)nullptr, 0, 0, 0, s->sc);
     });
     time_end(s, t);
     F.res = F.tmp;
@@ -2677,10 +2676,10 @@ static int launch_pcd(tq_sim* s, int l) {
       using SC = typename decltype(tc)::type;
       if (zd)
         launch<256, 0, k_pcd<SX, SB, SC, true>>(s, s->st, g, dim3(256), 0, (const SX*)F.tmp, (SX*)F.x,
-                           (const SB*)F.b, F.mask, F.nx, F.ny, (const SC*)G.res, G.mask, G.nx, G.ny, za, s->sc);
+            (const SB*)F.b, F.mask, F.nx, F.ny, (const SC*)G.res, G.mask, G.nx, G.ny, za, s->sc);
       else
         launch<256, 0, k_pcd<SX, SB, SC, false>>(s, s->st, g, dim3(256), 0, (const SX*)F.tmp, (SX*)F.x,
-                           (const SB*)F.b, F.mask, F.nx, F.ny, (const SC*)G.res, G.mask, G.nx, G.ny, za, s->sc);
+            (const SB*)F.b, F.mask, F.nx, F.ny, (const SC*)G.res, G.mask, G.nx, G.ny, za, s->sc);
     };
     if (G.hx) go(Tag<__half>{}); else go(Tag<float>{});
   });
@@ -2737,7 +2736,7 @@ static void launch_coarse_wg(tq_sim* s) {
     const int waves = (G.n + VOP_ROWS - 1) / VOP_ROWS;
     int t = time_begin(s, "mg_coarse_mv", 4.0 * s->vop_ld * G.n + 8.0 * G.n);
     launch<256, 0, k_vcycle_apply>(s, s->st, dim3((waves + 3) / 4), dim3(256), 0, (const float*)s->vop, s->vop_ld,
-                       G.n, (const float*)G.b, (float*)G.x, s->sc);
+                                   G.n, (const float*)G.b, (float*)G.x, s->sc);
     time_end(s, t);
     return;
   }
@@ -2748,7 +2747,7 @@ static void launch_coarse_wg(tq_sim* s) {
   if (s->dense) bytes += 4.0 * s->lv[L - 1].n * s->lv[L - 1].n;
   int t = time_begin(s, "mg_coarse_wg", bytes);
   launch<1024, 0, k_coarse_wg>(s, s->st, dim3(1), dim3(1024), 0, P, (const float*)G.b, (const float*)G.x,
-                     (float*)G.x, s->sc);
+                               (float*)G.x, s->sc);
   time_end(s, t);
 }
 
@@ -2786,12 +2785,12 @@ static int launch_up0(tq_sim* s) {
     TQ_TYPES(tx, tb);
     if (G.hx)
       launch<256, 0, k_prolong_pass<SX, SB, __half>>(s, s->st, g8, dim3(256), 0, (const SX*)F.tmp, (SX*)F.x,
-                         (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8, (const __half*)G.res, G.mask, G.nx, G.ny,
-                         s->sc);
+          (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8, (const __half*)G.res, G.mask, G.nx, G.ny,
+          s->sc);
     else
       launch<256, 0, k_prolong_pass<SX, SB, float>>(s, s->st, g8, dim3(256), 0, (const SX*)F.tmp, (SX*)F.x,
-                         (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8, (const float*)G.res, G.mask, G.nx, G.ny,
-                         s->sc);
+          (const SB*)F.b, F.mask, F.nx, F.ny, ntx8, nty8, (const float*)G.res, G.mask, G.nx, G.ny,
+          s->sc);
   });
   time_end(s, t);
   const int ntx = tiles_of(F.nx, 0), nty = tiles_of(F.ny, 0);
@@ -2800,10 +2799,10 @@ static int launch_up0(tq_sim* s) {
   t = time_begin(s, "mg_pass_z_l0", (4 + vbytes(F.hb) + 1 + 1 + 4 + 4 + 4) * (double)F.n);
   if (F.hb)
     launch<256, 0, k_pass_z<__half>>(s, s->st, g, dim3(256), 0, (const float*)F.x, (const __half*)F.b, F.mask, F.nx,
-                       F.ny, ntx, nty, za, s->sc);
+                                     F.ny, ntx, nty, za, s->sc);
   else
     launch<256, 0, k_pass_z<float>>(s, s->st, g, dim3(256), 0, (const float*)F.x, (const float*)F.b, F.mask, F.nx,
-                       F.ny, ntx, nty, za, s->sc);
+                                    F.ny, ntx, nty, za, s->sc);
   time_end(s, t);
   return (int)(g.x * g.y);
 }
@@ -2814,7 +2813,7 @@ static void precondition_rest(tq_sim* s) {
   if (lc == 0 && s->it_k >= 0) {  // no level-0 down kernel to carry the norm and stop test
     int t = time_begin(s, "pcg_update_fold", 0);
     launch<FIN_THREADS, 0, k_fold_update>(s, s->st, dim3(1), dim3(FIN_THREADS), 0,
-                       (const double*)(s->partials + ps_off(PS_RR)), s->nb_upd, it_par(s), s->sc);
+                                          (const double*)(s->partials + ps_off(PS_RR)), s->nb_upd, it_par(s), s->sc);
     time_end(s, t);
   }
   for (int l = 0; l < lc; l++) launch_bra(s, l);
@@ -2841,8 +2840,8 @@ static void pcg_iteration(tq_sim* s) {
   const RedSrc rz = red_src(s, PS_Z, 2, 1, s->nb_z);
   int t = time_begin(s, "pcg_spmv", (1 + 4 + 4 + 4 + 4) * (double)C);
   launch<256, 4, k_spmv>(s, s->st, dim3(nbs), dim3(256), 0, s->opMask, s->rot, s->zbuf[s->zc], s->pbuf[s->pc],
-                     s->pbuf[pn], s->ap, s->nx, s->ny, bubargs(s), rz.part, rz.bpart, rz.nb, it_par(s),
-                     s->it_k == 0 ? 1 : 0, s->partials + ps_off(PS_SPMV), bpart_a(s), s->sc);
+                         s->pbuf[pn], s->ap, s->nx, s->ny, bubargs(s), rz.part, rz.bpart, rz.nb, it_par(s),
+                         s->it_k == 0 ? 1 : 0, s->partials + ps_off(PS_SPMV), bpart_a(s), s->sc);
   time_end(s, t);
   s->nb_spmv = nbs;
   s->pc = pn;
@@ -2925,7 +2924,7 @@ static int solve_setup(tq_sim* s) {
     launch<256, 0, k_coarse_type>(s, s->st, dim3(blocks_of(G.n)), dim3(256), 0, F.mask, F.nx, F.ny, G.mask, G.nx, G.ny);
   }
   launch<1024, 0, k_bub_init>(s, s->st, dim3(1), dim3(1024), 0, s->facesB, s->yB32, s->d.ar, s->bvec, s->activeList,
-                     s->labelSlot, s->sc);
+                              s->labelSlot, s->sc);
   {
     // neighbour bits of levels 1.., and whether the dense operators must be rebuilt (their masks
     // changed, or the level-lc matrix was not kept current while it was switched off)
@@ -2950,7 +2949,7 @@ static int solve_setup(tq_sim* s) {
   if (s->dense) {
     const Level& G = s->lv[L - 1];
     launch<64, 0, k_coarsest_op>(s, s->st, dim3(G.n), dim3(64), 0, G.mask, G.nx, G.n, s->dense,
-                       (const DevScalars*)s->sc);
+                                 (const DevScalars*)s->sc);
   }
   time_end(s, t);
   if (vop_active(s))
@@ -2966,9 +2965,9 @@ static void launch_true_check(tq_sim* s) {
   const int C = s->C;
   const int nb = blocks_of(C);
   launch<256, 0, k_true_res>(s, s->st, dim3(nb), dim3(256), 0, s->opMask, s->rot, s->div, s->q, (const float*)s->qlo,
-                     s->nx, C, s->bvec, s->labelSlot, s->partials, s->bpart, 1, (const DevScalars*)s->sc);
+                             s->nx, C, s->bvec, s->labelSlot, s->partials, s->bpart, 1, (const DevScalars*)s->sc);
   launch<FIN_THREADS, 0, k_fin_true_res>(s, s->st, dim3(1), dim3(FIN_THREADS), 0, s->facesB, s->bvec, s->rhsB, s->activeList,
-                     s->labelSlot, s->partials, s->bpart, nb, 1, s->sc, (double*)nullptr);
+                                         s->labelSlot, s->partials, s->bpart, nb, 1, s->sc, (double*)nullptr);
 }
 
 constexpr int MAX_RESTARTS = 3;  // residual replacements per solve (each one re-runs the set-up residual)
@@ -2992,10 +2991,10 @@ int pressure_solve(tq_sim* s) {
     int t = time_begin(s, "pcg_init", (1 + 1 + 4 + 4 + 4 + 4 + 4 + (s->qcomp ? 4 : 0)) * (double)C);
     if (restart) launch<1024, 0, k_bub_restart>(s, s->st, dim3(1), dim3(1024), 0, s->bvec);
     launch<256, 0, k_pcg_init>(s, s->st, dim3(nb), dim3(256), 0, s->opMask, s->rot, s->div, s->q, s->qlo,
-                       s->qcomp ? (restart ? 2 : 1) : 0, s->r, s->pbuf[s->pc], s->nx, C, s->bvec, s->labelSlot,
-                       s->partials, s->bpart, s->sc);
+                               s->qcomp ? (restart ? 2 : 1) : 0, s->r, s->pbuf[s->pc], s->nx, C, s->bvec, s->labelSlot,
+                               s->partials, s->bpart, s->sc);
     launch<FIN_THREADS, 0, k_fin_init>(s, s->st, dim3(1), dim3(FIN_THREADS), 0, s->facesB, s->bvec, s->rhsB, s->activeList,
-                       s->labelSlot, s->partials, s->bpart, nb, tol, maxit, mult, restart ? 1 : 0, s->sc);
+        s->labelSlot, s->partials, s->bpart, nb, tol, maxit, mult, restart ? 1 : 0, s->sc);
     time_end(s, t);
     TQ_LAUNCH_CHECK();
     s->it_k = -1;
@@ -3039,9 +3038,9 @@ int true_residual(tq_sim* s, double* out) {
   const int nb = blocks_of(C);
   double* dres = s->partials + ps_off(PS_LAST) + MAX_RED_BLOCKS - 1;
   launch<256, 0, k_true_res>(s, s->st, dim3(nb), dim3(256), 0, s->opMask, s->rot, s->div, s->q, (const float*)nullptr,
-                     s->nx, C, s->bvec, s->labelSlot, s->partials, s->bpart, 0, (const DevScalars*)s->sc);
+                             s->nx, C, s->bvec, s->labelSlot, s->partials, s->bpart, 0, (const DevScalars*)s->sc);
   launch<FIN_THREADS, 0, k_fin_true_res>(s, s->st, dim3(1), dim3(FIN_THREADS), 0, s->facesB, s->bvec, s->rhsB, s->activeList,
-                     s->labelSlot, s->partials, s->bpart, nb, 0, s->sc, dres);
+                                         s->labelSlot, s->partials, s->bpart, nb, 0, s->sc, dres);
   TQ_LAUNCH_CHECK();
   TQ_HIP(hipMemcpyAsync(out, dres, sizeof(double), hipMemcpyDeviceToHost, s->st));
   TQ_HIP(hipStreamSynchronize(s->st));
