@@ -174,6 +174,21 @@ template <class S> __device__ inline float rnd(float v);
 template <> __device__ inline float rnd<float>(float v) { return v; }
 template <> __device__ inline float rnd<__half>(float v) { return __half2float(sat_half(v)); }
 
+// Guarded loads: the element at c where `in` holds, else z.  The load itself is unconditional
+// (index 0 when `in` fails: every array here has an element 0), so the loads of a tile are
+// issued back to back and their latencies overlap.  Written as `in ? p[c] : z`, each load sits
+// in a branch of its own and the wave waits for it before issuing the next one.
+template <class T>
+__device__ __forceinline__ T ld_if(const T* __restrict__ p, size_t c, bool in, T z) {
+  const T v = p[in ? c : 0];
+  return in ? v : z;
+}
+template <class S>
+__device__ __forceinline__ float ldf_if(const S* __restrict__ p, size_t c, bool in) {
+  const float v = ld<S>(p, in ? c : 0);
+  return in ? v : 0.f;
+}
+
 // all LDS operations of this wavefront are complete (and not reordered across by the compiler)
 __device__ inline void wave_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
@@ -221,10 +236,24 @@ __device__ inline void sum_partials(const double* __restrict__ part, int nb, int
   double acc[NMAX];
 #pragma unroll
   for (int k = 0; k < NMAX; k++) acc[k] = 0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+  // partials b, b + blockDim.x, ... in that order; two of them per step, every quantity's load
+  // issued before any is added (quantities k >= nv load quantity 0 and add nothing)
+  const int bd = blockDim.x;
+  for (int b = threadIdx.x; b - (int)threadIdx.x < nb; b += 2 * bd) {
+    const bool in0 = b < nb, in1 = b + bd < nb;
+    double x0[NMAX], x1[NMAX];
 #pragma unroll
-    for (int k = 0; k < NMAX; k++)
-      if (k < nv) acc[k] += part[(size_t)k * MAX_RED_BLOCKS + b];
+    for (int k = 0; k < NMAX; k++) {
+      const size_t row = (size_t)(k < nv ? k : 0) * MAX_RED_BLOCKS;
+      x0[k] = part[row + (in0 ? b : 0)];
+      x1[k] = part[row + (in1 ? b + bd : 0)];
+    }
+#pragma unroll
+    for (int k = 0; k < NMAX; k++) {
+      if (k < nv && in0) acc[k] += x0[k];
+      if (k < nv && in1) acc[k] += x1[k];
+    }
+  }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
   for (int k = 0; k < NMAX; k++)
@@ -796,9 +825,11 @@ __device__ __forceinline__ void k_fin_init(Blk B_, const uint32_t* __restrict__ 
 // 2p + 1 (p = lane & 7); one cell of each pair is red and one black.  Cells4 indexes them
 // [colour][row], so a half-sweep touches registers at compile-time indices only.
 // m: bits 0-3 free neighbours, bit 4 updated by the sweeps (water with a free neighbour),
-// bit 5 water (kept in the output; other cells are written as 0).
+// bit 5 water (kept in the output; other cells are written as 0).  ik = 1/count and kp = 0 on
+// updated cells, ik = 0 and kp = 1 on the others, so one fma gives either the update or the
+// unchanged value.
 struct Cells4 {
-  float v[2][2], b[2][2], ik[2][2];
+  float v[2][2], b[2][2], ik[2][2], kp[2][2];
   int m[2][2];
 };
 
@@ -811,37 +842,44 @@ __device__ inline int cell_e(int lane, int q, int r) { return cell_lx(lane, q) +
 __device__ inline void cell_setup(Cells4& c, int q, int r, uint8_t mk, float b) {
   const int cnt = mask_cnt(mk);
   const bool water = mask_type(mk) == AGUA;
-  c.m[q][r] = mask_viz(mk) | (water && cnt > 0 ? 16 : 0) | (water ? 32 : 0);
+  const bool upd = water && cnt > 0;
+  c.m[q][r] = mask_viz(mk) | (upd ? 16 : 0) | (water ? 32 : 0);
   c.b[q][r] = b;
-  c.ik[q][r] = cnt > 0 ? 1.f / (float)cnt : 0.f;
+  c.ik[q][r] = upd ? 1.f / (float)cnt : 0.f;
+  c.kp[q][r] = upd ? 0.f : 1.f;
 }
 
 __device__ inline void cell_clear(Cells4& c, int q, int r) {
   c.m[q][r] = 0;
   c.b[q][r] = 0.f;
   c.ik[q][r] = 0.f;
+  c.kp[q][r] = 1.f;
 }
 
-// the cells of colour COR: x = (sum of free neighbours + b) / count
+// the cells of colour COR: x = (sum of free neighbours + b) / count.  Every cell that is not a
+// free neighbour (solid, or outside the grid) holds +0 in the tile: x is 0 on every cell that
+// is not water (the passes write 0 there, the tile's border outside the grid is 0), so the sum
+// over all four neighbours in the reference's order (left, right, down, up) has the bits of
+// the sum over the free ones.  Both rows' neighbours are read before any arithmetic (one LDS
+// latency per half-sweep), and the update is branch-free: fma(s + b, 1/count, 0) on updated
+// cells, fma(s + b, 0, x) = x on the others.
 template <int COR>
 __device__ inline void half_sweep(float* T, Cells4& c, int lane) {
-  // the lane's other-colour cell is the right neighbour when this one sits at column 2p
-  const bool left_is_mine = ((COR + (lane >> 3) + 1) & 1) != 0;
+  float tl[2], tr[2], td[2], tu[2];
 #pragma unroll
   for (int r = 0; r < 2; r++) {
     const int e = cell_e(lane, COR, r);
-    const float tl = T[e - 1], tr = T[e + 1], td = T[e - TS], tu = T[e + TS];
-    const float oth = c.v[1 - COR][r];
-    const int m = c.m[COR][r];
-    const float L = left_is_mine ? oth : tl;
-    const float R = left_is_mine ? tr : oth;
-    float sum = (m & 1) ? L : 0.f;
-    sum += (m & 2) ? R : 0.f;
-    sum += (m & 4) ? td : 0.f;
-    sum += (m & 8) ? tu : 0.f;
-    const float nv = (m & 16) ? (sum + c.b[COR][r]) * c.ik[COR][r] : c.v[COR][r];
+    tl[r] = T[e - 1];
+    tr[r] = T[e + 1];
+    td[r] = T[e - TS];
+    tu[r] = T[e + TS];
+  }
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const float sum = ((tl[r] + tr[r]) + td[r]) + tu[r];
+    const float nv = __builtin_fmaf(sum + c.b[COR][r], c.ik[COR][r], c.kp[COR][r] * c.v[COR][r]);
     c.v[COR][r] = nv;
-    T[e] = nv;
+    T[cell_e(lane, COR, r)] = nv;
   }
 }
 
@@ -858,17 +896,23 @@ __device__ inline void tile_sweeps(float* T, Cells4& c, int lane) {
   }
 }
 
-// T := snapshot of src on the tile + halo (0 outside the grid)
+// T := snapshot of src on the tile + halo (0 outside the grid): the six loads of a lane first,
+// then the six LDS stores
 template <class SX>
 __device__ inline void fill_tile(float* T, const SX* __restrict__ src, int ox, int oy, int nx, int ny, int lane) {
+  float v[6];
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     const int e = lane + 64 * q;
-    if (e < TCELLS) {
-      const int ly = e / TS, lx = e - ly * TS;
-      const int i = ox - 1 + lx, j = oy - 1 + ly;
-      T[e] = (i >= 0 && j >= 0 && i < nx && j < ny) ? ld<SX>(src, (size_t)i + (size_t)j * nx) : 0.f;
-    }
+    const int ly = e / TS, lx = e - ly * TS;
+    const int i = ox - 1 + lx, j = oy - 1 + ly;
+    const bool in = e < TCELLS && i >= 0 && j >= 0 && i < nx && j < ny;
+    v[q] = ldf_if<SX>(src, (size_t)i + (size_t)j * nx, in);
+  }
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const int e = lane + 64 * q;
+    if (e < TCELLS) T[e] = v[q];
   }
 }
 
@@ -897,8 +941,8 @@ __device__ inline void prefetch_cells(Own4& o, const SB* __restrict__ b, const u
       const int i = ox + cell_lx(lane, q), j = oy + cell_ly(lane, r);
       const bool in = i < nx && j < ny && i >= 0 && j >= 0;
       const size_t cc = in ? (size_t)i + (size_t)j * nx : 0;
-      o.m[q][r] = in ? mask[cc] : 0;
-      o.b[q][r] = in ? ld<SB>(b, cc) : 0.f;
+      o.m[q][r] = ld_if<uint8_t>(mask, cc, in, 0);
+      o.b[q][r] = ldf_if<SB>(b, cc, in);
     }
 }
 
@@ -982,8 +1026,8 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
         in[cq][rw] = valid && i < nx && j < ny;
         const size_t c = in[cq][rw] ? (size_t)i + (size_t)j * nx : 0;
         ci[cq][rw] = c;
-        om[cq][rw] = in[cq][rw] ? opMask[c] : 0;
-        gm[cq][rw] = in[cq][rw] ? mg0[c] : 0;
+        om[cq][rw] = ld_if<uint8_t>(opMask, c, in[cq][rw], 0);
+        gm[cq][rw] = ld_if<uint8_t>(mg0, c, in[cq][rw], 0);
         rv[cq][rw] = r[c];
         if (UPD) {
           qv[cq][rw] = q[c];
@@ -1138,8 +1182,8 @@ __device__ __forceinline__ void k_prolong_pass(Blk B_, const SX* __restrict__ sr
     const int I = PX + ai, J = PY + bj;
     const bool in = e < 100 && I >= 0 && J >= 0 && I < cnx && J < cny;
     const size_t C = in ? (size_t)I + (size_t)J * cnx : 0;
-    cm[q] = in ? cmk[C] : 0;
-    cv[q] = in ? ld<SC>(cx, C) : 0.f;
+    cm[q] = ld_if<uint8_t>(cmk, C, in, 0);
+    cv[q] = ldf_if<SC>(cx, C, in);
   }
   float sv[6];
   uint8_t sm[6];
@@ -1150,8 +1194,8 @@ __device__ __forceinline__ void k_prolong_pass(Blk B_, const SX* __restrict__ sr
     const int i = ox - 1 + lx, j = oy - 1 + ly;
     const bool in = e < TCELLS && i >= 0 && j >= 0 && i < nx && j < ny;
     const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
-    sv[q] = in ? ld<SX>(src, c) : 0.f;
-    sm[q] = in ? mask[c] : 0;
+    sv[q] = ldf_if<SX>(src, c, in);
+    sm[q] = ld_if<uint8_t>(mask, c, in, 0);
   }
 #pragma unroll
   for (int q = 0; q < 2; q++) {
@@ -1219,7 +1263,7 @@ __device__ __forceinline__ void k_pass_z(Blk B_, const float* __restrict__ src, 
         const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
         const bool in = i < nx && j < ny;
         const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
-        om[cq][rw] = in ? za.opMask[c] : 0;
+        om[cq][rw] = ld_if<uint8_t>(za.opMask, c, in, 0);
         rv[cq][rw] = za.r[c];
         zo[cq][rw] = za.zold[c];
       }
@@ -1331,29 +1375,33 @@ __device__ __forceinline__ void k_restrict_a(Blk B_, const SXF* __restrict__ fx,
     const int i = fx0 + 1 + lx, j = fy0 + 1 + ly;
     const bool in = e < 34 * 34 && i >= 0 && j >= 0 && i < fnx && j < fny;
     const size_t c = in ? (size_t)i + (size_t)j * fnx : 0;
-    mr[q] = in ? fm[c] : 0;
-    br[q] = in ? ld<SBF>(fb, c) : 0.f;
+    mr[q] = ld_if<uint8_t>(fm, c, in, 0);
+    br[q] = ldf_if<SBF>(fb, c, in);
   }
   const int Im = CI0 + (tid & 15), Jm = CJ0 + (tid >> 4);
   const bool cin = Im < cnx && Jm < cny;
   const size_t Cm = cin ? (size_t)Im + (size_t)Jm * cnx : 0;
-  const uint8_t cmine = cin ? cmk[Cm] : 0;
+  const uint8_t cmine = ld_if<uint8_t>(cmk, Cm, cin, 0);
   uint8_t cma[2][2];
 #pragma unroll
   for (int cq = 0; cq < 2; cq++)
 #pragma unroll
     for (int rw = 0; rw < 2; rw++) {
       const int I = CI0 + cell_lx(tid & 63, cq), J = CJ0 + cell_ly(tid & 63, rw);
-      cma[cq][rw] = (PASSA && tid < 64 && I < cnx && J < cny) ? cmk[(size_t)I + (size_t)J * cnx] : 0;
+      cma[cq][rw] = ld_if<uint8_t>(cmk, (size_t)I + (size_t)J * cnx, PASSA && tid < 64 && I < cnx && J < cny, 0);
     }
+  float xv[6];
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     const int e = tid + 256 * q;
-    if (e < 36 * 36) {
-      const int ly = e / 36, lx = e - ly * 36;
-      const int i = fx0 + lx, j = fy0 + ly;
-      X[e] = (i >= 0 && j >= 0 && i < fnx && j < fny) ? ld<SXF>(fx, (size_t)i + (size_t)j * fnx) : 0.f;
-    }
+    const int ly = e / 36, lx = e - ly * 36;
+    const int i = fx0 + lx, j = fy0 + ly;
+    xv[q] = ldf_if<SXF>(fx, (size_t)i + (size_t)j * fnx, e < 36 * 36 && i >= 0 && j >= 0 && i < fnx && j < fny);
+  }
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const int e = tid + 256 * q;
+    if (e < 36 * 36) X[e] = xv[q];
   }
   __syncthreads();
 #pragma unroll
@@ -1460,20 +1508,20 @@ __device__ __forceinline__ void k_bra(Blk B_, const SXF* __restrict__ fx, SXF* _
     const int i = 2 * CI0 - 1 + lx, j = 2 * CJ0 - 1 + ly;
     const bool in = e < 34 * 34 && i >= 0 && j >= 0 && i < fnx && j < fny;
     const size_t c = in ? (size_t)i + (size_t)j * fnx : 0;
-    mr[q] = in ? fm[c] : 0;
-    br[q] = in ? ld<SBF>(fb, c) : 0.f;
+    mr[q] = ld_if<uint8_t>(fm, c, in, 0);
+    br[q] = ldf_if<SBF>(fb, c, in);
   }
   const int Im = CI0 + (tid & 15), Jm = CJ0 + ((tid >> 4) & 15);
   const bool cin = tid < 256 && Im < cnx && Jm < cny;
   const size_t Cm = cin ? (size_t)Im + (size_t)Jm * cnx : 0;
-  const uint8_t cmine = cin ? cmk[Cm] : 0;
+  const uint8_t cmine = ld_if<uint8_t>(cmk, Cm, cin, 0);
   uint8_t cma[2][2];
 #pragma unroll
   for (int cq = 0; cq < 2; cq++)
 #pragma unroll
     for (int rw = 0; rw < 2; rw++) {
       const int I = CI0 + cell_lx(lane, cq), J = CJ0 + cell_ly(lane, rw);
-      cma[cq][rw] = (PASSA && wave == 0 && I < cnx && J < cny) ? cmk[(size_t)I + (size_t)J * cnx] : 0;
+      cma[cq][rw] = ld_if<uint8_t>(cmk, (size_t)I + (size_t)J * cnx, PASSA && wave == 0 && I < cnx && J < cny, 0);
     }
   fill_tile<SXF>(T, fx, ox, oy, fnx, fny, lane);
   if (fold) {  // level 0 of an iteration: the norm and stop test (fold_update) while the loads land
@@ -1607,7 +1655,7 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
         const int i = ax + cell_lx(lane, cq), j = ay + cell_ly(lane, rw);
         const bool in = i < nx && j < ny;
         const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
-        om[cq][rw] = in ? za.opMask[c] : 0;
+        om[cq][rw] = ld_if<uint8_t>(za.opMask, c, in, 0);
         rv[cq][rw] = za.r[c];
         zo[cq][rw] = za.zold[c];
       }
@@ -1621,8 +1669,8 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
     const int I = PX + ai, J = PY + bj;
     const bool in = e < 100 && I >= 0 && J >= 0 && I < cnx && J < cny;
     const size_t C = in ? (size_t)I + (size_t)J * cnx : 0;
-    cm[q] = in ? cmk[C] : 0;
-    cv[q] = in ? ld<SC>(cx, C) : 0.f;
+    cm[q] = ld_if<uint8_t>(cmk, C, in, 0);
+    cv[q] = ldf_if<SC>(cx, C, in);
   }
   float sv[6];
   uint8_t sm[6];
@@ -1633,8 +1681,8 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
     const int i = ox - 1 + lx, j = oy - 1 + ly;
     const bool in = e < TCELLS && i >= 0 && j >= 0 && i < nx && j < ny;
     const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
-    sv[q] = in ? ld<SX>(xb, c) : 0.f;
-    sm[q] = in ? mask[c] : 0;
+    sv[q] = ldf_if<SX>(xb, c, in);
+    sm[q] = ld_if<uint8_t>(mask, c, in, 0);
   }
 #pragma unroll
   for (int q = 0; q < 2; q++) {
@@ -2069,18 +2117,20 @@ __device__ __forceinline__ void k_vcycle_apply(Blk B_, const float* __restrict__
   const int nk = ld >> 8;  // float4 columns per lane (ld is a multiple of 256, at most 2048)
   float4 cv[VOP_ROWS][8];
   float4 bv[8];
+  // every load unconditional (columns beyond ld read column block 0 and are not summed), so all
+  // of them are in flight together
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    if (k < nk) {
-      const int c = 4 * lane + 256 * k;
+    const int c = 4 * lane + 256 * (k < nk ? k : 0);
 #pragma unroll
-      for (int r = 0; r < VOP_ROWS; r++)
-        cv[r][k] = r0 + r < n ? *(const float4*)(C + (size_t)(r0 + r) * ld + c) : make_float4(0.f, 0.f, 0.f, 0.f);
-      bv[k].x = c < n ? b[c] : 0.f;
-      bv[k].y = c + 1 < n ? b[c + 1] : 0.f;
-      bv[k].z = c + 2 < n ? b[c + 2] : 0.f;
-      bv[k].w = c + 3 < n ? b[c + 3] : 0.f;
+    for (int r = 0; r < VOP_ROWS; r++) {
+      const float4 v = *(const float4*)(C + (size_t)(r0 + r < n ? r0 + r : r0) * ld + c);
+      cv[r][k] = r0 + r < n ? v : make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    bv[k].x = ld_if<float>(b, c, c < n, 0.f);
+    bv[k].y = ld_if<float>(b, c + 1, c + 1 < n, 0.f);
+    bv[k].z = ld_if<float>(b, c + 2, c + 2 < n, 0.f);
+    bv[k].w = ld_if<float>(b, c + 3, c + 3 < n, 0.f);
   }
 #pragma unroll
   for (int r = 0; r < VOP_ROWS; r++) {
@@ -2140,9 +2190,9 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
       const bool in = e < SPMV_HN && i >= 0 && j >= 0 && i < nx && j < ny;
       const int c = in ? i + j * nx : -1;
       gidx[q] = c;
-      mk[q] = in ? opMask[c] : 0;
-      zv[q] = in ? z[c] : 0.f;
-      pv[q] = in ? pold[c] : 0.f;
+      mk[q] = ld_if<uint8_t>(opMask, (size_t)c, in, 0);
+      zv[q] = ld_if<float>(z, (size_t)c, in, 0.f);
+      pv[q] = ld_if<float>(pold, (size_t)c, in, 0.f);
     }
   };
   load(ch);
