@@ -153,8 +153,13 @@ __host__ __device__ inline size_t ps_off(int slot) { return (size_t)slot * MAX_R
 constexpr int BSLOTS = 8;          // bubbles reduced through per-block slots
 constexpr int TS = 18;             // smoother tile with its 1-cell halo
 constexpr int TCELLS = TS * TS;
+// LDS row pitch of a smoother tile: with 24 floats the lanes' reads of one colour and its
+// neighbours (two columns per lane, rows lane / 8 and lane / 8 + 8) fall in distinct banks; the
+// natural pitch 18 gives two-way conflicts on every read of a half-sweep
+constexpr int TP = 24;
+constexpr int TLDS = TS * TP;      // floats of a tile in LDS
 constexpr int WG_WAVES = 16;       // one-workgroup kernel: 1024 threads
-constexpr int WG_POOL = 139264;    // its LDS pool for the level arrays (x, tmp, b, mask)
+constexpr int WG_POOL = 136192;    // its LDS pool for the level arrays (x, tmp, b, mask): 160 KiB - 16 tiles
 constexpr int WG_AUTO_CELLS = 2048;  // default: levels from the first one this small run in one workgroup
 constexpr int DENSE_MAX = 192;       // coarsest levels up to this many cells use the dense operator
 constexpr int VOP_MAX = 2048;        // largest level whose V-cycle may be applied as a dense matrix
@@ -207,6 +212,31 @@ __device__ inline void bub_add(double (&sl)[BSLOTS], const int* __restrict__ lab
   if (s >= 0) slot_add(sl, s, val);
   else bagg_add(agg, lab, val);
 }
+// the same with the label's slot already read
+__device__ inline void bub_add_s(double (&sl)[BSLOTS], int s, const BubAgg& agg, uint32_t lab, double val) {
+  if (s >= 0) slot_add(sl, s, val);
+  else bagg_add(agg, lab, val);
+}
+
+// The slotted bubbles of a solve: the set-up numbers the active labels in label order and gives
+// the first BSLOTS of them the slots 0..BSLOTS-1 (activeList[k] has slot k), so a label's slot is
+// its position among activeList[0 .. nslots(na)).  A kernel reads those labels beside its first
+// loads (their addresses depend on nothing) and finds a cell's slot by comparison, instead of a
+// load of labelSlot[label] that could only be issued after the label itself had arrived.
+struct Slots {
+  int lab[BSLOTS];
+  int n;
+};
+__device__ inline void slots_load(Slots& S, const int* __restrict__ activeList) {
+#pragma unroll
+  for (int k = 0; k < BSLOTS; k++) S.lab[k] = activeList[k];
+}
+__device__ inline int slot_of(const Slots& S, uint32_t lab) {
+  int s = -1;
+#pragma unroll
+  for (int k = BSLOTS - 1; k >= 0; k--) s = (k < S.n && S.lab[k] == (int)lab) ? k : s;
+  return s;
+}
 
 // Block sum of the first nv of v[]; thread 0 stores partial `bid` of each quantity.  nv must
 // be uniform over the block; `sh` holds NMAX*16 doubles.
@@ -228,23 +258,23 @@ __device__ inline void store_partials(const Blk& B_, double (&v)[NMAX], int nv, 
   __syncthreads();  // `sh` is free again for the caller's next reduction
 }
 
-// Totals of nb partials of the first nv quantities, in a fixed order (deterministic); the
-// result is valid in every thread.
+// Totals of nb partials of the first nv quantities, in a fixed order (deterministic), in two
+// stages so that a kernel can issue other loads between them: part_load adds this thread's
+// partials into acc[], part_reduce sums acc[] over the block (valid in every thread).
 template <int NMAX>
-__device__ inline void sum_partials(const double* __restrict__ part, int nb, int nv, double (&tot)[NMAX], double* sh,
-                                    double* res) {
-  double acc[NMAX];
+__device__ inline void part_load(const double* __restrict__ part, int nb, int nv, double (&acc)[NMAX]) {
 #pragma unroll
   for (int k = 0; k < NMAX; k++) acc[k] = 0;
   // partials b, b + blockDim.x, ... in that order; two of them per step, every quantity's load
-  // issued before any is added (quantities k >= nv load quantity 0 and add nothing)
+  // issued before any is added (the rows of quantities k >= nv are read and not added, so the
+  // addresses do not wait for nv)
   const int bd = blockDim.x;
   for (int b = threadIdx.x; b - (int)threadIdx.x < nb; b += 2 * bd) {
     const bool in0 = b < nb, in1 = b + bd < nb;
     double x0[NMAX], x1[NMAX];
 #pragma unroll
     for (int k = 0; k < NMAX; k++) {
-      const size_t row = (size_t)(k < nv ? k : 0) * MAX_RED_BLOCKS;
+      const size_t row = (size_t)k * MAX_RED_BLOCKS;
       x0[k] = part[row + (in0 ? b : 0)];
       x1[k] = part[row + (in1 ? b + bd : 0)];
     }
@@ -254,6 +284,9 @@ __device__ inline void sum_partials(const double* __restrict__ part, int nb, int
       if (k < nv && in1) acc[k] += x1[k];
     }
   }
+}
+template <int NMAX>
+__device__ inline void part_reduce(const double (&acc)[NMAX], int nv, double (&tot)[NMAX], double* sh, double* res) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
   for (int k = 0; k < NMAX; k++)
@@ -271,6 +304,13 @@ __device__ inline void sum_partials(const double* __restrict__ part, int nb, int
 #pragma unroll
   for (int k = 0; k < NMAX; k++) tot[k] = k < nv ? res[k] : 0.0;
   __syncthreads();
+}
+template <int NMAX>
+__device__ inline void sum_partials(const double* __restrict__ part, int nb, int nv, double (&tot)[NMAX], double* sh,
+                                    double* res) {
+  double acc[NMAX];
+  part_load<NMAX>(part, nb, nv, acc);
+  part_reduce<NMAX>(acc, nv, tot, sh, res);
 }
 
 // block sum valid in every thread (`res` holds NV doubles of LDS)
@@ -301,35 +341,72 @@ struct BubArgs {
 // accumulator) and Polak-Ribiere beta = max(0, (r.z - r.z_old) / rz_prev), 0 on the first
 // iteration.  Block 0 stores z of this parity, clears the accumulator the next z dots use
 // (the other parity) and records rz.
-__device__ inline double fold_zdots(const Blk& B_, const double* __restrict__ part, const double* __restrict__ bpartZ, int nb,
-                                    int par, int first, const BubArgs& B, double* sh, double* res, double* zs,
-                                    DevScalars* __restrict__ sc) {
-  const int na = sc->active_bubbles;
+// In three stages, so that the kernel's own loads overlap the fold's chain of dependent loads:
+// zf_load issues the partials' loads and this thread's first active bubble (k = threadIdx.x),
+// zf_load_bub that bubble's slot and vectors, zf_finish reduces (sums in the order of a
+// sequential loop over the partials and the active bubbles).
+struct ZBub {  // one active bubble's values read by the z-dot fold
+  int b, sl;
+  double rc, zo, az, po;
+};
+// bubble b = activeList[k] (slot k when k < BSLOTS)
+__device__ inline ZBub zbub_load(const BubArgs& B, int par, int b, int k) {
+  ZBub v;
+  v.b = b;
+  v.sl = k < BSLOTS ? k : -1;
+  v.rc = bv(B.bvec, BV_R0 + 1 - par)[b];
+  v.zo = bv(B.bvec, BV_Z0 + 1 - par)[b];
+  v.az = fx_read(bv(B.bvec, BV_ACCZ0 + par), b);
+  v.po = bv(B.bvec, BV_P0 + 1 - par)[b];
+  return v;
+}
+struct ZFold {
+  double a[2], s[BSLOTS];
   double tot[2];
-  sum_partials<2>(part, nb, 2, tot, sh, res);
+  int na, b0;
+  ZBub v0;
+};
+// (activeList holds MAX_BOLHAS > blockDim.x entries: this thread's entry is read before na is
+// known and kept only when threadIdx.x < na)
+__device__ inline void zf_load(ZFold& f, const double* __restrict__ part, const double* __restrict__ bpartZ, int nb,
+                               const BubArgs& B, int na) {
+  f.na = na;
+  const int b0 = B.activeList[threadIdx.x];
+  part_load<2>(part, nb, 2, f.a);
+  part_load<BSLOTS>(bpartZ, nb, nslots(na), f.s);
+  f.b0 = (int)threadIdx.x < na ? b0 : 0;
+}
+__device__ inline void zf_load_bub(ZFold& f, const BubArgs& B, int par) { f.v0 = zbub_load(B, par, f.b0, threadIdx.x); }
+// the partials' totals: r.z, r.z_old in f.tot, the slotted bubbles' z sums in zs[]
+__device__ inline void zf_reduce(ZFold& f, double* sh, double* res, double* zs) {
+  part_reduce<2>(f.a, 2, f.tot, sh, res);
   double stt[BSLOTS];
-  sum_partials<BSLOTS>(bpartZ, nb, nslots(na), stt, sh, res);
+  part_reduce<BSLOTS>(f.s, nslots(f.na), stt, sh, res);
   if (threadIdx.x == 0)
     for (int k = 0; k < BSLOTS; k++) zs[k] = stt[k];
   __syncthreads();
+}
+// beta; ps[] gets the new p of every slotted bubble (its z plus beta times its old p), valid
+// after the block's next barrier
+__device__ inline double zf_finish(const ZFold& f, const Blk& B_, int par, int first, const BubArgs& B, double* sh,
+                                   double* res, const double* zs, float* ps, DevScalars* __restrict__ sc) {
+  const int na = f.na;
+  const double* tot = f.tot;
   double* Zn = bv(B.bvec, BV_Z0 + par);
-  const double* Zo = bv(B.bvec, BV_Z0 + 1 - par);
-  const double* Rc = bv(B.bvec, BV_R0 + 1 - par);
-  const double* accZ = bv(B.bvec, BV_ACCZ0 + par);
   double* accZnext = bv(B.bvec, BV_ACCZ0 + 1 - par);
   const bool b0 = first_block(B_);
   double bacc[2] = {0, 0};
-  for (int k = threadIdx.x; k < na; k += blockDim.x) {
-    const int b = B.activeList[k];
-    const int sl = B.labelSlot[b];
-    const double zn = sl >= 0 ? zs[sl] : fx_read(accZ, b);
-    bacc[0] += Rc[b] * zn;
-    bacc[1] += Rc[b] * Zo[b];
+  auto add = [&](const ZBub& v) {
+    const double zn = v.sl >= 0 ? zs[v.sl] : v.az;
+    bacc[0] += v.rc * zn;
+    bacc[1] += v.rc * v.zo;
     if (b0) {
-      Zn[b] = zn;
-      fx_clear(accZnext, b);
+      Zn[v.b] = zn;
+      fx_clear(accZnext, v.b);
     }
-  }
+  };
+  if ((int)threadIdx.x < na) add(f.v0);
+  for (int k = threadIdx.x + blockDim.x; k < na; k += blockDim.x) add(zbub_load(B, par, B.activeList[k], k));
   block_sum_all<2>(bacc, sh, res);
   const double rzNovo = tot[0] + bacc[0], cruzado = tot[1] + bacc[1];
   const double rzPrev = sc->rzp[1 - par];
@@ -339,41 +416,89 @@ __device__ inline double fold_zdots(const Blk& B_, const double* __restrict__ pa
     sc->rz = rzNovo;
     sc->beta = beta;
   }
+  auto pslot = [&](const ZBub& v) {
+    if (v.sl >= 0) ps[v.sl] = (float)(zs[v.sl] + beta * v.po);
+  };
+  if ((int)threadIdx.x < na) pslot(f.v0);
+  for (int k = threadIdx.x + blockDim.x; k < na; k += blockDim.x) pslot(zbub_load(B, par, B.activeList[k], k));
   return beta;
 }
 
 // alpha = rz / p.Ap (pressao-cpu.js:370-380) at the start of in_a0, with the bubbles' p and Ap
 // (slot totals, else the atomic accumulator) and their new residuals (rbs[] for the slotted
 // ones).  Block 0 stores p, Ap, y += alpha p, r -= alpha Ap of the bubbles, the bubble part of
-// |r|^2 and clears the accumulator of the next spmv (the other parity).
-__device__ inline double fold_spmv(const Blk& B_, const double* __restrict__ part, const double* __restrict__ bpartA, int nb, int par,
-                                   const BubArgs& B, double* sh, double* res, double* aps, double* rbs,
-                                   DevScalars* __restrict__ sc) {
-  const int na = sc->active_bubbles;
+// |r|^2 and clears the accumulator of the next spmv (the other parity).  Staged like the z-dot
+// fold: af_load, af_load_bub, af_finish.
+struct ABub {  // one active bubble's values read by the alpha fold
+  int b, sl;
+  uint32_t fb, cb;
+  double zn, po, az, ro, yb;
+};
+// bubble b = activeList[k] (slot k when k < BSLOTS); cb its cell count
+__device__ inline ABub abub_load(const BubArgs& B, const uint32_t* __restrict__ cellsB, int par, int b, int k) {
+  ABub v;
+  v.b = b;
+  v.sl = k < BSLOTS ? k : -1;
+  v.cb = cellsB[b];
+  v.fb = B.facesB[b];
+  v.zn = bv(B.bvec, BV_Z0 + par)[b];
+  v.po = bv(B.bvec, BV_P0 + 1 - par)[b];
+  v.az = fx_read(bv(B.bvec, BV_ACCA0 + par), b);
+  v.ro = bv(B.bvec, BV_R0 + 1 - par)[b];
+  v.yb = bv(B.bvec, BV_Y)[b];
+  return v;
+}
+struct AFold {
+  double a[1], s[BSLOTS];
   double tot[1];
-  sum_partials<1>(part, nb, 1, tot, sh, res);
+  int na, b0;
+  double beta, rz;
+  ABub v0;
+};
+__device__ inline void af_load(AFold& f, const double* __restrict__ part, const double* __restrict__ bpartA, int nb,
+                               int par, const BubArgs& B, const DevScalars* __restrict__ sc) {
+  f.na = sc->active_bubbles;
+  f.beta = sc->beta;
+  f.rz = sc->rzp[par];
+  const int b0 = B.activeList[threadIdx.x];  // kept only when threadIdx.x < na (see zf_load)
+  part_load<1>(part, nb, 1, f.a);
+  part_load<BSLOTS>(bpartA, nb, nslots(f.na), f.s);
+  f.b0 = (int)threadIdx.x < f.na ? b0 : 0;
+}
+__device__ inline void af_load_bub(AFold& f, const BubArgs& B, const uint32_t* __restrict__ cellsB, int par) {
+  f.v0 = abub_load(B, cellsB, par, f.b0, threadIdx.x);
+}
+// the partials' totals: p.Ap in f.tot, the slotted bubbles' Ap sums in aps[]
+__device__ inline void af_reduce(AFold& f, double* sh, double* res, double* aps) {
+  part_reduce<1>(f.a, 1, f.tot, sh, res);
   double stt[BSLOTS];
-  sum_partials<BSLOTS>(bpartA, nb, nslots(na), stt, sh, res);
+  part_reduce<BSLOTS>(f.s, nslots(f.na), stt, sh, res);
   if (threadIdx.x == 0)
     for (int k = 0; k < BSLOTS; k++) aps[k] = stt[k];
   __syncthreads();
-  const double beta = sc->beta, rz = sc->rzp[par];
-  const double* Zn = bv(B.bvec, BV_Z0 + par);
-  const double* Po = bv(B.bvec, BV_P0 + 1 - par);
-  const double* Ro = bv(B.bvec, BV_R0 + 1 - par);
-  const double* accA = bv(B.bvec, BV_ACCA0 + par);
-  auto pap_of = [&](int b, double* pn_out) -> double {
-    const int sl = B.labelSlot[b];
-    const double pn = Zn[b] + beta * Po[b];
+}
+// alpha; rbs[] gets the slotted bubbles' new residuals and bsl[] their cells' V-cycle input
+// (residual over cell count, binary32), valid after the block's next barrier
+__device__ inline double af_finish(const AFold& f, const Blk& B_, int par, const BubArgs& B,
+                                   const uint32_t* __restrict__ cellsB, double* sh, double* res, const double* aps,
+                                   double* rbs, float* bsl, DevScalars* __restrict__ sc) {
+  const int na = f.na;
+  const double* tot = f.tot;
+  const double beta = f.beta, rz = f.rz;
+  auto pap_of = [&](const ABub& v, double* pn_out) -> double {
+    const double pn = v.zn + beta * v.po;
     *pn_out = pn;
-    return (double)B.facesB[b] * pn + (sl >= 0 ? aps[sl] : fx_read(accA, b));
+    return (double)v.fb * pn + (v.sl >= 0 ? aps[v.sl] : v.az);
   };
   double bacc[1] = {0};
-  for (int k = threadIdx.x; k < na; k += blockDim.x) {
+  auto add_pap = [&](const ABub& v) {
     double pn;
-    const double apb = pap_of(B.activeList[k], &pn);
+    const double apb = pap_of(v, &pn);
     bacc[0] += pn * apb;
-  }
+  };
+  const bool mine = (int)threadIdx.x < na;
+  if (mine) add_pap(f.v0);
+  for (int k = threadIdx.x + blockDim.x; k < na; k += blockDim.x) add_pap(abub_load(B, cellsB, par, B.activeList[k], k));
   block_sum_all<1>(bacc, sh, res);
   const double pap = tot[0] + bacc[0];
   const double alpha = fabs(pap) < 1e-30 ? 0.0 : rz / pap;
@@ -384,22 +509,25 @@ __device__ inline double fold_spmv(const Blk& B_, const double* __restrict__ par
   double* APb = bv(B.bvec, BV_AP);
   double* accAnext = bv(B.bvec, BV_ACCA0 + 1 - par);
   double rr[1] = {0};
-  for (int k = threadIdx.x; k < na; k += blockDim.x) {
-    const int b = B.activeList[k];
+  auto update = [&](const ABub& v) {
     double pn;
-    const double apb = pap_of(b, &pn);
-    const double rbn = Ro[b] - alpha * apb;
-    const int sl = B.labelSlot[b];
-    if (sl >= 0) rbs[sl] = rbn;
+    const double apb = pap_of(v, &pn);
+    const double rbn = v.ro - alpha * apb;
+    if (v.sl >= 0) {
+      rbs[v.sl] = rbn;
+      bsl[v.sl] = (float)(rbn / v.cb);
+    }
     rr[0] += rbn * rbn;
     if (b0) {
-      Pn[b] = pn;
-      APb[b] = apb;
-      Yb[b] += alpha * pn;
-      Rn[b] = rbn;
-      fx_clear(accAnext, b);
+      Pn[v.b] = pn;
+      APb[v.b] = apb;
+      Yb[v.b] = v.yb + alpha * pn;
+      Rn[v.b] = rbn;
+      fx_clear(accAnext, v.b);
     }
-  }
+  };
+  if (mine) update(f.v0);
+  for (int k = threadIdx.x + blockDim.x; k < na; k += blockDim.x) update(abub_load(B, cellsB, par, B.activeList[k], k));
   block_sum<1>(rr, sh);
   if (b0 && threadIdx.x == 0) {
     sc->alpha = alpha;
@@ -409,7 +537,7 @@ __device__ inline double fold_spmv(const Blk& B_, const double* __restrict__ par
   return alpha;
 }
 
-// the new residual of bubble b inside in_a0 (slotted: rbs[]; else recomputed like fold_spmv)
+// the new residual of bubble b inside in_a0 (slotted: rbs[]; else recomputed as the alpha fold does)
 __device__ inline double bubble_r(const BubArgs& B, int b, int par, double alpha, double beta, const double* rbs) {
   const int sl = B.labelSlot[b];
   if (sl >= 0) return rbs[sl];
@@ -837,7 +965,7 @@ struct Cells4 {
 // tile origins even)
 __device__ inline int cell_lx(int lane, int q) { return 2 * (lane & 7) + ((q + (lane >> 3) + 1) & 1); }
 __device__ inline int cell_ly(int lane, int r) { return (lane >> 3) + 8 * r; }
-__device__ inline int cell_e(int lane, int q, int r) { return cell_lx(lane, q) + 1 + (cell_ly(lane, r) + 1) * TS; }
+__device__ inline int cell_e(int lane, int q, int r) { return cell_lx(lane, q) + 1 + (cell_ly(lane, r) + 1) * TP; }
 
 __device__ inline void cell_setup(Cells4& c, int q, int r, uint8_t mk, float b) {
   const int cnt = mask_cnt(mk);
@@ -871,8 +999,8 @@ __device__ inline void half_sweep(float* T, Cells4& c, int lane) {
     const int e = cell_e(lane, COR, r);
     tl[r] = T[e - 1];
     tr[r] = T[e + 1];
-    td[r] = T[e - TS];
-    tu[r] = T[e + TS];
+    td[r] = T[e - TP];
+    tu[r] = T[e + TP];
   }
 #pragma unroll
   for (int r = 0; r < 2; r++) {
@@ -912,7 +1040,8 @@ __device__ inline void fill_tile(float* T, const SX* __restrict__ src, int ox, i
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     const int e = lane + 64 * q;
-    if (e < TCELLS) T[e] = v[q];
+    const int ly = e / TS, lx = e - ly * TS;
+    if (e < TCELLS) T[lx + ly * TP] = v[q];
   }
 }
 
@@ -920,7 +1049,8 @@ __device__ inline void zero_tile(float* T, int lane) {
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     const int e = lane + 64 * q;
-    if (e < TCELLS) T[e] = 0.f;
+    const int ly = e / TS, lx = e - ly * TS;
+    if (e < TCELLS) T[lx + ly * TP] = 0.f;
   }
 }
 
@@ -984,7 +1114,7 @@ __device__ inline void block_tile(const Blk& B_, int* tx, int* ty) {
 // pressao-cpu.js:243) when level 0 is not the coarsest.  One-dimensional grid over the 2x2-tile
 // groups (vgx of them per row, nvb in all); large grids launch at most IN_A0_GRID blocks, each
 // walking its groups after one fold.
-constexpr int IN_A0_GRID = 1280;  // 5 blocks per CU at this kernel's 92 VGPRs
+constexpr int IN_A0_GRID = 1024;  // 4 blocks per CU at this kernel's 123 VGPRs
 template <class SB, bool UPD, bool PASSA>
 __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMask, const uint8_t* __restrict__ mg0,
                                         const uint32_t* __restrict__ rot, const uint32_t* __restrict__ cellsB,
@@ -996,9 +1126,12 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
                                         SB* __restrict__ b0, float* __restrict__ x0,
                                         double* __restrict__ partRR, int scaled, const float* __restrict__ mgs,
                                         DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ float Tt[4][TCELLS];
+  // the stop flag is read beside the first loads (one memory latency for both); a stopped solve
+  // returns once they are issued
+  const int sc_done = sc->done;
+  __shared__ float Tt[4][TLDS];
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], aps[BSLOTS], rbs[BSLOTS];
+  __shared__ float bsl[BSLOTS];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int vb = xcd_block(B_.x, B_.gx);
   int ox, oy;
@@ -1037,12 +1170,34 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
         }
       }
   };
+  // bubble cells: their labels (loaded once the masks have arrived) and slots (by comparison
+  // with the slotted labels read beside the first loads); the first cycle also reads their
+  // residuals and cell counts
+  uint32_t lbv[2][2];
+  auto load_bub_labels = [&]() {
+#pragma unroll
+    for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+      for (int rw = 0; rw < 2; rw++)
+        lbv[cq][rw] = ld_if<uint32_t>(rot, ci[cq][rw], in[cq][rw] && (om[cq][rw] & MK_BUB), 0u);
+  };
+  Slots S;
   load(vb);
-  // UPD: alpha and the bubbles' new residuals from spmv's partials (fold_spmv, while the loads
-  // above are in flight); else (the first cycle) the current bubble residuals are those of
+  slots_load(S, B.activeList);
+  // UPD: alpha and the bubbles' new residuals from spmv's partials (the alpha fold, its loads
+  // beside the tile's); else (the first cycle) the current bubble residuals are those of
   // parity 1 - par
-  const double alpha = UPD ? fold_spmv(B_, partA, bpartA, nbs, par, B, sh, res, aps, rbs, sc) : 0.0;
-  const double beta = UPD ? sc->beta : 0.0;
+  AFold af;
+  if (UPD) af_load(af, partA, bpartA, nbs, par, B, sc);
+  if (sc_done) return;
+  S.n = nslots(sc->active_bubbles);
+  load_bub_labels();
+  if (UPD) {
+    af_load_bub(af, B, cellsB, par);
+    af_reduce(af, sh, res, aps);
+  }
+  const double alpha = UPD ? af_finish(af, B_, par, B, cellsB, sh, res, aps, rbs, bsl, sc) : 0.0;
+  const double beta = UPD ? af.beta : 0.0;
   for (;;) {
     Cells4 cc;
 #pragma unroll
@@ -1074,9 +1229,10 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
             }
             bval = rn;
           } else if (m & MK_BUB) {
-            const uint32_t lb = rot[c];
-            const double rbl = UPD ? bubble_r(B, lb, par, alpha, beta, rbs) : rb[lb];
-            bval = (float)(rbl / cellsB[lb]);
+            const uint32_t lb = lbv[cq][rw];
+            const int sl = slot_of(S, lb);
+            if (UPD && sl >= 0) bval = bsl[sl];
+            else bval = (float)((UPD ? bubble_r(B, lb, par, alpha, beta, rbs) : rb[lb]) / cellsB[lb]);
           }
           bval = rnd<SB>(bval * scale);
           st<SB>(b0, c, bval);
@@ -1094,6 +1250,7 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
     vb += B_.gx;
     if (vb >= nvb) break;
     load(vb);
+    load_bub_labels();
   }
   if (UPD) store_partials<1>(B_, rr, 1, partRR, sh);
 }
@@ -1103,6 +1260,7 @@ struct ZArgs {  // level-0 outputs: z = MG result and the dots r.z, r.z_old
   const uint32_t* rot;
   const uint32_t* cellsB;
   const int* labelSlot;
+  const int* activeList;  // its first BSLOTS entries are the slotted labels
   const float* r;
   const float* zold;
   float* znew;
@@ -1113,9 +1271,52 @@ struct ZArgs {  // level-0 outputs: z = MG result and the dots r.z, r.z_old
   int scaled;
 };
 
-// z and its dot/bubble contributions for one cell (x = MG output of level 0)
+// The slotted bubbles (labels read beside a kernel's first loads) and their cell counts (read
+// once the labels have arrived, beside the cells' own labels)
+struct ZSlots {
+  Slots S;
+  uint32_t cb[BSLOTS];
+};
+__device__ inline void zslots_cells(ZSlots& Z, const ZArgs& za) {
+#pragma unroll
+  for (int k = 0; k < BSLOTS; k++) {
+    const uint32_t lab = (uint32_t)Z.S.lab[k];  // entries past the active count are not used
+    Z.cb[k] = za.cellsB[lab < (uint32_t)MAX_BOLHAS ? lab : 0u];
+  }
+}
+// The bubble cells among a lane's four level-0 cells: their labels (one load in flight each,
+// issued once the masks have arrived; other cells read entry 0), then slots by comparison and
+// cell counts from the slotted bubbles' (a label without a slot reads its own)
+struct ZBubCells {
+  uint32_t lb[2][2], cb[2][2];
+  int sl[2][2];
+};
+__device__ inline void zbub_labels(ZBubCells& zb, const ZArgs& za, const uint8_t (&om)[2][2], const size_t (&c)[2][2]) {
+#pragma unroll
+  for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+    for (int rw = 0; rw < 2; rw++)
+      zb.lb[cq][rw] = ld_if<uint32_t>(za.rot, c[cq][rw], (om[cq][rw] & MK_BUB) != 0, 0u);
+}
+__device__ inline void zbub_vals(ZBubCells& zb, const ZArgs& za, const ZSlots& Z) {
+#pragma unroll
+  for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+    for (int rw = 0; rw < 2; rw++) {
+      const int sl = slot_of(Z.S, zb.lb[cq][rw]);
+      uint32_t cb = 0;
+#pragma unroll
+      for (int k = 0; k < BSLOTS; k++) cb = k == sl ? Z.cb[k] : cb;
+      zb.sl[cq][rw] = sl;
+      zb.cb[cq][rw] = (sl >= 0 || zb.lb[cq][rw] == 0u) ? cb : za.cellsB[zb.lb[cq][rw]];
+    }
+}
+
+// z and its dot/bubble contributions for one cell (x = MG output of level 0; lb, lsl, cb: the
+// cell's bubble label, its slot and cell count when it is a bubble cell)
 __device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, float zoldc, float xv, float inv,
-                              double (&dv)[2], double (&sl)[BSLOTS], const BubAgg& agg) {
+                              double (&dv)[2], double (&sl)[BSLOTS], const BubAgg& agg, uint32_t lb, int lsl,
+                              uint32_t cb) {
   const float x = xv * inv;
   float z = 0.f;
   if (mask_type(om) == AGUA) {
@@ -1123,8 +1324,7 @@ __device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, f
     dv[0] += (double)rc * z;
     dv[1] += (double)rc * zoldc;
   } else if (om & MK_BUB) {
-    const uint32_t lb = za.rot[c];
-    bub_add(sl, za.labelSlot, agg, lb, (double)x / za.cellsB[lb]);
+    bub_add_s(sl, lsl, agg, lb, (double)x / cb);
   }
   za.znew[c] = z;
 }
@@ -1142,7 +1342,11 @@ __device__ __forceinline__ void k_zdots0(Blk B_, const float* __restrict__ x0, Z
   if (agg.t) __syncthreads();
   const int c = B_.x * blockDim.x + threadIdx.x;
   const float inv = za.scaled ? 1.f / *za.mgs : 1.f;
-  if (c < C) z_cell(za, c, za.opMask[c], za.r[c], za.zold[c], x0[c], inv, dv, sl, agg);
+  if (c < C) {
+    const uint8_t om = za.opMask[c];
+    const uint32_t lb = (om & MK_BUB) ? za.rot[c] : 0u;
+    z_cell(za, c, om, za.r[c], za.zold[c], x0[c], inv, dv, sl, agg, lb, za.labelSlot[lb], za.cellsB[lb]);
+  }
   bagg_flush(agg);
   store_partials<2>(B_, dv, 2, za.partials, sh);
   if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, za.bpart, sh);
@@ -1159,8 +1363,10 @@ __device__ __forceinline__ void k_prolong_pass(Blk B_, const SX* __restrict__ sr
                                                int nx, int ny, int ntx, int nty, const SC* __restrict__ cx,
                                                const uint8_t* __restrict__ cmk, int cnx, int cny,
                                                const DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ float Tt[4][TCELLS];
+  // the stop flag is read beside the first loads (one memory latency for both); a stopped solve
+  // returns once they are issued
+  const int sc_done = sc->done;
+  __shared__ float Tt[4][TLDS];
   __shared__ float CPt[4][100];
   const int lane = threadIdx.x & 63;
   int tx, ty;
@@ -1197,6 +1403,7 @@ __device__ __forceinline__ void k_prolong_pass(Blk B_, const SX* __restrict__ sr
     sv[q] = ldf_if<SX>(src, c, in);
     sm[q] = ld_if<uint8_t>(mask, c, in, 0);
   }
+  if (sc_done) return;
 #pragma unroll
   for (int q = 0; q < 2; q++) {
     const int e = lane + 64 * q;
@@ -1222,7 +1429,7 @@ __device__ __forceinline__ void k_prolong_pass(Blk B_, const SX* __restrict__ sr
         e2 += CP[I0 + 1 + (J0 + 1) * 10] * wx1 * wy1;
         val += e2;
       }
-      T[e] = val;
+      T[lx + ly * TP] = val;
     }
   }
   wave_sync();
@@ -1237,8 +1444,10 @@ template <class SB>
 __device__ __forceinline__ void k_pass_z(Blk B_, const float* __restrict__ src, const SB* __restrict__ b,
                                          const uint8_t* __restrict__ mask, int nx, int ny, int ntx, int nty,
                                          ZArgs za, const DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ float Tt[4][TCELLS];
+  // the stop flag is read beside the first loads (one memory latency for both); a stopped solve
+  // returns once they are issued
+  const int sc_done = sc->done;
+  __shared__ float Tt[4][TLDS];
   __shared__ double sh[BSLOTS * 16];
   __shared__ BubLds aggt;
   const int lane = threadIdx.x & 63;
@@ -1249,30 +1458,45 @@ __device__ __forceinline__ void k_pass_z(Blk B_, const float* __restrict__ src, 
   block_tile(B_, &tx, &ty);
   double dv[2] = {0, 0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (tx < ntx && ty < nty) {
-    const int ox = tx * 16, oy = ty * 16;
-    float* T = Tt[threadIdx.x >> 6];
-    Own4 own;
-    prefetch_cells<SB>(own, b, mask, ox, oy, nx, ny, lane);
-    uint8_t om[2][2];
-    float rv[2][2], zo[2][2];
+  // every wave loads (a wave past the grid's tiles reads in-bounds cells it never uses), so the
+  // stop test below is uniform over the block
+  const bool valid = tx < ntx && ty < nty;
+  const int ox = tx * 16, oy = ty * 16;
+  float* T = Tt[threadIdx.x >> 6];
+  Own4 own;
+  prefetch_cells<SB>(own, b, mask, ox, oy, nx, ny, lane);
+  const float inv = za.scaled ? 1.f / *za.mgs : 1.f;
+  uint8_t om[2][2];
+  float rv[2][2], zo[2][2];
+  size_t cz[2][2];
 #pragma unroll
-    for (int cq = 0; cq < 2; cq++)
+  for (int cq = 0; cq < 2; cq++)
 #pragma unroll
-      for (int rw = 0; rw < 2; rw++) {
-        const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
-        const bool in = i < nx && j < ny;
-        const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
-        om[cq][rw] = ld_if<uint8_t>(za.opMask, c, in, 0);
-        rv[cq][rw] = za.r[c];
-        zo[cq][rw] = za.zold[c];
-      }
-    fill_tile<float>(T, src, ox, oy, nx, ny, lane);
+    for (int rw = 0; rw < 2; rw++) {
+      const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
+      const bool in = i < nx && j < ny;
+      const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
+      cz[cq][rw] = c;
+      om[cq][rw] = ld_if<uint8_t>(za.opMask, c, in, 0);
+      rv[cq][rw] = za.r[c];
+      zo[cq][rw] = za.zold[c];
+    }
+  ZSlots Z;
+  slots_load(Z.S, za.activeList);
+  fill_tile<float>(T, src, ox, oy, nx, ny, lane);
+  if (sc_done) return;
+  if (valid) {
+    // the bubble cells' labels and the slotted bubbles' cell counts load while the tile is
+    // smoothed
+    Z.S.n = nslots(sc->active_bubbles);
+    ZBubCells zb;
+    zbub_labels(zb, za, om, cz);
+    zslots_cells(Z, za);
     wave_sync();
     Cells4 cc;
     init_cells(cc, own, T, lane);
     tile_sweeps<1>(T, cc, lane);
-    const float inv = za.scaled ? 1.f / *za.mgs : 1.f;
+    zbub_vals(zb, za, Z);
 #pragma unroll
     for (int cq = 0; cq < 2; cq++)
 #pragma unroll
@@ -1280,7 +1504,8 @@ __device__ __forceinline__ void k_pass_z(Blk B_, const float* __restrict__ src, 
         const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
         if (i < nx && j < ny)
           z_cell(za, (size_t)i + (size_t)j * nx, om[cq][rw], rv[cq][rw], zo[cq][rw],
-                 (cc.m[cq][rw] & 32) ? cc.v[cq][rw] : 0.f, inv, dv, sl, agg);
+                 (cc.m[cq][rw] & 32) ? cc.v[cq][rw] : 0.f, inv, dv, sl, agg, zb.lb[cq][rw], zb.sl[cq][rw],
+                 zb.cb[cq][rw]);
       }
   }
   bagg_flush(agg);
@@ -1307,8 +1532,10 @@ __device__ __forceinline__ void k_pass(Blk B_, const SX* __restrict__ src, SX* _
                                        int ny, int desl, int ntx, int nty, int vgx, int nvb,
                                        const double* __restrict__ partRR, int nbu, int par, int fold,
                                        DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ float Tt[4][TCELLS];
+  // the stop flag is read beside the first loads (one memory latency for both); a stopped solve
+  // returns once they are issued
+  const int sc_done = sc->done;
+  __shared__ float Tt[4][TLDS];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   float* T = Tt[w];
   // one-dimensional grid over the 2x2-tile groups (vgx per row, nvb in all); a folding launch
@@ -1329,6 +1556,7 @@ __device__ __forceinline__ void k_pass(Blk B_, const SX* __restrict__ src, SX* _
     }
   };
   load(vb);
+  if (sc_done) return;
   if (fold) {
     __shared__ double fsh[16], fres[1];
     if (fold_update(B_, partRR, nbu, par, fsh, fres, sc)) return;
@@ -1358,8 +1586,10 @@ __device__ __forceinline__ void k_restrict_a(Blk B_, const SXF* __restrict__ fx,
                                              SCB* __restrict__ cb, SCX* __restrict__ cxo,
                                              const uint8_t* __restrict__ cmk, int cnx, int cny,
                                              const DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ float X[36 * 36], R[34 * 34], Bt[256], T[TCELLS];
+  // the stop flag is read beside the first loads (one memory latency for both); a stopped solve
+  // returns once they are issued
+  const int sc_done = sc->done;
+  __shared__ float X[36 * 36], R[34 * 34], Bt[256], T[TLDS];
   const int tid = threadIdx.x;
   int bxl, byl;
   xcd_block2(B_, &bxl, &byl);
@@ -1398,6 +1628,7 @@ __device__ __forceinline__ void k_restrict_a(Blk B_, const SXF* __restrict__ fx,
     const int i = fx0 + lx, j = fy0 + ly;
     xv[q] = ldf_if<SXF>(fx, (size_t)i + (size_t)j * fnx, e < 36 * 36 && i >= 0 && j >= 0 && i < fnx && j < fny);
   }
+  if (sc_done) return;
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     const int e = tid + 256 * q;
@@ -1483,8 +1714,10 @@ __device__ __forceinline__ void k_bra(Blk B_, const SXF* __restrict__ fx, SXF* _
                                       const uint8_t* __restrict__ cmk, int cnx, int cny,
                                       const double* __restrict__ partRR, int nbu, int par, int fold,
                                       DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ float Tt[BRA_WAVES][TCELLS];
+  // the stop flag is read beside the first loads (one memory latency for both); a stopped solve
+  // returns once they are issued
+  const int sc_done = sc->done;
+  __shared__ float Tt[BRA_WAVES][TLDS];
   __shared__ float XB[XBW * XBW];
   __shared__ float R[34 * 34];
   __shared__ float Bt[256];
@@ -1524,6 +1757,7 @@ __device__ __forceinline__ void k_bra(Blk B_, const SXF* __restrict__ fx, SXF* _
       cma[cq][rw] = ld_if<uint8_t>(cmk, (size_t)I + (size_t)J * cnx, PASSA && wave == 0 && I < cnx && J < cny, 0);
     }
   fill_tile<SXF>(T, fx, ox, oy, fnx, fny, lane);
+  if (sc_done) return;
   if (fold) {  // level 0 of an iteration: the norm and stop test (fold_update) while the loads land
     __shared__ double fsh[16], fres[1];
     if (fold_update(B_, partRR, nbu, par, fsh, fres, sc)) return;
@@ -1624,8 +1858,10 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
                                       const SB* __restrict__ b, const uint8_t* __restrict__ mask, int nx,
                                       int ny, const SC* __restrict__ cx, const uint8_t* __restrict__ cmk,
                                       int cnx, int cny, ZArgs za, const DevScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ float Tt[4][TCELLS];
+  // the stop flag is read beside the first loads (one memory latency for both); a stopped solve
+  // returns once they are issued
+  const int sc_done = sc->done;
+  __shared__ float Tt[4][TLDS];
   __shared__ float CPt[4][100];
   __shared__ float W[CW * CW];
   __shared__ double sh[BSLOTS * 16];
@@ -1647,6 +1883,10 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
   if (wave == 0) prefetch_cells<SB>(ownD, b, mask, ax, ay, nx, ny, lane);
   uint8_t om[2][2] = {{0, 0}, {0, 0}};
   float rv[2][2] = {{0, 0}, {0, 0}}, zo[2][2] = {{0, 0}, {0, 0}};
+  size_t cz[2][2] = {{0, 0}, {0, 0}};
+  const float inv = (ZD && za.scaled) ? 1.f / *za.mgs : 1.f;
+  ZSlots Z;
+  if (ZD) slots_load(Z.S, za.activeList);
   if (ZD && wave == 0) {
 #pragma unroll
     for (int cq = 0; cq < 2; cq++)
@@ -1655,6 +1895,7 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
         const int i = ax + cell_lx(lane, cq), j = ay + cell_ly(lane, rw);
         const bool in = i < nx && j < ny;
         const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
+        cz[cq][rw] = c;
         om[cq][rw] = ld_if<uint8_t>(za.opMask, c, in, 0);
         rv[cq][rw] = za.r[c];
         zo[cq][rw] = za.zold[c];
@@ -1684,10 +1925,19 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
     sv[q] = ldf_if<SX>(xb, c, in);
     sm[q] = ld_if<uint8_t>(mask, c, in, 0);
   }
+  if (sc_done) return;
 #pragma unroll
   for (int q = 0; q < 2; q++) {
     const int e = lane + 64 * q;
     if (e < 100) CP[e] = mask_type(cm[q]) == AGUA ? cv[q] : 0.f;
+  }
+  // z's bubble cells: labels and the slotted bubbles' cell counts now (the masks and the slotted
+  // labels have arrived), slots after pass C
+  ZBubCells zb;
+  if (ZD && wave == 0) {
+    Z.S.n = nslots(sc->active_bubbles);
+    zbub_labels(zb, za, om, cz);
+    zslots_cells(Z, za);
   }
   wave_sync();
   // ---- snapshot x_B + P e, pass C on this wave's tile, into the window
@@ -1710,7 +1960,7 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
         e2 += CP[I0 + 1 + (J0 + 1) * 10] * wx1 * wy1;
         val += e2;
       }
-      T[e] = val;
+      T[lx + ly * TP] = val;
     }
   }
   wave_sync();
@@ -1727,6 +1977,7 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
         W[(i - wx0) + (j - wy0) * CW] = (in && (cc.m[cq][rw] & 32)) ? rnd<SX>(cc.v[cq][rw]) : 0.f;
       }
   }
+  if (ZD && wave == 0) zbub_vals(zb, za, Z);
   __syncthreads();
   double dv[2] = {0, 0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1737,7 +1988,7 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
       const int e = lane + 64 * q;
       if (e < TCELLS) {
         const int ly = e / TS, lx = e - ly * TS;
-        T[e] = W[(lx + 7) + (ly + 7) * CW];
+        T[lx + ly * TP] = W[(lx + 7) + (ly + 7) * CW];
       }
     }
     wave_sync();
@@ -1747,7 +1998,6 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
     if (!ZD) {
       store_cells<SX>(cc, xout, ax, ay, nx, ny, lane);
     } else {
-      const float inv = za.scaled ? 1.f / *za.mgs : 1.f;
 #pragma unroll
       for (int cq = 0; cq < 2; cq++)
 #pragma unroll
@@ -1755,7 +2005,8 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
           const int i = ax + cell_lx(lane, cq), j = ay + cell_ly(lane, rw);
           if (i < nx && j < ny)
             z_cell(za, (size_t)i + (size_t)j * nx, om[cq][rw], rv[cq][rw], zo[cq][rw],
-                   (cc.m[cq][rw] & 32) ? cc.v[cq][rw] : 0.f, inv, dv, sl, agg);
+                   (cc.m[cq][rw] & 32) ? cc.v[cq][rw] : 0.f, inv, dv, sl, agg, zb.lb[cq][rw], zb.sl[cq][rw],
+                   zb.cb[cq][rw]);
         }
     }
   }
@@ -1840,7 +2091,7 @@ __device__ __forceinline__ void wg_pass(const float* S, float* W, const float* B
       if (e < TCELLS) {
         const int ly = e / TS, lx = e - ly * TS;
         const int i = ox - 1 + lx, j = oy - 1 + ly;
-        T[e] = (i >= 0 && j >= 0 && i < nx && j < ny) ? S[i + j * nx] : 0.f;
+        T[lx + ly * TP] = (i >= 0 && j >= 0 && i < nx && j < ny) ? S[i + j * nx] : 0.f;
       }
     }
     wave_sync();
@@ -2060,7 +2311,7 @@ __device__ __forceinline__ void k_coarse_wg(Blk B_, const WGPlan& P, const float
                                             float* out_lc, const DevScalars* __restrict__ sc) {
   if (sc->done) return;
   __shared__ __attribute__((aligned(16))) char pool[WG_POOL];
-  __shared__ float tiles[WG_WAVES][TCELLS];
+  __shared__ float tiles[WG_WAVES][TLDS];
   wg_load_masks(P, pool);
   const bool xin = P.lc < P.L - 1;
   const int n = P.nx[P.lc] * P.ny[P.lc];
@@ -2096,7 +2347,7 @@ __device__ __forceinline__ void k_vcycle_op(Blk B_, const WGPlan& P, int ld, flo
   __syncthreads();
   if (threadIdx.x == 0) ((float*)(vpool + P.offb[lc]))[j] = 1.f;
   __syncthreads();
-  float* T = (float*)(vpool + P.pool) + (threadIdx.x >> 6) * TCELLS;
+  float* T = (float*)(vpool + P.pool) + (threadIdx.x >> 6) * TLDS;
   const float* X = wg_vcycle<VOP_WAVES>(P, vpool, T, false);
   for (int c = threadIdx.x; c < ld; c += blockDim.x) row[c] = c < n ? X[c] : 0.f;
 }
@@ -2110,7 +2361,9 @@ constexpr int VOP_ROWS = 2;
 __device__ __forceinline__ void k_vcycle_apply(Blk B_, const float* __restrict__ C, int ld, int n,
                                                const float* __restrict__ b, float* __restrict__ x,
                                                const DevScalars* __restrict__ sc) {
-  if (sc->done) return;
+  // the stop flag is read beside the first loads (one memory latency for both); a stopped solve
+  // returns once they are issued
+  const int sc_done = sc->done;
   const int lane = threadIdx.x & 63;
   const int r0 = (B_.x * 4 + (threadIdx.x >> 6)) * VOP_ROWS;
   if (r0 >= n) return;
@@ -2132,6 +2385,7 @@ __device__ __forceinline__ void k_vcycle_apply(Blk B_, const float* __restrict__
     bv[k].z = ld_if<float>(b, c + 2, c + 2 < n, 0.f);
     bv[k].w = ld_if<float>(b, c + 3, c + 3 < n, 0.f);
   }
+  if (sc_done) return;
 #pragma unroll
   for (int r = 0; r < VOP_ROWS; r++) {
     float acc = 0.f;
@@ -2164,10 +2418,15 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
                                        const double* __restrict__ bpartZ, int nbz, int par, int first,
                                        double* __restrict__ partA, double* __restrict__ bpartA,
                                        DevScalars* __restrict__ sc) {
-  if (sc->done) return;
+  // the stop flag is read beside the first loads (one memory latency for both); a stopped solve
+  // returns once they are issued
+  const int sc_done = sc->done;
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], zs[BSLOTS];
   __shared__ float P[SPMV_HN];
   __shared__ uint8_t M[SPMV_HN];
+  __shared__ uint32_t HL[SPMV_HN];  // labels of the halo tile's bubble cells
+  __shared__ int8_t HS[SPMV_HN];    // and their reduction slots
+  __shared__ float pslt[BSLOTS];    // new p of the slotted bubbles
   __shared__ BubLds aggt;
   const int tid = threadIdx.x;
   const int ntx = (nx + SPMV_TW - 1) / SPMV_TW, nch = ntx * ((ny + SPMV_TH - 1) / SPMV_TH);
@@ -2195,21 +2454,43 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
       pv[q] = ld_if<float>(pold, (size_t)c, in, 0.f);
     }
   };
+  // the halo tile's bubble cells: labels (one load in flight per thread once the masks have
+  // arrived), slots by comparison with the slotted labels
+  uint32_t hlb[SPMV_LOADS];
+  int hsl[SPMV_LOADS];
+  Slots S;
+  auto load_halo_labels = [&]() {
+#pragma unroll
+    for (int q = 0; q < SPMV_LOADS; q++)
+      hlb[q] = ld_if<uint32_t>(rot, (size_t)gidx[q], gidx[q] >= 0 && (mk[q] & MK_BUB), 0u);
+  };
+  auto halo_slots = [&]() {
+#pragma unroll
+    for (int q = 0; q < SPMV_LOADS; q++) hsl[q] = slot_of(S, hlb[q]);
+  };
   load(ch);
-  const double beta = fold_zdots(B_, partZ, bpartZ, nbz, par, first, B, sh, res, zs, sc);
-  const int ns = nslots(sc->active_bubbles);
-  const double* Po = bv(B.bvec, BV_P0 + 1 - par);
-  const double* accZ = bv(B.bvec, BV_ACCZ0 + par);
+  slots_load(S, B.activeList);
+  ZFold zf;
+  zf_load(zf, partZ, bpartZ, nbz, B, sc->active_bubbles);
+  if (sc_done) return;
+  S.n = nslots(zf.na);
+  load_halo_labels();
+  zf_load_bub(zf, B, par);
+  zf_reduce(zf, sh, res, zs);
+  halo_slots();
+  const double beta = zf_finish(zf, B_, par, first, B, sh, res, zs, pslt, sc);
+  const int ns = nslots(zf.na);
   // (the loop's first barrier orders the table's clearing before the adds)
-  const BubAgg agg = bagg_begin(aggt, bv(B.bvec, BV_ACCA0 + par), sc->active_bubbles > BSLOTS);
-  // p of a cell from its loaded z, p_old and mask (bubble cells from the bubble vectors)
-  auto peff = [&](int d, uint8_t md, float zd, float pd) -> float {
-    if (mask_type(md) == AGUA) return (float)((double)zd + beta * (double)pd);
+  const BubAgg agg = bagg_begin(aggt, bv(B.bvec, BV_ACCA0 + par), zf.na > BSLOTS);
+  // p of a halo cell from its loaded z, p_old and mask (bubble cells: the bubble's new z and
+  // old p, the slotted ones from pslt[])
+  auto peff = [&](int q) -> float {
+    const uint8_t md = mk[q];
+    if (mask_type(md) == AGUA) return (float)((double)zv[q] + beta * (double)pv[q]);
     if (md & MK_BUB) {
-      const uint32_t lb = rot[d];
-      const int sl = B.labelSlot[lb];
-      const double zn = sl >= 0 ? zs[sl] : fx_read(accZ, lb);
-      return (float)(zn + beta * Po[lb]);
+      if (hsl[q] >= 0) return pslt[hsl[q]];
+      const uint32_t lb = hlb[q];
+      return (float)(fx_read(bv(B.bvec, BV_ACCZ0 + par), lb) + beta * bv(B.bvec, BV_P0 + 1 - par)[lb]);
     }
     return 0.f;
   };
@@ -2222,8 +2503,10 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
     for (int q = 0; q < SPMV_LOADS; q++) {
       const int e = tid + 256 * q;
       if (e < SPMV_HN) {
-        P[e] = gidx[q] >= 0 ? peff(gidx[q], mk[q], zv[q], pv[q]) : 0.f;
+        P[e] = gidx[q] >= 0 ? peff(q) : 0.f;
         M[e] = mk[q];
+        HL[e] = hlb[q];
+        HS[e] = (int8_t)hsl[q];
       }
     }
     __syncthreads();
@@ -2239,13 +2522,12 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
         pnew[c] = pc;
         if (mask_type(m) == AGUA) {
           const int ne[4] = {e - 1, e + 1, e - SPMV_HW, e + SPMV_HW};
-          const int nc[4] = {c - 1, c + 1, c - nx, c + nx};
           double s = 0;
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             if (!(m & (1 << k))) continue;
             s += (double)P[ne[k]];
-            if (M[ne[k]] & MK_BUBIN) bub_add(sl, B.labelSlot, agg, rot[nc[k]], -(double)pc);
+            if (M[ne[k]] & MK_BUBIN) bub_add_s(sl, HS[ne[k]], agg, HL[ne[k]], -(double)pc);
           }
           const float a = (float)(mask_cnt(m) * (double)pc - s);
           ap[c] = a;
@@ -2256,6 +2538,8 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
     ch += B_.gx;
     if (ch >= nch) break;
     load(ch);
+    load_halo_labels();
+    halo_slots();
   }
   bagg_flush(agg);
   store_partials<1>(B_, v, 1, partA, sh);
@@ -2442,7 +2726,7 @@ int solver_alloc(tq_sim* s) {
   if (lc > 0 && lc < L - 1 && s->lv[lc].n <= VOP_MAX) {
     long pool = 0;
     for (int l = lc; l < L; l++) pool += wg_bytes(s->lv[l].n);
-    if (pool + VOP_WAVES * TCELLS * 4 <= 65536) {
+    if (pool + VOP_WAVES * TLDS * 4 <= 65536) {
       s->vop_ld = (s->lv[lc].n + 255) & ~255;
       TQ_HIP(hipMalloc(&s->vop, sizeof(float) * (size_t)s->vop_ld * s->lv[lc].n));
     }
@@ -2741,7 +3025,7 @@ static int launch_pcd(tq_sim* s, int l) {
 static ZArgs zargs(tq_sim* s) {
   ZArgs za;
   za.opMask = s->opMask; za.rot = s->rot; za.cellsB = s->cellsB;
-  za.labelSlot = s->labelSlot; za.r = s->r; za.zold = s->zbuf[s->zc]; za.znew = s->zbuf[1 - s->zc];
+  za.labelSlot = s->labelSlot; za.activeList = s->activeList; za.r = s->r; za.zold = s->zbuf[s->zc]; za.znew = s->zbuf[1 - s->zc];
   za.accZ = bv(s->bvec, BV_ACCZ0 + z_par(s));
   za.partials = s->partials + ps_off(PS_Z);
   za.bpart = bpart_z(s);
@@ -2808,7 +3092,7 @@ static void launch_coarse_wg(tq_sim* s) {
 static int launch_vcycle_op(tq_sim* s) {
   const WGPlan P = wg_plan(s);
   const Level& G = s->lv[s->lc];
-  const size_t lds = P.pool + VOP_WAVES * TCELLS * sizeof(float);
+  const size_t lds = P.pool + VOP_WAVES * TLDS * sizeof(float);
   if (s->st2 && !s->timing && !s->capturing) {
     TQ_HIP(hipEventRecord(s->ev_masks, s->st));
     TQ_HIP(hipStreamWaitEvent(s->st2, s->ev_masks, 0));

@@ -6,25 +6,74 @@
 namespace tq {
 
 // ------------------------------------------------------------------ wavefront (64 lanes) reductions
+// Data-parallel-primitive (DPP) moves: a VALU operand taken from another lane of the same row of
+// 16 (row_shr:k) or broadcast from lane 15 of the previous row (row_bcast:15, rows 1 and 3) and
+// from lane 31 (row_bcast:31, rows 2 and 3).  A lane without a source reads `fill`.  An inclusive
+// scan over the rows and then across them leaves the wave total in lane 63 after six steps of one
+// or two VALU moves and an add; the order of the additions is fixed, so sums are deterministic.
+// (A butterfly over ds_bpermute would cost six LDS-latency round trips per value.)
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118;
+constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i(int v, int fill) {
+  return __builtin_amdgcn_update_dpp(fill, v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f(float v, float fill) {
+  return __int_as_float(dpp_i<CTRL, ROWS>(__float_as_int(v), __float_as_int(fill)));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_d(double v, double fill) {
+  const int lo = dpp_i<CTRL, ROWS>(__double2loint(v), __double2loint(fill));
+  const int hi = dpp_i<CTRL, ROWS>(__double2hiint(v), __double2hiint(fill));
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane63_d(double v) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane63_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+// the total of the wave's 64 lanes, in every lane (every lane must be active)
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_d<DPP_ROW_SHR1, 0xF>(v, 0.0);
+  v += dpp_d<DPP_ROW_SHR2, 0xF>(v, 0.0);
+  v += dpp_d<DPP_ROW_SHR4, 0xF>(v, 0.0);
+  v += dpp_d<DPP_ROW_SHR8, 0xF>(v, 0.0);
+  v += dpp_d<DPP_ROW_BCAST15, 0xA>(v, 0.0);
+  v += dpp_d<DPP_ROW_BCAST31, 0xC>(v, 0.0);
+  return lane63_d(v);
 }
 __device__ inline float wave_sumf(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_f<DPP_ROW_SHR1, 0xF>(v, 0.f);
+  v += dpp_f<DPP_ROW_SHR2, 0xF>(v, 0.f);
+  v += dpp_f<DPP_ROW_SHR4, 0xF>(v, 0.f);
+  v += dpp_f<DPP_ROW_SHR8, 0xF>(v, 0.f);
+  v += dpp_f<DPP_ROW_BCAST15, 0xA>(v, 0.f);
+  v += dpp_f<DPP_ROW_BCAST31, 0xC>(v, 0.f);
+  return lane63_f(v);
 }
 __device__ inline float wave_maxf(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  return v;
+  const float lo = -__builtin_inff();
+  v = fmaxf(v, dpp_f<DPP_ROW_SHR1, 0xF>(v, lo));
+  v = fmaxf(v, dpp_f<DPP_ROW_SHR2, 0xF>(v, lo));
+  v = fmaxf(v, dpp_f<DPP_ROW_SHR4, 0xF>(v, lo));
+  v = fmaxf(v, dpp_f<DPP_ROW_SHR8, 0xF>(v, lo));
+  v = fmaxf(v, dpp_f<DPP_ROW_BCAST15, 0xA>(v, lo));
+  v = fmaxf(v, dpp_f<DPP_ROW_BCAST31, 0xC>(v, lo));
+  return lane63_f(v);
 }
 __device__ inline double wave_maxd(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
+  const double lo = -__builtin_inf();
+  v = fmax(v, dpp_d<DPP_ROW_SHR1, 0xF>(v, lo));
+  v = fmax(v, dpp_d<DPP_ROW_SHR2, 0xF>(v, lo));
+  v = fmax(v, dpp_d<DPP_ROW_SHR4, 0xF>(v, lo));
+  v = fmax(v, dpp_d<DPP_ROW_SHR8, 0xF>(v, lo));
+  v = fmax(v, dpp_d<DPP_ROW_BCAST15, 0xA>(v, lo));
+  v = fmax(v, dpp_d<DPP_ROW_BCAST31, 0xC>(v, lo));
+  return lane63_d(v);
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin by linear id, and each XCD has its own L2.
