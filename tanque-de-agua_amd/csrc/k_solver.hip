@@ -218,25 +218,6 @@ __device__ inline void bub_add_s(double (&sl)[BSLOTS], int s, const BubAgg& agg,
   else bagg_add(agg, lab, val);
 }
 
-// The slotted bubbles of a solve: the set-up numbers the active labels in label order and gives
-// the first BSLOTS of them the slots 0..BSLOTS-1 (activeList[k] has slot k), so a label's slot is
-// its position among activeList[0 .. nslots(na)).  A kernel reads those labels beside its first
-// loads (their addresses depend on nothing) and finds a cell's slot by comparison, instead of a
-// load of labelSlot[label] that could only be issued after the label itself had arrived.
-struct Slots {
-  int lab[BSLOTS];
-  int n;
-};
-__device__ inline void slots_load(Slots& S, const int* __restrict__ activeList) {
-#pragma unroll
-  for (int k = 0; k < BSLOTS; k++) S.lab[k] = activeList[k];
-}
-__device__ inline int slot_of(const Slots& S, uint32_t lab) {
-  int s = -1;
-#pragma unroll
-  for (int k = BSLOTS - 1; k >= 0; k--) s = (k < S.n && S.lab[k] == (int)lab) ? k : s;
-  return s;
-}
 
 // Block sum of the first nv of v[]; thread 0 stores partial `bid` of each quantity.  nv must
 // be uniform over the block; `sh` holds NMAX*16 doubles.
@@ -274,7 +255,7 @@ __device__ inline void part_load(const double* __restrict__ part, int nb, int nv
     double x0[NMAX], x1[NMAX];
 #pragma unroll
     for (int k = 0; k < NMAX; k++) {
-      const size_t row = (size_t)k * MAX_RED_BLOCKS;
+      const size_t row = (size_t)(k < nv ? k : 0) * MAX_RED_BLOCKS;
       x0[k] = part[row + (in0 ? b : 0)];
       x1[k] = part[row + (in1 ? b + bd : 0)];
     }
@@ -1170,27 +1151,35 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
         }
       }
   };
-  // bubble cells: their labels (loaded once the masks have arrived) and slots (by comparison
-  // with the slotted labels read beside the first loads); the first cycle also reads their
-  // residuals and cell counts
+  // bubble cells: their labels (loaded once the masks have arrived, only in waves that hold
+  // one); the first cycle and bubbles without a slot also read their residuals and cell counts
   uint32_t lbv[2][2];
-  auto load_bub_labels = [&]() {
+  auto load_bub_labels = [&]() {  // only waves that hold a bubble cell wait for the labels
+    bool hb = false;
 #pragma unroll
     for (int cq = 0; cq < 2; cq++)
 #pragma unroll
-      for (int rw = 0; rw < 2; rw++)
-        lbv[cq][rw] = ld_if<uint32_t>(rot, ci[cq][rw], in[cq][rw] && (om[cq][rw] & MK_BUB), 0u);
+      for (int rw = 0; rw < 2; rw++) hb = hb || (in[cq][rw] && (om[cq][rw] & MK_BUB));
+    if (__ballot(hb)) {
+#pragma unroll
+      for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+        for (int rw = 0; rw < 2; rw++)
+          lbv[cq][rw] = ld_if<uint32_t>(rot, ci[cq][rw], in[cq][rw] && (om[cq][rw] & MK_BUB), 0u);
+    } else {
+#pragma unroll
+      for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+        for (int rw = 0; rw < 2; rw++) lbv[cq][rw] = 0u;
+    }
   };
-  Slots S;
   load(vb);
-  slots_load(S, B.activeList);
   // UPD: alpha and the bubbles' new residuals from spmv's partials (the alpha fold, its loads
   // beside the tile's); else (the first cycle) the current bubble residuals are those of
   // parity 1 - par
   AFold af;
   if (UPD) af_load(af, partA, bpartA, nbs, par, B, sc);
   if (sc_done) return;
-  S.n = nslots(sc->active_bubbles);
   load_bub_labels();
   if (UPD) {
     af_load_bub(af, B, cellsB, par);
@@ -1230,7 +1219,7 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
             bval = rn;
           } else if (m & MK_BUB) {
             const uint32_t lb = lbv[cq][rw];
-            const int sl = slot_of(S, lb);
+            const int sl = B.labelSlot[lb];
             if (UPD && sl >= 0) bval = bsl[sl];
             else bval = (float)((UPD ? bubble_r(B, lb, par, alpha, beta, rbs) : rb[lb]) / cellsB[lb]);
           }
@@ -1260,7 +1249,6 @@ struct ZArgs {  // level-0 outputs: z = MG result and the dots r.z, r.z_old
   const uint32_t* rot;
   const uint32_t* cellsB;
   const int* labelSlot;
-  const int* activeList;  // its first BSLOTS entries are the slotted labels
   const float* r;
   const float* zold;
   float* znew;
@@ -1271,22 +1259,9 @@ struct ZArgs {  // level-0 outputs: z = MG result and the dots r.z, r.z_old
   int scaled;
 };
 
-// The slotted bubbles (labels read beside a kernel's first loads) and their cell counts (read
-// once the labels have arrived, beside the cells' own labels)
-struct ZSlots {
-  Slots S;
-  uint32_t cb[BSLOTS];
-};
-__device__ inline void zslots_cells(ZSlots& Z, const ZArgs& za) {
-#pragma unroll
-  for (int k = 0; k < BSLOTS; k++) {
-    const uint32_t lab = (uint32_t)Z.S.lab[k];  // entries past the active count are not used
-    Z.cb[k] = za.cellsB[lab < (uint32_t)MAX_BOLHAS ? lab : 0u];
-  }
-}
 // The bubble cells among a lane's four level-0 cells: their labels (one load in flight each,
-// issued once the masks have arrived; other cells read entry 0), then slots by comparison and
-// cell counts from the slotted bubbles' (a label without a slot reads its own)
+// issued once the masks have arrived; other cells read entry 0), then their slots and cell
+// counts
 struct ZBubCells {
   uint32_t lb[2][2], cb[2][2];
   int sl[2][2];
@@ -1298,17 +1273,13 @@ __device__ inline void zbub_labels(ZBubCells& zb, const ZArgs& za, const uint8_t
     for (int rw = 0; rw < 2; rw++)
       zb.lb[cq][rw] = ld_if<uint32_t>(za.rot, c[cq][rw], (om[cq][rw] & MK_BUB) != 0, 0u);
 }
-__device__ inline void zbub_vals(ZBubCells& zb, const ZArgs& za, const ZSlots& Z) {
+__device__ inline void zbub_vals(ZBubCells& zb, const ZArgs& za) {
 #pragma unroll
   for (int cq = 0; cq < 2; cq++)
 #pragma unroll
     for (int rw = 0; rw < 2; rw++) {
-      const int sl = slot_of(Z.S, zb.lb[cq][rw]);
-      uint32_t cb = 0;
-#pragma unroll
-      for (int k = 0; k < BSLOTS; k++) cb = k == sl ? Z.cb[k] : cb;
-      zb.sl[cq][rw] = sl;
-      zb.cb[cq][rw] = (sl >= 0 || zb.lb[cq][rw] == 0u) ? cb : za.cellsB[zb.lb[cq][rw]];
+      zb.sl[cq][rw] = za.labelSlot[zb.lb[cq][rw]];
+      zb.cb[cq][rw] = za.cellsB[zb.lb[cq][rw]];
     }
 }
 
@@ -1481,22 +1452,17 @@ __device__ __forceinline__ void k_pass_z(Blk B_, const float* __restrict__ src, 
       rv[cq][rw] = za.r[c];
       zo[cq][rw] = za.zold[c];
     }
-  ZSlots Z;
-  slots_load(Z.S, za.activeList);
   fill_tile<float>(T, src, ox, oy, nx, ny, lane);
   if (sc_done) return;
   if (valid) {
-    // the bubble cells' labels and the slotted bubbles' cell counts load while the tile is
-    // smoothed
-    Z.S.n = nslots(sc->active_bubbles);
+    // the bubble cells' labels, slots and cell counts load while the tile is smoothed
     ZBubCells zb;
     zbub_labels(zb, za, om, cz);
-    zslots_cells(Z, za);
+    zbub_vals(zb, za);
     wave_sync();
     Cells4 cc;
     init_cells(cc, own, T, lane);
     tile_sweeps<1>(T, cc, lane);
-    zbub_vals(zb, za, Z);
 #pragma unroll
     for (int cq = 0; cq < 2; cq++)
 #pragma unroll
@@ -1885,8 +1851,6 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
   float rv[2][2] = {{0, 0}, {0, 0}}, zo[2][2] = {{0, 0}, {0, 0}};
   size_t cz[2][2] = {{0, 0}, {0, 0}};
   const float inv = (ZD && za.scaled) ? 1.f / *za.mgs : 1.f;
-  ZSlots Z;
-  if (ZD) slots_load(Z.S, za.activeList);
   if (ZD && wave == 0) {
 #pragma unroll
     for (int cq = 0; cq < 2; cq++)
@@ -1931,14 +1895,9 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
     const int e = lane + 64 * q;
     if (e < 100) CP[e] = mask_type(cm[q]) == AGUA ? cv[q] : 0.f;
   }
-  // z's bubble cells: labels and the slotted bubbles' cell counts now (the masks and the slotted
-  // labels have arrived), slots after pass C
+  // z's bubble cells: labels now (the masks have arrived), slots and cell counts after pass C
   ZBubCells zb;
-  if (ZD && wave == 0) {
-    Z.S.n = nslots(sc->active_bubbles);
-    zbub_labels(zb, za, om, cz);
-    zslots_cells(Z, za);
-  }
+  if (ZD && wave == 0) zbub_labels(zb, za, om, cz);
   wave_sync();
   // ---- snapshot x_B + P e, pass C on this wave's tile, into the window
 #pragma unroll
@@ -1977,7 +1936,7 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
         W[(i - wx0) + (j - wy0) * CW] = (in && (cc.m[cq][rw] & 32)) ? rnd<SX>(cc.v[cq][rw]) : 0.f;
       }
   }
-  if (ZD && wave == 0) zbub_vals(zb, za, Z);
+  if (ZD && wave == 0) zbub_vals(zb, za);
   __syncthreads();
   double dv[2] = {0, 0};
   double sl[BSLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -2424,8 +2383,6 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
   __shared__ double sh[BSLOTS * 16], res[BSLOTS], zs[BSLOTS];
   __shared__ float P[SPMV_HN];
   __shared__ uint8_t M[SPMV_HN];
-  __shared__ uint32_t HL[SPMV_HN];  // labels of the halo tile's bubble cells
-  __shared__ int8_t HS[SPMV_HN];    // and their reduction slots
   __shared__ float pslt[BSLOTS];    // new p of the slotted bubbles
   __shared__ BubLds aggt;
   const int tid = threadIdx.x;
@@ -2454,30 +2411,12 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
       pv[q] = ld_if<float>(pold, (size_t)c, in, 0.f);
     }
   };
-  // the halo tile's bubble cells: labels (one load in flight per thread once the masks have
-  // arrived), slots by comparison with the slotted labels
-  uint32_t hlb[SPMV_LOADS];
-  int hsl[SPMV_LOADS];
-  Slots S;
-  auto load_halo_labels = [&]() {
-#pragma unroll
-    for (int q = 0; q < SPMV_LOADS; q++)
-      hlb[q] = ld_if<uint32_t>(rot, (size_t)gidx[q], gidx[q] >= 0 && (mk[q] & MK_BUB), 0u);
-  };
-  auto halo_slots = [&]() {
-#pragma unroll
-    for (int q = 0; q < SPMV_LOADS; q++) hsl[q] = slot_of(S, hlb[q]);
-  };
   load(ch);
-  slots_load(S, B.activeList);
   ZFold zf;
   zf_load(zf, partZ, bpartZ, nbz, B, sc->active_bubbles);
   if (sc_done) return;
-  S.n = nslots(zf.na);
-  load_halo_labels();
   zf_load_bub(zf, B, par);
   zf_reduce(zf, sh, res, zs);
-  halo_slots();
   const double beta = zf_finish(zf, B_, par, first, B, sh, res, zs, pslt, sc);
   const int ns = nslots(zf.na);
   // (the loop's first barrier orders the table's clearing before the adds)
@@ -2488,8 +2427,9 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
     const uint8_t md = mk[q];
     if (mask_type(md) == AGUA) return (float)((double)zv[q] + beta * (double)pv[q]);
     if (md & MK_BUB) {
-      if (hsl[q] >= 0) return pslt[hsl[q]];
-      const uint32_t lb = hlb[q];
+      const uint32_t lb = rot[gidx[q]];
+      const int sl = B.labelSlot[lb];
+      if (sl >= 0) return pslt[sl];
       return (float)(fx_read(bv(B.bvec, BV_ACCZ0 + par), lb) + beta * bv(B.bvec, BV_P0 + 1 - par)[lb]);
     }
     return 0.f;
@@ -2505,8 +2445,6 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
       if (e < SPMV_HN) {
         P[e] = gidx[q] >= 0 ? peff(q) : 0.f;
         M[e] = mk[q];
-        HL[e] = hlb[q];
-        HS[e] = (int8_t)hsl[q];
       }
     }
     __syncthreads();
@@ -2522,12 +2460,16 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
         pnew[c] = pc;
         if (mask_type(m) == AGUA) {
           const int ne[4] = {e - 1, e + 1, e - SPMV_HW, e + SPMV_HW};
+          const int nc[4] = {c - 1, c + 1, c - nx, c + nx};
           double s = 0;
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             if (!(m & (1 << k))) continue;
             s += (double)P[ne[k]];
-            if (M[ne[k]] & MK_BUBIN) bub_add_s(sl, HS[ne[k]], agg, HL[ne[k]], -(double)pc);
+            if (M[ne[k]] & MK_BUBIN) {
+              const uint32_t lb = rot[nc[k]];
+              bub_add(sl, B.labelSlot, agg, lb, -(double)pc);
+            }
           }
           const float a = (float)(mask_cnt(m) * (double)pc - s);
           ap[c] = a;
@@ -2538,8 +2480,6 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
     ch += B_.gx;
     if (ch >= nch) break;
     load(ch);
-    load_halo_labels();
-    halo_slots();
   }
   bagg_flush(agg);
   store_partials<1>(B_, v, 1, partA, sh);
@@ -3025,7 +2965,7 @@ static int launch_pcd(tq_sim* s, int l) {
 static ZArgs zargs(tq_sim* s) {
   ZArgs za;
   za.opMask = s->opMask; za.rot = s->rot; za.cellsB = s->cellsB;
-  za.labelSlot = s->labelSlot; za.activeList = s->activeList; za.r = s->r; za.zold = s->zbuf[s->zc]; za.znew = s->zbuf[1 - s->zc];
+  za.labelSlot = s->labelSlot; za.r = s->r; za.zold = s->zbuf[s->zc]; za.znew = s->zbuf[1 - s->zc];
   za.accZ = bv(s->bvec, BV_ACCZ0 + z_par(s));
   za.partials = s->partials + ps_off(PS_Z);
   za.bpart = bpart_z(s);
