@@ -565,7 +565,7 @@ def solver_kernel_map(nx, ny):
     t16 = lambda n: (n + 15) // 16
     groups = lambda a, b, d: ((a + d + 15) // 16 + 1) // 2 * (((b + d + 15) // 16 + 1) // 2)
     m = {"pcg_spmv": ("tq::k_spmv", min(((nx + 63) // 64) * ((ny + 15) // 16), 1024) * 256),
-         "pcg_update_mg_in_l0": ("tq::k_in_a0<float, true, true>", min(groups(nx, ny, 0), 1280) * 256),
+         "pcg_update_mg_in_l0": ("tq::k_in_a0<float, true, true>", min(groups(nx, ny, 0), 1024) * 256),
          "mg_pass_z_l0": ("tq::k_pass_z<float>", groups(nx, ny, 0) * 256),
          "mg_prolong_c_l0": ("tq::k_prolong_pass<float, float, float>", groups(nx, ny, 8) * 256)}
     for l in range(len(dims) - 1):
@@ -689,6 +689,31 @@ def summarize_batch_trace(trace_dir, frames, commands):
     return "\n".join(lines) + "\n"
 
 
+def summarize_trace(trace_dir, title, commands, note, top=24):
+    """Markdown table of every kernel of a rocprofv3 --kernel-trace run (by body for the library's
+    launch wrappers): calls, total and average device time, share; the adjoint's reverse kernels
+    (bodies k_adj_* and k_ten_*) listed in full below it."""
+    rows = _csv_rows(_find(trace_dir, "kernel_trace"))
+    per = {}
+    for r in rows:
+        per.setdefault(_kname(r), []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    total = sum(sum(v) for v in per.values())
+    order = sorted(per.items(), key=lambda kv: -sum(kv[1]))
+
+    def table(items):
+        out = ["| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
+        for k, v in items:
+            out.append("| `%s` | %d | %.2f | %.2f | %.1f |" % (k, len(v), sum(v) / 1e3, sum(v) / len(v),
+                                                             100 * sum(v) / total))
+        return out
+    lines = ["# " + title, "", "Command (from the repository root, on one MI355X):", "", "```"] + commands + \
+        ["```", "", note, "", "Device time of all kernels: %.1f ms." % (total / 1e3), ""] + table(order[:top])
+    rev = [(k, v) for k, v in order if k.startswith(("tq::k_adj_", "tq::k_ten_"))]
+    if rev:
+        lines += ["", "The reverse kernels (all of them):", ""] + table(rev)
+    return "\n".join(lines) + "\n"
+
+
 def profile_markdown(summary, title, top=30):
     """The kernel table of a summary as markdown (profiles/*.md)."""
     lines = ["# " + title, "", "Commands (from the repository root, on one MI355X):", "", "```"]
@@ -722,6 +747,10 @@ def main():
     ap.add_argument("--batched-frames", type=int, default=0, help="frames of the batched runs (0: all 500)")
     ap.add_argument("--batched-table-only", action="store_true",
                     help="only the table-batched configuration for --batched-frames frames (default 20; for profiling)")
+    ap.add_argument("--summarize-trace", nargs=3, metavar=("TRACE_DIR", "OUT_MD", "TITLE"),
+                    help="write the kernel table of a rocprofv3 --kernel-trace run (with --profile-commands "
+                         "and --profile-note)")
+    ap.add_argument("--profile-note", default="", help="a paragraph for --summarize-trace")
     ap.add_argument("--summarize-batch-trace", nargs=2, metavar=("TRACE_DIR", "OUT_MD"),
                     help="write the kernel table of a --batched-table-only rocprofv3 --kernel-trace run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (funnel) runs")
@@ -771,6 +800,12 @@ def main():
         with open(om, "w") as f:
             f.write(profile_markdown(sm, "Kernel time and HBM traffic, %s" % name))
         print(json.dumps({"summary": oj, "kernels": sm["kernels"]}), flush=True)
+        return
+    if args.summarize_trace:
+        td, om, title = args.summarize_trace
+        with open(om, "w") as f:
+            f.write(summarize_trace(td, title, args.profile_commands, args.profile_note))
+        print(json.dumps({"summary": om}), flush=True)
         return
     if args.summarize_batch_trace:
         td, om = args.summarize_batch_trace

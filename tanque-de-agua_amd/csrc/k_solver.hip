@@ -712,19 +712,24 @@ __device__ __forceinline__ void k_viz_all(Blk B_, const VizPlan& P, uint8_t* __r
   }
 }
 
-// bubble vectors, the active list (labels with water faces, in label order) and their slots
+// bubble vectors, the active list (labels with water faces, in label order) and their slots.
+// Block 0 sets y, the list and the slots; block q = 1 .. BV_N - 1 clears bubble vector q.
 __device__ __forceinline__ void k_bub_init(Blk B_, const uint32_t* __restrict__ facesB, const float* __restrict__ yB32,
                                            int ar, double* __restrict__ bvec, int* __restrict__ activeList,
                                            int* __restrict__ labelSlot, DevScalars* __restrict__ sc) {
   __shared__ int cnt[16];
   const int tid = threadIdx.x;
+  if (B_.x > 0) {
+    double* v = bv(bvec, B_.x);
+    for (int r = tid; r < MAX_BOLHAS; r += blockDim.x) v[r] = 0.0;
+    return;
+  }
   int mine = 0;
   for (int k = 0; k < 4; k++) {
     const int r = tid * 4 + k;
     const bool act = ar && r >= 2 && facesB[r] > 0;
     mine += act;
-    for (int q = 0; q < BV_N; q++) bv(bvec, q)[r] = 0.0;
-    if (act) bv(bvec, BV_Y)[r] = (double)yB32[r];
+    bv(bvec, BV_Y)[r] = act ? (double)yB32[r] : 0.0;
     labelSlot[r] = -1;
   }
   // exclusive scan of the per-thread counts in label order: within each wavefront by shuffles,
@@ -3197,8 +3202,8 @@ static int solve_setup(tq_sim* s) {
     const Level& G = s->lv[l];
     launch<256, 0, k_coarse_type>(s, s->st, dim3(blocks_of(G.n)), dim3(256), 0, F.mask, F.nx, F.ny, G.mask, G.nx, G.ny);
   }
-  launch<1024, 0, k_bub_init>(s, s->st, dim3(1), dim3(1024), 0, s->facesB, s->yB32, s->d.ar, s->bvec, s->activeList,
-                              s->labelSlot, s->sc);
+  launch<1024, 0, k_bub_init>(s, s->st, dim3(BV_N), dim3(1024), 0, s->facesB, s->yB32, s->d.ar, s->bvec,
+                              s->activeList, s->labelSlot, s->sc);
   {
     // neighbour bits of levels 1.., and whether the dense operators must be rebuilt (their masks
     // changed, or the level-lc matrix was not kept current while it was switched off)
