@@ -102,23 +102,25 @@ __device__ inline BubAgg bagg_begin(BubLds& tab, double* acc, bool on) {
 }
 // not inlined: it is the cold branch of the cells' loops (bubbles beyond the slots), and inlining
 // its probe loop at every call site of an unrolled stencil slowed spmv by ~2.7 us per launch
-// even where it never runs
-__device__ __attribute__((noinline)) void bagg_add(const BubAgg& a, uint32_t lab, double v) {
+// even where it never runs.  The handle's two pointers go by value: a handle passed by reference
+// needs a home in private memory, which every wave of the kernel then writes at its start.
+__device__ __attribute__((noinline)) void bagg_add_at(BubLds* t, double* acc, uint32_t lab, double v) {
   unsigned long long hi, lo;
   fx_split(v, &hi, &lo);
-  if (a.t) {
+  if (t) {
     uint32_t e = (lab * 2654435761u) >> 26;  // 6 bits: BAGG entries
     for (int p = 0; p < BAGG_PROBE && lab != 0u; p++, e = (e + 1) & (BAGG - 1)) {
-      const uint32_t k = atomicCAS(&a.t->key[e], 0u, lab);
+      const uint32_t k = atomicCAS(&t->key[e], 0u, lab);
       if (k == 0u || k == lab) {
-        fx_add_words(&a.t->lo[e], &a.t->hi[e], hi, lo);
+        fx_add_words(&t->lo[e], &t->hi[e], hi, lo);
         return;
       }
     }
   }
-  fx_add_words(reinterpret_cast<unsigned long long*>(a.acc) + lab,
-               reinterpret_cast<unsigned long long*>(a.acc + (size_t)BV_FX_HI * MAX_BOLHAS) + lab, hi, lo);
+  fx_add_words(reinterpret_cast<unsigned long long*>(acc) + lab,
+               reinterpret_cast<unsigned long long*>(acc + (size_t)BV_FX_HI * MAX_BOLHAS) + lab, hi, lo);
 }
+__device__ inline void bagg_add(const BubAgg& a, uint32_t lab, double v) { bagg_add_at(a.t, a.acc, lab, v); }
 // every thread of the block, after its last addend
 __device__ inline void bagg_flush(const BubAgg& a) {
   if (!a.t) return;
@@ -1084,20 +1086,25 @@ __device__ inline void store_cells(const Cells4& c, SX* __restrict__ dst, int ox
     }
 }
 
-// tile of wave w of block (bx, by): 2x2 tiles per 256-thread block
+// A 256-thread block of the level-0 tile kernels owns a group of GW x GH tiles, one per wave:
+// four tiles side by side, so that a block reads 64-cell (256-byte) row segments.
+constexpr int GW = 4, GH = 1;
+__device__ inline void group_tile(int bx, int by, int w, int* tx, int* ty) {
+  *tx = GW * bx + (w % GW);
+  *ty = GH * by + (w / GW);
+}
+// tile of wave w of block (bx, by)
 __device__ inline void block_tile(const Blk& B_, int* tx, int* ty) {
-  const int w = threadIdx.x >> 6;
   int bx, by;
   xcd_block2(B_, &bx, &by);
-  *tx = 2 * bx + (w & 1);
-  *ty = 2 * by + (w >> 1);
+  group_tile(bx, by, threadIdx.x >> 6, tx, ty);
 }
 
 // ================================================================== level 0 input + first pass
 // UPD: q += alpha p, r -= alpha Ap and |r|^2 (PCG update, pressao-cpu.js:373-381); else the
 // first cycle (r minus the projected mean).  b0 = scale * r on water, rb/cells on bubble cells
 // (pressao-cpu.js:326-332), then pass A (offset 0, colours 0,1) from x = 0 (cicloV,
-// pressao-cpu.js:243) when level 0 is not the coarsest.  One-dimensional grid over the 2x2-tile
+// pressao-cpu.js:243) when level 0 is not the coarsest.  One-dimensional grid over the tile
 // groups (vgx of them per row, nvb in all); large grids launch at most IN_A0_GRID blocks, each
 // walking its groups after one fold.
 constexpr int IN_A0_GRID = 1024;  // 4 blocks per CU at this kernel's 123 VGPRs
@@ -1133,7 +1140,8 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
   bool in[2][2];
   auto load = [&](int g) {
     const int by = g / vgx, bx = g - by * vgx;
-    const int tx = 2 * bx + (w & 1), ty = 2 * by + (w >> 1);
+    int tx, ty;
+    group_tile(bx, by, w, &tx, &ty);
     valid = tx < ntx && ty < nty;
     ox = tx * 16;
     oy = ty * 16;
@@ -1509,7 +1517,7 @@ __device__ __forceinline__ void k_pass(Blk B_, const SX* __restrict__ src, SX* _
   __shared__ float Tt[4][TLDS];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   float* T = Tt[w];
-  // one-dimensional grid over the 2x2-tile groups (vgx per row, nvb in all); a folding launch
+  // one-dimensional grid over the tile groups (vgx per row, nvb in all); a folding launch
   // is capped and its blocks walk several groups after one fold
   int vb = xcd_block(B_.x, B_.gx);
   bool valid;
@@ -1517,7 +1525,8 @@ __device__ __forceinline__ void k_pass(Blk B_, const SX* __restrict__ src, SX* _
   Own4 own;
   auto load = [&](int g) {
     const int by = g / vgx, bx = g - by * vgx;
-    const int tx = 2 * bx + (w & 1), ty = 2 * by + (w >> 1);
+    int tx, ty;
+    group_tile(bx, by, w, &tx, &ty);
     valid = tx < ntx && ty < nty;
     ox = tx * 16 - desl;
     oy = ty * 16 - desl;
@@ -2734,7 +2743,7 @@ static bool plan_precision(tq_sim* s) {
 
 static int tiles_of(int n, int desl) { return (n + desl + 15) / 16; }
 static dim3 tile_grid(int nx, int ny, int desl) {
-  return dim3((tiles_of(nx, desl) + 1) / 2, (tiles_of(ny, desl) + 1) / 2);
+  return dim3((tiles_of(nx, desl) + GW - 1) / GW, (tiles_of(ny, desl) + GH - 1) / GH);
 }
 static int blocks_of(int C) { return (C + 255) / 256; }
 
