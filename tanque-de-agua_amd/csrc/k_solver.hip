@@ -196,6 +196,31 @@ __device__ __forceinline__ float ldf_if(const S* __restrict__ p, size_t c, bool 
   return in ? v : 0.f;
 }
 
+// Row pairs.  In the smoother tiles a lane's two cells of a row are the adjacent columns 2p,
+// 2p + 1 from an even tile origin, so with an even row length the pair starts at an even index
+// and moves as one 8-byte (FP16: 4-byte) access: half the memory instructions, and every store
+// writes whole 8-byte words instead of every other 4-byte word of a row.
+struct F2 {
+  float x, y;
+};
+template <class S> __device__ inline F2 ld2(const S* p, size_t c);
+template <> __device__ inline F2 ld2<float>(const float* p, size_t c) {
+  const float2 v = *reinterpret_cast<const float2*>(p + c);
+  return {v.x, v.y};
+}
+template <> __device__ inline F2 ld2<__half>(const __half* p, size_t c) {
+  const __half2 v = *reinterpret_cast<const __half2*>(p + c);
+  return {__low2float(v), __high2float(v)};
+}
+template <class S> __device__ inline void st2(S* p, size_t c, float a, float b);
+template <> __device__ inline void st2<float>(float* p, size_t c, float a, float b) {
+  *reinterpret_cast<float2*>(p + c) = make_float2(a, b);
+}
+template <> __device__ inline void st2<__half>(__half* p, size_t c, float a, float b) {
+  *reinterpret_cast<__half2*>(p + c) = __halves2half2(sat_half(a), sat_half(b));
+}
+__device__ inline uint16_t ld2b(const uint8_t* p, size_t c) { return *reinterpret_cast<const uint16_t*>(p + c); }
+
 // all LDS operations of this wavefront are complete (and not reordered across by the compiler)
 __device__ inline void wave_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
@@ -1049,9 +1074,41 @@ struct Own4 {
   float b[2][2];
 };
 
+// the lane's row pair r: index of its even column (pair 0 outside the grid), whether it is in
+// the grid (both cells are, or neither: even row length, even origin), and the colour of the
+// even column
+struct Pair {
+  size_t c;
+  bool in;
+};
+__device__ inline Pair cell_pair(int lane, int r, int ox, int oy, int nx, int ny) {
+  const int i = ox + 2 * (lane & 7), j = oy + cell_ly(lane, r);
+  const bool in = i < nx && j < ny && i >= 0 && j >= 0;
+  return {in ? (size_t)i + (size_t)j * nx : 0, in};
+}
+__device__ inline int pair_c0(int lane) { return ((lane >> 3) + 1) & 1; }
+// the pair's two values in colour order [c][r] (x is the even column's)
+template <class T>
+__device__ inline void by_colour(T (&v)[2][2], int r, int c0, T x, T y) {
+  v[0][r] = c0 ? y : x;
+  v[1][r] = c0 ? x : y;
+}
+
 template <class SB>
 __device__ inline void prefetch_cells(Own4& o, const SB* __restrict__ b, const uint8_t* __restrict__ mask, int ox,
                                       int oy, int nx, int ny, int lane) {
+  if ((nx & 1) == 0) {
+    const int c0 = pair_c0(lane);
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      const Pair pr = cell_pair(lane, r, ox, oy, nx, ny);
+      const uint16_t m2 = ld2b(mask, pr.c);
+      const F2 b2 = ld2<SB>(b, pr.c);
+      by_colour(o.m, r, c0, (uint8_t)(pr.in ? m2 & 0xff : 0), (uint8_t)(pr.in ? m2 >> 8 : 0));
+      by_colour(o.b, r, c0, pr.in ? b2.x : 0.f, pr.in ? b2.y : 0.f);
+    }
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < 2; q++)
 #pragma unroll
@@ -1076,6 +1133,16 @@ __device__ inline void init_cells(Cells4& c, const Own4& o, const float* T, int 
 
 template <class SX>
 __device__ inline void store_cells(const Cells4& c, SX* __restrict__ dst, int ox, int oy, int nx, int ny, int lane) {
+  if ((nx & 1) == 0) {
+    const int c0 = pair_c0(lane);
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      const Pair pr = cell_pair(lane, r, ox, oy, nx, ny);
+      const float v0 = (c.m[0][r] & 32) ? c.v[0][r] : 0.f, v1 = (c.m[1][r] & 32) ? c.v[1][r] : 0.f;
+      if (pr.in) st2<SX>(dst, pr.c, c0 ? v1 : v0, c0 ? v0 : v1);
+    }
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < 2; q++)
 #pragma unroll
@@ -1138,6 +1205,8 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
   float qv[2][2], pv[2][2], rv[2][2], av[2][2], lv[2][2];
   size_t ci[2][2];
   bool in[2][2];
+  const bool pairs = (nx & 1) == 0;  // a lane's two cells of a row move as one access
+  const int c0 = pair_c0(lane);
   auto load = [&](int g) {
     const int by = g / vgx, bx = g - by * vgx;
     int tx, ty;
@@ -1145,6 +1214,35 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
     valid = tx < ntx && ty < nty;
     ox = tx * 16;
     oy = ty * 16;
+    if (pairs) {
+      // raw values (pair 0 outside the grid): every use is under in[][]
+#pragma unroll
+      for (int rw = 0; rw < 2; rw++) {
+        const Pair pr = cell_pair(lane, rw, ox, oy, nx, ny);
+        const bool inr = valid && pr.in;
+        const size_t c = inr ? pr.c : 0;
+        in[0][rw] = in[1][rw] = inr;
+        by_colour(ci, rw, c0, c, c + 1);
+        const uint16_t o2 = ld2b(opMask, c), g2 = ld2b(mg0, c);
+        by_colour(om, rw, c0, (uint8_t)(o2 & 0xff), (uint8_t)(o2 >> 8));
+        by_colour(gm, rw, c0, (uint8_t)(g2 & 0xff), (uint8_t)(g2 >> 8));
+        const F2 r2 = ld2<float>(r, c);
+        by_colour(rv, rw, c0, r2.x, r2.y);
+        if (UPD) {
+          const F2 q2 = ld2<float>(q, c), p2 = ld2<float>(p, c), a2 = ld2<float>(ap, c);
+          by_colour(qv, rw, c0, q2.x, q2.y);
+          by_colour(pv, rw, c0, p2.x, p2.y);
+          by_colour(av, rw, c0, a2.x, a2.y);
+          if (comp) {
+            const F2 l2 = ld2<float>(qlo, c);
+            by_colour(lv, rw, c0, l2.x, l2.y);
+          } else {
+            lv[0][rw] = lv[1][rw] = 0.f;
+          }
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int cq = 0; cq < 2; cq++)
 #pragma unroll
@@ -1200,36 +1298,47 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
   }
   const double alpha = UPD ? af_finish(af, B_, par, B, cellsB, sh, res, aps, rbs, bsl, sc) : 0.0;
   const double beta = UPD ? af.beta : 0.0;
+  const bool wr = UPD || mean != 0;  // r is rewritten
   for (;;) {
     Cells4 cc;
+    // the new q, q_lo, r and b0 of each cell (cells other than water keep their q, q_lo and r)
+    float qn[2][2], ln[2][2], rn[2][2], bn[2][2];
 #pragma unroll
     for (int cq = 0; cq < 2; cq++)
 #pragma unroll
       for (int rw = 0; rw < 2; rw++) {
         cc.v[cq][rw] = 0.f;
+        qn[cq][rw] = UPD ? qv[cq][rw] : 0.f;
+        ln[cq][rw] = UPD ? lv[cq][rw] : 0.f;
+        rn[cq][rw] = rv[cq][rw];
+        bn[cq][rw] = 0.f;
         if (in[cq][rw]) {
           const size_t c = ci[cq][rw];
           const uint8_t m = om[cq][rw];
           float bval = 0.f;
           if (mask_type(m) == AGUA) {
-            float rn;
             if (UPD) {
               if (comp) {  // the compensated iterate: q + qlo keeps the update's binary64 sum
                 const double sx = (double)qv[cq][rw] + (double)lv[cq][rw] + alpha * (double)pv[cq][rw];
                 const float hi = (float)sx;
-                q[c] = hi;
-                qlo[c] = (float)(sx - (double)hi);
+                qn[cq][rw] = hi;
+                ln[cq][rw] = (float)(sx - (double)hi);
               } else {
-                q[c] = (float)((double)qv[cq][rw] + alpha * (double)pv[cq][rw]);
+                qn[cq][rw] = (float)((double)qv[cq][rw] + alpha * (double)pv[cq][rw]);
               }
-              rn = (float)((double)rv[cq][rw] - alpha * (double)av[cq][rw]);
-              r[c] = rn;
-              rr[0] += (double)rn * rn;
-            } else {
-              rn = rv[cq][rw];
-              if (mean != 0) { rn = (float)((double)rn - mean); r[c] = rn; }
+              rn[cq][rw] = (float)((double)rv[cq][rw] - alpha * (double)av[cq][rw]);
+              rr[0] += (double)rn[cq][rw] * rn[cq][rw];
+            } else if (mean != 0) {
+              rn[cq][rw] = (float)((double)rv[cq][rw] - mean);
             }
-            bval = rn;
+            bval = rn[cq][rw];
+            if (!pairs) {
+              if (UPD) {
+                q[c] = qn[cq][rw];
+                if (comp) qlo[c] = ln[cq][rw];
+              }
+              if (wr) r[c] = rn[cq][rw];
+            }
           } else if (m & MK_BUB) {
             const uint32_t lb = lbv[cq][rw];
             const int sl = B.labelSlot[lb];
@@ -1237,10 +1346,24 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
             else bval = (float)((UPD ? bubble_r(B, lb, par, alpha, beta, rbs) : rb[lb]) / cellsB[lb]);
           }
           bval = rnd<SB>(bval * scale);
-          st<SB>(b0, c, bval);
+          bn[cq][rw] = bval;
+          if (!pairs) st<SB>(b0, c, bval);
           cell_setup(cc, cq, rw, gm[cq][rw], bval);
         } else cell_clear(cc, cq, rw);
       }
+    if (pairs) {
+#pragma unroll
+      for (int rw = 0; rw < 2; rw++)
+        if (in[0][rw]) {
+          const size_t c = c0 ? ci[1][rw] : ci[0][rw];  // the even column
+          if (UPD) {
+            st2<float>(q, c, c0 ? qn[1][rw] : qn[0][rw], c0 ? qn[0][rw] : qn[1][rw]);
+            if (comp) st2<float>(qlo, c, c0 ? ln[1][rw] : ln[0][rw], c0 ? ln[0][rw] : ln[1][rw]);
+          }
+          if (wr) st2<float>(r, c, c0 ? rn[1][rw] : rn[0][rw], c0 ? rn[0][rw] : rn[1][rw]);
+          st2<SB>(b0, c, c0 ? bn[1][rw] : bn[0][rw], c0 ? bn[0][rw] : bn[1][rw]);
+        }
+    }
     if (PASSA && valid) {
       float* T = Tt[w];
       zero_tile(T, lane);
@@ -1298,9 +1421,8 @@ __device__ inline void zbub_vals(ZBubCells& zb, const ZArgs& za) {
 
 // z and its dot/bubble contributions for one cell (x = MG output of level 0; lb, lsl, cb: the
 // cell's bubble label, its slot and cell count when it is a bubble cell)
-__device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, float zoldc, float xv, float inv,
-                              double (&dv)[2], double (&sl)[BSLOTS], const BubAgg& agg, uint32_t lb, int lsl,
-                              uint32_t cb) {
+__device__ inline float z_val(uint8_t om, float rc, float zoldc, float xv, float inv, double (&dv)[2],
+                             double (&sl)[BSLOTS], const BubAgg& agg, uint32_t lb, int lsl, uint32_t cb) {
   const float x = xv * inv;
   float z = 0.f;
   if (mask_type(om) == AGUA) {
@@ -1310,7 +1432,12 @@ __device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, f
   } else if (om & MK_BUB) {
     bub_add_s(sl, lsl, agg, lb, (double)x / cb);
   }
-  za.znew[c] = z;
+  return z;
+}
+__device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, float zoldc, float xv, float inv,
+                              double (&dv)[2], double (&sl)[BSLOTS], const BubAgg& agg, uint32_t lb, int lsl,
+                              uint32_t cb) {
+  za.znew[c] = z_val(om, rc, zoldc, xv, inv, dv, sl, agg, lb, lsl, cb);
 }
 
 // z and the dots when level 0 itself is solved inside the workgroup (small grids)
@@ -1453,18 +1580,33 @@ __device__ __forceinline__ void k_pass_z(Blk B_, const float* __restrict__ src, 
   uint8_t om[2][2];
   float rv[2][2], zo[2][2];
   size_t cz[2][2];
-#pragma unroll
-  for (int cq = 0; cq < 2; cq++)
+  const bool pairs = (nx & 1) == 0;  // a lane's two cells of a row move as one access
+  const int c0 = pair_c0(lane);
+  if (pairs) {
 #pragma unroll
     for (int rw = 0; rw < 2; rw++) {
-      const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
-      const bool in = i < nx && j < ny;
-      const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
-      cz[cq][rw] = c;
-      om[cq][rw] = ld_if<uint8_t>(za.opMask, c, in, 0);
-      rv[cq][rw] = za.r[c];
-      zo[cq][rw] = za.zold[c];
+      const Pair pr = cell_pair(lane, rw, ox, oy, nx, ny);
+      by_colour(cz, rw, c0, pr.c, pr.in ? pr.c + 1 : (size_t)0);
+      const uint16_t o2 = ld2b(za.opMask, pr.c);
+      const F2 r2 = ld2<float>(za.r, pr.c), z2 = ld2<float>(za.zold, pr.c);
+      by_colour(om, rw, c0, (uint8_t)(pr.in ? o2 & 0xff : 0), (uint8_t)(pr.in ? o2 >> 8 : 0));
+      by_colour(rv, rw, c0, r2.x, r2.y);
+      by_colour(zo, rw, c0, z2.x, z2.y);
     }
+  } else {
+#pragma unroll
+    for (int cq = 0; cq < 2; cq++)
+#pragma unroll
+      for (int rw = 0; rw < 2; rw++) {
+        const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
+        const bool in = i < nx && j < ny;
+        const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
+        cz[cq][rw] = c;
+        om[cq][rw] = ld_if<uint8_t>(za.opMask, c, in, 0);
+        rv[cq][rw] = za.r[c];
+        zo[cq][rw] = za.zold[c];
+      }
+  }
   fill_tile<float>(T, src, ox, oy, nx, ny, lane);
   if (sc_done) return;
   if (valid) {
@@ -1476,16 +1618,26 @@ __device__ __forceinline__ void k_pass_z(Blk B_, const float* __restrict__ src, 
     Cells4 cc;
     init_cells(cc, own, T, lane);
     tile_sweeps<1>(T, cc, lane);
+    float zz[2][2];
 #pragma unroll
     for (int cq = 0; cq < 2; cq++)
 #pragma unroll
       for (int rw = 0; rw < 2; rw++) {
         const int i = ox + cell_lx(lane, cq), j = oy + cell_ly(lane, rw);
-        if (i < nx && j < ny)
-          z_cell(za, (size_t)i + (size_t)j * nx, om[cq][rw], rv[cq][rw], zo[cq][rw],
-                 (cc.m[cq][rw] & 32) ? cc.v[cq][rw] : 0.f, inv, dv, sl, agg, zb.lb[cq][rw], zb.sl[cq][rw],
-                 zb.cb[cq][rw]);
+        zz[cq][rw] = 0.f;
+        if (i < nx && j < ny) {
+          zz[cq][rw] = z_val(om[cq][rw], rv[cq][rw], zo[cq][rw], (cc.m[cq][rw] & 32) ? cc.v[cq][rw] : 0.f, inv, dv,
+                             sl, agg, zb.lb[cq][rw], zb.sl[cq][rw], zb.cb[cq][rw]);
+          if (!pairs) za.znew[(size_t)i + (size_t)j * nx] = zz[cq][rw];
+        }
       }
+    if (pairs) {
+#pragma unroll
+      for (int rw = 0; rw < 2; rw++) {
+        const Pair pr = cell_pair(lane, rw, ox, oy, nx, ny);
+        if (pr.in) st2<float>(za.znew, pr.c, c0 ? zz[1][rw] : zz[0][rw], c0 ? zz[0][rw] : zz[1][rw]);
+      }
+    }
   }
   bagg_flush(agg);
   store_partials<2>(B_, dv, 2, za.partials, sh);
