@@ -220,6 +220,13 @@ template <> __device__ inline void st2<__half>(__half* p, size_t c, float a, flo
   *reinterpret_cast<__half2*>(p + c) = __halves2half2(sat_half(a), sat_half(b));
 }
 __device__ inline uint16_t ld2b(const uint8_t* p, size_t c) { return *reinterpret_cast<const uint16_t*>(p + c); }
+// a row pair: the index of its even column (0 outside the grid) and whether it is in the grid
+// (both cells are, or neither: even row length, even origin)
+struct Pair {
+  size_t c;
+  bool in;
+};
+
 
 // all LDS operations of this wavefront are complete (and not reordered across by the compiler)
 __device__ inline void wave_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
@@ -979,6 +986,19 @@ struct Cells4 {
 __device__ inline int cell_lx(int lane, int q) { return 2 * (lane & 7) + ((q + (lane >> 3) + 1) & 1); }
 __device__ inline int cell_ly(int lane, int r) { return (lane >> 3) + 8 * r; }
 __device__ inline int cell_e(int lane, int q, int r) { return cell_lx(lane, q) + 1 + (cell_ly(lane, r) + 1) * TP; }
+// the lane's row pair r, and the colour of its even column
+__device__ inline Pair cell_pair(int lane, int r, int ox, int oy, int nx, int ny) {
+  const int i = ox + 2 * (lane & 7), j = oy + cell_ly(lane, r);
+  const bool in = i < nx && j < ny && i >= 0 && j >= 0;
+  return {in ? (size_t)i + (size_t)j * nx : 0, in};
+}
+__device__ inline int pair_c0(int lane) { return ((lane >> 3) + 1) & 1; }
+// the pair's two values in colour order [c][r] (x is the even column's)
+template <class T>
+__device__ inline void by_colour(T (&v)[2][2], int r, int c0, T x, T y) {
+  v[0][r] = c0 ? y : x;
+  v[1][r] = c0 ? x : y;
+}
 
 __device__ inline void cell_setup(Cells4& c, int q, int r, uint8_t mk, float b) {
   const int cnt = mask_cnt(mk);
@@ -1039,8 +1059,46 @@ __device__ inline void tile_sweeps(float* T, Cells4& c, int lane) {
 
 // T := snapshot of src on the tile + halo (0 outside the grid): the six loads of a lane first,
 // then the six LDS stores
-template <class SX>
+// the tile with its halo as row pairs from the even column ox - 2: 10 pairs per row, 180 in all
+// (three per lane); a pair's two values go to columns 2k - 1 and 2k of the tile (the first
+// pair's first and the last pair's second column lie outside it)
+constexpr int TPAIRS = 10 * TS;
+__device__ inline Pair tile_pair(int p, int ox, int oy, int nx, int ny, int* lx, int* ly) {
+  const int y = p / 10, k = p - 10 * y;
+  const int i = ox - 2 + 2 * k, j = oy - 1 + y;
+  const bool in = p < TPAIRS && i >= 0 && j >= 0 && i < nx && j < ny;
+  *lx = 2 * k - 1;
+  *ly = y;
+  return {in ? (size_t)i + (size_t)j * nx : 0, in};
+}
+__device__ inline void tile_put(float* T, int p, int lx, int ly, float a, float b) {
+  if (p < TPAIRS) {
+    if (lx >= 0) T[lx + ly * TP] = a;
+    if (lx < TS - 1) T[lx + 1 + ly * TP] = b;
+  }
+}
+
+// (PAIRS false: cells only -- the walking pass kernel, whose registers the pair path would take
+// past 64, i.e. below 8 waves per SIMD)
+template <class SX, bool PAIRS = true>
 __device__ inline void fill_tile(float* T, const SX* __restrict__ src, int ox, int oy, int nx, int ny, int lane) {
+  if (PAIRS && (nx & 1) == 0) {
+    F2 v[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      int lx, ly;
+      const Pair pr = tile_pair(lane + 64 * q, ox, oy, nx, ny, &lx, &ly);
+      const F2 t = ld2<SX>(src, pr.c);
+      v[q] = pr.in ? t : F2{0.f, 0.f};
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      int lx, ly;
+      (void)tile_pair(lane + 64 * q, ox, oy, nx, ny, &lx, &ly);
+      tile_put(T, lane + 64 * q, lx, ly, v[q].x, v[q].y);
+    }
+    return;
+  }
   float v[6];
 #pragma unroll
   for (int q = 0; q < 6; q++) {
@@ -1073,26 +1131,6 @@ struct Own4 {
   uint8_t m[2][2];
   float b[2][2];
 };
-
-// the lane's row pair r: index of its even column (pair 0 outside the grid), whether it is in
-// the grid (both cells are, or neither: even row length, even origin), and the colour of the
-// even column
-struct Pair {
-  size_t c;
-  bool in;
-};
-__device__ inline Pair cell_pair(int lane, int r, int ox, int oy, int nx, int ny) {
-  const int i = ox + 2 * (lane & 7), j = oy + cell_ly(lane, r);
-  const bool in = i < nx && j < ny && i >= 0 && j >= 0;
-  return {in ? (size_t)i + (size_t)j * nx : 0, in};
-}
-__device__ inline int pair_c0(int lane) { return ((lane >> 3) + 1) & 1; }
-// the pair's two values in colour order [c][r] (x is the even column's)
-template <class T>
-__device__ inline void by_colour(T (&v)[2][2], int r, int c0, T x, T y) {
-  v[0][r] = c0 ? y : x;
-  v[1][r] = c0 ? x : y;
-}
 
 template <class SB>
 __device__ inline void prefetch_cells(Own4& o, const SB* __restrict__ b, const uint8_t* __restrict__ mask, int ox,
@@ -1463,6 +1501,85 @@ __device__ __forceinline__ void k_zdots0(Blk B_, const float* __restrict__ x0, Z
   if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, za.bpart, sh);
 }
 
+// snapshot value of fine cell (i, j) with its prolongation addend: x_B + P e on water cells, from
+// the coarse values of the tile's 10x10 patch CP at (PX, PY) (prolongar, pressao-cpu.js:191-214)
+__device__ inline float snap_cell(float val, uint8_t mk, int i, int j, const float* CP, int PX, int PY) {
+  if (mask_type(mk) == AGUA) {  // in the grid (masks are 0 outside)
+    const int I0 = ((i & 1) == 0 ? (i >> 1) - 1 : (i - 1) >> 1) - PX;
+    const int J0 = ((j & 1) == 0 ? (j >> 1) - 1 : (j - 1) >> 1) - PY;
+    const float wx0 = (i & 1) == 0 ? 0.25f : 0.75f, wy0 = (j & 1) == 0 ? 0.25f : 0.75f;
+    const float wx1 = 1.f - wx0, wy1 = 1.f - wy0;
+    float e2 = 0.f;
+    e2 += CP[I0 + J0 * 10] * wx0 * wy0;
+    e2 += CP[I0 + 1 + J0 * 10] * wx1 * wy0;
+    e2 += CP[I0 + (J0 + 1) * 10] * wx0 * wy1;
+    e2 += CP[I0 + 1 + (J0 + 1) * 10] * wx1 * wy1;
+    val += e2;
+  }
+  return val;
+}
+
+// The snapshot of a fine tile with halo: values and masks, as row pairs (even row length) or
+// cells; put() then writes each tile cell's snapshot value with its prolongation addend to T.
+template <class SX>
+struct Snap {
+  float v[6];
+  uint8_t m[6];
+  bool pairs;
+  __device__ void load(const SX* __restrict__ src, const uint8_t* __restrict__ mask, int ox, int oy, int nx, int ny,
+                       int lane) {
+    pairs = (nx & 1) == 0;
+    if (pairs) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        int lx, ly;
+        const Pair pr = tile_pair(lane + 64 * q, ox, oy, nx, ny, &lx, &ly);
+        const F2 t = ld2<SX>(src, pr.c);
+        const uint16_t m2 = ld2b(mask, pr.c);
+        v[2 * q] = pr.in ? t.x : 0.f;
+        v[2 * q + 1] = pr.in ? t.y : 0.f;
+        m[2 * q] = pr.in ? (uint8_t)(m2 & 0xff) : 0;
+        m[2 * q + 1] = pr.in ? (uint8_t)(m2 >> 8) : 0;
+      }
+      return;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const int e = lane + 64 * q;
+      const int ly = e / TS, lx = e - ly * TS;
+      const int i = ox - 1 + lx, j = oy - 1 + ly;
+      const bool in = e < TCELLS && i >= 0 && j >= 0 && i < nx && j < ny;
+      const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
+      v[q] = ldf_if<SX>(src, c, in);
+      m[q] = ld_if<uint8_t>(mask, c, in, 0);
+    }
+  }
+  __device__ void put(float* T, const float* CP, int PX, int PY, int ox, int oy, int nx, int ny, int lane) const {
+    if (pairs) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        int lx, ly;
+        (void)tile_pair(lane + 64 * q, ox, oy, nx, ny, &lx, &ly);
+        const int i = ox - 1 + lx, j = oy - 1 + ly;
+        // the columns outside the tile (dropped by tile_put) take no addend: their coarse
+        // neighbours lie outside the patch
+        const uint8_t m0 = lx >= 0 ? m[2 * q] : 0, m1 = lx + 1 < TS ? m[2 * q + 1] : 0;
+        tile_put(T, lane + 64 * q, lx, ly, snap_cell(v[2 * q], m0, i, j, CP, PX, PY),
+                 snap_cell(v[2 * q + 1], m1, i + 1, j, CP, PX, PY));
+      }
+      return;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const int e = lane + 64 * q;
+      if (e < TCELLS) {
+        const int ly = e / TS, lx = e - ly * TS;
+        T[lx + ly * TP] = snap_cell(v[q], m[q], ox - 1 + lx, oy - 1 + ly, CP, PX, PY);
+      }
+    }
+  }
+};
+
 // ================================================================== up on level 0 (unfused)
 // Level 0 is large enough that recomputing pass C four times costs more than a kernel
 // boundary, so its up path stays two kernels: prolongation + pass C, then pass D writing z.
@@ -1502,18 +1619,8 @@ __device__ __forceinline__ void k_prolong_pass(Blk B_, const SX* __restrict__ sr
     cm[q] = ld_if<uint8_t>(cmk, C, in, 0);
     cv[q] = ldf_if<SC>(cx, C, in);
   }
-  float sv[6];
-  uint8_t sm[6];
-#pragma unroll
-  for (int q = 0; q < 6; q++) {
-    const int e = lane + 64 * q;
-    const int ly = e / TS, lx = e - ly * TS;
-    const int i = ox - 1 + lx, j = oy - 1 + ly;
-    const bool in = e < TCELLS && i >= 0 && j >= 0 && i < nx && j < ny;
-    const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
-    sv[q] = ldf_if<SX>(src, c, in);
-    sm[q] = ld_if<uint8_t>(mask, c, in, 0);
-  }
+  Snap<SX> sn;
+  sn.load(src, mask, ox, oy, nx, ny, lane);
   if (sc_done) return;
 #pragma unroll
   for (int q = 0; q < 2; q++) {
@@ -1521,28 +1628,7 @@ __device__ __forceinline__ void k_prolong_pass(Blk B_, const SX* __restrict__ sr
     if (e < 100) CP[e] = mask_type(cm[q]) == AGUA ? cv[q] : 0.f;
   }
   wave_sync();
-#pragma unroll
-  for (int q = 0; q < 6; q++) {
-    const int e = lane + 64 * q;
-    if (e < TCELLS) {
-      const int ly = e / TS, lx = e - ly * TS;
-      const int i = ox - 1 + lx, j = oy - 1 + ly;
-      float val = sv[q];
-      if (mask_type(sm[q]) == AGUA) {  // in the grid (sm is 0 outside)
-        const int I0 = ((i & 1) == 0 ? (i >> 1) - 1 : (i - 1) >> 1) - PX;
-        const int J0 = ((j & 1) == 0 ? (j >> 1) - 1 : (j - 1) >> 1) - PY;
-        const float wx0 = (i & 1) == 0 ? 0.25f : 0.75f, wy0 = (j & 1) == 0 ? 0.25f : 0.75f;
-        const float wx1 = 1.f - wx0, wy1 = 1.f - wy0;
-        float e2 = 0.f;
-        e2 += CP[I0 + J0 * 10] * wx0 * wy0;
-        e2 += CP[I0 + 1 + J0 * 10] * wx1 * wy0;
-        e2 += CP[I0 + (J0 + 1) * 10] * wx0 * wy1;
-        e2 += CP[I0 + 1 + (J0 + 1) * 10] * wx1 * wy1;
-        val += e2;
-      }
-      T[lx + ly * TP] = val;
-    }
-  }
+  sn.put(T, CP, PX, PY, ox, oy, nx, ny, lane);
   wave_sync();
   Cells4 cc;
   init_cells(cc, own, T, lane);
@@ -1684,7 +1770,7 @@ __device__ __forceinline__ void k_pass(Blk B_, const SX* __restrict__ src, SX* _
     oy = ty * 16 - desl;
     if (valid) {
       prefetch_cells<SB>(own, b, mask, ox, oy, nx, ny, lane);
-      fill_tile<SX>(T, src, ox, oy, nx, ny, lane);
+      fill_tile<SX, false>(T, src, ox, oy, nx, ny, lane);
     }
   };
   load(vb);
@@ -2043,18 +2129,8 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
     cm[q] = ld_if<uint8_t>(cmk, C, in, 0);
     cv[q] = ldf_if<SC>(cx, C, in);
   }
-  float sv[6];
-  uint8_t sm[6];
-#pragma unroll
-  for (int q = 0; q < 6; q++) {
-    const int e = lane + 64 * q;
-    const int ly = e / TS, lx = e - ly * TS;
-    const int i = ox - 1 + lx, j = oy - 1 + ly;
-    const bool in = e < TCELLS && i >= 0 && j >= 0 && i < nx && j < ny;
-    const size_t c = in ? (size_t)i + (size_t)j * nx : 0;
-    sv[q] = ldf_if<SX>(xb, c, in);
-    sm[q] = ld_if<uint8_t>(mask, c, in, 0);
-  }
+  Snap<SX> sn;
+  sn.load(xb, mask, ox, oy, nx, ny, lane);
   if (sc_done) return;
 #pragma unroll
   for (int q = 0; q < 2; q++) {
@@ -2066,28 +2142,7 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
   if (ZD && wave == 0) zbub_labels(zb, za, om, cz);
   wave_sync();
   // ---- snapshot x_B + P e, pass C on this wave's tile, into the window
-#pragma unroll
-  for (int q = 0; q < 6; q++) {
-    const int e = lane + 64 * q;
-    if (e < TCELLS) {
-      const int ly = e / TS, lx = e - ly * TS;
-      const int i = ox - 1 + lx, j = oy - 1 + ly;
-      float val = sv[q];
-      if (mask_type(sm[q]) == AGUA) {  // in the grid (sm is 0 outside)
-        const int I0 = ((i & 1) == 0 ? (i >> 1) - 1 : (i - 1) >> 1) - PX;
-        const int J0 = ((j & 1) == 0 ? (j >> 1) - 1 : (j - 1) >> 1) - PY;
-        const float wx0 = (i & 1) == 0 ? 0.25f : 0.75f, wy0 = (j & 1) == 0 ? 0.25f : 0.75f;
-        const float wx1 = 1.f - wx0, wy1 = 1.f - wy0;
-        float e2 = 0.f;
-        e2 += CP[I0 + J0 * 10] * wx0 * wy0;
-        e2 += CP[I0 + 1 + J0 * 10] * wx1 * wy0;
-        e2 += CP[I0 + (J0 + 1) * 10] * wx0 * wy1;
-        e2 += CP[I0 + 1 + (J0 + 1) * 10] * wx1 * wy1;
-        val += e2;
-      }
-      T[lx + ly * TP] = val;
-    }
-  }
+  sn.put(T, CP, PX, PY, ox, oy, nx, ny, lane);
   wave_sync();
   {
     Cells4 cc;
