@@ -769,6 +769,8 @@ def main():
                     help="only the adjoint configuration with its central differences and windowed check")
     ap.add_argument("--sync-probe", action="store_true",
                     help="only the kernel-boundary / grid-barrier cost probe (tanque-de-agua_amd/lib/bench_sync)")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="only the HBM streaming-bandwidth probe (tanque-de-agua_amd/lib/bench_stream)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="start with a 256 MiB device copy (reference for rocprofv3 --pmc byte counters)")
     ap.add_argument("--nx", type=int, default=896)
@@ -781,15 +783,17 @@ def main():
     import tanque_hip as tq
     from tanque_hip import scenes
     tq.load()
-    if args.sync_probe:
+    for flag, probe in ((args.sync_probe, "bench_sync"), (args.stream_probe, "bench_stream")):
+        if not flag:
+            continue
         import subprocess
-        exe = os.path.join(ROOT, "tanque-de-agua_amd", "lib", "bench_sync")
+        exe = os.path.join(ROOT, "tanque-de-agua_amd", "lib", probe)
         if not os.path.exists(exe):
-            raise SystemExit("bench_sync is not built (__graft_entry__.build())")
+            raise SystemExit("%s is not built (__graft_entry__.build())" % probe)
         r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
         if r.returncode != 0:
-            raise SystemExit("bench_sync failed: %s" % r.stderr)
-        print(json.dumps({"sync_probe": json.loads(r.stdout)}), flush=True)
+            raise SystemExit("%s failed: %s" % (probe, r.stderr))
+        print(json.dumps({probe[len("bench_"):] + "_probe": json.loads(r.stdout)}), flush=True)
         return
     if args.summarize_profiles:
         fd, wd, td, oj, om = args.summarize_profiles

@@ -84,6 +84,23 @@ def build_sync_bench(force=False):
     return SYNC_BENCH
 
 
+STREAM_BENCH = os.path.join(LIB_DIR, "bench_stream")
+
+
+def build_stream_bench(force=False):
+    """The HBM streaming-bandwidth probe (csrc/bench_stream.hip; bench.py --stream-probe)."""
+    src = os.path.join(CSRC, "bench_stream.hip")
+    if not force and not _newer(STREAM_BENCH, [src]):
+        return STREAM_BENCH
+    tmp = STREAM_BENCH + ".tmp"
+    r = subprocess.run([HIPCC] + [f for f in HIP_FLAGS if f != "-fPIC"] + [src, "-o", tmp], capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError("bench_stream build failed:\n%s\n%s" % (r.stdout, r.stderr))
+    os.replace(tmp, STREAM_BENCH)
+    return STREAM_BENCH
+
+
 def build_node_addon(force=False):
     """The thin Node-API addon over the C ABI (needs node's headers; skipped if absent)."""
     src = os.path.join(CSRC, "napi_addon.cpp")
@@ -106,6 +123,7 @@ def build(force=False):
     lib = build_lib(force)
     addon = build_node_addon(force)
     build_sync_bench(force)
+    build_stream_bench(force)
     return lib, addon
 
 
