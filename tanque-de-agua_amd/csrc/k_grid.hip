@@ -17,14 +17,20 @@ constexpr int SF_THREADS = 1024;
 constexpr int SF_K = (SF_N + SF_THREADS - 1) / SF_THREADS;   // region cells per thread (4)
 static_assert(SF_T * SF_T == SF_THREADS, "one owned cell per thread");
 
-// fn(e, lx, ly) for the region cells within RX columns and RY rows of the owned tile
+// fn(e, lx, ly) for the region cells within RX columns and RY rows of the owned tile; the loop
+// over a thread's cells is unrolled (its trip count is known), so their LDS reads are issued
+// together instead of one cell's chain after the other's
 template <int RX, int RY, class Fn>
 __device__ __forceinline__ void sf_box(Fn fn) {
   constexpr int W = SF_T + 2 * RX, H = SF_T + 2 * RY, N = W * H;
-  for (int i = threadIdx.x; i < N; i += SF_THREADS) {
-    const int by = i / W, bx = i - by * W;
-    const int lx = SF_H - RX + bx, ly = SF_H - RY + by;
-    fn(lx + ly * SF_R, lx, ly);
+#pragma unroll
+  for (int k = 0; k < (N + SF_THREADS - 1) / SF_THREADS; k++) {
+    const int i = (int)threadIdx.x + k * SF_THREADS;
+    if (i < N) {
+      const int by = i / W, bx = i - by * W;
+      const int lx = SF_H - RX + bx, ly = SF_H - RY + by;
+      fn(lx + ly * SF_R, lx, ly);
+    }
   }
 }
 
@@ -114,8 +120,18 @@ __device__ __forceinline__ void k_surface(Blk B_, const uint8_t* __restrict__ ti
   }
   // The wall extension fills solid cells (cells outside the grid are never filled) from filled
   // neighbours: a region without solid cells, or without a filled one, is left as it is by all its
-  // layers, which are skipped (the tank's interior, the inside of thick walls).
-  const bool walls = (__syncthreads_or(solid_here) != 0) && (__syncthreads_or(fluid_here) != 0);
+  // layers, which are skipped (the tank's interior, the inside of thick walls).  Both flags of
+  // the region in one barrier: each wave's two ballots, then the 16 waves' words.
+  __shared__ int wflag[SF_THREADS / 64];
+  {
+    const bool ws = __ballot(solid_here) != 0, wf = __ballot(fluid_here) != 0;
+    if ((tid & 63) == 0) wflag[tid >> 6] = (ws ? 1 : 0) | (wf ? 2 : 0);
+  }
+  __syncthreads();
+  int any = 0;
+#pragma unroll
+  for (int k = 0; k < SF_THREADS / 64; k++) any |= wflag[k];
+  const bool walls = any == 3;
   // ext3 (radius 10), blur (9), ext3 (6), blur (5), ext3 (2): what the normals (1) read
   if (walls) sf_extend<10>(f, t, fl, ty, I0, J0, nx, ny);
   sf_blur<9>(f, t, I0, J0, nx, ny);
