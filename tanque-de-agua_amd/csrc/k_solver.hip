@@ -255,6 +255,80 @@ __device__ inline void bub_add_s(double (&sl)[BSLOTS], int s, const BubAgg& agg,
 
 // Block sum of the first nv of v[]; thread 0 stores partial `bid` of each quantity.  nv must
 // be uniform over the block; `sh` holds NMAX*16 doubles.
+// Block totals of the first nv of v[] in every thread, with one barrier: each wave's sums (DPP)
+// go to sh[k * 16 + wave], then every thread adds the waves' sums of each quantity in wave order
+// (the order one thread's loop over them would take, so the totals are those of the two-barrier
+// form that has thread k sum quantity k and share it).  v[k] = 0 for k >= nv.  The other threads
+// may still be reading sh when a thread returns: the caller puts a barrier before writing it
+// again, or uses another region.
+template <int NV>
+__device__ inline void block_total(double (&v)[NV], int nv, double* sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+    if (k < nv) {
+      const double t = wave_sum(v[k]);
+      if (lane == 0) sh[k * 16 + w] = t;
+    }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double t = 0;
+    if (k < nv)
+      for (int q = 0; q < nw; q++) t += sh[k * 16 + q];
+    v[k] = t;
+  }
+}
+// LDS of a kernel's folds: the partials' totals ((2 + BSLOTS) quantities), then two regions for
+// the bubbles' sums, so that consecutive reductions need no barrier between them
+constexpr int FOLD_SH = (2 + BSLOTS) * 16 + 2 * 16 + 16;
+__device__ inline double* fold_sh2(double* sh) { return sh + (2 + BSLOTS) * 16; }
+__device__ inline double* fold_sh3(double* sh) { return sh + (2 + BSLOTS) * 16 + 32; }
+
+// the value of slot s (s < BSLOTS) of a register array (no indexed private memory)
+__device__ inline double slot_val(const double (&v)[BSLOTS], int s) {
+  double r = 0;
+#pragma unroll
+  for (int k = 0; k < BSLOTS; k++) r = k == s ? v[k] : r;
+  return r;
+}
+
+// The block partials of a kernel's last reduction: the first na of a[] to part, the first nb of
+// b[] to bpart (rows of MAX_RED_BLOCKS), with one barrier; sh ((NA + NB) * 16 doubles) is left
+// being read, so nothing may write it afterwards.
+template <int NA, int NB>
+__device__ inline void store_partials_last(const Blk& B_, double (&a)[NA], int na, double* __restrict__ part,
+                                           double (&b)[NB], int nb, double* __restrict__ bpart, double* sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NA; k++)
+    if (k < na) {
+      const double t = wave_sum(a[k]);
+      if (lane == 0) sh[k * 16 + w] = t;
+    }
+#pragma unroll
+  for (int k = 0; k < NB; k++)
+    if (k < nb) {
+      const double t = wave_sum(b[k]);
+      if (lane == 0) sh[(NA + k) * 16 + w] = t;
+    }
+  __syncthreads();
+  const int t = threadIdx.x;
+  const int bid = B_.x + B_.y * B_.gx;
+  if (t < na || (t >= NA && t < NA + nb)) {
+    double s = 0;
+    for (int q = 0; q < nw; q++) s += sh[t * 16 + q];
+    if (t < NA) part[(size_t)t * MAX_RED_BLOCKS + bid] = s;
+    else bpart[(size_t)(t - NA) * MAX_RED_BLOCKS + bid] = s;
+  }
+}
+
+template <int NA>
+__device__ inline void store_partials_last(const Blk& B_, double (&a)[NA], int na, double* __restrict__ part, double* sh) {
+  double none[1] = {0};
+  store_partials_last<NA, 1>(B_, a, na, part, none, 0, nullptr, sh);
+}
+
 template <int NMAX>
 __device__ inline void store_partials(const Blk& B_, double (&v)[NMAX], int nv, double* __restrict__ part, double* sh) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -302,23 +376,11 @@ __device__ inline void part_load(const double* __restrict__ part, int nb, int nv
 }
 template <int NMAX>
 __device__ inline void part_reduce(const double (&acc)[NMAX], int nv, double (&tot)[NMAX], double* sh, double* res) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  (void)res;
 #pragma unroll
-  for (int k = 0; k < NMAX; k++)
-    if (k < nv) {
-      const double t = wave_sum(acc[k]);
-      if (lane == 0) sh[k * 16 + w] = t;
-    }
-  __syncthreads();
-  if ((int)threadIdx.x < nv) {
-    double t = 0;
-    for (int q = 0; q < nw; q++) t += sh[threadIdx.x * 16 + q];
-    res[threadIdx.x] = t;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NMAX; k++) tot[k] = k < nv ? res[k] : 0.0;
-  __syncthreads();
+  for (int k = 0; k < NMAX; k++) tot[k] = acc[k];
+  block_total<NMAX>(tot, nv, sh);
+  __syncthreads();  // `sh` is free again
 }
 template <int NMAX>
 __device__ inline void sum_partials(const double* __restrict__ part, int nb, int nv, double (&tot)[NMAX], double* sh,
@@ -392,19 +454,24 @@ __device__ inline void zf_load(ZFold& f, const double* __restrict__ part, const 
   f.b0 = (int)threadIdx.x < na ? b0 : 0;
 }
 __device__ inline void zf_load_bub(ZFold& f, const BubArgs& B, int par) { f.v0 = zbub_load(B, par, f.b0, threadIdx.x); }
-// the partials' totals: r.z, r.z_old in f.tot, the slotted bubbles' z sums in zs[]
-__device__ inline void zf_reduce(ZFold& f, double* sh, double* res, double* zs) {
-  part_reduce<2>(f.a, 2, f.tot, sh, res);
-  double stt[BSLOTS];
-  part_reduce<BSLOTS>(f.s, nslots(f.na), stt, sh, res);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < BSLOTS; k++) zs[k] = stt[k];
-  __syncthreads();
+// the partials' totals in every thread, in one block reduction (sh: FOLD_SH doubles): r.z,
+// r.z_old in f.tot, the slotted bubbles' z sums in zs[]
+__device__ inline void zf_reduce(ZFold& f, double* sh, double (&zs)[BSLOTS]) {
+  double q[2 + BSLOTS];
+  q[0] = f.a[0];
+  q[1] = f.a[1];
+#pragma unroll
+  for (int k = 0; k < BSLOTS; k++) q[2 + k] = f.s[k];
+  block_total<2 + BSLOTS>(q, 2 + nslots(f.na), sh);
+  f.tot[0] = q[0];
+  f.tot[1] = q[1];
+#pragma unroll
+  for (int k = 0; k < BSLOTS; k++) zs[k] = q[2 + k];
 }
 // beta; ps[] gets the new p of every slotted bubble (its z plus beta times its old p), valid
 // after the block's next barrier
 __device__ inline double zf_finish(const ZFold& f, const Blk& B_, int par, int first, const BubArgs& B, double* sh,
-                                   double* res, const double* zs, float* ps, DevScalars* __restrict__ sc) {
+                                   const double (&zs)[BSLOTS], float* ps, DevScalars* __restrict__ sc) {
   const int na = f.na;
   const double* tot = f.tot;
   double* Zn = bv(B.bvec, BV_Z0 + par);
@@ -412,7 +479,7 @@ __device__ inline double zf_finish(const ZFold& f, const Blk& B_, int par, int f
   const bool b0 = first_block(B_);
   double bacc[2] = {0, 0};
   auto add = [&](const ZBub& v) {
-    const double zn = v.sl >= 0 ? zs[v.sl] : v.az;
+    const double zn = v.sl >= 0 ? slot_val(zs, v.sl) : v.az;
     bacc[0] += v.rc * zn;
     bacc[1] += v.rc * v.zo;
     if (b0) {
@@ -422,7 +489,7 @@ __device__ inline double zf_finish(const ZFold& f, const Blk& B_, int par, int f
   };
   if ((int)threadIdx.x < na) add(f.v0);
   for (int k = threadIdx.x + blockDim.x; k < na; k += blockDim.x) add(zbub_load(B, par, B.activeList[k], k));
-  block_sum_all<2>(bacc, sh, res);
+  block_total<2>(bacc, 2, fold_sh2(sh));
   const double rzNovo = tot[0] + bacc[0], cruzado = tot[1] + bacc[1];
   const double rzPrev = sc->rzp[1 - par];
   const double beta = first ? 0.0 : (fabs(rzPrev) < 1e-30 ? 0.0 : fmax(0.0, (rzNovo - cruzado) / rzPrev));
@@ -432,7 +499,7 @@ __device__ inline double zf_finish(const ZFold& f, const Blk& B_, int par, int f
     sc->beta = beta;
   }
   auto pslot = [&](const ZBub& v) {
-    if (v.sl >= 0) ps[v.sl] = (float)(zs[v.sl] + beta * v.po);
+    if (v.sl >= 0) ps[v.sl] = (float)(slot_val(zs, v.sl) + beta * v.po);
   };
   if ((int)threadIdx.x < na) pslot(f.v0);
   for (int k = threadIdx.x + blockDim.x; k < na; k += blockDim.x) pslot(zbub_load(B, par, B.activeList[k], k));
@@ -483,19 +550,22 @@ __device__ inline void af_load(AFold& f, const double* __restrict__ part, const 
 __device__ inline void af_load_bub(AFold& f, const BubArgs& B, const uint32_t* __restrict__ cellsB, int par) {
   f.v0 = abub_load(B, cellsB, par, f.b0, threadIdx.x);
 }
-// the partials' totals: p.Ap in f.tot, the slotted bubbles' Ap sums in aps[]
-__device__ inline void af_reduce(AFold& f, double* sh, double* res, double* aps) {
-  part_reduce<1>(f.a, 1, f.tot, sh, res);
-  double stt[BSLOTS];
-  part_reduce<BSLOTS>(f.s, nslots(f.na), stt, sh, res);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < BSLOTS; k++) aps[k] = stt[k];
-  __syncthreads();
+// the partials' totals in every thread, in one block reduction (sh: FOLD_SH doubles): p.Ap in
+// f.tot, the slotted bubbles' Ap sums in aps[]
+__device__ inline void af_reduce(AFold& f, double* sh, double (&aps)[BSLOTS]) {
+  double q[1 + BSLOTS];
+  q[0] = f.a[0];
+#pragma unroll
+  for (int k = 0; k < BSLOTS; k++) q[1 + k] = f.s[k];
+  block_total<1 + BSLOTS>(q, 1 + nslots(f.na), sh);
+  f.tot[0] = q[0];
+#pragma unroll
+  for (int k = 0; k < BSLOTS; k++) aps[k] = q[1 + k];
 }
 // alpha; rbs[] gets the slotted bubbles' new residuals and bsl[] their cells' V-cycle input
 // (residual over cell count, binary32), valid after the block's next barrier
 __device__ inline double af_finish(const AFold& f, const Blk& B_, int par, const BubArgs& B,
-                                   const uint32_t* __restrict__ cellsB, double* sh, double* res, const double* aps,
+                                   const uint32_t* __restrict__ cellsB, double* sh, const double (&aps)[BSLOTS],
                                    double* rbs, float* bsl, DevScalars* __restrict__ sc) {
   const int na = f.na;
   const double* tot = f.tot;
@@ -503,7 +573,7 @@ __device__ inline double af_finish(const AFold& f, const Blk& B_, int par, const
   auto pap_of = [&](const ABub& v, double* pn_out) -> double {
     const double pn = v.zn + beta * v.po;
     *pn_out = pn;
-    return (double)v.fb * pn + (v.sl >= 0 ? aps[v.sl] : v.az);
+    return (double)v.fb * pn + (v.sl >= 0 ? slot_val(aps, v.sl) : v.az);
   };
   double bacc[1] = {0};
   auto add_pap = [&](const ABub& v) {
@@ -514,7 +584,7 @@ __device__ inline double af_finish(const AFold& f, const Blk& B_, int par, const
   const bool mine = (int)threadIdx.x < na;
   if (mine) add_pap(f.v0);
   for (int k = threadIdx.x + blockDim.x; k < na; k += blockDim.x) add_pap(abub_load(B, cellsB, par, B.activeList[k], k));
-  block_sum_all<1>(bacc, sh, res);
+  block_total<1>(bacc, 1, fold_sh2(sh));
   const double pap = tot[0] + bacc[0];
   const double alpha = fabs(pap) < 1e-30 ? 0.0 : rz / pap;
   const bool b0 = first_block(B_);
@@ -543,12 +613,12 @@ __device__ inline double af_finish(const AFold& f, const Blk& B_, int par, const
   };
   if (mine) update(f.v0);
   for (int k = threadIdx.x + blockDim.x; k < na; k += blockDim.x) update(abub_load(B, cellsB, par, B.activeList[k], k));
-  block_sum<1>(rr, sh);
+  block_total<1>(rr, 1, fold_sh3(sh));
   if (b0 && threadIdx.x == 0) {
     sc->alpha = alpha;
     sc->pap = rr[0];  // bubble part of |r|^2
   }
-  __syncthreads();
+  __syncthreads();  // rbs[], bsl[] written; sh free again
   return alpha;
 }
 
@@ -566,8 +636,10 @@ __device__ inline double bubble_r(const BubArgs& B, int b, int par, double alpha
 // scale.  Returns true when the solve stops.
 __device__ inline bool fold_update(const Blk& B_, const double* __restrict__ part, int nb, int par, double* sh, double* res,
                                    DevScalars* __restrict__ sc) {
+  (void)res;
   double tot[1];
-  sum_partials<1>(part, nb, 1, tot, sh, res);
+  part_load<1>(part, nb, 1, tot);
+  block_total<1>(tot, 1, sh);  // sh is this fold's own: no barrier after
   const double n2 = tot[0] + sc->pap;
   const double nr = sqrt(n2);
   const int itn = sc->itp[1 - par] + 1;
@@ -813,7 +885,7 @@ __device__ __forceinline__ void k_pcg_init(Blk B_, const uint8_t* __restrict__ o
                                            double* __restrict__ bvec, const int* __restrict__ labelSlot,
                                            double* __restrict__ partials, double* __restrict__ bpart,
                                            DevScalars* __restrict__ sc) {
-  __shared__ double sh[BSLOTS * 16];
+  __shared__ double sh[FOLD_SH];
   __shared__ BubLds aggt;
   const int ns = nslots(sc->active_bubbles);
   double v[5] = {0, 0, 0, 0, 0};  // r.r, div.div, sum(b), n_water, anchored
@@ -867,8 +939,7 @@ __device__ __forceinline__ void k_pcg_init(Blk B_, const uint8_t* __restrict__ o
     if (qmode == 1 || (qmode == 2 && mask_type(m) != AGUA)) qlo[c] = 0.f;
   }
   bagg_flush(agg);
-  store_partials<5>(B_, v, 5, partials, sh);
-  if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, bpart, sh);
+  store_partials_last<5, BSLOTS>(B_, v, 5, partials, sl, ns, bpart, sh);
 }
 
 // restart = 0 starts a solve (tolerance, iteration cap and the mixed modes' scale numerator are
@@ -1228,7 +1299,7 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
   // returns once they are issued
   const int sc_done = sc->done;
   __shared__ float Tt[4][TLDS];
-  __shared__ double sh[BSLOTS * 16], res[BSLOTS], aps[BSLOTS], rbs[BSLOTS];
+  __shared__ double sh[FOLD_SH], rbs[BSLOTS];
   __shared__ float bsl[BSLOTS];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int vb = xcd_block(B_.x, B_.gx);
@@ -1330,11 +1401,12 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
   if (UPD) af_load(af, partA, bpartA, nbs, par, B, sc);
   if (sc_done) return;
   load_bub_labels();
+  double aps[BSLOTS];
   if (UPD) {
     af_load_bub(af, B, cellsB, par);
-    af_reduce(af, sh, res, aps);
+    af_reduce(af, sh, aps);
   }
-  const double alpha = UPD ? af_finish(af, B_, par, B, cellsB, sh, res, aps, rbs, bsl, sc) : 0.0;
+  const double alpha = UPD ? af_finish(af, B_, par, B, cellsB, sh, aps, rbs, bsl, sc) : 0.0;
   const double beta = UPD ? af.beta : 0.0;
   const bool wr = UPD || mean != 0;  // r is rewritten
   for (;;) {
@@ -1415,7 +1487,7 @@ __device__ __forceinline__ void k_in_a0(Blk B_, const uint8_t* __restrict__ opMa
     load(vb);
     load_bub_labels();
   }
-  if (UPD) store_partials<1>(B_, rr, 1, partRR, sh);
+  if (UPD) store_partials_last<1>(B_, rr, 1, partRR, sh);
 }
 
 struct ZArgs {  // level-0 outputs: z = MG result and the dots r.z, r.z_old
@@ -1482,7 +1554,7 @@ __device__ inline void z_cell(const ZArgs& za, size_t c, uint8_t om, float rc, f
 __device__ __forceinline__ void k_zdots0(Blk B_, const float* __restrict__ x0, ZArgs za, int C,
                                          const DevScalars* __restrict__ sc) {
   if (sc->done) return;
-  __shared__ double sh[BSLOTS * 16];
+  __shared__ double sh[FOLD_SH];
   __shared__ BubLds aggt;
   const int ns = nslots(sc->active_bubbles);
   double dv[2] = {0, 0};
@@ -1497,8 +1569,7 @@ __device__ __forceinline__ void k_zdots0(Blk B_, const float* __restrict__ x0, Z
     z_cell(za, c, om, za.r[c], za.zold[c], x0[c], inv, dv, sl, agg, lb, za.labelSlot[lb], za.cellsB[lb]);
   }
   bagg_flush(agg);
-  store_partials<2>(B_, dv, 2, za.partials, sh);
-  if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, za.bpart, sh);
+  store_partials_last<2, BSLOTS>(B_, dv, 2, za.partials, sl, ns, za.bpart, sh);
 }
 
 // snapshot value of fine cell (i, j) with its prolongation addend: x_B + P e on water cells, from
@@ -1645,7 +1716,7 @@ __device__ __forceinline__ void k_pass_z(Blk B_, const float* __restrict__ src, 
   // returns once they are issued
   const int sc_done = sc->done;
   __shared__ float Tt[4][TLDS];
-  __shared__ double sh[BSLOTS * 16];
+  __shared__ double sh[FOLD_SH];
   __shared__ BubLds aggt;
   const int lane = threadIdx.x & 63;
   const int ns = nslots(sc->active_bubbles);
@@ -1726,8 +1797,7 @@ __device__ __forceinline__ void k_pass_z(Blk B_, const float* __restrict__ src, 
     }
   }
   bagg_flush(agg);
-  store_partials<2>(B_, dv, 2, za.partials, sh);
-  if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, za.bpart, sh);
+  store_partials_last<2, BSLOTS>(B_, dv, 2, za.partials, sl, ns, za.bpart, sh);
 }
 
 // ================================================================== large levels: one kernel per pass
@@ -2082,7 +2152,7 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
   __shared__ float Tt[4][TLDS];
   __shared__ float CPt[4][100];
   __shared__ float W[CW * CW];
-  __shared__ double sh[BSLOTS * 16];
+  __shared__ double sh[FOLD_SH];
   __shared__ BubLds aggt;
   // (the barrier before pass D orders the table's clearing before the adds)
   const BubAgg agg = bagg_begin(aggt, za.accZ, ZD && sc->active_bubbles > BSLOTS);
@@ -2193,8 +2263,7 @@ __device__ __forceinline__ void k_pcd(Blk B_, const SX* __restrict__ xb, SX* __r
   if (ZD) {
     const int ns = nslots(sc->active_bubbles);
     bagg_flush(agg);
-    store_partials<2>(B_, dv, 2, za.partials, sh);
-    if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, za.bpart, sh);
+    store_partials_last<2, BSLOTS>(B_, dv, 2, za.partials, sl, ns, za.bpart, sh);
   }
 }
 
@@ -2601,7 +2670,7 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
   // the stop flag is read beside the first loads (one memory latency for both); a stopped solve
   // returns once they are issued
   const int sc_done = sc->done;
-  __shared__ double sh[BSLOTS * 16], res[BSLOTS], zs[BSLOTS];
+  __shared__ double sh[FOLD_SH];
   __shared__ float P[SPMV_HN];
   __shared__ uint8_t M[SPMV_HN];
   __shared__ float pslt[BSLOTS];    // new p of the slotted bubbles
@@ -2637,8 +2706,9 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
   zf_load(zf, partZ, bpartZ, nbz, B, sc->active_bubbles);
   if (sc_done) return;
   zf_load_bub(zf, B, par);
-  zf_reduce(zf, sh, res, zs);
-  const double beta = zf_finish(zf, B_, par, first, B, sh, res, zs, pslt, sc);
+  double zs[BSLOTS];
+  zf_reduce(zf, sh, zs);
+  const double beta = zf_finish(zf, B_, par, first, B, sh, zs, pslt, sc);
   const int ns = nslots(zf.na);
   // (the loop's first barrier orders the table's clearing before the adds)
   const BubAgg agg = bagg_begin(aggt, bv(B.bvec, BV_ACCA0 + par), zf.na > BSLOTS);
@@ -2703,8 +2773,7 @@ __device__ __forceinline__ void k_spmv(Blk B_, const uint8_t* __restrict__ opMas
     load(ch);
   }
   bagg_flush(agg);
-  store_partials<1>(B_, v, 1, partA, sh);
-  if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, bpartA, sh);
+  store_partials_last<1, BSLOTS>(B_, v, 1, partA, sl, ns, bpartA, sh);
 }
 
 // the update fold as a kernel of its own, when level 0 has no down kernel (one-workgroup grids)
@@ -2748,7 +2817,7 @@ __device__ __forceinline__ void k_true_res(Blk B_, const uint8_t* __restrict__ o
                                            double* __restrict__ partials, double* __restrict__ bpart, int gated,
                                            const DevScalars* __restrict__ sc) {
   if (true_res_skip(sc, gated)) return;
-  __shared__ double sh[BSLOTS * 16];
+  __shared__ double sh[FOLD_SH];
   __shared__ BubLds aggt;
   const int ns = nslots(sc->active_bubbles);
   const double* yb = bv(bvec, BV_Y);
@@ -2780,8 +2849,7 @@ __device__ __forceinline__ void k_true_res(Blk B_, const uint8_t* __restrict__ o
     }
   }
   bagg_flush(agg);
-  store_partials<1>(B_, v, 1, partials, sh);
-  if (ns > 0) store_partials<BSLOTS>(B_, sl, ns, bpart, sh);
+  store_partials_last<1, BSLOTS>(B_, v, 1, partials, sl, ns, bpart, sh);
 }
 
 __device__ __forceinline__ void k_fin_true_res(Blk B_, const uint32_t* __restrict__ facesB,
