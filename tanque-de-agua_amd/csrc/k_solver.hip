@@ -228,8 +228,11 @@ struct Pair {
 };
 
 
-// all LDS operations of this wavefront are complete (and not reordered across by the compiler)
-__device__ inline void wave_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+// orders this wavefront's LDS accesses across lanes: a compiler barrier only.  The LDS executes a
+// wavefront's DS instructions in issue order, so a read issued after another lane's write sees it
+// without waiting for the write to complete; what must not happen is the compiler moving an access
+// across (to it, other lanes' cells at other offsets look independent).
+__device__ inline void wave_sync() { asm volatile("" ::: "memory"); }
 
 __device__ inline int nslots(int na) { return na < BSLOTS ? na : BSLOTS; }
 
