@@ -1098,20 +1098,24 @@ __device__ inline void cell_clear(Cells4& c, int q, int r) {
 // the sum over the free ones.  Both rows' neighbours are read before any arithmetic (one LDS
 // latency per half-sweep), and the update is branch-free: fma(s + b, 1/count, 0) on updated
 // cells, fma(s + b, 0, x) = x on the others.
+// One horizontal neighbour of each cell is the lane's own cell of the other colour in that row
+// (its value in registers equals the tile's): only the other three come from T.
 template <int COR>
 __device__ inline void half_sweep(float* T, Cells4& c, int lane) {
-  float tl[2], tr[2], td[2], tu[2];
+  const int off = (COR + (lane >> 3) + 1) & 1;  // the cells of colour COR sit at column 2p + off
+  float th[2], td[2], tu[2];
 #pragma unroll
   for (int r = 0; r < 2; r++) {
     const int e = cell_e(lane, COR, r);
-    tl[r] = T[e - 1];
-    tr[r] = T[e + 1];
+    th[r] = T[off ? e + 1 : e - 1];
     td[r] = T[e - TP];
     tu[r] = T[e + TP];
   }
 #pragma unroll
   for (int r = 0; r < 2; r++) {
-    const float sum = ((tl[r] + tr[r]) + td[r]) + tu[r];
+    const float own = c.v[1 - COR][r];
+    const float tl = off ? own : th[r], tr = off ? th[r] : own;
+    const float sum = ((tl + tr) + td[r]) + tu[r];
     const float nv = __builtin_fmaf(sum + c.b[COR][r], c.ik[COR][r], c.kp[COR][r] * c.v[COR][r]);
     c.v[COR][r] = nv;
     T[cell_e(lane, COR, r)] = nv;
