@@ -38,6 +38,8 @@ SCENARIOS = {
     "bolha": (lambda orc: tanque_com_bolha(orc), 6),
     "ampulheta": (lambda orc: ampulheta(orc), 20),
     "dam160": (lambda orc: dam(orc, nx=160, ny=90, h=0.8 / 90, seed=0), 12),
+    # odd row length: the smoother tiles move cells one at a time instead of as row pairs
+    "dam161": (lambda orc: dam(orc, nx=161, ny=90, h=0.8 / 90, seed=0), 12),
 }
 
 

@@ -43,6 +43,8 @@ def prepared(orc, build, frames):
     ("bolha", lambda o: tanque_com_bolha(o), 6),
     ("ampulheta", lambda o: ampulheta(o), 20),
     ("dam160", lambda o: dam(o, nx=160, ny=90, h=0.8 / 90, seed=0), 12),
+    # odd row length: the smoother tiles and the update move cells one at a time, not as row pairs
+    ("dam161", lambda o: dam(o, nx=161, ny=90, h=0.8 / 90, seed=0), 12),
     # 256x144: level 1 (128x72) is large enough for FP16 storage in the mixed preconditioner
     ("dam256", lambda o: dam(o, nx=256, ny=144, h=1.0 / 144, seed=0), 8),
     # twelve bubbles: more than the per-block reduction slots, the rest in 128-bit fixed point

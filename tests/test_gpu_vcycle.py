@@ -45,6 +45,8 @@ def state(orc, build, frames):
     ("bolha", lambda o: tanque_com_bolha(o), 6),                        # an enclosed bubble
     ("ampulheta", lambda o: ampulheta(o), 12),                          # walls, air pockets
     ("dam256", lambda o: dam(o, nx=256, ny=144, h=1.0 / 144, seed=0), 4),
+    # odd row lengths on levels 0 and 1 (161, 81): cell-by-cell tile loads and stores
+    ("dam161", lambda o: dam(o, nx=161, ny=90, h=0.8 / 90, seed=0), 4),
     # 1600x800: 8 levels, the last one past the 64x64-tile type pyramid; level 0 over 1M cells
     # runs its down passes as separate kernels
     ("dam1600", lambda o: dam(o, nx=1600, ny=800, h=1.0 / 800, seed=0), 0),
@@ -80,6 +82,7 @@ def test_vcycle_matches_oracle(oracle, tq, name, build, frames, prec):
 @pytest.mark.parametrize("name,build,frames", [
     ("dam256", lambda o: dam(o, nx=256, ny=144, h=1.0 / 144, seed=0), 4),
     ("ampulheta", lambda o: ampulheta(o), 12),
+    ("dam161", lambda o: dam(o, nx=161, ny=90, h=0.8 / 90, seed=0), 4),
 ])
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
 def test_split_levels_match_oracle(oracle, tq, name, build, frames, prec):
