@@ -3295,13 +3295,6 @@ static void launch_coarse_wg(tq_sim* s) {
   const int L = (int)s->lv.size();
   Level& G = s->lv[s->lc];
   if (vop_active(s)) {
-    if (s->vop_pending) {  // the matrix of this solve is built on the second stream
-      if (hipStreamWaitEvent(s->st, s->ev_vop, 0) != hipSuccess) {
-        (void)hipGetLastError();
-        (void)hipStreamSynchronize(s->st2);
-      }
-      s->vop_pending = false;
-    }
     // x(lc) = C b(lc): the matrix and b in, x out
     const int waves = (G.n + VOP_ROWS - 1) / VOP_ROWS;
     int t = time_begin(s, "mg_coarse_mv", 4.0 * s->vop_ld * G.n + 8.0 * G.n);
@@ -3321,22 +3314,13 @@ static void launch_coarse_wg(tq_sim* s) {
   time_end(s, t);
 }
 
-// builds this solve's matrix of level lc's V-cycle (after the level masks and the coarsest operator)
-// On the second stream (when there is one, and outside kernel timing and graph capture) the
-// build runs beside the solve's set-up, first residual and first down kernels; the first dense
-// product waits for it (launch_coarse_wg).
+// builds this solve's matrix of level lc's V-cycle (after the level masks and the coarsest operator),
+// on the tank's own stream: a second stream that ran the build beside the solve's set-up and first
+// kernels slowed those kernels by more than the ~90 us of overlap it won
 static int launch_vcycle_op(tq_sim* s) {
   const WGPlan P = wg_plan(s);
   const Level& G = s->lv[s->lc];
   const size_t lds = P.pool + VOP_WAVES * TLDS * sizeof(float);
-  if (s->st2 && !s->timing && !s->capturing) {
-    TQ_HIP(hipEventRecord(s->ev_masks, s->st));
-    TQ_HIP(hipStreamWaitEvent(s->st2, s->ev_masks, 0));
-    launch<64 * VOP_WAVES, 0, k_vcycle_op>(s, s->st2, dim3(G.n), dim3(64 * VOP_WAVES), lds, P, s->vop_ld, s->vop, s->sc);
-    TQ_HIP(hipEventRecord(s->ev_vop, s->st2));
-    s->vop_pending = true;
-    return 0;
-  }
   int t = time_begin(s, "mg_vcycle_op", 4.0 * s->vop_ld * G.n);
   launch<64 * VOP_WAVES, 0, k_vcycle_op>(s, s->st, dim3(G.n), dim3(64 * VOP_WAVES), lds, P, s->vop_ld, s->vop, s->sc);
   time_end(s, t);
@@ -3471,12 +3455,6 @@ static int run_iterations(tq_sim* s, int n) {
 static int solve_setup(tq_sim* s) {
   const int C = s->C;
   const int L = (int)s->lv.size();
-  // a matrix build of an earlier solve that left early (an error return) may still be running on
-  // the second stream: the set-up below rewrites the masks and flags it reads
-  if (s->vop_pending) {
-    TQ_HIP(hipStreamWaitEvent(s->st, s->ev_vop, 0));
-    s->vop_pending = false;
-  }
   int t = time_begin(s, "pcg_setup", (1 + 4 + 1 + 1) * (double)C);
   {
     PyrPlan Pp{};

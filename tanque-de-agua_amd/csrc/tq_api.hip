@@ -176,19 +176,6 @@ static int alloc_all(tq_sim* s) {
   return 0;
 }
 
-// Batched tanks keep every kernel on their own stream: a second stream per tank (and the waits
-// between them) crowds the hardware queues the tanks share.
-void drop_second_stream(tq_sim* s) {
-  if (!s->st2) return;
-  (void)hipStreamSynchronize(s->st2);
-  (void)hipStreamDestroy(s->st2);
-  (void)hipEventDestroy(s->ev_masks);
-  (void)hipEventDestroy(s->ev_vop);
-  s->st2 = nullptr;
-  s->ev_masks = s->ev_vop = nullptr;
-  s->vop_pending = false;
-}
-
 // A tank of a table batch enqueues on the batch's stream (shared by every tank of the batch, which
 // then orders the tanks' own calls and the batch's table launches): its own stream is released.
 int share_stream(tq_sim* s, hipStream_t st) {
@@ -200,7 +187,6 @@ int share_stream(tq_sim* s, hipStream_t st) {
 }
 
 static void free_all(tq_sim* s) {
-  if (s->st2) (void)hipStreamSynchronize(s->st2);  // a matrix build may still be writing
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   for (int b = 0; b < 2; b++) { f(s->pos[b]); f(s->vel[b]); f(s->cm[b]); f(s->ext[b]); f(s->seq[b]); }
   f(s->pkey); f(s->prank); f(s->addflag); f(s->stage); f(s->render_img); f(s->sdf); f(s->solid); f(s->countKey); f(s->startKey);
@@ -221,9 +207,6 @@ static void free_all(tq_sim* s) {
   if (s->ev_frame1) (void)hipEventDestroy(s->ev_frame1);
   solver_free(s);
   for (auto& e : s->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
-  if (s->st2) (void)hipStreamDestroy(s->st2);
-  if (s->ev_masks) (void)hipEventDestroy(s->ev_masks);
-  if (s->ev_vop) (void)hipEventDestroy(s->ev_vop);
   if (s->st && s->own_st) (void)hipStreamDestroy(s->st);
 }
 
@@ -471,17 +454,6 @@ int tq_create(const tq_desc* d, tq_sim** out) {
                                 : (int)std::ceil((double)d->nx * d->ny * PARTICULAS_MAX_POR_CELULA);
   hipError_t e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
   if (e != hipSuccess) { hip_fail(e, "hipStreamCreate", __FILE__, __LINE__); delete s; return TQ_E_HIP; }
-  // the second stream and its two events (default priority; without them the matrix is built on st)
-  if (hipStreamCreateWithFlags(&s->st2, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&s->ev_masks, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&s->ev_vop, hipEventDisableTiming) != hipSuccess) {
-    (void)hipGetLastError();
-    if (s->st2) (void)hipStreamDestroy(s->st2);
-    if (s->ev_masks) (void)hipEventDestroy(s->ev_masks);
-    if (s->ev_vop) (void)hipEventDestroy(s->ev_vop);
-    s->st2 = nullptr;
-    s->ev_masks = s->ev_vop = nullptr;
-  }
   if (int rc = alloc_all(s)) { free_all(s); delete s; return rc; }
   // the empty tank: frame-only SDF (cena.js:139-144), solid cells, fresh air tracker
   std::vector<float> sdf((size_t)4 * s->C);

@@ -489,7 +489,6 @@ int create(const tq_desc* descs, int n, int threads, bool table, tq_batch** out)
     tq_sim* s = nullptr;
     if (int e = tq_create(&descs[k], &s)) return fail(e, "tank " + std::to_string(k) + ": " + tq_last_error());
     b->tanks.push_back(s);
-    tq::drop_second_stream(s);
     if (table)
       if (int e = tq::share_stream(s, b->st)) return fail(e, "tank " + std::to_string(k) + ": " + tq_last_error());
   }

@@ -135,9 +135,6 @@ struct tq_sim {
   int ntx, nty, K;        // binning tiles and key space (ntx*nty*256)
   double h;
   hipStream_t st = nullptr;
-  hipStream_t st2 = nullptr;     // builds the dense coarse V-cycle matrix beside the solve's first kernels
-  hipEvent_t ev_masks = nullptr, ev_vop = nullptr;  // masks ready on st / matrix built on st2
-  bool vop_pending = false;      // st has not yet waited for the matrix st2 is building
 
   // particles
   int cap = 0;
@@ -344,7 +341,6 @@ void solver_free(tq_sim* s);
 int pressure_solve(tq_sim* s);
 void solver_launch_changed(tq_sim* s);
 void tune_apply_cached(tq_sim* s);  // tq_tune.hip: a tq_autotune result cached for this grid
-void drop_second_stream(tq_sim* s);  // build the coarse matrix on the tank's own stream
 int share_stream(tq_sim* s, hipStream_t st);  // enqueue on a stream the tank does not own
 int true_residual(tq_sim* s, double* out);
 int debug_vcycle(tq_sim* s, const float* b, float* z);
