@@ -714,6 +714,46 @@ def summarize_trace(trace_dir, title, commands, note, top=24):
     return "\n".join(lines) + "\n"
 
 
+WAVE_COUNTERS = ("SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY",
+                 "SQ_ACTIVE_INST_ANY", "GRBM_GUI_ACTIVE")
+
+
+def summarize_wave_counters(runs, commands, note, top=16):
+    """Markdown tables of a rocprofv3 --pmc run of WAVE_COUNTERS per (kernel symbol, grid): waves
+    per launch, and the wave-cycles split into cycles issuing (SQ_ACTIVE_INST_ANY) and waiting
+    (SQ_WAIT_ANY: for data or a dependency; SQ_WAIT_INST_ANY: for an issue slot), each over
+    SQ_WAVE_CYCLES; SQ_WAVE_CYCLES over SQ_BUSY_CYCLES is the mean count of resident waves (in the
+    counters' own units, comparable between kernels and runs).  `runs` is a list of
+    (counter directory, title)."""
+    lines = ["# Wave residency and stall cycles of the solver kernels", "",
+             "Commands (from the repository root, on one MI355X):", "", "```"] + commands + ["```", "", note, ""]
+    for d, title in runs:
+        per, disp, dur = {}, {}, {}
+        for r in _csv_rows(_find(d, "counter_collection")):
+            k = (_kname(r), _grid_total(r))
+            c = per.setdefault(k, {})
+            c[r["Counter_Name"]] = c.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+            disp.setdefault(k, set()).add(r["Dispatch_Id"])
+            dur.setdefault(k, {})[r["Dispatch_Id"]] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        rows = []
+        for k, c in per.items():
+            n = len(disp[k])
+            us = sum(dur[k].values()) / n
+            wc = c.get("SQ_WAVE_CYCLES", 0.0) or 1.0
+            rows.append((us * n, k, n, us, c.get("SQ_WAVES", 0.0) / n, c.get("SQ_ACTIVE_INST_ANY", 0.0) / wc,
+                         c.get("SQ_WAIT_ANY", 0.0) / wc, c.get("SQ_WAIT_INST_ANY", 0.0) / wc,
+                         c.get("SQ_WAVE_CYCLES", 0.0) / (c.get("SQ_BUSY_CYCLES", 0.0) or 1.0)))
+        rows.sort(key=lambda t: -t[0])
+        lines += ["## " + title, "",
+                  "| kernel | grid | calls | avg us | waves / launch | issuing | waiting (data, dependency) "
+                  "| waiting (issue) | resident waves |",
+                  "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+        for _, k, n, us, w, a, wa, wi, res in rows[:top]:
+            lines.append("| `%s` | %d | %d | %.2f | %.0f | %.3f | %.3f | %.3f | %.1f |" % (k[0], k[1], n, us, w, a, wa, wi, res))
+        lines.append("")
+    return "\n".join(lines) + "\n"
+
+
 def profile_markdown(summary, title, top=30):
     """The kernel table of a summary as markdown (profiles/*.md)."""
     lines = ["# " + title, "", "Commands (from the repository root, on one MI355X):", "", "```"]
@@ -751,6 +791,9 @@ def main():
                     help="write the kernel table of a rocprofv3 --kernel-trace run (with --profile-commands "
                          "and --profile-note)")
     ap.add_argument("--profile-note", default="", help="a paragraph for --summarize-trace")
+    ap.add_argument("--summarize-wave-counters", nargs="+", metavar="ARG",
+                    help="OUT_MD then pairs of (counter directory, title) of rocprofv3 --pmc runs of the wave "
+                         "counters (with --profile-commands and --profile-note)")
     ap.add_argument("--summarize-batch-trace", nargs=2, metavar=("TRACE_DIR", "OUT_MD"),
                     help="write the kernel table of a --batched-table-only rocprofv3 --kernel-trace run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (funnel) runs")
@@ -804,6 +847,15 @@ def main():
         with open(om, "w") as f:
             f.write(profile_markdown(sm, "Kernel time and HBM traffic, %s" % name))
         print(json.dumps({"summary": oj, "kernels": sm["kernels"]}), flush=True)
+        return
+    if args.summarize_wave_counters:
+        om, rest = args.summarize_wave_counters[0], args.summarize_wave_counters[1:]
+        if len(rest) < 2 or len(rest) % 2:
+            ap.error("--summarize-wave-counters needs OUT_MD and pairs of (directory, title)")
+        with open(om, "w") as f:
+            f.write(summarize_wave_counters(list(zip(rest[0::2], rest[1::2])), args.profile_commands,
+                                            args.profile_note))
+        print(json.dumps({"summary": om}), flush=True)
         return
     if args.summarize_trace:
         td, om, title = args.summarize_trace
